@@ -1,0 +1,201 @@
+/*
+ * rvs.h - C ABI of the MI355X capsule collision / distance engine.
+ *
+ * Drop-in for the capsule-primitive RealVectorSpace path of RPMPLv2
+ * (include/state_spaces/real_vector_space/RealVectorSpace.h:17-53, the
+ * base::StateSpace pure virtuals of include/state_spaces/StateSpace.h:52-56,
+ * and the robot-side queries of include/robots/AbstractRobot.h:58-61).
+ *
+ * Conventions
+ *   - Plain C types only; every call returns rvs_status; no exceptions cross
+ *     the ABI.  rvs_last_error() describes the last failure of a context.
+ *   - Buffers are caller-owned.  With RVS_F_DEVICE every array argument is a
+ *     device pointer on the context's GPU; otherwise arrays are host memory
+ *     and the call stages them through the context's pinned buffers.  No
+ *     pointer is kept after a synchronous call returns.
+ *   - A context is internally synchronised: calls from several host threads
+ *     on one context are serialised; separate contexts run independently.
+ *   - Layouts are configuration-major: q [N][n], profile [N][L],
+ *     nearest [N][M][L][6] (robot xyz, obstacle xyz), verdict [N] (1 = free).
+ */
+#ifndef RVS_H
+#define RVS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rvs_status {
+    RVS_OK = 0,
+    RVS_EINVAL = 1,   /* bad argument (null pointer, size, id) */
+    RVS_ENOMEM = 2,   /* device or pinned allocation failed */
+    RVS_EDEVICE = 3,  /* HIP runtime or kernel launch error */
+    RVS_ERANGE = 4,   /* size beyond a documented limit */
+    RVS_ENOTSUP = 5   /* combination not supported */
+} rvs_status;
+
+/* Arithmetic of a query. */
+typedef enum rvs_precision {
+    /* Reference arithmetic: FP64 kinematics, FP32 skeleton and geometry in
+     * the reference's operation order.  Bit-identical to RPMPLv2. */
+    RVS_PREC_REF32 = 0,
+    /* The same algorithm carried out entirely in FP64. */
+    RVS_PREC_FP64 = 1,
+    /* FP32 bulk pass; configurations with a pair inside the near-contact
+     * band (or an ill-conditioned segment pair) are recomputed: verdict and
+     * profile with RVS_PREC_REF32, distance with RVS_PREC_FP64. */
+    RVS_PREC_FP32_REFINE = 2
+} rvs_precision;
+
+/* Call flags. */
+#define RVS_F_DEVICE 0x1u          /* array arguments are device pointers */
+#define RVS_F_PREC_SHIFT 4         /* bits 4..5: precision override + 1 (0 = context default) */
+#define RVS_F_PREC(p) ((uint32_t)((p) + 1) << RVS_F_PREC_SHIFT)
+
+typedef struct rvs_options {
+    int32_t device;             /* HIP device ordinal */
+    int32_t precision;          /* rvs_precision used when a call does not override it */
+    float equality_threshold;   /* RealVectorSpaceConfig::EQUALITY_THRESHOLD (yaml: 1e-4) */
+    float max_distance;         /* RealVectorSpaceConfig::MAX_DISTANCE (1000) */
+    int32_t num_checks;         /* RealVectorSpaceConfig::NUM_INTERPOLATION_VALIDITY_CHECKS (yaml: 10) */
+    float band;                 /* near-contact band of RVS_PREC_FP32_REFINE, metres */
+} rvs_options;
+
+/* Fills the defaults the reference's apps load from
+ * data/configurations/configuration_realvectorspace.yaml:1-3. */
+void rvs_default_options(rvs_options* opt);
+
+typedef struct rvs_context rvs_context;
+
+/* ---------------------------------------------------------------- robots */
+enum { RVS_JOINT_FIXED = 0, RVS_JOINT_REVOLUTE = 1 };
+enum { RVS_SKEL_CHAIN = 0, RVS_SKEL_XARM6 = 1 };
+
+typedef struct rvs_joint {
+    int32_t type;        /* RVS_JOINT_* */
+    int32_t reserved;
+    double xyz[3];       /* URDF joint origin */
+    double rpy[3];
+    double axis[3];      /* URDF joint axis */
+} rvs_joint;
+
+/* A serial chain from the base link to the tool, in URDF joint order.
+ * skeleton_kind RVS_SKEL_CHAIN: capsule end points are the segment origins
+ * (Planar2DOF::computeSkeleton, src/robots/Planar2DOF.cpp:117-130);
+ * RVS_SKEL_XARM6: the xArm6 recipe (src/robots/xArm6.cpp:209-235). */
+typedef struct rvs_robot_desc {
+    int32_t skeleton_kind;
+    int32_t num_joints;
+    const rvs_joint* joints;
+    int32_t num_links;             /* capsules; points = num_links + 1 */
+    const float* capsule_radius;   /* [num_links] (AbstractRobot::getCapsuleRadius) */
+    int32_t ground_included;       /* links [0, ground_included) ignore "ground" boxes */
+    float gripper_length;          /* xArm6 only */
+    int32_t num_self_pairs;        /* link pairs (a, b) for self-collision */
+    const int32_t* self_pairs;     /* [num_self_pairs][2] */
+} rvs_robot_desc;
+
+/* Replaces robots::*::computeSkeleton set-up + AbstractRobot::setCapsulesRadius
+ * (include/robots/AbstractRobot.h:46,58). */
+rvs_status rvs_robot_load(rvs_context* ctx, const rvs_robot_desc* desc, int32_t* robot_id);
+
+/* ---------------------------------------------------------------- obstacles */
+#define RVS_BOX_GROUND 0x1u    /* label "ground" (Environment.cpp:32-33) */
+#define RVS_BOX_DYNAMIC 0x2u   /* label "dynamic_obstacle" */
+
+typedef struct rvs_box {
+    float dim[3];
+    float pos[3];
+    float quat_wxyz[4];        /* identity for axis-aligned boxes */
+    uint32_t flags;            /* RVS_BOX_* */
+    float min_dist_tol;        /* Object::getMinDistTol (default MAX_DISTANCE) */
+} rvs_box;
+
+/* Obstacle semantics of an environment. */
+enum {
+    /* Reference semantics: every box is queried through its FCL world AABB
+     * (RealVectorSpace.cpp:293-295, 344-346); a rotated box becomes the cube
+     * centre +- |dim|/2. */
+    RVS_ENV_AABB = 0,
+    /* Build extension: rotated boxes are queried as oriented boxes (the
+     * capsule is moved into the box frame). */
+    RVS_ENV_OBB = 1
+};
+
+/* Replaces env::Environment construction from a scenario
+ * (src/environments/Environment.cpp:13-109, Box.cpp:3-15). */
+rvs_status rvs_env_create(rvs_context* ctx, const rvs_box* boxes, int32_t num_boxes, int32_t semantics,
+                          int32_t* env_id);
+/* In-place move of boxes idx[k] to pos[k] (Object::setPosition,
+ * src/environments/Object.cpp:21-26); bounds are recomputed on the GPU. */
+rvs_status rvs_env_update(rvs_context* ctx, int32_t env_id, const int32_t* idx, const float* pos, int32_t count,
+                          uint32_t flags);
+/* Axis-aligned bounds [num][6] the queries use (host memory). */
+rvs_status rvs_env_bounds(rvs_context* ctx, int32_t env_id, float* aabb, int32_t num);
+
+/* ---------------------------------------------------------------- queries */
+/* RealVectorSpace::computeDistance(q, compute_again = true)
+ * (RealVectorSpace.cpp:318-392) for N configurations: d, per-link profile,
+ * nearest points (optional), verdict = (d > 0).  profile and nearest may be
+ * NULL. */
+rvs_status rvs_distance(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n, float* d,
+                        float* profile, float* nearest, uint8_t* verdict, uint32_t flags);
+
+/* The FP64 path with double outputs (RVS_PREC_FP64 always). */
+rvs_status rvs_distance_f64(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n, double* d,
+                            double* profile, uint8_t* verdict, uint32_t flags);
+
+/* RealVectorSpace::isValid(q) (RealVectorSpace.cpp:277-309): collision tree,
+ * environment only.  valid[k] = 1 when configuration k is collision-free. */
+rvs_status rvs_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n, uint8_t* valid,
+                        uint32_t flags);
+
+/* RealVectorSpace::computeDistanceUnderestimation(q, nearest_points)
+ * (RealVectorSpace.cpp:402-446) on fresh states.  nearest_ref is [M][L][6]
+ * shared by all configurations (ref_stride = 0) or one set per configuration
+ * (ref_stride = M*L*6).  stored[k] = 0 where the reference returns 0 without
+ * storing a profile (a negative plane distance); profile rows are then 0. */
+rvs_status rvs_distance_underestimation(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n,
+                                        const float* nearest_ref, int64_t ref_stride, float* d, float* profile,
+                                        uint8_t* stored, uint32_t flags);
+
+/* Capsule-capsule distances of the robot's self-collision pairs
+ * (xArm6::computeCapsulesDistance, src/robots/xArm6.cpp:417-424), [N][P].
+ * A pair with distance < 0 is the capsule part of checkSelfCollision
+ * (xArm6.cpp:384-411; the FCL mesh confirmation is not part of this path). */
+rvs_status rvs_self_distance(rvs_context* ctx, int32_t robot, const float* q, int64_t n, float* dist,
+                             uint32_t flags);
+
+/* RealVectorSpace::isValid(q1, q2) (RealVectorSpace.cpp:261-275): num_checks
+ * interpolated states from the far end, q1 not checked. */
+rvs_status rvs_edge_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const float* q1, const float* q2,
+                             int64_t n, uint8_t* valid, uint32_t flags);
+
+/* Discretised edges: state k of edge e is q1[e] + (step*k) * dir[e],
+ * k = 0..K[e]-1.  first[e] = first k whose state collides with the
+ * environment (collision tree) or has a self-collision pair with distance
+ * < 0, or -1.  Edges of several robots go in one persistent launch:
+ * robots[e] selects the robot of edge e, q1/dir rows are padded to
+ * row_stride floats. */
+rvs_status rvs_edge_first_collision(rvs_context* ctx, int32_t env, const int32_t* robots, const float* q1,
+                                    const float* dir, const int32_t* K, int64_t num_edges, int32_t row_stride,
+                                    float step, int32_t* first, uint32_t flags);
+
+/* ---------------------------------------------------------------- context */
+rvs_status rvs_create(const rvs_options* opt, rvs_context** ctx);
+void rvs_destroy(rvs_context* ctx);
+const char* rvs_last_error(const rvs_context* ctx);
+/* The HIP stream (hipStream_t) the context launches on. */
+void* rvs_stream(rvs_context* ctx);
+/* Statistics of the last RVS_PREC_FP32_REFINE query: configurations
+ * recomputed by the refinement pass. */
+int64_t rvs_last_refined(const rvs_context* ctx);
+/* Library version string. */
+const char* rvs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RVS_H */

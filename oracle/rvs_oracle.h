@@ -1,0 +1,120 @@
+/*
+ * CPU oracle for the capsule collision / distance path of RPMPLv2's
+ * RealVectorSpace.  TEST INFRASTRUCTURE ONLY: this library is the checker
+ * used by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
+ * leg.  The product (rpmplv2_amd/librvs.so) never links, loads or calls it.
+ *
+ * Two instantiations of one restatement:
+ *   ref32 - reference-faithful: double-precision KDL-style FK, skeleton
+ *           rounded to float, all geometry in float in the reference's
+ *           operation order (no FMA contraction, IEEE div/sqrt).
+ *   ref64 - the same algorithm with the skeleton and all geometry in double.
+ *
+ * Parity status of each function is stated in oracle/README.md and in the
+ * header comment of rvs_oracle.cpp.
+ */
+#ifndef RVS_ORACLE_H
+#define RVS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_SEG 72
+#define ORC_MAX_LINKS 64
+
+enum { ORC_JOINT_FIXED = 0, ORC_JOINT_REVOLUTE = 1 };
+enum { ORC_SKEL_CHAIN = 0, ORC_SKEL_XARM6 = 1 };
+
+typedef struct orc_joint {
+    int32_t type;        /* ORC_JOINT_FIXED / ORC_JOINT_REVOLUTE */
+    int32_t pad;
+    double xyz[3];       /* URDF joint origin xyz */
+    double rpy[3];       /* URDF joint origin rpy */
+    double axis[3];      /* URDF joint axis (joint frame) */
+} orc_joint;
+
+typedef struct orc_robot {
+    int32_t skeleton_kind;     /* ORC_SKEL_CHAIN or ORC_SKEL_XARM6 */
+    int32_t num_segments;      /* chain segments root->tip, incl. fixed ones */
+    int32_t num_dofs;          /* revolute joints */
+    int32_t num_links;         /* capsules */
+    int32_t ground_included;
+    float gripper_length;
+    float radius[ORC_MAX_LINKS];
+    orc_joint joints[ORC_MAX_SEG];
+} orc_robot;
+
+typedef struct orc_params {
+    float eq_threshold;        /* RealVectorSpaceConfig::EQUALITY_THRESHOLD */
+    float max_distance;        /* RealVectorSpaceConfig::MAX_DISTANCE */
+    int32_t num_checks;        /* NUM_INTERPOLATION_VALIDITY_CHECKS */
+} orc_params;
+
+typedef struct orc_scene {
+    int32_t num_obj;
+    int32_t pad;
+    const float* obs;          /* num_obj x 6 : xmin ymin zmin xmax ymax zmax */
+    const uint8_t* is_ground;  /* num_obj */
+    const float* min_dist_tol; /* num_obj */
+} orc_scene;
+
+/* ---- scene ---- */
+void orc_box_aabb(const float dim[3], const float pos[3], const float quat_wxyz[4], float out6[6]);
+
+/* ---- kinematics ---- */
+int orc_skeleton32(const orc_robot* rb, const float* q, int64_t n, float* skel);
+int orc_skeleton64(const orc_robot* rb, const float* q, int64_t n, double* skel);
+
+/* ---- pair geometry (single pair) ---- */
+float orc_seg_seg32(const float A[3], const float B[3], const float C[3], const float D[3],
+                    const orc_params* p, float np[6], int32_t* np_valid);
+double orc_seg_seg64(const double A[3], const double B[3], const double C[3], const double D[3],
+                     const orc_params* p, double np[6], int32_t* np_valid);
+float orc_seg_pt32(const float A[3], const float B[3], const float C[3],
+                   const orc_params* p, float np[6], int32_t* np_valid);
+double orc_seg_pt64(const double A[3], const double B[3], const double C[3],
+                    const orc_params* p, double np[6], int32_t* np_valid);
+float orc_capsule_box32(const float A[3], const float B[3], float r, const float obs[6],
+                        const orc_params* p, float np[6], int32_t* np_valid, int32_t* path);
+double orc_capsule_box64(const double A[3], const double B[3], double r, const double obs[6],
+                         const orc_params* p, double np[6], int32_t* np_valid, int32_t* path);
+int32_t orc_collision_capsule_box32(const float A[3], const float B[3], float r, const float obs[6],
+                                    const orc_params* p);
+int32_t orc_collision_capsule_box64(const double A[3], const double B[3], double r, const double obs[6],
+                                    const orc_params* p);
+
+/* ---- batched RealVectorSpace loops (static partition over nthreads) ---- */
+void orc_distance32(const orc_robot* rb, const orc_params* p, const orc_scene* sc,
+                    const float* q, int64_t n, float* d, float* profile, float* np,
+                    uint8_t* verdict, int32_t nthreads);
+void orc_distance64(const orc_robot* rb, const orc_params* p, const orc_scene* sc,
+                    const float* q, int64_t n, double* d, double* profile, double* np,
+                    uint8_t* verdict, int32_t nthreads);
+void orc_is_valid32(const orc_robot* rb, const orc_params* p, const orc_scene* sc,
+                    const float* q, int64_t n, uint8_t* valid, int32_t nthreads);
+void orc_is_valid64(const orc_robot* rb, const orc_params* p, const orc_scene* sc,
+                    const float* q, int64_t n, uint8_t* valid, int32_t nthreads);
+void orc_underestimation32(const orc_robot* rb, const orc_params* p, const orc_scene* sc,
+                           const float* q, int64_t n, const float* np_ref, int64_t np_stride,
+                           float* d, float* profile, uint8_t* stored, int32_t nthreads);
+void orc_self_distance32(const orc_robot* rb, const orc_params* p, const float* q, int64_t n,
+                         const int32_t* pairs, int32_t npairs, float* dist, int32_t nthreads);
+void orc_self_distance64(const orc_robot* rb, const orc_params* p, const float* q, int64_t n,
+                         const int32_t* pairs, int32_t npairs, double* dist, int32_t nthreads);
+void orc_edge_valid32(const orc_robot* rb, const orc_params* p, const orc_scene* sc,
+                      const float* q1, const float* q2, int64_t n, uint8_t* valid, int32_t nthreads);
+void orc_edge_first_collision32(const orc_robot* rb, const orc_params* p, const orc_scene* sc,
+                                const int32_t* pairs, int32_t npairs, float step,
+                                const float* q1, const float* dir, const int32_t* K, int64_t E,
+                                int32_t* first, int32_t nthreads);
+
+/* ---- independent checker: exact segment-AABB distance in double ---- */
+double orc_true_seg_box(const double A[3], const double B[3], const double obs[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

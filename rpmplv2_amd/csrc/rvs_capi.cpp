@@ -1,0 +1,639 @@
+// C ABI of the capsule collision / distance engine (include/rvs.h).
+//
+// Host side: robots and obstacle sets are converted once into the constant
+// tables the kernels read (segment frames in double, skeleton recipe, radii,
+// axis-aligned bounds) and kept resident on the GPU; queries stage host
+// arrays through a per-context device arena and launch on the context's
+// stream.  Every entry point catches its own failures and returns a status.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/rvs.h"
+#include "rvs_internal.hpp"
+
+using namespace rvs;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 1 << 20);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct RobotRec {
+    RobotDev dev{};
+    std::vector<void*> allocs;
+    int num_dofs = 0, num_links = 0, num_pairs = 0;
+};
+
+struct EnvRec {
+    EnvDev dev{};
+    std::vector<void*> allocs;
+    std::vector<rvs_box> boxes;
+    std::vector<float> aabb;
+    int semantics = RVS_ENV_AABB;
+};
+
+}  // namespace
+
+struct rvs_context {
+    rvs_options opt{};
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+    std::vector<RobotRec> robots;
+    std::vector<EnvRec> envs;
+    DevBuf arena;
+    int64_t last_refined = 0;
+};
+
+namespace {
+
+rvs_status fail(rvs_context* c, rvs_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+rvs_status hip_fail(rvs_context* c, hipError_t e, const char* what) {
+    return fail(c, e == hipErrorOutOfMemory ? RVS_ENOMEM : RVS_EDEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---- host restatement of the joint-origin constants (URDF rpy -> quaternion
+// -> rotation as urdfdom and kdl_parser do it; KDL::Joint normalises the axis)
+void rpy_to_matrix(const double rpy[3], double M[9]) {
+    const double phi = rpy[0] / 2.0, the = rpy[1] / 2.0, psi = rpy[2] / 2.0;
+    double x = std::sin(phi) * std::cos(the) * std::cos(psi) - std::cos(phi) * std::sin(the) * std::sin(psi);
+    double y = std::cos(phi) * std::sin(the) * std::cos(psi) + std::sin(phi) * std::cos(the) * std::sin(psi);
+    double z = std::cos(phi) * std::cos(the) * std::sin(psi) - std::sin(phi) * std::sin(the) * std::cos(psi);
+    double w = std::cos(phi) * std::cos(the) * std::cos(psi) + std::sin(phi) * std::sin(the) * std::sin(psi);
+    const double s = std::sqrt(x * x + y * y + z * z + w * w);
+    x /= s; y /= s; z /= s; w /= s;
+    const double x2 = x * x, y2 = y * y, z2 = z * z, w2 = w * w;
+    M[0] = w2 + x2 - y2 - z2; M[1] = 2 * x * y - 2 * w * z; M[2] = 2 * x * z + 2 * w * y;
+    M[3] = 2 * x * y + 2 * w * z; M[4] = w2 - x2 + y2 - z2; M[5] = 2 * y * z - 2 * w * x;
+    M[6] = 2 * x * z - 2 * w * y; M[7] = 2 * y * z + 2 * w * x; M[8] = w2 - x2 - y2 + z2;
+}
+
+// Overflow-safe vector norm in the form KDL::Vector::Norm uses.
+double safe_norm(const double v[3]) {
+    const double a = std::fabs(v[0]), b = std::fabs(v[1]), c = std::fabs(v[2]);
+    auto sq = [](double t) { return t * t; };
+    if (a >= b) {
+        if (a >= c) {
+            if (a == 0) return 0;
+            return a * std::sqrt(1 + sq(v[1] / v[0]) + sq(v[2] / v[0]));
+        }
+        return c * std::sqrt(1 + sq(v[0] / v[2]) + sq(v[1] / v[2]));
+    }
+    if (b > c) return b * std::sqrt(1 + sq(v[0] / v[1]) + sq(v[2] / v[1]));
+    return c * std::sqrt(1 + sq(v[0] / v[2]) + sq(v[1] / v[2]));
+}
+
+template <typename T>
+hipError_t upload(std::vector<void*>& allocs, const T* src, size_t count, const T** dst) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    allocs.push_back(p);
+    if (count) e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+    *dst = static_cast<const T*>(p);
+    return e;
+}
+
+// FCL CollisionObject::computeAABB for a box (restated in workloads.py and
+// include/rvs.h): identity rotation -> pos -+ dim/2; any other rotation ->
+// pos -+ |dim/2| on every axis.
+void box_bounds(const rvs_box& b, float out[6]) {
+    const float w = b.quat_wxyz[0], x = b.quat_wxyz[1], y = b.quat_wxyz[2], z = b.quat_wxyz[3];
+    const float tx = 2.0f * x, ty = 2.0f * y, tz = 2.0f * z;
+    const float twx = tx * w, twy = ty * w, twz = tz * w;
+    const float txx = tx * x, txy = ty * x, txz = tz * x;
+    const float tyy = ty * y, tyz = tz * y, tzz = tz * z;
+    const float R[9] = {1.0f - (tyy + tzz), txy - twz, txz + twy, txy + twz, 1.0f - (txx + tzz),
+                        tyz - twx, txz - twy, tyz + twx, 1.0f - (txx + tyy)};
+    const float prec = 1e-5f;
+    bool ident = true;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const float v = R[3 * i + j];
+            if (i == j) ident = ident && (std::fabs(v - 1.0f) <= prec * std::min(std::fabs(v), 1.0f));
+            else ident = ident && (std::fabs(v) <= prec);
+        }
+    const float h[3] = {0.5f * b.dim[0], 0.5f * b.dim[1], 0.5f * b.dim[2]};
+    if (ident) {
+        for (int k = 0; k < 3; ++k) { out[k] = -h[k] + b.pos[k]; out[3 + k] = h[k] + b.pos[k]; }
+    } else {
+        const float m0 = -h[0], m1 = -h[1], m2 = -h[2];
+        const float rad = std::sqrt(m0 * m0 + (m1 * m1 + m2 * m2));
+        for (int k = 0; k < 3; ++k) { out[k] = b.pos[k] - rad; out[3 + k] = b.pos[k] + rad; }
+    }
+}
+
+void box_obb(const rvs_box& b, float out[15]) {
+    // centre, rotation rows (unit quaternion -> matrix), half extents
+    double w = b.quat_wxyz[0], x = b.quat_wxyz[1], y = b.quat_wxyz[2], z = b.quat_wxyz[3];
+    const double n = std::sqrt(w * w + x * x + y * y + z * z);
+    w /= n; x /= n; y /= n; z /= n;
+    const double R[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                         2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                         2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)};
+    for (int k = 0; k < 3; ++k) out[k] = b.pos[k];
+    for (int k = 0; k < 9; ++k) out[3 + k] = (float)R[k];
+    for (int k = 0; k < 3; ++k) out[12 + k] = 0.5f * b.dim[k];
+}
+
+int prec_of(const rvs_context* c, uint32_t flags) {
+    const uint32_t v = (flags >> RVS_F_PREC_SHIFT) & 3u;
+    return v ? (int)v - 1 : c->opt.precision;
+}
+
+QueryParams params_of(const rvs_context* c) {
+    QueryParams qp;
+    qp.eq = c->opt.equality_threshold;
+    qp.maxd = c->opt.max_distance;
+    qp.band = c->opt.band;
+    qp.num_checks = c->opt.num_checks;
+    return qp;
+}
+
+// Device arena carving: host arrays are staged into one allocation.
+struct Carve {
+    char* base;
+    size_t off = 0;
+    explicit Carve(void* b) : base(static_cast<char*>(b)) {}
+    template <typename T> T* take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+    static size_t need(std::initializer_list<size_t> bytes) {
+        size_t s = 0;
+        for (size_t b : bytes) s += ((b + 255) & ~size_t(255));
+        return s + 256;
+    }
+};
+
+bool robot_ok(rvs_context* c, int32_t id) { return id >= 0 && id < (int32_t)c->robots.size(); }
+bool env_ok(rvs_context* c, int32_t id) { return id >= 0 && id < (int32_t)c->envs.size(); }
+
+}  // namespace
+
+// ================================================================ C API
+extern "C" {
+
+const char* rvs_version(void) { return "rvs-mi355x 0.1 (gfx950)"; }
+
+void rvs_default_options(rvs_options* o) {
+    if (!o) return;
+    o->device = 0;
+    o->precision = RVS_PREC_REF32;
+    o->equality_threshold = 1e-4f;
+    o->max_distance = 1000.0f;
+    o->num_checks = 10;
+    o->band = 1e-4f;
+}
+
+rvs_status rvs_create(const rvs_options* opt, rvs_context** out) {
+    if (!out) return RVS_EINVAL;
+    *out = nullptr;
+    rvs_context* c = new (std::nothrow) rvs_context();
+    if (!c) return RVS_ENOMEM;
+    if (opt) c->opt = *opt;
+    else rvs_default_options(&c->opt);
+    if (c->opt.precision < 0 || c->opt.precision > RVS_PREC_FP32_REFINE) { delete c; return RVS_EINVAL; }
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0 || c->opt.device < 0 || c->opt.device >= ndev) {
+        delete c;
+        return RVS_EDEVICE;
+    }
+    e = hipSetDevice(c->opt.device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return RVS_EDEVICE; }
+    *out = c;
+    return RVS_OK;
+}
+
+void rvs_destroy(rvs_context* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->opt.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& r : c->robots) for (void* p : r.allocs) (void)hipFree(p);
+    for (auto& e : c->envs) for (void* p : e.allocs) (void)hipFree(p);
+    c->arena.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* rvs_last_error(const rvs_context* c) { return c ? c->err.c_str() : "null context"; }
+void* rvs_stream(rvs_context* c) { return c ? (void*)c->stream : nullptr; }
+int64_t rvs_last_refined(const rvs_context* c) { return c ? c->last_refined : 0; }
+
+rvs_status rvs_robot_load(rvs_context* c, const rvs_robot_desc* d, int32_t* robot_id) {
+    if (!c || !d || !robot_id || !d->joints) return fail(c, RVS_EINVAL, "rvs_robot_load: null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (d->num_joints <= 0 || d->num_joints > kMaxSeg) return fail(c, RVS_ERANGE, "rvs_robot_load: joint count");
+    if (d->num_links <= 0 || d->num_links > kMaxLinks) return fail(c, RVS_ERANGE, "rvs_robot_load: link count");
+    if (!d->capsule_radius) return fail(c, RVS_EINVAL, "rvs_robot_load: capsule radii missing");
+    std::vector<SegDev> seg(d->num_joints);
+    int ndof = 0;
+    for (int k = 0; k < d->num_joints; ++k) {
+        const rvs_joint& j = d->joints[k];
+        SegDev& s = seg[k];
+        std::memset(&s, 0, sizeof(s));
+        rpy_to_matrix(j.rpy, s.R);
+        for (int i = 0; i < 3; ++i) s.o[i] = j.xyz[i];
+        s.rev = (j.type == RVS_JOINT_REVOLUTE) ? 1 : 0;
+        ndof += s.rev;
+        double a[3];
+        for (int i = 0; i < 3; ++i) a[i] = s.R[3 * i] * j.axis[0] + s.R[3 * i + 1] * j.axis[1] + s.R[3 * i + 2] * j.axis[2];
+        const double nrm = safe_norm(a);
+        if (s.rev && !(nrm > 0)) return fail(c, RVS_EINVAL, "rvs_robot_load: zero joint axis");
+        for (int i = 0; i < 3; ++i) s.axis[i] = s.rev ? a[i] / nrm : 0.0;
+    }
+    const int L = d->num_links;
+    std::vector<PointDev> pts(L + 1);
+    for (auto& p : pts) std::memset(&p, 0, sizeof(p));
+    if (d->skeleton_kind == RVS_SKEL_CHAIN) {
+        if (d->num_joints < L + 1) return fail(c, RVS_EINVAL, "rvs_robot_load: chain needs num_links+1 segments");
+        for (int k = 0; k <= L; ++k) { pts[k].frame = k; pts[k].axis = 0; pts[k].flags = 0; }
+    } else if (d->skeleton_kind == RVS_SKEL_XARM6) {
+        if (L != 6 || ndof != 6) return fail(c, RVS_EINVAL, "rvs_robot_load: xArm6 recipe needs 6 revolute links");
+        const double g = (double)d->gripper_length;
+        pts[0].frame = -1;
+        pts[1].frame = 1;
+        pts[2].frame = 2;
+        pts[3].frame = 3; pts[3].axis = 2; pts[3].pre = -0.25; pts[3].flags = 1;
+        pts[4].frame = 4;
+        pts[5].frame = 4; pts[5].axis = 0; pts[5].pre = 0.076; pts[5].flags = 1;
+        pts[6].frame = 5; pts[6].axis = 2;
+        if (g != 0) { pts[6].pre = g; pts[6].post = 0.3 * g; pts[6].flags = 3; }
+    } else {
+        return fail(c, RVS_EINVAL, "rvs_robot_load: unknown skeleton kind");
+    }
+    std::vector<float> rad(d->capsule_radius, d->capsule_radius + L);
+    std::vector<float> rn32(L);
+    std::vector<double> rn64(L);
+    for (int i = 0; i < L; ++i) {
+        if (!(rad[i] >= 0)) return fail(c, RVS_EINVAL, "rvs_robot_load: negative capsule radius");
+        // collisionCapsuleToBox's inflated radius (CollisionAndDistance.cpp:7)
+        rn32[i] = (float)((double)rad[i] * std::sqrt(3.0) / 3.0);
+        rn64[i] = (double)rad[i] * std::sqrt(3.0) / 3.0;
+    }
+    std::vector<int32_t> pairs;
+    for (int m = 0; m < d->num_self_pairs; ++m) {
+        const int a = d->self_pairs[2 * m], b = d->self_pairs[2 * m + 1];
+        if (a < 0 || b < 0 || a >= L || b >= L) return fail(c, RVS_EINVAL, "rvs_robot_load: self pair out of range");
+        pairs.push_back(a);
+        pairs.push_back(b);
+    }
+    (void)hipSetDevice(c->opt.device);
+    RobotRec r;
+    hipError_t e = upload(r.allocs, seg.data(), seg.size(), &r.dev.seg);
+    if (e == hipSuccess) e = upload(r.allocs, pts.data(), pts.size(), &r.dev.pts);
+    if (e == hipSuccess) e = upload(r.allocs, rad.data(), rad.size(), &r.dev.radius);
+    if (e == hipSuccess) e = upload(r.allocs, rn32.data(), rn32.size(), &r.dev.rnew32);
+    if (e == hipSuccess) e = upload(r.allocs, rn64.data(), rn64.size(), &r.dev.rnew64);
+    if (e == hipSuccess) e = upload(r.allocs, pairs.data(), pairs.size(), &r.dev.pairs);
+    if (e != hipSuccess) {
+        for (void* p : r.allocs) (void)hipFree(p);
+        return hip_fail(c, e, "rvs_robot_load");
+    }
+    r.dev.num_segments = d->num_joints;
+    r.dev.num_dofs = ndof;
+    r.dev.num_links = L;
+    r.dev.ground_included = d->ground_included;
+    r.dev.num_pairs = (int32_t)pairs.size() / 2;
+    r.num_dofs = ndof;
+    r.num_links = L;
+    r.num_pairs = r.dev.num_pairs;
+    c->robots.push_back(r);
+    *robot_id = (int32_t)c->robots.size() - 1;
+    return RVS_OK;
+}
+
+rvs_status rvs_env_create(rvs_context* c, const rvs_box* boxes, int32_t num, int32_t semantics, int32_t* env_id) {
+    if (!c || !env_id || (num > 0 && !boxes)) return fail(c, RVS_EINVAL, "rvs_env_create: null argument");
+    if (num < 0 || num > kMaxLdsObstacles) return fail(c, RVS_ERANGE, "rvs_env_create: at most 1024 boxes");
+    if (semantics != RVS_ENV_AABB && semantics != RVS_ENV_OBB) return fail(c, RVS_EINVAL, "rvs_env_create: semantics");
+    std::lock_guard<std::mutex> lk(c->mu);
+    EnvRec r;
+    r.semantics = semantics;
+    r.boxes.assign(boxes, boxes + num);
+    r.aabb.resize((size_t)num * 6);
+    std::vector<float> tol(num), obb((size_t)num * 15);
+    std::vector<uint8_t> fl(num);
+    for (int j = 0; j < num; ++j) {
+        box_bounds(boxes[j], &r.aabb[6 * j]);
+        box_obb(boxes[j], &obb[15 * j]);
+        tol[j] = boxes[j].min_dist_tol;
+        fl[j] = (uint8_t)((boxes[j].flags & RVS_BOX_GROUND ? kObjGround : 0) |
+                          (boxes[j].flags & RVS_BOX_DYNAMIC ? kObjDynamic : 0));
+    }
+    (void)hipSetDevice(c->opt.device);
+    hipError_t e = upload(r.allocs, r.aabb.data(), r.aabb.size(), &r.dev.aabb);
+    if (e == hipSuccess) e = upload(r.allocs, tol.data(), tol.size(), &r.dev.tol);
+    if (e == hipSuccess) e = upload(r.allocs, fl.data(), fl.size(), &r.dev.flags);
+    if (e == hipSuccess) e = upload(r.allocs, obb.data(), obb.size(), &r.dev.obb);
+    if (e != hipSuccess) {
+        for (void* p : r.allocs) (void)hipFree(p);
+        return hip_fail(c, e, "rvs_env_create");
+    }
+    r.dev.num_obj = num;
+    r.dev.use_obb = semantics == RVS_ENV_OBB;
+    c->envs.push_back(std::move(r));
+    *env_id = (int32_t)c->envs.size() - 1;
+    return RVS_OK;
+}
+
+rvs_status rvs_env_update(rvs_context* c, int32_t env_id, const int32_t* idx, const float* pos, int32_t count,
+                          uint32_t flags) {
+    if (!c || (count > 0 && (!idx || !pos))) return fail(c, RVS_EINVAL, "rvs_env_update: null argument");
+    if (flags & RVS_F_DEVICE) return fail(c, RVS_ENOTSUP, "rvs_env_update: host arrays only");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!env_ok(c, env_id)) return fail(c, RVS_EINVAL, "rvs_env_update: bad env id");
+    EnvRec& r = c->envs[env_id];
+    for (int k = 0; k < count; ++k)
+        if (idx[k] < 0 || idx[k] >= (int)r.boxes.size()) return fail(c, RVS_EINVAL, "rvs_env_update: bad box index");
+    (void)hipSetDevice(c->opt.device);
+    for (int k = 0; k < count; ++k) {
+        const int j = idx[k];
+        for (int i = 0; i < 3; ++i) r.boxes[j].pos[i] = pos[3 * k + i];
+        box_bounds(r.boxes[j], &r.aabb[6 * j]);
+        hipError_t e = hipMemcpyAsync((void*)(r.dev.aabb + 6 * j), &r.aabb[6 * j], 6 * sizeof(float),
+                                      hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            float o[15];
+            box_obb(r.boxes[j], o);
+            e = hipMemcpyAsync((void*)(r.dev.obb + 15 * j), o, 15 * sizeof(float), hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        }
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_env_update");
+    }
+    return RVS_OK;
+}
+
+rvs_status rvs_env_bounds(rvs_context* c, int32_t env_id, float* aabb, int32_t num) {
+    if (!c || !aabb) return fail(c, RVS_EINVAL, "rvs_env_bounds: null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!env_ok(c, env_id)) return fail(c, RVS_EINVAL, "rvs_env_bounds: bad env id");
+    const EnvRec& r = c->envs[env_id];
+    if (num != (int)r.boxes.size()) return fail(c, RVS_EINVAL, "rvs_env_bounds: size mismatch");
+    std::memcpy(aabb, r.aabb.data(), r.aabb.size() * sizeof(float));
+    return RVS_OK;
+}
+
+// ---------------------------------------------------------------- queries
+namespace {
+
+rvs_status check_query(rvs_context* c, int32_t robot, int32_t env, const void* q, int64_t n, bool need_env) {
+    if (!c) return RVS_EINVAL;
+    if (!robot_ok(c, robot)) return fail(c, RVS_EINVAL, "bad robot id");
+    if (need_env && !env_ok(c, env)) return fail(c, RVS_EINVAL, "bad env id");
+    if (n < 0) return fail(c, RVS_EINVAL, "negative configuration count");
+    if (n > 0 && !q) return fail(c, RVS_EINVAL, "null configuration array");
+    return RVS_OK;
+}
+
+hipError_t h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+}
+hipError_t d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    return (bytes && dst) ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
+}
+
+}  // namespace
+
+rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n, float* d,
+                        float* profile, float* nearest, uint8_t* verdict, uint32_t flags) {
+    rvs_status s = check_query(c, robot, env, q, n, true);
+    if (s != RVS_OK) return s;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_distance: oriented-box environments need rvs_distance_obb");
+    const int prec = prec_of(c, flags);
+    if (prec == RVS_PREC_FP32_REFINE) return fail(c, RVS_ENOTSUP, "rvs_distance: FP32_REFINE not available yet");
+    if (n == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
+    const QueryParams qp = params_of(c);
+    DistOut out{};
+    const float* qd = q;
+    hipError_t e = hipSuccess;
+    if (flags & RVS_F_DEVICE) {
+        out.d = d; out.profile = profile; out.nearest = nearest; out.verdict = verdict;
+    } else {
+        const size_t need = Carve::need({(size_t)n * D * 4, (size_t)n * 4, profile ? (size_t)n * L * 4 : 0,
+                                         nearest ? (size_t)n * M * L * 24 : 0, (size_t)n});
+        e = c->arena.ensure(need);
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_distance: staging");
+        Carve cv(c->arena.p);
+        float* qs = cv.take<float>((size_t)n * D);
+        out.d = cv.take<float>(n);
+        out.profile = profile ? cv.take<float>((size_t)n * L) : nullptr;
+        out.nearest = nearest ? cv.take<float>((size_t)n * M * L * 6) : nullptr;
+        out.verdict = cv.take<uint8_t>(n);
+        e = h2d(qs, q, (size_t)n * D * 4, c->stream);
+        qd = qs;
+    }
+    if (e == hipSuccess) {
+        e = (prec == RVS_PREC_FP64) ? launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream)
+                                    : launch_distance_ref32(rb.dev, ev.dev, qp, qd, n, out, c->stream);
+    }
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
+        e = d2h(d, out.d, (size_t)n * 4, c->stream);
+        if (e == hipSuccess) e = d2h(profile, out.profile, (size_t)n * L * 4, c->stream);
+        if (e == hipSuccess) e = d2h(nearest, out.nearest, (size_t)n * M * L * 24, c->stream);
+        if (e == hipSuccess) e = d2h(verdict, out.verdict, (size_t)n, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_distance");
+    return RVS_OK;
+}
+
+rvs_status rvs_distance_f64(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n, double* d,
+                            double* profile, uint8_t* verdict, uint32_t flags) {
+    rvs_status s = check_query(c, robot, env, q, n, true);
+    if (s != RVS_OK) return s;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_distance_f64: oriented-box environments not supported");
+    if (n == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const int L = rb.num_links, D = rb.num_dofs;
+    const QueryParams qp = params_of(c);
+    DistOut out{};
+    const float* qd = q;
+    hipError_t e = hipSuccess;
+    if (flags & RVS_F_DEVICE) {
+        out.d64 = d; out.profile64 = profile; out.verdict = verdict;
+    } else {
+        const size_t need = Carve::need({(size_t)n * D * 4, (size_t)n * 8, profile ? (size_t)n * L * 8 : 0, (size_t)n});
+        e = c->arena.ensure(need);
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_f64: staging");
+        Carve cv(c->arena.p);
+        float* qs = cv.take<float>((size_t)n * D);
+        out.d64 = cv.take<double>(n);
+        out.profile64 = profile ? cv.take<double>((size_t)n * L) : nullptr;
+        out.verdict = cv.take<uint8_t>(n);
+        e = h2d(qs, q, (size_t)n * D * 4, c->stream);
+        qd = qs;
+    }
+    if (e == hipSuccess) e = launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
+        e = d2h(d, out.d64, (size_t)n * 8, c->stream);
+        if (e == hipSuccess) e = d2h(profile, out.profile64, (size_t)n * L * 8, c->stream);
+        if (e == hipSuccess) e = d2h(verdict, out.verdict, (size_t)n, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_f64");
+    return RVS_OK;
+}
+
+rvs_status rvs_is_valid(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n, uint8_t* valid,
+                        uint32_t flags) {
+    rvs_status s = check_query(c, robot, env, q, n, true);
+    if (s != RVS_OK) return s;
+    if (n > 0 && !valid) return fail(c, RVS_EINVAL, "rvs_is_valid: null output");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_is_valid: oriented-box environments not supported");
+    if (n == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const int D = rb.num_dofs;
+    const int fp64 = prec_of(c, flags) == RVS_PREC_FP64;
+    const QueryParams qp = params_of(c);
+    hipError_t e = hipSuccess;
+    const float* qd = q;
+    uint8_t* vd = valid;
+    if (!(flags & RVS_F_DEVICE)) {
+        e = c->arena.ensure(Carve::need({(size_t)n * D * 4, (size_t)n}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_is_valid: staging");
+        Carve cv(c->arena.p);
+        float* qs = cv.take<float>((size_t)n * D);
+        vd = cv.take<uint8_t>(n);
+        e = h2d(qs, q, (size_t)n * D * 4, c->stream);
+        qd = qs;
+    }
+    if (e == hipSuccess) e = launch_is_valid(rb.dev, ev.dev, qp, qd, n, vd, fp64, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(valid, vd, (size_t)n, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_is_valid");
+    return RVS_OK;
+}
+
+rvs_status rvs_distance_underestimation(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n,
+                                        const float* nearest_ref, int64_t ref_stride, float* d, float* profile,
+                                        uint8_t* stored, uint32_t flags) {
+    rvs_status s = check_query(c, robot, env, q, n, true);
+    if (s != RVS_OK) return s;
+    if (n > 0 && (!nearest_ref || !d)) return fail(c, RVS_EINVAL, "rvs_distance_underestimation: null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
+    const int64_t set = (int64_t)M * L * 6;
+    if (ref_stride != 0 && ref_stride != set) return fail(c, RVS_EINVAL, "rvs_distance_underestimation: ref_stride");
+    if (n == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const QueryParams qp = params_of(c);
+    hipError_t e = hipSuccess;
+    const float* qd = q;
+    const float* npd = nearest_ref;
+    float *dd = d, *pd = profile;
+    uint8_t* sd = stored;
+    if (!(flags & RVS_F_DEVICE)) {
+        const size_t nref = ref_stride ? (size_t)n * set : (size_t)set;
+        e = c->arena.ensure(Carve::need({(size_t)n * D * 4, nref * 4, (size_t)n * 4, profile ? (size_t)n * L * 4 : 0,
+                                         (size_t)n}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_underestimation: staging");
+        Carve cv(c->arena.p);
+        float* qs = cv.take<float>((size_t)n * D);
+        float* ns = cv.take<float>(nref);
+        dd = cv.take<float>(n);
+        pd = profile ? cv.take<float>((size_t)n * L) : nullptr;
+        sd = cv.take<uint8_t>(n);
+        e = h2d(qs, q, (size_t)n * D * 4, c->stream);
+        if (e == hipSuccess) e = h2d(ns, nearest_ref, nref * 4, c->stream);
+        qd = qs;
+        npd = ns;
+    }
+    if (e == hipSuccess) e = launch_underestimation(rb.dev, ev.dev, qp, qd, n, npd, ref_stride, dd, pd, sd, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
+        e = d2h(d, dd, (size_t)n * 4, c->stream);
+        if (e == hipSuccess) e = d2h(profile, pd, (size_t)n * L * 4, c->stream);
+        if (e == hipSuccess) e = d2h(stored, sd, (size_t)n, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_underestimation");
+    return RVS_OK;
+}
+
+rvs_status rvs_self_distance(rvs_context* c, int32_t robot, const float* q, int64_t n, float* dist, uint32_t flags) {
+    rvs_status s = check_query(c, robot, -1, q, n, false);
+    if (s != RVS_OK) return s;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    if (n == 0 || rb.num_pairs == 0) return RVS_OK;
+    if (!dist) return fail(c, RVS_EINVAL, "rvs_self_distance: null output");
+    (void)hipSetDevice(c->opt.device);
+    const int D = rb.num_dofs, P = rb.num_pairs;
+    const QueryParams qp = params_of(c);
+    hipError_t e = hipSuccess;
+    const float* qd = q;
+    float* od = dist;
+    if (!(flags & RVS_F_DEVICE)) {
+        e = c->arena.ensure(Carve::need({(size_t)n * D * 4, (size_t)n * P * 4}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_self_distance: staging");
+        Carve cv(c->arena.p);
+        float* qs = cv.take<float>((size_t)n * D);
+        od = cv.take<float>((size_t)n * P);
+        e = h2d(qs, q, (size_t)n * D * 4, c->stream);
+        qd = qs;
+    }
+    if (e == hipSuccess) e = launch_self_distance(rb.dev, qp, qd, n, od, nullptr, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(dist, od, (size_t)n * P * 4, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_self_distance");
+    return RVS_OK;
+}
+
+rvs_status rvs_edge_is_valid(rvs_context* c, int32_t, int32_t, const float*, const float*, int64_t, uint8_t*,
+                             uint32_t) {
+    return fail(c, RVS_ENOTSUP, "rvs_edge_is_valid: not available yet");
+}
+
+rvs_status rvs_edge_first_collision(rvs_context* c, int32_t, const int32_t*, const float*, const float*,
+                                    const int32_t*, int64_t, int32_t, float, int32_t*, uint32_t) {
+    return fail(c, RVS_ENOTSUP, "rvs_edge_first_collision: not available yet");
+}
+
+}  // extern "C"

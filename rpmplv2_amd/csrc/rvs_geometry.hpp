@@ -1,0 +1,483 @@
+// Capsule geometry for the gfx950 kernels: capsule-box distance, capsule-box
+// collision, segment-segment and segment-point distance.
+//
+// Semantics follow RPMPLv2's CollisionAndDistance.cpp (cited per function);
+// the control flow is re-shaped for 64-wide wavefronts.  The reference walks
+// a deep case tree and calls the segment-segment routine from a dozen call
+// sites; here every case first selects a small work list (faces to test for a
+// crossing, box edges to measure, or one corner point) and then ONE loop runs
+// the segment-segment routine over the list.  Lanes that took different cases
+// therefore share the same instructions and only differ in trip count.
+//
+// Every arithmetic expression keeps the reference's operation order: 3-vector
+// dot products sum as e0 + (e1 + e2), 2-vectors as e0 + e1, lerps are
+// a + t * (b - a) per component, and this file is compiled without FMA
+// contraction with IEEE division and square root.  With T = float the results
+// are bit-identical to the reference's float arithmetic; with T = double the
+// same algorithm runs in FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace rvs {
+
+template <typename T>
+struct Vec3 {
+    T x, y, z;
+    __host__ __device__ T operator[](int k) const { return k == 0 ? x : (k == 1 ? y : z); }
+};
+
+template <typename T> __host__ __device__ inline Vec3<T> mk(T x, T y, T z) { return Vec3<T>{x, y, z}; }
+template <typename T> __host__ __device__ inline Vec3<T> operator-(const Vec3<T>& a, const Vec3<T>& b) {
+    return Vec3<T>{a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <typename T> __host__ __device__ inline T dot(const Vec3<T>& a, const Vec3<T>& b) {
+    return a.x * b.x + (a.y * b.y + a.z * b.z);
+}
+template <typename T> __host__ __device__ inline T sqnorm(const Vec3<T>& a) { return dot(a, a); }
+// IEEE square root of the operand's own precision
+__host__ __device__ inline float vsqrt(float x) { return sqrtf(x); }
+__host__ __device__ inline double vsqrt(double x) { return sqrt(x); }
+template <typename T> __host__ __device__ inline T vnorm(const Vec3<T>& a) { return vsqrt(sqnorm(a)); }
+// a + s * (b - a)
+template <typename T> __host__ __device__ inline Vec3<T> lerp(const Vec3<T>& a, const Vec3<T>& b, T s) {
+    return Vec3<T>{a.x + s * (b.x - a.x), a.y + s * (b.y - a.y), a.z + s * (b.z - a.z)};
+}
+template <typename T> __host__ __device__ inline T smin(T a, T b) { return (b < a) ? b : a; }  // std::min
+
+// Axis-aligned box (xmin, ymin, zmin, xmax, ymax, zmax).
+template <typename T>
+struct Box {
+    T lo[3], hi[3];
+    __host__ __device__ T f(int face) const { return face < 3 ? lo[face] : hi[face - 3]; }
+    // corner c: bit0 -> x max, bit1 -> y max, bit2 -> z max
+    __host__ __device__ Vec3<T> corner(int c) const {
+        return Vec3<T>{(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+    }
+};
+
+template <typename T>
+struct Params {
+    T eq;    // EQUALITY_THRESHOLD
+    T maxd;  // MAX_DISTANCE
+};
+
+// Nearest points (robot point, obstacle point) of one pair.
+template <typename T>
+struct NP {
+    Vec3<T> r, o;
+    bool valid;
+};
+
+// ---------------------------------------------------------------- seg-seg
+// distanceLineSegToLineSeg, CollisionAndDistance.cpp:217-320.  Returns the
+// distance of AB to CD; np receives (point on AB, point on CD), invalid when
+// the reference returns no nearest points.
+template <typename T, bool WANT_NP>
+__host__ __device__ inline T seg_seg(const Vec3<T>& A, const Vec3<T>& B, const Vec3<T>& C, const Vec3<T>& D,
+                                     const Params<T>& p, NP<T>* np) {
+    const Vec3<T> BA = B - A, DC = D - C, CD = C - D, AC = A - C, AB = A - B;
+    const T alpha1 = sqnorm(BA);
+    const T alpha2 = dot(BA, DC);
+    const T beta1 = dot(CD, BA);
+    const T beta2 = dot(CD, DC);
+    const T gamma1 = dot(AC, AB);
+    const T gamma2 = dot(AC, CD);
+    const T s = (alpha1 * gamma2 - alpha2 * gamma1) / (alpha1 * beta2 - alpha2 * beta1);
+    const T t = (gamma1 - beta1 * s) / alpha1;
+    T d_c = p.maxd;
+    if (t > 0 && t < 1 && s > 0 && s < 1) {
+        const Vec3<T> P1 = lerp(A, B, t), P2 = lerp(C, D, s);
+        d_c = vnorm(P2 - P1);
+        if (d_c < p.eq) {
+            if (WANT_NP) np->valid = false;
+            return T(0);
+        }
+        if (WANT_NP) { np->r = P1; np->o = P2; np->valid = true; }
+        return d_c;
+    }
+    // the four clamped candidates: C and D projected on AB, A and B on CD
+    const T alpha3 = sqnorm(CD);
+    const Vec3<T> AD = A - D, BC = B - C;
+    const T opt0 = dot(AC, AB) / alpha1;
+    const T opt1 = dot(AD, AB) / alpha1;
+    const T opt2 = dot(AC, DC) / alpha3;
+    const T opt3 = dot(BC, DC) / alpha3;
+    if (WANT_NP) np->valid = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const T o = (i == 0) ? opt0 : (i == 1) ? opt1 : (i == 2) ? opt2 : opt3;
+        // end points chosen when the parameter is clamped (lines 251-286)
+        Vec3<T> p0, p1;
+        if (o < 0) {
+            p0 = (i == 3) ? B : A;
+            p1 = (i == 1) ? D : C;
+        } else if (o > 1) {
+            p0 = (i == 2) ? A : B;
+            p1 = (i == 0) ? C : D;
+        } else if (i < 2) {
+            p0 = lerp(A, B, o);
+            p1 = (i == 0) ? C : D;
+        } else {
+            p0 = (i == 2) ? A : B;
+            p1 = lerp(C, D, o);
+        }
+        const T d_tmp = vnorm(p1 - p0);
+        if (d_tmp < d_c) {
+            d_c = d_tmp;
+            if (WANT_NP) { np->r = p0; np->o = p1; np->valid = true; }
+        }
+    }
+    return d_c;
+}
+
+// Squared sine of the angle between AB and CD, from the same quantities the
+// reference divides by; small values flag an ill-conditioned interior solve.
+template <typename T>
+__host__ __device__ inline T seg_seg_sin2(const Vec3<T>& A, const Vec3<T>& B, const Vec3<T>& C, const Vec3<T>& D) {
+    const Vec3<T> u = B - A, v = D - C;
+    const T uu = sqnorm(u), vv = sqnorm(v), uv = dot(u, v);
+    const T den = uu * vv;
+    return den > 0 ? (den - uv * uv) / den : T(1);
+}
+
+// ---------------------------------------------------------------- seg-pt
+// distanceLineSegToPoint, CollisionAndDistance.cpp:323-342
+template <typename T, bool WANT_NP>
+__host__ __device__ inline T seg_pt(const Vec3<T>& A, const Vec3<T>& B, const Vec3<T>& C, const Params<T>& p,
+                                    NP<T>* np) {
+    const Vec3<T> BA = B - A;
+    const T t_opt = dot(C - A, BA) / sqnorm(BA);
+    const Vec3<T> Pn = (t_opt < 0) ? A : ((t_opt > 1) ? B : lerp(A, B, t_opt));
+    const T d_c = vnorm(C - Pn);
+    if (d_c < p.eq) {
+        if (WANT_NP) np->valid = false;
+        return T(0);
+    }
+    if (WANT_NP) { np->r = Pn; np->o = C; np->valid = true; }
+    return d_c;
+}
+
+// ---------------------------------------------------------------- faces
+// Segment AB crosses the open rectangle of face 'face' (0..5 = xmin, ymin,
+// zmin, xmax, ymax, zmax).  This is collisionCapsuleToRectangle
+// (CollisionAndDistance.cpp:46-111) at radius 0, where only the plane-crossing
+// test of lines 57-65 can report a hit.
+template <typename T>
+__host__ __device__ inline bool face_crossed(const Vec3<T>& A, const Vec3<T>& B, const Box<T>& b, int face,
+                                             const Params<T>& p) {
+    const int c = face % 3;
+    const int o1 = (c == 0) ? 1 : 0, o2 = (c == 2) ? 1 : 2;
+    const T v = b.f(face);
+    const T den = B[c] - A[c];
+    const T t = (den != 0) ? (v - A[c]) / den : p.maxd;
+    if (!(t > 0 && t < 1)) return false;
+    const T m0 = A[o1] + t * (B[o1] - A[o1]);
+    const T m1 = A[o2] + t * (B[o2] - A[o2]);
+    return m0 > b.lo[o1] && m0 < b.hi[o1] && m1 > b.lo[o2] && m1 < b.hi[o2];
+}
+
+// Work list entries: an edge joins two box corners (3-bit corner indices).
+// Encodings: edge e = c0 | (c1 << 3).
+__host__ __device__ inline int edge_of(int c0, int c1) { return c0 | (c1 << 3); }
+
+// Ring of the four edges of face 'face', in the reference's order
+// (checkOtherCases, CollisionAndDistance.cpp:631-634, 693-696, 721-724).
+// The ring starts at the face's (min,min) corner and walks through the
+// (max of the first free axis) corner.
+__host__ __device__ inline void face_ring(int face, int* corners /*4*/) {
+    const int c = face % 3;
+    const int fix = (face >= 3) ? (1 << c) : 0;
+    // free axes in increasing order
+    const int a1 = (c == 0) ? 1 : 0, a2 = (c == 2) ? 1 : 2;
+    corners[0] = fix;
+    corners[1] = fix | (1 << a1);
+    corners[2] = fix | (1 << a1) | (1 << a2);
+    corners[3] = fix | (1 << a2);
+}
+
+// Which corners a face edge uses for the checkEdges step: getLineSegments
+// (CollisionAndDistance.cpp:542-574) picks the edge(s) of the face rectangle
+// on the side where the other end point lies, first along the first free
+// axis, then along the second.  Returns the number of edges (0..2).
+template <typename T>
+__host__ __device__ inline int face_side_edges(const Box<T>& b, int face, const Vec3<T>& other, int* edges) {
+    const int c = face % 3;
+    const int fix = (face >= 3) ? (1 << c) : 0;
+    const int a1 = (c == 0) ? 1 : 0, a2 = (c == 2) ? 1 : 2;
+    const int b1 = 1 << a1, b2 = 1 << a2;
+    int n = 0;
+    const T u = other[a1], v = other[a2];
+    if (u < b.lo[a1]) edges[n++] = edge_of(fix, fix | b2);
+    else if (u > b.hi[a1]) edges[n++] = edge_of(fix | b1, fix | b1 | b2);
+    if (v < b.lo[a2]) edges[n++] = edge_of(fix, fix | b1);
+    else if (v > b.hi[a2]) edges[n++] = edge_of(fix | b2, fix | b1 | b2);
+    return n;
+}
+
+// ---------------------------------------------------------------- capsule-box
+// distanceCapsuleToBox -> CapsuleToBox::compute, CollisionAndDistance.cpp:
+// 208-214, 395-755.  Returns d - radius (or 0 when an end point lies strictly
+// inside the box).  'status' bit 0 is set when the reference would
+// dereference a missing nearest-point matrix (end point projections more than
+// MAX_DISTANCE away, or an end point exactly on a face plane with the other
+// inside the face's span); the distance is then NaN.
+template <typename T, bool WANT_NP>
+__host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T radius, const Box<T>& b,
+                                         const Params<T>& p, NP<T>* np, T* sin2_min, uint32_t* status) {
+    // --- projectionLineSegOnSide x3 (lines 411-413, 450-474)
+    int pm[2] = {0, 0};
+    T dist[2] = {p.maxd, p.maxd};
+    bool inside = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int o1 = (k == 0) ? 1 : 0, o2 = (k == 2) ? 1 : 2;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const Vec3<T>& X = i ? B : A;
+            if (X[o1] >= b.lo[o1] && X[o1] <= b.hi[o1] && X[o2] >= b.lo[o2] && X[o2] <= b.hi[o2]) {
+                if (X[k] > b.lo[k] && X[k] < b.hi[k]) inside = true;
+                else if (X[k] <= b.lo[k]) { pm[i] |= 1 << k; dist[i] = b.lo[k] - X[k]; }
+                else if (X[k] >= b.hi[k]) { pm[i] |= 8 << k; dist[i] = X[k] - b.hi[k]; }
+            }
+        }
+    }
+    if (inside) {
+        if (WANT_NP) np->valid = false;
+        return T(0);
+    }
+
+    // --- work list
+    T d_c = p.maxd;
+    int nfaces = 0;       // faces to test for a crossing: 0, 1 or 6
+    int face1 = 0;        // the single face when nfaces == 1
+    int nedges = 0;
+    int edges[12];
+    bool direct = false;  // single edge / corner assigned directly (tie(...))
+    int corner = -1;      // corner point case (segment-point distance)
+    bool no_edge = false;
+
+    const int both = pm[0] & pm[1];
+    const int any = pm[0] | pm[1];
+    if (any) {
+        const int idx = (dist[0] < dist[1]) ? 0 : 1;
+        d_c = smin(dist[0], dist[1]);
+        const Vec3<T>& X = idx ? B : A;
+        const int m = pm[idx];
+        if (WANT_NP) {
+            // first face (0..5 order) of the chosen point's projections
+            int fc = 0;
+            for (int f = 0; f < 6; ++f) if ((m >> f) & 1) { fc = f; break; }
+            const int ax = fc % 3;
+            np->r = X;
+            np->o = X;
+            const T v = b.f(fc);
+            if (ax == 0) np->o.x = v; else if (ax == 1) np->o.y = v; else np->o.z = v;
+            np->valid = true;
+        }
+        if (!both) {
+            // checkEdges (lines 476-540): face order xmin, xmax, ymin, ymax, zmin, zmax
+            int face = -1;
+            if (m & 1) face = 0; else if (m & 8) face = 3; else if (m & 2) face = 1;
+            else if (m & 16) face = 4; else if (m & 4) face = 2; else if (m & 32) face = 5;
+            if (face < 0) { if (status) *status |= 1u; return T(NAN); }
+            nfaces = 1;
+            face1 = face;
+            nedges = face_side_edges(b, face, idx ? A : B, edges);
+            // no edge: the reference would read an uninitialised matrix
+            // (only if the face is not crossed, which is tested first)
+            no_edge = (nedges == 0);
+        }
+    } else {
+        // checkOtherCases (lines 597-755): classify each axis as both-below,
+        // both-above or mixed
+        int s[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            s[k] = (A[k] < b.lo[k] && B[k] < b.lo[k]) ? 0 : ((A[k] > b.hi[k] && B[k] > b.hi[k]) ? 1 : 2);
+        // the reference tests x, then y, then z; an axis is "fixed" when both
+        // points lie beyond the same side
+        const int nfix = (s[0] != 2) + (s[1] != 2) + (s[2] != 2);
+        if (nfix == 3) {
+            corner = (s[0] ? 1 : 0) | (s[1] ? 2 : 0) | (s[2] ? 4 : 0);
+            direct = true;
+        } else if (nfix == 2) {
+            // edge along the free axis through the fixed coordinates
+            int base = 0, free = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { if (s[k] == 2) free = k; else base |= s[k] << k; }
+            edges[0] = edge_of(base, base | (1 << free));
+            nedges = 1;
+            direct = true;
+        } else if (nfix == 1) {
+            int k = (s[0] != 2) ? 0 : ((s[1] != 2) ? 1 : 2);
+            int ring[4];
+            face_ring(k + 3 * s[k], ring);
+            for (int e = 0; e < 4; ++e) edges[e] = edge_of(ring[e], ring[(e + 1) & 3]);
+            nedges = 4;
+        } else {
+            nfaces = 6;
+            // bottom ring, top ring, then the four vertical edges
+            int ring[4];
+            face_ring(2, ring);
+            for (int e = 0; e < 4; ++e) edges[e] = edge_of(ring[e], ring[(e + 1) & 3]);
+            face_ring(5, ring);
+            for (int e = 0; e < 4; ++e) edges[4 + e] = edge_of(ring[e], ring[(e + 1) & 3]);
+            const int vx[4] = {0, 1, 3, 2};
+            for (int e = 0; e < 4; ++e) edges[8 + e] = edge_of(vx[e], vx[e] | 4);
+            nedges = 12;
+        }
+    }
+
+    // --- crossing tests: any hit makes the distance 0 (then -radius)
+    bool hit = false;
+    if (nfaces == 1) hit = face_crossed(A, B, b, face1, p);
+    else if (nfaces == 6)
+        for (int f = 0; f < 6; ++f) hit = hit || face_crossed(A, B, b, f, p);
+
+    if (hit) {
+        d_c = T(0);
+    } else if (no_edge) {
+        if (status) *status |= 1u;
+        return T(NAN);
+    } else if (corner >= 0) {
+        NP<T> t;
+        d_c = seg_pt<T, WANT_NP>(A, B, b.corner(corner), p, &t);
+        if (WANT_NP) *np = t;
+    } else {
+        for (int e = 0; e < nedges; ++e) {
+            const Vec3<T> C = b.corner(edges[e] & 7), D = b.corner(edges[e] >> 3);
+            NP<T> t;
+            const T d_tmp = seg_seg<T, WANT_NP>(A, B, C, D, p, &t);
+            if (sin2_min) *sin2_min = smin(*sin2_min, seg_seg_sin2(A, B, C, D));
+            if (direct) {
+                d_c = d_tmp;
+                if (WANT_NP) *np = t;
+            } else if (d_tmp <= 0) {
+                d_c = T(0);
+                break;
+            } else if (d_tmp < d_c) {
+                d_c = d_tmp;
+                if (WANT_NP) *np = t;
+            }
+        }
+    }
+    return d_c - radius;
+}
+
+// ---------------------------------------------------------------- collision
+// checkCases, CollisionAndDistance.cpp:113-148: distance of AB to the one or
+// two rectangle edges on the side of point 'pt' (2-D coordinates u, v).
+template <typename T>
+__host__ __device__ inline T check_cases(const Vec3<T>& A, const Vec3<T>& B, const Box<T>& b, int face, T u, T v,
+                                         const Params<T>& p) {
+    const int c = face % 3;
+    const int fix = (face >= 3) ? (1 << c) : 0;
+    const int a1 = (c == 0) ? 1 : 0, a2 = (c == 2) ? 1 : 2;
+    const int b1 = 1 << a1, b2 = 1 << a2;
+    T d1 = p.maxd, d2 = p.maxd;
+    int e = -1;
+    if (u < b.lo[a1]) e = edge_of(fix, fix | b2);
+    else if (u > b.hi[a1]) e = edge_of(fix | b1, fix | b1 | b2);
+    if (e >= 0) d1 = seg_seg<T, false>(A, B, b.corner(e & 7), b.corner(e >> 3), p, nullptr);
+    e = -1;
+    if (d1 > 0 && v < b.lo[a2]) e = edge_of(fix, fix | b1);
+    else if (d1 > 0 && v > b.hi[a2]) e = edge_of(fix | b2, fix | b1 | b2);
+    if (e >= 0) d2 = seg_seg<T, false>(A, B, b.corner(e & 7), b.corner(e >> 3), p, nullptr);
+    return smin(d1, d2);
+}
+
+// collisionCapsuleToRectangle, CollisionAndDistance.cpp:46-111, general radius
+template <typename T>
+__host__ __device__ inline bool rect_collision(const Vec3<T>& A, const Vec3<T>& B, T radius, const Box<T>& b,
+                                               int face, const Params<T>& p) {
+    const int c = face % 3;
+    const int o1 = (c == 0) ? 1 : 0, o2 = (c == 2) ? 1 : 2;
+    const T v = b.f(face);
+    // rec = (min1, min2, max1, max2)
+    const T r0 = b.lo[o1], r1 = b.lo[o2], r2 = b.hi[o1], r3 = b.hi[o2];
+    const T Au = A[o1], Av = A[o2], Bu = B[o1], Bv = B[o2];
+    const T den = B[c] - A[c];
+    const T t = (den != 0) ? (v - A[c]) / den : p.maxd;
+    const T M0 = Au + t * (Bu - Au), M1 = Av + t * (Bv - Av);
+    // |A - A_proj| and |B - B_proj|, A_proj = A with coordinate c set to v
+    Vec3<T> Ap = A, Bp = B;
+    if (c == 0) { Ap.x = v; Bp.x = v; } else if (c == 1) { Ap.y = v; Bp.y = v; } else { Ap.z = v; Bp.z = v; }
+    const T nA = vnorm(A - Ap), nB = vnorm(B - Bp);
+    if (t > 0 && t < 1) {
+        if (M0 > r0 - radius && M0 < r2 + radius && M1 > r1 - radius && M1 < r3 + radius) {
+            // the corner tests keep the reference's index pattern (lines 69-72)
+            const T c00 = M0 - r0, c01 = M1 - r2, c11 = M1 - r3, c20 = M0 - r1;
+            if ((M0 > r0 - radius && M0 < r2 + radius && M1 > r1 && M1 < r3) ||
+                (M1 > r1 - radius && M1 < r3 + radius && M0 > r0 && M0 < r2) ||
+                (M0 < r0 && M1 < r2 && vsqrt(c00 * c00 + c01 * c01) < radius) ||
+                (M0 < r0 && M1 > r3 && vsqrt(c00 * c00 + c11 * c11) < radius) ||
+                (M0 > r1 && M1 < r2 && vsqrt(c20 * c20 + c01 * c01) < radius) ||
+                (M0 > r1 && M1 > r3 && vsqrt(c20 * c20 + c11 * c11) < radius))
+                return true;
+        }
+    } else if (smin(nA, nB) > radius) {
+        return false;
+    }
+    if (radius > 0) {
+        const bool a_in = Au > r0 && Au < r2 && Av > r1 && Av < r3;
+        const bool b_in = Bu > r0 && Bu < r2 && Bv > r1 && Bv < r3;
+        if (a_in) {
+            if (b_in) return smin(nA, nB) < radius;
+            return smin(check_cases(A, B, b, face, Bu, Bv, p), nA) < radius;
+        }
+        if (b_in) return smin(check_cases(A, B, b, face, Au, Av, p), nB) < radius;
+        if (check_cases(A, B, b, face, Au, Av, p) < radius) return true;
+        return check_cases(A, B, b, face, Bu, Bv, p) < radius;
+    }
+    return false;
+}
+
+// collisionCapsuleToBox, CollisionAndDistance.cpp:4-42.  r_new is the
+// inflated radius radius*sqrt(3)/3 evaluated in double and stored in T
+// (line 7); it depends only on the link, so callers pass it in.
+template <typename T>
+__host__ __device__ inline bool collision_box(const Vec3<T>& A, const Vec3<T>& B, T radius, T r_new, const Box<T>& b,
+                                              const Params<T>& p) {
+    if ((A.x > b.lo[0] - r_new && A.y > b.lo[1] - r_new && A.z > b.lo[2] - r_new &&
+         A.x < b.hi[0] + r_new && A.y < b.hi[1] + r_new && A.z < b.hi[2] + r_new) ||
+        (B.x > b.lo[0] - r_new && B.y > b.lo[1] - r_new && B.z > b.lo[2] - r_new &&
+         B.x < b.hi[0] + r_new && B.y < b.hi[1] + r_new && B.z < b.hi[2] + r_new))
+        return true;
+    if ((A.x < b.lo[0] - radius && B.x < b.lo[0] - radius) || (A.x > b.hi[0] + radius && B.x > b.hi[0] + radius) ||
+        (A.y < b.lo[1] - radius && B.y < b.lo[1] - radius) || (A.y > b.hi[1] + radius && B.y > b.hi[1] + radius) ||
+        (A.z < b.lo[2] - radius && B.z < b.lo[2] - radius) || (A.z > b.hi[2] + radius && B.z > b.hi[2] + radius))
+        return false;
+    // one face per axis, chosen by A's position, tested in x, y, z order
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int face = -1;
+        if (A[k] < b.lo[k]) face = k;
+        else if (A[k] > b.hi[k]) face = k + 3;
+        if (face >= 0 && rect_collision(A, B, radius, b, face, p)) return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- underestimation
+// One (link, obstacle) term of computeDistanceUnderestimation,
+// RealVectorSpace.cpp:420-427: min over the link's end points of the signed
+// distance to the plane through O with normal normalize(R - O), minus radius.
+// Eigen's normalized() divides by the norm and leaves a zero vector as is.
+template <typename T>
+__host__ __device__ inline T plane_term(const Vec3<T>& P0, const Vec3<T>& P1, const Vec3<T>& R, const Vec3<T>& O,
+                                        T radius) {
+    const Vec3<T> RO = R - O;
+    const T z = sqnorm(RO);
+    Vec3<T> n = RO;
+    if (z > 0) {
+        const T s = vsqrt(z);
+        n = Vec3<T>{RO.x / s, RO.y / s, RO.z / s};
+    }
+    return smin(dot(P0 - O, n), dot(P1 - O, n)) - radius;
+}
+
+}  // namespace rvs

@@ -1,0 +1,56 @@
+// Internal interface between the C ABI layer (rvs_capi.cpp) and the kernel
+// launchers (rvs_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rvs_fk.hpp"
+
+namespace rvs {
+
+enum : uint8_t { kObjGround = 1, kObjDynamic = 2, kObjObb = 4 };
+
+// Obstacle set as the kernels see it.
+struct EnvDev {
+    int32_t num_obj;
+    int32_t use_obb;          // 1: true capsule-OBB distance (build extension)
+    const float* aabb;        // [M][6] xmin ymin zmin xmax ymax zmax
+    const float* tol;         // [M] min_dist_tol
+    const uint8_t* flags;     // [M] kObj*
+    const float* obb;         // [M][15] center(3), rotation rows(9), half extents(3)
+};
+
+struct QueryParams {
+    float eq;       // EQUALITY_THRESHOLD
+    float maxd;     // MAX_DISTANCE
+    float band;     // near-contact band of the FP32 path
+    int32_t num_checks;
+};
+
+// Outputs of the distance query.  Pointers may be null when not wanted.
+struct DistOut {
+    float* d;          // [N]
+    float* profile;    // [N][L]
+    float* nearest;    // [N][M][L][6]
+    uint8_t* verdict;  // [N]  1 = collision-free (d > 0)
+    double* d64;       // [N]     FP64 path only
+    double* profile64; // [N][L]  FP64 path only
+};
+
+// Largest obstacle count staged whole in LDS by the distance kernels.
+constexpr int kMaxLdsObstacles = 1024;
+
+hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                 int64_t n, const DistOut& out, hipStream_t st);
+hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                int64_t n, const DistOut& out, hipStream_t st);
+hipError_t launch_is_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
+                           uint8_t* valid, int fp64, hipStream_t st);
+hipError_t launch_underestimation(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                  int64_t n, const float* np_ref, int64_t np_stride, float* d, float* profile,
+                                  uint8_t* stored, hipStream_t st);
+hipError_t launch_self_distance(const RobotDev& rb, const QueryParams& qp, const float* q, int64_t n, float* dist,
+                                double* dist64, hipStream_t st);
+
+}  // namespace rvs

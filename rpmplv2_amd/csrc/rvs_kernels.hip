@@ -1,0 +1,321 @@
+// gfx950 kernels of the capsule collision / distance engine.
+//
+// Layout in HBM (configuration-major):
+//   q        [N][n]       float joint coordinates
+//   d        [N]          float (or double) minimum distance
+//   profile  [N][L]       per-link distance profile
+//   nearest  [N][M][L][6] nearest points (robot xyz, obstacle xyz)
+//   verdict  [N]          uint8, 1 = collision-free
+// One lane owns one configuration.  The robot (segments, skeleton recipe,
+// radii) is read with wave-uniform addresses, so it comes through the scalar
+// cache; the obstacle set is staged once per workgroup in LDS and every lane
+// of a wave reads the same obstacle at the same time (an LDS broadcast).
+//
+// This file is compiled without FMA contraction, with IEEE division and
+// square root: the float kernels reproduce the reference's float rounding
+// bit for bit, the double kernels run the same algorithm in FP64.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "rvs_fk.hpp"
+#include "rvs_geometry.hpp"
+#include "rvs_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace rvs {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename T>
+struct Staged {
+    Box<T>* box;
+    T* tol;
+    uint8_t* gnd;
+};
+
+template <typename T>
+__host__ __device__ inline size_t staged_bytes(int M) {
+    return (size_t)M * (sizeof(Box<T>) + sizeof(T) + 1);
+}
+
+// Copy the obstacle set into LDS (whole workgroup participates).
+template <typename T>
+__device__ inline Staged<T> stage_env(const EnvDev& env, unsigned char* smem) {
+    const int M = env.num_obj;
+    Staged<T> s;
+    s.box = reinterpret_cast<Box<T>*>(smem);
+    s.tol = reinterpret_cast<T*>(s.box + M);
+    s.gnd = reinterpret_cast<uint8_t*>(s.tol + M);
+    for (int j = threadIdx.x; j < M; j += blockDim.x) {
+        Box<T> b;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            b.lo[k] = (T)env.aabb[6 * j + k];
+            b.hi[k] = (T)env.aabb[6 * j + 3 + k];
+        }
+        s.box[j] = b;
+        s.tol[j] = (T)env.tol[j];
+        s.gnd[j] = env.flags[j] & kObjGround;
+    }
+    __syncthreads();
+    return s;
+}
+
+template <typename T> __device__ inline T rnew_of(const RobotDev& rb, int i);
+template <> __device__ inline float rnew_of<float>(const RobotDev& rb, int i) { return rb.rnew32[i]; }
+template <> __device__ inline double rnew_of<double>(const RobotDev& rb, int i) { return rb.rnew64[i]; }
+
+// ---------------------------------------------------------------- distance
+// RealVectorSpace::computeDistance (RealVectorSpace.cpp:318-392) for a fresh
+// state: per-link profile, global minimum, early exit on the first pair whose
+// running minimum is <= 0 (the profile keeps that value, later links stay 0,
+// the distance is 0 and the nearest points are dropped).
+template <typename T, bool WANT_NP>
+__global__ __launch_bounds__(kBlock) void k_distance(RobotDev rb, EnvDev env, QueryParams qp,
+                                                     const float* __restrict__ q, int64_t n, DistOut out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<T> s = stage_env<T>(env, smem);
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int M = env.num_obj;
+    const int L = rb.num_links;
+    const float* qc = q + c * rb.num_dofs;
+    const Params<T> p{(T)qp.eq, (T)qp.maxd};
+
+    FKStream fk;
+    fk_init(fk);
+    Vec3<T> P0 = skel_point<T>(fk, rb, qc, 0);
+    T d = p.maxd;
+    bool hit = false;
+    int i = 0;
+    for (; i < L; ++i) {
+        const Vec3<T> P1 = skel_point<T>(fk, rb, qc, i + 1);
+        const T r = (T)rb.radius[i];
+        T prof = p.maxd;
+        for (int j = 0; j < M; ++j) {
+            T dt;
+            NP<T> np;
+            if (s.gnd[j] && i < rb.ground_included) {
+                dt = p.maxd;
+                if (WANT_NP) {
+                    np.r = Vec3<T>{T(0), T(0), T(0)};
+                    np.o = Vec3<T>{T(0), T(0), -p.maxd};
+                    np.valid = true;
+                }
+            } else {
+                dt = capsule_box<T, WANT_NP>(P0, P1, r, s.box[j], p, &np, nullptr, nullptr);
+            }
+            if (dt > s.tol[j]) dt = p.maxd;
+            prof = smin(prof, dt);
+            if (prof <= 0) { hit = true; break; }
+            if (WANT_NP) {
+                float* e = out.nearest + ((c * M + j) * L + i) * 6;
+                if (np.valid) {
+                    e[0] = (float)np.r.x; e[1] = (float)np.r.y; e[2] = (float)np.r.z;
+                    e[3] = (float)np.o.x; e[4] = (float)np.o.y; e[5] = (float)np.o.z;
+                } else {
+                    for (int k = 0; k < 6; ++k) e[k] = NAN;
+                }
+            }
+        }
+        if (out.profile) out.profile[c * L + i] = (float)prof;
+        if (out.profile64) out.profile64[c * L + i] = (double)prof;
+        if (hit) break;
+        d = smin(d, prof);
+        P0 = P1;
+    }
+    for (int k = i + 1; k < L; ++k) {
+        if (out.profile) out.profile[c * L + k] = 0.0f;
+        if (out.profile64) out.profile64[c * L + k] = 0.0;
+    }
+    if (hit) {
+        d = T(0);
+        if (WANT_NP) {
+            float* e = out.nearest + c * (int64_t)M * L * 6;
+            for (int k = 0; k < M * L * 6; ++k) e[k] = NAN;
+        }
+    }
+    if (out.d) out.d[c] = (float)d;
+    if (out.d64) out.d64[c] = (double)d;
+    if (out.verdict) out.verdict[c] = hit ? 0 : 1;
+}
+
+// ---------------------------------------------------------------- isValid
+// RealVectorSpace::isValid(q) (RealVectorSpace.cpp:277-309): collision tree
+// per pair, ground skipped for the first ground_included links, stop on the
+// first hit.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_valid(RobotDev rb, EnvDev env, QueryParams qp,
+                                                  const float* __restrict__ q, int64_t n, uint8_t* valid) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<T> s = stage_env<T>(env, smem);
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int M = env.num_obj;
+    const int L = rb.num_links;
+    const float* qc = q + c * rb.num_dofs;
+    const Params<T> p{(T)qp.eq, (T)qp.maxd};
+    FKStream fk;
+    fk_init(fk);
+    Vec3<T> P0 = skel_point<T>(fk, rb, qc, 0);
+    bool ok = true;
+    for (int i = 0; i < L && ok; ++i) {
+        const Vec3<T> P1 = skel_point<T>(fk, rb, qc, i + 1);
+        const T r = (T)rb.radius[i];
+        const T rn = rnew_of<T>(rb, i);
+        for (int j = 0; j < M; ++j) {
+            if (s.gnd[j] && i < rb.ground_included) continue;
+            if (collision_box(P0, P1, r, rn, s.box[j], p)) { ok = false; break; }
+        }
+        P0 = P1;
+    }
+    valid[c] = ok ? 1 : 0;
+}
+
+// ---------------------------------------------------------------- underestimation
+// RealVectorSpace::computeDistanceUnderestimation (RealVectorSpace.cpp:
+// 402-446) on a fresh state, in float.  np_ref holds [M][L][6] per
+// configuration (stride np_stride floats; 0 = one set for all).
+__global__ __launch_bounds__(kBlock) void k_under(RobotDev rb, EnvDev env, QueryParams qp,
+                                                  const float* __restrict__ q, int64_t n,
+                                                  const float* __restrict__ np_ref, int64_t np_stride, float* d_out,
+                                                  float* prof_out, uint8_t* stored) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int M = env.num_obj;
+    const int L = rb.num_links;
+    const float* qc = q + c * rb.num_dofs;
+    const float* npc = np_ref + c * np_stride;
+    const float maxd = qp.maxd;
+    FKStream fk;
+    fk_init(fk);
+    Vec3<float> P0 = skel_point<float>(fk, rb, qc, 0);
+    float d = maxd;
+    bool neg = false;
+    int i = 0;
+    for (; i < L && !neg; ++i) {
+        const Vec3<float> P1 = skel_point<float>(fk, rb, qc, i + 1);
+        const float r = rb.radius[i];
+        float prof = maxd;
+        for (int j = 0; j < M; ++j) {
+            const float* e = npc + ((int64_t)j * L + i) * 6;
+            const Vec3<float> O{e[3], e[4], e[5]};
+            if (vnorm(O) == maxd) continue;
+            const Vec3<float> R{e[0], e[1], e[2]};
+            const float t = plane_term(P0, P1, R, O, r);
+            if (t < 0) { neg = true; break; }
+            prof = smin(prof, t);
+        }
+        if (neg) break;
+        if (prof_out) prof_out[c * L + i] = prof;
+        d = smin(d, prof);
+        P0 = P1;
+    }
+    if (neg) {
+        if (prof_out) for (int k = 0; k < L; ++k) prof_out[c * L + k] = 0.0f;
+        d = 0.0f;
+    }
+    d_out[c] = d;
+    if (stored) stored[c] = neg ? 0 : 1;
+}
+
+// ---------------------------------------------------------------- self distance
+// Capsule-capsule distance of the robot's listed link pairs:
+// distanceLineSegToLineSeg - r_a - r_b (xArm6::computeCapsulesDistance,
+// xArm6.cpp:417-424).  The skeleton of each lane is kept in LDS, one column
+// per lane.
+template <typename T>
+__global__ __launch_bounds__(128) void k_self(RobotDev rb, QueryParams qp, const float* __restrict__ q, int64_t n,
+                                              float* dist, double* dist64) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* sk = reinterpret_cast<T*>(smem);
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int L = rb.num_links;
+    const int B = blockDim.x, t = threadIdx.x;
+    const float* qc = q + c * rb.num_dofs;
+    const Params<T> p{(T)qp.eq, (T)qp.maxd};
+    FKStream fk;
+    fk_init(fk);
+    for (int k = 0; k <= L; ++k) {
+        const Vec3<T> P = skel_point<T>(fk, rb, qc, k);
+        sk[(3 * k + 0) * B + t] = P.x;
+        sk[(3 * k + 1) * B + t] = P.y;
+        sk[(3 * k + 2) * B + t] = P.z;
+    }
+    auto pt = [&](int k) { return Vec3<T>{sk[(3 * k) * B + t], sk[(3 * k + 1) * B + t], sk[(3 * k + 2) * B + t]}; };
+    for (int m = 0; m < rb.num_pairs; ++m) {
+        const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
+        const T sd = seg_seg<T, false>(pt(a), pt(a + 1), pt(b), pt(b + 1), p, nullptr);
+        const T v = sd - (T)rb.radius[a] - (T)rb.radius[b];
+        if (dist) dist[c * rb.num_pairs + m] = (float)v;
+        if (dist64) dist64[c * rb.num_pairs + m] = (double)v;
+    }
+}
+
+inline dim3 grid_for(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
+
+}  // namespace
+
+hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                 int64_t n, const DistOut& out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const size_t sm = staged_bytes<float>(env.num_obj);
+    if (out.nearest)
+        hipLaunchKernelGGL((k_distance<float, true>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+    else
+        hipLaunchKernelGGL((k_distance<float, false>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                int64_t n, const DistOut& out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const size_t sm = staged_bytes<double>(env.num_obj);
+    if (out.nearest)
+        hipLaunchKernelGGL((k_distance<double, true>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+    else
+        hipLaunchKernelGGL((k_distance<double, false>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_is_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
+                           uint8_t* valid, int fp64, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (fp64)
+        hipLaunchKernelGGL((k_valid<double>), grid_for(n, kBlock), dim3(kBlock), staged_bytes<double>(env.num_obj), st,
+                           rb, env, qp, q, n, valid);
+    else
+        hipLaunchKernelGGL((k_valid<float>), grid_for(n, kBlock), dim3(kBlock), staged_bytes<float>(env.num_obj), st,
+                           rb, env, qp, q, n, valid);
+    return hipGetLastError();
+}
+
+hipError_t launch_underestimation(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                  int64_t n, const float* np_ref, int64_t np_stride, float* d, float* profile,
+                                  uint8_t* stored, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_under, grid_for(n, kBlock), dim3(kBlock), 0, st, rb, env, qp, q, n, np_ref, np_stride, d,
+                       profile, stored);
+    return hipGetLastError();
+}
+
+hipError_t launch_self_distance(const RobotDev& rb, const QueryParams& qp, const float* q, int64_t n, float* dist,
+                                double* dist64, hipStream_t st) {
+    if (n <= 0 || rb.num_pairs == 0) return hipSuccess;
+    const int B = 128;
+    if (dist64) {
+        const size_t sm = (size_t)3 * (rb.num_links + 1) * B * sizeof(double);
+        hipLaunchKernelGGL((k_self<double>), grid_for(n, B), dim3(B), sm, st, rb, qp, q, n, dist, dist64);
+    } else {
+        const size_t sm = (size_t)3 * (rb.num_links + 1) * B * sizeof(float);
+        hipLaunchKernelGGL((k_self<float>), grid_for(n, B), dim3(B), sm, st, rb, qp, q, n, dist, dist64);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rvs

@@ -1,0 +1,271 @@
+"""Python binding of the engine's C ABI (include/rvs.h) over ctypes.
+
+The engine itself is librvs.so (hand-written gfx950 kernels + C++ host code).
+This module only marshals arguments: numpy arrays are passed as host memory,
+torch CUDA tensors as device pointers (RVS_F_DEVICE).  There is no CPU
+fallback: when librvs.so is missing or no GPU is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import robots as R
+from . import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librvs.so")
+
+RVS_OK, RVS_EINVAL, RVS_ENOMEM, RVS_EDEVICE, RVS_ERANGE, RVS_ENOTSUP = range(6)
+PREC_REF32, PREC_FP64, PREC_FP32_REFINE = 0, 1, 2
+F_DEVICE = 0x1
+ENV_AABB, ENV_OBB = 0, 1
+BOX_GROUND, BOX_DYNAMIC = 0x1, 0x2
+
+
+def F_PREC(p: int) -> int:
+    return (p + 1) << 4
+
+
+class RvsError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"rvs status {status}: {msg}")
+        self.status = status
+
+
+class Options(C.Structure):
+    _fields_ = [("device", C.c_int32), ("precision", C.c_int32), ("equality_threshold", C.c_float),
+                ("max_distance", C.c_float), ("num_checks", C.c_int32), ("band", C.c_float)]
+
+
+class Joint(C.Structure):
+    _fields_ = [("type", C.c_int32), ("reserved", C.c_int32), ("xyz", C.c_double * 3), ("rpy", C.c_double * 3),
+                ("axis", C.c_double * 3)]
+
+
+class RobotDesc(C.Structure):
+    _fields_ = [("skeleton_kind", C.c_int32), ("num_joints", C.c_int32), ("joints", C.POINTER(Joint)),
+                ("num_links", C.c_int32), ("capsule_radius", C.POINTER(C.c_float)),
+                ("ground_included", C.c_int32), ("gripper_length", C.c_float),
+                ("num_self_pairs", C.c_int32), ("self_pairs", C.POINTER(C.c_int32))]
+
+
+class Box(C.Structure):
+    _fields_ = [("dim", C.c_float * 3), ("pos", C.c_float * 3), ("quat_wxyz", C.c_float * 4),
+                ("flags", C.c_uint32), ("min_dist_tol", C.c_float)]
+
+
+EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error", "rvs_stream", "rvs_last_refined",
+           "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds", "rvs_distance",
+           "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_self_distance",
+           "rvs_edge_is_valid", "rvs_edge_first_collision"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m rpmplv2_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, u32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32
+        ctx = C.c_void_p
+        L.rvs_default_options.argtypes = [C.POINTER(Options)]
+        L.rvs_default_options.restype = None
+        L.rvs_create.argtypes = [C.POINTER(Options), C.POINTER(ctx)]
+        L.rvs_destroy.argtypes = [ctx]
+        L.rvs_destroy.restype = None
+        L.rvs_last_error.argtypes = [ctx]
+        L.rvs_last_error.restype = C.c_char_p
+        L.rvs_stream.argtypes = [ctx]
+        L.rvs_stream.restype = vp
+        L.rvs_last_refined.argtypes = [ctx]
+        L.rvs_last_refined.restype = i64
+        L.rvs_version.restype = C.c_char_p
+        L.rvs_robot_load.argtypes = [ctx, C.POINTER(RobotDesc), C.POINTER(i32)]
+        L.rvs_env_create.argtypes = [ctx, C.POINTER(Box), i32, i32, C.POINTER(i32)]
+        L.rvs_env_update.argtypes = [ctx, i32, vp, vp, i32, u32]
+        L.rvs_env_bounds.argtypes = [ctx, i32, vp, i32]
+        L.rvs_distance.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, vp, u32]
+        L.rvs_distance_f64.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, u32]
+        L.rvs_is_valid.argtypes = [ctx, i32, i32, vp, i64, vp, u32]
+        L.rvs_distance_underestimation.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, vp, vp, u32]
+        L.rvs_self_distance.argtypes = [ctx, i32, vp, i64, vp, u32]
+        L.rvs_edge_is_valid.argtypes = [ctx, i32, i32, vp, vp, i64, vp, u32]
+        L.rvs_edge_first_collision.argtypes = [ctx, i32, vp, vp, vp, vp, i64, i32, C.c_float, vp, u32]
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if f.restype is C.c_int:
+                f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch") and hasattr(x, "data_ptr")
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if _is_torch(x):
+        return C.c_void_p(x.data_ptr())
+    return C.c_void_p(x.ctypes.data)
+
+
+def robot_desc(rb: R.Robot):
+    """Build an rvs_robot_desc; returns (desc, keepalive)."""
+    js = (Joint * len(rb.joints))()
+    for k, j in enumerate(rb.joints):
+        js[k].type = j.type
+        for i in range(3):
+            js[k].xyz[i] = j.xyz[i]
+            js[k].rpy[i] = j.rpy[i]
+            js[k].axis[i] = j.axis[i]
+    rad = np.ascontiguousarray(rb.radius, np.float32)
+    pairs = np.ascontiguousarray(np.asarray(rb.self_pairs, np.int32).reshape(-1, 2))
+    d = RobotDesc(rb.skeleton_kind, len(rb.joints), js, rb.num_links, rad.ctypes.data_as(C.POINTER(C.c_float)),
+                  rb.ground_included, rb.gripper_length, len(pairs), pairs.ctypes.data_as(C.POINTER(C.c_int32)))
+    return d, (js, rad, pairs)
+
+
+def boxes_of(scene: W.Scene):
+    arr = (Box * scene.num_obj)()
+    for j in range(scene.num_obj):
+        for i in range(3):
+            arr[j].dim[i] = scene.dim[j, i]
+            arr[j].pos[i] = scene.pos[j, i]
+        for i in range(4):
+            arr[j].quat_wxyz[i] = scene.quat[j, i]
+        arr[j].flags = BOX_GROUND if scene.is_ground[j] else 0
+        arr[j].min_dist_tol = scene.min_dist_tol[j]
+    return arr
+
+
+class Engine:
+    """One engine context on one GPU (wraps rvs_context)."""
+
+    def __init__(self, device: int = 0, precision: int = PREC_REF32, equality_threshold: float = 1e-4,
+                 max_distance: float = 1000.0, num_checks: int = 10, band: float = 1e-4):
+        L = lib()
+        o = Options()
+        L.rvs_default_options(C.byref(o))
+        o.device, o.precision = device, precision
+        o.equality_threshold, o.max_distance, o.num_checks, o.band = equality_threshold, max_distance, num_checks, band
+        self._ctx = C.c_void_p()
+        self._check(L.rvs_create(C.byref(o), C.byref(self._ctx)), "rvs_create")
+        self.robots = {}
+        self.envs = {}
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            lib().rvs_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != RVS_OK:
+            msg = lib().rvs_last_error(self._ctx).decode() if getattr(self, "_ctx", None) else what
+            raise RvsError(st, f"{what}: {msg}")
+
+    @property
+    def stream(self) -> int:
+        return lib().rvs_stream(self._ctx)
+
+    def last_refined(self) -> int:
+        return lib().rvs_last_refined(self._ctx)
+
+    # ---- set-up
+    def load_robot(self, rb: R.Robot) -> int:
+        d, keep = robot_desc(rb)
+        rid = C.c_int32()
+        self._check(lib().rvs_robot_load(self._ctx, C.byref(d), C.byref(rid)), "rvs_robot_load")
+        self.robots[rid.value] = rb
+        return rid.value
+
+    def create_env(self, scene: W.Scene, semantics: int = ENV_AABB) -> int:
+        arr = boxes_of(scene)
+        eid = C.c_int32()
+        self._check(lib().rvs_env_create(self._ctx, arr, scene.num_obj, semantics, C.byref(eid)), "rvs_env_create")
+        self.envs[eid.value] = scene
+        return eid.value
+
+    def update_env(self, env: int, idx, pos):
+        idx = np.ascontiguousarray(idx, np.int32)
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        self._check(lib().rvs_env_update(self._ctx, env, _ptr(idx), _ptr(pos), len(idx), 0), "rvs_env_update")
+
+    def env_bounds(self, env: int) -> np.ndarray:
+        m = self.envs[env].num_obj
+        out = np.zeros((m, 6), np.float32)
+        self._check(lib().rvs_env_bounds(self._ctx, env, _ptr(out), m), "rvs_env_bounds")
+        return out
+
+    # ---- queries on host arrays
+    def distance(self, robot: int, env: int, q, precision: Optional[int] = None, want_nearest=False):
+        rb = self.robots[robot]
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+        n, L, M = len(q), rb.num_links, self.envs[env].num_obj
+        d = np.zeros(n, np.float32)
+        prof = np.zeros((n, L), np.float32)
+        ver = np.zeros(n, np.uint8)
+        npo = np.zeros((n, M, L, 6), np.float32) if want_nearest else None
+        fl = F_PREC(precision) if precision is not None else 0
+        self._check(lib().rvs_distance(self._ctx, robot, env, _ptr(q), n, _ptr(d), _ptr(prof), _ptr(npo), _ptr(ver),
+                                       fl), "rvs_distance")
+        return d, prof, ver, npo
+
+    def distance_f64(self, robot: int, env: int, q):
+        rb = self.robots[robot]
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+        n, L = len(q), rb.num_links
+        d = np.zeros(n, np.float64)
+        prof = np.zeros((n, L), np.float64)
+        ver = np.zeros(n, np.uint8)
+        self._check(lib().rvs_distance_f64(self._ctx, robot, env, _ptr(q), n, _ptr(d), _ptr(prof), _ptr(ver), 0),
+                    "rvs_distance_f64")
+        return d, prof, ver
+
+    def is_valid(self, robot: int, env: int, q, precision: Optional[int] = None):
+        rb = self.robots[robot]
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+        out = np.zeros(len(q), np.uint8)
+        fl = F_PREC(precision) if precision is not None else 0
+        self._check(lib().rvs_is_valid(self._ctx, robot, env, _ptr(q), len(q), _ptr(out), fl), "rvs_is_valid")
+        return out
+
+    def underestimation(self, robot: int, env: int, q, nearest_ref):
+        rb = self.robots[robot]
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+        n, L, M = len(q), rb.num_links, self.envs[env].num_obj
+        npr = np.ascontiguousarray(nearest_ref, np.float32)
+        stride = 0 if npr.ndim == 3 else M * L * 6
+        d = np.zeros(n, np.float32)
+        prof = np.zeros((n, L), np.float32)
+        st = np.zeros(n, np.uint8)
+        self._check(lib().rvs_distance_underestimation(self._ctx, robot, env, _ptr(q), n, _ptr(npr), stride, _ptr(d),
+                                                       _ptr(prof), _ptr(st), 0), "rvs_distance_underestimation")
+        return d, prof, st
+
+    def self_distance(self, robot: int, q):
+        rb = self.robots[robot]
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+        out = np.zeros((len(q), len(rb.self_pairs)), np.float32)
+        self._check(lib().rvs_self_distance(self._ctx, robot, _ptr(q), len(q), _ptr(out), 0), "rvs_self_distance")
+        return out
+
+    # ---- queries on device tensors (torch CUDA tensors; no staging)
+    def distance_device(self, robot: int, env: int, q, d, profile=None, verdict=None, nearest=None,
+                        precision: Optional[int] = None):
+        fl = F_DEVICE | (F_PREC(precision) if precision is not None else 0)
+        self._check(lib().rvs_distance(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d), _ptr(profile),
+                                       _ptr(nearest), _ptr(verdict), fl), "rvs_distance")

@@ -1,0 +1,245 @@
+"""Seeded synthetic workloads for the capsule collision / distance engine.
+
+Every draw uses numpy's PCG64 generator seeded with 1000 + workload number,
+in double, rounded to float32 where the reference stores float.  Scenes are
+lists of boxes (dim, pos, quaternion w-x-y-z, ground flag, min_dist_tol);
+their axis-aligned bounds follow FCL's rule for CollisionObject::computeAABB
+(identity rotation: pos -+ dim/2; otherwise pos -+ |dim/2| on every axis),
+restated in Box.cpp:3-15 / Object.cpp:21-26 of the reference.
+
+Workloads (sizes as quoted; each function takes a scale-down factor for tests):
+  1  planar 2-DOF, r = 0, 4 boxes in a 2 m disk, 65,536 configurations
+  2  xArm6, r = 0.05, ground box + 16 boxes in a 1 m ball at (0,0,0.4),
+     boxes that overlap the start pose (q = 0) rejected, 1,048,576 configurations
+  3  chains of 10/14/18/22 DOF with random link lengths, r = 0.04, self-collision
+     on, 64 boxes in a 1.5 m ball, 4,096 discretised edges per chain
+  4  serving stream (see serving.py)
+  5  22-DOF chain, r = 0.05, 512 rotated boxes in a 2 m ball, 16,777,216 configurations
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import robots as R
+
+
+@dataclass
+class Scene:
+    dim: np.ndarray          # (M, 3) float32
+    pos: np.ndarray          # (M, 3) float32
+    quat: np.ndarray         # (M, 4) float32, w x y z
+    is_ground: np.ndarray    # (M,) uint8
+    min_dist_tol: np.ndarray # (M,) float32
+    aabb: np.ndarray = field(init=False)
+
+    def __post_init__(self):
+        self.dim = np.ascontiguousarray(self.dim, np.float32).reshape(-1, 3)
+        self.pos = np.ascontiguousarray(self.pos, np.float32).reshape(-1, 3)
+        self.quat = np.ascontiguousarray(self.quat, np.float32).reshape(-1, 4)
+        self.is_ground = np.ascontiguousarray(self.is_ground, np.uint8)
+        self.min_dist_tol = np.ascontiguousarray(self.min_dist_tol, np.float32)
+        self.aabb = aabb_of(self.dim, self.pos, self.quat)
+
+    @property
+    def num_obj(self) -> int:
+        return len(self.dim)
+
+    def subset(self, idx) -> "Scene":
+        idx = np.asarray(idx)
+        return Scene(self.dim[idx], self.pos[idx], self.quat[idx], self.is_ground[idx], self.min_dist_tol[idx])
+
+    def concat(self, other: "Scene") -> "Scene":
+        return Scene(np.concatenate([self.dim, other.dim]), np.concatenate([self.pos, other.pos]),
+                     np.concatenate([self.quat, other.quat]), np.concatenate([self.is_ground, other.is_ground]),
+                     np.concatenate([self.min_dist_tol, other.min_dist_tol]))
+
+
+MAX_DISTANCE = np.float32(1000.0)
+
+
+def _quat_matrix_f32(q):
+    """Eigen Quaternion<float>::toRotationMatrix, vectorised (float32)."""
+    q = np.asarray(q, np.float32)
+    w, x, y, z = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    two = np.float32(2)
+    one = np.float32(1)
+    tx, ty, tz = two * x, two * y, two * z
+    twx, twy, twz = tx * w, ty * w, tz * w
+    txx, txy, txz = tx * x, ty * x, tz * x
+    tyy, tyz, tzz = ty * y, tz * y, tz * z
+    return np.stack([np.stack([one - (tyy + tzz), txy - twz, txz + twy], -1),
+                     np.stack([txy + twz, one - (txx + tzz), tyz - twx], -1),
+                     np.stack([txz - twy, tyz + twx, one - (txx + tyy)], -1)], -2)
+
+
+def is_identity_rot(Rm) -> np.ndarray:
+    """Eigen isIdentity with float dummy precision 1e-5."""
+    Rm = np.asarray(Rm, np.float32)
+    prec = np.float32(1e-5)
+    eye = np.eye(3, dtype=bool)
+    diag_ok = np.abs(Rm - 1) <= prec * np.minimum(np.abs(Rm), 1)
+    off_ok = np.abs(Rm) <= prec
+    ok = np.where(eye, diag_ok, off_ok)
+    return ok.all(axis=(-1, -2))
+
+
+def aabb_of(dim, pos, quat) -> np.ndarray:
+    dim = np.asarray(dim, np.float32).reshape(-1, 3)
+    pos = np.asarray(pos, np.float32).reshape(-1, 3)
+    quat = np.asarray(quat, np.float32).reshape(-1, 4)
+    h = np.float32(0.5) * dim
+    ident = is_identity_rot(_quat_matrix_f32(quat))
+    out = np.empty((len(dim), 6), np.float32)
+    out[:, :3] = -h + pos
+    out[:, 3:] = h + pos
+    if (~ident).any():
+        m = -h[~ident]
+        rad = np.sqrt(m[:, 0] * m[:, 0] + (m[:, 1] * m[:, 1] + m[:, 2] * m[:, 2])).astype(np.float32)
+        p = pos[~ident]
+        out[~ident, :3] = p - rad[:, None]
+        out[~ident, 3:] = p + rad[:, None]
+    return out
+
+
+def ground_box() -> Scene:
+    """The repository's ground box (data/xarm6/scenario_test/scenario_test.yaml:2-6)."""
+    return Scene([[1.5, 1.5, 0.1]], [[0, 0, -0.05]], [[1, 0, 0, 0]], [1], [0.05])
+
+
+def _uniform_ball(rng, n, radius, center=(0, 0, 0)):
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    r = radius * rng.random(n) ** (1.0 / 3.0)
+    return np.asarray(center, np.float64) + u * r[:, None]
+
+
+def _boxes(rng, n, side_lo, side_hi, radius, center=(0, 0, 0), rotated=False, tol=None) -> Scene:
+    dim = rng.uniform(side_lo, side_hi, size=(n, 3))
+    pos = _uniform_ball(rng, n, radius, center)
+    if rotated:
+        quat = _shoemake(rng, n)
+    else:
+        quat = np.tile([1.0, 0.0, 0.0, 0.0], (n, 1))
+    t = np.full(n, MAX_DISTANCE if tol is None else tol, np.float32)
+    return Scene(dim, pos, quat, np.zeros(n, np.uint8), t)
+
+
+def _shoemake(rng, n):
+    """Uniform random unit quaternions (Shoemake 1992), returned w x y z."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1 - u1), np.sqrt(u1)
+    x = a * np.sin(2 * np.pi * u2)
+    y = a * np.cos(2 * np.pi * u2)
+    z = b * np.sin(2 * np.pi * u3)
+    w = b * np.cos(2 * np.pi * u3)
+    return np.stack([w, x, y, z], 1)
+
+
+def _overlaps_pose(rb: R.Robot, aabb: np.ndarray, q) -> bool:
+    """Conservative start-pose test: the box's bounds overlap the bounds of a
+    capsule of the pose (each capsule's segment bounds grown by its radius)."""
+    P = R.skeleton_np(rb, q)[0]
+    for i in range(rb.num_links):
+        lo = np.minimum(P[i], P[i + 1]) - rb.radius[i]
+        hi = np.maximum(P[i], P[i + 1]) + rb.radius[i]
+        if np.all(aabb[:3] <= hi) and np.all(aabb[3:] >= lo):
+            return True
+    return False
+
+
+def uniform_configs(rng, n, limits) -> np.ndarray:
+    lo = np.asarray(limits, np.float64)[:, 0]
+    hi = np.asarray(limits, np.float64)[:, 1]
+    return (lo + (hi - lo) * rng.random((n, len(lo)))).astype(np.float32)
+
+
+# ---------------------------------------------------------------- workload 1
+@dataclass
+class ConfigBatch:
+    name: str
+    robot: R.Robot
+    scene: Scene
+    q: np.ndarray
+
+
+def workload1(n: int = 65536) -> ConfigBatch:
+    rng = np.random.default_rng(1001)
+    rb = R.planar_2dof(radius=0.0)
+    m = 4
+    rr = 2.0 * np.sqrt(rng.random(m))
+    th = 2 * np.pi * rng.random(m)
+    pos = np.stack([rr * np.cos(th), rr * np.sin(th), np.zeros(m)], 1)
+    dim = np.stack([rng.uniform(0.1, 0.5, m), rng.uniform(0.1, 0.5, m), np.full(m, 0.2)], 1)
+    sc = Scene(dim, pos, np.tile([1.0, 0, 0, 0], (m, 1)), np.zeros(m, np.uint8), np.full(m, MAX_DISTANCE))
+    q = (rng.uniform(-np.pi, np.pi, size=(n, 2))).astype(np.float32)
+    return ConfigBatch("planar2dof_4aabb", rb, sc, q)
+
+
+# ---------------------------------------------------------------- workload 2
+def scene2(rng=None, num_boxes: int = 16) -> Scene:
+    rng = rng or np.random.default_rng(1002)
+    rb = R.xarm6()
+    q0 = np.zeros(6)
+    kept: List[Scene] = []
+    while len(kept) < num_boxes:
+        b = _boxes(rng, 1, 0.05, 0.3, 1.0, (0, 0, 0.4))
+        if _overlaps_pose(rb, b.aabb[0], q0):
+            continue
+        kept.append(b)
+    boxes = kept[0]
+    for b in kept[1:]:
+        boxes = boxes.concat(b)
+    return ground_box().concat(boxes)
+
+
+def workload2(n: int = 1048576) -> ConfigBatch:
+    rng = np.random.default_rng(1002)
+    sc = scene2(rng)
+    rb = R.xarm6(radius=0.05, ground_included=1)
+    q = uniform_configs(rng, n, rb.limits)
+    return ConfigBatch("xarm6_16aabb_ground", rb, sc, q)
+
+
+# ---------------------------------------------------------------- workload 3
+@dataclass
+class EdgeBatch:
+    robot: R.Robot
+    q1: np.ndarray     # (E, n)
+    dir: np.ndarray    # (E, n) unit vectors
+    K: np.ndarray      # (E,) states per edge
+    step: float = 0.05
+
+
+@dataclass
+class EdgeWorkload:
+    scene: Scene
+    batches: List[EdgeBatch]
+
+
+def workload3(edges_per_robot: int = 4096, dofs=(10, 14, 18, 22)) -> EdgeWorkload:
+    rng = np.random.default_rng(1003)
+    sc = _boxes(rng, 64, 0.1, 0.5, 1.5)
+    batches = []
+    for n in dofs:
+        lengths = rng.uniform(0.1, 0.3, n)
+        rb = R.chain(lengths, radius=0.04, name=f"chain{n}", self_collision=True)
+        q1 = rng.uniform(-np.pi, np.pi, size=(edges_per_robot, n)).astype(np.float32)
+        u = rng.normal(size=(edges_per_robot, n))
+        u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+        K = np.rint(np.exp(rng.uniform(np.log(8), np.log(256), edges_per_robot))).astype(np.int32)
+        batches.append(EdgeBatch(rb, q1, u, K))
+    return EdgeWorkload(sc, batches)
+
+
+# ---------------------------------------------------------------- workload 5
+def workload5(n: int = 16777216, rotated_semantics: bool = True) -> ConfigBatch:
+    rng = np.random.default_rng(1005)
+    sc = _boxes(rng, 512, 0.02, 0.2, 2.0, rotated=True)
+    lengths = rng.uniform(0.1, 0.3, 22)
+    rb = R.chain(lengths, radius=0.05, name="chain22", self_collision=True)
+    q = rng.uniform(-np.pi, np.pi, size=(n, 22)).astype(np.float32)
+    return ConfigBatch("chain22_512obb", rb, sc, q)
