@@ -68,95 +68,105 @@ struct RobotDev {
     const int32_t* pairs;    // self-collision pairs (a, b)
 };
 
-// KDL::Rotation::Rot2: Rodrigues' formula with the unit axis a.
-__host__ __device__ inline void rot2(const double* a, double st, double ct, double* M) {
-    const double vt = 1 - ct;
-    const double m_vt_0 = vt * a[0], m_vt_1 = vt * a[1], m_vt_2 = vt * a[2];
-    const double m_st_0 = a[0] * st, m_st_1 = a[1] * st, m_st_2 = a[2] * st;
-    const double m_vt_0_1 = m_vt_0 * a[1], m_vt_0_2 = m_vt_0 * a[2], m_vt_1_2 = m_vt_1 * a[2];
-    M[0] = ct + m_vt_0 * a[0];
-    M[1] = -m_st_2 + m_vt_0_1;
-    M[2] = m_st_1 + m_vt_0_2;
-    M[3] = m_st_2 + m_vt_0_1;
-    M[4] = ct + m_vt_1 * a[1];
-    M[5] = -m_st_0 + m_vt_1_2;
-    M[6] = -m_st_1 + m_vt_0_2;
-    M[7] = m_st_0 + m_vt_1_2;
-    M[8] = ct + m_vt_2 * a[2];
+// 3x3 rotation held in scalars (row major: m0 m1 m2 / m3 m4 m5 / m6 m7 m8)
+// so that frames stay in registers.
+struct Mat3 {
+    double m0, m1, m2, m3, m4, m5, m6, m7, m8;
+};
+
+__host__ __device__ inline Mat3 mat_load(const double* a) {
+    return Mat3{a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8]};
 }
 
-// 3x3 product, each entry summed left to right as KDL does
-__host__ __device__ inline void mat_mul(const double* a, const double* b, double* r) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+// Product with each entry summed left to right, as KDL's Rotation operator*
+__host__ __device__ inline Mat3 mat_mul(const Mat3& a, const Mat3& b) {
+    return Mat3{a.m0 * b.m0 + a.m1 * b.m3 + a.m2 * b.m6, a.m0 * b.m1 + a.m1 * b.m4 + a.m2 * b.m7,
+                a.m0 * b.m2 + a.m1 * b.m5 + a.m2 * b.m8, a.m3 * b.m0 + a.m4 * b.m3 + a.m5 * b.m6,
+                a.m3 * b.m1 + a.m4 * b.m4 + a.m5 * b.m7, a.m3 * b.m2 + a.m4 * b.m5 + a.m5 * b.m8,
+                a.m6 * b.m0 + a.m7 * b.m3 + a.m8 * b.m6, a.m6 * b.m1 + a.m7 * b.m4 + a.m8 * b.m7,
+                a.m6 * b.m2 + a.m7 * b.m5 + a.m8 * b.m8};
+}
+
+// KDL::Rotation::Rot2: Rodrigues' formula with the unit axis a.
+__host__ __device__ inline Mat3 rot2(const double* a, double st, double ct) {
+    const double a0 = a[0], a1 = a[1], a2 = a[2];
+    const double vt = 1 - ct;
+    const double m_vt_0 = vt * a0, m_vt_1 = vt * a1, m_vt_2 = vt * a2;
+    const double m_st_0 = a0 * st, m_st_1 = a1 * st, m_st_2 = a2 * st;
+    const double m_vt_0_1 = m_vt_0 * a1, m_vt_0_2 = m_vt_0 * a2, m_vt_1_2 = m_vt_1 * a2;
+    return Mat3{ct + m_vt_0 * a0,  -m_st_2 + m_vt_0_1, m_st_1 + m_vt_0_2,
+                m_st_2 + m_vt_0_1, ct + m_vt_1 * a1,   -m_st_0 + m_vt_1_2,
+                -m_st_1 + m_vt_0_2, m_st_0 + m_vt_1_2, ct + m_vt_2 * a2};
 }
 
 struct FKStream {
-    double M[9];
-    double p[3];
+    Mat3 M;
+    double px, py, pz;
     int seg;   // next segment to apply
     int qi;    // next joint coordinate
 };
 
-__host__ __device__ inline void fk_init(FKStream& s) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) s.M[i] = (i % 4 == 0) ? 1.0 : 0.0;
-    s.p[0] = s.p[1] = s.p[2] = 0.0;
+__host__ __device__ __attribute__((always_inline)) inline void fk_init(FKStream& s) {
+    s.M = Mat3{1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
+    s.px = s.py = s.pz = 0.0;
     s.seg = 0;
     s.qi = 0;
 }
 
 // Apply segments until frame 'upto' is the current frame.
-__host__ __device__ inline void fk_advance(FKStream& s, const SegDev* seg, const float* q, int upto) {
+__host__ __device__ __attribute__((always_inline)) inline void fk_advance(FKStream& s, const SegDev* seg,
+                                                                        const float* q, int upto) {
     while (s.seg <= upto) {
         const SegDev& g = seg[s.seg];
-        double SM[9];
+        const Mat3 R = mat_load(g.R);
+        Mat3 SM;
         if (g.rev) {
             double st, ct;
             sincos((double)q[s.qi++], &st, &ct);
-            double Rq[9];
-            rot2(g.axis, st, ct, Rq);
-            mat_mul(Rq, g.R, SM);
+            SM = mat_mul(rot2(g.axis, st, ct), R);
         } else {
-#pragma unroll
-            for (int i = 0; i < 9; ++i) SM[i] = g.R[i];
+            SM = R;
         }
-        double np[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            np[i] = s.M[3 * i] * g.o[0] + s.M[3 * i + 1] * g.o[1] + s.M[3 * i + 2] * g.o[2] + s.p[i];
-        double nM[9];
-        mat_mul(s.M, SM, nM);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) s.M[i] = nM[i];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) s.p[i] = np[i];
+        const double o0 = g.o[0], o1 = g.o[1], o2 = g.o[2];
+        const Mat3& M = s.M;
+        const double nx = M.m0 * o0 + M.m1 * o1 + M.m2 * o2 + s.px;
+        const double ny = M.m3 * o0 + M.m4 * o1 + M.m5 * o2 + s.py;
+        const double nz = M.m6 * o0 + M.m7 * o1 + M.m8 * o2 + s.pz;
+        s.M = mat_mul(M, SM);
+        s.px = nx;
+        s.py = ny;
+        s.pz = nz;
         ++s.seg;
     }
 }
 
 // Skeleton point k (points must be requested in increasing k).
 template <typename T>
-__host__ __device__ inline Vec3<T> skel_point(FKStream& s, const RobotDev& rb, const float* q, int k) {
+__host__ __device__ __attribute__((always_inline)) inline Vec3<T> skel_point(FKStream& s, const RobotDev& rb,
+                                                                           const float* q, int k) {
     const PointDev pt = rb.pts[k];
     if (pt.frame < 0) return Vec3<T>{T(0), T(0), T(0)};
     fk_advance(s, rb.seg, q, pt.frame);
-    double x = s.p[0], y = s.p[1], z = s.p[2];
-    const int a = pt.axis;
+    double x = s.px, y = s.py, z = s.pz;
+    // column 'axis' of the frame, picked with one-hot weights (exact: the
+    // other two terms are signed zeros) rather than an indexed load, which
+    // would keep the frame in scratch memory
+    const double w0 = pt.axis == 0 ? 1.0 : 0.0, w1 = pt.axis == 1 ? 1.0 : 0.0, w2 = pt.axis == 2 ? 1.0 : 0.0;
+    const Mat3& M = s.M;
+    const double ax = M.m0 * w0 + M.m1 * w1 + M.m2 * w2;
+    const double ay = M.m3 * w0 + M.m4 * w1 + M.m5 * w2;
+    const double az = M.m6 * w0 + M.m7 * w1 + M.m8 * w2;
     if (pt.flags & 1) {
-        x = x + pt.pre * s.M[a];
-        y = y + pt.pre * s.M[3 + a];
-        z = z + pt.pre * s.M[6 + a];
+        x = x + pt.pre * ax;
+        y = y + pt.pre * ay;
+        z = z + pt.pre * az;
     }
     Vec3<T> P{(T)x, (T)y, (T)z};
     if (pt.flags & 2) {
         const T f = (T)pt.post;
-        P.x = P.x - f * (T)s.M[a];
-        P.y = P.y - f * (T)s.M[3 + a];
-        P.z = P.z - f * (T)s.M[6 + a];
+        P.x = P.x - f * (T)ax;
+        P.y = P.y - f * (T)ay;
+        P.z = P.z - f * (T)az;
     }
     return P;
 }

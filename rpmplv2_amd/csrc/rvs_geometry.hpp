@@ -24,10 +24,21 @@
 
 namespace rvs {
 
+// Run-time indexed reads go through a register vector (extractelement),
+// never through a stack copy: an indexed load would put the operand in
+// scratch memory.
+template <typename T> using vec4_t = T __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__host__ __device__ inline T pick(T a, T b, T c, int k) {
+    const vec4_t<T> v = {a, b, c, T(0)};
+    return v[k];
+}
+
 template <typename T>
 struct Vec3 {
     T x, y, z;
-    __host__ __device__ T operator[](int k) const { return k == 0 ? x : (k == 1 ? y : z); }
+    __host__ __device__ T operator[](int k) const { return pick(x, y, z, k); }
 };
 
 template <typename T> __host__ __device__ inline Vec3<T> mk(T x, T y, T z) { return Vec3<T>{x, y, z}; }
@@ -52,7 +63,11 @@ template <typename T> __host__ __device__ inline T smin(T a, T b) { return (b < 
 template <typename T>
 struct Box {
     T lo[3], hi[3];
-    __host__ __device__ T f(int face) const { return face < 3 ? lo[face] : hi[face - 3]; }
+    // run-time indexed accessors written as selects so a box held in
+    // registers is never spilled to scratch for indexing
+    __host__ __device__ T lo_at(int k) const { return pick(lo[0], lo[1], lo[2], k); }
+    __host__ __device__ T hi_at(int k) const { return pick(hi[0], hi[1], hi[2], k); }
+    __host__ __device__ T f(int face) const { return face < 3 ? lo_at(face) : hi_at(face - 3); }
     // corner c: bit0 -> x max, bit1 -> y max, bit2 -> z max
     __host__ __device__ Vec3<T> corner(int c) const {
         return Vec3<T>{(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
@@ -177,44 +192,64 @@ __host__ __device__ inline bool face_crossed(const Vec3<T>& A, const Vec3<T>& B,
     if (!(t > 0 && t < 1)) return false;
     const T m0 = A[o1] + t * (B[o1] - A[o1]);
     const T m1 = A[o2] + t * (B[o2] - A[o2]);
-    return m0 > b.lo[o1] && m0 < b.hi[o1] && m1 > b.lo[o2] && m1 < b.hi[o2];
+    return m0 > b.lo_at(o1) && m0 < b.hi_at(o1) && m1 > b.lo_at(o2) && m1 < b.hi_at(o2);
 }
 
-// Work list entries: an edge joins two box corners (3-bit corner indices).
-// Encodings: edge e = c0 | (c1 << 3).
+// Work list entries: an edge joins two box corners (3-bit corner indices),
+// encoded as c0 | (c1 << 3).  Lists are generated on the fly from a small
+// descriptor so they stay in registers:
+//   kListExplicit  up to two edges packed in 'param' (6 bits each)
+//   kListRing      the four edges of face 'param' (face 0..5)
+//   kListAll       all twelve edges
+// Ring order follows checkOtherCases (CollisionAndDistance.cpp:631-634,
+// 693-696, 721-724): start at the face's (min, min) corner, walk through the
+// corner that is max on the first free axis.  The twelve-edge order is
+// bottom ring, top ring, then the vertical edges at (xmin,ymin), (xmax,ymin),
+// (xmax,ymax), (xmin,ymax) (lines 747-751).
 __host__ __device__ inline int edge_of(int c0, int c1) { return c0 | (c1 << 3); }
 
-// Ring of the four edges of face 'face', in the reference's order
-// (checkOtherCases, CollisionAndDistance.cpp:631-634, 693-696, 721-724).
-// The ring starts at the face's (min,min) corner and walks through the
-// (max of the first free axis) corner.
-__host__ __device__ inline void face_ring(int face, int* corners /*4*/) {
+enum { kListExplicit = 0, kListRing = 1, kListAll = 2 };
+
+__host__ __device__ inline int ring_corner(int face, int k) {
     const int c = face % 3;
     const int fix = (face >= 3) ? (1 << c) : 0;
-    // free axes in increasing order
-    const int a1 = (c == 0) ? 1 : 0, a2 = (c == 2) ? 1 : 2;
-    corners[0] = fix;
-    corners[1] = fix | (1 << a1);
-    corners[2] = fix | (1 << a1) | (1 << a2);
-    corners[3] = fix | (1 << a2);
+    const int b1 = 1 << ((c == 0) ? 1 : 0), b2 = 1 << ((c == 2) ? 1 : 2);
+    return fix | ((k == 1 || k == 2) ? b1 : 0) | ((k >= 2) ? b2 : 0);
 }
 
-// Which corners a face edge uses for the checkEdges step: getLineSegments
-// (CollisionAndDistance.cpp:542-574) picks the edge(s) of the face rectangle
-// on the side where the other end point lies, first along the first free
-// axis, then along the second.  Returns the number of edges (0..2).
+__host__ __device__ inline int ring_edge(int face, int e) {
+    return edge_of(ring_corner(face, e), ring_corner(face, (e + 1) & 3));
+}
+
+__host__ __device__ inline int list_edge(int kind, int param, int e) {
+    if (kind == kListExplicit) return (param >> (6 * e)) & 63;
+    if (kind == kListRing) return ring_edge(param, e);
+    if (e < 8) return ring_edge(e < 4 ? 2 : 5, e & 3);
+    const int v = e - 8;
+    const int c = ((v == 1 || v == 2) ? 1 : 0) | ((v >= 2) ? 2 : 0);
+    return edge_of(c, c | 4);
+}
+
+// getLineSegments (CollisionAndDistance.cpp:542-574): the edge(s) of face
+// 'face' on the side where the other end point lies, first along the first
+// free axis, then along the second.  Returns the count, edges packed.
 template <typename T>
-__host__ __device__ inline int face_side_edges(const Box<T>& b, int face, const Vec3<T>& other, int* edges) {
+__host__ __device__ inline int face_side_edges(const Box<T>& b, int face, const Vec3<T>& other, int* packed) {
     const int c = face % 3;
     const int fix = (face >= 3) ? (1 << c) : 0;
     const int a1 = (c == 0) ? 1 : 0, a2 = (c == 2) ? 1 : 2;
     const int b1 = 1 << a1, b2 = 1 << a2;
-    int n = 0;
     const T u = other[a1], v = other[a2];
-    if (u < b.lo[a1]) edges[n++] = edge_of(fix, fix | b2);
-    else if (u > b.hi[a1]) edges[n++] = edge_of(fix | b1, fix | b1 | b2);
-    if (v < b.lo[a2]) edges[n++] = edge_of(fix, fix | b1);
-    else if (v > b.hi[a2]) edges[n++] = edge_of(fix | b2, fix | b1 | b2);
+    const T lo1 = b.lo_at(a1), hi1 = b.hi_at(a1);
+    const T lo2 = b.lo_at(a2), hi2 = b.hi_at(a2);
+    int n = 0, pk = 0;
+    if (u < lo1) { pk = edge_of(fix, fix | b2); n = 1; }
+    else if (u > hi1) { pk = edge_of(fix | b1, fix | b1 | b2); n = 1; }
+    int e2 = -1;
+    if (v < lo2) e2 = edge_of(fix, fix | b1);
+    else if (v > hi2) e2 = edge_of(fix | b2, fix | b1 | b2);
+    if (e2 >= 0) { pk |= e2 << (6 * n); ++n; }
+    *packed = pk;
     return n;
 }
 
@@ -228,9 +263,10 @@ __host__ __device__ inline int face_side_edges(const Box<T>& b, int face, const 
 template <typename T, bool WANT_NP>
 __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T radius, const Box<T>& b,
                                          const Params<T>& p, NP<T>* np, T* sin2_min, uint32_t* status) {
-    // --- projectionLineSegOnSide x3 (lines 411-413, 450-474)
-    int pm[2] = {0, 0};
-    T dist[2] = {p.maxd, p.maxd};
+    // --- projectionLineSegOnSide x3 (lines 411-413, 450-474); bit f of the
+    // masks is face f (xmin, ymin, zmin, xmax, ymax, zmax)
+    int pmA = 0, pmB = 0;
+    T distA = p.maxd, distB = p.maxd;
     bool inside = false;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -238,10 +274,12 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const Vec3<T>& X = i ? B : A;
+            int& pm = i ? pmB : pmA;
+            T& dist = i ? distB : distA;
             if (X[o1] >= b.lo[o1] && X[o1] <= b.hi[o1] && X[o2] >= b.lo[o2] && X[o2] <= b.hi[o2]) {
                 if (X[k] > b.lo[k] && X[k] < b.hi[k]) inside = true;
-                else if (X[k] <= b.lo[k]) { pm[i] |= 1 << k; dist[i] = b.lo[k] - X[k]; }
-                else if (X[k] >= b.hi[k]) { pm[i] |= 8 << k; dist[i] = X[k] - b.hi[k]; }
+                else if (X[k] <= b.lo[k]) { pm |= 1 << k; dist = b.lo[k] - X[k]; }
+                else if (X[k] >= b.hi[k]) { pm |= 8 << k; dist = X[k] - b.hi[k]; }
             }
         }
     }
@@ -255,22 +293,20 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
     int nfaces = 0;       // faces to test for a crossing: 0, 1 or 6
     int face1 = 0;        // the single face when nfaces == 1
     int nedges = 0;
-    int edges[12];
+    int kind = kListExplicit, param = 0;
     bool direct = false;  // single edge / corner assigned directly (tie(...))
     int corner = -1;      // corner point case (segment-point distance)
     bool no_edge = false;
 
-    const int both = pm[0] & pm[1];
-    const int any = pm[0] | pm[1];
+    const int any = pmA | pmB;
     if (any) {
-        const int idx = (dist[0] < dist[1]) ? 0 : 1;
-        d_c = smin(dist[0], dist[1]);
-        const Vec3<T>& X = idx ? B : A;
-        const int m = pm[idx];
+        const bool useB = !(distA < distB);
+        d_c = smin(distA, distB);
+        const Vec3<T>& X = useB ? B : A;
+        const int m = useB ? pmB : pmA;
         if (WANT_NP) {
             // first face (0..5 order) of the chosen point's projections
-            int fc = 0;
-            for (int f = 0; f < 6; ++f) if ((m >> f) & 1) { fc = f; break; }
+            const int fc = (m & 1) ? 0 : (m & 2) ? 1 : (m & 4) ? 2 : (m & 8) ? 3 : (m & 16) ? 4 : (m & 32) ? 5 : 0;
             const int ax = fc % 3;
             np->r = X;
             np->o = X;
@@ -278,56 +314,43 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
             if (ax == 0) np->o.x = v; else if (ax == 1) np->o.y = v; else np->o.z = v;
             np->valid = true;
         }
-        if (!both) {
+        if (!(pmA & pmB)) {
             // checkEdges (lines 476-540): face order xmin, xmax, ymin, ymax, zmin, zmax
-            int face = -1;
-            if (m & 1) face = 0; else if (m & 8) face = 3; else if (m & 2) face = 1;
-            else if (m & 16) face = 4; else if (m & 4) face = 2; else if (m & 32) face = 5;
+            const int face = (m & 1) ? 0 : (m & 8) ? 3 : (m & 2) ? 1 : (m & 16) ? 4 : (m & 4) ? 2 : (m & 32) ? 5 : -1;
             if (face < 0) { if (status) *status |= 1u; return T(NAN); }
             nfaces = 1;
             face1 = face;
-            nedges = face_side_edges(b, face, idx ? A : B, edges);
+            nedges = face_side_edges(b, face, useB ? A : B, &param);
             // no edge: the reference would read an uninitialised matrix
             // (only if the face is not crossed, which is tested first)
             no_edge = (nedges == 0);
         }
     } else {
-        // checkOtherCases (lines 597-755): classify each axis as both-below,
-        // both-above or mixed
-        int s[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            s[k] = (A[k] < b.lo[k] && B[k] < b.lo[k]) ? 0 : ((A[k] > b.hi[k] && B[k] > b.hi[k]) ? 1 : 2);
-        // the reference tests x, then y, then z; an axis is "fixed" when both
-        // points lie beyond the same side
-        const int nfix = (s[0] != 2) + (s[1] != 2) + (s[2] != 2);
+        // checkOtherCases (lines 597-755): classify each axis as both-below
+        // (0), both-above (1) or mixed (2)
+        const int s0 = (A.x < b.lo[0] && B.x < b.lo[0]) ? 0 : ((A.x > b.hi[0] && B.x > b.hi[0]) ? 1 : 2);
+        const int s1 = (A.y < b.lo[1] && B.y < b.lo[1]) ? 0 : ((A.y > b.hi[1] && B.y > b.hi[1]) ? 1 : 2);
+        const int s2 = (A.z < b.lo[2] && B.z < b.lo[2]) ? 0 : ((A.z > b.hi[2] && B.z > b.hi[2]) ? 1 : 2);
+        const int nfix = (s0 != 2) + (s1 != 2) + (s2 != 2);
         if (nfix == 3) {
-            corner = (s[0] ? 1 : 0) | (s[1] ? 2 : 0) | (s[2] ? 4 : 0);
+            corner = s0 | (s1 << 1) | (s2 << 2);
             direct = true;
         } else if (nfix == 2) {
             // edge along the free axis through the fixed coordinates
-            int base = 0, free = 0;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) { if (s[k] == 2) free = k; else base |= s[k] << k; }
-            edges[0] = edge_of(base, base | (1 << free));
+            const int free = (s0 == 2) ? 0 : ((s1 == 2) ? 1 : 2);
+            const int base = ((s0 == 1) ? 1 : 0) | ((s1 == 1) ? 2 : 0) | ((s2 == 1) ? 4 : 0);
+            param = edge_of(base, base | (1 << free));
             nedges = 1;
             direct = true;
         } else if (nfix == 1) {
-            int k = (s[0] != 2) ? 0 : ((s[1] != 2) ? 1 : 2);
-            int ring[4];
-            face_ring(k + 3 * s[k], ring);
-            for (int e = 0; e < 4; ++e) edges[e] = edge_of(ring[e], ring[(e + 1) & 3]);
+            const int k = (s0 != 2) ? 0 : ((s1 != 2) ? 1 : 2);
+            const int sk = (k == 0) ? s0 : ((k == 1) ? s1 : s2);
+            kind = kListRing;
+            param = k + 3 * sk;
             nedges = 4;
         } else {
             nfaces = 6;
-            // bottom ring, top ring, then the four vertical edges
-            int ring[4];
-            face_ring(2, ring);
-            for (int e = 0; e < 4; ++e) edges[e] = edge_of(ring[e], ring[(e + 1) & 3]);
-            face_ring(5, ring);
-            for (int e = 0; e < 4; ++e) edges[4 + e] = edge_of(ring[e], ring[(e + 1) & 3]);
-            const int vx[4] = {0, 1, 3, 2};
-            for (int e = 0; e < 4; ++e) edges[8 + e] = edge_of(vx[e], vx[e] | 4);
+            kind = kListAll;
             nedges = 12;
         }
     }
@@ -349,7 +372,8 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
         if (WANT_NP) *np = t;
     } else {
         for (int e = 0; e < nedges; ++e) {
-            const Vec3<T> C = b.corner(edges[e] & 7), D = b.corner(edges[e] >> 3);
+            const int ed = list_edge(kind, param, e);
+            const Vec3<T> C = b.corner(ed & 7), D = b.corner(ed >> 3);
             NP<T> t;
             const T d_tmp = seg_seg<T, WANT_NP>(A, B, C, D, p, &t);
             if (sin2_min) *sin2_min = smin(*sin2_min, seg_seg_sin2(A, B, C, D));
@@ -380,12 +404,12 @@ __host__ __device__ inline T check_cases(const Vec3<T>& A, const Vec3<T>& B, con
     const int b1 = 1 << a1, b2 = 1 << a2;
     T d1 = p.maxd, d2 = p.maxd;
     int e = -1;
-    if (u < b.lo[a1]) e = edge_of(fix, fix | b2);
-    else if (u > b.hi[a1]) e = edge_of(fix | b1, fix | b1 | b2);
+    if (u < b.lo_at(a1)) e = edge_of(fix, fix | b2);
+    else if (u > b.hi_at(a1)) e = edge_of(fix | b1, fix | b1 | b2);
     if (e >= 0) d1 = seg_seg<T, false>(A, B, b.corner(e & 7), b.corner(e >> 3), p, nullptr);
     e = -1;
-    if (d1 > 0 && v < b.lo[a2]) e = edge_of(fix, fix | b1);
-    else if (d1 > 0 && v > b.hi[a2]) e = edge_of(fix | b2, fix | b1 | b2);
+    if (d1 > 0 && v < b.lo_at(a2)) e = edge_of(fix, fix | b1);
+    else if (d1 > 0 && v > b.hi_at(a2)) e = edge_of(fix | b2, fix | b1 | b2);
     if (e >= 0) d2 = seg_seg<T, false>(A, B, b.corner(e & 7), b.corner(e >> 3), p, nullptr);
     return smin(d1, d2);
 }
@@ -398,7 +422,7 @@ __host__ __device__ inline bool rect_collision(const Vec3<T>& A, const Vec3<T>& 
     const int o1 = (c == 0) ? 1 : 0, o2 = (c == 2) ? 1 : 2;
     const T v = b.f(face);
     // rec = (min1, min2, max1, max2)
-    const T r0 = b.lo[o1], r1 = b.lo[o2], r2 = b.hi[o1], r3 = b.hi[o2];
+    const T r0 = b.lo_at(o1), r1 = b.lo_at(o2), r2 = b.hi_at(o1), r3 = b.hi_at(o2);
     const T Au = A[o1], Av = A[o2], Bu = B[o1], Bv = B[o2];
     const T den = B[c] - A[c];
     const T t = (den != 0) ? (v - A[c]) / den : p.maxd;

@@ -75,8 +75,9 @@ template <> __device__ inline double rnew_of<double>(const RobotDev& rb, int i) 
 // running minimum is <= 0 (the profile keeps that value, later links stay 0,
 // the distance is 0 and the nearest points are dropped).
 template <typename T, bool WANT_NP>
-__global__ __launch_bounds__(kBlock) void k_distance(RobotDev rb, EnvDev env, QueryParams qp,
-                                                     const float* __restrict__ q, int64_t n, DistOut out) {
+__device__ __attribute__((always_inline)) inline void distance_body(const RobotDev& rb, const EnvDev& env,
+                                                                    const QueryParams& qp, const float* __restrict__ q,
+                                                                    int64_t n, const DistOut& out) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<T> s = stage_env<T>(env, smem);
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -107,7 +108,8 @@ __global__ __launch_bounds__(kBlock) void k_distance(RobotDev rb, EnvDev env, Qu
                     np.valid = true;
                 }
             } else {
-                dt = capsule_box<T, WANT_NP>(P0, P1, r, s.box[j], p, &np, nullptr, nullptr);
+                const Box<T> bx = s.box[j];
+                dt = capsule_box<T, WANT_NP>(P0, P1, r, bx, p, &np, nullptr, nullptr);
             }
             if (dt > s.tol[j]) dt = p.maxd;
             prof = smin(prof, dt);
@@ -142,6 +144,20 @@ __global__ __launch_bounds__(kBlock) void k_distance(RobotDev rb, EnvDev env, Qu
     if (out.d) out.d[c] = (float)d;
     if (out.d64) out.d64[c] = (double)d;
     if (out.verdict) out.verdict[c] = hit ? 0 : 1;
+}
+
+// The float kernel is held to 128 VGPRs (4 waves per SIMD); the double
+// kernel keeps its natural allocation (a cap would spill).
+template <bool WANT_NP>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance32(
+    RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out) {
+    distance_body<float, WANT_NP>(rb, env, qp, q, n, out);
+}
+
+template <bool WANT_NP>
+__global__ __launch_bounds__(kBlock) void k_distance64(RobotDev rb, EnvDev env, QueryParams qp,
+                                                       const float* __restrict__ q, int64_t n, DistOut out) {
+    distance_body<double, WANT_NP>(rb, env, qp, q, n, out);
 }
 
 // ---------------------------------------------------------------- isValid
@@ -266,9 +282,9 @@ hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const Qu
     if (n <= 0) return hipSuccess;
     const size_t sm = staged_bytes<float>(env.num_obj);
     if (out.nearest)
-        hipLaunchKernelGGL((k_distance<float, true>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+        hipLaunchKernelGGL((k_distance32<true>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
     else
-        hipLaunchKernelGGL((k_distance<float, false>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+        hipLaunchKernelGGL((k_distance32<false>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
     return hipGetLastError();
 }
 
@@ -277,9 +293,9 @@ hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const Que
     if (n <= 0) return hipSuccess;
     const size_t sm = staged_bytes<double>(env.num_obj);
     if (out.nearest)
-        hipLaunchKernelGGL((k_distance<double, true>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+        hipLaunchKernelGGL((k_distance64<true>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
     else
-        hipLaunchKernelGGL((k_distance<double, false>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+        hipLaunchKernelGGL((k_distance64<false>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
     return hipGetLastError();
 }
 

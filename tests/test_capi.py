@@ -1,0 +1,102 @@
+"""CPU tests of the boundary and the host logic (no GPU needed).
+
+* librvs.so loads and exports every function include/rvs.h declares.
+* Context creation without a usable GPU fails with a status, never a crash.
+* The kernels' geometry source, compiled for the host, is bit-identical to
+  the oracle on random and adversarial pairs (tests/host/geom_check.cpp).
+* Workload generators are deterministic and follow their specifications.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from rpmplv2_amd import robots as R, rvs, workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    with open(os.path.join(ROOT, "include", "rvs.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rvs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_declared_symbols():
+    lib = rvs.lib()
+    names = declared_functions()
+    assert len(names) >= 17
+    for name in names:
+        assert hasattr(lib, name), name
+    # and nothing the binding expects is missing from the header
+    assert set(rvs.EXPORTS) <= set(names)
+
+
+def test_exports_are_c_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", rvs.LIB_PATH], capture_output=True, text=True).stdout
+    syms = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    for name in declared_functions():
+        assert name in syms, name
+
+
+def test_create_without_gpu_reports_status():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    L = rvs.lib()
+    ctx = C.c_void_p()
+    o = rvs.Options()
+    L.rvs_default_options(C.byref(o))
+    st = L.rvs_create(C.byref(o), C.byref(ctx))
+    assert st == rvs.RVS_EDEVICE and not ctx.value
+    assert L.rvs_create(None, None) == rvs.RVS_EINVAL
+    with pytest.raises(rvs.RvsError):
+        rvs.Engine()
+
+
+def test_default_options_follow_reference_yaml():
+    o = rvs.Options()
+    rvs.lib().rvs_default_options(C.byref(o))
+    assert o.equality_threshold == np.float32(1e-4) and o.max_distance == 1000 and o.num_checks == 10
+    assert o.precision == rvs.PREC_REF32
+
+
+def test_device_geometry_source_matches_oracle_on_host():
+    exe = os.path.join(ROOT, "tests", "host", "_build", "geom_check")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "host")], check=True, capture_output=True)
+    r = subprocess.run([exe, "400000", "17"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "capsule_box=0 nearest=0 collision=0 seg_seg=0 seg_pt=0" in r.stdout
+
+
+def test_workloads_deterministic_and_in_spec():
+    a, b = W.workload2(5000), W.workload2(5000)
+    assert np.array_equal(a.q, b.q) and np.array_equal(a.scene.aabb, b.scene.aabb)
+    lim = a.robot.limits
+    assert (a.q >= lim[:, 0]).all() and (a.q <= lim[:, 1]).all()
+    sc = a.scene
+    assert sc.num_obj == 17 and sc.is_ground[0] == 1 and sc.is_ground[1:].sum() == 0
+    assert (sc.dim[1:] >= 0.05).all() and (sc.dim[1:] <= 0.3).all()
+    assert (np.linalg.norm(sc.pos[1:] - [0, 0, 0.4], axis=1) <= 1.0 + 1e-6).all()
+    w1 = W.workload1(1000)
+    assert w1.scene.num_obj == 4 and (np.hypot(w1.scene.pos[:, 0], w1.scene.pos[:, 1]) <= 2).all()
+    w3 = W.workload3(64)
+    assert [b.robot.num_dofs for b in w3.batches] == [10, 14, 18, 22]
+    for bt in w3.batches:
+        assert (bt.K >= 8).all() and (bt.K <= 256).all()
+        assert np.allclose(np.linalg.norm(bt.dir, axis=1), 1, atol=1e-6)
+        assert len(bt.robot.self_pairs) == (bt.robot.num_links - 1) * (bt.robot.num_links - 2) // 2
+
+
+def test_robot_files_match_reference_kinematics():
+    xa = R.xarm6()
+    assert xa.num_dofs == 6 and xa.num_links == 6
+    assert np.allclose(R.skeleton_np(xa, np.zeros(6))[0, 2], [0.0535, 0, 0.5515], atol=2e-6)
+    sp = R.spatial(14)
+    assert sp.num_dofs == 14 and abs(sp.joints[2].xyz[2] - 0.0692) < 1e-12

@@ -1,0 +1,402 @@
+"""CPU tests that pin the oracle (oracle/rvs_oracle.cpp) before it is trusted.
+
+Each check is chosen so that a plausible mistake in the restatement fails it:
+brute force and closed forms for the pair geometry, closed forms and
+invariants for the kinematics, FCL's box rule for the bounds, hand-built
+scenes for the RealVectorSpace loop semantics, and the known answers held by
+RPMPLv2's own data files (tests/golden/reference_scenarios.json).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from rpmplv2_amd import robots as R, workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EQ = 1e-4
+
+
+# ---------------------------------------------------------------- brute force
+def brute_seg_seg(A, B, C, D, n=801):
+    """Distance of two segments by a dense grid, then a local refinement."""
+    t = np.linspace(0, 1, n)
+    P = A[None] + t[:, None] * (B - A)[None]
+    Q = C[None] + t[:, None] * (D - C)[None]
+    d2 = ((P[:, None, :] - Q[None, :, :]) ** 2).sum(-1)
+    i, j = np.unravel_index(np.argmin(d2), d2.shape)
+    ti, tj = t[i], t[j]
+    h = 1.0 / (n - 1)
+    for _ in range(40):
+        tt = np.clip(np.linspace(ti - h, ti + h, 21), 0, 1)
+        ss = np.clip(np.linspace(tj - h, tj + h, 21), 0, 1)
+        P = A[None] + tt[:, None] * (B - A)[None]
+        Q = C[None] + ss[:, None] * (D - C)[None]
+        d2 = ((P[:, None, :] - Q[None, :, :]) ** 2).sum(-1)
+        i, j = np.unravel_index(np.argmin(d2), d2.shape)
+        ti, tj = tt[i], ss[j]
+        h *= 0.3
+    return float(np.sqrt(d2[i, j]))
+
+
+def brute_seg_box(A, B, obs, n=20001):
+    t = np.linspace(0, 1, n)
+    P = A[None] + t[:, None] * (B - A)[None]
+    e = np.maximum(np.maximum(obs[:3] - P, P - obs[3:]), 0)
+    return float(np.sqrt((e ** 2).sum(-1)).min())
+
+
+def test_seg_seg_matches_brute_force(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(300):
+        A, B, C, D = (rng.uniform(-1, 1, 3) for _ in range(4))
+        d64, _, _ = oracle.seg_seg(A, B, C, D, 64)
+        ref = brute_seg_seg(A, B, C, D)
+        if ref < EQ:
+            assert d64 == 0.0
+        else:
+            assert abs(d64 - ref) < 1e-7
+        # symmetric in the two segments
+        assert abs(oracle.seg_seg(C, D, A, B, 64)[0] - d64) < 1e-12
+
+
+def test_seg_seg_special_cases(oracle):
+    A, B = np.array([0.0, 0, 0]), np.array([1.0, 0, 0])
+    # crossing segments snap to zero
+    assert oracle.seg_seg(A, B, np.array([0.5, -1, 0]), np.array([0.5, 1, 0]), 64)[0] == 0.0
+    # parallel overlapping segments: the line distance
+    d, np_, v = oracle.seg_seg(A, B, np.array([0.2, 0.3, 0]), np.array([0.8, 0.3, 0]), 64)
+    assert abs(d - 0.3) < 1e-12 and v == 1
+    # collinear disjoint: end point to end point
+    assert abs(oracle.seg_seg(A, B, np.array([2.0, 0, 0]), np.array([3.0, 0, 0]), 64)[0] - 1.0) < 1e-12
+    # skew, closest points interior: distance along z
+    d, np_, v = oracle.seg_seg(A, B, np.array([0.5, -1, 0.7]), np.array([0.5, 1, 0.7]), 64)
+    assert abs(d - 0.7) < 1e-12 and np.allclose(np_, [0.5, 0, 0, 0.5, 0, 0.7])
+    # degenerate second segment (a point) still gets its distance
+    assert abs(oracle.seg_seg(A, B, np.array([0.5, 2, 0]), np.array([0.5, 2, 0]), 64)[0] - 2.0) < 1e-12
+
+
+def test_seg_pt_closed_form(oracle):
+    rng = np.random.default_rng(2)
+    for _ in range(500):
+        A, B, C = (rng.uniform(-1, 1, 3) for _ in range(3))
+        u = B - A
+        t = np.clip(np.dot(C - A, u) / np.dot(u, u), 0, 1)
+        ref = np.linalg.norm(C - (A + t * u))
+        d = oracle.seg_pt(A, B, C, 64)[0]
+        assert (d == 0.0) if ref < EQ else abs(d - ref) < 1e-12
+
+
+def test_true_seg_box_checker(oracle):
+    """The exact segment-box checker against dense sampling."""
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        lo = rng.uniform(-1, 0.5, 3)
+        obs = np.concatenate([lo, lo + rng.uniform(0.05, 0.6, 3)])
+        A, B = rng.uniform(-1.5, 1.5, 3), rng.uniform(-1.5, 1.5, 3)
+        assert abs(oracle.true_seg_box(A, B, obs) - brute_seg_box(A, B, obs)) < 2e-4
+
+
+def test_capsule_box_tree_equals_true_distance(oracle):
+    """Away from the EQUALITY_THRESHOLD snap the reference's case tree returns
+    the exact segment-box distance minus the radius; inside -> 0."""
+    rng = np.random.default_rng(4)
+    paths = set()
+    for _ in range(20000):
+        lo = rng.uniform(-1, 1, 3) - rng.uniform(0.02, 0.4, 3)
+        obs = np.concatenate([lo, lo + 2 * rng.uniform(0.02, 0.4, 3)])
+        A = rng.uniform(-1.6, 1.6, 3)
+        B = A + rng.uniform(-0.6, 0.6, 3)
+        r = rng.uniform(0, 0.1)
+        d, npt, v, path = oracle.capsule_box(A, B, r, obs, 64)
+        paths.add(path)
+        true = oracle.true_seg_box(A, B, obs)
+        if path == 0:
+            assert true == 0.0 and d == 0.0
+        elif true > 2 * EQ:
+            assert abs((d + r) - true) < 1e-9, (path, d + r, true)
+            if v:  # nearest points realise the distance
+                assert abs(np.linalg.norm(npt[3:] - npt[:3]) - true) < 1e-9
+        elif true == 0.0:
+            assert d <= 0.0
+    assert paths >= {0, 1, 2, 3, 4, 5, 6, 8}
+
+
+def test_capsule_box_float_close_to_double(oracle):
+    rng = np.random.default_rng(5)
+    for _ in range(5000):
+        lo = rng.uniform(-1, 1, 3) - 0.2
+        obs = np.concatenate([lo, lo + rng.uniform(0.05, 0.5, 3)]).astype(np.float32)
+        A = rng.uniform(-1.6, 1.6, 3).astype(np.float32)
+        B = (A + rng.uniform(-0.6, 0.6, 3)).astype(np.float32)
+        d32 = oracle.capsule_box(A, B, 0.05, obs, 32)[0]
+        d64 = oracle.capsule_box(A.astype(np.float64), B.astype(np.float64), float(np.float32(0.05)),
+                                 obs.astype(np.float64), 64)[0]
+        if abs(d64 + 0.05) > 2 * EQ:
+            assert abs(d32 - d64) < 1e-6
+
+
+def test_collision_tree_rules(oracle):
+    obs = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    r = 0.1
+    rn = r * np.sqrt(3) / 3
+    # end point inside the box inflated by r/sqrt(3): collision
+    assert oracle.collision_capsule_box([1 + 0.9 * rn, 0.5, 0.5], [3, 3, 3], r, obs)
+    # both end points beyond the same face by more than r: free
+    assert not oracle.collision_capsule_box([1.2, -5, 0.5], [1.3, 5, 0.5], r, obs)
+    # a segment crossing the box
+    assert oracle.collision_capsule_box([-1, 0.5, 0.5], [2, 0.5, 0.5], r, obs)
+    # statistical agreement with the exact distance test
+    rng = np.random.default_rng(6)
+    agree = n = 0
+    for _ in range(5000):
+        A = rng.uniform(-0.5, 1.5, 3).astype(np.float32)
+        B = (A + rng.uniform(-0.5, 0.5, 3)).astype(np.float32)
+        true = oracle.true_seg_box(A.astype(np.float64), B.astype(np.float64), obs.astype(np.float64))
+        if abs(true - r) < 1e-3:
+            continue
+        n += 1
+        agree += oracle.collision_capsule_box(A, B, r, obs) == (true < r)
+    assert agree / n > 0.97
+
+
+# ---------------------------------------------------------------- kinematics
+def test_fk_planar_closed_form(oracle):
+    rb = R.planar_2dof()
+    q = np.random.default_rng(7).uniform(-np.pi, np.pi, (200, 2)).astype(np.float32)
+    P = oracle.skeleton(rb, q, 64)
+    q64 = q.astype(np.float64)
+    p1 = np.stack([np.cos(q64[:, 0]), np.sin(q64[:, 0]), 0 * q64[:, 0]], 1)
+    p2 = p1 + np.stack([np.cos(q64.sum(1)), np.sin(q64.sum(1)), 0 * q64[:, 0]], 1)
+    assert np.abs(P[:, 0]).max() == 0
+    assert np.abs(P[:, 1] - p1).max() < 1e-14
+    assert np.abs(P[:, 2] - p2).max() < 1e-14
+
+
+def test_fk_xarm6_zero_pose(oracle):
+    """Skeleton of xArm6 at q = 0 from the robot file (hand-derived; the
+    off-axis residue comes from rpy = 1.5708 != pi/2)."""
+    P = oracle.skeleton(R.xarm6(), np.zeros(6, np.float32), 64)[0]
+    want = np.array([[0, 0, 0], [0, 0, 0.267], [0.0535, 0, 0.5515], [0.131, 0, 0.459], [0.131, 0, 0.209],
+                     [0.207, 0, 0.209], [0.207, 0, 0.112]])
+    assert np.abs(P - want).max() < 2e-6
+
+
+def test_fk_chain_invariants(oracle):
+    rb = R.spatial(22)
+    q = np.random.default_rng(8).uniform(-np.pi, np.pi, (100, 22)).astype(np.float32)
+    P = oracle.skeleton(rb, q, 64)
+    lengths = np.linalg.norm(np.diff(P, axis=1), axis=2)
+    want = np.array([0.1] + [0.042857] * 21)
+    assert np.abs(lengths - want[None]).max() < 1e-12
+    assert np.abs(P[:, 0]).max() == 0 and np.abs(P[:, 1] - [0, 0, 0.1]).max() < 1e-15
+    # q = 0 points straight up
+    P0 = oracle.skeleton(rb, np.zeros(22, np.float32), 64)[0]
+    assert np.abs(P0[:, :2]).max() < 1e-15 and abs(P0[-1, 2] - (0.1 + 21 * 0.042857)) < 1e-12
+    # float skeleton is the double one rounded
+    P32 = oracle.skeleton(rb, q, 32)
+    assert np.array_equal(P32, P.astype(np.float32))
+
+
+def test_fk_xarm6_link_lengths(oracle):
+    rb = R.xarm6()
+    q = W.workload2(500).q
+    P = oracle.skeleton(rb, q, 64)
+    L = np.linalg.norm(np.diff(P, axis=1), axis=2)
+    assert np.abs(L - L[0]).max() < 1e-9   # rigid links
+    assert abs(L[0, 0] - 0.267) < 1e-12 and abs(L[0, 4] - 0.076) < 1e-12
+
+
+# ---------------------------------------------------------------- bounds
+def test_box_aabb_rules(oracle):
+    dim, pos = [0.3, 0.5, 0.7], [1.0, -2.0, 0.25]
+    aabb = oracle.box_aabb(dim, pos, [1, 0, 0, 0])
+    h = np.float32(0.5) * np.float32(dim)          # FCL: local half extents, then translate
+    assert np.array_equal(aabb, np.concatenate([-h + np.float32(pos), h + np.float32(pos)]))
+    assert np.allclose(aabb, [0.85, -2.25, -0.1, 1.15, -1.75, 0.6], atol=1e-7)
+    q = np.array([np.cos(0.3), np.sin(0.3), 0, 0])  # 0.6 rad about x
+    rad = np.float32(np.linalg.norm(np.float32(dim) / 2))
+    aabb = oracle.box_aabb(dim, pos, q)
+    assert np.allclose(aabb, np.concatenate([np.float32(pos) - rad, np.float32(pos) + rad]), atol=1e-7)
+    # tiny rotations fall under Eigen's isIdentity tolerance
+    assert np.array_equal(oracle.box_aabb(dim, pos, [1, 1e-7, 0, 0]), oracle.box_aabb(dim, pos, [1, 0, 0, 0]))
+    # the package's vectorised rule is bit-identical to the oracle's
+    rng = np.random.default_rng(9)
+    for rot in (False, True):
+        sc = W._boxes(rng, 64, 0.02, 0.5, 2.0, rotated=rot)
+        ref = np.stack([oracle.box_aabb(sc.dim[j], sc.pos[j], sc.quat[j]) for j in range(64)])
+        assert np.array_equal(sc.aabb, ref)
+
+
+# ---------------------------------------------------------------- loop semantics
+def _scene(boxes, ground=None, tol=None):
+    boxes = np.asarray(boxes, np.float32).reshape(-1, 6)
+    m = len(boxes)
+    g = np.zeros(m, np.uint8) if ground is None else np.asarray(ground, np.uint8)
+    t = np.full(m, 1000, np.float32) if tol is None else np.asarray(tol, np.float32)
+    return boxes, g, t
+
+
+def test_distance_early_exit_partial_profile(oracle):
+    rb = R.planar_2dof(radius=0.05)
+    # q = (0, pi/2): link 0 along x to (1,0,0), link 1 up to (1,1,0)
+    q = np.array([[0, np.pi / 2]], np.float32)
+    boxes, g, t = _scene([[0.3, 0.5, -0.1, 0.4, 0.6, 0.1],     # clear of both links
+                          [1.02, 0.4, -0.1, 1.2, 0.6, 0.1],    # 0.02 from link 1 -> -0.03
+                          [0.5, 0.02, -0.1, 0.6, 0.2, 0.1]])   # 0.02 from link 0 -> -0.03
+    sc = oracle.Scene(boxes, g, t)
+    d, prof, ver, _ = oracle.distance(rb, sc, q, 64)
+    # link 0 meets its first non-positive pair at box 2: profile keeps it,
+    # link 1 is never reached and stays 0, the distance is 0
+    assert ver[0] == 0 and d[0] == 0
+    assert abs(prof[0, 0] - (0.02 - 0.05)) < 1e-6 and prof[0, 1] == 0
+    # without box 2 the first collision is link 1 / box 1
+    sc2 = oracle.Scene(boxes[:2], g[:2], t[:2])
+    d, prof, ver, _ = oracle.distance(rb, sc2, q, 64)
+    assert ver[0] == 0 and abs(prof[0, 1] - (-0.03)) < 1e-6
+    link0 = min(oracle.true_seg_box(np.zeros(3), np.array([1.0, 0, 0]), b.astype(np.float64)) for b in boxes[:2])
+    assert abs(prof[0, 0] - (link0 - 0.05)) < 1e-6
+
+
+def test_distance_ground_skip_and_tolerance(oracle):
+    rb = R.planar_2dof(radius=0.0)
+    rb.ground_included = 1
+    q = np.array([[0, np.pi / 2]], np.float32)
+    ground = [-5, -5, -1, 5, 5, -0.5]
+    boxes, g, t = _scene([ground, [1.3, 0.4, -0.1, 1.5, 0.6, 0.1]], ground=[1, 0], tol=[0.05, 1000])
+    sc = oracle.Scene(boxes, g, t)
+    d, prof, ver, npo = oracle.distance(rb, sc, q, 64, want_np=True)
+    # link 0 ignores the ground; link 1 sees it at 0.5 > tol 0.05 -> MAX_DISTANCE
+    assert ver[0] == 1
+    assert abs(prof[0, 1] - 0.3) < 1e-7
+    assert np.array_equal(npo[0, 0, 0], [0, 0, 0, 0, 0, -1000])
+    # a ground closer than its tolerance is reported
+    boxes[0, 5] = 0.0 - 0.04 + 0.0   # top at -0.04
+    boxes[0, 5] = -0.04
+    sc = oracle.Scene(boxes, g, t)
+    d, prof, ver, _ = oracle.distance(rb, sc, q, 64)
+    assert abs(prof[0, 1] - 0.04) < 1e-7
+
+
+def test_underestimation_lower_bound(oracle):
+    w = W.workload2(600)
+    sc = oracle.scene_of(w.scene)
+    d, prof, ver, npo = oracle.distance(w.robot, sc, w.q, 32, want_np=True)
+    ok = np.flatnonzero(ver == 1)
+    # planes from each configuration's own nearest points: a lower bound that
+    # is tight for most configurations (apps/test_real_vector_space.cpp:46-57
+    # checks one configuration at 1e-3)
+    du, pu, st = oracle.underestimation(w.robot, sc, w.q[ok], npo[ok])
+    assert (st == 1).all() and (du <= d[ok] + 1e-6).all()
+    assert np.mean(np.abs(du - d[ok]) < 1e-3) > 0.8
+    # planes from another configuration's nearest points are lower bounds
+    ref = npo[ok[0]]
+    du, pu, st = oracle.underestimation(w.robot, sc, w.q[ok], ref)
+    stored = st == 1
+    assert (pu[stored] <= prof[ok][stored] + 1e-5).all()
+
+
+def test_underestimation_closed_form(oracle):
+    # planar link 0 along x at height 0 above a box whose top face is z = -0.3:
+    # nearest points R = (0.5, 0, 0), O = (0.5, 0, -0.3); the plane is z = -0.3
+    rb = R.planar_2dof(radius=0.05)
+    boxes, g, t = _scene([[0.2, -0.2, -1.0, 0.8, 0.2, -0.3]])
+    sc = oracle.Scene(boxes, g, t)
+    ref = np.zeros((1, 2, 6), np.float32)
+    ref[0, 0] = [0.5, 0, 0, 0.5, 0, -0.3]
+    ref[0, 1] = [0, 0, 1000, 0, 0, -1000]   # |O| == MAX_DISTANCE: skipped
+    q = np.array([[0.0, 0.7]], np.float32)
+    du, pu, st = oracle.underestimation(rb, sc, q, ref)
+    assert st[0] == 1 and abs(pu[0, 0] - (np.float32(0.3) - np.float32(0.05))) < 1e-7
+    assert pu[0, 1] == 1000 and abs(du[0] - pu[0, 0]) == 0
+    # a negative plane distance returns 0 without storing
+    ref[0, 0] = [0.5, 0, 0.5, 0.5, 0, 0.2]
+    du, pu, st = oracle.underestimation(rb, sc, q, ref)
+    assert st[0] == 0 and du[0] == 0
+
+
+def test_edge_validity_semantics(oracle):
+    rb = R.planar_2dof(radius=0.0)
+    # box hit only at q1 = (0, 0): the edge check never tests q1
+    boxes, g, t = _scene([[1.9, -0.05, -0.1, 2.1, 0.05, 0.1]])
+    sc = oracle.Scene(boxes, g, t)
+    q1 = np.array([[0.0, 0.0]], np.float32)
+    q2 = np.array([[1.0, 0.0]], np.float32)
+    assert oracle.is_valid(rb, sc, q1)[0] == 0
+    assert oracle.edge_valid(rb, sc, q1, q2)[0] == 1
+    assert oracle.edge_valid(rb, sc, q2, q1)[0] == 0
+
+
+# ---------------------------------------------------------------- known answers
+def _ref_fixtures():
+    with open(os.path.join(HERE, "golden", "reference_scenarios.json")) as f:
+        return json.load(f)
+
+
+def _robot_of(key, radius, ground_included):
+    if key == "planar_2dof":
+        return R.planar_2dof(radius=radius)
+    n = int(key.split("_")[1][:-3])
+    return R.spatial(n, radius=radius, ground_included=ground_included)
+
+
+@pytest.mark.parametrize("key", ["planar_2dof", "spatial_10dof", "spatial_14dof", "spatial_18dof", "spatial_22dof"])
+def test_reference_random_scenarios(oracle, key):
+    """Every stored start is >= 0.3 m from that run's boxes (printed with 6
+    significant digits, hence the 3e-4 m slack), and start and goal are
+    >= D_CRIT from the scenario's fixed boxes."""
+    e = _ref_fixtures()["random"][key]
+    r = e["capsules_radius"]
+    rb = _robot_of(key, r if len(r) > 1 else r[0], e["ground_included"])
+    dim = e["obstacle_dim"]
+    for run in e["runs"]:
+        pos = np.asarray(run["objects"], np.float32).reshape(-1, 3)
+        m = len(pos)
+        sc = W.Scene(np.tile(dim, (m, 1)), pos, np.tile([1, 0, 0, 0], (m, 1)), np.zeros(m, np.uint8),
+                     np.full(m, 1000, np.float32))
+        d, _, _, _ = oracle.distance(rb, oracle.scene_of(sc), np.asarray(run["start"], np.float32), 32)
+        assert d[0] >= 0.3 - 3e-4, (key, run["scenario"], run["run"], d[0])
+        fixed = e["fixed_boxes"]
+        if fixed:
+            fs = W.Scene([b["dim"] for b in fixed], [b["pos"] for b in fixed], np.tile([1, 0, 0, 0], (len(fixed), 1)),
+                         [1 if b["label"] == "ground" else 0 for b in fixed], [b["min_dist_tol"] for b in fixed])
+            for qq in (run["start"], run["goal"]):
+                d, _, _, _ = oracle.distance(rb, oracle.scene_of(fs), np.asarray(qq, np.float32), 32)
+                assert d[0] >= 0.05 - 3e-4
+
+
+def test_reference_fixed_scenarios_valid(oracle):
+    for f in _ref_fixtures()["fixed"]:
+        n = f["num_dofs"]
+        r = f["capsules_radius"]
+        if f["type"] == "xarm6":
+            rb = R.xarm6(radius=r, ground_included=f["ground_included"], gripper_length=f["gripper_length"])
+        elif f["type"] == "planar_2dof":
+            rb = R.planar_2dof(radius=0.025 if r is None else r)
+        else:
+            rb = R.spatial(n, radius=r[0] if len(r) == 1 else r, ground_included=f["ground_included"])
+        boxes = f["boxes"]
+        quat = [[b["rot"][3], b["rot"][0], b["rot"][1], b["rot"][2]] if b["rot"] else [1, 0, 0, 0] for b in boxes]
+        sc = W.Scene([b["dim"] for b in boxes], [b["pos"] for b in boxes], quat,
+                     [1 if b["label"] == "ground" else 0 for b in boxes], [b["min_dist_tol"] for b in boxes])
+        q = np.asarray([f["q_start"], f["q_goal"]], np.float32)
+        assert oracle.is_valid(rb, oracle.scene_of(sc), q).all(), (f["robot"], f["scenario"])
+        d, _, ver, _ = oracle.distance(rb, oracle.scene_of(sc), q, 32)
+        assert (ver == 1).all() and (d > 0).all()
+
+
+# ---------------------------------------------------------------- frozen vectors
+def test_oracle_vectors_unchanged(oracle):
+    z = np.load(os.path.join(HERE, "golden", "oracle_vectors.npz"))
+    for k in range(len(z["cb_A"])):
+        d32 = oracle.capsule_box(z["cb_A"][k], z["cb_B"][k], z["cb_r"][k], z["cb_obs"][k], 32)[0]
+        assert np.float32(d32).tobytes() == z["cb_d32"][k].tobytes() or (np.isnan(d32) and np.isnan(z["cb_d32"][k]))
+    for name, w in (("planar_2dof", W.workload1(256)), ("xarm6", W.workload2(256))):
+        assert np.array_equal(w.q, z[f"{name}_q"])
+        sc = oracle.scene_of(w.scene)
+        d, prof, ver, _ = oracle.distance(w.robot, sc, w.q, 32)
+        assert np.array_equal(d, z[f"{name}_d32"]) and np.array_equal(prof, z[f"{name}_profile32"])
+        assert np.array_equal(ver, z[f"{name}_verdict"])
+        assert np.array_equal(oracle.skeleton(w.robot, w.q, 32), z[f"{name}_skel32"])
+        assert np.array_equal(oracle.is_valid(w.robot, sc, w.q), z[f"{name}_valid"])
