@@ -25,7 +25,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librvs.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["rvs_kernels.hip", "rvs_capi.cpp"]
+SOURCES = ["rvs_kernels.hip", "rvs_edges.hip", "rvs_capi.cpp"]
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

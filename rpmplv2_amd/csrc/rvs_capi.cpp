@@ -626,14 +626,145 @@ rvs_status rvs_self_distance(rvs_context* c, int32_t robot, const float* q, int6
     return RVS_OK;
 }
 
-rvs_status rvs_edge_is_valid(rvs_context* c, int32_t, int32_t, const float*, const float*, int64_t, uint8_t*,
-                             uint32_t) {
-    return fail(c, RVS_ENOTSUP, "rvs_edge_is_valid: not available yet");
+rvs_status rvs_edge_is_valid(rvs_context* c, int32_t robot, int32_t env, const float* q1, const float* q2,
+                             int64_t n, uint8_t* valid, uint32_t flags) {
+    rvs_status s = check_query(c, robot, env, q1, n, true);
+    if (s != RVS_OK) return s;
+    if (n > 0 && (!q2 || !valid)) return fail(c, RVS_EINVAL, "rvs_edge_is_valid: null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_edge_is_valid: oriented-box environments not supported");
+    if (prec_of(c, flags) == RVS_PREC_FP64) return fail(c, RVS_ENOTSUP, "rvs_edge_is_valid: reference arithmetic only");
+    if (n == 0) return RVS_OK;
+    if (c->opt.num_checks <= 0) return fail(c, RVS_EINVAL, "rvs_edge_is_valid: num_checks must be positive");
+    (void)hipSetDevice(c->opt.device);
+    const int D = rb.num_dofs;
+    const QueryParams qp = params_of(c);
+    hipError_t e = hipSuccess;
+    const float *a = q1, *b = q2;
+    uint8_t* vd = valid;
+    if (!(flags & RVS_F_DEVICE)) {
+        e = c->arena.ensure(Carve::need({(size_t)n * D * 4, (size_t)n * D * 4, (size_t)n}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_is_valid: staging");
+        Carve cv(c->arena.p);
+        float* as = cv.take<float>((size_t)n * D);
+        float* bs = cv.take<float>((size_t)n * D);
+        vd = cv.take<uint8_t>(n);
+        e = h2d(as, q1, (size_t)n * D * 4, c->stream);
+        if (e == hipSuccess) e = h2d(bs, q2, (size_t)n * D * 4, c->stream);
+        a = as;
+        b = bs;
+    }
+    if (e == hipSuccess) e = launch_edge_valid(rb.dev, ev.dev, qp, a, b, n, vd, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(valid, vd, (size_t)n, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_is_valid");
+    return RVS_OK;
 }
 
-rvs_status rvs_edge_first_collision(rvs_context* c, int32_t, const int32_t*, const float*, const float*,
-                                    const int32_t*, int64_t, int32_t, float, int32_t*, uint32_t) {
-    return fail(c, RVS_ENOTSUP, "rvs_edge_first_collision: not available yet");
+rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* robots, const float* q1,
+                                    const float* dir, const int32_t* K, int64_t num_edges, int32_t row_stride,
+                                    float step, int32_t* first, uint32_t flags) {
+    if (!c) return RVS_EINVAL;
+    if (num_edges < 0) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: negative edge count");
+    if (num_edges > 0 && (!robots || !q1 || !dir || !K || !first))
+        return fail(c, RVS_EINVAL, "rvs_edge_first_collision: null argument");
+    if (num_edges > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: too many edges");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!env_ok(c, env)) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: bad env id");
+    const EnvRec& ev = c->envs[env];
+    if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_edge_first_collision: oriented-box environments not supported");
+    if (prec_of(c, flags) == RVS_PREC_FP64)
+        return fail(c, RVS_ENOTSUP, "rvs_edge_first_collision: reference arithmetic only");
+    if (num_edges == 0) return RVS_OK;
+    // group the edges by robot, longest edges first inside a group
+    std::vector<int32_t> gid;     // robot ids in order of first appearance
+    std::vector<std::vector<int32_t>> members;
+    for (int64_t e = 0; e < num_edges; ++e) {
+        const int32_t r = robots[e];
+        if (!robot_ok(c, r)) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: bad robot id");
+        if (K[e] < 0 || K[e] > (1 << 20)) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: state count");
+        auto it = std::find(gid.begin(), gid.end(), r);
+        if (it == gid.end()) {
+            if ((int)gid.size() == kMaxEdgeGroups)
+                return fail(c, RVS_ERANGE, "rvs_edge_first_collision: at most 8 robots per call");
+            gid.push_back(r);
+            members.emplace_back();
+            it = gid.end() - 1;
+        }
+        members[it - gid.begin()].push_back((int32_t)e);
+    }
+    EdgeGroups gr{};
+    gr.num = (int32_t)gid.size();
+    std::vector<int32_t> order, rounds;
+    int64_t items = 0;
+    for (int g = 0; g < gr.num; ++g) {
+        const RobotRec& rb = c->robots[gid[g]];
+        if (row_stride < rb.num_dofs) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: row_stride < dofs");
+        gr.rb[g] = rb.dev;
+        gr.max_dofs = std::max(gr.max_dofs, rb.num_dofs);
+        gr.max_links = std::max(gr.max_links, rb.num_links);
+        auto& m = members[g];
+        std::stable_sort(m.begin(), m.end(), [&](int32_t x, int32_t y) { return K[x] > K[y]; });
+        gr.order_off[g] = (int32_t)order.size();
+        order.insert(order.end(), m.begin(), m.end());
+        const int32_t kmax = m.empty() ? 0 : K[m[0]];
+        gr.round_off[g] = (int32_t)rounds.size();
+        gr.num_rounds[g] = kmax;
+        // rounds[k] = slots before round k; edges alive in round k are a prefix
+        int64_t acc = 0;
+        size_t alive = m.size();
+        for (int32_t k = 0; k <= kmax; ++k) {
+            rounds.push_back((int32_t)acc);
+            while (alive > 0 && K[m[alive - 1]] <= k) --alive;
+            acc += (int64_t)alive;
+        }
+        if (acc > INT32_MAX / 2) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: too many states");
+        gr.num_slots[g] = (int32_t)acc;
+        gr.item_begin[g] = (int32_t)items;
+        items += (acc + 63) / 64;
+    }
+    gr.item_begin[gr.num] = (int32_t)items;
+    const size_t lds = edge_first_lds_bytes(gr, ev.dev);
+    if (lds > 64 * 1024) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: robots too large for LDS staging");
+    (void)hipSetDevice(c->opt.device);
+    const QueryParams qp = params_of(c);
+    const size_t rows = (size_t)num_edges * row_stride;
+    hipError_t e = c->arena.ensure(Carve::need({order.size() * 4, rounds.size() * 4, 256,
+                                                (flags & RVS_F_DEVICE) ? 0 : rows * 4,
+                                                (flags & RVS_F_DEVICE) ? 0 : rows * 4,
+                                                (flags & RVS_F_DEVICE) ? 0 : (size_t)num_edges * 4}));
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_first_collision: staging");
+    Carve cv(c->arena.p);
+    int32_t* od = cv.take<int32_t>(order.size());
+    int32_t* rd = cv.take<int32_t>(rounds.size());
+    uint32_t* counter = cv.take<uint32_t>(64);
+    const float *qa = q1, *qd = dir;
+    int32_t* fd = first;
+    e = h2d(od, order.data(), order.size() * 4, c->stream);
+    if (e == hipSuccess) e = h2d(rd, rounds.data(), rounds.size() * 4, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
+        float* as = cv.take<float>(rows);
+        float* ds = cv.take<float>(rows);
+        fd = cv.take<int32_t>(num_edges);
+        e = h2d(as, q1, rows * 4, c->stream);
+        if (e == hipSuccess) e = h2d(ds, dir, rows * 4, c->stream);
+        qa = as;
+        qd = ds;
+    }
+    if (e == hipSuccess) {
+        int dev_cus = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->opt.device) == hipSuccess) dev_cus = prop.multiProcessorCount;
+        const int waves = dev_cus * 16;
+        e = launch_edge_first(gr, ev.dev, qp, qa, qd, row_stride, step, od, rd, fd, num_edges, counter,
+                              (int32_t)items, waves, c->stream);
+    }
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(first, fd, (size_t)num_edges * 4, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_first_collision");
+    return RVS_OK;
 }
 
 }  // extern "C"

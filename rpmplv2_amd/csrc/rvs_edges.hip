@@ -1,0 +1,227 @@
+// Edge checking kernels (gfx950).
+//
+// k_edge_valid: RealVectorSpace::isValid(q1, q2) (RealVectorSpace.cpp:
+// 261-275) for a batch of edges.  The reference walks k = K..1 and stops at
+// the first invalid state; the edge is valid exactly when all K states are,
+// so here each lane takes one (edge, k) state and a collision clears the
+// edge's flag (plain stores of 0; the flags start at 1).
+//
+// k_edge_first: discretised edges of several robots in ONE persistent launch.
+// The states of each robot's edges are laid out round-major (state 0 of every
+// edge, then state 1 of every edge that has one, ...) after sorting the edges
+// by length, so consecutive slots share a robot and a state index and early
+// states are visited first.  Each wave repeatedly takes the next 64-slot item
+// from a global counter until the counter passes the last item, skips states
+// that lie beyond an already found collision of their edge, and records a
+// collision with atomicMin on the edge's first index.  Robot ranges are padded
+// to whole items, so the robot is uniform across a wave.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "rvs_fk.hpp"
+#include "rvs_geometry.hpp"
+#include "rvs_internal.hpp"
+#include "rvs_stage.hpp"
+
+#pragma clang fp contract(off)
+
+namespace rvs {
+
+namespace {
+
+// Norm of a dynamic-size float vector in the order Eigen's SSE reduction uses
+// for VectorXf (RealVectorSpace::getNorm, RealVectorSpace.cpp:49-52): packets
+// of four accumulated pairwise, reduced as (p0 + p2) + (p1 + p3), then the
+// remaining coefficients left to right; sizes below four go left to right.
+__device__ inline float vecxf_norm_diff(const float* a, const float* b, int n) {
+    auto sq = [&](int i) { const float d = a[i] - b[i]; return d * d; };
+    if (n < 4) {
+        float r = sq(0);
+        for (int i = 1; i < n; ++i) r = r + sq(i);
+        return sqrtf(r);
+    }
+    const int aligned = (n / 4) * 4, aligned2 = (n / 8) * 8;
+    float p0 = sq(0), p1 = sq(1), p2 = sq(2), p3 = sq(3);
+    if (aligned > 4) {
+        float s0 = sq(4), s1 = sq(5), s2 = sq(6), s3 = sq(7);
+        for (int idx = 8; idx < aligned2; idx += 8) {
+            p0 = p0 + sq(idx); p1 = p1 + sq(idx + 1); p2 = p2 + sq(idx + 2); p3 = p3 + sq(idx + 3);
+            s0 = s0 + sq(idx + 4); s1 = s1 + sq(idx + 5); s2 = s2 + sq(idx + 6); s3 = s3 + sq(idx + 7);
+        }
+        p0 = p0 + s0; p1 = p1 + s1; p2 = p2 + s2; p3 = p3 + s3;
+        if (aligned > aligned2) {
+            p0 = p0 + sq(aligned2); p1 = p1 + sq(aligned2 + 1); p2 = p2 + sq(aligned2 + 2); p3 = p3 + sq(aligned2 + 3);
+        }
+    }
+    float r = (p0 + p2) + (p1 + p3);
+    for (int i = aligned; i < n; ++i) r = r + sq(i);
+    return sqrtf(r);
+}
+
+// Collision tree against every obstacle (RealVectorSpace::isValid(q)) on a
+// configuration held in LDS column 'lane' (joint k at qcol[k * stride]).
+// Skeleton points are written to skel (point k, coordinate a at
+// skel[(3k + a) * stride]) when skel is not null.
+__device__ inline bool env_free(const RobotDev& rb, const Staged<float>& s, int M, const Params<float>& p,
+                                const float* qcol, int stride, float* skel) {
+    FKStream fk;
+    fk_init(fk);
+    Vec3<float> P0 = skel_point<float>(fk, rb, qcol, 0, stride);
+    if (skel) { skel[0] = P0.x; skel[stride] = P0.y; skel[2 * stride] = P0.z; }
+    for (int i = 0; i < rb.num_links; ++i) {
+        const Vec3<float> P1 = skel_point<float>(fk, rb, qcol, i + 1, stride);
+        if (skel) {
+            skel[(3 * i + 3) * stride] = P1.x;
+            skel[(3 * i + 4) * stride] = P1.y;
+            skel[(3 * i + 5) * stride] = P1.z;
+        }
+        const float r = rb.radius[i];
+        const float rn = rb.rnew32[i];
+        for (int j = 0; j < M; ++j) {
+            if (s.gnd[j] && i < rb.ground_included) continue;
+            const Box<float> bx = s.box[j];
+            if (collision_box(P0, P1, r, rn, bx, p)) return false;
+        }
+        P0 = P1;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_edge_valid(RobotDev rb, EnvDev env, QueryParams qp,
+                                                    const float* __restrict__ q1, const float* __restrict__ q2,
+                                                    int64_t n, uint8_t* valid) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<float> s = stage_env<float>(env, smem);
+    float* qbuf = reinterpret_cast<float*>(smem + ((staged_bytes<float>(env.num_obj) + 15) & ~size_t(15)));
+    const int K = qp.num_checks;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * K) return;
+    const int64_t e = g / K;
+    const int kk = K - (int)(g % K);          // k = K .. 1
+    const int D = rb.num_dofs;
+    const float* a = q1 + e * D;
+    const float* b = q2 + e * D;
+    const float dist = vecxf_norm_diff(a, b, D);
+    // RealVectorSpace::interpolateEdge (RealVectorSpace.cpp:75-88) with
+    // step = k / num_checks * dist
+    const float step = (float)kk / (float)K * dist;
+    const int B = blockDim.x, t = threadIdx.x;
+    if (dist > 0) {
+        const float f = step / dist;
+        for (int i = 0; i < D; ++i) qbuf[i * B + t] = a[i] + (b[i] - a[i]) * f;
+    } else {
+        for (int i = 0; i < D; ++i) qbuf[i * B + t] = b[i];
+    }
+    const Params<float> p{qp.eq, qp.maxd};
+    if (!env_free(rb, s, env.num_obj, p, qbuf + t, B, nullptr)) valid[e] = 0;
+}
+
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- first collision
+__global__ void k_first_init(int32_t* first, int64_t n, uint32_t* counter) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) first[i] = INT32_MAX;
+    if (i == 0) *counter = 0;
+}
+
+__global__ void k_first_final(int32_t* first, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && first[i] == INT32_MAX) first[i] = -1;
+}
+
+// Binary search: largest k with off[k] <= s (off is non-decreasing, off[0] = 0).
+__device__ inline int round_of(const int32_t* off, int nrounds, int32_t s) {
+    int lo = 0, hi = nrounds - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= s) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(64) void k_edge_first(EdgeGroups gr, EnvDev env, QueryParams qp,
+                                                   const float* __restrict__ q1, const float* __restrict__ dir,
+                                                   int32_t row_stride, float step, const int32_t* __restrict__ order,
+                                                   const int32_t* __restrict__ rounds, int32_t* first,
+                                                   uint32_t* counter, int32_t total_items) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<float> s = stage_env<float>(env, smem);
+    const size_t base = (staged_bytes<float>(env.num_obj) + 15) & ~size_t(15);
+    float* qbuf = reinterpret_cast<float*>(smem + base);
+    float* skel = qbuf + gr.max_dofs * 64;
+    const int lane = threadIdx.x;
+    const Params<float> p{qp.eq, qp.maxd};
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = (int)atomicAdd(counter, 1u);
+        item = __shfl(item, 0, 64);
+        if (item >= total_items) break;
+        // robot group of this item (uniform across the wave)
+        int g = 0;
+        while (g + 1 < gr.num && item >= gr.item_begin[g + 1]) ++g;
+        const RobotDev& rb = gr.rb[g];
+        const int32_t slot = (item - gr.item_begin[g]) * 64 + lane;
+        if (slot >= gr.num_slots[g]) continue;
+        const int k = round_of(rounds + gr.round_off[g], gr.num_rounds[g] + 1, slot);
+        const int32_t e = order[gr.order_off[g] + (slot - rounds[gr.round_off[g] + k])];
+        if (__atomic_load_n(&first[e], __ATOMIC_RELAXED) <= k) continue;
+        const int D = rb.num_dofs;
+        const float sk = step * (float)k;
+        for (int i = 0; i < D; ++i)
+            qbuf[i * 64 + lane] = q1[(int64_t)e * row_stride + i] + sk * dir[(int64_t)e * row_stride + i];
+        bool bad = !env_free(rb, s, env.num_obj, p, qbuf + lane, 64, rb.num_pairs ? skel + lane : nullptr);
+        for (int m = 0; !bad && m < rb.num_pairs; ++m) {
+            const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
+            auto pt = [&](int kk) {
+                return Vec3<float>{skel[(3 * kk) * 64 + lane], skel[(3 * kk + 1) * 64 + lane],
+                                   skel[(3 * kk + 2) * 64 + lane]};
+            };
+            const float sd = seg_seg<float, false>(pt(a), pt(a + 1), pt(b), pt(b + 1), p, nullptr);
+            if (sd - rb.radius[a] - rb.radius[b] < 0) bad = true;
+        }
+        if (bad) atomicMin(&first[e], k);
+    }
+}
+
+inline dim3 blocks_for(int64_t n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
+
+}  // namespace
+
+hipError_t launch_edge_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q1,
+                             const float* q2, int64_t n, uint8_t* valid, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u8, blocks_for(n, 256), dim3(256), 0, st, valid, n, (uint8_t)1);
+    const int B = 256;
+    const size_t sm = ((staged_bytes<float>(env.num_obj) + 15) & ~size_t(15)) + (size_t)rb.num_dofs * B * 4;
+    hipLaunchKernelGGL(k_edge_valid, blocks_for(n * qp.num_checks, B), dim3(B), sm, st, rb, env, qp, q1, q2, n, valid);
+    return hipGetLastError();
+}
+
+size_t edge_first_lds_bytes(const EdgeGroups& gr, const EnvDev& env) {
+    return ((staged_bytes<float>(env.num_obj) + 15) & ~size_t(15)) +
+           (size_t)64 * 4 * (gr.max_dofs + 3 * (gr.max_links + 1));
+}
+
+hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const QueryParams& qp, const float* q1,
+                             const float* dir, int32_t row_stride, float step, const int32_t* order,
+                             const int32_t* rounds, int32_t* first, int64_t num_edges, uint32_t* counter,
+                             int32_t total_items, int waves, hipStream_t st) {
+    if (num_edges <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_first_init, blocks_for(num_edges, 256), dim3(256), 0, st, first, num_edges, counter);
+    if (total_items > 0) {
+        const int grid = std::max(1, std::min(total_items, waves));
+        hipLaunchKernelGGL(k_edge_first, dim3(grid), dim3(64), edge_first_lds_bytes(gr, env), st, gr, env, qp, q1, dir,
+                           row_stride, step, order, rounds, first, counter, total_items);
+    }
+    hipLaunchKernelGGL(k_first_final, blocks_for(num_edges, 256), dim3(256), 0, st, first, num_edges);
+    return hipGetLastError();
+}
+
+}  // namespace rvs

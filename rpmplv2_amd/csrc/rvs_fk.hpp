@@ -113,16 +113,17 @@ __host__ __device__ __attribute__((always_inline)) inline void fk_init(FKStream&
     s.qi = 0;
 }
 
-// Apply segments until frame 'upto' is the current frame.
+// Apply segments until frame 'upto' is the current frame.  Joint k of the
+// configuration is q[k * qstride].
 __host__ __device__ __attribute__((always_inline)) inline void fk_advance(FKStream& s, const SegDev* seg,
-                                                                        const float* q, int upto) {
+                                                                        const float* q, int upto, int qstride = 1) {
     while (s.seg <= upto) {
         const SegDev& g = seg[s.seg];
         const Mat3 R = mat_load(g.R);
         Mat3 SM;
         if (g.rev) {
             double st, ct;
-            sincos((double)q[s.qi++], &st, &ct);
+            sincos((double)q[(s.qi++) * qstride], &st, &ct);
             SM = mat_mul(rot2(g.axis, st, ct), R);
         } else {
             SM = R;
@@ -143,10 +144,10 @@ __host__ __device__ __attribute__((always_inline)) inline void fk_advance(FKStre
 // Skeleton point k (points must be requested in increasing k).
 template <typename T>
 __host__ __device__ __attribute__((always_inline)) inline Vec3<T> skel_point(FKStream& s, const RobotDev& rb,
-                                                                           const float* q, int k) {
+                                                                           const float* q, int k, int qstride = 1) {
     const PointDev pt = rb.pts[k];
     if (pt.frame < 0) return Vec3<T>{T(0), T(0), T(0)};
-    fk_advance(s, rb.seg, q, pt.frame);
+    fk_advance(s, rb.seg, q, pt.frame, qstride);
     double x = s.px, y = s.py, z = s.pz;
     // column 'axis' of the frame, picked with one-hot weights (exact: the
     // other two terms are signed zeros) rather than an indexed load, which

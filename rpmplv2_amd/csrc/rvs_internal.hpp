@@ -41,6 +41,32 @@ struct DistOut {
 // Largest obstacle count staged whole in LDS by the distance kernels.
 constexpr int kMaxLdsObstacles = 1024;
 
+// Edges of several robots for the persistent first-collision kernel.  Group g
+// owns items [item_begin[g], item_begin[g+1]) of 64 slots each; slot s of the
+// group is state k of edge order[order_off[g] + s - rounds[round_off[g] + k]]
+// where k is the largest round with rounds[round_off[g] + k] <= s.
+constexpr int kMaxEdgeGroups = 8;
+struct EdgeGroups {
+    RobotDev rb[kMaxEdgeGroups];
+    int32_t num;
+    int32_t max_dofs;
+    int32_t max_links;
+    int32_t pad;
+    int32_t item_begin[kMaxEdgeGroups + 1];
+    int32_t num_slots[kMaxEdgeGroups];
+    int32_t num_rounds[kMaxEdgeGroups];
+    int32_t round_off[kMaxEdgeGroups];
+    int32_t order_off[kMaxEdgeGroups];
+};
+
+hipError_t launch_edge_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q1,
+                             const float* q2, int64_t n, uint8_t* valid, hipStream_t st);
+size_t edge_first_lds_bytes(const EdgeGroups& gr, const EnvDev& env);
+hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const QueryParams& qp, const float* q1,
+                             const float* dir, int32_t row_stride, float step, const int32_t* order,
+                             const int32_t* rounds, int32_t* first, int64_t num_edges, uint32_t* counter,
+                             int32_t total_items, int waves, hipStream_t st);
+
 hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                  int64_t n, const DistOut& out, hipStream_t st);
 hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,

@@ -269,3 +269,46 @@ class Engine:
         fl = F_DEVICE | (F_PREC(precision) if precision is not None else 0)
         self._check(lib().rvs_distance(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d), _ptr(profile),
                                        _ptr(nearest), _ptr(verdict), fl), "rvs_distance")
+
+    # ---- edges
+    def edge_is_valid(self, robot: int, env: int, q1, q2):
+        rb = self.robots[robot]
+        q1 = np.ascontiguousarray(q1, np.float32).reshape(-1, rb.num_dofs)
+        q2 = np.ascontiguousarray(q2, np.float32).reshape(-1, rb.num_dofs)
+        out = np.zeros(len(q1), np.uint8)
+        self._check(lib().rvs_edge_is_valid(self._ctx, robot, env, _ptr(q1), _ptr(q2), len(q1), _ptr(out), 0),
+                    "rvs_edge_is_valid")
+        return out
+
+    def edge_first_collision(self, env: int, robot_ids, q1, direction, K, step: float = 0.05):
+        """Edges of several robots in one launch; q1/direction rows padded to
+        the largest joint count.  Returns the first colliding state per edge."""
+        robot_ids = np.ascontiguousarray(robot_ids, np.int32)
+        q1 = np.ascontiguousarray(q1, np.float32)
+        direction = np.ascontiguousarray(direction, np.float32)
+        K = np.ascontiguousarray(K, np.int32)
+        n = len(robot_ids)
+        out = np.zeros(n, np.int32)
+        self._check(lib().rvs_edge_first_collision(self._ctx, env, _ptr(robot_ids), _ptr(q1), _ptr(direction), _ptr(K),
+                                                   n, q1.shape[1], C.c_float(step), _ptr(out), 0),
+                    "rvs_edge_first_collision")
+        return out
+
+    def edge_batches(self, env: int, batches):
+        """Workload-3 helper: stack EdgeBatch objects (one robot each) into a
+        single first-collision call; returns one array per batch."""
+        ids, rows_q, rows_u, ks = [], [], [], []
+        width = max(b.robot.num_dofs for b in batches)
+        for b in batches:
+            rid = self.load_robot(b.robot)
+            n = len(b.q1)
+            pq = np.zeros((n, width), np.float32); pq[:, :b.robot.num_dofs] = b.q1
+            pu = np.zeros((n, width), np.float32); pu[:, :b.robot.num_dofs] = b.dir
+            ids.append(np.full(n, rid, np.int32)); rows_q.append(pq); rows_u.append(pu); ks.append(b.K)
+        first = self.edge_first_collision(env, np.concatenate(ids), np.concatenate(rows_q), np.concatenate(rows_u),
+                                          np.concatenate(ks), batches[0].step)
+        out, o = [], 0
+        for b in batches:
+            out.append(first[o:o + len(b.q1)])
+            o += len(b.q1)
+        return out

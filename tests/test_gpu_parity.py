@@ -162,3 +162,44 @@ def test_nan_configurations(oracle, engine):
     assert ulps(d[m], od[m]).max() <= 2
     v = engine.is_valid(rid, eid, q)
     assert (v == oracle.is_valid(w.robot, oracle.scene_of(w.scene), q)).all()
+
+
+def test_edge_is_valid(oracle, engine):
+    w = W.workload2(8192)
+    rid = engine.load_robot(w.robot)
+    eid = engine.create_env(w.scene)
+    q1, q2 = w.q[:4096], w.q[4096:]
+    # short edges too, so both outcomes occur
+    q2s = (q1 + 0.05 * (q2 - q1)).astype(np.float32)
+    for b in (q2, q2s, q1):
+        v = engine.edge_is_valid(rid, eid, q1, b)
+        ov = oracle.edge_valid(w.robot, oracle.scene_of(w.scene), q1, b)
+        assert (v == ov).all()
+    assert 0 < engine.edge_is_valid(rid, eid, q1, q2s).mean() < 1
+
+
+def test_edge_first_collision_workload3(oracle, engine):
+    w3 = W.workload3(256)
+    eid = engine.create_env(w3.scene)
+    firsts = engine.edge_batches(eid, w3.batches)
+    sc = oracle.scene_of(w3.scene)
+    for b, f in zip(w3.batches, firsts):
+        of = oracle.edge_first_collision(b.robot, sc, b.q1, b.dir, b.K, b.step)
+        print(b.robot.name, "collide frac", (of >= 0).mean(), "mismatch", (f != of).sum())
+        assert np.array_equal(f, of)
+
+
+def test_edge_first_collision_sparse_scene(oracle, engine):
+    # few, small obstacles so many edges run to their last state
+    rng = np.random.default_rng(11)
+    sc = W._boxes(rng, 8, 0.05, 0.1, 1.5)
+    eid = engine.create_env(sc)
+    rb = R.chain(rng.uniform(0.05, 0.1, 10), radius=0.01, self_collision=True)
+    n = 512
+    q1 = rng.uniform(-0.3, 0.3, (n, 10)).astype(np.float32)
+    u = rng.normal(size=(n, 10)); u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+    K = rng.integers(1, 200, n).astype(np.int32)
+    b = W.EdgeBatch(rb, q1, u, K)
+    f = engine.edge_batches(eid, [b])[0]
+    of = oracle.edge_first_collision(rb, oracle.scene_of(sc), q1, u, K, 0.05)
+    assert np.array_equal(f, of) and (of == -1).mean() > 0.1
