@@ -4,10 +4,12 @@
 
 Compile flags:
   --offload-arch=gfx950                    MI355X only
-  -ffp-contract=off                        no FMA contraction anywhere, so the
-                                           float kernels keep the reference's
-                                           rounding (the sources also carry
-                                           '#pragma clang fp contract(off)')
+  -ffp-contract=off                        no FMA contraction, so the exact
+                                           kernels keep the reference's rounding
+                                           (those sources also carry '#pragma
+                                           clang fp contract(off)'); only the
+                                           FP32 bulk pass (rvs_fast.hip) is
+                                           built with -ffp-contract=fast
   -target-feature -packed-fp32-ops         no packed FP32 instructions
 Division and square root stay IEEE (hipcc's default
 -fhip-fp32-correctly-rounded-divide-sqrt), denormals are kept.
@@ -25,7 +27,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librvs.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["rvs_kernels.hip", "rvs_edges.hip", "rvs_capi.cpp"]
+SOURCES = ["rvs_kernels.hip", "rvs_edges.hip", "rvs_fast.hip", "rvs_capi.cpp"]
+# the FP32 bulk pass is the one translation unit that allows FMA contraction
+EXTRA = {"rvs_fast.hip": ["-ffp-contract=fast"]}
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -44,7 +48,7 @@ def build(verbose: bool = False) -> str:
     objs = []
     for src in SOURCES:
         obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + COMMON + EXTRA.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd[1:1] = ["-x", "hip"]
         if verbose:

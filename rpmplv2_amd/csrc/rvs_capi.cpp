@@ -437,23 +437,36 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
     if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_distance: oriented-box environments need rvs_distance_obb");
-    const int prec = prec_of(c, flags);
-    if (prec == RVS_PREC_FP32_REFINE) return fail(c, RVS_ENOTSUP, "rvs_distance: FP32_REFINE not available yet");
+    int prec = prec_of(c, flags);
+    // nearest points come from the reference-exact pass
+    if (prec == RVS_PREC_FP32_REFINE && nearest) prec = RVS_PREC_REF32;
     if (n == 0) return RVS_OK;
+    if (prec == RVS_PREC_FP32_REFINE && n > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_distance: batch too large");
+    if (prec == RVS_PREC_FP32_REFINE && refine_lds_bytes(rb.dev, ev.dev) > 64 * 1024) prec = RVS_PREC_REF32;
     (void)hipSetDevice(c->opt.device);
     const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
     const QueryParams qp = params_of(c);
     DistOut out{};
     const float* qd = q;
     hipError_t e = hipSuccess;
+    // refinement list (FP32_REFINE) lives at the start of the arena
+    const size_t refine_bytes = (prec == RVS_PREC_FP32_REFINE) ? (size_t)n * 4 + 256 : 0;
+    int32_t* ridx = nullptr;
+    uint32_t* rcount = nullptr;
     if (flags & RVS_F_DEVICE) {
+        e = c->arena.ensure(Carve::need({refine_bytes}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_distance: staging");
+        Carve cv(c->arena.p);
+        if (refine_bytes) { rcount = cv.take<uint32_t>(64); ridx = cv.take<int32_t>(n); }
         out.d = d; out.profile = profile; out.nearest = nearest; out.verdict = verdict;
     } else {
-        const size_t need = Carve::need({(size_t)n * D * 4, (size_t)n * 4, profile ? (size_t)n * L * 4 : 0,
-                                         nearest ? (size_t)n * M * L * 24 : 0, (size_t)n});
+        const size_t need = Carve::need({refine_bytes, (size_t)n * D * 4, (size_t)n * 4,
+                                         profile ? (size_t)n * L * 4 : 0, nearest ? (size_t)n * M * L * 24 : 0,
+                                         (size_t)n});
         e = c->arena.ensure(need);
         if (e != hipSuccess) return hip_fail(c, e, "rvs_distance: staging");
         Carve cv(c->arena.p);
+        if (refine_bytes) { rcount = cv.take<uint32_t>(64); ridx = cv.take<int32_t>(n); }
         float* qs = cv.take<float>((size_t)n * D);
         out.d = cv.take<float>(n);
         out.profile = profile ? cv.take<float>((size_t)n * L) : nullptr;
@@ -462,9 +475,18 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
         e = h2d(qs, q, (size_t)n * D * 4, c->stream);
         qd = qs;
     }
+    uint32_t refined = 0;
     if (e == hipSuccess) {
-        e = (prec == RVS_PREC_FP64) ? launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream)
-                                    : launch_distance_ref32(rb.dev, ev.dev, qp, qd, n, out, c->stream);
+        if (prec == RVS_PREC_FP32_REFINE) {
+            e = launch_zero_u32(rcount, 1, c->stream);
+            if (e == hipSuccess) e = launch_distance_fast(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, c->stream);
+            if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, c->stream);
+            if (e == hipSuccess) e = d2h(&refined, rcount, 4, c->stream);
+        } else if (prec == RVS_PREC_FP64) {
+            e = launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream);
+        } else {
+            e = launch_distance_ref32(rb.dev, ev.dev, qp, qd, n, out, c->stream);
+        }
     }
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
         e = d2h(d, out.d, (size_t)n * 4, c->stream);
@@ -474,6 +496,7 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_distance");
+    c->last_refined = (prec == RVS_PREC_FP32_REFINE) ? (int64_t)refined : 0;
     return RVS_OK;
 }
 

@@ -15,6 +15,10 @@
 // that lie beyond an already found collision of their edge, and records a
 // collision with atomicMin on the edge's first index.  Robot ranges are padded
 // to whole items, so the robot is uniform across a wave.
+// No FMA contraction anywhere in this translation unit, headers included:
+// the float kernels must keep the reference's rounding.
+#pragma clang fp contract(off)
+
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -26,7 +30,6 @@
 #include "rvs_internal.hpp"
 #include "rvs_stage.hpp"
 
-#pragma clang fp contract(off)
 
 namespace rvs {
 

@@ -28,8 +28,6 @@
 
 #include "rvs_geometry.hpp"
 
-#pragma clang fp contract(off)
-
 namespace rvs {
 
 constexpr int kMaxSeg = 72;

@@ -11,16 +11,16 @@
 //
 // Every arithmetic expression keeps the reference's operation order: 3-vector
 // dot products sum as e0 + (e1 + e2), 2-vectors as e0 + e1, lerps are
-// a + t * (b - a) per component, and this file is compiled without FMA
-// contraction with IEEE division and square root.  With T = float the results
-// are bit-identical to the reference's float arithmetic; with T = double the
-// same algorithm runs in FP64.
+// a + t * (b - a) per component.  Division and square root go through a math
+// policy: ExactMath (IEEE division and square root) in translation units
+// compiled without FMA contraction gives results bit-identical to the
+// reference's float arithmetic (T = float) or the same algorithm in FP64
+// (T = double); FastMath (hardware reciprocal and square root, contraction
+// allowed by its translation unit) is the relaxed FP32 bulk pass.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-
-#pragma clang fp contract(off)
 
 namespace rvs {
 
@@ -52,7 +52,19 @@ template <typename T> __host__ __device__ inline T sqnorm(const Vec3<T>& a) { re
 // IEEE square root of the operand's own precision
 __host__ __device__ inline float vsqrt(float x) { return sqrtf(x); }
 __host__ __device__ inline double vsqrt(double x) { return sqrt(x); }
-template <typename T> __host__ __device__ inline T vnorm(const Vec3<T>& a) { return vsqrt(sqnorm(a)); }
+struct ExactMath {
+    template <typename T> __host__ __device__ static T div(T a, T b) { return a / b; }
+    template <typename T> __host__ __device__ static T root(T x) { return vsqrt(x); }
+};
+
+// Relaxed FP32: v_rcp_f32 / v_sqrt_f32 (about 1 ulp), device only.
+struct FastMath {
+    __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static float root(float x) { return __builtin_amdgcn_sqrtf(x); }
+};
+
+template <typename T, class MP = ExactMath>
+__host__ __device__ inline T vnorm(const Vec3<T>& a) { return MP::root(sqnorm(a)); }
 // a + s * (b - a)
 template <typename T> __host__ __device__ inline Vec3<T> lerp(const Vec3<T>& a, const Vec3<T>& b, T s) {
     return Vec3<T>{a.x + s * (b.x - a.x), a.y + s * (b.y - a.y), a.z + s * (b.z - a.z)};
@@ -91,7 +103,7 @@ struct NP {
 // distanceLineSegToLineSeg, CollisionAndDistance.cpp:217-320.  Returns the
 // distance of AB to CD; np receives (point on AB, point on CD), invalid when
 // the reference returns no nearest points.
-template <typename T, bool WANT_NP>
+template <typename T, bool WANT_NP, class MP = ExactMath>
 __host__ __device__ inline T seg_seg(const Vec3<T>& A, const Vec3<T>& B, const Vec3<T>& C, const Vec3<T>& D,
                                      const Params<T>& p, NP<T>* np) {
     const Vec3<T> BA = B - A, DC = D - C, CD = C - D, AC = A - C, AB = A - B;
@@ -101,12 +113,12 @@ __host__ __device__ inline T seg_seg(const Vec3<T>& A, const Vec3<T>& B, const V
     const T beta2 = dot(CD, DC);
     const T gamma1 = dot(AC, AB);
     const T gamma2 = dot(AC, CD);
-    const T s = (alpha1 * gamma2 - alpha2 * gamma1) / (alpha1 * beta2 - alpha2 * beta1);
-    const T t = (gamma1 - beta1 * s) / alpha1;
+    const T s = MP::div(alpha1 * gamma2 - alpha2 * gamma1, alpha1 * beta2 - alpha2 * beta1);
+    const T t = MP::div(gamma1 - beta1 * s, alpha1);
     T d_c = p.maxd;
     if (t > 0 && t < 1 && s > 0 && s < 1) {
         const Vec3<T> P1 = lerp(A, B, t), P2 = lerp(C, D, s);
-        d_c = vnorm(P2 - P1);
+        d_c = vnorm<T, MP>(P2 - P1);
         if (d_c < p.eq) {
             if (WANT_NP) np->valid = false;
             return T(0);
@@ -117,10 +129,10 @@ __host__ __device__ inline T seg_seg(const Vec3<T>& A, const Vec3<T>& B, const V
     // the four clamped candidates: C and D projected on AB, A and B on CD
     const T alpha3 = sqnorm(CD);
     const Vec3<T> AD = A - D, BC = B - C;
-    const T opt0 = dot(AC, AB) / alpha1;
-    const T opt1 = dot(AD, AB) / alpha1;
-    const T opt2 = dot(AC, DC) / alpha3;
-    const T opt3 = dot(BC, DC) / alpha3;
+    const T opt0 = MP::div(dot(AC, AB), alpha1);
+    const T opt1 = MP::div(dot(AD, AB), alpha1);
+    const T opt2 = MP::div(dot(AC, DC), alpha3);
+    const T opt3 = MP::div(dot(BC, DC), alpha3);
     if (WANT_NP) np->valid = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -140,7 +152,7 @@ __host__ __device__ inline T seg_seg(const Vec3<T>& A, const Vec3<T>& B, const V
             p0 = (i == 2) ? A : B;
             p1 = lerp(C, D, o);
         }
-        const T d_tmp = vnorm(p1 - p0);
+        const T d_tmp = vnorm<T, MP>(p1 - p0);
         if (d_tmp < d_c) {
             d_c = d_tmp;
             if (WANT_NP) { np->r = p0; np->o = p1; np->valid = true; }
@@ -151,23 +163,23 @@ __host__ __device__ inline T seg_seg(const Vec3<T>& A, const Vec3<T>& B, const V
 
 // Squared sine of the angle between AB and CD, from the same quantities the
 // reference divides by; small values flag an ill-conditioned interior solve.
-template <typename T>
+template <typename T, class MP = ExactMath>
 __host__ __device__ inline T seg_seg_sin2(const Vec3<T>& A, const Vec3<T>& B, const Vec3<T>& C, const Vec3<T>& D) {
     const Vec3<T> u = B - A, v = D - C;
     const T uu = sqnorm(u), vv = sqnorm(v), uv = dot(u, v);
     const T den = uu * vv;
-    return den > 0 ? (den - uv * uv) / den : T(1);
+    return den > 0 ? MP::div(den - uv * uv, den) : T(1);
 }
 
 // ---------------------------------------------------------------- seg-pt
 // distanceLineSegToPoint, CollisionAndDistance.cpp:323-342
-template <typename T, bool WANT_NP>
+template <typename T, bool WANT_NP, class MP = ExactMath>
 __host__ __device__ inline T seg_pt(const Vec3<T>& A, const Vec3<T>& B, const Vec3<T>& C, const Params<T>& p,
                                     NP<T>* np) {
     const Vec3<T> BA = B - A;
-    const T t_opt = dot(C - A, BA) / sqnorm(BA);
+    const T t_opt = MP::div(dot(C - A, BA), sqnorm(BA));
     const Vec3<T> Pn = (t_opt < 0) ? A : ((t_opt > 1) ? B : lerp(A, B, t_opt));
-    const T d_c = vnorm(C - Pn);
+    const T d_c = vnorm<T, MP>(C - Pn);
     if (d_c < p.eq) {
         if (WANT_NP) np->valid = false;
         return T(0);
@@ -181,14 +193,14 @@ __host__ __device__ inline T seg_pt(const Vec3<T>& A, const Vec3<T>& B, const Ve
 // zmin, xmax, ymax, zmax).  This is collisionCapsuleToRectangle
 // (CollisionAndDistance.cpp:46-111) at radius 0, where only the plane-crossing
 // test of lines 57-65 can report a hit.
-template <typename T>
+template <typename T, class MP = ExactMath>
 __host__ __device__ inline bool face_crossed(const Vec3<T>& A, const Vec3<T>& B, const Box<T>& b, int face,
                                              const Params<T>& p) {
     const int c = face % 3;
     const int o1 = (c == 0) ? 1 : 0, o2 = (c == 2) ? 1 : 2;
     const T v = b.f(face);
     const T den = B[c] - A[c];
-    const T t = (den != 0) ? (v - A[c]) / den : p.maxd;
+    const T t = (den != 0) ? MP::div(v - A[c], den) : p.maxd;
     if (!(t > 0 && t < 1)) return false;
     const T m0 = A[o1] + t * (B[o1] - A[o1]);
     const T m1 = A[o2] + t * (B[o2] - A[o2]);
@@ -260,9 +272,40 @@ __host__ __device__ inline int face_side_edges(const Box<T>& b, int face, const 
 // dereference a missing nearest-point matrix (end point projections more than
 // MAX_DISTANCE away, or an end point exactly on a face plane with the other
 // inside the face's span); the distance is then NaN.
-template <typename T, bool WANT_NP>
+// Side information for the FP32 bulk pass's refinement trigger.
+template <typename T>
+struct PairInfo {
+    T raw;         // segment-box distance before the radius is subtracted
+    T depth;       // end point inside the box: its distance to the nearest face
+    T hit_margin;  // face crossing: distance of the decision from flipping
+    T sin2;        // smallest squared sine over the segment-segment solves
+    bool inside;
+    bool hit;
+};
+
+// How far the crossing of face 'face' is from being decided the other way:
+// the smaller of the crossing point's distances to the rectangle's sides and
+// the end points' distances to the face plane (negative when not crossed).
+template <typename T, class MP = ExactMath>
+__host__ __device__ inline T face_margin(const Vec3<T>& A, const Vec3<T>& B, const Box<T>& b, int face) {
+    const int c = face % 3;
+    const int o1 = (c == 0) ? 1 : 0, o2 = (c == 2) ? 1 : 2;
+    const T v = b.f(face);
+    const T den = B[c] - A[c];
+    if (den == 0) return T(-1);
+    const T t = MP::div(v - A[c], den);
+    if (!(t > 0 && t < 1)) return T(-1);
+    const T m0 = A[o1] + t * (B[o1] - A[o1]);
+    const T m1 = A[o2] + t * (B[o2] - A[o2]);
+    T mg = smin(smin(m0 - b.lo_at(o1), b.hi_at(o1) - m0), smin(m1 - b.lo_at(o2), b.hi_at(o2) - m1));
+    const T ea = v - A[c], eb = B[c] - v;
+    mg = smin(mg, smin(ea < 0 ? -ea : ea, eb < 0 ? -eb : eb));
+    return mg;
+}
+
+template <typename T, bool WANT_NP, class MP = ExactMath>
 __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T radius, const Box<T>& b,
-                                         const Params<T>& p, NP<T>* np, T* sin2_min, uint32_t* status) {
+                                         const Params<T>& p, NP<T>* np, PairInfo<T>* info, uint32_t* status) {
     // --- projectionLineSegOnSide x3 (lines 411-413, 450-474); bit f of the
     // masks is face f (xmin, ymin, zmin, xmax, ymax, zmax)
     int pmA = 0, pmB = 0;
@@ -277,7 +320,16 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
             int& pm = i ? pmB : pmA;
             T& dist = i ? distB : distA;
             if (X[o1] >= b.lo[o1] && X[o1] <= b.hi[o1] && X[o2] >= b.lo[o2] && X[o2] <= b.hi[o2]) {
-                if (X[k] > b.lo[k] && X[k] < b.hi[k]) inside = true;
+                if (X[k] > b.lo[k] && X[k] < b.hi[k]) {
+                    inside = true;
+                    if (info) {
+                        // distance of the inside point to the box surface
+                        T dm = smin(smin(X.x - b.lo[0], b.hi[0] - X.x), smin(X.y - b.lo[1], b.hi[1] - X.y));
+                        dm = smin(dm, smin(X.z - b.lo[2], b.hi[2] - X.z));
+                        info->depth = info->inside ? smin(info->depth, dm) : dm;
+                        info->inside = true;
+                    }
+                }
                 else if (X[k] <= b.lo[k]) { pm |= 1 << k; dist = b.lo[k] - X[k]; }
                 else if (X[k] >= b.hi[k]) { pm |= 8 << k; dist = X[k] - b.hi[k]; }
             }
@@ -285,6 +337,7 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
     }
     if (inside) {
         if (WANT_NP) np->valid = false;
+        if (info) info->raw = T(0);
         return T(0);
     }
 
@@ -357,10 +410,18 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
 
     // --- crossing tests: any hit makes the distance 0 (then -radius)
     bool hit = false;
-    if (nfaces == 1) hit = face_crossed(A, B, b, face1, p);
+    if (nfaces == 1) hit = face_crossed<T, MP>(A, B, b, face1, p);
     else if (nfaces == 6)
-        for (int f = 0; f < 6; ++f) hit = hit || face_crossed(A, B, b, f, p);
+        for (int f = 0; f < 6; ++f) hit = hit || face_crossed<T, MP>(A, B, b, f, p);
 
+    if (info && hit) {
+        T mg = T(-1);
+        if (nfaces == 1) mg = face_margin<T, MP>(A, B, b, face1);
+        else
+            for (int f = 0; f < 6; ++f) { const T m = face_margin<T, MP>(A, B, b, f); mg = (m > mg) ? m : mg; }
+        info->hit = true;
+        info->hit_margin = mg;
+    }
     if (hit) {
         d_c = T(0);
     } else if (no_edge) {
@@ -368,15 +429,15 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
         return T(NAN);
     } else if (corner >= 0) {
         NP<T> t;
-        d_c = seg_pt<T, WANT_NP>(A, B, b.corner(corner), p, &t);
+        d_c = seg_pt<T, WANT_NP, MP>(A, B, b.corner(corner), p, &t);
         if (WANT_NP) *np = t;
     } else {
         for (int e = 0; e < nedges; ++e) {
             const int ed = list_edge(kind, param, e);
             const Vec3<T> C = b.corner(ed & 7), D = b.corner(ed >> 3);
             NP<T> t;
-            const T d_tmp = seg_seg<T, WANT_NP>(A, B, C, D, p, &t);
-            if (sin2_min) *sin2_min = smin(*sin2_min, seg_seg_sin2(A, B, C, D));
+            const T d_tmp = seg_seg<T, WANT_NP, MP>(A, B, C, D, p, &t);
+            if (info) info->sin2 = smin(info->sin2, seg_seg_sin2<T, MP>(A, B, C, D));
             if (direct) {
                 d_c = d_tmp;
                 if (WANT_NP) *np = t;
@@ -389,6 +450,7 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
             }
         }
     }
+    if (info) info->raw = d_c;
     return d_c - radius;
 }
 

@@ -67,6 +67,17 @@ hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const Quer
                              const int32_t* rounds, int32_t* first, int64_t num_edges, uint32_t* counter,
                              int32_t total_items, int waves, hipStream_t st);
 
+// FP32 bulk pass: relaxed arithmetic, writes its results and appends every
+// configuration with an uncertain pair to idx (count in *count, which the
+// pass expects to be zero).
+hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, hipStream_t st);
+hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                  int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
+                                  hipStream_t st);
+hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st);
+size_t refine_lds_bytes(const RobotDev& rb, const EnvDev& env);
+
 hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                  int64_t n, const DistOut& out, hipStream_t st);
 hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,

@@ -14,16 +14,21 @@
 // This file is compiled without FMA contraction, with IEEE division and
 // square root: the float kernels reproduce the reference's float rounding
 // bit for bit, the double kernels run the same algorithm in FP64.
+// No FMA contraction anywhere in this translation unit, headers included:
+// the float kernels must keep the reference's rounding.
+#pragma clang fp contract(off)
+
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "rvs_fk.hpp"
 #include "rvs_geometry.hpp"
 #include "rvs_internal.hpp"
 #include "rvs_stage.hpp"
 
-#pragma clang fp contract(off)
 
 namespace rvs {
 
@@ -36,13 +41,21 @@ constexpr int kBlock = 256;
 // state: per-link profile, global minimum, early exit on the first pair whose
 // running minimum is <= 0 (the profile keeps that value, later links stay 0,
 // the distance is 0 and the nearest points are dropped).
+// With an index list, thread g handles configuration idx[g] for g < *count
+// (the refinement pass of RVS_PREC_FP32_REFINE).
 template <typename T, bool WANT_NP>
 __device__ __attribute__((always_inline)) inline void distance_body(const RobotDev& rb, const EnvDev& env,
                                                                     const QueryParams& qp, const float* __restrict__ q,
-                                                                    int64_t n, const DistOut& out) {
+                                                                    int64_t n, const DistOut& out,
+                                                                    const int32_t* idx = nullptr,
+                                                                    const uint32_t* count = nullptr) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<T> s = stage_env<T>(env, smem);
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx) {
+        if (c >= (int64_t)*count) return;
+        c = idx[c];
+    }
     if (c >= n) return;
     const int M = env.num_obj;
     const int L = rb.num_links;
@@ -114,6 +127,74 @@ template <bool WANT_NP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance32(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out) {
     distance_body<float, WANT_NP>(rb, env, qp, q, n, out);
+}
+
+// Exact recomputation of the configurations the FP32 bulk pass flagged: one
+// wave per configuration.  Lane t evaluates pairs p = t, t+64, ... (link
+// p / M, obstacle p % M) with the exact tree and parks the filtered pair
+// distance in LDS; lane 0 then folds them in the reference's loop order, so
+// the early exit, the partial profile and std::min's NaN handling come out as
+// in RealVectorSpace::computeDistance.  A few thousand flagged configurations
+// then finish in about one configuration's latency instead of one wave's
+// whole sequential walk.
+__global__ __launch_bounds__(64) void k_distance_refine(RobotDev rb, EnvDev env, QueryParams qp,
+                                                        const float* __restrict__ q, DistOut out,
+                                                        const int32_t* __restrict__ idx,
+                                                        const uint32_t* __restrict__ count) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<float> s = stage_env<float>(env, smem);
+    float* f = reinterpret_cast<float*>(smem + ((staged_bytes<float>(env.num_obj) + 15) & ~size_t(15)));
+    const int M = env.num_obj, L = rb.num_links, P = L * M;
+    const int t = threadIdx.x;
+    const Params<float> p{qp.eq, qp.maxd};
+    const uint32_t total = *count;
+    for (uint32_t k = blockIdx.x; k < total; k += gridDim.x) {
+        const int64_t c = idx[k];
+        const float* qc = q + c * rb.num_dofs;
+        FKStream fk;
+        fk_init(fk);
+        int have = -1;          // index of the link whose end points are in P0/P1
+        Vec3<float> P0{0, 0, 0}, P1{0, 0, 0};
+        for (int pi = t; pi < P; pi += 64) {
+            const int i = pi / M, j = pi - i * M;
+            if (i != have) {
+                // links visited by this lane increase, so the stream only moves forward
+                if (have < 0 || i != have + 1) P0 = skel_point<float>(fk, rb, qc, i);
+                else P0 = P1;
+                P1 = skel_point<float>(fk, rb, qc, i + 1);
+                have = i;
+            }
+            float dt;
+            if (s.gnd[j] && i < rb.ground_included) {
+                dt = p.maxd;
+            } else {
+                const Box<float> bx = s.box[j];
+                dt = capsule_box<float, false>(P0, P1, rb.radius[i], bx, p, nullptr, nullptr, nullptr);
+            }
+            if (dt > s.tol[j]) dt = p.maxd;
+            f[pi] = dt;
+        }
+        __syncthreads();
+        if (t == 0) {
+            float d = p.maxd;
+            bool hit = false;
+            int i = 0;
+            for (; i < L && !hit; ++i) {
+                float prof = p.maxd;
+                for (int j = 0; j < M; ++j) {
+                    prof = smin(prof, f[i * M + j]);
+                    if (prof <= 0) { hit = true; break; }
+                }
+                if (out.profile) out.profile[c * L + i] = prof;
+                if (!hit) d = smin(d, prof);
+            }
+            for (; i < L; ++i)
+                if (out.profile) out.profile[c * L + i] = 0.0f;
+            if (out.d) out.d[c] = hit ? 0.0f : d;
+            if (out.verdict) out.verdict[c] = hit ? 0 : 1;
+        }
+        __syncthreads();
+    }
 }
 
 template <bool WANT_NP>
@@ -258,6 +339,22 @@ hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const Que
         hipLaunchKernelGGL((k_distance64<true>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
     else
         hipLaunchKernelGGL((k_distance64<false>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+    return hipGetLastError();
+}
+
+size_t refine_lds_bytes(const RobotDev& rb, const EnvDev& env) {
+    return ((staged_bytes<float>(env.num_obj) + 15) & ~size_t(15)) + (size_t)rb.num_links * env.num_obj * 4;
+}
+
+hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                  int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
+                                  hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    // enough waves to cover a typical flagged set in one round; the kernel
+    // loops when more configurations are flagged
+    const unsigned grid = (unsigned)std::min<int64_t>(n, 8192);
+    hipLaunchKernelGGL(k_distance_refine, dim3(grid), dim3(64), refine_lds_bytes(rb, env), st, rb, env, qp, q, out,
+                       idx, count);
     return hipGetLastError();
 }
 

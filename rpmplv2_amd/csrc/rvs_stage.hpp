@@ -8,8 +8,6 @@
 #include "rvs_geometry.hpp"
 #include "rvs_internal.hpp"
 
-#pragma clang fp contract(off)
-
 namespace rvs {
 
 template <typename T>

@@ -203,3 +203,27 @@ def test_edge_first_collision_sparse_scene(oracle, engine):
     f = engine.edge_batches(eid, [b])[0]
     of = oracle.edge_first_collision(rb, oracle.scene_of(sc), q1, u, K, 0.05)
     assert np.array_equal(f, of) and (of == -1).mean() > 0.1
+
+
+@pytest.mark.parametrize("wl", ["workload1", "workload2"])
+def test_fp32_refine_verdict_exact(oracle, engine, wl):
+    """FP32 bulk pass + exact refinement: verdicts identical to the reference
+    arithmetic, distances within 1e-5 m of the FP64 oracle (and profiles of
+    collision-free configurations too)."""
+    w = W.workload1(65536) if wl == "workload1" else W.workload2(1 << 18)
+    rid = engine.load_robot(w.robot)
+    eid = engine.create_env(w.scene)
+    d, prof, ver, _ = engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
+    refined = engine.last_refined()
+    sc = oracle.scene_of(w.scene)
+    od, oprof, over, _ = oracle.distance(w.robot, sc, w.q, 32)
+    d64, prof64, ver64, _ = oracle.distance(w.robot, sc, w.q, 64)
+    print(f"{wl}: refined {refined} of {len(w.q)}; verdict mismatches {(ver != over).sum()}")
+    assert (ver == over).all()
+    assert np.abs(d - d64).max() <= 1e-5
+    ok = (ver == 1) & (ver64 == 1)
+    assert np.abs(prof[ok] - prof64[ok]).max() <= 1e-5
+    # collision configurations: same first colliding link, same partial profile
+    bad = ver == 0
+    assert np.abs(prof[bad] - oprof[bad]).max() <= 1e-5
+    assert refined < 0.05 * len(w.q)
