@@ -2,13 +2,18 @@
 """Benchmark: distance queries per second, xArm6 with 16 boxes + ground.
 
 One step = one rvs_distance call over the whole batch of 1,048,576 joint
-configurations (xArm6, capsule radius 0.05 m, ground box + 16 random boxes):
-per-link distance profile, minimum distance and collision verdict for every
-configuration, inputs and outputs resident in HBM.  The timed precision is
-the engine's reference-exact arithmetic (verdicts bit-identical to RPMPLv2);
-the FP64 path is measured beside it.
+configurations (xArm6, capsule radius 0.05 m, ground box + 16 random boxes,
+configurations uniform in the joint limits): per-link distance profile,
+minimum distance and collision verdict for every configuration, inputs and
+outputs resident in HBM.  The timed path is the FP32 bulk pass with exact
+near-contact refinement, whose verdicts are identical to the reference
+arithmetic; the exact path and the FP64 path are timed beside it.
 
-Prints one JSON line (rank 0).  Only one GPU is supported.
+After timing, the same batch is checked against the CPU oracle: verdict
+mismatches against the reference arithmetic (must be 0) and the largest
+distance error against the FP64 twin (must be <= 1e-5 m).
+
+Prints one JSON line.  One GPU only.
 """
 from __future__ import annotations
 
@@ -22,10 +27,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic work per unit (fixed nominal counts used for the roofline):
+# revolute FK joint 100 flop, capsule-AABB pair 160 flop.
 FLOP_FK_JOINT = 100
 FLOP_CAPSULE_AABB = 160
 PEAK_FP32_TFLOPS = 157.3
-PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of k_distance_fast on this workload from two
+# `rocprofv3 --pmc` runs: FETCH_SIZE 12,504.7 KiB x 2.0 + WRITE_SIZE
+# 58,144.8 KiB x 1.0, the factors being what the same counters read for a
+# 256 MiB copy (profiles/k_distance_fast_xarm6_1M.md).
+TRAFFIC_BYTES = (2.0 * 12504.7 + 1.0 * 58144.8) * 1024
 
 
 def workload_flops(rb, scene) -> int:
@@ -38,29 +48,34 @@ def workload_flops(rb, scene) -> int:
     return rb.num_dofs * FLOP_FK_JOINT + pairs * FLOP_CAPSULE_AABB
 
 
-def cpu_baseline(w, n_sample: int):
-    """The oracle's reference arithmetic (ref32) timed on this host."""
+def host_cores() -> int:
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(cores or 1, 16))
+
+
+def cpu_baseline(w, cores: int):
+    """The oracle's reference arithmetic (ref32) timed on this host: a fresh
+    state per query, FK + skeleton + pair loop with the early exit."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
-    import numpy as np
     sc = O.scene_of(w.scene)
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = max(1, min(cores, 16))
-    q = w.q[:n_sample]
+    q = w.q
     O.distance(w.robot, sc, q[:4096], 32, nthreads=cores)  # warm-up
-    runs = []
+    runs, res = [], None
     for _ in range(3):
         t = time.perf_counter()
-        O.distance(w.robot, sc, q, 32, nthreads=cores)
+        res = O.distance(w.robot, sc, q, 32, nthreads=cores)
         runs.append(time.perf_counter() - t)
     t_all = sorted(runs)[1]
-    q1 = q[:32768]
+    q1 = q[:65536]
     t = time.perf_counter()
     O.distance(w.robot, sc, q1, 32, nthreads=1)
     t_one = time.perf_counter() - t
-    return {"value": len(q) / t_all, "unit": "configs/s", "cores": cores, "kind": "port",
-            "sample": f"{len(q)} configurations of the timed workload, median of 3 passes on {cores} threads",
-            "value_1thread": len(q1) / t_one}
+    cb = {"value": len(q) / t_all, "unit": "configs/s", "cores": cores, "kind": "port",
+          "sample": f"all {len(q)} configurations of the timed batch, median of 3 passes on {cores} threads "
+                    f"({t_all:.2f} s each); 1-thread rate on the first 65536",
+          "value_1thread": len(q1) / t_one}
+    return cb, res
 
 
 def main():
@@ -69,10 +84,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only; multi-GPU runs are not supported by this build", file=sys.stderr)
+        print("bench.py measures one GPU only; this build does not run across GPUs", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -81,51 +96,42 @@ def main():
 
     w = W.workload2(args.n)
     n = len(w.q)
-    eng = rvs.Engine(device=0, precision=rvs.PREC_REF32)
+    L = w.robot.num_links
+    eng = rvs.Engine(device=0, precision=rvs.PREC_FP32_REFINE)
     rid = eng.load_robot(w.robot)
     eid = eng.create_env(w.scene)
-    L = w.robot.num_links
     dev = torch.device("cuda:0")
     q = torch.from_numpy(w.q).to(dev)
     d = torch.empty(n, dtype=torch.float32, device=dev)
     prof = torch.empty((n, L), dtype=torch.float32, device=dev)
     ver = torch.empty(n, dtype=torch.uint8, device=dev)
-    stream = torch.cuda.ExternalStream(eng.stream)
     torch.cuda.synchronize()
 
-    def run(prec):
-        eng.distance_device(rid, eid, q, d, prof, ver, precision=prec)
+    def timed(prec, steps, warmup):
+        for _ in range(warmup):
+            eng.distance_device(rid, eid, q, d, prof, ver, precision=prec)
+        torch.cuda.synchronize()
+        bulk, refine, refined = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.distance_device(rid, eid, q, d, prof, ver, precision=prec, timing=True)
+            st = eng.last_timing()
+            bulk.append(st[0])
+            refine.append(st[1] if len(st) > 1 else 0.0)
+            refined.append(eng.last_refined())
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        return ms, float(np.mean(bulk)), float(np.mean(refine)), int(np.mean(refined))
 
-    for _ in range(args.warmup):
-        run(rvs.PREC_REF32)
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        evs[k][0].record(stream)
-        run(rvs.PREC_REF32)
-        evs[k][1].record(stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / args.steps
+    ms_step, bulk_ms, refine_ms, refined = timed(rvs.PREC_FP32_REFINE, args.steps, args.warmup)
     value = n / (ms_step * 1e-3)
-
-    # FP64 path beside it (same inputs)
-    for _ in range(2):
-        run(rvs.PREC_FP64)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    reps = max(3, args.steps // 4)
-    for _ in range(reps):
-        run(rvs.PREC_FP64)
-    torch.cuda.synchronize()
-    fp64_rate = n * reps / (time.perf_counter() - t)
+    d_gpu, prof_gpu, ver_gpu = d.cpu().numpy(), prof.cpu().numpy(), ver.cpu().numpy()
+    ms_ref32, k_ref32, _, _ = timed(rvs.PREC_REF32, max(3, args.steps // 2), 2)
+    ms_fp64, k_fp64, _, _ = timed(rvs.PREC_FP64, max(3, args.steps // 2), 2)
 
     flop_cfg = workload_flops(w.robot, w.scene)
-    achieved_tflops = flop_cfg * n / (kern_ms * 1e-3) / 1e12
+    achieved = flop_cfg * n / (bulk_ms * 1e-3) / 1e12
     hbm_bytes = n * (w.robot.num_dofs * 4 + 4 + L * 4 + 1)
-
     out = {
         "metric": "distance queries/sec (xArm6, 16 boxes) at verdict-exact",
         "value": value,
@@ -138,20 +144,36 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded xArm6 configurations uniform in joint limits, 16 random boxes + ground)",
+        "data": "synthetic (seeded xArm6 configurations uniform in the joint limits; ground box + 16 random boxes)",
         "config": {"workload": "xarm6_16aabb_ground", "configs_per_step": n, "links": L,
-                   "objects": w.scene.num_obj, "precision": "ref32 (FP64 kinematics, reference-order FP32 geometry)",
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_distance<float>", "kernel_ms": kern_ms, "flop_per_config": flop_cfg,
-                     "hbm_GBps_algorithmic": hbm_bytes / (kern_ms * 1e-3) / 1e9},
-        "fp64_configs_per_s": fp64_rate,
+                   "objects": w.scene.num_obj, "capsule_radius": 0.05,
+                   "precision": "fp32 bulk + exact near-contact refinement (band 1e-4 m)"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES,
+                     "kernel": "k_distance_fast", "kernel_ms": bulk_ms, "flop_per_config": flop_cfg,
+                     "algorithmic_bytes": hbm_bytes,
+                     "hbm_GBps_algorithmic": hbm_bytes / (bulk_ms * 1e-3) / 1e9},
+        "refine": {"configs": refined, "fraction": refined / n, "kernel": "k_distance_refine",
+                   "kernel_ms": refine_ms},
+        "ref32_exact": {"configs_per_s": n / (ms_ref32 * 1e-3), "kernel_ms": k_ref32},
+        "fp64": {"configs_per_s": n / (ms_fp64 * 1e-3), "kernel_ms": k_fp64},
     }
     if not args.no_cpu:
-        cb = cpu_baseline(w, min(n, 1 << 20))
+        cores = host_cores()
+        cb, ref = cpu_baseline(w, cores)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu_all"] = value / cb["value"]
+        od, oprof, over, _ = ref
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        d64, prof64, ver64, _ = O.distance(w.robot, O.scene_of(w.scene), w.q, 64, nthreads=cores)
+        free = (ver_gpu == 1) & (ver64 == 1)
+        out["parity"] = {
+            "verdict_mismatch_vs_ref32": int((ver_gpu != over).sum()),
+            "max_abs_dist_err_vs_ref64": float(np.abs(d_gpu - d64).max()),
+            "max_abs_profile_err_vs_ref64_free": float(np.abs(prof_gpu[free] - prof64[free]).max()),
+            "collision_free_fraction": float(ver_gpu.mean()),
+        }
     print(json.dumps(out))
     eng.close()
 

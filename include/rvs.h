@@ -11,7 +11,7 @@
  *     the ABI.  rvs_last_error() describes the last failure of a context.
  *   - Buffers are caller-owned.  With RVS_F_DEVICE every array argument is a
  *     device pointer on the context's GPU; otherwise arrays are host memory
- *     and the call stages them through the context's pinned buffers.  No
+ *     and the call stages them through the context's device arena.  No
  *     pointer is kept after a synchronous call returns.
  *   - A context is internally synchronised: calls from several host threads
  *     on one context are serialised; separate contexts run independently.
@@ -43,14 +43,19 @@ typedef enum rvs_precision {
     RVS_PREC_REF32 = 0,
     /* The same algorithm carried out entirely in FP64. */
     RVS_PREC_FP64 = 1,
-    /* FP32 bulk pass; configurations with a pair inside the near-contact
-     * band (or an ill-conditioned segment pair) are recomputed: verdict and
-     * profile with RVS_PREC_REF32, distance with RVS_PREC_FP64. */
+    /* FP32 bulk pass in relaxed single precision; every configuration with
+     * an uncertain pair (inside the near-contact band, near the
+     * EQUALITY_THRESHOLD snap, a marginal inside/crossing decision, near a
+     * min_dist_tol, a near-parallel segment solve, or non-finite input) is
+     * recomputed with the RVS_PREC_REF32 arithmetic (FP64 kinematics,
+     * reference-order FP32 geometry).  Verdicts are those of
+     * RVS_PREC_REF32; distances stay within 1e-5 m of RVS_PREC_FP64. */
     RVS_PREC_FP32_REFINE = 2
 } rvs_precision;
 
 /* Call flags. */
 #define RVS_F_DEVICE 0x1u          /* array arguments are device pointers */
+#define RVS_F_TIMING 0x2u          /* time the call's kernel stages with HIP events (rvs_last_timing) */
 #define RVS_F_PREC_SHIFT 4         /* bits 4..5: precision override + 1 (0 = context default) */
 #define RVS_F_PREC(p) ((uint32_t)((p) + 1) << RVS_F_PREC_SHIFT)
 
@@ -192,6 +197,12 @@ void* rvs_stream(rvs_context* ctx);
 /* Statistics of the last RVS_PREC_FP32_REFINE query: configurations
  * recomputed by the refinement pass. */
 int64_t rvs_last_refined(const rvs_context* ctx);
+/* Stage durations of the last call made with RVS_F_TIMING, in ms, measured
+ * with HIP events on the context's stream.  rvs_distance reports
+ * [0] the bulk kernel (k_distance_fast for RVS_PREC_FP32_REFINE, the exact
+ * kernel otherwise) and [1] the refinement kernel (0 when there is none).
+ * Returns the number of stages written. */
+int32_t rvs_last_timing(const rvs_context* ctx, float* stage_ms, int32_t max_stages);
 /* Library version string. */
 const char* rvs_version(void);
 

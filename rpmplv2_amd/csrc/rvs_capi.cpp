@@ -68,6 +68,9 @@ struct rvs_context {
     std::vector<EnvRec> envs;
     DevBuf arena;
     int64_t last_refined = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    float stage_ms[2] = {0, 0};
+    int32_t nstages = 0;
 };
 
 namespace {
@@ -234,6 +237,7 @@ rvs_status rvs_create(const rvs_options* opt, rvs_context** out) {
     }
     e = hipSetDevice(c->opt.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
     if (e != hipSuccess) { delete c; return RVS_EDEVICE; }
     *out = c;
     return RVS_OK;
@@ -246,6 +250,8 @@ void rvs_destroy(rvs_context* c) {
     for (auto& r : c->robots) for (void* p : r.allocs) (void)hipFree(p);
     for (auto& e : c->envs) for (void* p : e.allocs) (void)hipFree(p);
     c->arena.release();
+    for (int k = 0; k < 3; ++k)
+        if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -253,6 +259,13 @@ void rvs_destroy(rvs_context* c) {
 const char* rvs_last_error(const rvs_context* c) { return c ? c->err.c_str() : "null context"; }
 void* rvs_stream(rvs_context* c) { return c ? (void*)c->stream : nullptr; }
 int64_t rvs_last_refined(const rvs_context* c) { return c ? c->last_refined : 0; }
+
+int32_t rvs_last_timing(const rvs_context* c, float* stage_ms, int32_t max_stages) {
+    if (!c || !stage_ms) return 0;
+    const int32_t k = std::min<int32_t>(max_stages, c->nstages);
+    for (int32_t i = 0; i < k; ++i) stage_ms[i] = c->stage_ms[i];
+    return k;
+}
 
 rvs_status rvs_robot_load(rvs_context* c, const rvs_robot_desc* d, int32_t* robot_id) {
     if (!c || !d || !robot_id || !d->joints) return fail(c, RVS_EINVAL, "rvs_robot_load: null argument");
@@ -476,16 +489,22 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
         qd = qs;
     }
     uint32_t refined = 0;
+    const bool timing = (flags & RVS_F_TIMING) != 0;
     if (e == hipSuccess) {
         if (prec == RVS_PREC_FP32_REFINE) {
             e = launch_zero_u32(rcount, 1, c->stream);
+            if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
             if (e == hipSuccess) e = launch_distance_fast(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, c->stream);
+            if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
             if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, c->stream);
+            if (e == hipSuccess && timing) e = hipEventRecord(c->ev[2], c->stream);
             if (e == hipSuccess) e = d2h(&refined, rcount, 4, c->stream);
-        } else if (prec == RVS_PREC_FP64) {
-            e = launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream);
         } else {
-            e = launch_distance_ref32(rb.dev, ev.dev, qp, qd, n, out, c->stream);
+            if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
+            if (e == hipSuccess)
+                e = (prec == RVS_PREC_FP64) ? launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream)
+                                            : launch_distance_ref32(rb.dev, ev.dev, qp, qd, n, out, c->stream);
+            if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
         }
     }
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
@@ -497,6 +516,12 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_distance");
     c->last_refined = (prec == RVS_PREC_FP32_REFINE) ? (int64_t)refined : 0;
+    if (timing) {
+        c->nstages = 2;
+        c->stage_ms[0] = c->stage_ms[1] = 0;
+        (void)hipEventElapsedTime(&c->stage_ms[0], c->ev[0], c->ev[1]);
+        if (prec == RVS_PREC_FP32_REFINE) (void)hipEventElapsedTime(&c->stage_ms[1], c->ev[1], c->ev[2]);
+    }
     return RVS_OK;
 }
 

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "librvs.so")
 RVS_OK, RVS_EINVAL, RVS_ENOMEM, RVS_EDEVICE, RVS_ERANGE, RVS_ENOTSUP = range(6)
 PREC_REF32, PREC_FP64, PREC_FP32_REFINE = 0, 1, 2
 F_DEVICE = 0x1
+F_TIMING = 0x2
 ENV_AABB, ENV_OBB = 0, 1
 BOX_GROUND, BOX_DYNAMIC = 0x1, 0x2
 
@@ -61,7 +62,7 @@ class Box(C.Structure):
 EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error", "rvs_stream", "rvs_last_refined",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds", "rvs_distance",
            "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_self_distance",
-           "rvs_edge_is_valid", "rvs_edge_first_collision"]
+           "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing"]
 
 _lib = None
 
@@ -86,6 +87,8 @@ def lib():
         L.rvs_last_refined.argtypes = [ctx]
         L.rvs_last_refined.restype = i64
         L.rvs_version.restype = C.c_char_p
+        L.rvs_last_timing.argtypes = [ctx, vp, i32]
+        L.rvs_last_timing.restype = i32
         L.rvs_robot_load.argtypes = [ctx, C.POINTER(RobotDesc), C.POINTER(i32)]
         L.rvs_env_create.argtypes = [ctx, C.POINTER(Box), i32, i32, C.POINTER(i32)]
         L.rvs_env_update.argtypes = [ctx, i32, vp, vp, i32, u32]
@@ -184,6 +187,12 @@ class Engine:
     def last_refined(self) -> int:
         return lib().rvs_last_refined(self._ctx)
 
+    def last_timing(self):
+        """Stage durations (ms) of the last call made with timing=True."""
+        buf = np.zeros(4, np.float32)
+        k = lib().rvs_last_timing(self._ctx, _ptr(buf), 4)
+        return [float(x) for x in buf[:k]]
+
     # ---- set-up
     def load_robot(self, rb: R.Robot) -> int:
         d, keep = robot_desc(rb)
@@ -265,8 +274,8 @@ class Engine:
 
     # ---- queries on device tensors (torch CUDA tensors; no staging)
     def distance_device(self, robot: int, env: int, q, d, profile=None, verdict=None, nearest=None,
-                        precision: Optional[int] = None):
-        fl = F_DEVICE | (F_PREC(precision) if precision is not None else 0)
+                        precision: Optional[int] = None, timing: bool = False):
+        fl = F_DEVICE | (F_PREC(precision) if precision is not None else 0) | (F_TIMING if timing else 0)
         self._check(lib().rvs_distance(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d), _ptr(profile),
                                        _ptr(nearest), _ptr(verdict), fl), "rvs_distance")
 
