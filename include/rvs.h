@@ -125,7 +125,10 @@ enum {
      * centre +- |dim|/2. */
     RVS_ENV_AABB = 0,
     /* Build extension: rotated boxes are queried as oriented boxes (the
-     * capsule is moved into the box frame). */
+     * capsule is moved into the box frame; nearest points come back in the
+     * world frame).  Accepted by every environment query.  The obstacle set is
+     * staged in 64 KB of LDS: up to 851 oriented boxes for float queries and
+     * 624 for RVS_PREC_FP64 (RVS_ERANGE beyond). */
     RVS_ENV_OBB = 1
 };
 
@@ -172,6 +175,16 @@ rvs_status rvs_distance_underestimation(rvs_context* ctx, int32_t robot, int32_t
  * (xArm6.cpp:384-411; the FCL mesh confirmation is not part of this path). */
 rvs_status rvs_self_distance(rvs_context* ctx, int32_t robot, const float* q, int64_t n, float* dist,
                              uint32_t flags);
+
+/* Capsule part of checkSelfCollision(q) per configuration
+ * (xArm6::checkSelfCollision, src/robots/xArm6.cpp:374-411, pair loop at
+ * :386-411 with computeCapsulesDistance :417-424): min_dist[N] = smallest pair
+ * distance (std::min order from max_distance, NaN pairs ignored) and
+ * free_flag[N] = 1 when no pair distance is < 0.  Either output may be null.
+ * For chains the pair list (j >= i + 2) is a build extension; the reference
+ * chains have no self-collision (Spatial10DOF.cpp:32-43). */
+rvs_status rvs_self_collision(rvs_context* ctx, int32_t robot, const float* q, int64_t n, float* min_dist,
+                              uint8_t* free_flag, uint32_t flags);
 
 /* RealVectorSpace::isValid(q1, q2) (RealVectorSpace.cpp:261-275): num_checks
  * interpolated states from the far end, q1 not checked. */

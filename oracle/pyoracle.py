@@ -36,7 +36,8 @@ class OrcParams(C.Structure):
 
 class OrcScene(C.Structure):
     _fields_ = [("num_obj", C.c_int32), ("pad", C.c_int32), ("obs", C.POINTER(C.c_float)),
-                ("is_ground", C.POINTER(C.c_uint8)), ("min_dist_tol", C.POINTER(C.c_float))]
+                ("is_ground", C.POINTER(C.c_uint8)), ("min_dist_tol", C.POINTER(C.c_float)),
+                ("obb", C.POINTER(C.c_float))]
 
 
 def build(verbose: bool = False) -> str:
@@ -92,6 +93,9 @@ def _declare(L):
         "orc_edge_first_collision32": (None, [R, Pp, S, i32p, C.c_int32, C.c_float, f32p, f32p, i32p,
                                               C.c_int64, i32p, C.c_int32]),
         "orc_true_seg_box": (C.c_double, [f64p, f64p, f64p]),
+        "orc_capsule_obb32": (C.c_float, [f32p, f32p, C.c_float, f32p, Pp, f32p, i32p]),
+        "orc_capsule_obb64": (C.c_double, [f64p, f64p, C.c_double, f64p, Pp, f64p, i32p]),
+        "orc_self_min32": (None, [R, Pp, f32p, C.c_int64, i32p, C.c_int32, f32p, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -131,17 +135,30 @@ def params_struct(eq=1e-4, maxd=1000.0, num_checks=10) -> OrcParams:
 class Scene:
     """Keeps the numpy arrays alive behind an OrcScene."""
 
-    def __init__(self, obs: np.ndarray, is_ground: np.ndarray, tol: np.ndarray):
+    def __init__(self, obs: np.ndarray, is_ground: np.ndarray, tol: np.ndarray, obb: Optional[np.ndarray] = None):
         self.obs = np.ascontiguousarray(obs, dtype=np.float32).reshape(-1, 6)
         self.is_ground = np.ascontiguousarray(is_ground, dtype=np.uint8)
         self.tol = np.ascontiguousarray(tol, dtype=np.float32)
+        self.obb = None if obb is None else np.ascontiguousarray(obb, dtype=np.float32).reshape(-1, 15)
         self.s = OrcScene(len(self.obs), 0, _ptr(self.obs, C.c_float), _ptr(self.is_ground, C.c_uint8),
-                          _ptr(self.tol, C.c_float))
+                          _ptr(self.tol, C.c_float), _ptr(self.obb, C.c_float))
 
 
-def scene_of(sc) -> Scene:
-    """From a rpmplv2_amd.workloads.SceneSpec-like object (aabb, is_ground, tol)."""
-    return Scene(sc.aabb, sc.is_ground, sc.min_dist_tol)
+def scene_of(sc, oriented: bool = False) -> Scene:
+    """From a rpmplv2_amd.workloads.Scene (aabb, is_ground, tol); with
+    oriented=True the boxes are used as oriented boxes (build extension)."""
+    return Scene(sc.aabb, sc.is_ground, sc.min_dist_tol, sc.obb_table() if oriented else None)
+
+
+def self_min(rb, q, params=None, nthreads=8):
+    p = params or params_struct()
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+    pr = np.ascontiguousarray(np.asarray(rb.self_pairs, np.int32).reshape(-1, 2))
+    out = np.zeros(len(q), np.float32)
+    rs = robot_struct(rb)
+    lib().orc_self_min32(C.byref(rs), C.byref(p), _ptr(q, C.c_float), len(q), _ptr(pr, C.c_int32), len(pr),
+                         _ptr(out, C.c_float), nthreads)
+    return out
 
 
 # ---------------------------------------------------------------- wrappers
@@ -286,3 +303,14 @@ def edge_first_collision(rb, scene: Scene, q1, direction, K, step=0.05, params=N
                                      np.float32(step), _ptr(q1, C.c_float), _ptr(u, C.c_float),
                                      _ptr(K, C.c_int32), len(q1), _ptr(out, C.c_int32), nthreads)
     return out
+
+
+def capsule_obb(A, B, r, obb, precision=32, params=None):
+    """Oriented-box extension: obb = centre(3), rotation rows(9), half extents(3)."""
+    p = params or params_struct()
+    dt, ct = (np.float32, C.c_float) if precision == 32 else (np.float64, C.c_double)
+    a, b, o = _vec(A, dt), _vec(B, dt), _vec(obb, dt)
+    np_ = np.zeros(6, dt); v = C.c_int32(0)
+    fn = lib().orc_capsule_obb32 if precision == 32 else lib().orc_capsule_obb64
+    d = fn(_ptr(a, ct), _ptr(b, ct), dt(r), _ptr(o, ct), C.byref(p), _ptr(np_, ct), C.byref(v))
+    return d, np_, v.value

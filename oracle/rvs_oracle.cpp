@@ -607,14 +607,62 @@ template <typename T> struct SceneT {
     std::vector<T> obs;
     std::vector<uint8_t> ground;
     std::vector<T> tol;
+    std::vector<T> obb;   // empty unless oriented boxes
     SceneT(const orc_scene* sc) : M(sc->num_obj), obs(6 * sc->num_obj), ground(sc->num_obj), tol(sc->num_obj) {
         for (int j = 0; j < M; ++j) {
             for (int k = 0; k < 6; ++k) obs[6 * j + k] = (T)sc->obs[6 * j + k];
             ground[j] = sc->is_ground ? sc->is_ground[j] : 0;
             tol[j] = sc->min_dist_tol ? (T)sc->min_dist_tol[j] : (T)1000.0f;
         }
+        if (sc->obb) {
+            obb.resize(15 * M);
+            for (int k = 0; k < 15 * M; ++k) obb[k] = (T)sc->obb[k];
+        }
     }
 };
+
+// Capsule-OBB distance (build extension, no reference counterpart): both end
+// points are expressed in the box frame, local_k = sum_i R[i][k] (X_i - c_i)
+// summed left to right, the capsule-AABB tree runs against [-h, h], and the
+// nearest points come back to the world frame as c_i + sum_k R[i][k] local_k.
+template <typename T>
+T capsule_obb(const T* A, const T* B, T r, const T* obb, const P<T>& pr, T* np, int* np_valid) {
+    const T* c = obb;
+    const T* R = obb + 3;
+    const T* h = obb + 12;
+    T a[3], b[3];
+    const T va[3] = {A[0] - c[0], A[1] - c[1], A[2] - c[2]};
+    const T vb[3] = {B[0] - c[0], B[1] - c[1], B[2] - c[2]};
+    for (int k = 0; k < 3; ++k) {
+        a[k] = R[k] * va[0] + R[3 + k] * va[1] + R[6 + k] * va[2];
+        b[k] = R[k] * vb[0] + R[3 + k] * vb[1] + R[6 + k] * vb[2];
+    }
+    const T box[6] = {-h[0], -h[1], -h[2], h[0], h[1], h[2]};
+    T loc[6];
+    const T d = capsule_box(a, b, r, box, pr, loc, np_valid, nullptr);
+    if (!*np_valid) { np_nan(np); return d; }
+    for (int e = 0; e < 2; ++e)
+        for (int i = 0; i < 3; ++i)
+            np[3 * e + i] = c[i] + (R[3 * i] * loc[3 * e] + R[3 * i + 1] * loc[3 * e + 1] + R[3 * i + 2] * loc[3 * e + 2]);
+    return d;
+}
+
+// Collision tree against an oriented box, in the box frame.
+template <typename T>
+bool collision_obb(const T* A, const T* B, T r, const T* obb, const P<T>& pr) {
+    const T* c = obb;
+    const T* R = obb + 3;
+    const T* h = obb + 12;
+    T a[3], b[3];
+    const T va[3] = {A[0] - c[0], A[1] - c[1], A[2] - c[2]};
+    const T vb[3] = {B[0] - c[0], B[1] - c[1], B[2] - c[2]};
+    for (int k = 0; k < 3; ++k) {
+        a[k] = R[k] * va[0] + R[3 + k] * va[1] + R[6 + k] * va[2];
+        b[k] = R[k] * vb[0] + R[3 + k] * vb[1] + R[6 + k] * vb[2];
+    }
+    const T box[6] = {-h[0], -h[1], -h[2], h[0], h[1], h[2]};
+    return collision_box(a, b, r, box, pr);
+}
 
 // computeDistance: RealVectorSpace.cpp:318-392 (fresh state, compute_again)
 template <typename T>
@@ -636,6 +684,8 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
             if (sc.ground[j] && i < rb->ground_included) {
                 d_tmp = pr.maxd;
                 np[0] = np[1] = np[2] = 0; np[3] = 0; np[4] = 0; np[5] = -pr.maxd;
+            } else if (!sc.obb.empty()) {
+                d_tmp = capsule_obb(A, B, r, &sc.obb[15 * j], pr, np, &v);
             } else {
                 d_tmp = capsule_box(A, B, r, &sc.obs[6 * j], pr, np, &v, nullptr);
             }
@@ -661,7 +711,10 @@ bool valid_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, const T
     for (int i = 0; i < rb->num_links; ++i)
         for (int j = 0; j < sc.M; ++j) {
             if (sc.ground[j] && i < rb->ground_included) continue;
-            if (collision_box(Pk + 3 * i, Pk + 3 * (i + 1), (T)rb->radius[i], &sc.obs[6 * j], pr)) return false;
+            const bool col = sc.obb.empty()
+                                 ? collision_box(Pk + 3 * i, Pk + 3 * (i + 1), (T)rb->radius[i], &sc.obs[6 * j], pr)
+                                 : collision_obb(Pk + 3 * i, Pk + 3 * (i + 1), (T)rb->radius[i], &sc.obb[15 * j], pr);
+            if (col) return false;
         }
     return true;
 }
@@ -996,6 +1049,34 @@ void orc_edge_first_collision32(const orc_robot* rb, const orc_params* p, const 
                 if (bad) { hit = k; break; }
             }
             first[c] = hit;
+        }
+    });
+}
+
+float orc_capsule_obb32(const float A[3], const float B[3], float r, const float obb[15], const orc_params* p,
+                        float np[6], int32_t* np_valid) {
+    P<float> pr(p); int v; float d = capsule_obb(A, B, r, obb, pr, np, &v); *np_valid = v; return d;
+}
+double orc_capsule_obb64(const double A[3], const double B[3], double r, const double obb[15], const orc_params* p,
+                         double np[6], int32_t* np_valid) {
+    P<double> pr(p); int v; double d = capsule_obb(A, B, r, obb, pr, np, &v); *np_valid = v; return d;
+}
+
+// Smallest self-pair capsule distance per configuration (std::min order
+// over the pair list, starting from MAX_DISTANCE).
+void orc_self_min32(const orc_robot* rb, const orc_params* p, const float* q, int64_t n, const int32_t* pairs,
+                    int32_t npairs, float* out, int32_t nthreads) {
+    P<float> pr(p);
+    std::vector<Seg> segs = build_segments(rb);
+    const int L = rb->num_links, D = rb->num_dofs;
+    parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+        std::vector<float> Pk(3 * (L + 1));
+        for (int64_t c = b; c < e; ++c) {
+            skeleton<float>(rb, segs, q + c * D, Pk.data());
+            float m = pr.maxd;
+            for (int k = 0; k < npairs; ++k)
+                m = stdmin(m, capsules_distance(rb, pr, Pk.data(), pairs[2 * k], pairs[2 * k + 1]));
+            out[c] = m;
         }
     });
 }

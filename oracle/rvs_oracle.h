@@ -59,6 +59,8 @@ typedef struct orc_scene {
     const float* obs;          /* num_obj x 6 : xmin ymin zmin xmax ymax zmax */
     const uint8_t* is_ground;  /* num_obj */
     const float* min_dist_tol; /* num_obj */
+    const float* obb;          /* NULL, or num_obj x 15: centre, rotation rows, half extents
+                                  (oriented-box extension: capsules are moved into the box frame) */
 } orc_scene;
 
 /* ---- scene ---- */
@@ -110,6 +112,14 @@ void orc_edge_first_collision32(const orc_robot* rb, const orc_params* p, const 
                                 const int32_t* pairs, int32_t npairs, float step,
                                 const float* q1, const float* dir, const int32_t* K, int64_t E,
                                 int32_t* first, int32_t nthreads);
+
+/* oriented-box extension: obb = centre(3), rotation rows(9), half extents(3) */
+float orc_capsule_obb32(const float A[3], const float B[3], float r, const float obb[15], const orc_params* p,
+                        float np[6], int32_t* np_valid);
+double orc_capsule_obb64(const double A[3], const double B[3], double r, const double obb[15], const orc_params* p,
+                         double np[6], int32_t* np_valid);
+void orc_self_min32(const orc_robot* rb, const orc_params* p, const float* q, int64_t n, const int32_t* pairs,
+                    int32_t npairs, float* out, int32_t nthreads);
 
 /* ---- independent checker: exact segment-AABB distance in double ---- */
 double orc_true_seg_box(const double A[3], const double B[3], const double obs[6]);

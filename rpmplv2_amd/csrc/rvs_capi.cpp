@@ -18,6 +18,7 @@
 
 #include "../../include/rvs.h"
 #include "rvs_internal.hpp"
+#include "rvs_stage.hpp"
 
 using namespace rvs;
 
@@ -83,6 +84,17 @@ rvs_status fail(rvs_context* c, rvs_status s, const std::string& msg) {
 rvs_status hip_fail(rvs_context* c, hipError_t e, const char* what) {
     return fail(c, e == hipErrorOutOfMemory ? RVS_ENOMEM : RVS_EDEVICE,
                 std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Workgroup LDS the kernels may ask for.
+constexpr size_t kLdsLimit = 64 * 1024;
+
+// The obstacle set must fit the LDS image of the kernels (oriented boxes
+// take 48 more bytes each than axis-aligned ones).
+rvs_status env_fits(rvs_context* c, const EnvDev& env, bool fp64, const char* what) {
+    const size_t b = fp64 ? staged_bytes<double>(env) : staged_bytes<float>(env);
+    if (b > kLdsLimit) return fail(c, RVS_ERANGE, std::string(what) + ": obstacle set too large for LDS staging");
+    return RVS_OK;
 }
 
 // ---- host restatement of the joint-origin constants (URDF rpy -> quaternion
@@ -449,13 +461,14 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     std::lock_guard<std::mutex> lk(c->mu);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
-    if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_distance: oriented-box environments need rvs_distance_obb");
     int prec = prec_of(c, flags);
+    s = env_fits(c, ev.dev, prec == RVS_PREC_FP64, "rvs_distance");
+    if (s != RVS_OK) return s;
     // nearest points come from the reference-exact pass
     if (prec == RVS_PREC_FP32_REFINE && nearest) prec = RVS_PREC_REF32;
     if (n == 0) return RVS_OK;
     if (prec == RVS_PREC_FP32_REFINE && n > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_distance: batch too large");
-    if (prec == RVS_PREC_FP32_REFINE && refine_lds_bytes(rb.dev, ev.dev) > 64 * 1024) prec = RVS_PREC_REF32;
+    if (prec == RVS_PREC_FP32_REFINE && refine_lds_bytes(rb.dev, ev.dev) > kLdsLimit) prec = RVS_PREC_REF32;
     (void)hipSetDevice(c->opt.device);
     const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
     const QueryParams qp = params_of(c);
@@ -532,7 +545,8 @@ rvs_status rvs_distance_f64(rvs_context* c, int32_t robot, int32_t env, const fl
     std::lock_guard<std::mutex> lk(c->mu);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
-    if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_distance_f64: oriented-box environments not supported");
+    s = env_fits(c, ev.dev, true, "rvs_distance_f64");
+    if (s != RVS_OK) return s;
     if (n == 0) return RVS_OK;
     (void)hipSetDevice(c->opt.device);
     const int L = rb.num_links, D = rb.num_dofs;
@@ -573,11 +587,12 @@ rvs_status rvs_is_valid(rvs_context* c, int32_t robot, int32_t env, const float*
     std::lock_guard<std::mutex> lk(c->mu);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
-    if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_is_valid: oriented-box environments not supported");
+    const int fp64 = prec_of(c, flags) == RVS_PREC_FP64;
+    s = env_fits(c, ev.dev, fp64, "rvs_is_valid");
+    if (s != RVS_OK) return s;
     if (n == 0) return RVS_OK;
     (void)hipSetDevice(c->opt.device);
     const int D = rb.num_dofs;
-    const int fp64 = prec_of(c, flags) == RVS_PREC_FP64;
     const QueryParams qp = params_of(c);
     hipError_t e = hipSuccess;
     const float* qd = q;
@@ -667,10 +682,45 @@ rvs_status rvs_self_distance(rvs_context* c, int32_t robot, const float* q, int6
         e = h2d(qs, q, (size_t)n * D * 4, c->stream);
         qd = qs;
     }
-    if (e == hipSuccess) e = launch_self_distance(rb.dev, qp, qd, n, od, nullptr, c->stream);
+    if (e == hipSuccess) e = launch_self_distance(rb.dev, qp, qd, n, od, nullptr, nullptr, nullptr, c->stream);
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(dist, od, (size_t)n * P * 4, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_self_distance");
+    return RVS_OK;
+}
+
+rvs_status rvs_self_collision(rvs_context* c, int32_t robot, const float* q, int64_t n, float* min_dist,
+                              uint8_t* free_flag, uint32_t flags) {
+    rvs_status s = check_query(c, robot, -1, q, n, false);
+    if (s != RVS_OK) return s;
+    if (n > 0 && !min_dist && !free_flag) return fail(c, RVS_EINVAL, "rvs_self_collision: no output");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    if (n == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const int D = rb.num_dofs;
+    const QueryParams qp = params_of(c);
+    hipError_t e = hipSuccess;
+    const float* qd = q;
+    float* md = min_dist;
+    uint8_t* fd = free_flag;
+    if (!(flags & RVS_F_DEVICE)) {
+        e = c->arena.ensure(Carve::need({(size_t)n * D * 4, min_dist ? (size_t)n * 4 : 0, free_flag ? (size_t)n : 0}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_self_collision: staging");
+        Carve cv(c->arena.p);
+        float* qs = cv.take<float>((size_t)n * D);
+        md = min_dist ? cv.take<float>(n) : nullptr;
+        fd = free_flag ? cv.take<uint8_t>(n) : nullptr;
+        e = h2d(qs, q, (size_t)n * D * 4, c->stream);
+        qd = qs;
+    }
+    if (e == hipSuccess) e = launch_self_distance(rb.dev, qp, qd, n, nullptr, nullptr, md, fd, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
+        e = d2h(min_dist, md, min_dist ? (size_t)n * 4 : 0, c->stream);
+        if (e == hipSuccess) e = d2h(free_flag, fd, free_flag ? (size_t)n : 0, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_self_collision");
     return RVS_OK;
 }
 
@@ -682,12 +732,13 @@ rvs_status rvs_edge_is_valid(rvs_context* c, int32_t robot, int32_t env, const f
     std::lock_guard<std::mutex> lk(c->mu);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
-    if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_edge_is_valid: oriented-box environments not supported");
     if (prec_of(c, flags) == RVS_PREC_FP64) return fail(c, RVS_ENOTSUP, "rvs_edge_is_valid: reference arithmetic only");
     if (n == 0) return RVS_OK;
     if (c->opt.num_checks <= 0) return fail(c, RVS_EINVAL, "rvs_edge_is_valid: num_checks must be positive");
-    (void)hipSetDevice(c->opt.device);
     const int D = rb.num_dofs;
+    if (((staged_bytes<float>(ev.dev) + 15) & ~size_t(15)) + (size_t)D * 256 * 4 > kLdsLimit)
+        return fail(c, RVS_ERANGE, "rvs_edge_is_valid: obstacle set too large for LDS staging");
+    (void)hipSetDevice(c->opt.device);
     const QueryParams qp = params_of(c);
     hipError_t e = hipSuccess;
     const float *a = q1, *b = q2;
@@ -722,7 +773,6 @@ rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* 
     std::lock_guard<std::mutex> lk(c->mu);
     if (!env_ok(c, env)) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: bad env id");
     const EnvRec& ev = c->envs[env];
-    if (ev.dev.use_obb) return fail(c, RVS_ENOTSUP, "rvs_edge_first_collision: oriented-box environments not supported");
     if (prec_of(c, flags) == RVS_PREC_FP64)
         return fail(c, RVS_ENOTSUP, "rvs_edge_first_collision: reference arithmetic only");
     if (num_edges == 0) return RVS_OK;
@@ -775,7 +825,7 @@ rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* 
     }
     gr.item_begin[gr.num] = (int32_t)items;
     const size_t lds = edge_first_lds_bytes(gr, ev.dev);
-    if (lds > 64 * 1024) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: robots too large for LDS staging");
+    if (lds > kLdsLimit) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: robots too large for LDS staging");
     (void)hipSetDevice(c->opt.device);
     const QueryParams qp = params_of(c);
     const size_t rows = (size_t)num_edges * row_stride;

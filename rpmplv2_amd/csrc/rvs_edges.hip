@@ -68,6 +68,7 @@ __device__ inline float vecxf_norm_diff(const float* a, const float* b, int n) {
 // configuration held in LDS column 'lane' (joint k at qcol[k * stride]).
 // Skeleton points are written to skel (point k, coordinate a at
 // skel[(3k + a) * stride]) when skel is not null.
+template <bool OBB>
 __device__ inline bool env_free(const RobotDev& rb, const Staged<float>& s, int M, const Params<float>& p,
                                 const float* qcol, int stride, float* skel) {
     FKStream fk;
@@ -86,19 +87,25 @@ __device__ inline bool env_free(const RobotDev& rb, const Staged<float>& s, int 
         for (int j = 0; j < M; ++j) {
             if (s.gnd[j] && i < rb.ground_included) continue;
             const Box<float> bx = s.box[j];
-            if (collision_box(P0, P1, r, rn, bx, p)) return false;
+            if (OBB) {
+                const float* x = s.xf + 12 * j;
+                if (collision_box(to_box(x, P0), to_box(x, P1), r, rn, bx, p)) return false;
+            } else if (collision_box(P0, P1, r, rn, bx, p)) {
+                return false;
+            }
         }
         P0 = P1;
     }
     return true;
 }
 
+template <bool OBB>
 __global__ __launch_bounds__(256) void k_edge_valid(RobotDev rb, EnvDev env, QueryParams qp,
                                                     const float* __restrict__ q1, const float* __restrict__ q2,
                                                     int64_t n, uint8_t* valid) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<float> s = stage_env<float>(env, smem);
-    float* qbuf = reinterpret_cast<float*>(smem + ((staged_bytes<float>(env.num_obj) + 15) & ~size_t(15)));
+    const Staged<float> s = stage_env<float, OBB>(env, smem);
+    float* qbuf = reinterpret_cast<float*>(smem + ((staged_bytes<float>(env) + 15) & ~size_t(15)));
     const int K = qp.num_checks;
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n * K) return;
@@ -119,7 +126,7 @@ __global__ __launch_bounds__(256) void k_edge_valid(RobotDev rb, EnvDev env, Que
         for (int i = 0; i < D; ++i) qbuf[i * B + t] = b[i];
     }
     const Params<float> p{qp.eq, qp.maxd};
-    if (!env_free(rb, s, env.num_obj, p, qbuf + t, B, nullptr)) valid[e] = 0;
+    if (!env_free<OBB>(rb, s, env.num_obj, p, qbuf + t, B, nullptr)) valid[e] = 0;
 }
 
 __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
@@ -149,14 +156,15 @@ __device__ inline int round_of(const int32_t* off, int nrounds, int32_t s) {
     return lo;
 }
 
+template <bool OBB>
 __global__ __launch_bounds__(64) void k_edge_first(EdgeGroups gr, EnvDev env, QueryParams qp,
                                                    const float* __restrict__ q1, const float* __restrict__ dir,
                                                    int32_t row_stride, float step, const int32_t* __restrict__ order,
                                                    const int32_t* __restrict__ rounds, int32_t* first,
                                                    uint32_t* counter, int32_t total_items) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<float> s = stage_env<float>(env, smem);
-    const size_t base = (staged_bytes<float>(env.num_obj) + 15) & ~size_t(15);
+    const Staged<float> s = stage_env<float, OBB>(env, smem);
+    const size_t base = (staged_bytes<float>(env) + 15) & ~size_t(15);
     float* qbuf = reinterpret_cast<float*>(smem + base);
     float* skel = qbuf + gr.max_dofs * 64;
     const int lane = threadIdx.x;
@@ -179,7 +187,7 @@ __global__ __launch_bounds__(64) void k_edge_first(EdgeGroups gr, EnvDev env, Qu
         const float sk = step * (float)k;
         for (int i = 0; i < D; ++i)
             qbuf[i * 64 + lane] = q1[(int64_t)e * row_stride + i] + sk * dir[(int64_t)e * row_stride + i];
-        bool bad = !env_free(rb, s, env.num_obj, p, qbuf + lane, 64, rb.num_pairs ? skel + lane : nullptr);
+        bool bad = !env_free<OBB>(rb, s, env.num_obj, p, qbuf + lane, 64, rb.num_pairs ? skel + lane : nullptr);
         for (int m = 0; !bad && m < rb.num_pairs; ++m) {
             const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
             auto pt = [&](int kk) {
@@ -202,13 +210,18 @@ hipError_t launch_edge_valid(const RobotDev& rb, const EnvDev& env, const QueryP
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u8, blocks_for(n, 256), dim3(256), 0, st, valid, n, (uint8_t)1);
     const int B = 256;
-    const size_t sm = ((staged_bytes<float>(env.num_obj) + 15) & ~size_t(15)) + (size_t)rb.num_dofs * B * 4;
-    hipLaunchKernelGGL(k_edge_valid, blocks_for(n * qp.num_checks, B), dim3(B), sm, st, rb, env, qp, q1, q2, n, valid);
+    const size_t sm = ((staged_bytes<float>(env) + 15) & ~size_t(15)) + (size_t)rb.num_dofs * B * 4;
+    if (env.use_obb)
+        hipLaunchKernelGGL(k_edge_valid<true>, blocks_for(n * qp.num_checks, B), dim3(B), sm, st, rb, env, qp, q1, q2,
+                           n, valid);
+    else
+        hipLaunchKernelGGL(k_edge_valid<false>, blocks_for(n * qp.num_checks, B), dim3(B), sm, st, rb, env, qp, q1, q2,
+                           n, valid);
     return hipGetLastError();
 }
 
 size_t edge_first_lds_bytes(const EdgeGroups& gr, const EnvDev& env) {
-    return ((staged_bytes<float>(env.num_obj) + 15) & ~size_t(15)) +
+    return ((staged_bytes<float>(env) + 15) & ~size_t(15)) +
            (size_t)64 * 4 * (gr.max_dofs + 3 * (gr.max_links + 1));
 }
 
@@ -220,8 +233,13 @@ hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const Quer
     hipLaunchKernelGGL(k_first_init, blocks_for(num_edges, 256), dim3(256), 0, st, first, num_edges, counter);
     if (total_items > 0) {
         const int grid = std::max(1, std::min(total_items, waves));
-        hipLaunchKernelGGL(k_edge_first, dim3(grid), dim3(64), edge_first_lds_bytes(gr, env), st, gr, env, qp, q1, dir,
-                           row_stride, step, order, rounds, first, counter, total_items);
+        const size_t sm = edge_first_lds_bytes(gr, env);
+        if (env.use_obb)
+            hipLaunchKernelGGL(k_edge_first<true>, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride, step,
+                               order, rounds, first, counter, total_items);
+        else
+            hipLaunchKernelGGL(k_edge_first<false>, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride, step,
+                               order, rounds, first, counter, total_items);
     }
     hipLaunchKernelGGL(k_first_final, blocks_for(num_edges, 256), dim3(256), 0, st, first, num_edges);
     return hipGetLastError();

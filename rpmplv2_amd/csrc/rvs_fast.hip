@@ -113,11 +113,12 @@ __device__ __attribute__((always_inline)) inline Vec3<float> skelf_point(FKf& s,
     return P;
 }
 
+template <bool OBB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
     uint32_t* count) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<float> s = stage_env<float>(env, smem);
+    const Staged<float> s = stage_env<float, OBB>(env, smem);
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const int M = env.num_obj;
@@ -147,7 +148,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             } else {
                 const Box<float> bx = s.box[j];
                 PairInfo<float> info{p.maxd, p.maxd, -1.0f, 1.0f, false, false};
-                dt = capsule_box<float, false, FastMath>(P0, P1, r, bx, p, nullptr, &info, nullptr);
+                if (OBB) {
+                    // oriented box: solved in the box frame (the frame change
+                    // adds a few ulp of the lever arm, far inside the band)
+                    const float* x = s.xf + 12 * j;
+                    dt = capsule_box<float, false, FastMath>(to_box(x, P0), to_box(x, P1), r, bx, p, nullptr, &info,
+                                                             nullptr);
+                } else {
+                    dt = capsule_box<float, false, FastMath>(P0, P1, r, bx, p, nullptr, &info, nullptr);
+                }
                 const float tol = s.tol[j];
                 // inside points and crossings are decided by comparisons: the
                 // result (0 or -r) is exact unless the decision is marginal
@@ -201,8 +210,13 @@ hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st) {
 hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                 int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_distance_fast, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock),
-                       staged_bytes<float>(env.num_obj), st, rb, env, qp, q, n, out, idx, count);
+    const dim3 g((unsigned)((n + kBlock - 1) / kBlock)), b(kBlock);
+    if (env.use_obb)
+        hipLaunchKernelGGL(k_distance_fast<true>, g, b, staged_bytes<float>(env), st, rb, env, qp, q, n, out, idx,
+                           count);
+    else
+        hipLaunchKernelGGL(k_distance_fast<false>, g, b, staged_bytes<float>(env), st, rb, env, qp, q, n, out, idx,
+                           count);
     return hipGetLastError();
 }
 

@@ -87,7 +87,9 @@ hipError_t launch_is_valid(const RobotDev& rb, const EnvDev& env, const QueryPar
 hipError_t launch_underestimation(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                   int64_t n, const float* np_ref, int64_t np_stride, float* d, float* profile,
                                   uint8_t* stored, hipStream_t st);
+// Per-pair distances (dist / dist64, [N][pairs]) and/or the per-configuration
+// minimum and collision-free flag (min_out / free_out); null outputs are skipped.
 hipError_t launch_self_distance(const RobotDev& rb, const QueryParams& qp, const float* q, int64_t n, float* dist,
-                                double* dist64, hipStream_t st);
+                                double* dist64, float* min_out, uint8_t* free_out, hipStream_t st);
 
 }  // namespace rvs

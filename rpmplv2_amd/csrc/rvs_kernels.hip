@@ -43,14 +43,14 @@ constexpr int kBlock = 256;
 // the distance is 0 and the nearest points are dropped).
 // With an index list, thread g handles configuration idx[g] for g < *count
 // (the refinement pass of RVS_PREC_FP32_REFINE).
-template <typename T, bool WANT_NP>
+template <typename T, bool WANT_NP, bool OBB>
 __device__ __attribute__((always_inline)) inline void distance_body(const RobotDev& rb, const EnvDev& env,
                                                                     const QueryParams& qp, const float* __restrict__ q,
                                                                     int64_t n, const DistOut& out,
                                                                     const int32_t* idx = nullptr,
                                                                     const uint32_t* count = nullptr) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<T> s = stage_env<T>(env, smem);
+    const Staged<T> s = stage_env<T, OBB>(env, smem);
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx) {
         if (c >= (int64_t)*count) return;
@@ -81,6 +81,15 @@ __device__ __attribute__((always_inline)) inline void distance_body(const RobotD
                     np.r = Vec3<T>{T(0), T(0), T(0)};
                     np.o = Vec3<T>{T(0), T(0), -p.maxd};
                     np.valid = true;
+                }
+            } else if (OBB) {
+                // oriented box: the pair is solved in the box frame
+                const float* x = s.xf + 12 * j;
+                const Box<T> bx = s.box[j];
+                dt = capsule_box<T, WANT_NP>(to_box(x, P0), to_box(x, P1), r, bx, p, &np, nullptr, nullptr);
+                if (WANT_NP && np.valid) {
+                    np.r = from_box(x, np.r);
+                    np.o = from_box(x, np.o);
                 }
             } else {
                 const Box<T> bx = s.box[j];
@@ -123,10 +132,10 @@ __device__ __attribute__((always_inline)) inline void distance_body(const RobotD
 
 // The float kernel is held to 128 VGPRs (4 waves per SIMD); the double
 // kernel keeps its natural allocation (a cap would spill).
-template <bool WANT_NP>
+template <bool WANT_NP, bool OBB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance32(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out) {
-    distance_body<float, WANT_NP>(rb, env, qp, q, n, out);
+    distance_body<float, WANT_NP, OBB>(rb, env, qp, q, n, out);
 }
 
 // Exact recomputation of the configurations the FP32 bulk pass flagged: one
@@ -137,13 +146,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // in RealVectorSpace::computeDistance.  A few thousand flagged configurations
 // then finish in about one configuration's latency instead of one wave's
 // whole sequential walk.
+template <bool OBB>
 __global__ __launch_bounds__(64) void k_distance_refine(RobotDev rb, EnvDev env, QueryParams qp,
                                                         const float* __restrict__ q, DistOut out,
                                                         const int32_t* __restrict__ idx,
                                                         const uint32_t* __restrict__ count) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<float> s = stage_env<float>(env, smem);
-    float* f = reinterpret_cast<float*>(smem + ((staged_bytes<float>(env.num_obj) + 15) & ~size_t(15)));
+    const Staged<float> s = stage_env<float, OBB>(env, smem);
+    float* f = reinterpret_cast<float*>(smem + ((staged_bytes<float>(env) + 15) & ~size_t(15)));
     const int M = env.num_obj, L = rb.num_links, P = L * M;
     const int t = threadIdx.x;
     const Params<float> p{qp.eq, qp.maxd};
@@ -167,6 +177,11 @@ __global__ __launch_bounds__(64) void k_distance_refine(RobotDev rb, EnvDev env,
             float dt;
             if (s.gnd[j] && i < rb.ground_included) {
                 dt = p.maxd;
+            } else if (OBB) {
+                const float* x = s.xf + 12 * j;
+                const Box<float> bx = s.box[j];
+                dt = capsule_box<float, false>(to_box(x, P0), to_box(x, P1), rb.radius[i], bx, p, nullptr, nullptr,
+                                               nullptr);
             } else {
                 const Box<float> bx = s.box[j];
                 dt = capsule_box<float, false>(P0, P1, rb.radius[i], bx, p, nullptr, nullptr, nullptr);
@@ -197,21 +212,21 @@ __global__ __launch_bounds__(64) void k_distance_refine(RobotDev rb, EnvDev env,
     }
 }
 
-template <bool WANT_NP>
+template <bool WANT_NP, bool OBB>
 __global__ __launch_bounds__(kBlock) void k_distance64(RobotDev rb, EnvDev env, QueryParams qp,
                                                        const float* __restrict__ q, int64_t n, DistOut out) {
-    distance_body<double, WANT_NP>(rb, env, qp, q, n, out);
+    distance_body<double, WANT_NP, OBB>(rb, env, qp, q, n, out);
 }
 
 // ---------------------------------------------------------------- isValid
 // RealVectorSpace::isValid(q) (RealVectorSpace.cpp:277-309): collision tree
 // per pair, ground skipped for the first ground_included links, stop on the
 // first hit.
-template <typename T>
+template <typename T, bool OBB>
 __global__ __launch_bounds__(kBlock) void k_valid(RobotDev rb, EnvDev env, QueryParams qp,
                                                   const float* __restrict__ q, int64_t n, uint8_t* valid) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<T> s = stage_env<T>(env, smem);
+    const Staged<T> s = stage_env<T, OBB>(env, smem);
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const int M = env.num_obj;
@@ -228,7 +243,14 @@ __global__ __launch_bounds__(kBlock) void k_valid(RobotDev rb, EnvDev env, Query
         const T rn = rnew_of<T>(rb, i);
         for (int j = 0; j < M; ++j) {
             if (s.gnd[j] && i < rb.ground_included) continue;
-            if (collision_box(P0, P1, r, rn, s.box[j], p)) { ok = false; break; }
+            bool col;
+            if (OBB) {
+                const float* x = s.xf + 12 * j;
+                col = collision_box(to_box(x, P0), to_box(x, P1), r, rn, s.box[j], p);
+            } else {
+                col = collision_box(P0, P1, r, rn, s.box[j], p);
+            }
+            if (col) { ok = false; break; }
         }
         P0 = P1;
     }
@@ -289,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_under(RobotDev rb, EnvDev env, Query
 // per lane.
 template <typename T>
 __global__ __launch_bounds__(128) void k_self(RobotDev rb, QueryParams qp, const float* __restrict__ q, int64_t n,
-                                              float* dist, double* dist64) {
+                                              float* dist, double* dist64, float* min_out, uint8_t* free_out) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* sk = reinterpret_cast<T*>(smem);
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -307,13 +329,19 @@ __global__ __launch_bounds__(128) void k_self(RobotDev rb, QueryParams qp, const
         sk[(3 * k + 2) * B + t] = P.z;
     }
     auto pt = [&](int k) { return Vec3<T>{sk[(3 * k) * B + t], sk[(3 * k + 1) * B + t], sk[(3 * k + 2) * B + t]}; };
+    // smallest pair distance in std::min order from MAX_DISTANCE (a NaN pair
+    // distance never replaces the running minimum); colliding = some pair < 0
+    T lo = p.maxd;
     for (int m = 0; m < rb.num_pairs; ++m) {
         const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
         const T sd = seg_seg<T, false>(pt(a), pt(a + 1), pt(b), pt(b + 1), p, nullptr);
         const T v = sd - (T)rb.radius[a] - (T)rb.radius[b];
         if (dist) dist[c * rb.num_pairs + m] = (float)v;
         if (dist64) dist64[c * rb.num_pairs + m] = (double)v;
+        lo = smin(lo, v);
     }
+    if (min_out) min_out[c] = (float)lo;
+    if (free_out) free_out[c] = lo < 0 ? 0 : 1;
 }
 
 inline dim3 grid_for(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
@@ -323,27 +351,35 @@ inline dim3 grid_for(int64_t n, int block) { return dim3((unsigned)((n + block -
 hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                  int64_t n, const DistOut& out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    const size_t sm = staged_bytes<float>(env.num_obj);
-    if (out.nearest)
-        hipLaunchKernelGGL((k_distance32<true>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
-    else
-        hipLaunchKernelGGL((k_distance32<false>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+    const size_t sm = staged_bytes<float>(env);
+    const dim3 g = grid_for(n, kBlock), b(kBlock);
+    if (env.use_obb) {
+        if (out.nearest) hipLaunchKernelGGL((k_distance32<true, true>), g, b, sm, st, rb, env, qp, q, n, out);
+        else hipLaunchKernelGGL((k_distance32<false, true>), g, b, sm, st, rb, env, qp, q, n, out);
+    } else {
+        if (out.nearest) hipLaunchKernelGGL((k_distance32<true, false>), g, b, sm, st, rb, env, qp, q, n, out);
+        else hipLaunchKernelGGL((k_distance32<false, false>), g, b, sm, st, rb, env, qp, q, n, out);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                 int64_t n, const DistOut& out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    const size_t sm = staged_bytes<double>(env.num_obj);
-    if (out.nearest)
-        hipLaunchKernelGGL((k_distance64<true>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
-    else
-        hipLaunchKernelGGL((k_distance64<false>), grid_for(n, kBlock), dim3(kBlock), sm, st, rb, env, qp, q, n, out);
+    const size_t sm = staged_bytes<double>(env);
+    const dim3 g = grid_for(n, kBlock), b(kBlock);
+    if (env.use_obb) {
+        if (out.nearest) hipLaunchKernelGGL((k_distance64<true, true>), g, b, sm, st, rb, env, qp, q, n, out);
+        else hipLaunchKernelGGL((k_distance64<false, true>), g, b, sm, st, rb, env, qp, q, n, out);
+    } else {
+        if (out.nearest) hipLaunchKernelGGL((k_distance64<true, false>), g, b, sm, st, rb, env, qp, q, n, out);
+        else hipLaunchKernelGGL((k_distance64<false, false>), g, b, sm, st, rb, env, qp, q, n, out);
+    }
     return hipGetLastError();
 }
 
 size_t refine_lds_bytes(const RobotDev& rb, const EnvDev& env) {
-    return ((staged_bytes<float>(env.num_obj) + 15) & ~size_t(15)) + (size_t)rb.num_links * env.num_obj * 4;
+    return ((staged_bytes<float>(env) + 15) & ~size_t(15)) + (size_t)rb.num_links * env.num_obj * 4;
 }
 
 hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
@@ -353,20 +389,28 @@ hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const Q
     // enough waves to cover a typical flagged set in one round; the kernel
     // loops when more configurations are flagged
     const unsigned grid = (unsigned)std::min<int64_t>(n, 8192);
-    hipLaunchKernelGGL(k_distance_refine, dim3(grid), dim3(64), refine_lds_bytes(rb, env), st, rb, env, qp, q, out,
-                       idx, count);
+    if (env.use_obb)
+        hipLaunchKernelGGL(k_distance_refine<true>, dim3(grid), dim3(64), refine_lds_bytes(rb, env), st, rb, env, qp,
+                           q, out, idx, count);
+    else
+        hipLaunchKernelGGL(k_distance_refine<false>, dim3(grid), dim3(64), refine_lds_bytes(rb, env), st, rb, env, qp,
+                           q, out, idx, count);
     return hipGetLastError();
 }
 
 hipError_t launch_is_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
                            uint8_t* valid, int fp64, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (fp64)
-        hipLaunchKernelGGL((k_valid<double>), grid_for(n, kBlock), dim3(kBlock), staged_bytes<double>(env.num_obj), st,
-                           rb, env, qp, q, n, valid);
-    else
-        hipLaunchKernelGGL((k_valid<float>), grid_for(n, kBlock), dim3(kBlock), staged_bytes<float>(env.num_obj), st,
-                           rb, env, qp, q, n, valid);
+    const dim3 g = grid_for(n, kBlock), b(kBlock);
+    if (fp64) {
+        const size_t sm = staged_bytes<double>(env);
+        if (env.use_obb) hipLaunchKernelGGL((k_valid<double, true>), g, b, sm, st, rb, env, qp, q, n, valid);
+        else hipLaunchKernelGGL((k_valid<double, false>), g, b, sm, st, rb, env, qp, q, n, valid);
+    } else {
+        const size_t sm = staged_bytes<float>(env);
+        if (env.use_obb) hipLaunchKernelGGL((k_valid<float, true>), g, b, sm, st, rb, env, qp, q, n, valid);
+        else hipLaunchKernelGGL((k_valid<float, false>), g, b, sm, st, rb, env, qp, q, n, valid);
+    }
     return hipGetLastError();
 }
 
@@ -380,15 +424,17 @@ hipError_t launch_underestimation(const RobotDev& rb, const EnvDev& env, const Q
 }
 
 hipError_t launch_self_distance(const RobotDev& rb, const QueryParams& qp, const float* q, int64_t n, float* dist,
-                                double* dist64, hipStream_t st) {
-    if (n <= 0 || rb.num_pairs == 0) return hipSuccess;
+                                double* dist64, float* min_out, uint8_t* free_out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
     const int B = 128;
     if (dist64) {
         const size_t sm = (size_t)3 * (rb.num_links + 1) * B * sizeof(double);
-        hipLaunchKernelGGL((k_self<double>), grid_for(n, B), dim3(B), sm, st, rb, qp, q, n, dist, dist64);
+        hipLaunchKernelGGL((k_self<double>), grid_for(n, B), dim3(B), sm, st, rb, qp, q, n, dist, dist64, min_out,
+                           free_out);
     } else {
         const size_t sm = (size_t)3 * (rb.num_links + 1) * B * sizeof(float);
-        hipLaunchKernelGGL((k_self<float>), grid_for(n, B), dim3(B), sm, st, rb, qp, q, n, dist, dist64);
+        hipLaunchKernelGGL((k_self<float>), grid_for(n, B), dim3(B), sm, st, rb, qp, q, n, dist, dist64, min_out,
+                           free_out);
     }
     return hipGetLastError();
 }

@@ -62,7 +62,7 @@ class Box(C.Structure):
 EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error", "rvs_stream", "rvs_last_refined",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds", "rvs_distance",
            "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_self_distance",
-           "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing"]
+           "rvs_self_collision", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing"]
 
 _lib = None
 
@@ -98,6 +98,7 @@ def lib():
         L.rvs_is_valid.argtypes = [ctx, i32, i32, vp, i64, vp, u32]
         L.rvs_distance_underestimation.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, vp, vp, u32]
         L.rvs_self_distance.argtypes = [ctx, i32, vp, i64, vp, u32]
+        L.rvs_self_collision.argtypes = [ctx, i32, vp, i64, vp, vp, u32]
         L.rvs_edge_is_valid.argtypes = [ctx, i32, i32, vp, vp, i64, vp, u32]
         L.rvs_edge_first_collision.argtypes = [ctx, i32, vp, vp, vp, vp, i64, i32, C.c_float, vp, u32]
         for name in EXPORTS:
@@ -271,6 +272,20 @@ class Engine:
         out = np.zeros((len(q), len(rb.self_pairs)), np.float32)
         self._check(lib().rvs_self_distance(self._ctx, robot, _ptr(q), len(q), _ptr(out), 0), "rvs_self_distance")
         return out
+
+    def self_collision(self, robot: int, q):
+        """(min pair distance [N], collision-free flag [N]) of the robot's self pairs."""
+        rb = self.robots[robot]
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+        m = np.zeros(len(q), np.float32)
+        f = np.zeros(len(q), np.uint8)
+        self._check(lib().rvs_self_collision(self._ctx, robot, _ptr(q), len(q), _ptr(m), _ptr(f), 0),
+                    "rvs_self_collision")
+        return m, f
+
+    def self_collision_device(self, robot: int, q, min_dist=None, free_flag=None):
+        self._check(lib().rvs_self_collision(self._ctx, robot, _ptr(q), q.shape[0], _ptr(min_dist), _ptr(free_flag),
+                                             F_DEVICE), "rvs_self_collision")
 
     # ---- queries on device tensors (torch CUDA tensors; no staging)
     def distance_device(self, robot: int, env: int, q, d, profile=None, verdict=None, nearest=None,

@@ -48,6 +48,22 @@ class Scene:
     def num_obj(self) -> int:
         return len(self.dim)
 
+    def obb_table(self) -> np.ndarray:
+        """[M][15]: centre, rotation rows, half extents - the oriented-box
+        table the engine builds (unit quaternion -> matrix in double)."""
+        q = self.quat.astype(np.float64)
+        w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+        nrm = np.sqrt(((w * w + x * x) + y * y) + z * z)   # left to right, as the C ABI does
+        w, x, y, z = w / nrm, x / nrm, y / nrm, z / nrm
+        R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                      2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                      2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1)
+        out = np.empty((self.num_obj, 15), np.float32)
+        out[:, :3] = self.pos
+        out[:, 3:12] = R.astype(np.float32)
+        out[:, 12:] = np.float32(0.5) * self.dim
+        return out
+
     def subset(self, idx) -> "Scene":
         idx = np.asarray(idx)
         return Scene(self.dim[idx], self.pos[idx], self.quat[idx], self.is_ground[idx], self.min_dist_tol[idx])

@@ -1,0 +1,150 @@
+"""GPU parity of the oriented-box extension, the reference's AABB-of-OBB
+semantics on the same rotated scene, and the per-configuration self-collision
+query, on the 22-DOF / 512-rotated-box workload (reduced batch sizes).
+
+Bar: verdicts identical to the oracle; float distances within 2 ulps (device
+and host double sine / cosine may differ in the last bit); FP64 within 1e-12 m;
+FP32 + refinement within 1e-5 m of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from rpmplv2_amd import robots as R, rvs, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def ulps(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    ia = a.view(np.int32).astype(np.int64)
+    ib = b.view(np.int32).astype(np.int64)
+    ia = np.where(ia < 0, -(ia & 0x7FFFFFFF), ia)
+    ib = np.where(ib < 0, -(ib & 0x7FFFFFFF), ib)
+    return np.abs(ia - ib)
+
+
+@pytest.fixture(scope="module")
+def w5():
+    return W.workload5(4096)
+
+
+@pytest.fixture(scope="module")
+def ids(engine, w5):
+    rid = engine.load_robot(w5.robot)
+    return rid, engine.create_env(w5.scene, rvs.ENV_OBB), engine.create_env(w5.scene, rvs.ENV_AABB)
+
+
+def test_obb_ref32(oracle, engine, w5, ids):
+    rid, eo, _ = ids
+    d, prof, ver, _ = engine.distance(rid, eo, w5.q)
+    od, oprof, over, _ = oracle.distance(w5.robot, oracle.scene_of(w5.scene, oriented=True), w5.q, 32)
+    print(f"obb ref32: free {ver.mean():.4f} mismatches {(ver != over).sum()}")
+    assert (ver == over).all()
+    assert 0 < ver.mean() < 1
+    assert ulps(d, od).max() <= 2
+    assert ulps(prof, oprof).max() <= 2
+
+
+def test_aabb_of_obb_semantics(oracle, engine, w5, ids):
+    """Reference semantics on the rotated scene: each box is its FCL world AABB."""
+    rid, _, ea = ids
+    for prec in (rvs.PREC_REF32, rvs.PREC_FP32_REFINE):
+        d, prof, ver, _ = engine.distance(rid, ea, w5.q, precision=prec)
+        od, oprof, over, _ = oracle.distance(w5.robot, oracle.scene_of(w5.scene), w5.q, 32)
+        assert (ver == over).all()
+        assert np.abs(d - od).max() <= 1e-5
+
+
+def test_obb_fp64(oracle, engine, w5, ids):
+    rid, eo, _ = ids
+    d, prof, ver = engine.distance_f64(rid, eo, w5.q)
+    od, oprof, over, _ = oracle.distance(w5.robot, oracle.scene_of(w5.scene, oriented=True), w5.q, 64)
+    assert (ver == over).all()
+    assert np.abs(d - od).max() <= 1e-12
+    assert np.abs(prof - oprof).max() <= 1e-12
+
+
+def test_obb_fp32_refine(oracle, engine, w5, ids):
+    rid, eo, _ = ids
+    d, prof, ver, _ = engine.distance(rid, eo, w5.q, precision=rvs.PREC_FP32_REFINE)
+    refined = engine.last_refined()
+    sc = oracle.scene_of(w5.scene, oriented=True)
+    od, oprof, over, _ = oracle.distance(w5.robot, sc, w5.q, 32)
+    d64, prof64, ver64, _ = oracle.distance(w5.robot, sc, w5.q, 64)
+    print(f"obb fp32r: refined {refined} of {len(w5.q)}; mismatches {(ver != over).sum()}")
+    assert (ver == over).all()
+    assert np.abs(d - d64).max() <= 1e-5
+    ok = (ver == 1) & (ver64 == 1)
+    assert np.abs(prof[ok] - prof64[ok]).max() <= 1e-5
+    assert np.abs(prof[ver == 0] - oprof[ver == 0]).max() <= 1e-5
+    assert refined < 0.05 * len(w5.q)
+
+
+def test_obb_nearest_points(oracle, engine, w5, ids):
+    rid, eo, _ = ids
+    q = w5.q[:256]
+    d, prof, ver, npo = engine.distance(rid, eo, q, want_nearest=True)
+    od, oprof, over, onp = oracle.distance(w5.robot, oracle.scene_of(w5.scene, oriented=True), q, 32, want_np=True)
+    assert (ver == over).all()
+    assert np.array_equal(np.isnan(npo), np.isnan(onp))
+    m = ~np.isnan(npo)
+    assert m.any()
+    assert np.abs(npo[m] - onp[m]).max() <= 1e-5
+
+
+def test_obb_is_valid_and_edges(oracle, engine, w5, ids):
+    rid, eo, _ = ids
+    sc = oracle.scene_of(w5.scene, oriented=True)
+    v = engine.is_valid(rid, eo, w5.q)
+    assert (v == oracle.is_valid(w5.robot, sc, w5.q)).all()
+    v64 = engine.is_valid(rid, eo, w5.q, precision=rvs.PREC_FP64)
+    assert (v64 == oracle.is_valid(w5.robot, sc, w5.q, 64)).all()
+    q1, q2 = w5.q[:1024], w5.q[1024:2048]
+    q2s = (q1 + 0.02 * (q2 - q1)).astype(np.float32)
+    ev = engine.edge_is_valid(rid, eo, q1, q2s)
+    assert (ev == oracle.edge_valid(w5.robot, sc, q1, q2s)).all()
+    rng = np.random.default_rng(21)
+    u = rng.normal(size=(512, 22))
+    u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+    K = rng.integers(1, 64, 512).astype(np.int32)
+    b = W.EdgeBatch(w5.robot, w5.q[:512], u, K)
+    f = engine.edge_batches(eo, [b])[0]
+    of = oracle.edge_first_collision(w5.robot, sc, b.q1, b.dir, b.K, b.step)
+    assert np.array_equal(f, of)
+
+
+def test_obb_identity_rotation_matches_aabb(engine):
+    """With identity rotations the oriented-box path must reproduce the
+    axis-aligned one.  The axis-aligned bounds are stored in float (c - h and
+    c + h each rounded, up to 6e-8 m at 1 m), the oriented box keeps c and h,
+    so FP64 distances agree to that rounding."""
+    rng = np.random.default_rng(31)
+    sc = W._boxes(rng, 24, 0.05, 0.3, 1.0, (0, 0, 0.4))
+    rb = R.xarm6()
+    q = W.uniform_configs(rng, 1 << 14, rb.limits)
+    rid = engine.load_robot(rb)
+    ea, eo = engine.create_env(sc, rvs.ENV_AABB), engine.create_env(sc, rvs.ENV_OBB)
+    da, pa, va = engine.distance_f64(rid, ea, q)
+    do, po, vo = engine.distance_f64(rid, eo, q)
+    assert (va == vo).mean() > 0.9999
+    assert np.abs(da - do)[(va == vo)].max() <= 2e-7
+
+
+def test_self_collision(oracle, engine, w5):
+    rb = w5.robot
+    rid = engine.load_robot(rb)
+    m, f = engine.self_collision(rid, w5.q)
+    om = oracle.self_min(rb, w5.q)
+    assert ulps(m, om).max() <= 2
+    assert np.array_equal(f, (om >= 0).astype(np.uint8))
+    # consistent with the per-pair query
+    pd = engine.self_distance(rid, w5.q[:512])
+    assert np.array_equal(m[:512], np.minimum(np.float32(1000.0), pd.min(axis=1)))
+    xa = R.xarm6()
+    rx = engine.load_robot(xa)
+    q2 = W.workload2(4096).q
+    m2, f2 = engine.self_collision(rx, q2)
+    om2 = oracle.self_min(xa, q2)
+    assert ulps(m2, om2).max() <= 2
+    assert np.array_equal(f2, (om2 >= 0).astype(np.uint8))

@@ -400,3 +400,104 @@ def test_oracle_vectors_unchanged(oracle):
         assert np.array_equal(ver, z[f"{name}_verdict"])
         assert np.array_equal(oracle.skeleton(w.robot, w.q, 32), z[f"{name}_skel32"])
         assert np.array_equal(oracle.is_valid(w.robot, sc, w.q), z[f"{name}_valid"])
+
+
+# ---------------------------------------------------------------- oriented boxes (build extension)
+def _rotation(q):
+    q = np.asarray(q, np.float64)
+    q = q / np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_obb_rigid_motion_invariance(oracle):
+    """Distance to an oriented box = distance to the axis-aligned box it is a
+    rigid motion of (independent double checker); a transposed rotation or a
+    dropped centre term fails this."""
+    rng = np.random.default_rng(41)
+    worst = 0.0
+    for _ in range(3000):
+        lo = rng.uniform(-1, 0, 3)
+        hi = lo + rng.uniform(0.05, 0.6, 3)
+        A = rng.uniform(-1.5, 1.5, 3)
+        B = A + rng.uniform(-0.5, 0.5, 3)
+        true = oracle.true_seg_box(A, B, np.r_[lo, hi])
+        Q = _rotation(rng.normal(size=4))
+        t = rng.uniform(-2, 2, 3)
+        obb = np.r_[Q @ ((lo + hi) / 2) + t, Q.ravel(), (hi - lo) / 2]
+        d, npo, v = oracle.capsule_obb(Q @ A + t, Q @ B + t, 0.0, obb, precision=64)
+        if true > 2e-4:      # below EQUALITY_THRESHOLD the reference snaps to 0
+            worst = max(worst, abs(d - true))
+            # nearest points: on the segment, on the box, |np_r - np_o| = d
+            r_, o_ = npo[:3], npo[3:]
+            assert abs(np.linalg.norm(r_ - o_) - d) < 1e-9
+            loc = Q.T @ (o_ - obb[:3])
+            assert (np.abs(loc) <= obb[12:] + 1e-9).all()
+    assert worst < 1e-12
+
+
+def test_obb_identity_is_aabb(oracle):
+    rng = np.random.default_rng(42)
+    for _ in range(500):
+        c = rng.uniform(-1, 1, 3)
+        h = rng.uniform(0.05, 0.3, 3)
+        A = rng.uniform(-1.5, 1.5, 3)
+        B = A + rng.uniform(-0.5, 0.5, 3)
+        r = rng.uniform(0, 0.1)
+        da = oracle.capsule_box(A, B, r, np.r_[c - h, c + h], 64)[0]
+        do = oracle.capsule_obb(A, B, r, np.r_[c, np.eye(3).ravel(), h], 64)[0]
+        assert abs(da - do) < 1e-12
+
+
+def test_obb_table_matches_quaternion_matrix():
+    rng = np.random.default_rng(43)
+    sc = W._boxes(rng, 64, 0.02, 0.2, 2.0, rotated=True)
+    t = sc.obb_table()
+    Rm = t[:, 3:12].reshape(-1, 3, 3).astype(np.float64)
+    assert np.abs(Rm @ Rm.transpose(0, 2, 1) - np.eye(3)).max() < 1e-6
+    assert np.abs(Rm - W._quat_matrix_f32(sc.quat)).max() < 1e-6
+    assert np.array_equal(t[:, :3], sc.pos) and np.array_equal(t[:, 12:], np.float32(0.5) * sc.dim)
+
+
+def test_obb_distance_loop_consistent(oracle):
+    """The batched loop with oriented boxes agrees with the single-pair entry
+    and its FP64 twin on the 22-DOF, 512-box scene."""
+    w = W.workload5(64)
+    sc32 = oracle.scene_of(w.scene, oriented=True)
+    d, prof, ver, npo = oracle.distance(w.robot, sc32, w.q, 32, want_np=True)
+    d64, prof64, ver64, _ = oracle.distance(w.robot, sc32, w.q, 64)
+    assert (ver == ver64).all()
+    assert np.abs(d - d64).max() < 1e-5
+    Pk = oracle.skeleton(w.robot, w.q, 32)
+    tab = w.scene.obb_table()
+    c = int(np.argmax(ver == 1)) if ver.any() else 0
+    for i in range(w.robot.num_links):
+        for j in range(0, w.scene.num_obj, 37):
+            dd, nn, v = oracle.capsule_obb(Pk[c, i], Pk[c, i + 1], w.robot.radius[i], tab[j], 32)
+            if ver[c] == 1:
+                ref = nn if v else np.full(6, np.nan, np.float32)
+                assert np.array_equal(npo[c, j, i], ref, equal_nan=True)
+
+
+# ---------------------------------------------------------------- self-collision minimum
+def test_self_min_straight_chain(oracle):
+    """A straight chain (q = 0): non-adjacent links are collinear, so the
+    closest pair is the shortest single link between two others, minus 2r."""
+    lengths = np.array([0.25, 0.12, 0.3, 0.18, 0.22, 0.15, 0.27])
+    rb = R.chain(lengths, radius=0.01, self_collision=True)
+    q = np.zeros((1, rb.num_dofs), np.float32)
+    P = oracle.skeleton(rb, q, 64)[0]
+    seg = np.linalg.norm(np.diff(P, axis=0), axis=1)
+    assert np.allclose(np.cross(P[-1] - P[0], P[1:] - P[0]), 0, atol=1e-9)   # collinear
+    expect = min(seg[i + 1] for i, j in rb.self_pairs if j == i + 2) - 2 * 0.01
+    m = oracle.self_min(rb, q)
+    assert abs(m[0] - expect) < 1e-6
+
+
+def test_self_min_matches_pairs(oracle):
+    w = W.workload5(512)
+    m = oracle.self_min(w.robot, w.q)
+    pd = oracle.self_distance(w.robot, w.q, w.robot.self_pairs)
+    assert np.array_equal(m, np.minimum(np.float32(1000.0), np.nanmin(pd, axis=1)))
