@@ -18,7 +18,8 @@
 //     ignored and a NaN first argument is kept
 //   * KDL frame products sum left to right: (a0*b0 + a1*b3) + a2*b6 [KDL, 3P]
 //
-// Parity status (what pins each part) is in oracle/README.md.
+// Parity status (what pins each part, what is unpinned) is listed in
+// rvs_oracle.h and oracle/README.md.
 
 #include "rvs_oracle.h"
 

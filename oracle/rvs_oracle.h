@@ -10,8 +10,13 @@
  *           operation order (no FMA contraction, IEEE div/sqrt).
  *   ref64 - the same algorithm with the skeleton and all geometry in double.
  *
- * Parity status of each function is stated in oracle/README.md and in the
- * header comment of rvs_oracle.cpp.
+ * Parity: segment distances, the capsule-box tree (where it is exact), FK
+ * closed forms, the loop semantics (reference scenario data) and the
+ * oriented-box extension are pinned by tests/test_oracle.py.  Parity
+ * unpinned: Eigen 3.4's FP32 reduction order, KDL's FP64 frame rounding,
+ * FCL's rotated-box AABB rule and the reference tree's non-exact quirks
+ * (corner-index test, face/edge lists, 0 vs -r contact outputs); they are
+ * restated from the source without an executable reference.
  */
 #ifndef RVS_ORACLE_H
 #define RVS_ORACLE_H

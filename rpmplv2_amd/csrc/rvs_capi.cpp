@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -72,6 +73,8 @@ struct rvs_context {
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float stage_ms[2] = {0, 0};
     int32_t nstages = 0;
+    int32_t num_cus = 0;
+    int32_t fast_variant = 1;   // FP32 bulk pass: 1 = persistent lane refill, 0 = one lane per configuration
 };
 
 namespace {
@@ -248,6 +251,8 @@ rvs_status rvs_create(const rvs_options* opt, rvs_context** out) {
         return RVS_EDEVICE;
     }
     e = hipSetDevice(c->opt.device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, c->opt.device);
+    if (const char* v = std::getenv("RVS_FAST_VARIANT")) c->fast_variant = std::atoi(v);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
     if (e != hipSuccess) { delete c; return RVS_EDEVICE; }
@@ -468,7 +473,8 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     if (prec == RVS_PREC_FP32_REFINE && nearest) prec = RVS_PREC_REF32;
     if (n == 0) return RVS_OK;
     if (prec == RVS_PREC_FP32_REFINE && n > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_distance: batch too large");
-    if (prec == RVS_PREC_FP32_REFINE && refine_lds_bytes(rb.dev, ev.dev) > kLdsLimit) prec = RVS_PREC_REF32;
+    const int rgroup = refine_group(rb.dev, ev.dev, kLdsLimit);
+    if (prec == RVS_PREC_FP32_REFINE && rgroup == 0) prec = RVS_PREC_REF32;
     (void)hipSetDevice(c->opt.device);
     const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
     const QueryParams qp = params_of(c);
@@ -505,11 +511,13 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     const bool timing = (flags & RVS_F_TIMING) != 0;
     if (e == hipSuccess) {
         if (prec == RVS_PREC_FP32_REFINE) {
-            e = launch_zero_u32(rcount, 1, c->stream);
+            e = launch_zero_u32(rcount, 64, c->stream);
+            const int pool = c->fast_variant == 1 ? c->num_cus * fast_pool_blocks_per_cu(ev.dev) : 0;
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
-            if (e == hipSuccess) e = launch_distance_fast(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, c->stream);
+            if (e == hipSuccess)
+                e = launch_distance_fast(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, pool, c->stream);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
-            if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, c->stream);
+            if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, rgroup, c->stream);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[2], c->stream);
             if (e == hipSuccess) e = d2h(&refined, rcount, 4, c->stream);
         } else {

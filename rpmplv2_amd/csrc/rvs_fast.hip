@@ -24,6 +24,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "rvs_fk.hpp"
 #include "rvs_geometry.hpp"
 #include "rvs_internal.hpp"
@@ -113,6 +115,52 @@ __device__ __attribute__((always_inline)) inline Vec3<float> skelf_point(FKf& s,
     return P;
 }
 
+// One (link, obstacle) pair of the bulk pass: filtered distance, and the
+// screening rules that send the configuration to the exact pass.
+template <bool OBB>
+__device__ __attribute__((always_inline)) inline float fast_pair(const Staged<float>& s, int j, const Vec3<float>& P0,
+                                                                 const Vec3<float>& P1, float r, const Params<float>& p,
+                                                                 float band, bool& uncertain) {
+    const Box<float> bx = s.box[j];
+    PairInfo<float> info{p.maxd, p.maxd, -1.0f, 1.0f, false, false};
+    float dt;
+    if (OBB) {
+        // oriented box: solved in the box frame (the frame change adds a few
+        // ulp of the lever arm, far inside the band)
+        const float* x = s.xf + 12 * j;
+        dt = capsule_box<float, false, FastMath>(to_box(x, P0), to_box(x, P1), r, bx, p, nullptr, &info, nullptr);
+    } else {
+        dt = capsule_box<float, false, FastMath>(P0, P1, r, bx, p, nullptr, &info, nullptr);
+    }
+    const float tol = s.tol[j];
+    // inside points and crossings are decided by comparisons: the result
+    // (0 or -r) is exact unless the decision is marginal
+    if (info.inside) uncertain |= info.depth < band;
+    else if (info.hit) uncertain |= info.hit_margin < band;
+    else uncertain |= (info.raw < p.eq + band) || (fabsf(dt) < band) || (info.sin2 < 1e-6f && info.raw < 0.01f);
+    if (tol < p.maxd && fabsf(dt - tol) < band) uncertain = true;
+    if (dt > tol) dt = p.maxd;
+    return dt;
+}
+
+// Wave-aggregated append of configuration c for every lane with flag set
+// (called by the whole wave).
+__device__ __attribute__((always_inline)) inline void append_flagged(bool flag, int64_t c, int32_t* idx,
+                                                                     uint32_t* count) {
+    const uint64_t mask = __ballot(flag);
+    if (mask) {
+        const int lane = __lane_id();
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (flag) {
+            const uint64_t below = (lane == 0) ? 0ull : (mask & (~0ull >> (64 - lane)));
+            idx[base + __popcll(below)] = (int32_t)c;
+        }
+    }
+}
+
 template <bool OBB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
@@ -142,31 +190,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const float r = rb.radius[i];
         float prof = p.maxd;
         for (int j = 0; j < M; ++j) {
-            float dt;
-            if (s.gnd[j] && i < rb.ground_included) {
-                dt = p.maxd;
-            } else {
-                const Box<float> bx = s.box[j];
-                PairInfo<float> info{p.maxd, p.maxd, -1.0f, 1.0f, false, false};
-                if (OBB) {
-                    // oriented box: solved in the box frame (the frame change
-                    // adds a few ulp of the lever arm, far inside the band)
-                    const float* x = s.xf + 12 * j;
-                    dt = capsule_box<float, false, FastMath>(to_box(x, P0), to_box(x, P1), r, bx, p, nullptr, &info,
-                                                             nullptr);
-                } else {
-                    dt = capsule_box<float, false, FastMath>(P0, P1, r, bx, p, nullptr, &info, nullptr);
-                }
-                const float tol = s.tol[j];
-                // inside points and crossings are decided by comparisons: the
-                // result (0 or -r) is exact unless the decision is marginal
-                if (info.inside) uncertain |= info.depth < band;
-                else if (info.hit) uncertain |= info.hit_margin < band;
-                else uncertain |= (info.raw < p.eq + band) || (fabsf(dt) < band) ||
-                                  (info.sin2 < 1e-6f && info.raw < 0.01f);
-                if (tol < p.maxd && fabsf(dt - tol) < band) uncertain = true;
-            }
-            if (dt > s.tol[j]) dt = p.maxd;
+            const float dt = (s.gnd[j] && i < rb.ground_included) ? p.maxd
+                                                                   : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
             prof = smin(prof, dt);
             if (prof <= 0) { hit = true; break; }
         }
@@ -180,18 +205,105 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (hit) d = 0.0f;
     if (out.d) out.d[c] = d;
     if (out.verdict) out.verdict[c] = hit ? 0 : 1;
-    // wave-aggregated append of the uncertain configurations
-    const uint64_t mask = __ballot(uncertain);
-    if (mask) {
-        const int lane = __lane_id();
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
-        base = __shfl(base, leader, 64);
-        if (uncertain) {
-            const uint64_t below = (lane == 0) ? 0ull : (mask & (~0ull >> (64 - lane)));
-            idx[base + __popcll(below)] = (int32_t)c;
+    append_flagged(uncertain, c, idx, count);
+}
+
+// The same pass with lane refill.  The classic kernel keeps a lane on one
+// configuration, so a wave runs as long as its slowest configuration while
+// lanes whose configuration exited early (first collision) sit idle.  Here a
+// persistent wave keeps every lane busy: the obstacle loop runs one link at a
+// time for all lanes together (obstacle j is the same for every lane, so the
+// LDS reads stay broadcasts), and after each link a lane whose configuration
+// is finished takes the next one from a global counter (one atomic per wave
+// and link).  Lanes therefore sit on different links of different
+// configurations; per configuration the pair order, the early exit and every
+// result are those of the classic kernel.
+template <bool OBB>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_pool(
+    RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
+    uint32_t* count, uint32_t* next) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<float> s = stage_env<float, OBB>(env, smem);
+    const int M = env.num_obj;
+    const int L = rb.num_links;
+    const int D = rb.num_dofs;
+    const Params<float> p{qp.eq, qp.maxd};
+    const float band = qp.band;
+    const int lane = __lane_id();
+
+    int64_t c = -1;           // configuration of this lane, -1 = none
+    bool drained = false;     // the counter has passed n
+    int i = 0;                // current link
+    const float* qc = q;
+    FKf fk;
+    Vec3<float> P0{0.0f, 0.0f, 0.0f};
+    float d = p.maxd;
+    bool uncertain = false;
+    for (;;) {
+        // refill the lanes without a configuration
+        const bool need = c < 0 && !drained;
+        const uint64_t nm = __ballot(need);
+        if (nm) {
+            const int leader = __ffsll((unsigned long long)nm) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(next, (uint32_t)__popcll(nm));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const uint64_t below = (lane == 0) ? 0ull : (nm & (~0ull >> (64 - lane)));
+                const int64_t cand = (int64_t)base + __popcll(below);
+                if (cand < n) {
+                    c = cand;
+                    qc = q + c * D;
+                    fk.M = Mat3f{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+                    fk.px = fk.py = fk.pz = 0.0f;
+                    fk.seg = 0;
+                    fk.qi = 0;
+                    P0 = skelf_point(fk, rb, qc, 0);
+                    i = 0;
+                    d = p.maxd;
+                    uncertain = false;
+                } else {
+                    drained = true;
+                }
+            }
         }
+        const bool busy = c >= 0;
+        if (__ballot(busy) == 0) break;
+        Vec3<float> P1 = P0;
+        float r = 0.0f;
+        if (busy) {
+            P1 = skelf_point(fk, rb, qc, i + 1);
+            if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
+            r = rb.radius[i];
+        }
+        const bool gnd_skip = i < rb.ground_included;
+        float prof = p.maxd;
+        bool active = busy, hit = false;
+        for (int j = 0; j < M; ++j) {
+            if (active) {
+                const float dt = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+                prof = smin(prof, dt);
+                if (prof <= 0) { hit = true; active = false; }
+            }
+            if (__ballot(active) == 0) break;
+        }
+        bool finished = false;
+        if (busy) {
+            if (out.profile) out.profile[c * L + i] = prof;
+            if (!hit) d = smin(d, prof);
+            if (hit || i == L - 1) {
+                for (int k = i + 1; k < L; ++k)
+                    if (out.profile) out.profile[c * L + k] = 0.0f;
+                if (out.d) out.d[c] = hit ? 0.0f : d;
+                if (out.verdict) out.verdict[c] = hit ? 0 : 1;
+                finished = true;
+            } else {
+                ++i;
+                P0 = P1;
+            }
+        }
+        append_flagged(finished && uncertain, c, idx, count);
+        if (finished) c = -1;
     }
 }
 
@@ -202,14 +314,36 @@ __global__ void k_zero_u32(uint32_t* p, int n) {
 
 }  // namespace
 
+int fast_pool_blocks_per_cu(const EnvDev& env) {
+    int nb = 0;
+    const hipError_t e =
+        env.use_obb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<true>, kBlock,
+                                                                    staged_bytes<float>(env))
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<false>, kBlock,
+                                                                    staged_bytes<float>(env));
+    return e == hipSuccess ? nb : 0;
+}
+
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_u32, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
     return hipGetLastError();
 }
 
 hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, hipStream_t st) {
+                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, int pool_blocks,
+                                hipStream_t st) {
     if (n <= 0) return hipSuccess;
+    if (pool_blocks > 0) {
+        // persistent lane-refill variant; count[32] is its configuration counter
+        const dim3 g((unsigned)std::min<int64_t>(pool_blocks, (n + kBlock - 1) / kBlock)), b(kBlock);
+        if (env.use_obb)
+            hipLaunchKernelGGL(k_distance_fast_pool<true>, g, b, staged_bytes<float>(env), st, rb, env, qp, q, n, out,
+                               idx, count, count + 32);
+        else
+            hipLaunchKernelGGL(k_distance_fast_pool<false>, g, b, staged_bytes<float>(env), st, rb, env, qp, q, n, out,
+                               idx, count, count + 32);
+        return hipGetLastError();
+    }
     const dim3 g((unsigned)((n + kBlock - 1) / kBlock)), b(kBlock);
     if (env.use_obb)
         hipLaunchKernelGGL(k_distance_fast<true>, g, b, staged_bytes<float>(env), st, rb, env, qp, q, n, out, idx,

@@ -70,13 +70,21 @@ hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const Quer
 // FP32 bulk pass: relaxed arithmetic, writes its results and appends every
 // configuration with an uncertain pair to idx (count in *count, which the
 // pass expects to be zero).
+// pool_blocks > 0 selects the persistent lane-refill kernel with that many
+// workgroups (count[32] is then its configuration counter, expected zero).
 hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, hipStream_t st);
+                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, int pool_blocks,
+                                hipStream_t st);
+// Resident workgroups per CU of the lane-refill kernel for this obstacle set.
+int fast_pool_blocks_per_cu(const EnvDev& env);
 hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                  int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
+                                  int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count, int G,
                                   hipStream_t st);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st);
-size_t refine_lds_bytes(const RobotDev& rb, const EnvDev& env);
+// Links per group of the refinement kernel within lds_limit bytes (0: does
+// not fit) and the LDS it then takes.
+int refine_group(const RobotDev& rb, const EnvDev& env, size_t lds_limit);
+size_t refine_lds_bytes(const RobotDev& rb, const EnvDev& env, int G);
 
 hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                  int64_t n, const DistOut& out, hipStream_t st);

@@ -139,22 +139,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 }
 
 // Exact recomputation of the configurations the FP32 bulk pass flagged: one
-// wave per configuration.  Lane t evaluates pairs p = t, t+64, ... (link
-// p / M, obstacle p % M) with the exact tree and parks the filtered pair
-// distance in LDS; lane 0 then folds them in the reference's loop order, so
-// the early exit, the partial profile and std::min's NaN handling come out as
-// in RealVectorSpace::computeDistance.  A few thousand flagged configurations
-// then finish in about one configuration's latency instead of one wave's
-// whole sequential walk.
+// wave per configuration.  The links are taken in groups of G; for a group,
+// lane t evaluates the pairs p = t, t+64, ... (link g0 + p / M, obstacle
+// p % M) with the exact tree and parks the filtered pair distance in LDS, and
+// lane 0 then folds the group's links in the reference's loop order, so the
+// early exit, the partial profile and std::min's NaN handling come out as in
+// RealVectorSpace::computeDistance.  The next group starts only if no link
+// of this one collided.  A few thousand flagged configurations finish in
+// about one configuration's latency instead of one wave's sequential walk.
 template <bool OBB>
 __global__ __launch_bounds__(64) void k_distance_refine(RobotDev rb, EnvDev env, QueryParams qp,
                                                         const float* __restrict__ q, DistOut out,
                                                         const int32_t* __restrict__ idx,
-                                                        const uint32_t* __restrict__ count) {
+                                                        const uint32_t* __restrict__ count, int G) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<float> s = stage_env<float, OBB>(env, smem);
-    float* f = reinterpret_cast<float*>(smem + ((staged_bytes<float>(env) + 15) & ~size_t(15)));
-    const int M = env.num_obj, L = rb.num_links, P = L * M;
+    int* flag = reinterpret_cast<int*>(smem + ((staged_bytes<float>(env) + 15) & ~size_t(15)));
+    float* f = reinterpret_cast<float*>(flag + 4);
+    const int M = env.num_obj, L = rb.num_links;
     const int t = threadIdx.x;
     const Params<float> p{qp.eq, qp.maxd};
     const uint32_t total = *count;
@@ -165,46 +167,57 @@ __global__ __launch_bounds__(64) void k_distance_refine(RobotDev rb, EnvDev env,
         fk_init(fk);
         int have = -1;          // index of the link whose end points are in P0/P1
         Vec3<float> P0{0, 0, 0}, P1{0, 0, 0};
-        for (int pi = t; pi < P; pi += 64) {
-            const int i = pi / M, j = pi - i * M;
-            if (i != have) {
-                // links visited by this lane increase, so the stream only moves forward
-                if (have < 0 || i != have + 1) P0 = skel_point<float>(fk, rb, qc, i);
-                else P0 = P1;
-                P1 = skel_point<float>(fk, rb, qc, i + 1);
-                have = i;
-            }
-            float dt;
-            if (s.gnd[j] && i < rb.ground_included) {
-                dt = p.maxd;
-            } else if (OBB) {
-                const float* x = s.xf + 12 * j;
-                const Box<float> bx = s.box[j];
-                dt = capsule_box<float, false>(to_box(x, P0), to_box(x, P1), rb.radius[i], bx, p, nullptr, nullptr,
-                                               nullptr);
-            } else {
-                const Box<float> bx = s.box[j];
-                dt = capsule_box<float, false>(P0, P1, rb.radius[i], bx, p, nullptr, nullptr, nullptr);
-            }
-            if (dt > s.tol[j]) dt = p.maxd;
-            f[pi] = dt;
-        }
-        __syncthreads();
-        if (t == 0) {
-            float d = p.maxd;
-            bool hit = false;
-            int i = 0;
-            for (; i < L && !hit; ++i) {
-                float prof = p.maxd;
-                for (int j = 0; j < M; ++j) {
-                    prof = smin(prof, f[i * M + j]);
-                    if (prof <= 0) { hit = true; break; }
+        float d = p.maxd;       // lane 0's running minimum
+        bool hit = false;
+        for (int g0 = 0; g0 < L && !hit; g0 += G) {
+            const int gl = min(G, L - g0);
+            const int P = gl * M;
+            for (int pi = t; pi < P; pi += 64) {
+                const int i = g0 + pi / M, j = pi - (i - g0) * M;
+                if (i != have) {
+                    // links visited by this lane increase, so the stream only moves forward
+                    if (have < 0 || i != have + 1) P0 = skel_point<float>(fk, rb, qc, i);
+                    else P0 = P1;
+                    P1 = skel_point<float>(fk, rb, qc, i + 1);
+                    have = i;
                 }
-                if (out.profile) out.profile[c * L + i] = prof;
-                if (!hit) d = smin(d, prof);
+                float dt;
+                if (s.gnd[j] && i < rb.ground_included) {
+                    dt = p.maxd;
+                } else if (OBB) {
+                    const float* x = s.xf + 12 * j;
+                    const Box<float> bx = s.box[j];
+                    dt = capsule_box<float, false>(to_box(x, P0), to_box(x, P1), rb.radius[i], bx, p, nullptr,
+                                                   nullptr, nullptr);
+                } else {
+                    const Box<float> bx = s.box[j];
+                    dt = capsule_box<float, false>(P0, P1, rb.radius[i], bx, p, nullptr, nullptr, nullptr);
+                }
+                if (dt > s.tol[j]) dt = p.maxd;
+                f[pi] = dt;
             }
-            for (; i < L; ++i)
-                if (out.profile) out.profile[c * L + i] = 0.0f;
+            __syncthreads();
+            if (t == 0) {
+                int i = g0;
+                for (; i < g0 + gl && !hit; ++i) {
+                    float prof = p.maxd;
+                    for (int j = 0; j < M; ++j) {
+                        prof = smin(prof, f[(i - g0) * M + j]);
+                        if (prof <= 0) { hit = true; break; }
+                    }
+                    if (out.profile) out.profile[c * L + i] = prof;
+                    if (!hit) d = smin(d, prof);
+                }
+                if (hit) {
+                    for (; i < L; ++i)
+                        if (out.profile) out.profile[c * L + i] = 0.0f;
+                }
+                flag[0] = hit ? 1 : 0;
+            }
+            __syncthreads();
+            hit = flag[0] != 0;
+        }
+        if (t == 0) {
             if (out.d) out.d[c] = hit ? 0.0f : d;
             if (out.verdict) out.verdict[c] = hit ? 0 : 1;
         }
@@ -378,23 +391,33 @@ hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const Que
     return hipGetLastError();
 }
 
-size_t refine_lds_bytes(const RobotDev& rb, const EnvDev& env) {
-    return ((staged_bytes<float>(env) + 15) & ~size_t(15)) + (size_t)rb.num_links * env.num_obj * 4;
+// Links per group of the refinement kernel: as many as fit 16 KB of pair
+// distances next to the staged obstacles (0 = the obstacle set leaves no room
+// for one link).
+int refine_group(const RobotDev& rb, const EnvDev& env, size_t lds_limit) {
+    const size_t base = ((staged_bytes<float>(env) + 15) & ~size_t(15)) + 16;
+    const int M = std::max(1, env.num_obj);
+    if (base + (size_t)M * 4 > lds_limit) return 0;
+    const size_t cap = std::min<size_t>(4096, (lds_limit - base) / 4);
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)rb.num_links, cap / M));
+}
+
+size_t refine_lds_bytes(const RobotDev& rb, const EnvDev& env, int G) {
+    return ((staged_bytes<float>(env) + 15) & ~size_t(15)) + 16 + (size_t)G * env.num_obj * 4;
 }
 
 hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                  int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
+                                  int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count, int G,
                                   hipStream_t st) {
     if (n <= 0) return hipSuccess;
     // enough waves to cover a typical flagged set in one round; the kernel
     // loops when more configurations are flagged
     const unsigned grid = (unsigned)std::min<int64_t>(n, 8192);
+    const size_t sm = refine_lds_bytes(rb, env, G);
     if (env.use_obb)
-        hipLaunchKernelGGL(k_distance_refine<true>, dim3(grid), dim3(64), refine_lds_bytes(rb, env), st, rb, env, qp,
-                           q, out, idx, count);
+        hipLaunchKernelGGL(k_distance_refine<true>, dim3(grid), dim3(64), sm, st, rb, env, qp, q, out, idx, count, G);
     else
-        hipLaunchKernelGGL(k_distance_refine<false>, dim3(grid), dim3(64), refine_lds_bytes(rb, env), st, rb, env, qp,
-                           q, out, idx, count);
+        hipLaunchKernelGGL(k_distance_refine<false>, dim3(grid), dim3(64), sm, st, rb, env, qp, q, out, idx, count, G);
     return hipGetLastError();
 }
 

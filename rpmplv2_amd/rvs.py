@@ -294,6 +294,11 @@ class Engine:
         self._check(lib().rvs_distance(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d), _ptr(profile),
                                        _ptr(nearest), _ptr(verdict), fl), "rvs_distance")
 
+    def distance_f64_device(self, robot: int, env: int, q, d, profile=None, verdict=None, timing: bool = False):
+        fl = F_DEVICE | (F_TIMING if timing else 0)
+        self._check(lib().rvs_distance_f64(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d), _ptr(profile),
+                                           _ptr(verdict), fl), "rvs_distance_f64")
+
     # ---- edges
     def edge_is_valid(self, robot: int, env: int, q1, q2):
         rb = self.robots[robot]

@@ -205,11 +205,29 @@ def test_edge_first_collision_sparse_scene(oracle, engine):
     assert np.array_equal(f, of) and (of == -1).mean() > 0.1
 
 
+@pytest.fixture(scope="module")
+def engine_classic():
+    """An engine whose FP32 bulk pass keeps one lane per configuration."""
+    import os
+    old = os.environ.get("RVS_FAST_VARIANT")
+    os.environ["RVS_FAST_VARIANT"] = "0"
+    eng = rvs.Engine(device=0)
+    if old is None:
+        del os.environ["RVS_FAST_VARIANT"]
+    else:
+        os.environ["RVS_FAST_VARIANT"] = old
+    yield eng
+    eng.close()
+
+
+@pytest.mark.parametrize("variant", ["pool", "classic"])
 @pytest.mark.parametrize("wl", ["workload1", "workload2"])
-def test_fp32_refine_verdict_exact(oracle, engine, wl):
+def test_fp32_refine_verdict_exact(oracle, engine, engine_classic, wl, variant):
     """FP32 bulk pass + exact refinement: verdicts identical to the reference
     arithmetic, distances within 1e-5 m of the FP64 oracle (and profiles of
-    collision-free configurations too)."""
+    collision-free configurations too), for the lane-refill and the
+    one-lane-per-configuration bulk kernels."""
+    engine = engine if variant == "pool" else engine_classic
     w = W.workload1(65536) if wl == "workload1" else W.workload2(1 << 18)
     rid = engine.load_robot(w.robot)
     eid = engine.create_env(w.scene)
