@@ -33,7 +33,8 @@ typedef enum rvs_status {
     RVS_ENOMEM = 2,   /* device or pinned allocation failed */
     RVS_EDEVICE = 3,  /* HIP runtime or kernel launch error */
     RVS_ERANGE = 4,   /* size beyond a documented limit */
-    RVS_ENOTSUP = 5   /* combination not supported */
+    RVS_ENOTSUP = 5,  /* combination not supported */
+    RVS_EBUSY = 6     /* a served request has not finished within the wait */
 } rvs_status;
 
 /* Arithmetic of a query. */
@@ -200,6 +201,111 @@ rvs_status rvs_edge_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const
 rvs_status rvs_edge_first_collision(rvs_context* ctx, int32_t env, const int32_t* robots, const float* q1,
                                     const float* dir, const int32_t* K, int64_t num_edges, int32_t row_stride,
                                     float step, int32_t* first, uint32_t flags);
+
+/* ---------------------------------------------------------------- autotuning
+ * Launch shape of the RVS_PREC_FP32_REFINE bulk pass for one (robot,
+ * environment): configurations per wavefront (one per lane for the whole
+ * walk, or persistent lane refill that hands a lane the next configuration
+ * after each link) and workgroup size.  The obstacle set is always staged
+ * whole in LDS (every lane walks every obstacle of its current link, so a
+ * smaller obstacle tile would only add reloads); it must fit 64 KB. */
+typedef struct rvs_tuning {
+    int32_t variant;        /* 0: one lane per configuration; 1: persistent lane refill */
+    int32_t block;          /* threads per workgroup: 64, 128 or 256 */
+    int32_t blocks_per_cu;  /* lane refill: resident workgroups per CU (0 = as many as fit) */
+    float ms;               /* bulk-pass time on the sample (rvs_autotune) */
+} rvs_tuning;
+
+/* Times every candidate shape on the first min(n, 262144) configurations of
+ * q, keeps the fastest for later RVS_PREC_FP32_REFINE calls on (robot, env)
+ * and returns it in best (may be NULL).  No counterpart in the reference. */
+rvs_status rvs_autotune(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n, uint32_t flags,
+                        rvs_tuning* best);
+rvs_status rvs_set_tuning(rvs_context* ctx, int32_t robot, int32_t env, const rvs_tuning* tuning);
+rvs_status rvs_get_tuning(rvs_context* ctx, int32_t robot, int32_t env, rvs_tuning* tuning);
+
+/* ---------------------------------------------------------------- serving
+ * A device-resident request loop over one context (no counterpart in the
+ * reference, whose planners call RealVectorSpace one state at a time).
+ * Planner threads submit requests of any size; a server thread admits them
+ * in arrival order while the device memory budget allows, fuses the admitted
+ * requests of one (robot, environment, kind) into one launch, runs up to
+ * num_streams launches concurrently, and keeps moving obstacles up to date
+ * on the device.  Each configuration's result is the one rvs_distance /
+ * rvs_distance_underestimation computes for it alone against the obstacle
+ * positions of the request's tick (rvs_request_info.tick).
+ */
+typedef struct rvs_server rvs_server;
+
+enum {
+    /* computeDistance verdicts: d[n] (0 on collision), flag[n] = 1 when
+     * collision-free, profile[n][L] optional; the server's precision. */
+    RVS_REQ_VERDICT = 0,
+    /* computeDistanceUnderestimation of every configuration against the
+     * nearest points of the request's first configuration (computed by the
+     * reference arithmetic): d[n], flag[n] = 1 when stored, profile optional. */
+    RVS_REQ_UNDERESTIMATION = 1
+};
+
+typedef struct rvs_server_options {
+    int64_t memory_budget;      /* device bytes for request state (default 8 GiB) */
+    int32_t num_streams;        /* concurrent launches (default 4) */
+    int32_t max_batch_configs;  /* configurations per fused launch (default 1 << 20) */
+    float tick_seconds;         /* obstacle update period (default 0.01 s) */
+    int32_t precision;          /* of verdict requests (default RVS_PREC_FP32_REFINE) */
+} rvs_server_options;
+
+typedef struct rvs_request {
+    int32_t robot;
+    int32_t env;
+    int32_t kind;               /* RVS_REQ_* */
+    int32_t reserved;
+    int64_t n;                  /* configurations */
+    const float* q;             /* [n][dofs] host memory, untouched until the request is done */
+    float* d;                   /* [n] host */
+    float* profile;             /* [n][L] host, or NULL */
+    uint8_t* flag;              /* [n] host */
+} rvs_request;
+
+typedef struct rvs_request_info {
+    double submit_s;            /* seconds since the server started */
+    double admit_s;             /* launched on the device */
+    double done_s;              /* results in the caller's buffers */
+    int64_t tick;               /* obstacle tick used (-1: environment without motion) */
+    int32_t batch_requests;     /* requests fused into the same launch */
+    int32_t status;             /* rvs_status of the request */
+} rvs_request_info;
+
+typedef struct rvs_server_stats {
+    int64_t submitted, completed, configs_completed, batches, ticks;
+    int32_t max_inflight_batches;  /* largest number of launches on the device at once */
+    int32_t reserved;
+    int64_t peak_bytes;            /* largest device memory in use by requests */
+} rvs_server_stats;
+
+void rvs_server_default_options(rvs_server_options* opt);
+/* The context must outlive the server; other calls on the context stay
+ * allowed (they are serialised with the server's use of it). */
+rvs_status rvs_server_create(rvs_context* ctx, const rvs_server_options* opt, rvs_server** server);
+/* Finishes every submitted request, then stops. */
+void rvs_server_destroy(rvs_server* server);
+/* From the next tick on, box j of env moves at vel[3j..3j+2] m/s:
+ * pos = pos_at_set_up + vel * (tick * tick_seconds), tick counted from the
+ * server's start.  Boxes with zero velocity stay put. */
+rvs_status rvs_server_set_motion(rvs_server* server, int32_t env, const float* vel);
+/* Non-blocking.  Host buffers (pinned ones from rvs_host_alloc give
+ * asynchronous copies) must stay valid until the request is done. */
+rvs_status rvs_server_submit(rvs_server* server, const rvs_request* req, uint64_t* ticket);
+/* Waits up to timeout_ms (-1: no limit) for the request.  RVS_OK: done, info
+ * filled (may be NULL) and the ticket released; RVS_EBUSY: not yet. */
+rvs_status rvs_server_wait(rvs_server* server, uint64_t ticket, int32_t timeout_ms, rvs_request_info* info);
+rvs_status rvs_server_get_stats(rvs_server* server, rvs_server_stats* stats);
+/* Seconds since the server started (the clock of rvs_request_info). */
+double rvs_server_now(rvs_server* server);
+const char* rvs_server_last_error(rvs_server* server);
+/* Pinned host memory (hipHostMalloc) for request buffers. */
+void* rvs_host_alloc(int64_t bytes);
+void rvs_host_free(void* p);
 
 /* ---------------------------------------------------------------- context */
 rvs_status rvs_create(const rvs_options* opt, rvs_context** ctx);

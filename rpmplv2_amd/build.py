@@ -27,7 +27,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librvs.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["rvs_kernels.hip", "rvs_edges.hip", "rvs_fast.hip", "rvs_capi.cpp"]
+SOURCES = ["rvs_kernels.hip", "rvs_edges.hip", "rvs_fast.hip", "rvs_capi.cpp", "rvs_server.cpp"]
 # the FP32 bulk pass is the one translation unit that allows FMA contraction
 EXTRA = {"rvs_fast.hip": ["-ffp-contract=fast"]}
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -58,6 +58,11 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the serving replay driver (a host program over the C ABI)
+    exe = os.path.join(BUILD_DIR, "rvs_serve_replay")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-pthread", "-I", os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "serve_replay.cpp"), "-o", exe + ".tmp", LIB, "-Wl,-rpath," + HERE])
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

@@ -18,64 +18,11 @@
 #include <vector>
 
 #include "../../include/rvs.h"
+#include "rvs_context.hpp"
 #include "rvs_internal.hpp"
 #include "rvs_stage.hpp"
 
 using namespace rvs;
-
-namespace {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        size_t want = std::max<size_t>(bytes, 1 << 20);
-        hipError_t e = hipMalloc(&p, want);
-        if (e == hipSuccess) cap = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-struct RobotRec {
-    RobotDev dev{};
-    std::vector<void*> allocs;
-    int num_dofs = 0, num_links = 0, num_pairs = 0;
-};
-
-struct EnvRec {
-    EnvDev dev{};
-    std::vector<void*> allocs;
-    std::vector<rvs_box> boxes;
-    std::vector<float> aabb;
-    int semantics = RVS_ENV_AABB;
-};
-
-}  // namespace
-
-struct rvs_context {
-    rvs_options opt{};
-    hipStream_t stream = nullptr;
-    std::mutex mu;
-    std::string err;
-    std::vector<RobotRec> robots;
-    std::vector<EnvRec> envs;
-    DevBuf arena;
-    int64_t last_refined = 0;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    float stage_ms[2] = {0, 0};
-    int32_t nstages = 0;
-    int32_t num_cus = 0;
-    int32_t fast_variant = 1;   // FP32 bulk pass: 1 = persistent lane refill, 0 = one lane per configuration
-};
 
 namespace {
 
@@ -142,10 +89,26 @@ hipError_t upload(std::vector<void*>& allocs, const T* src, size_t count, const 
     return e;
 }
 
+int prec_of(const rvs_context* c, uint32_t flags) {
+    const uint32_t v = (flags >> RVS_F_PREC_SHIFT) & 3u;
+    return v ? (int)v - 1 : c->opt.precision;
+}
+
+
+// Device arena carving: host arrays are staged into one allocation.
+bool robot_ok(rvs_context* c, int32_t id) { return id >= 0 && id < (int32_t)c->robots.size(); }
+bool env_ok(rvs_context* c, int32_t id) { return id >= 0 && id < (int32_t)c->envs.size(); }
+
+}  // namespace
+
+namespace rvs {
+
 // FCL CollisionObject::computeAABB for a box (restated in workloads.py and
 // include/rvs.h): identity rotation -> pos -+ dim/2; any other rotation ->
 // pos -+ |dim/2| on every axis.
-void box_bounds(const rvs_box& b, float out[6]) {
+// Eigen's isIdentity(1e-5) on the float rotation matrix of the box's
+// quaternion (Quaternion::toRotationMatrix order).
+bool box_identity(const rvs_box& b) {
     const float w = b.quat_wxyz[0], x = b.quat_wxyz[1], y = b.quat_wxyz[2], z = b.quat_wxyz[3];
     const float tx = 2.0f * x, ty = 2.0f * y, tz = 2.0f * z;
     const float twx = tx * w, twy = ty * w, twz = tz * w;
@@ -161,8 +124,12 @@ void box_bounds(const rvs_box& b, float out[6]) {
             if (i == j) ident = ident && (std::fabs(v - 1.0f) <= prec * std::min(std::fabs(v), 1.0f));
             else ident = ident && (std::fabs(v) <= prec);
         }
+    return ident;
+}
+
+void box_bounds(const rvs_box& b, float out[6]) {
     const float h[3] = {0.5f * b.dim[0], 0.5f * b.dim[1], 0.5f * b.dim[2]};
-    if (ident) {
+    if (box_identity(b)) {
         for (int k = 0; k < 3; ++k) { out[k] = -h[k] + b.pos[k]; out[3 + k] = h[k] + b.pos[k]; }
     } else {
         const float m0 = -h[0], m1 = -h[1], m2 = -h[2];
@@ -184,11 +151,6 @@ void box_obb(const rvs_box& b, float out[15]) {
     for (int k = 0; k < 3; ++k) out[12 + k] = 0.5f * b.dim[k];
 }
 
-int prec_of(const rvs_context* c, uint32_t flags) {
-    const uint32_t v = (flags >> RVS_F_PREC_SHIFT) & 3u;
-    return v ? (int)v - 1 : c->opt.precision;
-}
-
 QueryParams params_of(const rvs_context* c) {
     QueryParams qp;
     qp.eq = c->opt.equality_threshold;
@@ -198,28 +160,7 @@ QueryParams params_of(const rvs_context* c) {
     return qp;
 }
 
-// Device arena carving: host arrays are staged into one allocation.
-struct Carve {
-    char* base;
-    size_t off = 0;
-    explicit Carve(void* b) : base(static_cast<char*>(b)) {}
-    template <typename T> T* take(size_t count) {
-        off = (off + 255) & ~size_t(255);
-        T* p = reinterpret_cast<T*>(base + off);
-        off += count * sizeof(T);
-        return p;
-    }
-    static size_t need(std::initializer_list<size_t> bytes) {
-        size_t s = 0;
-        for (size_t b : bytes) s += ((b + 255) & ~size_t(255));
-        return s + 256;
-    }
-};
-
-bool robot_ok(rvs_context* c, int32_t id) { return id >= 0 && id < (int32_t)c->robots.size(); }
-bool env_ok(rvs_context* c, int32_t id) { return id >= 0 && id < (int32_t)c->envs.size(); }
-
-}  // namespace
+}  // namespace rvs
 
 // ================================================================ C API
 extern "C" {
@@ -473,8 +414,6 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     if (prec == RVS_PREC_FP32_REFINE && nearest) prec = RVS_PREC_REF32;
     if (n == 0) return RVS_OK;
     if (prec == RVS_PREC_FP32_REFINE && n > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_distance: batch too large");
-    const int rgroup = refine_group(rb.dev, ev.dev, kLdsLimit);
-    if (prec == RVS_PREC_FP32_REFINE && rgroup == 0) prec = RVS_PREC_REF32;
     (void)hipSetDevice(c->opt.device);
     const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
     const QueryParams qp = params_of(c);
@@ -512,12 +451,12 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     if (e == hipSuccess) {
         if (prec == RVS_PREC_FP32_REFINE) {
             e = launch_zero_u32(rcount, 64, c->stream);
-            const int pool = c->fast_variant == 1 ? c->num_cus * fast_pool_blocks_per_cu(ev.dev) : 0;
+            const FastTuning ft = c->tuning_for(robot, env, L, M);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
             if (e == hipSuccess)
-                e = launch_distance_fast(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, pool, c->stream);
+                e = launch_distance_fast(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, ft, c->num_cus, c->stream);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
-            if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, rgroup, c->stream);
+            if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, c->stream);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[2], c->stream);
             if (e == hipSuccess) e = d2h(&refined, rcount, 4, c->stream);
         } else {
@@ -543,6 +482,90 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
         (void)hipEventElapsedTime(&c->stage_ms[0], c->ev[0], c->ev[1]);
         if (prec == RVS_PREC_FP32_REFINE) (void)hipEventElapsedTime(&c->stage_ms[1], c->ev[1], c->ev[2]);
     }
+    return RVS_OK;
+}
+
+rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n, uint32_t flags,
+                        rvs_tuning* best) {
+    rvs_status s = check_query(c, robot, env, q, n, true);
+    if (s != RVS_OK) return s;
+    if (n <= 0) return fail(c, RVS_EINVAL, "rvs_autotune: empty sample");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    s = env_fits(c, ev.dev, false, "rvs_autotune");
+    if (s != RVS_OK) return s;
+    (void)hipSetDevice(c->opt.device);
+    const int64_t m = std::min<int64_t>(n, 1 << 18);   // the sample: the first m configurations
+    const int L = rb.num_links, D = rb.num_dofs;
+    hipError_t e = c->arena.ensure(Carve::need({(size_t)m * D * 4, (size_t)m * 4, (size_t)m * L * 4, (size_t)m,
+                                                (size_t)m * 4, 256}));
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_autotune: staging");
+    Carve cv(c->arena.p);
+    float* qs = cv.take<float>((size_t)m * D);
+    DistOut out{};
+    out.d = cv.take<float>(m);
+    out.profile = cv.take<float>((size_t)m * L);
+    out.verdict = cv.take<uint8_t>(m);
+    int32_t* idx = cv.take<int32_t>(m);
+    uint32_t* count = cv.take<uint32_t>(64);
+    e = hipMemcpyAsync(qs, q, (size_t)m * D * 4, (flags & RVS_F_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                       c->stream);
+    const QueryParams qp = params_of(c);
+    // candidates: one lane per configuration at three workgroup sizes; lane
+    // refill at three workgroup sizes, with the occupancy limit and half of it
+    std::vector<FastTuning> cand;
+    for (int b : {64, 128, 256}) cand.push_back(FastTuning{0, b, 0});
+    for (int b : {64, 128, 256}) {
+        const int occ = std::max(1, fast_pool_blocks_per_cu(ev.dev, b));
+        cand.push_back(FastTuning{1, b, occ});
+        if (occ > 1) cand.push_back(FastTuning{1, b, (occ + 1) / 2});
+    }
+    FastTuning win{};
+    float win_ms = 1e30f;
+    for (const FastTuning& t : cand) {
+        float best_ms = 1e30f;
+        for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {   // first run warms up
+            e = launch_zero_u32(count, 64, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->stream);
+            if (e == hipSuccess) e = launch_distance_fast(rb.dev, ev.dev, qp, qs, m, out, idx, count, t, c->num_cus, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+            float ms = 0;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+            if (rep > 0) best_ms = std::min(best_ms, ms);
+        }
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_autotune");
+        if (best_ms < win_ms) { win_ms = best_ms; win = t; }
+    }
+    {
+        std::lock_guard<std::mutex> tk(c->tuning_mu);
+        c->tuning[{robot, env}] = win;
+    }
+    if (best) *best = rvs_tuning{win.variant, win.block, win.blocks_per_cu, win_ms};
+    return RVS_OK;
+}
+
+rvs_status rvs_set_tuning(rvs_context* c, int32_t robot, int32_t env, const rvs_tuning* t) {
+    if (!c || !t) return fail(c, RVS_EINVAL, "rvs_set_tuning: null argument");
+    if ((t->variant != 0 && t->variant != 1) || (t->block != 64 && t->block != 128 && t->block != 256) ||
+        t->blocks_per_cu < 0)
+        return fail(c, RVS_EINVAL, "rvs_set_tuning: variant 0/1, block 64/128/256, blocks_per_cu >= 0");
+    std::lock_guard<std::mutex> lk(c->tuning_mu);
+    c->tuning[{robot, env}] = FastTuning{t->variant, t->block, t->blocks_per_cu};
+    return RVS_OK;
+}
+
+rvs_status rvs_get_tuning(rvs_context* c, int32_t robot, int32_t env, rvs_tuning* t) {
+    if (!c || !t) return fail(c, RVS_EINVAL, "rvs_get_tuning: null argument");
+    int32_t L = 0, M = 0;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (robot_ok(c, robot)) L = c->robots[robot].num_links;
+        if (env_ok(c, env)) M = c->envs[env].dev.num_obj;
+    }
+    const FastTuning f = c->tuning_for(robot, env, L, M);
+    *t = rvs_tuning{f.variant, f.block, f.blocks_per_cu, 0.0f};
     return RVS_OK;
 }
 

@@ -1,0 +1,113 @@
+// Internal state of a context (shared by the C ABI layer rvs_capi.cpp and
+// the request server rvs_server.cpp).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <initializer_list>
+#include <map>
+#include <utility>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rvs.h"
+#include "rvs_internal.hpp"
+
+namespace rvs {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 1 << 20);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct RobotRec {
+    RobotDev dev{};
+    std::vector<void*> allocs;
+    int num_dofs = 0, num_links = 0, num_pairs = 0;
+};
+
+struct EnvRec {
+    EnvDev dev{};
+    std::vector<void*> allocs;
+    std::vector<rvs_box> boxes;
+    std::vector<float> aabb;
+    int semantics = RVS_ENV_AABB;
+};
+
+}  // namespace rvs
+
+// The opaque handle of include/rvs.h.
+struct rvs_context {
+    rvs_options opt{};
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+    std::vector<rvs::RobotRec> robots;
+    std::vector<rvs::EnvRec> envs;
+    rvs::DevBuf arena;
+    int64_t last_refined = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    float stage_ms[2] = {0, 0};
+    int32_t nstages = 0;
+    int32_t num_cus = 0;
+    // FP32 bulk pass variant when no tuning is recorded: -1 = by walk length
+    // (lane refill when links x obstacles >= 1024), 0 = one lane per
+    // configuration, 1 = lane refill
+    int32_t fast_variant = -1;
+    std::map<std::pair<int32_t, int32_t>, rvs::FastTuning> tuning;   // (robot, env) -> autotuned launch shape
+    std::mutex tuning_mu;
+    rvs::FastTuning tuning_for(int32_t robot, int32_t env, int32_t links, int32_t objects) {
+        std::lock_guard<std::mutex> lk(tuning_mu);
+        auto it = tuning.find({robot, env});
+        if (it != tuning.end()) return it->second;
+        const int32_t v = fast_variant >= 0 ? fast_variant : ((int64_t)links * objects >= 1024 ? 1 : 0);
+        return rvs::FastTuning{v, 256, 0};
+    }
+};
+
+namespace rvs {
+
+// Sub-allocation of one device block, 256-byte aligned pieces.
+struct Carve {
+    char* base;
+    size_t off = 0;
+    explicit Carve(void* b) : base(static_cast<char*>(b)) {}
+    template <typename T> T* take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+    static size_t need(std::initializer_list<size_t> bytes) {
+        size_t s = 0;
+        for (size_t b : bytes) s += ((b + 255) & ~size_t(255));
+        return s + 256;
+    }
+};
+
+
+// FCL's world AABB of a box (and whether its rotation counts as identity)
+// and the oriented-box table of the engine.
+bool box_identity(const rvs_box& b);
+void box_bounds(const rvs_box& b, float out[6]);
+void box_obb(const rvs_box& b, float out[15]);
+QueryParams params_of(const rvs_context* c);
+
+}  // namespace rvs

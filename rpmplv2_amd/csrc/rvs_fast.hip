@@ -314,12 +314,12 @@ __global__ void k_zero_u32(uint32_t* p, int n) {
 
 }  // namespace
 
-int fast_pool_blocks_per_cu(const EnvDev& env) {
+int fast_pool_blocks_per_cu(const EnvDev& env, int block) {
     int nb = 0;
     const hipError_t e =
-        env.use_obb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<true>, kBlock,
+        env.use_obb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<true>, block,
                                                                     staged_bytes<float>(env))
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<false>, kBlock,
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<false>, block,
                                                                     staged_bytes<float>(env));
     return e == hipSuccess ? nb : 0;
 }
@@ -330,27 +330,30 @@ hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st) {
 }
 
 hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, int pool_blocks,
-                                hipStream_t st) {
+                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
+                                int num_cus, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (pool_blocks > 0) {
+    const int B = (t.block == 64 || t.block == 128) ? t.block : kBlock;
+    const size_t sm = staged_bytes<float>(env);
+    if (t.variant == 1) {
         // persistent lane-refill variant; count[32] is its configuration counter
-        const dim3 g((unsigned)std::min<int64_t>(pool_blocks, (n + kBlock - 1) / kBlock)), b(kBlock);
+        int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : fast_pool_blocks_per_cu(env, B);
+        per_cu = std::max(1, per_cu);
+        const int64_t blocks = std::min<int64_t>((int64_t)num_cus * per_cu, (n + B - 1) / B);
+        const dim3 g((unsigned)std::max<int64_t>(1, blocks)), b(B);
         if (env.use_obb)
-            hipLaunchKernelGGL(k_distance_fast_pool<true>, g, b, staged_bytes<float>(env), st, rb, env, qp, q, n, out,
-                               idx, count, count + 32);
+            hipLaunchKernelGGL(k_distance_fast_pool<true>, g, b, sm, st, rb, env, qp, q, n, out, idx, count,
+                               count + 32);
         else
-            hipLaunchKernelGGL(k_distance_fast_pool<false>, g, b, staged_bytes<float>(env), st, rb, env, qp, q, n, out,
-                               idx, count, count + 32);
+            hipLaunchKernelGGL(k_distance_fast_pool<false>, g, b, sm, st, rb, env, qp, q, n, out, idx, count,
+                               count + 32);
         return hipGetLastError();
     }
-    const dim3 g((unsigned)((n + kBlock - 1) / kBlock)), b(kBlock);
+    const dim3 g((unsigned)((n + B - 1) / B)), b(B);
     if (env.use_obb)
-        hipLaunchKernelGGL(k_distance_fast<true>, g, b, staged_bytes<float>(env), st, rb, env, qp, q, n, out, idx,
-                           count);
+        hipLaunchKernelGGL(k_distance_fast<true>, g, b, sm, st, rb, env, qp, q, n, out, idx, count);
     else
-        hipLaunchKernelGGL(k_distance_fast<false>, g, b, staged_bytes<float>(env), st, rb, env, qp, q, n, out, idx,
-                           count);
+        hipLaunchKernelGGL(k_distance_fast<false>, g, b, sm, st, rb, env, qp, q, n, out, idx, count);
     return hipGetLastError();
 }
 

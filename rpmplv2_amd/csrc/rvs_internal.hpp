@@ -70,21 +70,24 @@ hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const Quer
 // FP32 bulk pass: relaxed arithmetic, writes its results and appends every
 // configuration with an uncertain pair to idx (count in *count, which the
 // pass expects to be zero).
-// pool_blocks > 0 selects the persistent lane-refill kernel with that many
-// workgroups (count[32] is then its configuration counter, expected zero).
+// Launch shape of the FP32 bulk pass (chosen by the autotuner).
+struct FastTuning {
+    int32_t variant;        // 0: one lane per configuration; 1: persistent lane refill
+    int32_t block;          // threads per workgroup: 64, 128 or 256
+    int32_t blocks_per_cu;  // lane refill: resident workgroups per CU, 0 = the occupancy limit
+};
+// The lane-refill variant uses count[32] as its configuration counter
+// (expected zero, like count[0]).
 hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, int pool_blocks,
-                                hipStream_t st);
+                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
+                                int num_cus, hipStream_t st);
 // Resident workgroups per CU of the lane-refill kernel for this obstacle set.
-int fast_pool_blocks_per_cu(const EnvDev& env);
+int fast_pool_blocks_per_cu(const EnvDev& env, int block);
 hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                  int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count, int G,
+                                  int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
                                   hipStream_t st);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st);
-// Links per group of the refinement kernel within lds_limit bytes (0: does
-// not fit) and the LDS it then takes.
-int refine_group(const RobotDev& rb, const EnvDev& env, size_t lds_limit);
-size_t refine_lds_bytes(const RobotDev& rb, const EnvDev& env, int G);
+
 
 hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                  int64_t n, const DistOut& out, hipStream_t st);
@@ -95,6 +98,27 @@ hipError_t launch_is_valid(const RobotDev& rb, const EnvDev& env, const QueryPar
 hipError_t launch_underestimation(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                   int64_t n, const float* np_ref, int64_t np_stride, float* d, float* profile,
                                   uint8_t* stored, hipStream_t st);
+// Underestimation for a served batch: configuration c takes nearest-point
+// set r of np_sets ([nseg][M][L][6]) where seg_off[r] <= c < seg_off[r + 1].
+hipError_t launch_underestimation_seg(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                      int64_t n, const float* np_sets, const int64_t* seg_off, int nseg, float* d,
+                                      float* profile, uint8_t* stored, hipStream_t st);
+
+// A moving obstacle set on the device (all arrays device pointers).
+struct MotionDev {
+    int32_t num_obj;
+    int32_t pad;
+    const float* pos0;    // [M][3]
+    const float* vel;     // [M][3] m/s (0 for static boxes)
+    const float* half;    // [M][3] dim / 2
+    const float* rad;     // [M] |dim / 2| (rotated boxes)
+    const uint8_t* ident; // [M] 1 = identity rotation
+    const float* obb0;    // [M][15] oriented-box rows at pos0
+};
+// Writes the AABB table [M][6] and oriented-box table [M][15] at a tick.
+hipError_t launch_env_motion(const MotionDev& m, int64_t tick, float tick_seconds, float* aabb, float* obb,
+                             hipStream_t st);
+
 // Per-pair distances (dist / dist64, [N][pairs]) and/or the per-configuration
 // minimum and collision-free flag (min_out / free_out); null outputs are skipped.
 hipError_t launch_self_distance(const RobotDev& rb, const QueryParams& qp, const float* q, int64_t n, float* dist,

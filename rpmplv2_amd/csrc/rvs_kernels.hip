@@ -23,6 +23,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <climits>
 
 #include "rvs_fk.hpp"
 #include "rvs_geometry.hpp"
@@ -139,23 +140,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 }
 
 // Exact recomputation of the configurations the FP32 bulk pass flagged: one
-// wave per configuration.  The links are taken in groups of G; for a group,
-// lane t evaluates the pairs p = t, t+64, ... (link g0 + p / M, obstacle
-// p % M) with the exact tree and parks the filtered pair distance in LDS, and
-// lane 0 then folds the group's links in the reference's loop order, so the
-// early exit, the partial profile and std::min's NaN handling come out as in
-// RealVectorSpace::computeDistance.  The next group starts only if no link
-// of this one collided.  A few thousand flagged configurations finish in
-// about one configuration's latency instead of one wave's sequential walk.
+// wave per configuration, one link at a time.  For link i, lane t evaluates
+// obstacles j = t, t + 64, ... with the exact tree; the wave then reduces the
+// smallest filtered distance and the first obstacle whose filtered distance
+// is <= 0.  That gives exactly what the reference's sequential loop gives:
+// its running minimum starts at MAX_DISTANCE, so a NaN distance never enters
+// it (std::min keeps the first argument), and it stops at the first pair
+// whose distance is <= 0, which then becomes the profile value (the running
+// minimum is positive up to that pair).  A minimum of floats is exact in any
+// order, and only the first non-positive value can reach the profile, so the
+// reduction order cannot change a bit of the result.
 template <bool OBB>
 __global__ __launch_bounds__(64) void k_distance_refine(RobotDev rb, EnvDev env, QueryParams qp,
                                                         const float* __restrict__ q, DistOut out,
                                                         const int32_t* __restrict__ idx,
-                                                        const uint32_t* __restrict__ count, int G) {
+                                                        const uint32_t* __restrict__ count) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<float> s = stage_env<float, OBB>(env, smem);
-    int* flag = reinterpret_cast<int*>(smem + ((staged_bytes<float>(env) + 15) & ~size_t(15)));
-    float* f = reinterpret_cast<float*>(flag + 4);
     const int M = env.num_obj, L = rb.num_links;
     const int t = threadIdx.x;
     const Params<float> p{qp.eq, qp.maxd};
@@ -165,63 +166,57 @@ __global__ __launch_bounds__(64) void k_distance_refine(RobotDev rb, EnvDev env,
         const float* qc = q + c * rb.num_dofs;
         FKStream fk;
         fk_init(fk);
-        int have = -1;          // index of the link whose end points are in P0/P1
-        Vec3<float> P0{0, 0, 0}, P1{0, 0, 0};
-        float d = p.maxd;       // lane 0's running minimum
+        Vec3<float> P0 = skel_point<float>(fk, rb, qc, 0);
+        float d = p.maxd;
         bool hit = false;
-        for (int g0 = 0; g0 < L && !hit; g0 += G) {
-            const int gl = min(G, L - g0);
-            const int P = gl * M;
-            for (int pi = t; pi < P; pi += 64) {
-                const int i = g0 + pi / M, j = pi - (i - g0) * M;
-                if (i != have) {
-                    // links visited by this lane increase, so the stream only moves forward
-                    if (have < 0 || i != have + 1) P0 = skel_point<float>(fk, rb, qc, i);
-                    else P0 = P1;
-                    P1 = skel_point<float>(fk, rb, qc, i + 1);
-                    have = i;
-                }
+        int i = 0;
+        for (; i < L; ++i) {
+            const Vec3<float> P1 = skel_point<float>(fk, rb, qc, i + 1);
+            const float r = rb.radius[i];
+            float lo = p.maxd;          // smallest filtered distance of this lane's obstacles
+            int first = INT_MAX;        // first of them with filtered distance <= 0
+            float first_v = 0.0f;
+            for (int j = t; j < M; j += 64) {
                 float dt;
                 if (s.gnd[j] && i < rb.ground_included) {
                     dt = p.maxd;
                 } else if (OBB) {
                     const float* x = s.xf + 12 * j;
                     const Box<float> bx = s.box[j];
-                    dt = capsule_box<float, false>(to_box(x, P0), to_box(x, P1), rb.radius[i], bx, p, nullptr,
-                                                   nullptr, nullptr);
+                    dt = capsule_box<float, false>(to_box(x, P0), to_box(x, P1), r, bx, p, nullptr, nullptr, nullptr);
                 } else {
                     const Box<float> bx = s.box[j];
-                    dt = capsule_box<float, false>(P0, P1, rb.radius[i], bx, p, nullptr, nullptr, nullptr);
+                    dt = capsule_box<float, false>(P0, P1, r, bx, p, nullptr, nullptr, nullptr);
                 }
                 if (dt > s.tol[j]) dt = p.maxd;
-                f[pi] = dt;
-            }
-            __syncthreads();
-            if (t == 0) {
-                int i = g0;
-                for (; i < g0 + gl && !hit; ++i) {
-                    float prof = p.maxd;
-                    for (int j = 0; j < M; ++j) {
-                        prof = smin(prof, f[(i - g0) * M + j]);
-                        if (prof <= 0) { hit = true; break; }
-                    }
-                    if (out.profile) out.profile[c * L + i] = prof;
-                    if (!hit) d = smin(d, prof);
+                lo = smin(lo, dt);
+                if (dt <= 0 && first == INT_MAX) {
+                    first = j;
+                    first_v = dt;
                 }
-                if (hit) {
-                    for (; i < L; ++i)
-                        if (out.profile) out.profile[c * L + i] = 0.0f;
-                }
-                flag[0] = hit ? 1 : 0;
             }
-            __syncthreads();
-            hit = flag[0] != 0;
+            for (int o = 32; o > 0; o >>= 1) {
+                lo = smin(lo, __shfl_xor(lo, o, 64));
+                const int f2 = __shfl_xor(first, o, 64);
+                const float v2 = __shfl_xor(first_v, o, 64);
+                if (f2 < first) {
+                    first = f2;
+                    first_v = v2;
+                }
+            }
+            hit = first != INT_MAX;
+            const float prof = hit ? first_v : lo;
+            if (t == 0 && out.profile) out.profile[c * L + i] = prof;
+            if (hit) break;
+            d = smin(d, prof);
+            P0 = P1;
         }
         if (t == 0) {
+            for (int k2 = i + 1; k2 < L; ++k2)
+                if (out.profile) out.profile[c * L + k2] = 0.0f;
             if (out.d) out.d[c] = hit ? 0.0f : d;
             if (out.verdict) out.verdict[c] = hit ? 0 : 1;
         }
-        __syncthreads();
     }
 }
 
@@ -274,16 +269,28 @@ __global__ __launch_bounds__(kBlock) void k_valid(RobotDev rb, EnvDev env, Query
 // RealVectorSpace::computeDistanceUnderestimation (RealVectorSpace.cpp:
 // 402-446) on a fresh state, in float.  np_ref holds [M][L][6] per
 // configuration (stride np_stride floats; 0 = one set for all).
+// With seg_off (nseg + 1 non-decreasing offsets), configuration c uses the
+// nearest-point set of its segment r (seg_off[r] <= c < seg_off[r + 1]):
+// requests of a served batch each bring their own set.
 __global__ __launch_bounds__(kBlock) void k_under(RobotDev rb, EnvDev env, QueryParams qp,
                                                   const float* __restrict__ q, int64_t n,
                                                   const float* __restrict__ np_ref, int64_t np_stride, float* d_out,
-                                                  float* prof_out, uint8_t* stored) {
+                                                  float* prof_out, uint8_t* stored,
+                                                  const int64_t* __restrict__ seg_off, int nseg) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const int M = env.num_obj;
     const int L = rb.num_links;
     const float* qc = q + c * rb.num_dofs;
     const float* npc = np_ref + c * np_stride;
+    if (seg_off) {
+        int lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (seg_off[mid] <= c) lo = mid; else hi = mid - 1;
+        }
+        npc = np_ref + (int64_t)lo * M * L * 6;
+    }
     const float maxd = qp.maxd;
     FKStream fk;
     fk_init(fk);
@@ -357,6 +364,32 @@ __global__ __launch_bounds__(128) void k_self(RobotDev rb, QueryParams qp, const
     if (free_out) free_out[c] = lo < 0 ? 0 : 1;
 }
 
+// ---------------------------------------------------------------- moving obstacles
+// Positions at a tick: pos = pos0 + vel * (tick * tick_seconds), in float,
+// without contraction (a host can reproduce them bit for bit), then FCL's
+// world AABB as the engine computes it at set-up: identity rotation
+// (-h) + pos, h + pos; any other rotation pos -+ rad.  Oriented-box rows get
+// the new centre.  Static boxes keep pos0.
+__global__ void k_env_motion(MotionDev m, int64_t tick, float tick_seconds, float* aabb, float* obb) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m.num_obj) return;
+    const float t = (float)tick * tick_seconds;
+    float p[3];
+    for (int k = 0; k < 3; ++k) p[k] = m.pos0[3 * j + k] + m.vel[3 * j + k] * t;
+    const float* hr = m.half + 3 * j;
+    for (int k = 0; k < 3; ++k) {
+        if (m.ident[j]) {
+            aabb[6 * j + k] = -hr[k] + p[k];
+            aabb[6 * j + 3 + k] = hr[k] + p[k];
+        } else {
+            aabb[6 * j + k] = p[k] - m.rad[j];
+            aabb[6 * j + 3 + k] = p[k] + m.rad[j];
+        }
+    }
+    for (int k = 0; k < 3; ++k) obb[15 * j + k] = p[k];
+    for (int k = 3; k < 15; ++k) obb[15 * j + k] = m.obb0[15 * j + k];
+}
+
 inline dim3 grid_for(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
 
 }  // namespace
@@ -391,33 +424,18 @@ hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const Que
     return hipGetLastError();
 }
 
-// Links per group of the refinement kernel: as many as fit 16 KB of pair
-// distances next to the staged obstacles (0 = the obstacle set leaves no room
-// for one link).
-int refine_group(const RobotDev& rb, const EnvDev& env, size_t lds_limit) {
-    const size_t base = ((staged_bytes<float>(env) + 15) & ~size_t(15)) + 16;
-    const int M = std::max(1, env.num_obj);
-    if (base + (size_t)M * 4 > lds_limit) return 0;
-    const size_t cap = std::min<size_t>(4096, (lds_limit - base) / 4);
-    return (int)std::max<size_t>(1, std::min<size_t>((size_t)rb.num_links, cap / M));
-}
-
-size_t refine_lds_bytes(const RobotDev& rb, const EnvDev& env, int G) {
-    return ((staged_bytes<float>(env) + 15) & ~size_t(15)) + 16 + (size_t)G * env.num_obj * 4;
-}
-
 hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                  int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count, int G,
+                                  int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
                                   hipStream_t st) {
     if (n <= 0) return hipSuccess;
     // enough waves to cover a typical flagged set in one round; the kernel
     // loops when more configurations are flagged
     const unsigned grid = (unsigned)std::min<int64_t>(n, 8192);
-    const size_t sm = refine_lds_bytes(rb, env, G);
+    const size_t sm = staged_bytes<float>(env);
     if (env.use_obb)
-        hipLaunchKernelGGL(k_distance_refine<true>, dim3(grid), dim3(64), sm, st, rb, env, qp, q, out, idx, count, G);
+        hipLaunchKernelGGL(k_distance_refine<true>, dim3(grid), dim3(64), sm, st, rb, env, qp, q, out, idx, count);
     else
-        hipLaunchKernelGGL(k_distance_refine<false>, dim3(grid), dim3(64), sm, st, rb, env, qp, q, out, idx, count, G);
+        hipLaunchKernelGGL(k_distance_refine<false>, dim3(grid), dim3(64), sm, st, rb, env, qp, q, out, idx, count);
     return hipGetLastError();
 }
 
@@ -442,7 +460,24 @@ hipError_t launch_underestimation(const RobotDev& rb, const EnvDev& env, const Q
                                   uint8_t* stored, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_under, grid_for(n, kBlock), dim3(kBlock), 0, st, rb, env, qp, q, n, np_ref, np_stride, d,
-                       profile, stored);
+                       profile, stored, nullptr, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_underestimation_seg(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                      int64_t n, const float* np_sets, const int64_t* seg_off, int nseg, float* d,
+                                      float* profile, uint8_t* stored, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_under, grid_for(n, kBlock), dim3(kBlock), 0, st, rb, env, qp, q, n, np_sets, (int64_t)0, d,
+                       profile, stored, seg_off, nseg);
+    return hipGetLastError();
+}
+
+hipError_t launch_env_motion(const MotionDev& m, int64_t tick, float tick_seconds, float* aabb, float* obb,
+                             hipStream_t st) {
+    if (m.num_obj <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_env_motion, dim3((m.num_obj + 255) / 256), dim3(256), 0, st, m, tick, tick_seconds, aabb,
+                       obb);
     return hipGetLastError();
 }
 

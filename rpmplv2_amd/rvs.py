@@ -59,10 +59,43 @@ class Box(C.Structure):
                 ("flags", C.c_uint32), ("min_dist_tol", C.c_float)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("memory_budget", C.c_int64), ("num_streams", C.c_int32), ("max_batch_configs", C.c_int32),
+                ("tick_seconds", C.c_float), ("precision", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("robot", C.c_int32), ("env", C.c_int32), ("kind", C.c_int32), ("reserved", C.c_int32),
+                ("n", C.c_int64), ("q", C.c_void_p), ("d", C.c_void_p), ("profile", C.c_void_p),
+                ("flag", C.c_void_p)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("submit_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double), ("tick", C.c_int64),
+                ("batch_requests", C.c_int32), ("status", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("configs_completed", C.c_int64),
+                ("batches", C.c_int64), ("ticks", C.c_int64), ("max_inflight_batches", C.c_int32),
+                ("reserved", C.c_int32), ("peak_bytes", C.c_int64)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("variant", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("ms", C.c_float)]
+
+
+REQ_VERDICT = 0
+REQ_UNDERESTIMATION = 1
+RVS_EBUSY = 6
+
 EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error", "rvs_stream", "rvs_last_refined",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds", "rvs_distance",
            "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_self_distance",
-           "rvs_self_collision", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing"]
+           "rvs_self_collision", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
+           "rvs_autotune", "rvs_set_tuning", "rvs_get_tuning", "rvs_server_default_options", "rvs_server_create", "rvs_server_destroy", "rvs_server_set_motion",
+           "rvs_server_submit", "rvs_server_wait", "rvs_server_get_stats", "rvs_server_now",
+           "rvs_server_last_error", "rvs_host_alloc", "rvs_host_free"]
 
 _lib = None
 
@@ -99,6 +132,26 @@ def lib():
         L.rvs_distance_underestimation.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, vp, vp, u32]
         L.rvs_self_distance.argtypes = [ctx, i32, vp, i64, vp, u32]
         L.rvs_self_collision.argtypes = [ctx, i32, vp, i64, vp, vp, u32]
+        L.rvs_autotune.argtypes = [ctx, i32, i32, vp, i64, u32, C.POINTER(Tuning)]
+        L.rvs_set_tuning.argtypes = [ctx, i32, i32, C.POINTER(Tuning)]
+        L.rvs_get_tuning.argtypes = [ctx, i32, i32, C.POINTER(Tuning)]
+        L.rvs_server_default_options.argtypes = [C.POINTER(ServerOptions)]
+        L.rvs_server_default_options.restype = None
+        L.rvs_server_create.argtypes = [ctx, C.POINTER(ServerOptions), C.POINTER(C.c_void_p)]
+        L.rvs_server_destroy.argtypes = [C.c_void_p]
+        L.rvs_server_destroy.restype = None
+        L.rvs_server_set_motion.argtypes = [C.c_void_p, i32, vp]
+        L.rvs_server_submit.argtypes = [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]
+        L.rvs_server_wait.argtypes = [C.c_void_p, C.c_uint64, i32, C.POINTER(RequestInfo)]
+        L.rvs_server_get_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+        L.rvs_server_now.argtypes = [C.c_void_p]
+        L.rvs_server_now.restype = C.c_double
+        L.rvs_server_last_error.argtypes = [C.c_void_p]
+        L.rvs_server_last_error.restype = C.c_char_p
+        L.rvs_host_alloc.argtypes = [i64]
+        L.rvs_host_alloc.restype = C.c_void_p
+        L.rvs_host_free.argtypes = [C.c_void_p]
+        L.rvs_host_free.restype = None
         L.rvs_edge_is_valid.argtypes = [ctx, i32, i32, vp, vp, i64, vp, u32]
         L.rvs_edge_first_collision.argtypes = [ctx, i32, vp, vp, vp, vp, i64, i32, C.c_float, vp, u32]
         for name in EXPORTS:
@@ -299,6 +352,25 @@ class Engine:
         self._check(lib().rvs_distance_f64(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d), _ptr(profile),
                                            _ptr(verdict), fl), "rvs_distance_f64")
 
+    # ---- autotuning of the FP32 bulk pass
+    def autotune(self, robot: int, env: int, q) -> dict:
+        t = Tuning()
+        dev = _is_torch(q)
+        if not dev:
+            q = np.ascontiguousarray(q, np.float32).reshape(-1, self.robots[robot].num_dofs)
+        self._check(lib().rvs_autotune(self._ctx, robot, env, _ptr(q), q.shape[0], F_DEVICE if dev else 0,
+                                       C.byref(t)), "rvs_autotune")
+        return {"variant": t.variant, "block": t.block, "blocks_per_cu": t.blocks_per_cu, "ms": t.ms}
+
+    def set_tuning(self, robot: int, env: int, variant: int, block: int = 256, blocks_per_cu: int = 0):
+        t = Tuning(variant, block, blocks_per_cu, 0.0)
+        self._check(lib().rvs_set_tuning(self._ctx, robot, env, C.byref(t)), "rvs_set_tuning")
+
+    def get_tuning(self, robot: int, env: int) -> dict:
+        t = Tuning()
+        self._check(lib().rvs_get_tuning(self._ctx, robot, env, C.byref(t)), "rvs_get_tuning")
+        return {"variant": t.variant, "block": t.block, "blocks_per_cu": t.blocks_per_cu}
+
     # ---- edges
     def edge_is_valid(self, robot: int, env: int, q1, q2):
         rb = self.robots[robot]
@@ -341,3 +413,74 @@ class Engine:
             out.append(first[o:o + len(b.q1)])
             o += len(b.q1)
         return out
+
+
+class Server:
+    """Request server over an Engine's context (rvs_server_*).  Results land
+    in numpy arrays owned by this object until wait() returns them."""
+
+    def __init__(self, engine: Engine, memory_budget: int = 8 << 30, num_streams: int = 4,
+                 max_batch_configs: int = 1 << 20, tick_seconds: float = 0.01, precision: int = PREC_FP32_REFINE):
+        L = lib()
+        o = ServerOptions()
+        L.rvs_server_default_options(C.byref(o))
+        o.memory_budget, o.num_streams, o.max_batch_configs = memory_budget, num_streams, max_batch_configs
+        o.tick_seconds, o.precision = tick_seconds, precision
+        self.engine = engine
+        self._srv = C.c_void_p()
+        st = L.rvs_server_create(engine._ctx, C.byref(o), C.byref(self._srv))
+        if st != RVS_OK:
+            raise RvsError(st, "rvs_server_create")
+        self._live = {}
+
+    def _check(self, st, what):
+        if st != RVS_OK:
+            raise RvsError(st, f"{what}: {lib().rvs_server_last_error(self._srv).decode()}")
+
+    def set_motion(self, env: int, vel):
+        v = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+        self._check(lib().rvs_server_set_motion(self._srv, env, _ptr(v)), "rvs_server_set_motion")
+
+    def now(self) -> float:
+        return lib().rvs_server_now(self._srv)
+
+    def submit(self, robot: int, env: int, kind: int, q, want_profile: bool = False) -> int:
+        rb = self.engine.robots[robot]
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+        n = len(q)
+        d = np.zeros(n, np.float32)
+        flag = np.zeros(n, np.uint8)
+        prof = np.zeros((n, rb.num_links), np.float32) if want_profile else None
+        r = Request(robot, env, kind, 0, n, q.ctypes.data if n else None, d.ctypes.data if n else None,
+                    prof.ctypes.data if (prof is not None and n) else None, flag.ctypes.data if n else None)
+        t = C.c_uint64()
+        self._check(lib().rvs_server_submit(self._srv, C.byref(r), C.byref(t)), "rvs_server_submit")
+        self._live[t.value] = (q, d, flag, prof)
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        """(d, flag, profile, info dict) once done; None on timeout."""
+        info = RequestInfo()
+        st = lib().rvs_server_wait(self._srv, ticket, timeout_ms, C.byref(info))
+        if st == RVS_EBUSY:
+            return None
+        self._check(st, "rvs_server_wait")
+        q, d, flag, prof = self._live.pop(ticket)
+        return d, flag, prof, {"submit_s": info.submit_s, "admit_s": info.admit_s, "done_s": info.done_s,
+                               "tick": info.tick, "batch_requests": info.batch_requests}
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        self._check(lib().rvs_server_get_stats(self._srv, C.byref(s)), "rvs_server_get_stats")
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_ if k != "reserved"}
+
+    def close(self):
+        if getattr(self, "_srv", None):
+            lib().rvs_server_destroy(self._srv)
+            self._srv = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -251,6 +251,60 @@ def workload3(edges_per_robot: int = 4096, dofs=(10, 14, 18, 22)) -> EdgeWorkloa
     return EdgeWorkload(sc, batches)
 
 
+# ---------------------------------------------------------------- workload 4
+@dataclass
+class ServingWorkload:
+    """Request stream for the serving loop.  Robots 0 = xArm6 (r = 0.05,
+    ground from link 1), 1 = the repository's spatial 10-DOF chain
+    (r = 0.025, ground from link 2); the scene is workload 2's (ground + 16
+    boxes); each of the 16 boxes moves at a constant velocity s*u,
+    s ~ U[0, 0.5] m/s, u a uniform direction.  Requests arrive as a Poisson
+    process at `rate` per second; sizes are floor(exp U[0, ln 65536]); robot
+    and kind (0 verdict, 1 underestimation) are 50/50.  Each request's
+    configurations are a slice [offset, offset + n) of its robot's pool."""
+    robots: List[R.Robot]
+    scene: Scene
+    vel: np.ndarray          # (M, 3) float32, 0 for the ground
+    pools: List[np.ndarray]  # per robot (P, dofs) float32
+    arrival: np.ndarray      # (K,) seconds from the start
+    robot: np.ndarray        # (K,) int32
+    kind: np.ndarray         # (K,) int32
+    size: np.ndarray         # (K,) int64
+    offset: np.ndarray       # (K,) int64
+    tick_seconds: float = 0.01
+
+
+def workload4(duration_s: float = 5.0, rate: float = 20000.0, pool: int = 1 << 21) -> ServingWorkload:
+    rng = np.random.default_rng(1004)
+    sc = scene2(np.random.default_rng(1002))
+    robots = [R.xarm6(radius=0.05, ground_included=1), R.spatial(10, radius=0.025, ground_included=2)]
+    M = sc.num_obj
+    speed = rng.uniform(0.0, 0.5, M)
+    u = rng.normal(size=(M, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    vel = (speed[:, None] * u).astype(np.float32)
+    vel[sc.is_ground == 1] = 0.0
+    pools = [uniform_configs(rng, pool, rb.limits) for rb in robots]
+    gaps = rng.exponential(1.0 / rate, size=int(duration_s * rate * 1.2) + 16)
+    arrival = np.cumsum(gaps)
+    arrival = arrival[arrival < duration_s]
+    K = len(arrival)
+    size = np.floor(np.exp(rng.uniform(0.0, np.log(65536.0), K))).astype(np.int64)
+    size = np.clip(size, 1, 65536)
+    robot = rng.integers(0, 2, K).astype(np.int32)
+    kind = rng.integers(0, 2, K).astype(np.int32)
+    offset = np.floor(rng.random(K) * (pool - size + 1)).astype(np.int64)
+    return ServingWorkload(robots, sc, vel, pools, arrival, robot, kind, size, offset)
+
+
+def scene_at_tick(scene: Scene, vel: np.ndarray, tick: int, tick_seconds: float) -> Scene:
+    """Box positions the serving loop uses at a tick:
+    pos = pos0 + vel * (tick * tick_seconds), in float32 (no contraction)."""
+    t = np.float32(tick) * np.float32(tick_seconds)
+    pos = (scene.pos + np.asarray(vel, np.float32) * t).astype(np.float32)
+    return Scene(scene.dim, pos, scene.quat, scene.is_ground, scene.min_dist_tol)
+
+
 # ---------------------------------------------------------------- workload 5
 def workload5(n: int = 16777216, rotated_semantics: bool = True) -> ConfigBatch:
     rng = np.random.default_rng(1005)

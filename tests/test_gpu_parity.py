@@ -205,29 +205,37 @@ def test_edge_first_collision_sparse_scene(oracle, engine):
     assert np.array_equal(f, of) and (of == -1).mean() > 0.1
 
 
-@pytest.fixture(scope="module")
-def engine_classic():
-    """An engine whose FP32 bulk pass keeps one lane per configuration."""
+def _engine_with_variant(v):
     import os
     old = os.environ.get("RVS_FAST_VARIANT")
-    os.environ["RVS_FAST_VARIANT"] = "0"
-    eng = rvs.Engine(device=0)
-    if old is None:
-        del os.environ["RVS_FAST_VARIANT"]
-    else:
-        os.environ["RVS_FAST_VARIANT"] = old
-    yield eng
-    eng.close()
+    os.environ["RVS_FAST_VARIANT"] = v
+    try:
+        return rvs.Engine(device=0)
+    finally:
+        if old is None:
+            del os.environ["RVS_FAST_VARIANT"]
+        else:
+            os.environ["RVS_FAST_VARIANT"] = old
+
+
+@pytest.fixture(scope="module")
+def engines_by_variant():
+    """Engines whose FP32 bulk pass is the lane-refill ("pool") or the
+    one-lane-per-configuration ("classic") kernel."""
+    e = {"pool": _engine_with_variant("1"), "classic": _engine_with_variant("0")}
+    yield e
+    for x in e.values():
+        x.close()
 
 
 @pytest.mark.parametrize("variant", ["pool", "classic"])
 @pytest.mark.parametrize("wl", ["workload1", "workload2"])
-def test_fp32_refine_verdict_exact(oracle, engine, engine_classic, wl, variant):
+def test_fp32_refine_verdict_exact(oracle, engines_by_variant, wl, variant):
     """FP32 bulk pass + exact refinement: verdicts identical to the reference
     arithmetic, distances within 1e-5 m of the FP64 oracle (and profiles of
     collision-free configurations too), for the lane-refill and the
     one-lane-per-configuration bulk kernels."""
-    engine = engine if variant == "pool" else engine_classic
+    engine = engines_by_variant[variant]
     w = W.workload1(65536) if wl == "workload1" else W.workload2(1 << 18)
     rid = engine.load_robot(w.robot)
     eid = engine.create_env(w.scene)
