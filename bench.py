@@ -1,17 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark: distance queries per second, xArm6 with 16 boxes + ground.
 
-One step = one rvs_distance call over the whole batch of 1,048,576 joint
-configurations (xArm6, capsule radius 0.05 m, ground box + 16 random boxes,
-configurations uniform in the joint limits): per-link distance profile,
-minimum distance and collision verdict for every configuration, inputs and
-outputs resident in HBM.  The timed path is the FP32 bulk pass with exact
-near-contact refinement, whose verdicts are identical to the reference
-arithmetic; the exact path and the FP64 path are timed beside it.
+Headline step = one rvs_distance call over the whole batch of 1,048,576
+joint configurations (xArm6, capsule radius 0.05 m, ground box + 16 random
+boxes, configurations uniform in the joint limits): per-link distance
+profile, minimum distance and collision verdict for every configuration,
+inputs and outputs resident in HBM.  The timed path is the FP32 bulk pass
+with exact near-contact refinement, whose verdicts are identical to the
+reference arithmetic; the exact path and the FP64 path are timed beside it.
+After timing, the whole batch is checked against the CPU oracle (verdict
+mismatches against the reference arithmetic, distance error against the
+FP64 twin), and the oracle's reference arithmetic is timed on the host cores
+as the CPU baseline.
 
-After timing, the same batch is checked against the CPU oracle: verdict
-mismatches against the reference arithmetic (must be 0) and the largest
-distance error against the FP64 twin (must be <= 1e-5 m).
+Further sections of the same JSON line (skip with --quick):
+  workload5  22-DOF chain, 512 rotated boxes, 16,777,216 configurations:
+             oriented boxes and the reference's AABB-of-OBB reading, self-
+             collision, error against the FP64 oracle on a 65,536-configuration
+             stride-256 subsample
+  edges      chains of 10/14/18/22 DOF, 4,096 discretised edges each, first
+             colliding state per edge in one persistent launch
+  serving    Poisson request stream at 20,000 requests/s for 5 s through the
+             request server (p50 / p99 latency, throughput)
 
 Prints one JSON line.  One GPU only.
 """
@@ -27,25 +37,29 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic work per unit (fixed nominal counts used for the roofline):
-# revolute FK joint 100 flop, capsule-AABB pair 160 flop.
+# revolute FK joint 100 flop, capsule-AABB pair 160, capsule-OBB pair 196
+# (the box-frame transform of the two end points), capsule-capsule pair 60.
 FLOP_FK_JOINT = 100
 FLOP_CAPSULE_AABB = 160
+FLOP_CAPSULE_OBB = 196
+FLOP_CAPSULE_CAPSULE = 60
 PEAK_FP32_TFLOPS = 157.3
-# HBM bytes per launch of k_distance_fast on this workload from two
-# `rocprofv3 --pmc` runs: FETCH_SIZE 12,504.7 KiB x 2.0 + WRITE_SIZE
+PEAK_HBM_GBPS = 8000.0
+# HBM bytes per launch of the FP32 bulk kernel on the headline workload from
+# two `rocprofv3 --pmc` runs: FETCH_SIZE 12,504.7 KiB x 2.0 + WRITE_SIZE
 # 58,144.8 KiB x 1.0, the factors being what the same counters read for a
 # 256 MiB copy (profiles/k_distance_fast_xarm6_1M.md).
 TRAFFIC_BYTES = (2.0 * 12504.7 + 1.0 * 58144.8) * 1024
 
 
-def workload_flops(rb, scene) -> int:
+def pair_count(rb, scene) -> int:
     pairs = 0
     for i in range(rb.num_links):
         for j in range(scene.num_obj):
             if scene.is_ground[j] and i < rb.ground_included:
                 continue
             pairs += 1
-    return rb.num_dofs * FLOP_FK_JOINT + pairs * FLOP_CAPSULE_AABB
+    return pairs
 
 
 def host_cores() -> int:
@@ -53,11 +67,16 @@ def host_cores() -> int:
     return max(1, min(cores or 1, 16))
 
 
+def oracle_module():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    return O
+
+
 def cpu_baseline(w, cores: int):
     """The oracle's reference arithmetic (ref32) timed on this host: a fresh
     state per query, FK + skeleton + pair loop with the early exit."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as O
+    O = oracle_module()
     sc = O.scene_of(w.scene)
     q = w.q
     O.distance(w.robot, sc, q[:4096], 32, nthreads=cores)  # warm-up
@@ -78,13 +97,145 @@ def cpu_baseline(w, cores: int):
     return cb, res
 
 
+def timed_distance(eng, rid, eid, q, d, prof, ver, prec, steps, warmup):
+    import numpy as np
+    import torch
+    for _ in range(warmup):
+        eng.distance_device(rid, eid, q, d, prof, ver, precision=prec)
+    torch.cuda.synchronize()
+    bulk, refine, refined = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.distance_device(rid, eid, q, d, prof, ver, precision=prec, timing=True)
+        st = eng.last_timing()
+        bulk.append(st[0])
+        refine.append(st[1] if len(st) > 1 else 0.0)
+        refined.append(eng.last_refined())
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return ms, float(np.mean(bulk)), float(np.mean(refine)), int(np.mean(refined))
+
+
+def workload5_section(eng, cores: int, n: int) -> dict:
+    import numpy as np
+    import torch
+    from rpmplv2_amd import rvs, workloads as W
+    O = oracle_module()
+    w = W.workload5(n)
+    rb = w.robot
+    L = rb.num_links
+    rid = eng.load_robot(rb)
+    eo = eng.create_env(w.scene, rvs.ENV_OBB)
+    ea = eng.create_env(w.scene, rvs.ENV_AABB)
+    dev = torch.device("cuda:0")
+    q = torch.from_numpy(w.q).to(dev)
+    d = torch.empty(n, device=dev)
+    prof = torch.empty((n, L), device=dev)
+    ver = torch.empty(n, dtype=torch.uint8, device=dev)
+    tune = eng.autotune(rid, eo, q)
+    ms_o, bulk_o, ref_o, nref_o = timed_distance(eng, rid, eo, q, d, prof, ver, rvs.PREC_FP32_REFINE, 2, 1)
+    sub = np.arange(0, n, 256)
+    d_o, ver_o, free_o = d.cpu().numpy()[sub], ver.cpu().numpy()[sub], float(ver.float().mean().item())
+    ms_a, bulk_a, ref_a, nref_a = timed_distance(eng, rid, ea, q, d, prof, ver, rvs.PREC_FP32_REFINE, 3, 1)
+    d_a, ver_a, free_a = d.cpu().numpy()[sub], ver.cpu().numpy()[sub], float(ver.float().mean().item())
+    ms_r32, k_r32, _, _ = timed_distance(eng, rid, eo, q, d, prof, ver, rvs.PREC_REF32, 1, 0)
+    d64 = torch.empty(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    eng.distance_f64_device(rid, eo, q, d64, None, ver)
+    torch.cuda.synchronize()
+    ms_f64 = (time.perf_counter() - t) * 1e3
+    smin = torch.empty(n, device=dev)
+    sfree = torch.empty(n, dtype=torch.uint8, device=dev)
+    eng.self_collision_device(rid, q, smin, sfree)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    eng.self_collision_device(rid, q, smin, sfree)
+    torch.cuda.synchronize()
+    ms_self = (time.perf_counter() - t) * 1e3
+    smin_s, sfree_all = smin.cpu().numpy()[sub], float(sfree.float().mean().item())
+    del q, d, prof, ver, d64, smin, sfree
+    torch.cuda.empty_cache()
+    # oracle on the stride-256 subsample
+    qs = w.q[sub]
+    so, sa = O.scene_of(w.scene, oriented=True), O.scene_of(w.scene)
+    od32, _, ov32, _ = O.distance(rb, so, qs, 32, nthreads=cores)
+    od64, _, ov64, _ = O.distance(rb, so, qs, 64, nthreads=cores)
+    ad32, _, av32, _ = O.distance(rb, sa, qs, 32, nthreads=cores)
+    osm = O.self_min(rb, qs, nthreads=cores)
+    flop_obb = rb.num_dofs * FLOP_FK_JOINT + L * w.scene.num_obj * FLOP_CAPSULE_OBB
+    flop_aabb = rb.num_dofs * FLOP_FK_JOINT + L * w.scene.num_obj * FLOP_CAPSULE_AABB
+    bytes_cfg = rb.num_dofs * 4 + 4 + L * 4 + 1
+    return {
+        "workload": "chain22_512rotated_boxes", "configs": n, "self_pairs": len(rb.self_pairs),
+        "obb": {"configs_per_s": n / (ms_o * 1e-3), "ms": ms_o, "bulk_ms": bulk_o, "refine_ms": ref_o,
+                "refined": nref_o, "collision_free_fraction": free_o, "tuning": tune,
+                "verdict_mismatch_vs_ref32_subsample": int((ver_o != ov32).sum()),
+                "max_abs_dist_err_vs_ref64_subsample": float(np.abs(d_o - od64).max()),
+                "roofline": {"bound": "valu", "kernel": "k_distance_fast_pool<OBB> / k_distance_fast<OBB>",
+                             "flop_per_config": flop_obb,
+                             "achieved_tflops": flop_obb * n / (bulk_o * 1e-3) / 1e12,
+                             "frac": flop_obb * n / (bulk_o * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+                             "algorithmic_bytes_per_config": bytes_cfg,
+                             "hbm_GBps_algorithmic": bytes_cfg * n / (bulk_o * 1e-3) / 1e9}},
+        "aabb_of_obb_reference_semantics": {
+            "configs_per_s": n / (ms_a * 1e-3), "ms": ms_a, "bulk_ms": bulk_a, "refine_ms": ref_a,
+            "refined": nref_a, "collision_free_fraction": free_a,
+            "verdict_mismatch_vs_ref32_subsample": int((ver_a != av32).sum()),
+            "max_abs_dist_err_vs_ref32_subsample": float(np.abs(d_a - ad32).max()),
+            "nominal_flop_per_config": flop_aabb},
+        "obb_ref32_exact": {"configs_per_s": n / (ms_r32 * 1e-3), "kernel_ms": k_r32},
+        "obb_fp64": {"configs_per_s": n / (ms_f64 * 1e-3), "ms": ms_f64},
+        "self_collision": {"configs_per_s": n / (ms_self * 1e-3), "ms": ms_self, "free_fraction": sfree_all,
+                           "max_abs_min_err_vs_ref32_subsample": float(np.abs(smin_s - osm).max())},
+        "subsample": "65,536 configurations (every 256th) checked against the oracle",
+    }
+
+
+def edges_section(eng, cores: int) -> dict:
+    import numpy as np
+    import torch
+    from rpmplv2_amd import workloads as W
+    O = oracle_module()
+    w = W.workload3()
+    eid = eng.create_env(w.scene)
+    ids, q1, u, K = eng.stack_edges(w.batches)
+    dev = torch.device("cuda:0")
+    q1d, ud = torch.from_numpy(q1).to(dev), torch.from_numpy(u).to(dev)
+    first = torch.empty(len(ids), dtype=torch.int32, device=dev)
+    tune = eng.autotune_edges(eid, ids, q1d, ud, K)
+    eng.edge_first_collision_device(eid, ids, q1d, ud, K, first)
+    torch.cuda.synchronize()
+    ks, t0 = [], time.perf_counter()
+    for _ in range(5):
+        eng.edge_first_collision_device(eid, ids, q1d, ud, K, first, timing=True)
+        ks.append(eng.last_timing()[0])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / 5
+    got = first.cpu().numpy()
+    sc = O.scene_of(w.scene)
+    mism, off = 0, 0
+    for b in w.batches:
+        of = O.edge_first_collision(b.robot, sc, b.q1, b.dir, b.K, b.step, nthreads=cores)
+        mism += int((got[off:off + len(b.q1)] != of).sum())
+        off += len(b.q1)
+    states = int(K.sum())
+    return {"workload": "chains_10_14_18_22dof_64boxes_4096edges_each", "edges": len(ids), "states": states,
+            "ms": ms, "kernel_ms": float(np.mean(ks)), "edges_per_s": len(ids) / (ms * 1e-3),
+            "states_per_s_dense": states / (ms * 1e-3), "tuning": tune,
+            "first_collision_mismatch_vs_oracle": mism,
+            "edges_colliding_at_state0": int((got == 0).sum()), "edges_free": int((got == -1).sum())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
+    ap.add_argument("--w5-n", type=int, default=1 << 24)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle checks")
+    ap.add_argument("--quick", action="store_true", help="headline workload only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only; this build does not run across GPUs", file=sys.stderr)
@@ -106,32 +257,21 @@ def main():
     prof = torch.empty((n, L), dtype=torch.float32, device=dev)
     ver = torch.empty(n, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
+    tune = eng.autotune(rid, eid, q)
 
-    def timed(prec, steps, warmup):
-        for _ in range(warmup):
-            eng.distance_device(rid, eid, q, d, prof, ver, precision=prec)
-        torch.cuda.synchronize()
-        bulk, refine, refined = [], [], []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            eng.distance_device(rid, eid, q, d, prof, ver, precision=prec, timing=True)
-            st = eng.last_timing()
-            bulk.append(st[0])
-            refine.append(st[1] if len(st) > 1 else 0.0)
-            refined.append(eng.last_refined())
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / steps
-        return ms, float(np.mean(bulk)), float(np.mean(refine)), int(np.mean(refined))
-
-    ms_step, bulk_ms, refine_ms, refined = timed(rvs.PREC_FP32_REFINE, args.steps, args.warmup)
+    ms_step, bulk_ms, refine_ms, refined = timed_distance(eng, rid, eid, q, d, prof, ver, rvs.PREC_FP32_REFINE,
+                                                          args.steps, args.warmup)
     value = n / (ms_step * 1e-3)
     d_gpu, prof_gpu, ver_gpu = d.cpu().numpy(), prof.cpu().numpy(), ver.cpu().numpy()
-    ms_ref32, k_ref32, _, _ = timed(rvs.PREC_REF32, max(3, args.steps // 2), 2)
-    ms_fp64, k_fp64, _, _ = timed(rvs.PREC_FP64, max(3, args.steps // 2), 2)
+    ms_ref32, k_ref32, _, _ = timed_distance(eng, rid, eid, q, d, prof, ver, rvs.PREC_REF32, max(3, args.steps // 2), 2)
+    ms_fp64, k_fp64, _, _ = timed_distance(eng, rid, eid, q, d, prof, ver, rvs.PREC_FP64, max(3, args.steps // 2), 2)
+    del q, d, prof, ver
+    torch.cuda.empty_cache()
 
-    flop_cfg = workload_flops(w.robot, w.scene)
+    flop_cfg = w.robot.num_dofs * FLOP_FK_JOINT + pair_count(w.robot, w.scene) * FLOP_CAPSULE_AABB
     achieved = flop_cfg * n / (bulk_ms * 1e-3) / 1e12
     hbm_bytes = n * (w.robot.num_dofs * 4 + 4 + L * 4 + 1)
+    kernel = "k_distance_fast_pool" if tune["variant"] == 1 else "k_distance_fast"
     out = {
         "metric": "distance queries/sec (xArm6, 16 boxes) at verdict-exact",
         "value": value,
@@ -150,22 +290,23 @@ def main():
                    "precision": "fp32 bulk + exact near-contact refinement (band 1e-4 m)"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES,
-                     "kernel": "k_distance_fast", "kernel_ms": bulk_ms, "flop_per_config": flop_cfg,
+                     "kernel": kernel, "kernel_ms": bulk_ms, "flop_per_config": flop_cfg,
+                     "peak_kind": "FP32 vector (VALU); the path has no MFMA work",
                      "algorithmic_bytes": hbm_bytes,
                      "hbm_GBps_algorithmic": hbm_bytes / (bulk_ms * 1e-3) / 1e9},
+        "tuning": tune,
         "refine": {"configs": refined, "fraction": refined / n, "kernel": "k_distance_refine",
                    "kernel_ms": refine_ms},
         "ref32_exact": {"configs_per_s": n / (ms_ref32 * 1e-3), "kernel_ms": k_ref32},
         "fp64": {"configs_per_s": n / (ms_fp64 * 1e-3), "kernel_ms": k_fp64},
     }
+    cores = host_cores()
     if not args.no_cpu:
-        cores = host_cores()
         cb, ref = cpu_baseline(w, cores)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu_all"] = value / cb["value"]
         od, oprof, over, _ = ref
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as O
+        O = oracle_module()
         d64, prof64, ver64, _ = O.distance(w.robot, O.scene_of(w.scene), w.q, 64, nthreads=cores)
         free = (ver_gpu == 1) & (ver64 == 1)
         out["parity"] = {
@@ -174,8 +315,18 @@ def main():
             "max_abs_profile_err_vs_ref64_free": float(np.abs(prof_gpu[free] - prof64[free]).max()),
             "collision_free_fraction": float(ver_gpu.mean()),
         }
+    if not args.quick:
+        print("bench: workload 5", file=sys.stderr, flush=True)
+        out["workload5"] = workload5_section(eng, cores, args.w5_n)
+        print("bench: edges", file=sys.stderr, flush=True)
+        out["edges"] = edges_section(eng, cores)
+        eng.close()
+        print("bench: serving", file=sys.stderr, flush=True)
+        from rpmplv2_amd import serving
+        out["serving"] = serving.run(5.0)
+    else:
+        eng.close()
     print(json.dumps(out))
-    eng.close()
 
 
 if __name__ == "__main__":

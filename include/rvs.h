@@ -197,7 +197,9 @@ rvs_status rvs_edge_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const
  * environment (collision tree) or has a self-collision pair with distance
  * < 0, or -1.  Edges of several robots go in one persistent launch:
  * robots[e] selects the robot of edge e, q1/dir rows are padded to
- * row_stride floats. */
+ * row_stride floats.  robots and K are host arrays (the launch is planned on
+ * the host); with RVS_F_DEVICE, q1, dir and first are device arrays.
+ * RVS_F_TIMING records the kernel time as stage 0 of rvs_last_timing. */
 rvs_status rvs_edge_first_collision(rvs_context* ctx, int32_t env, const int32_t* robots, const float* q1,
                                     const float* dir, const int32_t* K, int64_t num_edges, int32_t row_stride,
                                     float step, int32_t* first, uint32_t flags);
@@ -216,13 +218,29 @@ typedef struct rvs_tuning {
     float ms;               /* bulk-pass time on the sample (rvs_autotune) */
 } rvs_tuning;
 
-/* Times every candidate shape on the first min(n, 262144) configurations of
+/* Times every candidate shape on the first min(n, 2097152) configurations of
  * q, keeps the fastest for later RVS_PREC_FP32_REFINE calls on (robot, env)
  * and returns it in best (may be NULL).  No counterpart in the reference. */
 rvs_status rvs_autotune(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n, uint32_t flags,
                         rvs_tuning* best);
 rvs_status rvs_set_tuning(rvs_context* ctx, int32_t robot, int32_t env, const rvs_tuning* tuning);
 rvs_status rvs_get_tuning(rvs_context* ctx, int32_t robot, int32_t env, rvs_tuning* tuning);
+
+/* Launch shape of the persistent first-collision kernel: resident waves per
+ * CU, and slots (edge states) per lane in one 64-lane work item, i.e. 64 *
+ * slots_per_lane configurations per wavefront item. */
+typedef struct rvs_edge_tuning {
+    int32_t waves_per_cu;
+    int32_t slots_per_lane;
+    float ms;
+} rvs_edge_tuning;
+/* Times waves_per_cu in {4, 8, 16, 32} x slots_per_lane in {1, 2, 4} on the
+ * given edges and keeps the fastest for later rvs_edge_first_collision calls
+ * of the context.  No counterpart in the reference. */
+rvs_status rvs_autotune_edges(rvs_context* ctx, int32_t env, const int32_t* robots, const float* q1, const float* dir,
+                              const int32_t* K, int64_t num_edges, int32_t row_stride, float step, uint32_t flags,
+                              rvs_edge_tuning* best);
+rvs_status rvs_set_edge_tuning(rvs_context* ctx, const rvs_edge_tuning* tuning);
 
 /* ---------------------------------------------------------------- serving
  * A device-resident request loop over one context (no counterpart in the

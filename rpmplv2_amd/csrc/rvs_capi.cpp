@@ -496,7 +496,10 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
     s = env_fits(c, ev.dev, false, "rvs_autotune");
     if (s != RVS_OK) return s;
     (void)hipSetDevice(c->opt.device);
-    const int64_t m = std::min<int64_t>(n, 1 << 18);   // the sample: the first m configurations
+    // the sample: the first m configurations.  It must hold several
+    // configurations per resident lane, or the lane-refill variant has
+    // nothing to refill and the choice would not carry over to the batch.
+    const int64_t m = std::min<int64_t>(n, 1 << 21);
     const int L = rb.num_links, D = rb.num_dofs;
     hipError_t e = c->arena.ensure(Carve::need({(size_t)m * D * 4, (size_t)m * 4, (size_t)m * L * 4, (size_t)m,
                                                 (size_t)m * 4, 256}));
@@ -525,7 +528,7 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
     float win_ms = 1e30f;
     for (const FastTuning& t : cand) {
         float best_ms = 1e30f;
-        for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {   // first run warms up
+        for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {   // first run warms up
             e = launch_zero_u32(count, 64, c->stream);
             if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->stream);
             if (e == hipSuccess) e = launch_distance_fast(rb.dev, ev.dev, qp, qs, m, out, idx, count, t, c->num_cus, c->stream);
@@ -793,21 +796,18 @@ rvs_status rvs_edge_is_valid(rvs_context* c, int32_t robot, int32_t env, const f
     return RVS_OK;
 }
 
-rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* robots, const float* q1,
-                                    const float* dir, const int32_t* K, int64_t num_edges, int32_t row_stride,
-                                    float step, int32_t* first, uint32_t flags) {
-    if (!c) return RVS_EINVAL;
-    if (num_edges < 0) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: negative edge count");
-    if (num_edges > 0 && (!robots || !q1 || !dir || !K || !first))
-        return fail(c, RVS_EINVAL, "rvs_edge_first_collision: null argument");
-    if (num_edges > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: too many edges");
-    std::lock_guard<std::mutex> lk(c->mu);
-    if (!env_ok(c, env)) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: bad env id");
-    const EnvRec& ev = c->envs[env];
-    if (prec_of(c, flags) == RVS_PREC_FP64)
-        return fail(c, RVS_ENOTSUP, "rvs_edge_first_collision: reference arithmetic only");
-    if (num_edges == 0) return RVS_OK;
-    // group the edges by robot, longest edges first inside a group
+namespace {
+
+// Launch plan of a first-collision call: robot groups, the edge order inside
+// each group (longest first) and the round offsets (slots before each state
+// index).  Caller holds c->mu.
+struct EdgePlan {
+    EdgeGroups gr{};
+    std::vector<int32_t> order, rounds;
+};
+
+rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, int64_t num_edges, int32_t row_stride,
+                      EdgePlan& pl) {
     std::vector<int32_t> gid;     // robot ids in order of first appearance
     std::vector<std::vector<int32_t>> members;
     for (int64_t e = 0; e < num_edges; ++e) {
@@ -824,10 +824,8 @@ rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* 
         }
         members[it - gid.begin()].push_back((int32_t)e);
     }
-    EdgeGroups gr{};
+    EdgeGroups& gr = pl.gr;
     gr.num = (int32_t)gid.size();
-    std::vector<int32_t> order, rounds;
-    int64_t items = 0;
     for (int g = 0; g < gr.num; ++g) {
         const RobotRec& rb = c->robots[gid[g]];
         if (row_stride < rb.num_dofs) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: row_stride < dofs");
@@ -836,43 +834,88 @@ rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* 
         gr.max_links = std::max(gr.max_links, rb.num_links);
         auto& m = members[g];
         std::stable_sort(m.begin(), m.end(), [&](int32_t x, int32_t y) { return K[x] > K[y]; });
-        gr.order_off[g] = (int32_t)order.size();
-        order.insert(order.end(), m.begin(), m.end());
+        gr.order_off[g] = (int32_t)pl.order.size();
+        pl.order.insert(pl.order.end(), m.begin(), m.end());
         const int32_t kmax = m.empty() ? 0 : K[m[0]];
-        gr.round_off[g] = (int32_t)rounds.size();
+        gr.round_off[g] = (int32_t)pl.rounds.size();
         gr.num_rounds[g] = kmax;
         // rounds[k] = slots before round k; edges alive in round k are a prefix
         int64_t acc = 0;
         size_t alive = m.size();
         for (int32_t k = 0; k <= kmax; ++k) {
-            rounds.push_back((int32_t)acc);
+            pl.rounds.push_back((int32_t)acc);
             while (alive > 0 && K[m[alive - 1]] <= k) --alive;
             acc += (int64_t)alive;
         }
-        if (acc > INT32_MAX / 2) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: too many states");
+        if (acc > INT32_MAX / 8) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: too many states");
         gr.num_slots[g] = (int32_t)acc;
+    }
+    return RVS_OK;
+}
+
+// Items of 64 * spl slots per group; returns the total.
+int32_t set_items(EdgeGroups& gr, int spl) {
+    int64_t items = 0;
+    const int64_t per = 64 * (int64_t)spl;
+    for (int g = 0; g < gr.num; ++g) {
         gr.item_begin[g] = (int32_t)items;
-        items += (acc + 63) / 64;
+        items += (gr.num_slots[g] + per - 1) / per;
     }
     gr.item_begin[gr.num] = (int32_t)items;
-    const size_t lds = edge_first_lds_bytes(gr, ev.dev);
+    return (int32_t)items;
+}
+
+// Runs a planned call on device arrays (q1, dir, first) with a launch shape.
+hipError_t run_edges(rvs_context* c, const EnvRec& ev, EdgePlan& pl, const float* q1, const float* dir,
+                     int32_t row_stride, float step, int32_t* first, int64_t num_edges, int32_t* od, int32_t* rd,
+                     uint32_t* counter, int waves_per_cu, int spl, bool timing) {
+    const int32_t items = set_items(pl.gr, spl);
+    const QueryParams qp = params_of(c);
+    hipError_t e = hipSuccess;
+    if (timing) e = hipEventRecord(c->ev[0], c->stream);
+    if (e == hipSuccess)
+        e = launch_edge_first(pl.gr, ev.dev, qp, q1, dir, row_stride, step, od, rd, first, num_edges, counter, items,
+                              c->num_cus * waves_per_cu, spl, c->stream);
+    if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
+    return e;
+}
+
+}  // namespace
+
+rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* robots, const float* q1,
+                                    const float* dir, const int32_t* K, int64_t num_edges, int32_t row_stride,
+                                    float step, int32_t* first, uint32_t flags) {
+    if (!c) return RVS_EINVAL;
+    if (num_edges < 0) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: negative edge count");
+    if (num_edges > 0 && (!robots || !q1 || !dir || !K || !first))
+        return fail(c, RVS_EINVAL, "rvs_edge_first_collision: null argument");
+    if (num_edges > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: too many edges");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!env_ok(c, env)) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: bad env id");
+    const EnvRec& ev = c->envs[env];
+    if (prec_of(c, flags) == RVS_PREC_FP64)
+        return fail(c, RVS_ENOTSUP, "rvs_edge_first_collision: reference arithmetic only");
+    if (num_edges == 0) return RVS_OK;
+    EdgePlan pl;
+    rvs_status s = plan_edges(c, robots, K, num_edges, row_stride, pl);
+    if (s != RVS_OK) return s;
+    const size_t lds = edge_first_lds_bytes(pl.gr, ev.dev);
     if (lds > kLdsLimit) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: robots too large for LDS staging");
     (void)hipSetDevice(c->opt.device);
-    const QueryParams qp = params_of(c);
     const size_t rows = (size_t)num_edges * row_stride;
-    hipError_t e = c->arena.ensure(Carve::need({order.size() * 4, rounds.size() * 4, 256,
+    hipError_t e = c->arena.ensure(Carve::need({pl.order.size() * 4, pl.rounds.size() * 4, 256,
                                                 (flags & RVS_F_DEVICE) ? 0 : rows * 4,
                                                 (flags & RVS_F_DEVICE) ? 0 : rows * 4,
                                                 (flags & RVS_F_DEVICE) ? 0 : (size_t)num_edges * 4}));
     if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_first_collision: staging");
     Carve cv(c->arena.p);
-    int32_t* od = cv.take<int32_t>(order.size());
-    int32_t* rd = cv.take<int32_t>(rounds.size());
+    int32_t* od = cv.take<int32_t>(pl.order.size());
+    int32_t* rd = cv.take<int32_t>(pl.rounds.size());
     uint32_t* counter = cv.take<uint32_t>(64);
     const float *qa = q1, *qd = dir;
     int32_t* fd = first;
-    e = h2d(od, order.data(), order.size() * 4, c->stream);
-    if (e == hipSuccess) e = h2d(rd, rounds.data(), rounds.size() * 4, c->stream);
+    e = h2d(od, pl.order.data(), pl.order.size() * 4, c->stream);
+    if (e == hipSuccess) e = h2d(rd, pl.rounds.data(), pl.rounds.size() * 4, c->stream);
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
         float* as = cv.take<float>(rows);
         float* ds = cv.take<float>(rows);
@@ -882,17 +925,81 @@ rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* 
         qa = as;
         qd = ds;
     }
-    if (e == hipSuccess) {
-        int dev_cus = 256;
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, c->opt.device) == hipSuccess) dev_cus = prop.multiProcessorCount;
-        const int waves = dev_cus * 16;
-        e = launch_edge_first(gr, ev.dev, qp, qa, qd, row_stride, step, od, rd, fd, num_edges, counter,
-                              (int32_t)items, waves, c->stream);
-    }
+    const bool timing = (flags & RVS_F_TIMING) != 0;
+    if (e == hipSuccess)
+        e = run_edges(c, ev, pl, qa, qd, row_stride, step, fd, num_edges, od, rd, counter, c->edge_waves_per_cu,
+                      c->edge_spl, timing);
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(first, fd, (size_t)num_edges * 4, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_first_collision");
+    if (timing) {
+        c->nstages = 1;
+        c->stage_ms[0] = 0;
+        (void)hipEventElapsedTime(&c->stage_ms[0], c->ev[0], c->ev[1]);
+    }
+    return RVS_OK;
+}
+
+rvs_status rvs_set_edge_tuning(rvs_context* c, const rvs_edge_tuning* t) {
+    if (!c || !t) return fail(c, RVS_EINVAL, "rvs_set_edge_tuning: null argument");
+    if (t->waves_per_cu < 1 || t->waves_per_cu > 64 || t->slots_per_lane < 1 || t->slots_per_lane > 16)
+        return fail(c, RVS_EINVAL, "rvs_set_edge_tuning: waves_per_cu 1..64, slots_per_lane 1..16");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->edge_waves_per_cu = t->waves_per_cu;
+    c->edge_spl = t->slots_per_lane;
+    return RVS_OK;
+}
+
+rvs_status rvs_autotune_edges(rvs_context* c, int32_t env, const int32_t* robots, const float* q1, const float* dir,
+                              const int32_t* K, int64_t num_edges, int32_t row_stride, float step, uint32_t flags,
+                              rvs_edge_tuning* best) {
+    if (!c) return RVS_EINVAL;
+    if (num_edges <= 0 || num_edges > INT32_MAX || !robots || !q1 || !dir || !K)
+        return fail(c, RVS_EINVAL, "rvs_autotune_edges: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!env_ok(c, env)) return fail(c, RVS_EINVAL, "rvs_autotune_edges: bad env id");
+    const EnvRec& ev = c->envs[env];
+    EdgePlan pl;
+    rvs_status s = plan_edges(c, robots, K, num_edges, row_stride, pl);
+    if (s != RVS_OK) return s;
+    if (edge_first_lds_bytes(pl.gr, ev.dev) > kLdsLimit)
+        return fail(c, RVS_ERANGE, "rvs_autotune_edges: robots too large for LDS staging");
+    (void)hipSetDevice(c->opt.device);
+    const size_t rows = (size_t)num_edges * row_stride;
+    hipError_t e = c->arena.ensure(Carve::need({pl.order.size() * 4, pl.rounds.size() * 4, 256, rows * 4, rows * 4,
+                                                (size_t)num_edges * 4}));
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_autotune_edges: staging");
+    Carve cv(c->arena.p);
+    int32_t* od = cv.take<int32_t>(pl.order.size());
+    int32_t* rd = cv.take<int32_t>(pl.rounds.size());
+    uint32_t* counter = cv.take<uint32_t>(64);
+    float* as = cv.take<float>(rows);
+    float* ds = cv.take<float>(rows);
+    int32_t* fd = cv.take<int32_t>(num_edges);
+    const hipMemcpyKind kind = (flags & RVS_F_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    e = h2d(od, pl.order.data(), pl.order.size() * 4, c->stream);
+    if (e == hipSuccess) e = h2d(rd, pl.rounds.data(), pl.rounds.size() * 4, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(as, q1, rows * 4, kind, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds, dir, rows * 4, kind, c->stream);
+    int win_w = 16, win_s = 1;
+    float win_ms = 1e30f;
+    for (int w : {4, 8, 16, 32}) {
+        for (int spl : {1, 2, 4}) {
+            float best_ms = 1e30f;
+            for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {   // first run warms up
+                e = run_edges(c, ev, pl, as, ds, row_stride, step, fd, num_edges, od, rd, counter, w, spl, true);
+                if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+                float ms = 0;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+                if (rep > 0) best_ms = std::min(best_ms, ms);
+            }
+            if (e != hipSuccess) return hip_fail(c, e, "rvs_autotune_edges");
+            if (best_ms < win_ms) { win_ms = best_ms; win_w = w; win_s = spl; }
+        }
+    }
+    c->edge_waves_per_cu = win_w;
+    c->edge_spl = win_s;
+    if (best) *best = rvs_edge_tuning{win_w, win_s, win_ms};
     return RVS_OK;
 }
 

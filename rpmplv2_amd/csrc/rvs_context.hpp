@@ -71,6 +71,8 @@ struct rvs_context {
     // (lane refill when links x obstacles >= 1024), 0 = one lane per
     // configuration, 1 = lane refill
     int32_t fast_variant = -1;
+    int32_t edge_waves_per_cu = 16;   // persistent first-collision kernel: resident waves per CU
+    int32_t edge_spl = 1;             // ... and slots per lane in one 64-lane item
     std::map<std::pair<int32_t, int32_t>, rvs::FastTuning> tuning;   // (robot, env) -> autotuned launch shape
     std::mutex tuning_mu;
     rvs::FastTuning tuning_for(int32_t robot, int32_t env, int32_t links, int32_t objects) {

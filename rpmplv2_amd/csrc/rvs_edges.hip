@@ -161,7 +161,7 @@ __global__ __launch_bounds__(64) void k_edge_first(EdgeGroups gr, EnvDev env, Qu
                                                    const float* __restrict__ q1, const float* __restrict__ dir,
                                                    int32_t row_stride, float step, const int32_t* __restrict__ order,
                                                    const int32_t* __restrict__ rounds, int32_t* first,
-                                                   uint32_t* counter, int32_t total_items) {
+                                                   uint32_t* counter, int32_t total_items, int spl) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<float> s = stage_env<float, OBB>(env, smem);
     const size_t base = (staged_bytes<float>(env) + 15) & ~size_t(15);
@@ -178,26 +178,29 @@ __global__ __launch_bounds__(64) void k_edge_first(EdgeGroups gr, EnvDev env, Qu
         int g = 0;
         while (g + 1 < gr.num && item >= gr.item_begin[g + 1]) ++g;
         const RobotDev& rb = gr.rb[g];
-        const int32_t slot = (item - gr.item_begin[g]) * 64 + lane;
-        if (slot >= gr.num_slots[g]) continue;
-        const int k = round_of(rounds + gr.round_off[g], gr.num_rounds[g] + 1, slot);
-        const int32_t e = order[gr.order_off[g] + (slot - rounds[gr.round_off[g] + k])];
-        if (__atomic_load_n(&first[e], __ATOMIC_RELAXED) <= k) continue;
-        const int D = rb.num_dofs;
-        const float sk = step * (float)k;
-        for (int i = 0; i < D; ++i)
-            qbuf[i * 64 + lane] = q1[(int64_t)e * row_stride + i] + sk * dir[(int64_t)e * row_stride + i];
-        bool bad = !env_free<OBB>(rb, s, env.num_obj, p, qbuf + lane, 64, rb.num_pairs ? skel + lane : nullptr);
-        for (int m = 0; !bad && m < rb.num_pairs; ++m) {
-            const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
-            auto pt = [&](int kk) {
-                return Vec3<float>{skel[(3 * kk) * 64 + lane], skel[(3 * kk + 1) * 64 + lane],
-                                   skel[(3 * kk + 2) * 64 + lane]};
-            };
-            const float sd = seg_seg<float, false>(pt(a), pt(a + 1), pt(b), pt(b + 1), p, nullptr);
-            if (sd - rb.radius[a] - rb.radius[b] < 0) bad = true;
+        // an item holds 64 * spl slots: lane takes slots lane, lane + 64, ...
+        for (int sub = 0; sub < spl; ++sub) {
+            const int32_t slot = ((item - gr.item_begin[g]) * spl + sub) * 64 + lane;
+            if (slot >= gr.num_slots[g]) break;
+            const int k = round_of(rounds + gr.round_off[g], gr.num_rounds[g] + 1, slot);
+            const int32_t e = order[gr.order_off[g] + (slot - rounds[gr.round_off[g] + k])];
+            if (__atomic_load_n(&first[e], __ATOMIC_RELAXED) <= k) continue;
+            const int D = rb.num_dofs;
+            const float sk = step * (float)k;
+            for (int i = 0; i < D; ++i)
+                qbuf[i * 64 + lane] = q1[(int64_t)e * row_stride + i] + sk * dir[(int64_t)e * row_stride + i];
+            bool bad = !env_free<OBB>(rb, s, env.num_obj, p, qbuf + lane, 64, rb.num_pairs ? skel + lane : nullptr);
+            for (int m = 0; !bad && m < rb.num_pairs; ++m) {
+                const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
+                auto pt = [&](int kk) {
+                    return Vec3<float>{skel[(3 * kk) * 64 + lane], skel[(3 * kk + 1) * 64 + lane],
+                                       skel[(3 * kk + 2) * 64 + lane]};
+                };
+                const float sd = seg_seg<float, false>(pt(a), pt(a + 1), pt(b), pt(b + 1), p, nullptr);
+                if (sd - rb.radius[a] - rb.radius[b] < 0) bad = true;
+            }
+            if (bad) atomicMin(&first[e], k);
         }
-        if (bad) atomicMin(&first[e], k);
     }
 }
 
@@ -228,7 +231,7 @@ size_t edge_first_lds_bytes(const EdgeGroups& gr, const EnvDev& env) {
 hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const QueryParams& qp, const float* q1,
                              const float* dir, int32_t row_stride, float step, const int32_t* order,
                              const int32_t* rounds, int32_t* first, int64_t num_edges, uint32_t* counter,
-                             int32_t total_items, int waves, hipStream_t st) {
+                             int32_t total_items, int waves, int spl, hipStream_t st) {
     if (num_edges <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_first_init, blocks_for(num_edges, 256), dim3(256), 0, st, first, num_edges, counter);
     if (total_items > 0) {
@@ -236,10 +239,10 @@ hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const Quer
         const size_t sm = edge_first_lds_bytes(gr, env);
         if (env.use_obb)
             hipLaunchKernelGGL(k_edge_first<true>, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride, step,
-                               order, rounds, first, counter, total_items);
+                               order, rounds, first, counter, total_items, spl);
         else
             hipLaunchKernelGGL(k_edge_first<false>, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride, step,
-                               order, rounds, first, counter, total_items);
+                               order, rounds, first, counter, total_items, spl);
     }
     hipLaunchKernelGGL(k_first_final, blocks_for(num_edges, 256), dim3(256), 0, st, first, num_edges);
     return hipGetLastError();

@@ -42,7 +42,8 @@ struct DistOut {
 constexpr int kMaxLdsObstacles = 1024;
 
 // Edges of several robots for the persistent first-collision kernel.  Group g
-// owns items [item_begin[g], item_begin[g+1]) of 64 slots each; slot s of the
+// owns items [item_begin[g], item_begin[g+1]) of 64 * spl slots each (spl =
+// slots per lane); slot s of the
 // group is state k of edge order[order_off[g] + s - rounds[round_off[g] + k]]
 // where k is the largest round with rounds[round_off[g] + k] <= s.
 constexpr int kMaxEdgeGroups = 8;
@@ -65,7 +66,7 @@ size_t edge_first_lds_bytes(const EdgeGroups& gr, const EnvDev& env);
 hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const QueryParams& qp, const float* q1,
                              const float* dir, int32_t row_stride, float step, const int32_t* order,
                              const int32_t* rounds, int32_t* first, int64_t num_edges, uint32_t* counter,
-                             int32_t total_items, int waves, hipStream_t st);
+                             int32_t total_items, int waves, int spl, hipStream_t st);
 
 // FP32 bulk pass: relaxed arithmetic, writes its results and appends every
 // configuration with an uncertain pair to idx (count in *count, which the

@@ -19,6 +19,7 @@
 // its result does not depend on what it was batched with.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -45,20 +46,28 @@ using Clock = std::chrono::steady_clock;
 constexpr size_t kGranule = 256;
 constexpr int kMotionSlots = 4;
 constexpr size_t kLds = 64 * 1024;
+// Requests of at most this many configurations travel through the server's
+// pinned staging memory: one copy per launch each way instead of one per
+// request and array (large requests are copied directly).
+constexpr int64_t kStageMax = 16384;
+constexpr size_t kStageBytes = (size_t)256 << 20;
 
 inline size_t up(size_t b) { return (b + kGranule - 1) & ~(kGranule - 1); }
 
 // First-fit sub-allocator over one device block, with coalescing on release.
+// (device memory, or pinned host memory for the staging pool)
 class Pool {
   public:
-    hipError_t init(size_t cap) {
+    hipError_t init(size_t cap, bool host = false) {
         cap_ = up(cap);
-        hipError_t e = hipMalloc(reinterpret_cast<void**>(&base_), cap_);
+        host_ = host;
+        hipError_t e = host ? hipHostMalloc(reinterpret_cast<void**>(&base_), cap_, hipHostMallocDefault)
+                            : hipMalloc(reinterpret_cast<void**>(&base_), cap_);
         if (e == hipSuccess) free_[0] = cap_;
         return e;
     }
     void destroy() {
-        if (base_) (void)hipFree(base_);
+        if (base_) (void)(host_ ? hipHostFree(base_) : hipFree(base_));
         base_ = nullptr;
     }
     bool alloc(size_t bytes, size_t* off) {
@@ -98,6 +107,7 @@ class Pool {
   private:
     char* base_ = nullptr;
     size_t cap_ = 0, used_ = 0;
+    bool host_ = false;
     std::map<size_t, size_t> free_;   // offset -> size
 };
 
@@ -136,6 +146,14 @@ struct Batch {
     Motion* motion = nullptr;
     int slot = -1;
     std::vector<int64_t> seg_host;   // source of an asynchronous copy: lives until retirement
+    // staging: the first n_staged requests (batch order) go through pinned
+    // host memory [hoff, hoff + hbytes) of the staging pool
+    size_t n_staged = 0;
+    int64_t N_staged = 0;
+    size_t hoff = 0, hbytes = 0;
+    bool staged = false;
+    int L = 0;
+    bool prof = false;
 };
 
 // Device state of one launch, carved from its pool block.
@@ -159,6 +177,8 @@ struct rvs_server {
     rvs_context* ctx = nullptr;
     rvs_server_options opt{};
     Pool pool;
+    Pool stage;                                   // pinned host staging (server thread only)
+    bool have_stage = false;
     std::vector<hipStream_t> streams;
     std::vector<int> load;
     hipStream_t upd = nullptr;
@@ -194,6 +214,23 @@ void finish(rvs_server* s, Batch& b, int32_t status) {
     s->cv_done.notify_all();
 }
 
+// Host staging layout of a launch (pinned): staged inputs, first
+// configurations and segment offsets (underestimation), staged outputs.
+struct StageLayout {
+    size_t q, qf, seg, d, flag, prof, total;
+    StageLayout(int64_t Ns, int nreq, int D, int L, bool under, bool prof_) {
+        size_t o = 0;
+        auto put = [&](size_t bytes) { const size_t at = o; o = up(o + bytes); return at; };
+        q = put((size_t)Ns * D * 4);
+        qf = put(under ? (size_t)nreq * D * 4 : 0);
+        seg = put(under ? (size_t)(nreq + 1) * 8 : 0);
+        d = put((size_t)Ns * 4);
+        flag = put((size_t)Ns);
+        prof = put(prof_ ? (size_t)Ns * L * 4 : 0);
+        total = o;
+    }
+};
+
 // Issue one launch on its stream.  Returns the first HIP error.
 hipError_t issue(rvs_server* s, Batch& b, const Layout& lay) {
     hipStream_t st = s->streams[b.stream];
@@ -221,13 +258,24 @@ hipError_t issue(rvs_server* s, Batch& b, const Layout& lay) {
     float* df = lay.under ? cv.take<float>(lay.nreq) : nullptr;
     uint8_t* vf = lay.under ? cv.take<uint8_t>(lay.nreq) : nullptr;
     int64_t* seg = lay.under ? cv.take<int64_t>(lay.nreq + 1) : nullptr;
-    // inputs
+    // inputs: staged requests (a prefix of the batch) in one copy, the rest directly
+    const StageLayout sl(b.N_staged, lay.nreq, D, L, lay.under, lay.prof);
+    char* hs = b.staged ? s->stage.base() + b.hoff : nullptr;
     int64_t off = 0;
-    for (auto& r : b.reqs) {
-        if (e == hipSuccess && r->r.n)
-            e = hipMemcpyAsync(q + off * D, r->r.q, (size_t)r->r.n * D * 4, hipMemcpyHostToDevice, st);
-        off += r->r.n;
+    for (size_t k = 0; k < b.reqs.size(); ++k) {
+        const rvs_request& r = b.reqs[k]->r;
+        if (b.staged && k < b.n_staged) {
+            std::memcpy(hs + sl.q + (size_t)off * D * 4, r.q, (size_t)r.n * D * 4);
+            if (lay.under) std::memcpy(hs + sl.qf + k * D * 4, r.q, (size_t)D * 4);
+        } else {
+            if (e == hipSuccess && r.n)
+                e = hipMemcpyAsync(q + off * D, r.q, (size_t)r.n * D * 4, hipMemcpyHostToDevice, st);
+            if (lay.under && b.staged) std::memcpy(hs + sl.qf + k * D * 4, r.q, (size_t)D * 4);
+        }
+        off += r.n;
     }
+    if (b.staged && b.N_staged && e == hipSuccess)
+        e = hipMemcpyAsync(q, hs + sl.q, (size_t)b.N_staged * D * 4, hipMemcpyHostToDevice, st);
     if (!lay.under) {
         DistOut out{};
         out.d = d;
@@ -245,11 +293,21 @@ hipError_t issue(rvs_server* s, Batch& b, const Layout& lay) {
     } else {
         // nearest points of every request's first configuration, then the
         // underestimation of all configurations against their request's set
-        off = 0;
-        for (size_t k = 0; k < b.reqs.size(); ++k) {
+        if (b.staged) {
+            std::memcpy(hs + sl.seg, b.seg_host.data(), b.seg_host.size() * 8);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(qf + k * D, q + off * D, (size_t)D * 4, hipMemcpyDeviceToDevice, st);
-            off += b.reqs[k]->r.n;
+                e = hipMemcpyAsync(qf, hs + sl.qf, (size_t)lay.nreq * D * 4, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(seg, hs + sl.seg, b.seg_host.size() * 8, hipMemcpyHostToDevice, st);
+        } else {
+            off = 0;
+            for (size_t k = 0; k < b.reqs.size(); ++k) {
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(qf + k * D, q + off * D, (size_t)D * 4, hipMemcpyDeviceToDevice, st);
+                off += b.reqs[k]->r.n;
+            }
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(seg, b.seg_host.data(), b.seg_host.size() * 8, hipMemcpyHostToDevice, st);
         }
         DistOut o1{};
         o1.d = df;
@@ -257,24 +315,45 @@ hipError_t issue(rvs_server* s, Batch& b, const Layout& lay) {
         o1.verdict = vf;
         if (e == hipSuccess) e = launch_distance_ref32(rb, env, s->qp, qf, lay.nreq, o1, st);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(seg, b.seg_host.data(), b.seg_host.size() * 8, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
             e = launch_underestimation_seg(rb, env, s->qp, q, N, nps, seg, lay.nreq, d, prof, flag, st);
     }
-    // outputs
+    // outputs: the staged prefix into staging (scattered at retirement), the rest directly
+    if (b.staged && b.N_staged) {
+        if (e == hipSuccess) e = hipMemcpyAsync(hs + sl.d, d, (size_t)b.N_staged * 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(hs + sl.flag, flag, (size_t)b.N_staged, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && lay.prof)
+            e = hipMemcpyAsync(hs + sl.prof, prof, (size_t)b.N_staged * L * 4, hipMemcpyDeviceToHost, st);
+    }
     off = 0;
-    for (auto& r : b.reqs) {
-        const int64_t n = r->r.n;
-        if (n) {
-            if (e == hipSuccess) e = hipMemcpyAsync(r->r.d, d + off, (size_t)n * 4, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipMemcpyAsync(r->r.flag, flag + off, (size_t)n, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess && r->r.profile)
-                e = hipMemcpyAsync(r->r.profile, prof + off * L, (size_t)n * L * 4, hipMemcpyDeviceToHost, st);
+    for (size_t k = 0; k < b.reqs.size(); ++k) {
+        const rvs_request& r = b.reqs[k]->r;
+        const int64_t n = r.n;
+        if (n && !(b.staged && k < b.n_staged)) {
+            if (e == hipSuccess) e = hipMemcpyAsync(r.d, d + off, (size_t)n * 4, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(r.flag, flag + off, (size_t)n, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess && r.profile)
+                e = hipMemcpyAsync(r.profile, prof + off * L, (size_t)n * L * 4, hipMemcpyDeviceToHost, st);
         }
         off += n;
     }
     if (e == hipSuccess) e = hipEventRecord(b.done, st);
     return e;
+}
+
+// Copies the staged outputs of a finished launch to the requests' buffers.
+void scatter(rvs_server* s, Batch& b) {
+    if (!b.staged || !b.N_staged) return;
+    const StageLayout sl(b.N_staged, (int)b.reqs.size(), b.reqs.front()->dofs, b.L, b.reqs.front()->r.kind ==
+                         RVS_REQ_UNDERESTIMATION, b.prof);
+    const char* hs = s->stage.base() + b.hoff;
+    int64_t off = 0;
+    for (size_t k = 0; k < b.n_staged; ++k) {
+        const rvs_request& r = b.reqs[k]->r;
+        std::memcpy(r.d, hs + sl.d + (size_t)off * 4, (size_t)r.n * 4);
+        std::memcpy(r.flag, hs + sl.flag + (size_t)off, (size_t)r.n);
+        if (r.profile) std::memcpy(r.profile, hs + sl.prof + (size_t)off * b.L * 4, (size_t)r.n * b.L * 4);
+        off += r.n;
+    }
 }
 
 // Called with s->mu held.
@@ -307,7 +386,9 @@ bool retire(rvs_server* s) {
         Batch& b = s->inflight[i];
         const hipError_t q = hipEventQuery(b.done);
         if (q == hipErrorNotReady) { ++i; continue; }
+        if (q == hipSuccess) scatter(s, b);
         finish(s, b, q == hipSuccess ? RVS_OK : RVS_EDEVICE);
+        if (b.staged) s->stage.release(b.hoff, b.hbytes);
         if (q != hipSuccess) {
             std::lock_guard<std::mutex> lk(s->mu);
             s->err = std::string("launch failed: ") + hipGetErrorString(q);
@@ -364,6 +445,26 @@ bool form_batch(rvs_server* s, Batch& b, Layout& lay) {
     for (size_t i = take.size(); i-- > 0;) {
         b.reqs.insert(b.reqs.begin(), s->pending[take[i]]);
         s->pending.erase(s->pending.begin() + (long)take[i]);
+    }
+    // small requests first: they form the staged prefix of the launch
+    std::stable_partition(b.reqs.begin(), b.reqs.end(),
+                          [](const std::shared_ptr<Req>& r) { return r->r.n <= kStageMax; });
+    b.L = lay.L;
+    b.prof = lay.prof;
+    if (s->have_stage) {
+        for (auto& r : b.reqs) {
+            if (r->r.n > kStageMax) break;
+            b.n_staged++;
+            b.N_staged += r->r.n;
+        }
+        const StageLayout sl(b.N_staged, lay.nreq, lay.D, lay.L, lay.under, lay.prof);
+        if (sl.total && s->stage.alloc(sl.total, &b.hoff)) {
+            b.staged = true;
+            b.hbytes = sl.total;
+        } else {
+            b.n_staged = 0;
+            b.N_staged = 0;
+        }
     }
     if (lay.under) {
         b.seg_host.resize(b.reqs.size() + 1);
@@ -429,6 +530,7 @@ void serve(rvs_server* s) {
                 (void)hipStreamSynchronize(s->streams[fb.stream]);
                 finish(s, fb, RVS_EDEVICE);
                 s->pool.release(fb.off, fb.bytes);
+                if (fb.staged) s->stage.release(fb.hoff, fb.hbytes);
                 if (fb.motion) fb.motion->slot[fb.slot].refs--;
                 s->load[fb.stream]--;
                 s->spare.push_back(fb.done);
@@ -486,6 +588,7 @@ rvs_status rvs_server_create(rvs_context* c, const rvs_server_options* opt, rvs_
     }
     hipError_t e = hipSetDevice(c->opt.device);
     if (e == hipSuccess) e = s->pool.init((size_t)o.memory_budget);
+    if (e == hipSuccess) s->have_stage = s->stage.init(kStageBytes, true) == hipSuccess;
     for (int k = 0; k < o.num_streams && e == hipSuccess; ++k) {
         hipStream_t st = nullptr;
         e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
@@ -526,6 +629,7 @@ void rvs_server_destroy(rvs_server* s) {
     for (auto st : s->streams) (void)hipStreamDestroy(st);
     if (s->upd) (void)hipStreamDestroy(s->upd);
     s->pool.destroy();
+    if (s->have_stage) s->stage.destroy();
     delete s;
 }
 

@@ -85,6 +85,10 @@ class Tuning(C.Structure):
     _fields_ = [("variant", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("ms", C.c_float)]
 
 
+class EdgeTuning(C.Structure):
+    _fields_ = [("waves_per_cu", C.c_int32), ("slots_per_lane", C.c_int32), ("ms", C.c_float)]
+
+
 REQ_VERDICT = 0
 REQ_UNDERESTIMATION = 1
 RVS_EBUSY = 6
@@ -93,7 +97,7 @@ EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds", "rvs_distance",
            "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_self_distance",
            "rvs_self_collision", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
-           "rvs_autotune", "rvs_set_tuning", "rvs_get_tuning", "rvs_server_default_options", "rvs_server_create", "rvs_server_destroy", "rvs_server_set_motion",
+           "rvs_autotune", "rvs_set_tuning", "rvs_get_tuning", "rvs_autotune_edges", "rvs_set_edge_tuning", "rvs_server_default_options", "rvs_server_create", "rvs_server_destroy", "rvs_server_set_motion",
            "rvs_server_submit", "rvs_server_wait", "rvs_server_get_stats", "rvs_server_now",
            "rvs_server_last_error", "rvs_host_alloc", "rvs_host_free"]
 
@@ -134,6 +138,8 @@ def lib():
         L.rvs_self_collision.argtypes = [ctx, i32, vp, i64, vp, vp, u32]
         L.rvs_autotune.argtypes = [ctx, i32, i32, vp, i64, u32, C.POINTER(Tuning)]
         L.rvs_set_tuning.argtypes = [ctx, i32, i32, C.POINTER(Tuning)]
+        L.rvs_autotune_edges.argtypes = [ctx, i32, vp, vp, vp, vp, i64, i32, C.c_float, u32, C.POINTER(EdgeTuning)]
+        L.rvs_set_edge_tuning.argtypes = [ctx, C.POINTER(EdgeTuning)]
         L.rvs_get_tuning.argtypes = [ctx, i32, i32, C.POINTER(Tuning)]
         L.rvs_server_default_options.argtypes = [C.POINTER(ServerOptions)]
         L.rvs_server_default_options.restype = None
@@ -395,19 +401,55 @@ class Engine:
                     "rvs_edge_first_collision")
         return out
 
-    def edge_batches(self, env: int, batches):
-        """Workload-3 helper: stack EdgeBatch objects (one robot each) into a
-        single first-collision call; returns one array per batch."""
+    def stack_edges(self, batches):
+        """Stack EdgeBatch objects (one robot each; robots loaded once and
+        cached) into (robot_ids, q1, dir, K) rows padded to the widest robot."""
+        cache = self.__dict__.setdefault("_edge_robot_ids", {})
         ids, rows_q, rows_u, ks = [], [], [], []
         width = max(b.robot.num_dofs for b in batches)
         for b in batches:
-            rid = self.load_robot(b.robot)
+            key = id(b.robot)
+            if key not in cache:
+                cache[key] = (self.load_robot(b.robot), b.robot)
+            rid = cache[key][0]
             n = len(b.q1)
             pq = np.zeros((n, width), np.float32); pq[:, :b.robot.num_dofs] = b.q1
             pu = np.zeros((n, width), np.float32); pu[:, :b.robot.num_dofs] = b.dir
             ids.append(np.full(n, rid, np.int32)); rows_q.append(pq); rows_u.append(pu); ks.append(b.K)
-        first = self.edge_first_collision(env, np.concatenate(ids), np.concatenate(rows_q), np.concatenate(rows_u),
-                                          np.concatenate(ks), batches[0].step)
+        return np.concatenate(ids), np.concatenate(rows_q), np.concatenate(rows_u), np.concatenate(ks)
+
+    def edge_first_collision_device(self, env: int, robot_ids, q1, direction, K, first, step: float = 0.05,
+                                    timing: bool = False):
+        """robot_ids and K host arrays; q1, direction and first device tensors."""
+        robot_ids = np.ascontiguousarray(robot_ids, np.int32)
+        K = np.ascontiguousarray(K, np.int32)
+        fl = F_DEVICE | (F_TIMING if timing else 0)
+        self._check(lib().rvs_edge_first_collision(self._ctx, env, _ptr(robot_ids), _ptr(q1), _ptr(direction), _ptr(K),
+                                                   len(robot_ids), q1.shape[1], C.c_float(step), _ptr(first), fl),
+                    "rvs_edge_first_collision")
+
+    def autotune_edges(self, env: int, robot_ids, q1, direction, K, step: float = 0.05) -> dict:
+        robot_ids = np.ascontiguousarray(robot_ids, np.int32)
+        K = np.ascontiguousarray(K, np.int32)
+        dev = _is_torch(q1)
+        if not dev:
+            q1 = np.ascontiguousarray(q1, np.float32)
+            direction = np.ascontiguousarray(direction, np.float32)
+        t = EdgeTuning()
+        self._check(lib().rvs_autotune_edges(self._ctx, env, _ptr(robot_ids), _ptr(q1), _ptr(direction), _ptr(K),
+                                             len(robot_ids), q1.shape[1], C.c_float(step), F_DEVICE if dev else 0,
+                                             C.byref(t)), "rvs_autotune_edges")
+        return {"waves_per_cu": t.waves_per_cu, "slots_per_lane": t.slots_per_lane, "ms": t.ms}
+
+    def set_edge_tuning(self, waves_per_cu: int, slots_per_lane: int):
+        t = EdgeTuning(waves_per_cu, slots_per_lane, 0.0)
+        self._check(lib().rvs_set_edge_tuning(self._ctx, C.byref(t)), "rvs_set_edge_tuning")
+
+    def edge_batches(self, env: int, batches):
+        """Workload-3 helper: stack EdgeBatch objects (one robot each) into a
+        single first-collision call; returns one array per batch."""
+        ids, q1, u, K = self.stack_edges(batches)
+        first = self.edge_first_collision(env, ids, q1, u, K, batches[0].step)
         out, o = [], 0
         for b in batches:
             out.append(first[o:o + len(b.q1)])

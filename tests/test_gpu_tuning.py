@@ -4,7 +4,7 @@ configuration), and rvs_autotune keeps one of the candidate shapes."""
 import numpy as np
 import pytest
 
-from rpmplv2_amd import rvs, workloads as W
+from rpmplv2_amd import robots as R, rvs, workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -36,3 +36,34 @@ def test_launch_shapes_agree(engine, wl):
     assert engine.get_tuning(rid, eid)["variant"] == best["variant"]
     d, prof, ver, _ = engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
     assert np.array_equal(ver, ref[2]) and np.array_equal(d, ref[0])
+
+
+def test_edge_launch_shapes_agree(oracle, engine):
+    """Every (waves per CU, slots per lane) shape of the persistent
+    first-collision launch gives the oracle's first colliding state."""
+    rng = np.random.default_rng(12)
+    sc = W._boxes(rng, 8, 0.05, 0.1, 1.5)
+    eid = engine.create_env(sc)
+    batches = []
+    for n in (10, 14):
+        rb = R.chain(rng.uniform(0.05, 0.1, n), radius=0.01, self_collision=True)
+        m = 300
+        q1 = rng.uniform(-0.3, 0.3, (m, n)).astype(np.float32)
+        u = rng.normal(size=(m, n))
+        u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+        batches.append(W.EdgeBatch(rb, q1, u, rng.integers(1, 200, m).astype(np.int32)))
+    osc = oracle.scene_of(sc)
+    expect = [oracle.edge_first_collision(b.robot, osc, b.q1, b.dir, b.K, b.step) for b in batches]
+    for w in (4, 32):
+        for spl in (1, 2, 4):
+            engine.set_edge_tuning(w, spl)
+            got = engine.edge_batches(eid, batches)
+            for g, e in zip(got, expect):
+                assert np.array_equal(g, e), (w, spl)
+    ids, q1, u, K = engine.stack_edges(batches)
+    best = engine.autotune_edges(eid, ids, q1, u, K)
+    print("edge autotune:", best)
+    assert best["waves_per_cu"] in (4, 8, 16, 32) and best["slots_per_lane"] in (1, 2, 4)
+    got = engine.edge_batches(eid, batches)
+    for g, e in zip(got, expect):
+        assert np.array_equal(g, e)
