@@ -46,10 +46,10 @@ FLOP_CAPSULE_CAPSULE = 60
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBPS = 8000.0
 # HBM bytes per launch of the FP32 bulk kernel on the headline workload from
-# two `rocprofv3 --pmc` runs: FETCH_SIZE 12,504.7 KiB x 2.0 + WRITE_SIZE
-# 58,144.8 KiB x 1.0, the factors being what the same counters read for a
+# two `rocprofv3 --pmc` runs: FETCH_SIZE 12,586.2 KiB x 2.0 + WRITE_SIZE
+# 57,044.2 KiB x 1.0, the factors being what the same counters read for a
 # 256 MiB copy (profiles/k_distance_fast_xarm6_1M.md).
-TRAFFIC_BYTES = (2.0 * 12504.7 + 1.0 * 58144.8) * 1024
+TRAFFIC_BYTES = (2.0 * 12586.2 + 1.0 * 57044.2) * 1024
 
 
 def pair_count(rb, scene) -> int:
