@@ -96,6 +96,8 @@ def _declare(L):
         "orc_capsule_obb32": (C.c_float, [f32p, f32p, C.c_float, f32p, Pp, f32p, i32p]),
         "orc_capsule_obb64": (C.c_double, [f64p, f64p, C.c_double, f64p, Pp, f64p, i32p]),
         "orc_self_min32": (None, [R, Pp, f32p, C.c_int64, i32p, C.c_int32, f32p, C.c_int32]),
+        "orc_distance_cull32": (None, [R, Pp, S, f32p, C.c_int64, f32p, f32p, u8p, _p(C.c_int64), C.c_int32]),
+        "orc_distance_cull64": (None, [R, Pp, S, f32p, C.c_int64, f64p, f64p, u8p, _p(C.c_int64), C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -314,3 +316,19 @@ def capsule_obb(A, B, r, obb, precision=32, params=None):
     fn = lib().orc_capsule_obb32 if precision == 32 else lib().orc_capsule_obb64
     d = fn(_ptr(a, ct), _ptr(b, ct), dt(r), _ptr(o, ct), C.byref(p), _ptr(np_, ct), C.byref(v))
     return d, np_, v.value
+
+
+def distance_cull(rb, scene: Scene, q, precision=32, params=None, nthreads=8):
+    """computeDistance with the broad phase; returns (d, profile, verdict,
+    (pairs evaluated, pairs skipped))."""
+    p = params or params_struct()
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+    n, L = len(q), rb.num_links
+    dt, ct = (np.float32, C.c_float) if precision == 32 else (np.float64, C.c_double)
+    d = np.zeros(n, dt); prof = np.zeros((n, L), dt); ver = np.zeros(n, np.uint8)
+    counts = np.zeros(2, np.int64)
+    rs = robot_struct(rb)
+    fn = lib().orc_distance_cull32 if precision == 32 else lib().orc_distance_cull64
+    fn(C.byref(rs), C.byref(p), C.byref(scene.s), _ptr(q, C.c_float), n, _ptr(d, ct), _ptr(prof, ct),
+       _ptr(ver, C.c_uint8), _ptr(counts, C.c_int64), nthreads)
+    return d, prof, ver, (int(counts[0]), int(counts[1]))

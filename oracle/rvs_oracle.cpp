@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdint>
+#include <mutex>
 #include <thread>
 #include <vector>
 #include <algorithm>
@@ -665,10 +666,49 @@ bool collision_obb(const T* A, const T* B, T r, const T* obb, const P<T>& pr) {
     return collision_box(a, b, r, box, pr);
 }
 
+// Broad phase (build optimisation, no reference counterpart): Euclidean gap
+// between the segment's bounding box and a box that contains the obstacle
+// (the AABB itself, or the tight world AABB of an oriented box).  It is a
+// lower bound of every distance the case tree can return for the pair except
+// its snaps to 0 / -r, which need the raw distance below EQUALITY_THRESHOLD
+// or contact.  A pair with gap > max(prof + r, EQUALITY_THRESHOLD) + margin
+// therefore cannot lower the link's running minimum nor end the loop, and
+// may be skipped without changing any result (nearest points excepted: they
+// are stored for every pair, so the broad phase is off when they are wanted).
+template <typename T> inline T seg_box_gap(const T* A, const T* B, const T* lo, const T* hi) {
+    T g2 = 0;
+    for (int k = 0; k < 3; ++k) {
+        const T smin = std::min(A[k], B[k]), smax = std::max(A[k], B[k]);
+        const T g = std::max(std::max(lo[k] - smax, smin - hi[k]), T(0));
+        g2 += g * g;
+    }
+    return std::sqrt(g2);
+}
+
+template <typename T> void cull_box(const SceneT<T>& sc, int j, T* lo, T* hi) {
+    if (sc.obb.empty()) {
+        for (int k = 0; k < 3; ++k) { lo[k] = sc.obs[6 * j + k]; hi[k] = sc.obs[6 * j + 3 + k]; }
+        return;
+    }
+    const T* c = &sc.obb[15 * j];
+    const T* R = c + 3;
+    const T* h = c + 12;
+    for (int i = 0; i < 3; ++i) {
+        const T e = std::fabs(R[3 * i]) * h[0] + std::fabs(R[3 * i + 1]) * h[1] + std::fabs(R[3 * i + 2]) * h[2];
+        lo[i] = c[i] - e;
+        hi[i] = c[i] + e;
+    }
+}
+
+struct CullCount {
+    int64_t evaluated = 0, culled = 0;
+};
+
 // computeDistance: RealVectorSpace.cpp:318-392 (fresh state, compute_again)
 template <typename T>
 void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, const T* Pk,
-                  T* d_out, T* prof, T* np_out /* M*L*6 or null */, uint8_t* verdict) {
+                  T* d_out, T* prof, T* np_out /* M*L*6 or null */, uint8_t* verdict,
+                  CullCount* cull = nullptr) {
     const int L = rb->num_links;
     T d_c = pr.maxd;
     for (int i = 0; i < L; ++i) prof[i] = 0;
@@ -682,6 +722,15 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
             T d_tmp;
             T np[6];
             int v = 1;
+            if (cull && !np_out) {
+                T lo[3], hi[3];
+                cull_box(sc, j, lo, hi);
+                if (seg_box_gap(A, B, lo, hi) > std::max(prof[i] + r, pr.eq) + T(1e-5)) {
+                    cull->culled++;
+                    continue;
+                }
+                cull->evaluated++;
+            }
             if (sc.ground[j] && i < rb->ground_included) {
                 d_tmp = pr.maxd;
                 np[0] = np[1] = np[2] = 0; np[3] = 0; np[4] = 0; np[5] = -pr.maxd;
@@ -735,21 +784,29 @@ void parallel_for(int64_t n, int nthreads, F&& fn) {
 
 template <typename T>
 void distance_batch(const orc_robot* rb, const orc_params* p, const orc_scene* scn, const float* q, int64_t n,
-                    T* d, T* profile, T* np, uint8_t* verdict, int nthreads) {
+                    T* d, T* profile, T* np, uint8_t* verdict, int nthreads, int64_t* counts = nullptr) {
     P<T> pr(p);
     SceneT<T> sc(scn);
     std::vector<Seg> segs = build_segments(rb);
     const int L = rb->num_links, D = rb->num_dofs;
+    std::mutex mu;
     parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
         std::vector<T> Pk(3 * (L + 1));
         std::vector<T> prof(L);
+        CullCount cc;
         for (int64_t c = b; c < e; ++c) {
             skeleton<T>(rb, segs, q + c * D, Pk.data());
             T dd; uint8_t v;
-            distance_one(rb, pr, sc, Pk.data(), &dd, prof.data(), np ? np + c * (int64_t)sc.M * L * 6 : nullptr, &v);
+            distance_one(rb, pr, sc, Pk.data(), &dd, prof.data(), np ? np + c * (int64_t)sc.M * L * 6 : nullptr, &v,
+                         counts ? &cc : nullptr);
             if (d) d[c] = dd;
             if (profile) for (int i = 0; i < L; ++i) profile[c * L + i] = prof[i];
             if (verdict) verdict[c] = v;
+        }
+        if (counts) {
+            std::lock_guard<std::mutex> lk(mu);
+            counts[0] += cc.evaluated;
+            counts[1] += cc.culled;
         }
     });
 }
@@ -1052,6 +1109,17 @@ void orc_edge_first_collision32(const orc_robot* rb, const orc_params* p, const 
             first[c] = hit;
         }
     });
+}
+
+void orc_distance_cull32(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q, int64_t n,
+                         float* d, float* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads) {
+    counts[0] = counts[1] = 0;
+    distance_batch<float>(rb, p, sc, q, n, d, profile, nullptr, verdict, nthreads, counts);
+}
+void orc_distance_cull64(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q, int64_t n,
+                         double* d, double* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads) {
+    counts[0] = counts[1] = 0;
+    distance_batch<double>(rb, p, sc, q, n, d, profile, nullptr, verdict, nthreads, counts);
 }
 
 float orc_capsule_obb32(const float A[3], const float B[3], float r, const float obb[15], const orc_params* p,

@@ -151,6 +151,13 @@ void box_obb(const rvs_box& b, float out[15]) {
     for (int k = 0; k < 3; ++k) out[12 + k] = 0.5f * b.dim[k];
 }
 
+void obb_extent(const float o[15], float ext[3]) {
+    for (int i = 0; i < 3; ++i) {
+        const float e = std::fabs(o[3 + 3 * i]) * o[12] + std::fabs(o[4 + 3 * i]) * o[13] + std::fabs(o[5 + 3 * i]) * o[14];
+        ext[i] = e * (1.0f + 1e-6f) + 1e-7f;
+    }
+}
+
 QueryParams params_of(const rvs_context* c) {
     QueryParams qp;
     qp.eq = c->opt.equality_threshold;
@@ -317,11 +324,19 @@ rvs_status rvs_env_create(rvs_context* c, const rvs_box* boxes, int32_t num, int
     r.semantics = semantics;
     r.boxes.assign(boxes, boxes + num);
     r.aabb.resize((size_t)num * 6);
-    std::vector<float> tol(num), obb((size_t)num * 15);
+    r.obb.resize((size_t)num * 15);
+    r.cull.resize((size_t)num * 6);
+    std::vector<float> tol(num);
     std::vector<uint8_t> fl(num);
     for (int j = 0; j < num; ++j) {
         box_bounds(boxes[j], &r.aabb[6 * j]);
-        box_obb(boxes[j], &obb[15 * j]);
+        box_obb(boxes[j], &r.obb[15 * j]);
+        float ext[3];
+        obb_extent(&r.obb[15 * j], ext);
+        for (int k = 0; k < 3; ++k) {
+            r.cull[6 * j + k] = r.obb[15 * j + k] - ext[k];
+            r.cull[6 * j + 3 + k] = r.obb[15 * j + k] + ext[k];
+        }
         tol[j] = boxes[j].min_dist_tol;
         fl[j] = (uint8_t)((boxes[j].flags & RVS_BOX_GROUND ? kObjGround : 0) |
                           (boxes[j].flags & RVS_BOX_DYNAMIC ? kObjDynamic : 0));
@@ -330,11 +345,14 @@ rvs_status rvs_env_create(rvs_context* c, const rvs_box* boxes, int32_t num, int
     hipError_t e = upload(r.allocs, r.aabb.data(), r.aabb.size(), &r.dev.aabb);
     if (e == hipSuccess) e = upload(r.allocs, tol.data(), tol.size(), &r.dev.tol);
     if (e == hipSuccess) e = upload(r.allocs, fl.data(), fl.size(), &r.dev.flags);
-    if (e == hipSuccess) e = upload(r.allocs, obb.data(), obb.size(), &r.dev.obb);
+    if (e == hipSuccess) e = upload(r.allocs, r.obb.data(), r.obb.size(), &r.dev.obb);
+    if (e == hipSuccess) e = upload(r.allocs, r.cull.data(), r.cull.size(), &r.dev.cull);
     if (e != hipSuccess) {
         for (void* p : r.allocs) (void)hipFree(p);
         return hip_fail(c, e, "rvs_env_create");
     }
+    // reference semantics: the broad phase bounds each box by its AABB itself
+    if (semantics == RVS_ENV_AABB) r.dev.cull = r.dev.aabb;
     r.dev.num_obj = num;
     r.dev.use_obb = semantics == RVS_ENV_OBB;
     c->envs.push_back(std::move(r));
@@ -351,21 +369,28 @@ rvs_status rvs_env_update(rvs_context* c, int32_t env_id, const int32_t* idx, co
     EnvRec& r = c->envs[env_id];
     for (int k = 0; k < count; ++k)
         if (idx[k] < 0 || idx[k] >= (int)r.boxes.size()) return fail(c, RVS_EINVAL, "rvs_env_update: bad box index");
+    if (count == 0) return RVS_OK;
     (void)hipSetDevice(c->opt.device);
     for (int k = 0; k < count; ++k) {
         const int j = idx[k];
         for (int i = 0; i < 3; ++i) r.boxes[j].pos[i] = pos[3 * k + i];
         box_bounds(r.boxes[j], &r.aabb[6 * j]);
-        hipError_t e = hipMemcpyAsync((void*)(r.dev.aabb + 6 * j), &r.aabb[6 * j], 6 * sizeof(float),
-                                      hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) {
-            float o[15];
-            box_obb(r.boxes[j], o);
-            e = hipMemcpyAsync((void*)(r.dev.obb + 15 * j), o, 15 * sizeof(float), hipMemcpyHostToDevice, c->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        box_obb(r.boxes[j], &r.obb[15 * j]);
+        float ext[3];
+        obb_extent(&r.obb[15 * j], ext);
+        for (int i = 0; i < 3; ++i) {
+            r.cull[6 * j + i] = r.obb[15 * j + i] - ext[i];
+            r.cull[6 * j + 3 + i] = r.obb[15 * j + i] + ext[i];
         }
-        if (e != hipSuccess) return hip_fail(c, e, "rvs_env_update");
     }
+    // the tables are small (<= 1024 boxes): copy them whole, once
+    hipError_t e = hipMemcpyAsync((void*)r.dev.aabb, r.aabb.data(), r.aabb.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync((void*)r.dev.obb, r.obb.data(), r.obb.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && r.dev.cull != r.dev.aabb)
+        e = hipMemcpyAsync((void*)r.dev.cull, r.cull.data(), r.cull.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_env_update");
     return RVS_OK;
 }
 

@@ -47,7 +47,7 @@ struct EnvRec {
     EnvDev dev{};
     std::vector<void*> allocs;
     std::vector<rvs_box> boxes;
-    std::vector<float> aabb;
+    std::vector<float> aabb, obb, cull;   // host copies of the device tables
     int semantics = RVS_ENV_AABB;
 };
 
@@ -110,6 +110,9 @@ struct Carve {
 bool box_identity(const rvs_box& b);
 void box_bounds(const rvs_box& b, float out[6]);
 void box_obb(const rvs_box& b, float out[15]);
+// Half extents of the tight world AABB of an oriented box (|R| h, rounded up
+// by one part in 10^6 so that it contains the box despite float rounding).
+void obb_extent(const float obb[15], float ext[3]);
 QueryParams params_of(const rvs_context* c);
 
 }  // namespace rvs

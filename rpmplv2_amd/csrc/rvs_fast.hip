@@ -189,7 +189,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
         const float r = rb.radius[i];
         float prof = p.maxd;
+        Vec3<float> slo, shi;
+        seg_bounds(P0, P1, slo, shi);
         for (int j = 0; j < M; ++j) {
+            if (cull_pair(env.cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) continue;
             const float dt = (s.gnd[j] && i < rb.ground_included) ? p.maxd
                                                                    : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
             prof = smin(prof, dt);
@@ -279,8 +282,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const bool gnd_skip = i < rb.ground_included;
         float prof = p.maxd;
         bool active = busy, hit = false;
+        Vec3<float> slo, shi;
+        seg_bounds(P0, P1, slo, shi);
         for (int j = 0; j < M; ++j) {
-            if (active) {
+            if (active && !cull_pair(env.cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
                 const float dt = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
                 prof = smin(prof, dt);
                 if (prof <= 0) { hit = true; active = false; }

@@ -19,6 +19,8 @@ struct EnvDev {
     const float* tol;         // [M] min_dist_tol
     const uint8_t* flags;     // [M] kObj*
     const float* obb;         // [M][15] center(3), rotation rows(9), half extents(3)
+    const float* cull;        // [M][6] box containing the obstacle for the broad phase:
+                              // the AABB (reference semantics) or the OBB's tight world AABB
 };
 
 struct QueryParams {
@@ -115,10 +117,12 @@ struct MotionDev {
     const float* rad;     // [M] |dim / 2| (rotated boxes)
     const uint8_t* ident; // [M] 1 = identity rotation
     const float* obb0;    // [M][15] oriented-box rows at pos0
+    const float* ext;     // [M][3] half extents of the oriented box's tight world AABB
 };
-// Writes the AABB table [M][6] and oriented-box table [M][15] at a tick.
+// Writes the AABB table [M][6], the oriented-box table [M][15] and the
+// oriented boxes' broad-phase bounds [M][6] at a tick.
 hipError_t launch_env_motion(const MotionDev& m, int64_t tick, float tick_seconds, float* aabb, float* obb,
-                             hipStream_t st);
+                             float* cull, hipStream_t st);
 
 // Per-pair distances (dist / dist64, [N][pairs]) and/or the per-configuration
 // minimum and collision-free flag (min_out / free_out); null outputs are skipped.

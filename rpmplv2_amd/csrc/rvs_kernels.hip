@@ -73,9 +73,13 @@ __device__ __attribute__((always_inline)) inline void distance_body(const RobotD
         const Vec3<T> P1 = skel_point<T>(fk, rb, qc, i + 1);
         const T r = (T)rb.radius[i];
         T prof = p.maxd;
+        Vec3<T> slo, shi;
+        seg_bounds(P0, P1, slo, shi);
         for (int j = 0; j < M; ++j) {
             T dt;
             NP<T> np;
+            // broad phase (nearest points are stored for every pair, so not with them)
+            if (!WANT_NP && cull_pair(env.cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) continue;
             if (s.gnd[j] && i < rb.ground_included) {
                 dt = p.maxd;
                 if (WANT_NP) {
@@ -176,8 +180,13 @@ __global__ __launch_bounds__(64) void k_distance_refine(RobotDev rb, EnvDev env,
             float lo = p.maxd;          // smallest filtered distance of this lane's obstacles
             int first = INT_MAX;        // first of them with filtered distance <= 0
             float first_v = 0.0f;
+            Vec3<float> slo, shi;
+            seg_bounds(P0, P1, slo, shi);
             for (int j = t; j < M; j += 64) {
                 float dt;
+                // broad phase against this lane's running minimum, an upper
+                // bound of the link's minimum over all lanes
+                if (cull_pair(env.cull + 6 * j, slo, shi, cull_bound(lo, r, p.eq))) continue;
                 if (s.gnd[j] && i < rb.ground_included) {
                     dt = p.maxd;
                 } else if (OBB) {
@@ -370,7 +379,7 @@ __global__ __launch_bounds__(128) void k_self(RobotDev rb, QueryParams qp, const
 // world AABB as the engine computes it at set-up: identity rotation
 // (-h) + pos, h + pos; any other rotation pos -+ rad.  Oriented-box rows get
 // the new centre.  Static boxes keep pos0.
-__global__ void k_env_motion(MotionDev m, int64_t tick, float tick_seconds, float* aabb, float* obb) {
+__global__ void k_env_motion(MotionDev m, int64_t tick, float tick_seconds, float* aabb, float* obb, float* cull) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m.num_obj) return;
     const float t = (float)tick * tick_seconds;
@@ -388,6 +397,10 @@ __global__ void k_env_motion(MotionDev m, int64_t tick, float tick_seconds, floa
     }
     for (int k = 0; k < 3; ++k) obb[15 * j + k] = p[k];
     for (int k = 3; k < 15; ++k) obb[15 * j + k] = m.obb0[15 * j + k];
+    for (int k = 0; k < 3; ++k) {
+        cull[6 * j + k] = p[k] - m.ext[3 * j + k];
+        cull[6 * j + 3 + k] = p[k] + m.ext[3 * j + k];
+    }
 }
 
 inline dim3 grid_for(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
@@ -474,10 +487,10 @@ hipError_t launch_underestimation_seg(const RobotDev& rb, const EnvDev& env, con
 }
 
 hipError_t launch_env_motion(const MotionDev& m, int64_t tick, float tick_seconds, float* aabb, float* obb,
-                             hipStream_t st) {
+                             float* cull, hipStream_t st) {
     if (m.num_obj <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_env_motion, dim3((m.num_obj + 255) / 256), dim3(256), 0, st, m, tick, tick_seconds, aabb,
-                       obb);
+                       obb, cull);
     return hipGetLastError();
 }
 

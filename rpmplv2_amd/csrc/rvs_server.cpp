@@ -130,6 +130,7 @@ struct Motion {
     struct Slot {
         float* aabb = nullptr;
         float* obb = nullptr;
+        float* cull = nullptr;   // broad-phase bounds of oriented boxes
         int64_t tick = -1;
         int refs = 0;
         hipEvent_t ready = nullptr;
@@ -242,6 +243,7 @@ hipError_t issue(rvs_server* s, Batch& b, const Layout& lay) {
         Motion::Slot& sl = b.motion->slot[b.slot];
         env.aabb = sl.aabb;
         env.obb = sl.obb;
+        env.cull = env.use_obb ? sl.cull : sl.aabb;
         e = hipStreamWaitEvent(st, sl.ready, 0);
     }
     Carve cv(s->pool.base() + b.off);
@@ -368,7 +370,7 @@ void tick_motion(rvs_server* s) {
             if (i != m.current && m.slot[i].refs == 0) { k = i; break; }
         if (k < 0) continue;   // every other slot still read by running launches: retry next loop
         Motion::Slot& sl = m.slot[k];
-        hipError_t e = launch_env_motion(m.dev, tick, s->opt.tick_seconds, sl.aabb, sl.obb, s->upd);
+        hipError_t e = launch_env_motion(m.dev, tick, s->opt.tick_seconds, sl.aabb, sl.obb, sl.cull, s->upd);
         if (e == hipSuccess) e = hipEventRecord(sl.ready, s->upd);
         if (e != hipSuccess) {
             s->err = std::string("obstacle update: ") + hipGetErrorString(e);   // caller holds s->mu
@@ -647,10 +649,11 @@ rvs_status rvs_server_set_motion(rvs_server* s, int32_t env, const float* vel) {
         if (s->motion.count(env)) return sfail(s, RVS_ENOTSUP, "rvs_server_set_motion: motion already set");
     }
     const int M = (int)boxes.size();
-    std::vector<float> pos0(3 * M), v(vel, vel + 3 * M), half(3 * M), rad(M), obb0(15 * (size_t)M);
+    std::vector<float> pos0(3 * M), v(vel, vel + 3 * M), half(3 * M), rad(M), obb0(15 * (size_t)M), ext(3 * M);
     std::vector<uint8_t> ident(M);
     for (int j = 0; j < M; ++j) {
         box_obb(boxes[j], &obb0[15 * (size_t)j]);
+        obb_extent(&obb0[15 * (size_t)j], &ext[3 * j]);
         for (int k = 0; k < 3; ++k) {
             pos0[3 * j + k] = boxes[j].pos[k];
             half[3 * j + k] = 0.5f * boxes[j].dim[k];
@@ -678,6 +681,7 @@ rvs_status rvs_server_set_motion(rvs_server* s, int32_t env, const float* vel) {
     put(rad.data(), rad.size() * 4, reinterpret_cast<const void**>(&m->dev.rad));
     put(ident.data(), ident.size(), reinterpret_cast<const void**>(&m->dev.ident));
     put(obb0.data(), obb0.size() * 4, reinterpret_cast<const void**>(&m->dev.obb0));
+    put(ext.data(), ext.size() * 4, reinterpret_cast<const void**>(&m->dev.ext));
     for (auto& sl : m->slot) {
         void* a = nullptr;
         void* o = nullptr;
@@ -685,6 +689,10 @@ rvs_status rvs_server_set_motion(rvs_server* s, int32_t env, const float* vel) {
         if (e == hipSuccess) m->allocs.push_back(a);
         if (e == hipSuccess) e = hipMalloc(&o, std::max<size_t>((size_t)M * 60, 4));
         if (e == hipSuccess) m->allocs.push_back(o);
+        void* cb = nullptr;
+        if (e == hipSuccess) e = hipMalloc(&cb, std::max<size_t>((size_t)M * 24, 4));
+        if (e == hipSuccess) m->allocs.push_back(cb);
+        sl.cull = static_cast<float*>(cb);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
         sl.aabb = static_cast<float*>(a);
         sl.obb = static_cast<float*>(o);

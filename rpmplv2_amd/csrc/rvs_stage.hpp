@@ -87,6 +87,55 @@ __device__ __attribute__((always_inline)) inline Vec3<T> from_box(const float* x
                    (T)x[2] + ((T)x[9] * l.x + (T)x[10] * l.y + (T)x[11] * l.z)};
 }
 
+// Broad phase.  The distance the case tree returns for a (capsule, box) pair
+// is at least the raw segment-box distance minus r, except for its snaps to
+// 0 / -r, which need a raw distance below EQUALITY_THRESHOLD (or contact).
+// The gap between the segment's bounding box [slo, shi] and a box containing
+// the obstacle (cb = lo[3], hi[3]) is a lower bound of the raw distance, so a
+// pair whose gap exceeds max(running link minimum + r, EQUALITY_THRESHOLD)
+// plus a margin that covers float rounding can neither lower the link's
+// minimum nor end the loop: skipping it leaves every result unchanged.
+// Compares squared values (no square root); a NaN end point never culls.
+constexpr float kCullMargin = 1e-5f;
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline bool cull_pair(const float* __restrict__ cb, const Vec3<T>& slo,
+                                                                 const Vec3<T>& shi, T bound) {
+    T g2 = T(0);
+    {
+        T g = (T)cb[0] - shi.x;
+        const T g1 = slo.x - (T)cb[3];
+        g = g1 > g ? g1 : g;
+        g2 = g > T(0) ? g * g : T(0);
+    }
+    {
+        T g = (T)cb[1] - shi.y;
+        const T g1 = slo.y - (T)cb[4];
+        g = g1 > g ? g1 : g;
+        g2 = g > T(0) ? g2 + g * g : g2;
+    }
+    {
+        T g = (T)cb[2] - shi.z;
+        const T g1 = slo.z - (T)cb[5];
+        g = g1 > g ? g1 : g;
+        g2 = g > T(0) ? g2 + g * g : g2;
+    }
+    return g2 > bound * bound;
+}
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline T cull_bound(T prof, T r, T eq) {
+    const T a = prof + r;
+    return (a > eq ? a : eq) + (T)kCullMargin;
+}
+
+template <typename T>
+__device__ __attribute__((always_inline)) inline void seg_bounds(const Vec3<T>& A, const Vec3<T>& B, Vec3<T>& lo,
+                                                                 Vec3<T>& hi) {
+    lo = Vec3<T>{A.x < B.x ? A.x : B.x, A.y < B.y ? A.y : B.y, A.z < B.z ? A.z : B.z};
+    hi = Vec3<T>{A.x < B.x ? B.x : A.x, A.y < B.y ? B.y : A.y, A.z < B.z ? B.z : A.z};
+}
+
 template <typename T> __device__ inline T rnew_of(const RobotDev& rb, int i);
 template <> __device__ inline float rnew_of<float>(const RobotDev& rb, int i) { return rb.rnew32[i]; }
 template <> __device__ inline double rnew_of<double>(const RobotDev& rb, int i) { return rb.rnew64[i]; }

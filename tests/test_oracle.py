@@ -501,3 +501,30 @@ def test_self_min_matches_pairs(oracle):
     m = oracle.self_min(w.robot, w.q)
     pd = oracle.self_distance(w.robot, w.q, w.robot.self_pairs)
     assert np.array_equal(m, np.minimum(np.float32(1000.0), np.nanmin(pd, axis=1)))
+
+
+# ---------------------------------------------------------------- broad phase
+@pytest.mark.parametrize("name", ["workload1", "workload2", "workload5_obb", "workload5_aabb", "chain_ground"])
+def test_broad_phase_changes_nothing(oracle, name):
+    """Skipping pairs whose bounding-box gap exceeds max(link minimum + r,
+    EQUALITY_THRESHOLD) + margin leaves distances, profiles and verdicts
+    bit-identical (both arithmetics), and skips a real share of the pairs."""
+    oriented = False
+    if name == "workload1":
+        w = W.workload1(8192)
+    elif name == "workload2":
+        w = W.workload2(16384)
+    elif name == "chain_ground":
+        rb = R.spatial(10, radius=0.025, ground_included=2)
+        sc = W.ground_box().concat(W._boxes(np.random.default_rng(9), 5, 0.3, 0.3, 1.0, (0, 0, 0.1)))
+        q = np.random.default_rng(10).uniform(-np.pi, np.pi, (8192, 10)).astype(np.float32)
+        w = W.ConfigBatch("spatial10", rb, sc, q)
+    else:
+        w = W.workload5(4096)
+        oriented = name == "workload5_obb"
+    sc = oracle.scene_of(w.scene, oriented=oriented)
+    for prec in (32, 64):
+        d, prof, ver, _ = oracle.distance(w.robot, sc, w.q, prec)
+        dc, profc, verc, (ev, cu) = oracle.distance_cull(w.robot, sc, w.q, prec)
+        assert np.array_equal(d, dc) and np.array_equal(prof, profc) and np.array_equal(ver, verc)
+        assert cu > 0.2 * (ev + cu)
