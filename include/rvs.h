@@ -57,6 +57,7 @@ typedef enum rvs_precision {
 /* Call flags. */
 #define RVS_F_DEVICE 0x1u          /* array arguments are device pointers */
 #define RVS_F_TIMING 0x2u          /* time the call's kernel stages with HIP events (rvs_last_timing) */
+#define RVS_F_EAGER 0x4u           /* launch kernel by kernel even where a captured HIP graph would be replayed */
 #define RVS_F_PREC_SHIFT 4         /* bits 4..5: precision override + 1 (0 = context default) */
 #define RVS_F_PREC(p) ((uint32_t)((p) + 1) << RVS_F_PREC_SHIFT)
 

@@ -214,6 +214,8 @@ void rvs_destroy(rvs_context* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& r : c->robots) for (void* p : r.allocs) (void)hipFree(p);
     for (auto& e : c->envs) for (void* p : e.allocs) (void)hipFree(p);
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    if (c->count_host) (void)hipHostFree(c->count_host);
     c->arena.release();
     for (int k = 0; k < 3; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
@@ -425,6 +427,70 @@ hipError_t d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
 
 }  // namespace
 
+namespace {
+
+// The FP32 + refinement pipeline of a device-array rvs_distance call (zero
+// the counters, bulk pass, refinement, read back the refinement count into
+// pinned memory) is captured once per (robot, environment, arrays, size,
+// arena, launch shape) into a HIP graph and replayed; the cache keeps the 8
+// most recently used.  Kernel arguments hold pointers only, so in-place
+// updates of the inputs, of the obstacle tables and of the robot are seen
+// by every replay.  The graph holds no memset nodes (counters are cleared by
+// a kernel).
+constexpr size_t kMaxGraphs = 8;
+
+hipError_t fp32_graph_run(rvs_context* c, int32_t robot, int32_t env, const RobotDev& rb, const EnvDev& ev,
+                          const QueryParams& qp, const float* q, int64_t n, const DistOut& out, int32_t* ridx,
+                          uint32_t* rcount, const FastTuning& ft, uint32_t* refined) {
+    hipError_t e = hipSuccess;
+    if (!c->count_host) {
+        e = hipHostMalloc(reinterpret_cast<void**>(&c->count_host), 256, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+    }
+    rvs_context::Graph* hit = nullptr;
+    for (auto& g : c->graphs)
+        if (g.robot == robot && g.env == env && g.q == q && g.d == out.d && g.profile == out.profile &&
+            g.verdict == out.verdict && g.arena == c->arena.p && g.n == n && g.t.variant == ft.variant &&
+            g.t.block == ft.block && g.t.blocks_per_cu == ft.blocks_per_cu) {
+            hit = &g;
+            break;
+        }
+    if (!hit) {
+        if (c->graphs.size() == kMaxGraphs) {
+            auto old = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                        [](const rvs_context::Graph& a, const rvs_context::Graph& b) {
+                                            return a.used < b.used;
+                                        });
+            (void)hipGraphExecDestroy(old->exec);
+            c->graphs.erase(old);
+        }
+        hipGraph_t g = nullptr;
+        e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipError_t le = launch_zero_u32(rcount, 64, c->stream);
+        if (le == hipSuccess) le = launch_distance_fast(rb, ev, qp, q, n, out, ridx, rcount, ft, c->num_cus, c->stream);
+        if (le == hipSuccess) le = launch_distance_refine(rb, ev, qp, q, n, out, ridx, rcount, c->stream);
+        if (le == hipSuccess)
+            le = hipMemcpyAsync(c->count_host, rcount, 4, hipMemcpyDeviceToHost, c->stream);
+        e = hipStreamEndCapture(c->stream, &g);
+        if (le != hipSuccess) e = le;
+        hipGraphExec_t exec = nullptr;
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return e;
+        c->graphs.push_back(rvs_context::Graph{robot, env, q, out.d, out.profile, out.verdict, c->arena.p, n, ft,
+                                               exec, 0});
+        hit = &c->graphs.back();
+    }
+    hit->used = ++c->graph_clock;
+    e = hipGraphLaunch(hit->exec, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) *refined = *c->count_host;
+    return e;
+}
+
+}  // namespace
+
 rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n, float* d,
                         float* profile, float* nearest, uint8_t* verdict, uint32_t flags) {
     rvs_status s = check_query(c, robot, env, q, n, true);
@@ -474,7 +540,12 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     uint32_t refined = 0;
     const bool timing = (flags & RVS_F_TIMING) != 0;
     if (e == hipSuccess) {
-        if (prec == RVS_PREC_FP32_REFINE) {
+        const bool graph = prec == RVS_PREC_FP32_REFINE && (flags & RVS_F_DEVICE) && !timing &&
+                           !(flags & RVS_F_EAGER);
+        if (graph) {
+            const FastTuning ft = c->tuning_for(robot, env, L, M);
+            e = fp32_graph_run(c, robot, env, rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, ft, &refined);
+        } else if (prec == RVS_PREC_FP32_REFINE) {
             e = launch_zero_u32(rcount, 64, c->stream);
             const FastTuning ft = c->tuning_for(robot, env, L, M);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);

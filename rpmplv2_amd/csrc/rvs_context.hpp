@@ -74,6 +74,18 @@ struct rvs_context {
     int32_t edge_waves_per_cu = 16;   // persistent first-collision kernel: resident waves per CU
     int32_t edge_spl = 1;             // ... and slots per lane in one 64-lane item
     std::map<std::pair<int32_t, int32_t>, rvs::FastTuning> tuning;   // (robot, env) -> autotuned launch shape
+    // captured FP32 + refinement pipelines of device-array rvs_distance calls
+    struct Graph {
+        int32_t robot, env;
+        const void *q, *d, *profile, *verdict, *arena;
+        int64_t n;
+        rvs::FastTuning t;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::vector<Graph> graphs;
+    uint64_t graph_clock = 0;
+    uint32_t* count_host = nullptr;   // pinned: refinement count read back by the graphs
     std::mutex tuning_mu;
     rvs::FastTuning tuning_for(int32_t robot, int32_t env, int32_t links, int32_t objects) {
         std::lock_guard<std::mutex> lk(tuning_mu);

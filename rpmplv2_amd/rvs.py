@@ -23,6 +23,7 @@ RVS_OK, RVS_EINVAL, RVS_ENOMEM, RVS_EDEVICE, RVS_ERANGE, RVS_ENOTSUP = range(6)
 PREC_REF32, PREC_FP64, PREC_FP32_REFINE = 0, 1, 2
 F_DEVICE = 0x1
 F_TIMING = 0x2
+F_EAGER = 0x4
 ENV_AABB, ENV_OBB = 0, 1
 BOX_GROUND, BOX_DYNAMIC = 0x1, 0x2
 
@@ -348,8 +349,9 @@ class Engine:
 
     # ---- queries on device tensors (torch CUDA tensors; no staging)
     def distance_device(self, robot: int, env: int, q, d, profile=None, verdict=None, nearest=None,
-                        precision: Optional[int] = None, timing: bool = False):
-        fl = F_DEVICE | (F_PREC(precision) if precision is not None else 0) | (F_TIMING if timing else 0)
+                        precision: Optional[int] = None, timing: bool = False, eager: bool = False):
+        fl = (F_DEVICE | (F_PREC(precision) if precision is not None else 0) | (F_TIMING if timing else 0)
+              | (F_EAGER if eager else 0))
         self._check(lib().rvs_distance(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d), _ptr(profile),
                                        _ptr(nearest), _ptr(verdict), fl), "rvs_distance")
 
