@@ -268,10 +268,12 @@ enum {
 
 typedef struct rvs_server_options {
     int64_t memory_budget;      /* device bytes for request state (default 8 GiB) */
-    int32_t num_streams;        /* concurrent launches (default 4) */
+    int32_t num_streams;        /* streams the launches are spread over (default 2) */
     int32_t max_batch_configs;  /* configurations per fused launch (default 1 << 20) */
     float tick_seconds;         /* obstacle update period (default 0.01 s) */
     int32_t precision;          /* of verdict requests (default RVS_PREC_FP32_REFINE) */
+    int32_t launches_per_stream;/* launches in flight per stream (default 1: requests arriving
+                                   meanwhile wait and join the next, larger launch) */
 } rvs_server_options;
 
 typedef struct rvs_request {

@@ -154,18 +154,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // minimum is positive up to that pair).  A minimum of floats is exact in any
 // order, and only the first non-positive value can reach the profile, so the
 // reduction order cannot change a bit of the result.
+constexpr int kRefineWaves = 4;   // waves per workgroup of the refinement kernel (they share the staged obstacles)
+
 template <bool OBB>
-__global__ __launch_bounds__(64) void k_distance_refine(RobotDev rb, EnvDev env, QueryParams qp,
-                                                        const float* __restrict__ q, DistOut out,
-                                                        const int32_t* __restrict__ idx,
-                                                        const uint32_t* __restrict__ count) {
+__global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev rb, EnvDev env, QueryParams qp,
+                                                                       const float* __restrict__ q, DistOut out,
+                                                                       const int32_t* __restrict__ idx,
+                                                                       const uint32_t* __restrict__ count) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<float> s = stage_env<float, OBB>(env, smem);
     const int M = env.num_obj, L = rb.num_links;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
     const Params<float> p{qp.eq, qp.maxd};
     const uint32_t total = *count;
-    for (uint32_t k = blockIdx.x; k < total; k += gridDim.x) {
+    for (uint32_t k = blockIdx.x * kRefineWaves + wave; k < total; k += gridDim.x * kRefineWaves) {
         const int64_t c = idx[k];
         const float* qc = q + c * rb.num_dofs;
         FKStream fk;
@@ -443,12 +446,13 @@ hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const Q
     if (n <= 0) return hipSuccess;
     // enough waves to cover a typical flagged set in one round; the kernel
     // loops when more configurations are flagged
-    const unsigned grid = (unsigned)std::min<int64_t>(n, 8192);
+    const unsigned grid = (unsigned)std::min<int64_t>((n + kRefineWaves - 1) / kRefineWaves, 2048);
     const size_t sm = staged_bytes<float>(env);
+    const dim3 b(64 * kRefineWaves);
     if (env.use_obb)
-        hipLaunchKernelGGL(k_distance_refine<true>, dim3(grid), dim3(64), sm, st, rb, env, qp, q, out, idx, count);
+        hipLaunchKernelGGL(k_distance_refine<true>, dim3(grid), b, sm, st, rb, env, qp, q, out, idx, count);
     else
-        hipLaunchKernelGGL(k_distance_refine<false>, dim3(grid), dim3(64), sm, st, rb, env, qp, q, out, idx, count);
+        hipLaunchKernelGGL(k_distance_refine<false>, dim3(grid), b, sm, st, rb, env, qp, q, out, idx, count);
     return hipGetLastError();
 }
 

@@ -482,7 +482,7 @@ bool form_batch(rvs_server* s, Batch& b, Layout& lay) {
 
 void serve(rvs_server* s) {
     (void)hipSetDevice(s->ctx->opt.device);
-    const size_t max_inflight = (size_t)s->opt.num_streams * 2;
+    const size_t max_inflight = (size_t)s->opt.num_streams * s->opt.launches_per_stream;
     for (;;) {
         bool progress = retire(s);
         std::unique_lock<std::mutex> lk(s->mu);
@@ -565,10 +565,11 @@ extern "C" {
 void rvs_server_default_options(rvs_server_options* o) {
     if (!o) return;
     o->memory_budget = (int64_t)8 << 30;
-    o->num_streams = 4;
+    o->num_streams = 2;
     o->max_batch_configs = 1 << 20;
     o->tick_seconds = 0.01f;
     o->precision = RVS_PREC_FP32_REFINE;
+    o->launches_per_stream = 1;
 }
 
 rvs_status rvs_server_create(rvs_context* c, const rvs_server_options* opt, rvs_server** out) {
@@ -578,6 +579,7 @@ rvs_status rvs_server_create(rvs_context* c, const rvs_server_options* opt, rvs_
     if (opt) o = *opt;
     else rvs_server_default_options(&o);
     if (o.memory_budget < (1 << 20) || o.num_streams < 1 || o.num_streams > 16 || o.max_batch_configs < 1 ||
+        o.launches_per_stream < 1 || o.launches_per_stream > 16 ||
         !(o.tick_seconds > 0) || (o.precision != RVS_PREC_FP32_REFINE && o.precision != RVS_PREC_REF32))
         return RVS_EINVAL;
     rvs_server* s = new (std::nothrow) rvs_server();

@@ -116,6 +116,7 @@ int main(int argc, char** argv) {
     so.max_batch_configs = rd1<int32_t>();
     so.tick_seconds = tick_seconds;
     const int32_t P = std::max(1, rd1<int32_t>());
+    so.launches_per_stream = rd1<int32_t>();
     std::fclose(g_in);
 
     rvs_options o;
@@ -246,14 +247,15 @@ int main(int argc, char** argv) {
         "\"latency_ms_large\": {\"p50\": %.4f, \"p99\": %.4f}, \"queue_wait_ms\": {\"p50\": %.4f, \"p99\": %.4f}, "
         "\"requests_per_launch\": {\"mean\": %.2f, \"p99\": %.1f}, \"submit_lag_ms_p99\": %.4f, \"launches\": %lld, "
         "\"max_concurrent_launches\": %d, \"peak_device_bytes\": %lld, \"ticks\": %lld, \"planners\": %d, "
-        "\"streams\": %d, \"memory_budget\": %lld, \"failures\": %d}\n",
+        "\"streams\": %d, \"launches_per_stream\": %d, \"max_batch_configs\": %d, \"memory_budget\": %lld, "
+        "\"failures\": %d}\n",
         (long long)K, (long long)configs, span, K / std::max(1e-9, arrival.empty() ? 1.0 : arrival.back()),
         configs / std::max(1e-9, span), pct(lat, 0.5), pct(lat, 0.9), pct(lat, 0.99), pct(lat, 0.999),
         pct(lat, 1.0), mean, pct(lat_small, 0.5), pct(lat_small, 0.99), pct(lat_large, 0.5), pct(lat_large, 0.99),
         pct(qwait, 0.5), pct(qwait, 0.99), [&] { double s = 0; for (double v : batch) s += v; return s / std::max<int64_t>(1, K); }(),
         pct(batch, 0.99), pct([&] { std::vector<double> v(lag.begin(), lag.end()); for (auto& x : v) x *= 1e3; return v; }(), 0.99),
         (long long)stats.batches, stats.max_inflight_batches, (long long)stats.peak_bytes, (long long)stats.ticks, P,
-        so.num_streams, (long long)so.memory_budget, failures.load());
+        so.num_streams, so.launches_per_stream, so.max_batch_configs, (long long)so.memory_budget, failures.load());
     rvs_server_destroy(srv);
     for (auto& r : robots) rvs_host_free(r.pool);
     rvs_destroy(ctx);

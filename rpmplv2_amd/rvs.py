@@ -62,7 +62,7 @@ class Box(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("memory_budget", C.c_int64), ("num_streams", C.c_int32), ("max_batch_configs", C.c_int32),
-                ("tick_seconds", C.c_float), ("precision", C.c_int32)]
+                ("tick_seconds", C.c_float), ("precision", C.c_int32), ("launches_per_stream", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -463,12 +463,14 @@ class Server:
     """Request server over an Engine's context (rvs_server_*).  Results land
     in numpy arrays owned by this object until wait() returns them."""
 
-    def __init__(self, engine: Engine, memory_budget: int = 8 << 30, num_streams: int = 4,
-                 max_batch_configs: int = 1 << 20, tick_seconds: float = 0.01, precision: int = PREC_FP32_REFINE):
+    def __init__(self, engine: Engine, memory_budget: int = 8 << 30, num_streams: int = 2,
+                 max_batch_configs: int = 1 << 20, tick_seconds: float = 0.01, precision: int = PREC_FP32_REFINE,
+                 launches_per_stream: int = 1):
         L = lib()
         o = ServerOptions()
         L.rvs_server_default_options(C.byref(o))
         o.memory_budget, o.num_streams, o.max_batch_configs = memory_budget, num_streams, max_batch_configs
+        o.launches_per_stream = launches_per_stream
         o.tick_seconds, o.precision = tick_seconds, precision
         self.engine = engine
         self._srv = C.c_void_p()

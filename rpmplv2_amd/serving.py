@@ -25,8 +25,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPLAY = os.path.join(HERE, "_build", "rvs_serve_replay")
 
 
-def write_problem(w: W.ServingWorkload, path: str, memory_budget: int = 8 << 30, num_streams: int = 4,
-                  max_batch_configs: int = 1 << 20, planners: int = 8) -> None:
+def write_problem(w: W.ServingWorkload, path: str, memory_budget: int = 8 << 30, num_streams: int = 2,
+                  max_batch_configs: int = 1 << 20, planners: int = 8, launches_per_stream: int = 1) -> None:
     with open(path, "wb") as f:
         f.write(b"RVSW")
         f.write(np.int32(1).tobytes())
@@ -54,17 +54,18 @@ def write_problem(w: W.ServingWorkload, path: str, memory_budget: int = 8 << 30,
         f.write(np.ascontiguousarray(w.size, np.int64).tobytes())
         f.write(np.ascontiguousarray(w.offset, np.int64).tobytes())
         f.write(np.int64(memory_budget).tobytes())
-        f.write(np.array([num_streams, max_batch_configs, planners], np.int32).tobytes())
+        f.write(np.array([num_streams, max_batch_configs, planners, launches_per_stream], np.int32).tobytes())
 
 
-def run(duration_s: float = 5.0, rate: float = 20000.0, memory_budget: int = 8 << 30, num_streams: int = 4,
-        planners: int = 8, timeout: float = 240.0) -> dict:
+def run(duration_s: float = 5.0, rate: float = 20000.0, memory_budget: int = 8 << 30, num_streams: int = 2,
+        planners: int = 8, timeout: float = 240.0, max_batch_configs: int = 1 << 20,
+        launches_per_stream: int = 1) -> dict:
     if not os.path.exists(REPLAY):
         raise RuntimeError(f"{REPLAY} is not built: run `python -m rpmplv2_amd.build`")
     w = W.workload4(duration_s, rate, pool=1 << 20)
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "serving.bin")
-        write_problem(w, path, memory_budget, num_streams, 1 << 20, planners)
+        write_problem(w, path, memory_budget, num_streams, max_batch_configs, planners, launches_per_stream)
         r = subprocess.run([REPLAY, path], capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"rvs_serve_replay failed ({r.returncode}): {r.stderr.strip()}")
@@ -75,10 +76,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--duration", type=float, default=5.0)
     ap.add_argument("--rate", type=float, default=20000.0)
-    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--streams", type=int, default=2)
     ap.add_argument("--planners", type=int, default=8)
+    ap.add_argument("--max-batch", type=int, default=1 << 20)
+    ap.add_argument("--per-stream", type=int, default=1)
     args = ap.parse_args()
-    print(json.dumps(run(args.duration, args.rate, num_streams=args.streams, planners=args.planners)))
+    print(json.dumps(run(args.duration, args.rate, num_streams=args.streams, planners=args.planners,
+                         max_batch_configs=args.max_batch, launches_per_stream=args.per_stream)))
 
 
 if __name__ == "__main__":
