@@ -65,8 +65,10 @@ def test_obb_fp64(oracle, engine, w5, ids):
     assert np.abs(prof - oprof).max() <= 1e-12
 
 
-def test_obb_fp32_refine(oracle, engine, w5, ids):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_obb_fp32_refine(oracle, engine, w5, ids, variant):
     rid, eo, _ = ids
+    engine.set_tuning(rid, eo, variant, 256, 0)
     d, prof, ver, _ = engine.distance(rid, eo, w5.q, precision=rvs.PREC_FP32_REFINE)
     refined = engine.last_refined()
     sc = oracle.scene_of(w5.scene, oriented=True)
