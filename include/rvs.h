@@ -188,6 +188,16 @@ rvs_status rvs_self_distance(rvs_context* ctx, int32_t robot, const float* q, in
 rvs_status rvs_self_collision(rvs_context* ctx, int32_t robot, const float* q, int64_t n, float* min_dist,
                               uint8_t* free_flag, uint32_t flags);
 
+/* AbstractRobot::computeEnclosingRadii for n configurations (xArm6.cpp:237-275,
+ * Planar2DOF.cpp:132-148, Spatial10DOF.cpp:11-30): R [n][dofs][dofs+1], row i
+ * = starting skeleton point, column j = final point (the transpose of the
+ * reference's column-major Eigen::MatrixXf(dofs, dofs+1) read as [i][j]).
+ * Chains use the spatial rule (row 0 measured in the {x, y} plane), which is
+ * the planar rule for planar chains based at the origin.  Needs one link per
+ * revolute joint. */
+rvs_status rvs_enclosing_radii(rvs_context* ctx, int32_t robot, const float* q, int64_t n, float* R,
+                               uint32_t flags);
+
 /* RealVectorSpace::isValid(q1, q2) (RealVectorSpace.cpp:261-275): num_checks
  * interpolated states from the far end, q1 not checked. */
 rvs_status rvs_edge_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const float* q1, const float* q2,

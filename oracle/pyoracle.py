@@ -96,6 +96,7 @@ def _declare(L):
         "orc_capsule_obb32": (C.c_float, [f32p, f32p, C.c_float, f32p, Pp, f32p, i32p]),
         "orc_capsule_obb64": (C.c_double, [f64p, f64p, C.c_double, f64p, Pp, f64p, i32p]),
         "orc_self_min32": (None, [R, Pp, f32p, C.c_int64, i32p, C.c_int32, f32p, C.c_int32]),
+        "orc_enclosing_radii32": (None, [R, f32p, C.c_int64, f32p, C.c_int32]),
         "orc_distance_cull32": (None, [R, Pp, S, f32p, C.c_int64, f32p, f32p, u8p, _p(C.c_int64), C.c_int32]),
         "orc_distance_cull64": (None, [R, Pp, S, f32p, C.c_int64, f64p, f64p, u8p, _p(C.c_int64), C.c_int32]),
     }
@@ -332,3 +333,13 @@ def distance_cull(rb, scene: Scene, q, precision=32, params=None, nthreads=8):
     fn(C.byref(rs), C.byref(p), C.byref(scene.s), _ptr(q, C.c_float), n, _ptr(d, ct), _ptr(prof, ct),
        _ptr(ver, C.c_uint8), _ptr(counts, C.c_int64), nthreads)
     return d, prof, ver, (int(counts[0]), int(counts[1]))
+
+
+def enclosing_radii(rb, q, nthreads=8):
+    """AbstractRobot::computeEnclosingRadii: (N, dofs, dofs + 1) float32."""
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+    D = rb.num_dofs
+    out = np.zeros((len(q), D, D + 1), np.float32)
+    rs = robot_struct(rb)
+    lib().orc_enclosing_radii32(C.byref(rs), _ptr(q, C.c_float), len(q), _ptr(out, C.c_float), nthreads)
+    return out

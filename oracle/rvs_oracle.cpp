@@ -1122,6 +1122,67 @@ void orc_distance_cull64(const orc_robot* rb, const orc_params* p, const orc_sce
     distance_batch<double>(rb, p, sc, q, n, d, profile, nullptr, verdict, nthreads, counts);
 }
 
+// AbstractRobot::computeEnclosingRadii, float skeleton (ref32):
+//   xArm6 (xArm6.cpp:237-275) with capsules_radius_new (xArm6.cpp:98-105),
+//   chains as Spatial10DOF (Spatial10DOF.cpp:11-30) with capsules_radius.
+// R is [n][dofs][dofs+1], R[i][j] = the reference's R(i, j).
+void orc_enclosing_radii32(const orc_robot* rb, const float* q, int64_t n, float* Rout, int32_t nthreads) {
+    std::vector<Seg> segs = build_segments(rb);
+    const int D = rb->num_dofs;
+    std::vector<float> rn(D);
+    for (int i = 0; i < D; ++i) rn[i] = rb->radius[i];
+    const bool xarm = rb->skeleton_kind == ORC_SKEL_XARM6;
+    if (xarm) {
+        for (int i = 0; i + 1 < D; ++i) rn[i] = std::max(rb->radius[i], rb->radius[i + 1]);
+        rn[D - 1] = rb->radius[D - 1];
+    }
+    auto norm2d = [](const float* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1]); };
+    // (skeleton->col(j) - skeleton->col(i)).norm(): both operands are
+    // dynamic-size MatrixXf columns, so Eigen sums the squares left to right
+    auto norm_dyn = [](const float* v) { return std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); };
+    parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+        std::vector<float> Pk(3 * (rb->num_links + 1));
+        for (int64_t c = b; c < e; ++c) {
+            skeleton<float>(rb, segs, q + c * D, Pk.data());
+            float* R = Rout + c * (int64_t)D * (D + 1);
+            std::fill(R, R + (size_t)D * (D + 1), 0.0f);
+            auto col = [&](int k) { return &Pk[3 * k]; };
+            if (xarm) {
+                for (int i = 0; i < D - 1; ++i)
+                    for (int j = i + 1; j <= D; ++j) {
+                        float* Rij = &R[i * (D + 1) + j];
+                        const float prev = R[i * (D + 1) + j - 1];
+                        if (i == 0) {
+                            if (j >= 2) *Rij = std::max(prev, norm2d(col(j)) + rn[j - 1]);
+                        } else if (i == 3) {
+                            if (j >= 5) {
+                                float AB[3], v[3], proj[3], w[3];
+                                sub3(col(4), col(3), AB);
+                                sub3(col(j), col(3), v);
+                                const float t = dot3(v, AB) / sqn3(AB);
+                                for (int k = 0; k < 3; ++k) proj[k] = col(3)[k] + t * AB[k];
+                                sub3(col(j), proj, w);
+                                *Rij = std::max(prev, norm3(w) + rn[j - 1]);
+                            }
+                        } else {
+                            float v[3];
+                            sub3(col(j), col(i), v);
+                            *Rij = std::max(prev, norm_dyn(v) + rn[j - 1]);
+                        }
+                    }
+            } else {
+                for (int j = 1; j <= D; ++j) R[j] = std::max(R[j - 1], norm2d(col(j)) + rn[j - 1]);
+                for (int i = 1; i < D; ++i)
+                    for (int j = i + 1; j <= D; ++j) {
+                        float v[3];
+                        sub3(col(j), col(i), v);
+                        R[i * (D + 1) + j] = std::max(R[i * (D + 1) + j - 1], norm_dyn(v) + rn[j - 1]);
+                    }
+            }
+        }
+    });
+}
+
 float orc_capsule_obb32(const float A[3], const float B[3], float r, const float obb[15], const orc_params* p,
                         float np[6], int32_t* np_valid) {
     P<float> pr(p); int v; float d = capsule_obb(A, B, r, obb, pr, np, &v); *np_valid = v; return d;

@@ -134,6 +134,10 @@ void orc_distance_cull32(const orc_robot* rb, const orc_params* p, const orc_sce
 void orc_distance_cull64(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q, int64_t n,
                          double* d, double* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads);
 
+/* AbstractRobot::computeEnclosingRadii (xArm6 / spatial chains), float:
+ * R [n][dofs][dofs+1]. */
+void orc_enclosing_radii32(const orc_robot* rb, const float* q, int64_t n, float* R, int32_t nthreads);
+
 /* ---- independent checker: exact segment-AABB distance in double ---- */
 double orc_true_seg_box(const double A[3], const double B[3], const double obs[6]);
 

@@ -285,6 +285,12 @@ rvs_status rvs_robot_load(rvs_context* c, const rvs_robot_desc* d, int32_t* robo
         rn32[i] = (float)((double)rad[i] * std::sqrt(3.0) / 3.0);
         rn64[i] = (double)rad[i] * std::sqrt(3.0) / 3.0;
     }
+    // radii added in computeEnclosingRadii: xArm6 uses capsules_radius_new,
+    // max(r_i, r_i+1) and r_last (xArm6::setCapsulesRadius, xArm6.cpp:98-105);
+    // the chains use capsules_radius (Planar2DOF.cpp:144, Spatial10DOF.cpp:18-24)
+    std::vector<float> renc(rad);
+    if (d->skeleton_kind == RVS_SKEL_XARM6)
+        for (int i = 0; i + 1 < L; ++i) renc[i] = std::max(rad[i], rad[i + 1]);
     std::vector<int32_t> pairs;
     for (int m = 0; m < d->num_self_pairs; ++m) {
         const int a = d->self_pairs[2 * m], b = d->self_pairs[2 * m + 1];
@@ -300,10 +306,12 @@ rvs_status rvs_robot_load(rvs_context* c, const rvs_robot_desc* d, int32_t* robo
     if (e == hipSuccess) e = upload(r.allocs, rn32.data(), rn32.size(), &r.dev.rnew32);
     if (e == hipSuccess) e = upload(r.allocs, rn64.data(), rn64.size(), &r.dev.rnew64);
     if (e == hipSuccess) e = upload(r.allocs, pairs.data(), pairs.size(), &r.dev.pairs);
+    if (e == hipSuccess) e = upload(r.allocs, renc.data(), renc.size(), &r.dev.rad_enc);
     if (e != hipSuccess) {
         for (void* p : r.allocs) (void)hipFree(p);
         return hip_fail(c, e, "rvs_robot_load");
     }
+    r.dev.kind = d->skeleton_kind;
     r.dev.num_segments = d->num_joints;
     r.dev.num_dofs = ndof;
     r.dev.num_links = L;
@@ -851,6 +859,37 @@ rvs_status rvs_self_collision(rvs_context* c, int32_t robot, const float* q, int
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_self_collision");
+    return RVS_OK;
+}
+
+rvs_status rvs_enclosing_radii(rvs_context* c, int32_t robot, const float* q, int64_t n, float* R, uint32_t flags) {
+    rvs_status s = check_query(c, robot, -1, q, n, false);
+    if (s != RVS_OK) return s;
+    if (n > 0 && !R) return fail(c, RVS_EINVAL, "rvs_enclosing_radii: null output");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    if (rb.num_links != rb.num_dofs)
+        return fail(c, RVS_ENOTSUP, "rvs_enclosing_radii: needs one link per revolute joint");
+    if (n == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const int D = rb.num_dofs;
+    const size_t per = (size_t)D * (D + 1);
+    hipError_t e = hipSuccess;
+    const float* qd = q;
+    float* rd = R;
+    if (!(flags & RVS_F_DEVICE)) {
+        e = c->arena.ensure(Carve::need({(size_t)n * D * 4, (size_t)n * per * 4}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_enclosing_radii: staging");
+        Carve cv(c->arena.p);
+        float* qs = cv.take<float>((size_t)n * D);
+        rd = cv.take<float>((size_t)n * per);
+        e = h2d(qs, q, (size_t)n * D * 4, c->stream);
+        qd = qs;
+    }
+    if (e == hipSuccess) e = launch_enclosing_radii(rb.dev, qd, n, rd, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(R, rd, (size_t)n * per * 4, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_enclosing_radii");
     return RVS_OK;
 }
 

@@ -57,13 +57,14 @@ struct RobotDev {
     int32_t num_links;
     int32_t ground_included;
     int32_t num_pairs;
-    int32_t pad;
+    int32_t kind;            // skeleton recipe (RVS_SKEL_CHAIN / RVS_SKEL_XARM6)
     const SegDev* seg;
     const PointDev* pts;     // num_links + 1
     const float* radius;     // num_links
     const float* rnew32;     // collision-tree inflated radius, float
     const double* rnew64;    // same in double
     const int32_t* pairs;    // self-collision pairs (a, b)
+    const float* rad_enc;    // radii added by computeEnclosingRadii (capsules_radius, or xArm6's capsules_radius_new)
 };
 
 // 3x3 rotation held in scalars (row major: m0 m1 m2 / m3 m4 m5 / m6 m7 m8)

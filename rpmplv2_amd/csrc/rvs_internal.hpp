@@ -124,6 +124,10 @@ struct MotionDev {
 hipError_t launch_env_motion(const MotionDev& m, int64_t tick, float tick_seconds, float* aabb, float* obb,
                              float* cull, hipStream_t st);
 
+// AbstractRobot::computeEnclosingRadii for N configurations: R [N][n][n+1]
+// (row i = starting skeleton point, column j = final point).
+hipError_t launch_enclosing_radii(const RobotDev& rb, const float* q, int64_t n, float* R, hipStream_t st);
+
 // Per-pair distances (dist / dist64, [N][pairs]) and/or the per-configuration
 // minimum and collision-free flag (min_out / free_out); null outputs are skipped.
 hipError_t launch_self_distance(const RobotDev& rb, const QueryParams& qp, const float* q, int64_t n, float* dist,

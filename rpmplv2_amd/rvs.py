@@ -97,7 +97,7 @@ RVS_EBUSY = 6
 EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error", "rvs_stream", "rvs_last_refined",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds", "rvs_distance",
            "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_self_distance",
-           "rvs_self_collision", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
+           "rvs_self_collision", "rvs_enclosing_radii", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
            "rvs_autotune", "rvs_set_tuning", "rvs_get_tuning", "rvs_autotune_edges", "rvs_set_edge_tuning", "rvs_server_default_options", "rvs_server_create", "rvs_server_destroy", "rvs_server_set_motion",
            "rvs_server_submit", "rvs_server_wait", "rvs_server_get_stats", "rvs_server_now",
            "rvs_server_last_error", "rvs_host_alloc", "rvs_host_free"]
@@ -137,6 +137,7 @@ def lib():
         L.rvs_distance_underestimation.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, vp, vp, u32]
         L.rvs_self_distance.argtypes = [ctx, i32, vp, i64, vp, u32]
         L.rvs_self_collision.argtypes = [ctx, i32, vp, i64, vp, vp, u32]
+        L.rvs_enclosing_radii.argtypes = [ctx, i32, vp, i64, vp, u32]
         L.rvs_autotune.argtypes = [ctx, i32, i32, vp, i64, u32, C.POINTER(Tuning)]
         L.rvs_set_tuning.argtypes = [ctx, i32, i32, C.POINTER(Tuning)]
         L.rvs_autotune_edges.argtypes = [ctx, i32, vp, vp, vp, vp, i64, i32, C.c_float, u32, C.POINTER(EdgeTuning)]
@@ -342,6 +343,15 @@ class Engine:
         self._check(lib().rvs_self_collision(self._ctx, robot, _ptr(q), len(q), _ptr(m), _ptr(f), 0),
                     "rvs_self_collision")
         return m, f
+
+    def enclosing_radii(self, robot: int, q):
+        """AbstractRobot::computeEnclosingRadii: (N, dofs, dofs + 1)."""
+        rb = self.robots[robot]
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+        D = rb.num_dofs
+        out = np.zeros((len(q), D, D + 1), np.float32)
+        self._check(lib().rvs_enclosing_radii(self._ctx, robot, _ptr(q), len(q), _ptr(out), 0), "rvs_enclosing_radii")
+        return out
 
     def self_collision_device(self, robot: int, q, min_dist=None, free_flag=None):
         self._check(lib().rvs_self_collision(self._ctx, robot, _ptr(q), q.shape[0], _ptr(min_dist), _ptr(free_flag),

@@ -253,3 +253,21 @@ def test_fp32_refine_verdict_exact(oracle, engines_by_variant, wl, variant):
     bad = ver == 0
     assert np.abs(prof[bad] - oprof[bad]).max() <= 1e-5
     assert refined < 0.05 * len(w.q)
+
+
+@pytest.mark.parametrize("which", ["xarm6", "chain22", "planar2"])
+def test_enclosing_radii(oracle, engine, which):
+    if which == "xarm6":
+        rb, q = R.xarm6(), W.workload2(8192).q
+    elif which == "chain22":
+        w = W.workload5(4096)
+        rb, q = w.robot, w.q
+    else:
+        w = W.workload1(8192)
+        rb, q = w.robot, w.q
+    rid = engine.load_robot(rb)
+    got = engine.enclosing_radii(rid, q)
+    ref = oracle.enclosing_radii(rb, q)
+    assert got.shape == ref.shape
+    assert ulps(got, ref).max() <= 2
+    assert (got == ref).mean() > 0.999

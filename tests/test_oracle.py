@@ -528,3 +528,56 @@ def test_broad_phase_changes_nothing(oracle, name):
         dc, profc, verc, (ev, cu) = oracle.distance_cull(w.robot, sc, w.q, prec)
         assert np.array_equal(d, dc) and np.array_equal(prof, profc) and np.array_equal(ver, verc)
         assert cu > 0.2 * (ev + cu)
+
+
+# ---------------------------------------------------------------- enclosing radii
+def test_enclosing_radii_xarm6_closed_form(oracle):
+    """xArm6 at q = 0 from the URDF's skeleton table (xArm6.cpp:237-275)."""
+    rb = R.xarm6(radius=0.05)
+    Rm = oracle.enclosing_radii(rb, np.zeros((1, 6), np.float32))[0].astype(np.float64)
+    P = np.array([[0, 0, 0], [0, 0, 0.267], [0.0535, 0, 0.5515], [0.131, 0, 0.459], [0.131, 0, 0.209],
+                  [0.207, 0, 0.209], [0.207, 0, 0.112]])
+    r = 0.05
+    n = lambda v: float(np.linalg.norm(v))
+    expect = np.zeros((6, 7))
+    for j in range(2, 7):
+        expect[0, j] = max(expect[0, j - 1], n(P[j, :2]) + r)
+    for i in (1, 2, 4):
+        for j in range(i + 1, 7):
+            expect[i, j] = max(expect[i, j - 1], n(P[j] - P[i]) + r)
+    for j in (5, 6):
+        AB = P[4] - P[3]
+        proj = P[3] + np.dot(P[j] - P[3], AB) / np.dot(AB, AB) * AB
+        expect[3, j] = max(expect[3, j - 1], n(P[j] - proj) + r)
+    assert np.abs(Rm - expect).max() < 2e-5   # the URDF's 1.5708 leaves ~1e-6 m off-axis residue
+
+
+def test_enclosing_radii_chain_invariants(oracle):
+    rng = np.random.default_rng(17)
+    lengths = rng.uniform(0.1, 0.3, 10)
+    rb = R.chain(lengths, radius=0.03)
+    q = rng.uniform(-np.pi, np.pi, (256, 10)).astype(np.float32)
+    Rm = oracle.enclosing_radii(rb, q)
+    P = oracle.skeleton(rb, q, 32)
+    # consecutive points: the link length plus the radius, for every q
+    for i in range(1, 10):
+        assert np.allclose(Rm[:, i, i + 1], np.linalg.norm(P[:, i + 1] - P[:, i], axis=1) + 0.03, atol=1e-6)
+    # rows never decrease along j; entries left of the diagonal are zero
+    assert (np.diff(Rm[:, :, 1:], axis=2) >= 0).all()
+    assert all((Rm[:, i, : i + 1] == 0).all() for i in range(10))
+    # row 0 is measured in the {x, y} plane
+    assert np.allclose(Rm[:, 0, 10], np.maximum.accumulate(np.linalg.norm(P[:, 1:, :2], axis=2) + 0.03, axis=1)[:, -1],
+                       atol=1e-6)
+
+
+def test_enclosing_radii_planar_rule(oracle):
+    """For a planar chain based at the origin the spatial rule (row 0 in the
+    {x, y} plane) equals Planar2DOF's rule (|P_j - P_0|)."""
+    rb = R.planar_2dof(radius=0.025)
+    q = np.random.default_rng(18).uniform(-np.pi, np.pi, (512, 2)).astype(np.float32)
+    Rm = oracle.enclosing_radii(rb, q)
+    P = oracle.skeleton(rb, q, 32)
+    d = np.linalg.norm(P - P[:, :1], axis=2)
+    assert np.allclose(Rm[:, 0, 1], d[:, 1] + 0.025, atol=1e-7)
+    assert np.allclose(Rm[:, 0, 2], np.maximum(d[:, 1], d[:, 2]) + 0.025, atol=1e-7)
+    assert np.allclose(Rm[:, 1, 2], d[:, 1] * 0 + np.linalg.norm(P[:, 2] - P[:, 1], axis=1) + 0.025, atol=1e-7)
