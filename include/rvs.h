@@ -198,6 +198,46 @@ rvs_status rvs_self_collision(rvs_context* ctx, int32_t robot, const float* q, i
 rvs_status rvs_enclosing_radii(rvs_context* ctx, int32_t robot, const float* q, int64_t n, float* R,
                                uint32_t flags);
 
+/* ---------------------------------------------------------------- spines */
+/* Spine outcomes, numbered as base::State::Status (include/state_spaces/State.h:22). */
+enum { RVS_SPINE_ADVANCED = 1, RVS_SPINE_TRAPPED = 2, RVS_SPINE_REACHED = 3 };
+
+/* RBTConnect::extendSpine (src/planners/rbt/RBTConnect.cpp:85-150) for n
+ * spines.  Spine k starts at q[k] ([n][dofs]), whose distance d[k] and
+ * per-link profile[k] ([n][links]) come from rvs_distance or
+ * rvs_distance_underestimation, and heads for q_e[k].  num_iter =
+ * NUM_ITER_SPINE (>= 1), use_expanded_bubble = USE_EXPANDED_BUBBLE (profile
+ * may be NULL without it).  Outputs q_new [n][dofs] and status[k]
+ * (RVS_SPINE_*; d[k] <= 0 gives TRAPPED with q_new = q).  Reference float
+ * arithmetic, bit for bit.  The reference also runs
+ * robot->checkSelfCollision(q_temp, q_new) in every iteration (:129-131);
+ * these are the spines of a robot without self-collision checking (the
+ * chains; xArm6 with self_collision_checking false).  With num_iter = 1 that
+ * check is the iteration's last action, so a caller that checks
+ * self-collision applies it to (q, q_new) afterwards.  Needs one link per
+ * revolute joint. */
+rvs_status rvs_extend_spines(rvs_context* ctx, int32_t robot, const float* q, const float* d, const float* profile,
+                             const float* q_e, int64_t n, int32_t num_iter, int32_t use_expanded_bubble,
+                             float* q_new, int32_t* status, uint32_t flags);
+
+/* RGBTConnect::extendGenSpine (src/planners/rbt/RGBTConnect.cpp:71-90) from
+ * one root state q_root ([dofs]) towards n targets q_e ([n][dofs]), the way a
+ * generalized bur is built (PatternTree::generateGBur, PatternTree.cpp:20-60;
+ * DRGBT::computeReachedState, DRGBT.cpp:475-495): the root's distance
+ * (computeDistance, with nearest points), then up to num_layers spines per
+ * target (rvs_extend_spines rules), each followed by the distance of its end
+ * state: computeDistanceUnderestimation against the root's nearest points
+ * (use_real_dist = 0) or computeDistance (use_real_dist = 1).  A target stops
+ * when that distance is below d_crit (D_CRIT) or its spine did not advance.
+ * Outputs q_new [n][dofs], status [n] and d [n]: the distance the last layer
+ * compared with d_crit (0 where the underestimation returns early; the
+ * root's distance when the root is in contact).  Environment distances use
+ * the reference float arithmetic whatever the context precision. */
+rvs_status rvs_extend_gen_spines(rvs_context* ctx, int32_t robot, int32_t env, const float* q_root, const float* q_e,
+                                 int64_t n, int32_t num_layers, int32_t num_iter, int32_t use_expanded_bubble,
+                                 int32_t use_real_dist, float d_crit, float* q_new, int32_t* status, float* d,
+                                 uint32_t flags);
+
 /* RealVectorSpace::isValid(q1, q2) (RealVectorSpace.cpp:261-275): num_checks
  * interpolated states from the far end, q1 not checked. */
 rvs_status rvs_edge_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const float* q1, const float* q2,

@@ -97,6 +97,8 @@ def _declare(L):
         "orc_capsule_obb64": (C.c_double, [f64p, f64p, C.c_double, f64p, Pp, f64p, i32p]),
         "orc_self_min32": (None, [R, Pp, f32p, C.c_int64, i32p, C.c_int32, f32p, C.c_int32]),
         "orc_enclosing_radii32": (None, [R, f32p, C.c_int64, f32p, C.c_int32]),
+        "orc_extend_spines32": (None, [R, f32p, f32p, f32p, f32p, C.c_int64, C.c_int32, C.c_int32, f32p, i32p,
+                                       C.c_int32]),
         "orc_distance_cull32": (None, [R, Pp, S, f32p, C.c_int64, f32p, f32p, u8p, _p(C.c_int64), C.c_int32]),
         "orc_distance_cull64": (None, [R, Pp, S, f32p, C.c_int64, f64p, f64p, u8p, _p(C.c_int64), C.c_int32]),
     }
@@ -343,3 +345,65 @@ def enclosing_radii(rb, q, nthreads=8):
     rs = robot_struct(rb)
     lib().orc_enclosing_radii32(C.byref(rs), _ptr(q, C.c_float), len(q), _ptr(out, C.c_float), nthreads)
     return out
+
+
+def extend_spines(rb, q, d, profile, q_e, num_iter=5, expanded=True, nthreads=8):
+    """RBTConnect::extendSpine (RBTConnect.cpp:85-150) for a batch of spines,
+    without self-collision checking: (q_new (N, dofs) float32, status (N,))."""
+    D, L = rb.num_dofs, rb.num_links
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, D)
+    n = len(q)
+    d = np.ascontiguousarray(d, np.float32).reshape(n)
+    prof = np.ascontiguousarray(profile if profile is not None else np.zeros((n, L)), np.float32).reshape(n, L)
+    qe = np.ascontiguousarray(q_e, np.float32).reshape(n, D)
+    qn = np.zeros((n, D), np.float32)
+    st = np.zeros(n, np.int32)
+    rs = robot_struct(rb)
+    lib().orc_extend_spines32(C.byref(rs), _ptr(q, C.c_float), _ptr(d, C.c_float), _ptr(prof, C.c_float),
+                              _ptr(qe, C.c_float), n, int(num_iter), 1 if expanded else 0, _ptr(qn, C.c_float),
+                              _ptr(st, C.c_int32), nthreads)
+    return qn, st
+
+
+def extend_gen_spines(rb, scene: Scene, q_root, q_e, num_layers=5, num_iter=5, expanded=True,
+                      use_real_dist=False, d_crit=0.03, params=None, nthreads=8):
+    """RGBTConnect::extendGenSpine (RGBTConnect.cpp:71-90) from one root to
+    each target: the root's computeDistance (with nearest points), then per
+    layer extendSpine and the end state's computeDistanceUnderestimation
+    against the root's nearest points (or computeDistance); a target stops
+    when the distance is below d_crit or its spine did not advance.
+    Returns (q_new (N, dofs), status (N,), d (N,)): d is the distance the last
+    layer compared with d_crit, or the root's distance for a root in contact."""
+    D = rb.num_dofs
+    qe = np.ascontiguousarray(q_e, np.float32).reshape(-1, D)
+    n = len(qe)
+    qr = np.ascontiguousarray(q_root, np.float32).reshape(1, D)
+    d0, p0, _, np0 = distance(rb, scene, qr, 32, params, want_np=True, nthreads=nthreads)
+    q_new = np.repeat(qr, n, axis=0)
+    status = np.zeros(n, np.int32)
+    d_out = np.zeros(n, np.float32)
+    if not d0[0] > 0:                                   # :75-76
+        status[:] = 2
+        d_out[:] = d0[0]
+        return q_new, status, d_out
+    d_cur = np.repeat(d0, n)
+    p_cur = np.repeat(p0, n, axis=0)
+    active = np.ones(n, bool)
+    for _ in range(num_layers):                         # :81-88
+        idx = np.nonzero(active)[0]
+        if len(idx) == 0:
+            break
+        qn, st = extend_spines(rb, q_new[idx], d_cur[idx], p_cur[idx], qe[idx], num_iter, expanded, nthreads)
+        q_new[idx] = qn
+        status[idx] = st
+        if use_real_dist:
+            dl, pl, _, _ = distance(rb, scene, qn, 32, params, nthreads=nthreads)
+        else:
+            dl, pl, _ = underestimation(rb, scene, qn, np0[0], params, nthreads)
+        d_cur[idx] = dl
+        p_cur[idx] = pl
+        d_out[idx] = dl
+        stop = (dl < np.float32(d_crit)) | (st != 1)
+        active[idx[stop]] = False
+    return q_new, status, d_out
+

@@ -877,27 +877,87 @@ T capsules_distance(const orc_robot* rb, const P<T>& pr, const T* Pk, int a, int
 }
 
 // float interpolation of RealVectorSpace::interpolateEdge (:75-88) with the
-// norm of RealVectorSpace::getNorm (:49-52).  The VectorXf norm follows
-// Eigen's SSE reduction order for dynamic sizes [3P]: packets of 4 summed
-// pairwise, reduced as (p0 + p2) + (p1 + p3), then the tail left to right.
+// norm of RealVectorSpace::getNorm (:49-52).  VectorXf sums (norm, dot)
+// follow Eigen's SSE reduction order for dynamic sizes [3P]: below 4 terms
+// left to right; otherwise packets of 4 accumulated in two lanes over blocks
+// of 8, folded, plus a last packet of 4, reduced as (p0 + p2) + (p1 + p3),
+// then the tail left to right.
+float vecxf_sum(const float* c, int n) {
+    if (n < 4) { float r = c[0]; for (int i = 1; i < n; ++i) r = r + c[i]; return r; }
+    int aligned = (n / 4) * 4, aligned2 = (n / 8) * 8;
+    float p0[4], p1[4];
+    for (int k = 0; k < 4; ++k) p0[k] = c[k];
+    if (aligned > 4) {
+        for (int k = 0; k < 4; ++k) p1[k] = c[4 + k];
+        for (int idx = 8; idx < aligned2; idx += 8)
+            for (int k = 0; k < 4; ++k) { p0[k] = p0[k] + c[idx + k]; p1[k] = p1[k] + c[idx + 4 + k]; }
+        for (int k = 0; k < 4; ++k) p0[k] = p0[k] + p1[k];
+        if (aligned > aligned2)
+            for (int k = 0; k < 4; ++k) p0[k] = p0[k] + c[aligned2 + k];
+    }
+    float r = (p0[0] + p0[2]) + (p0[1] + p0[3]);
+    for (int i = aligned; i < n; ++i) r = r + c[i];
+    return r;
+}
+
 float vecxf_norm(const float* v, int n) {
     float sq[ORC_MAX_SEG];
     for (int i = 0; i < n; ++i) sq[i] = v[i] * v[i];
-    if (n < 4) { float r = sq[0]; for (int i = 1; i < n; ++i) r = r + sq[i]; return std::sqrt(r); }
-    int aligned = (n / 4) * 4, aligned2 = (n / 8) * 8;
-    float p0[4], p1[4];
-    for (int k = 0; k < 4; ++k) p0[k] = sq[k];
-    if (aligned > 4) {
-        for (int k = 0; k < 4; ++k) p1[k] = sq[4 + k];
-        for (int idx = 8; idx < aligned2; idx += 8)
-            for (int k = 0; k < 4; ++k) { p0[k] = p0[k] + sq[idx + k]; p1[k] = p1[k] + sq[idx + 4 + k]; }
-        for (int k = 0; k < 4; ++k) p0[k] = p0[k] + p1[k];
-        if (aligned > aligned2)
-            for (int k = 0; k < 4; ++k) p0[k] = p0[k] + sq[aligned2 + k];
+    return std::sqrt(vecxf_sum(sq, n));
+}
+
+// AbstractRobot::computeEnclosingRadii for one configuration, R [dofs][dofs+1]:
+// xArm6.cpp:237-275 (rows 0..n-2; row 0 from j = 2 in the {x, y} plane; row 3
+// from j = 5 against the line P3 P4; capsules_radius_new), Spatial10DOF.cpp:
+// 11-30 and Planar2DOF.cpp:132-148 for chains.  Pk is scratch for the skeleton.
+void radii32(const orc_robot* rb, const std::vector<Seg>& segs, const float* qc, float* R, float* Pk) {
+    const int D = rb->num_dofs;
+    float rn[ORC_MAX_SEG];
+    for (int i = 0; i < D; ++i) rn[i] = rb->radius[i];
+    const bool xarm = rb->skeleton_kind == ORC_SKEL_XARM6;
+    if (xarm) {
+        for (int i = 0; i + 1 < D; ++i) rn[i] = std::max(rb->radius[i], rb->radius[i + 1]);
+        rn[D - 1] = rb->radius[D - 1];
     }
-    float r = (p0[0] + p0[2]) + (p0[1] + p0[3]);
-    for (int i = aligned; i < n; ++i) r = r + sq[i];
-    return std::sqrt(r);
+    auto norm2d = [](const float* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1]); };
+    // (skeleton->col(j) - skeleton->col(i)).norm(): both operands are
+    // dynamic-size MatrixXf columns, so Eigen sums the squares left to right
+    auto norm_dyn = [](const float* v) { return std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); };
+    skeleton<float>(rb, segs, qc, Pk);
+    std::fill(R, R + (size_t)D * (D + 1), 0.0f);
+    auto col = [&](int k) { return &Pk[3 * k]; };
+    if (xarm) {
+        for (int i = 0; i < D - 1; ++i)
+            for (int j = i + 1; j <= D; ++j) {
+                float* Rij = &R[i * (D + 1) + j];
+                const float prev = R[i * (D + 1) + j - 1];
+                if (i == 0) {
+                    if (j >= 2) *Rij = std::max(prev, norm2d(col(j)) + rn[j - 1]);
+                } else if (i == 3) {
+                    if (j >= 5) {
+                        float AB[3], v[3], proj[3], w[3];
+                        sub3(col(4), col(3), AB);
+                        sub3(col(j), col(3), v);
+                        const float t = dot3(v, AB) / sqn3(AB);
+                        for (int k = 0; k < 3; ++k) proj[k] = col(3)[k] + t * AB[k];
+                        sub3(col(j), proj, w);
+                        *Rij = std::max(prev, norm3(w) + rn[j - 1]);
+                    }
+                } else {
+                    float v[3];
+                    sub3(col(j), col(i), v);
+                    *Rij = std::max(prev, norm_dyn(v) + rn[j - 1]);
+                }
+            }
+    } else {
+        for (int j = 1; j <= D; ++j) R[j] = std::max(R[j - 1], norm2d(col(j)) + rn[j - 1]);
+        for (int i = 1; i < D; ++i)
+            for (int j = i + 1; j <= D; ++j) {
+                float v[3];
+                sub3(col(j), col(i), v);
+                R[i * (D + 1) + j] = std::max(R[i * (D + 1) + j - 1], norm_dyn(v) + rn[j - 1]);
+            }
+    }
 }
 
 }  // namespace
@@ -1129,56 +1189,76 @@ void orc_distance_cull64(const orc_robot* rb, const orc_params* p, const orc_sce
 void orc_enclosing_radii32(const orc_robot* rb, const float* q, int64_t n, float* Rout, int32_t nthreads) {
     std::vector<Seg> segs = build_segments(rb);
     const int D = rb->num_dofs;
-    std::vector<float> rn(D);
-    for (int i = 0; i < D; ++i) rn[i] = rb->radius[i];
-    const bool xarm = rb->skeleton_kind == ORC_SKEL_XARM6;
-    if (xarm) {
-        for (int i = 0; i + 1 < D; ++i) rn[i] = std::max(rb->radius[i], rb->radius[i + 1]);
-        rn[D - 1] = rb->radius[D - 1];
-    }
-    auto norm2d = [](const float* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1]); };
-    // (skeleton->col(j) - skeleton->col(i)).norm(): both operands are
-    // dynamic-size MatrixXf columns, so Eigen sums the squares left to right
-    auto norm_dyn = [](const float* v) { return std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); };
     parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
         std::vector<float> Pk(3 * (rb->num_links + 1));
+        for (int64_t c = b; c < e; ++c) radii32(rb, segs, q + c * D, Rout + c * (int64_t)D * (D + 1), Pk.data());
+    });
+}
+
+// RBTConnect::extendSpine, src/planners/rbt/RBTConnect.cpp:85-150, with
+// robot->checkSelfCollision(q_temp, q_new) (:129-131) false (a robot without
+// self-collision checking).  R->col(i+1).dot(delta_q) is a VectorXf dot
+// (vecxf_sum order); (skeleton->col(k) - skeleton_new->col(k)).norm() a
+// dynamic-size 3-vector norm summed left to right.
+void orc_extend_spines32(const orc_robot* rb, const float* q, const float* d, const float* profile,
+                         const float* q_e, int64_t n, int32_t num_iter, int32_t expanded, float* q_new,
+                         int32_t* status, int32_t nthreads) {
+    std::vector<Seg> segs = build_segments(rb);
+    const int D = rb->num_dofs, L = rb->num_links;
+    auto norm_dyn = [](const float* v) { return std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); };
+    parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+        std::vector<float> P0(3 * (L + 1)), Pn(3 * (L + 1)), Pk(3 * (L + 1)), R((size_t)D * (D + 1));
+        std::vector<float> qt(D), dq(D), prod(D), rho_profile(L);
         for (int64_t c = b; c < e; ++c) {
-            skeleton<float>(rb, segs, q + c * D, Pk.data());
-            float* R = Rout + c * (int64_t)D * (D + 1);
-            std::fill(R, R + (size_t)D * (D + 1), 0.0f);
-            auto col = [&](int k) { return &Pk[3 * k]; };
-            if (xarm) {
-                for (int i = 0; i < D - 1; ++i)
-                    for (int j = i + 1; j <= D; ++j) {
-                        float* Rij = &R[i * (D + 1) + j];
-                        const float prev = R[i * (D + 1) + j - 1];
-                        if (i == 0) {
-                            if (j >= 2) *Rij = std::max(prev, norm2d(col(j)) + rn[j - 1]);
-                        } else if (i == 3) {
-                            if (j >= 5) {
-                                float AB[3], v[3], proj[3], w[3];
-                                sub3(col(4), col(3), AB);
-                                sub3(col(j), col(3), v);
-                                const float t = dot3(v, AB) / sqn3(AB);
-                                for (int k = 0; k < 3; ++k) proj[k] = col(3)[k] + t * AB[k];
-                                sub3(col(j), proj, w);
-                                *Rij = std::max(prev, norm3(w) + rn[j - 1]);
-                            }
-                        } else {
-                            float v[3];
-                            sub3(col(j), col(i), v);
-                            *Rij = std::max(prev, norm_dyn(v) + rn[j - 1]);
-                        }
-                    }
-            } else {
-                for (int j = 1; j <= D; ++j) R[j] = std::max(R[j - 1], norm2d(col(j)) + rn[j - 1]);
-                for (int i = 1; i < D; ++i)
-                    for (int j = i + 1; j <= D; ++j) {
-                        float v[3];
-                        sub3(col(j), col(i), v);
-                        R[i * (D + 1) + j] = std::max(R[i * (D + 1) + j - 1], norm_dyn(v) + rn[j - 1]);
-                    }
+            const float* qc = q + c * D;
+            const float* qe = q_e + c * D;
+            float* qn = q_new + c * D;
+            if (d[c] <= 0) {  // :90-91
+                std::copy(qc, qc + D, qn);
+                status[c] = 2;
+                continue;
             }
+            skeleton<float>(rb, segs, qc, P0.data());
+            std::fill(rho_profile.begin(), rho_profile.end(), 0.0f);
+            float rho = 0;
+            int32_t st = 1;
+            std::copy(qc, qc + D, qt.begin());
+            for (int counter = 1;; ++counter) {
+                radii32(rb, segs, qt.data(), R.data(), Pk.data());
+                for (int k = 0; k < D; ++k) dq[k] = std::fabs(qe[k] - qt[k]);
+                auto covered = [&](int j) {
+                    for (int k = 0; k < D; ++k) prod[k] = R[(size_t)k * (D + 1) + j] * dq[k];
+                    return vecxf_sum(prod.data(), D);
+                };
+                float step;
+                if (expanded) {
+                    step = std::numeric_limits<float>::infinity();
+                    for (int i = 0; i < D; ++i)
+                        step = std::min(step, (profile[c * L + i] - rho_profile[i]) / covered(i + 1));
+                } else {
+                    step = (d[c] - rho) / covered(D);
+                }
+                if (step > 1) {
+                    std::copy(qe, qe + D, qn);
+                    st = 3;
+                } else {
+                    for (int k = 0; k < D; ++k) qn[k] = qt[k] + step * (qe[k] - qt[k]);
+                }
+                if (counter == num_iter || st != 1) break;
+                skeleton<float>(rb, segs, qn, Pn.data());
+                std::fill(rho_profile.begin(), rho_profile.end(), 0.0f);
+                float rho_prev = 0;
+                for (int k = 1; k <= L; ++k) {
+                    float v[3];
+                    sub3(&P0[3 * k], &Pn[3 * k], v);
+                    const float rho_k = norm_dyn(v);
+                    rho_profile[k - 1] = std::max(rho_prev, rho_k);
+                    rho_prev = rho_k;
+                    rho = std::max(rho, rho_k);
+                }
+                std::copy(qn, qn + D, qt.begin());
+            }
+            status[c] = st;
         }
     });
 }

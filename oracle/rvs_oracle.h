@@ -138,6 +138,13 @@ void orc_distance_cull64(const orc_robot* rb, const orc_params* p, const orc_sce
  * R [n][dofs][dofs+1]. */
 void orc_enclosing_radii32(const orc_robot* rb, const float* q, int64_t n, float* R, int32_t nthreads);
 
+/* RBTConnect::extendSpine for n spines (robot without self-collision
+ * checking), float: q_new [n][dofs], status 1 advanced / 2 trapped /
+ * 3 reached. */
+void orc_extend_spines32(const orc_robot* rb, const float* q, const float* d, const float* profile,
+                         const float* q_e, int64_t n, int32_t num_iter, int32_t expanded, float* q_new,
+                         int32_t* status, int32_t nthreads);
+
 /* ---- independent checker: exact segment-AABB distance in double ---- */
 double orc_true_seg_box(const double A[3], const double B[3], const double obs[6]);
 

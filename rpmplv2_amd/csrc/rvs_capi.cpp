@@ -893,6 +893,156 @@ rvs_status rvs_enclosing_radii(rvs_context* c, int32_t robot, const float* q, in
     return RVS_OK;
 }
 
+rvs_status rvs_extend_spines(rvs_context* c, int32_t robot, const float* q, const float* d, const float* profile,
+                             const float* q_e, int64_t n, int32_t num_iter, int32_t use_expanded_bubble,
+                             float* q_new, int32_t* status, uint32_t flags) {
+    rvs_status s = check_query(c, robot, -1, q, n, false);
+    if (s != RVS_OK) return s;
+    if (n > 0 && (!d || !q_e || !q_new || !status || (use_expanded_bubble && !profile)))
+        return fail(c, RVS_EINVAL, "rvs_extend_spines: null argument");
+    if (num_iter < 1) return fail(c, RVS_EINVAL, "rvs_extend_spines: num_iter must be at least 1");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    if (rb.num_links != rb.num_dofs)
+        return fail(c, RVS_ENOTSUP, "rvs_extend_spines: needs one link per revolute joint");
+    if (n == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const int D = rb.num_dofs, L = rb.num_links;
+    const bool np = use_expanded_bubble != 0;
+    hipError_t e = hipSuccess;
+    SpineArgs a{};
+    a.q = q;
+    a.q_stride = D;
+    a.d = d;
+    a.d_stride = 1;
+    a.profile = np ? profile : d;
+    a.profile_stride = np ? L : 0;
+    a.q_e = q_e;
+    a.q_new = q_new;
+    a.status = status;
+    a.num_iter = num_iter;
+    a.expanded = np ? 1 : 0;
+    if (!(flags & RVS_F_DEVICE)) {
+        const size_t qb = (size_t)n * D * 4;
+        e = c->arena.ensure(Carve::need({qb, (size_t)n * 4, np ? (size_t)n * L * 4 : 0, qb, qb, (size_t)n * 4}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_extend_spines: staging");
+        Carve cv(c->arena.p);
+        float* qs = cv.take<float>((size_t)n * D);
+        float* ds = cv.take<float>(n);
+        float* ps = np ? cv.take<float>((size_t)n * L) : nullptr;
+        float* es = cv.take<float>((size_t)n * D);
+        a.q_new = cv.take<float>((size_t)n * D);
+        a.status = cv.take<int32_t>(n);
+        e = h2d(qs, q, qb, c->stream);
+        if (e == hipSuccess) e = h2d(ds, d, (size_t)n * 4, c->stream);
+        if (e == hipSuccess && np) e = h2d(ps, profile, (size_t)n * L * 4, c->stream);
+        if (e == hipSuccess) e = h2d(es, q_e, qb, c->stream);
+        a.q = qs;
+        a.d = ds;
+        a.profile = np ? ps : ds;
+        a.q_e = es;
+    }
+    if (e == hipSuccess) e = launch_spines(rb.dev, a, n, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
+        e = d2h(q_new, a.q_new, (size_t)n * D * 4, c->stream);
+        if (e == hipSuccess) e = d2h(status, a.status, (size_t)n * 4, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_extend_spines");
+    return RVS_OK;
+}
+
+rvs_status rvs_extend_gen_spines(rvs_context* c, int32_t robot, int32_t env, const float* q_root, const float* q_e,
+                                 int64_t n, int32_t num_layers, int32_t num_iter, int32_t use_expanded_bubble,
+                                 int32_t use_real_dist, float d_crit, float* q_new, int32_t* status, float* d,
+                                 uint32_t flags) {
+    rvs_status s = check_query(c, robot, env, q_e, n, true);
+    if (s != RVS_OK) return s;
+    if (n > 0 && (!q_root || !q_new || !status || !d)) return fail(c, RVS_EINVAL, "rvs_extend_gen_spines: null argument");
+    if (num_layers < 1 || num_iter < 1)
+        return fail(c, RVS_EINVAL, "rvs_extend_gen_spines: num_layers and num_iter must be at least 1");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    if (rb.num_links != rb.num_dofs)
+        return fail(c, RVS_ENOTSUP, "rvs_extend_gen_spines: needs one link per revolute joint");
+    s = env_fits(c, ev.dev, false, "rvs_extend_gen_spines");
+    if (s != RVS_OK) return s;
+    if (n == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const QueryParams qp = params_of(c);
+    const int D = rb.num_dofs, L = rb.num_links, M = ev.dev.num_obj;
+    const bool host = !(flags & RVS_F_DEVICE);
+    const size_t qb = (size_t)n * D * 4, set = (size_t)M * L * 6;
+    hipError_t e = c->arena.ensure(Carve::need({4, (size_t)L * 4, set * 4, (size_t)n * 4, (size_t)n * L * 4, (size_t)n,
+                                                (size_t)n, host ? (size_t)D * 4 : 0, host ? qb : 0, host ? qb : 0,
+                                                host ? (size_t)n * 4 : 0, host ? (size_t)n * 4 : 0}));
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_extend_gen_spines: staging");
+    Carve cv(c->arena.p);
+    float* d_root = cv.take<float>(1);
+    float* p_root = cv.take<float>(L);
+    float* np_root = cv.take<float>(set);
+    float* d_layer = cv.take<float>(n);
+    float* p_layer = cv.take<float>((size_t)n * L);
+    uint8_t* stored = cv.take<uint8_t>(n);
+    uint8_t* active = cv.take<uint8_t>(n);
+    const float* qr = q_root;
+    const float* qe = q_e;
+    float* qn = q_new;
+    int32_t* st = status;
+    float* dd = d;
+    if (host) {
+        float* qrs = cv.take<float>(D);
+        float* qes = cv.take<float>((size_t)n * D);
+        qn = cv.take<float>((size_t)n * D);
+        st = cv.take<int32_t>(n);
+        dd = cv.take<float>(n);
+        e = h2d(qrs, q_root, (size_t)D * 4, c->stream);
+        if (e == hipSuccess) e = h2d(qes, q_e, qb, c->stream);
+        qr = qrs;
+        qe = qes;
+    }
+    // the root's distance, profile and nearest points (computeDistance)
+    DistOut ro{};
+    ro.d = d_root;
+    ro.profile = p_root;
+    ro.nearest = np_root;
+    if (e == hipSuccess) e = launch_distance_ref32(rb.dev, ev.dev, qp, qr, 1, ro, c->stream);
+    if (e == hipSuccess) e = launch_fill_u8(active, n, 1, c->stream);
+    DistOut lo{};
+    lo.d = d_layer;
+    lo.profile = p_layer;
+    for (int layer = 0; layer < num_layers && e == hipSuccess; ++layer) {
+        SpineArgs a{};
+        a.q = layer == 0 ? qr : qn;
+        a.q_stride = layer == 0 ? 0 : D;
+        a.d = layer == 0 ? d_root : d_layer;
+        a.d_stride = layer == 0 ? 0 : 1;
+        a.profile = layer == 0 ? p_root : p_layer;
+        a.profile_stride = layer == 0 ? 0 : L;
+        a.q_e = qe;
+        a.active = active;
+        a.q_new = qn;
+        a.status = st;
+        a.num_iter = num_iter;
+        a.expanded = use_expanded_bubble ? 1 : 0;
+        e = launch_spines(rb.dev, a, n, c->stream);
+        if (e != hipSuccess) break;
+        e = use_real_dist ? launch_distance_ref32(rb.dev, ev.dev, qp, qn, n, lo, c->stream)
+                          : launch_underestimation(rb.dev, ev.dev, qp, qn, n, np_root, 0, d_layer, p_layer, stored,
+                                                   c->stream);
+        if (e == hipSuccess) e = launch_gen_layer(n, layer, d_root, d_layer, st, active, dd, d_crit, c->stream);
+    }
+    if (e == hipSuccess && host) {
+        e = d2h(q_new, qn, qb, c->stream);
+        if (e == hipSuccess) e = d2h(status, st, (size_t)n * 4, c->stream);
+        if (e == hipSuccess) e = d2h(d, dd, (size_t)n * 4, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_extend_gen_spines");
+    return RVS_OK;
+}
+
 rvs_status rvs_edge_is_valid(rvs_context* c, int32_t robot, int32_t env, const float* q1, const float* q2,
                              int64_t n, uint8_t* valid, uint32_t flags) {
     rvs_status s = check_query(c, robot, env, q1, n, true);

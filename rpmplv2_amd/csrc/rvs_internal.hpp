@@ -128,6 +128,33 @@ hipError_t launch_env_motion(const MotionDev& m, int64_t tick, float tick_second
 // (row i = starting skeleton point, column j = final point).
 hipError_t launch_enclosing_radii(const RobotDev& rb, const float* q, int64_t n, float* R, hipStream_t st);
 
+// One batch of spines (device pointers).  Spine c starts at q + c*q_stride
+// with distance d[c*d_stride] and link profile profile + c*profile_stride
+// (stride 0 = one start state shared by all spines) and heads for
+// q_e [N][n].  q_new [N][n] may be q itself when q_stride = n.  Spines with
+// active[c] = 0 are left untouched (active may be null).
+struct SpineArgs {
+    const float* q;
+    int64_t q_stride;
+    const float* d;
+    int64_t d_stride;
+    const float* profile;
+    int64_t profile_stride;
+    const float* q_e;
+    const uint8_t* active;
+    float* q_new;
+    int32_t* status;   // 1 advanced, 2 trapped, 3 reached
+    int32_t num_iter;
+    int32_t expanded;
+};
+hipError_t launch_spines(const RobotDev& rb, const SpineArgs& a, int64_t n, hipStream_t st);
+// Between layers of generalized spines: d_out = d_layer and deactivation of
+// spines with d_layer < d_crit or status != advanced (layer 0: a root in
+// contact stops every spine with the root's distance).
+hipError_t launch_gen_layer(int64_t n, int layer, const float* d_root, const float* d_layer, const int32_t* status,
+                            uint8_t* active, float* d_out, float d_crit, hipStream_t st);
+hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
+
 // Per-pair distances (dist / dist64, [N][pairs]) and/or the per-configuration
 // minimum and collision-free flag (min_out / free_out); null outputs are skipped.
 hipError_t launch_self_distance(const RobotDev& rb, const QueryParams& qp, const float* q, int64_t n, float* dist,

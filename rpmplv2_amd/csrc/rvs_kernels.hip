@@ -28,6 +28,7 @@
 #include "rvs_fk.hpp"
 #include "rvs_geometry.hpp"
 #include "rvs_internal.hpp"
+#include "rvs_radii.hpp"
 #include "rvs_stage.hpp"
 
 
@@ -377,26 +378,16 @@ __global__ __launch_bounds__(128) void k_self(RobotDev rb, QueryParams qp, const
 }
 
 // ---------------------------------------------------------------- enclosing radii
-// AbstractRobot::computeEnclosingRadii in the reference's float arithmetic:
-// R(i, j) = max(R(i, j-1), |P_j - P_i| + r_enc[j-1]) for j > i, zero
-// elsewhere (Planar2DOF.cpp:132-148).  Chains measure row 0 in the {x, y}
-// plane, |P_j.xy| (Spatial10DOF.cpp:11-30; the same as the planar rule for
-// planar chains based at the origin).  xArm6 (xArm6.cpp:237-275): rows
-// 0 .. n-2 only; row 0 from j = 2 with |P_j.xy|; row 3 from j = 5 with the
-// distance of P_j to its projection on the line P3 P4; capsules_radius_new.
-// std::max keeps the running value when the new one is NaN.  The skeleton is
-// a MatrixXf, so (col(j) - col(i)).norm() is a dynamic-size reduction,
-// summed left to right ((x^2 + y^2) + z^2); the projection in row 3 mixes in
-// fixed-size Vector3f operands and sums as x + (y + z).
-__device__ inline float norm_dyn3(const Vec3<float>& v) { return sqrtf((v.x * v.x + v.y * v.y) + v.z * v.z); }
-
+// R [n][dofs][dofs+1] column by column (radii_column): the skeleton and the
+// running column live in LDS, one slice per lane.
 __global__ __launch_bounds__(128) void k_radii(RobotDev rb, const float* __restrict__ q, int64_t n, float* R) {
     extern __shared__ __align__(16) unsigned char smem[];
-    float* sk = reinterpret_cast<float*>(smem);
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
     const int D = rb.num_dofs;
     const int B = blockDim.x, t = threadIdx.x;
+    float* sk = reinterpret_cast<float*>(smem);
+    float* ac = sk + (size_t)3 * (D + 1) * B;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
     const float* qc = q + c * D;
     FKStream fk;
     fk_init(fk);
@@ -407,41 +398,15 @@ __global__ __launch_bounds__(128) void k_radii(RobotDev rb, const float* __restr
         sk[(3 * k + 2) * B + t] = P.z;
     }
     auto pt = [&](int k) { return Vec3<float>{sk[(3 * k) * B + t], sk[(3 * k + 1) * B + t], sk[(3 * k + 2) * B + t]}; };
-    auto fmax_std = [](float a, float b) { return (a < b) ? b : a; };
+    auto acc = [&](int i) -> float& { return ac[i * B + t]; };
     float* out = R + c * (int64_t)D * (D + 1);
-    const bool xarm = rb.kind == 1;
     for (int i = 0; i < D; ++i) {
-        float acc = 0.0f;
+        acc(i) = 0.0f;
         out[i * (D + 1)] = 0.0f;
-        for (int j = 1; j <= D; ++j) {
-            if (j <= i || (xarm && i == D - 1)) {
-                if (j > i) out[i * (D + 1) + j] = 0.0f;
-                else out[i * (D + 1) + j] = 0.0f;
-                continue;
-            }
-            const Vec3<float> Pj = pt(j);
-            float v;
-            bool set = true;
-            if (i == 0) {
-                if (xarm && j < 2) set = false;
-                v = sqrtf(Pj.x * Pj.x + Pj.y * Pj.y);
-            } else if (xarm && i == 3) {
-                if (j < 5) {
-                    set = false;
-                    v = 0.0f;
-                } else {
-                    const Vec3<float> P3 = pt(3), P4 = pt(4);
-                    const Vec3<float> AB = P4 - P3;
-                    const float tt = dot(Pj - P3, AB) / sqnorm(AB);
-                    const Vec3<float> proj{P3.x + tt * AB.x, P3.y + tt * AB.y, P3.z + tt * AB.z};
-                    v = sqrtf(sqnorm(Pj - proj));
-                }
-            } else {
-                v = norm_dyn3(Pj - pt(i));
-            }
-            if (set) acc = fmax_std(acc, v + rb.rad_enc[j - 1]);
-            out[i * (D + 1) + j] = acc;
-        }
+    }
+    for (int j = 1; j <= D; ++j) {
+        radii_column(rb, pt, j, acc);
+        for (int i = 0; i < D; ++i) out[i * (D + 1) + j] = acc(i);
     }
 }
 
@@ -561,8 +526,12 @@ hipError_t launch_underestimation_seg(const RobotDev& rb, const EnvDev& env, con
 
 hipError_t launch_enclosing_radii(const RobotDev& rb, const float* q, int64_t n, float* R, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    const int B = 128;
-    const size_t sm = (size_t)3 * (rb.num_dofs + 1) * B * sizeof(float);
+    // per-lane slices of the skeleton and the running column; the block
+    // shrinks for long chains so that they stay within 64 KB of LDS
+    const size_t per = (size_t)(3 * (rb.num_dofs + 1) + rb.num_dofs) * sizeof(float);
+    int B = 128;
+    while (B > 32 && per * B > 64 * 1024) B /= 2;
+    const size_t sm = per * B;
     hipLaunchKernelGGL(k_radii, grid_for(n, B), dim3(B), sm, st, rb, q, n, R);
     return hipGetLastError();
 }

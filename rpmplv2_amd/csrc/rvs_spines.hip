@@ -1,0 +1,178 @@
+// Spine generation kernels (gfx950): the bubble extension of RBT planners.
+//
+// k_spines: RBTConnect::extendSpine (src/planners/rbt/RBTConnect.cpp:85-150)
+// for a batch of spines, one lane per spine.  Each iteration takes the
+// enclosing radii of the current state column by column (radii_column), the
+// covered-distance bound of every skeleton point, R.col(j).dot(|q_e - q|), in
+// Eigen's dynamic-size order, and the largest step whose bound stays inside
+// the link's remaining clearance (expanded bubble) or inside the remaining
+// distance (plain bubble).  The moved distance of every skeleton point then
+// shrinks the clearances for the next iteration.  Skeleton, radii column,
+// state and clearances of a lane live in its LDS slice.
+//
+// k_gen_layer: the bookkeeping of RGBTConnect::extendGenSpine
+// (src/planners/rbt/RGBTConnect.cpp:71-90) between layers.
+//
+// No FMA contraction anywhere in this translation unit, headers included:
+// the float arithmetic must keep the reference's rounding.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "rvs_fk.hpp"
+#include "rvs_geometry.hpp"
+#include "rvs_internal.hpp"
+#include "rvs_radii.hpp"
+
+namespace rvs {
+
+namespace {
+
+constexpr int32_t kAdvanced = 1, kTrapped = 2, kReached = 3;
+
+// Floats of one lane's LDS slice: skeleton of the start state and of the
+// current state (3 (D+1) each), radii column, current state, clearance
+// losses (D each).
+__host__ __device__ inline int spine_slice(int D) { return 6 * (D + 1) + 3 * D; }
+
+__global__ __launch_bounds__(64) void k_spines(RobotDev rb, SpineArgs a, int64_t n) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int D = rb.num_dofs;
+    const int B = blockDim.x, t = threadIdx.x;
+    float* base = reinterpret_cast<float*>(smem);
+    float* sk0 = base;                                  // [3 (D+1)][B]
+    float* sk = sk0 + (size_t)3 * (D + 1) * B;          // [3 (D+1)][B]
+    float* ac = sk + (size_t)3 * (D + 1) * B;           // [D][B]
+    float* qt = ac + (size_t)D * B;                     // [D][B]
+    float* rp = qt + (size_t)D * B;                     // [D][B]
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    if (a.active && !a.active[c]) return;
+    const float* qc = a.q + c * a.q_stride;
+    const float* pc = a.profile + c * a.profile_stride;
+    const float* qe = a.q_e + c * D;
+    const float dq = a.d[c * a.d_stride];
+    float* qn = a.q_new + c * D;
+    auto Q = [&](int k) -> float& { return qt[k * B + t]; };
+    auto RP = [&](int k) -> float& { return rp[k * B + t]; };
+    auto acc = [&](int i) -> float& { return ac[i * B + t]; };
+    auto P0 = [&](int k) { return Vec3<float>{sk0[(3 * k) * B + t], sk0[(3 * k + 1) * B + t], sk0[(3 * k + 2) * B + t]}; };
+    auto P = [&](int k) { return Vec3<float>{sk[(3 * k) * B + t], sk[(3 * k + 1) * B + t], sk[(3 * k + 2) * B + t]}; };
+    auto set_P = [&](float* s, int k, const Vec3<float>& v) {
+        s[(3 * k) * B + t] = v.x;
+        s[(3 * k + 1) * B + t] = v.y;
+        s[(3 * k + 2) * B + t] = v.z;
+    };
+    // q_new may alias q (in-place layers): read the whole state first
+    for (int k = 0; k < D; ++k) Q(k) = qc[k];
+    int32_t status = kAdvanced;
+    if (dq <= 0.0f) {
+        status = kTrapped;
+    } else {
+        {
+            FKStream fk;
+            fk_init(fk);
+            for (int k = 0; k <= D; ++k) {
+                const Vec3<float> v = skel_point<float>(fk, rb, qt + t, k, B);
+                set_P(sk0, k, v);
+                set_P(sk, k, v);
+            }
+        }
+        for (int k = 0; k < D; ++k) RP(k) = 0.0f;
+        float rho = 0.0f;
+        for (int it = 1;; ++it) {
+            for (int i = 0; i < D; ++i) acc(i) = 0.0f;
+            // covered distance of skeleton point j: R.col(j).dot(|q_e - q_temp|)
+            auto covered = [&]() {
+                return eigen_dyn_sum([&](int i) { return acc(i) * fabsf(qe[i] - Q(i)); }, D);
+            };
+            float step = INFINITY;
+            for (int j = 1; j <= D; ++j) {
+                radii_column(rb, P, j, acc);
+                if (a.expanded) step = min_std(step, (pc[j - 1] - RP(j - 1)) / covered());
+            }
+            if (!a.expanded) step = (dq - rho) / covered();
+            if (step > 1.0f) {
+                for (int k = 0; k < D; ++k) Q(k) = qe[k];
+                status = kReached;
+            } else {
+                for (int k = 0; k < D; ++k) Q(k) = Q(k) + step * (qe[k] - Q(k));
+            }
+            if (it >= a.num_iter || status != kAdvanced) break;
+            // moved distance of every skeleton point since the start state
+            FKStream fk;
+            fk_init(fk);
+            set_P(sk, 0, skel_point<float>(fk, rb, qt + t, 0, B));
+            float prev = 0.0f;
+            for (int k = 1; k <= D; ++k) {
+                const Vec3<float> v = skel_point<float>(fk, rb, qt + t, k, B);
+                set_P(sk, k, v);
+                const float rk = norm_dyn3(P0(k) - v);
+                RP(k - 1) = max_std(prev, rk);
+                prev = rk;
+                rho = max_std(rho, rk);
+            }
+        }
+    }
+    for (int k = 0; k < D; ++k) qn[k] = Q(k);
+    a.status[c] = status;
+}
+
+// After the distance of layer 'layer': the reference leaves the loop when
+// d_c < D_CRIT or the spine did not advance; at layer 0 a start state in
+// contact (d <= 0) stops with its own distance.
+__global__ void k_gen_layer(int64_t n, int layer, const float* d_root, const float* d_layer, const int32_t* status,
+                            uint8_t* active, float* d_out, float d_crit) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || !active[c]) return;
+    if (layer == 0 && d_root[0] <= 0.0f) {
+        d_out[c] = d_root[0];
+        active[c] = 0;
+        return;
+    }
+    const float d = d_layer[c];
+    d_out[c] = d;
+    if (d < d_crit || status[c] != kAdvanced) active[c] = 0;
+}
+
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) p[c] = v;
+}
+
+}  // namespace
+
+size_t spine_lds_per_lane(const RobotDev& rb) { return (size_t)spine_slice(rb.num_dofs) * sizeof(float); }
+
+int spine_block(const RobotDev& rb) {
+    const size_t per = spine_lds_per_lane(rb);
+    int B = 64;
+    while (B > 16 && per * B > 64 * 1024) B /= 2;
+    return B;
+}
+
+hipError_t launch_spines(const RobotDev& rb, const SpineArgs& a, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int B = spine_block(rb);
+    const size_t sm = spine_lds_per_lane(rb) * B;
+    hipLaunchKernelGGL(k_spines, dim3((unsigned)((n + B - 1) / B)), dim3(B), sm, st, rb, a, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_layer(int64_t n, int layer, const float* d_root, const float* d_layer, const int32_t* status,
+                            uint8_t* active, float* d_out, float d_crit, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gen_layer, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, layer, d_root, d_layer,
+                       status, active, d_out, d_crit);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace rvs

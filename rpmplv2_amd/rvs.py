@@ -97,7 +97,8 @@ RVS_EBUSY = 6
 EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error", "rvs_stream", "rvs_last_refined",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds", "rvs_distance",
            "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_self_distance",
-           "rvs_self_collision", "rvs_enclosing_radii", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
+           "rvs_self_collision", "rvs_enclosing_radii", "rvs_extend_spines", "rvs_extend_gen_spines",
+           "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
            "rvs_autotune", "rvs_set_tuning", "rvs_get_tuning", "rvs_autotune_edges", "rvs_set_edge_tuning", "rvs_server_default_options", "rvs_server_create", "rvs_server_destroy", "rvs_server_set_motion",
            "rvs_server_submit", "rvs_server_wait", "rvs_server_get_stats", "rvs_server_now",
            "rvs_server_last_error", "rvs_host_alloc", "rvs_host_free"]
@@ -138,6 +139,9 @@ def lib():
         L.rvs_self_distance.argtypes = [ctx, i32, vp, i64, vp, u32]
         L.rvs_self_collision.argtypes = [ctx, i32, vp, i64, vp, vp, u32]
         L.rvs_enclosing_radii.argtypes = [ctx, i32, vp, i64, vp, u32]
+        L.rvs_extend_spines.argtypes = [ctx, i32, vp, vp, vp, vp, i64, i32, i32, vp, vp, u32]
+        L.rvs_extend_gen_spines.argtypes = [ctx, i32, i32, vp, vp, i64, i32, i32, i32, i32, C.c_float, vp, vp, vp,
+                                            u32]
         L.rvs_autotune.argtypes = [ctx, i32, i32, vp, i64, u32, C.POINTER(Tuning)]
         L.rvs_set_tuning.argtypes = [ctx, i32, i32, C.POINTER(Tuning)]
         L.rvs_autotune_edges.argtypes = [ctx, i32, vp, vp, vp, vp, i64, i32, C.c_float, u32, C.POINTER(EdgeTuning)]
@@ -352,6 +356,50 @@ class Engine:
         out = np.zeros((len(q), D, D + 1), np.float32)
         self._check(lib().rvs_enclosing_radii(self._ctx, robot, _ptr(q), len(q), _ptr(out), 0), "rvs_enclosing_radii")
         return out
+
+    def extend_spines(self, robot: int, q, d, profile, q_e, num_iter: int = 5, expanded: bool = True):
+        """RBTConnect::extendSpine for a batch of spines (robot without
+        self-collision checking): (q_new (N, dofs), status (N,) int32:
+        1 advanced, 2 trapped, 3 reached)."""
+        rb = self.robots[robot]
+        D, L = rb.num_dofs, rb.num_links
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, D)
+        n = len(q)
+        d = np.ascontiguousarray(d, np.float32).reshape(n)
+        prof = None if profile is None else np.ascontiguousarray(profile, np.float32).reshape(n, L)
+        qe = np.ascontiguousarray(q_e, np.float32).reshape(n, D)
+        qn = np.zeros((n, D), np.float32)
+        st = np.zeros(n, np.int32)
+        self._check(lib().rvs_extend_spines(self._ctx, robot, _ptr(q), _ptr(d), _ptr(prof), _ptr(qe), n, int(num_iter),
+                                            1 if expanded else 0, _ptr(qn), _ptr(st), 0), "rvs_extend_spines")
+        return qn, st
+
+    def extend_gen_spines(self, robot: int, env: int, q_root, q_e, num_layers: int = 5, num_iter: int = 5,
+                          expanded: bool = True, use_real_dist: bool = False, d_crit: float = 0.03):
+        """RGBTConnect::extendGenSpine from one root towards each target:
+        (q_new (N, dofs), status (N,), d (N,))."""
+        rb = self.robots[robot]
+        D = rb.num_dofs
+        qr = np.ascontiguousarray(q_root, np.float32).reshape(D)
+        qe = np.ascontiguousarray(q_e, np.float32).reshape(-1, D)
+        n = len(qe)
+        qn = np.zeros((n, D), np.float32)
+        st = np.zeros(n, np.int32)
+        d = np.zeros(n, np.float32)
+        self._check(lib().rvs_extend_gen_spines(self._ctx, robot, env, _ptr(qr), _ptr(qe), n, int(num_layers),
+                                                int(num_iter), 1 if expanded else 0, 1 if use_real_dist else 0,
+                                                float(d_crit), _ptr(qn), _ptr(st), _ptr(d), 0),
+                    "rvs_extend_gen_spines")
+        return qn, st, d
+
+    def extend_gen_spines_device(self, robot: int, env: int, q_root, q_e, q_new, status, d, num_layers: int = 5,
+                                 num_iter: int = 5, expanded: bool = True, use_real_dist: bool = False,
+                                 d_crit: float = 0.03):
+        """Device-tensor form of extend_gen_spines (torch CUDA tensors)."""
+        self._check(lib().rvs_extend_gen_spines(self._ctx, robot, env, _ptr(q_root), _ptr(q_e), q_e.shape[0],
+                                                int(num_layers), int(num_iter), 1 if expanded else 0,
+                                                1 if use_real_dist else 0, float(d_crit), _ptr(q_new), _ptr(status),
+                                                _ptr(d), F_DEVICE), "rvs_extend_gen_spines")
 
     def self_collision_device(self, robot: int, q, min_dist=None, free_flag=None):
         self._check(lib().rvs_self_collision(self._ctx, robot, _ptr(q), q.shape[0], _ptr(min_dist), _ptr(free_flag),

@@ -271,3 +271,74 @@ def test_enclosing_radii(oracle, engine, which):
     assert got.shape == ref.shape
     assert ulps(got, ref).max() <= 2
     assert (got == ref).mean() > 0.999
+
+
+def _spine_inputs(which, n, seed):
+    rng = np.random.default_rng(seed)
+    if which == "xarm6":
+        rb, q = R.xarm6(), W.workload2(n).q
+    elif which == "chain22":
+        w = W.workload5(n)
+        rb, q = w.robot, w.q
+    else:
+        w = W.workload1(n)
+        rb, q = w.robot, w.q
+    u = rng.normal(size=q.shape)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    q_e = (q + np.pi * u).astype(np.float32)
+    d = rng.uniform(0.005, 0.3, n).astype(np.float32)
+    d[rng.random(n) < 0.02] = 0.0                                   # starts in contact: trapped
+    prof = (d[:, None] + rng.uniform(0, 0.2, (n, rb.num_links))).astype(np.float32)
+    return rb, q, d, prof, q_e
+
+
+@pytest.mark.parametrize("which", ["xarm6", "chain22", "planar2"])
+@pytest.mark.parametrize("num_iter,expanded", [(1, True), (5, True), (5, False)])
+def test_extend_spines(oracle, engine, which, num_iter, expanded):
+    """RBTConnect::extendSpine batch against the oracle: statuses identical,
+    end states identical bit for bit except where the device and host double
+    sine / cosine differ in the last bit (then within 1e-5 rad)."""
+    rb, q, d, prof, q_e = _spine_inputs(which, 8192, 41)
+    rid = engine.load_robot(rb)
+    qn, st = engine.extend_spines(rid, q, d, prof, q_e, num_iter=num_iter, expanded=expanded)
+    oqn, ost = oracle.extend_spines(rb, q, d, prof, q_e, num_iter=num_iter, expanded=expanded)
+    same = (qn == oqn).all(1)
+    print(f"{which} iter={num_iter} expanded={expanded}: status {np.bincount(st, minlength=4)[1:]}, "
+          f"status mismatches {(st != ost).sum()}, states identical {same.mean():.5f}, "
+          f"max diff {np.abs(qn - oqn).max():.3g}")
+    assert (st == ost).mean() >= 0.999
+    assert same.mean() >= 0.99
+    ok = st == ost
+    assert np.abs(qn[ok] - oqn[ok]).max() <= 1e-5
+    assert (st == 2).any() and (st == 1).any()
+
+
+@pytest.mark.parametrize("use_real_dist", [False, True])
+@pytest.mark.parametrize("num_iter", [1, 5])
+def test_extend_gen_spines(oracle, engine, use_real_dist, num_iter):
+    """RGBTConnect::extendGenSpine from one root towards 4096 targets (a
+    generalized bur) against the oracle's composition of the same steps."""
+    w = W.workload2(4096)
+    sc = oracle.scene_of(w.scene)
+    d, _, ver, _ = oracle.distance(w.robot, sc, w.q, 32)
+    root = w.q[np.nonzero((ver == 1) & (d > 0.06))[0][0]]
+    rng = np.random.default_rng(42)
+    u = rng.normal(size=(4096, 6))
+    q_e = (root + np.pi * u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+    rid = engine.load_robot(w.robot)
+    eid = engine.create_env(w.scene)
+    qn, st, dd = engine.extend_gen_spines(rid, eid, root, q_e, num_layers=5, num_iter=num_iter,
+                                          use_real_dist=use_real_dist, d_crit=0.03)
+    oqn, ost, odd = oracle.extend_gen_spines(w.robot, sc, root, q_e, num_layers=5, num_iter=num_iter,
+                                             use_real_dist=use_real_dist, d_crit=0.03)
+    same = (qn == oqn).all(1) & (dd == odd)
+    print(f"real={use_real_dist} iter={num_iter}: status {np.bincount(st, minlength=4)[1:]}, "
+          f"status mismatches {(st != ost).sum()}, identical {same.mean():.5f}")
+    assert (st == ost).mean() >= 0.999
+    assert same.mean() >= 0.99
+    ok = st == ost
+    assert np.abs(qn[ok] - oqn[ok]).max() <= 1e-4
+    # the root in contact: every target trapped at the root with d = 0
+    root_bad = w.q[np.nonzero(ver == 0)[0][0]]
+    qn, st, dd = engine.extend_gen_spines(rid, eid, root_bad, q_e[:64])
+    assert (st == 2).all() and (qn == root_bad[None]).all() and (dd == 0).all()

@@ -581,3 +581,173 @@ def test_enclosing_radii_planar_rule(oracle):
     assert np.allclose(Rm[:, 0, 1], d[:, 1] + 0.025, atol=1e-7)
     assert np.allclose(Rm[:, 0, 2], np.maximum(d[:, 1], d[:, 2]) + 0.025, atol=1e-7)
     assert np.allclose(Rm[:, 1, 2], d[:, 1] * 0 + np.linalg.norm(P[:, 2] - P[:, 1], axis=1) + 0.025, atol=1e-7)
+
+
+# ---------------------------------------------------------------- spines
+def _full_tol(scene):
+    """The scene with every min_dist_tol at MAX_DISTANCE.  An obstacle
+    farther than its tolerance counts as absent in the distance profile
+    (RealVectorSpace.cpp:362-363; the ground box's is 0.05 m), so the bubble
+    guarantees clearance only from the obstacles the profile sees: the
+    collision-freedom checks below use scenes where it sees all of them."""
+    return W.Scene(scene.dim, scene.pos, scene.quat, scene.is_ground,
+                   np.full(scene.num_obj, W.MAX_DISTANCE, np.float32))
+
+
+def _spine_case(oracle, n, seed, delta=np.pi):
+    """Collision-free xArm6 starts in the 16-box scene, targets at distance
+    delta (RBTConnect's DELTA) in random directions, with the starts'
+    distances and profiles."""
+    w = W.workload2(4096)
+    sc = oracle.scene_of(_full_tol(w.scene))
+    d, prof, ver, _ = oracle.distance(w.robot, sc, w.q, 32)
+    keep = np.nonzero(ver == 1)[0][:n]
+    q = w.q[keep]
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=q.shape)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    q_e = (q + delta * u).astype(np.float32)
+    return w, sc, q, d[keep], prof[keep], q_e
+
+
+def _line_fraction(q, q_e, q_new):
+    dq = (q_e - q).astype(np.float64)
+    t = ((q_new - q) * dq).sum(1) / (dq * dq).sum(1)
+    off = np.abs(q_new - (q + t[:, None] * dq)).max(1)
+    return t, off
+
+
+def test_spines_bubble_keeps_every_link_inside_its_clearance(oracle):
+    """The safety property of the expanded bubble (RBTConnect.cpp:85-150):
+    along the whole straight path from q to q_new every point of link i moves
+    at most profile[i] (its clearance), so the path is collision-free.  Checked
+    by brute force: FP64 kinematics at 201 states along each spine."""
+    w, sc, q, d, prof, q_e = _spine_case(oracle, 48, 31)
+    rb = w.robot
+    for num_iter in (1, 5):
+        qn, st = oracle.extend_spines(rb, q, d, prof, q_e, num_iter=num_iter)
+        assert set(np.unique(st)) <= {1, 3}
+        s = np.linspace(0.0, 1.0, 201)
+        P0 = oracle.skeleton(rb, q, 64)
+        worst = 0.0
+        for k in range(len(q)):
+            path = (q[k][None] + s[:, None] * (qn[k] - q[k])[None]).astype(np.float32)
+            P = oracle.skeleton(rb, path, 64)
+            move = np.linalg.norm(P - P0[k][None], axis=2)            # (201, L+1)
+            for i in range(rb.num_links):
+                worst = max(worst, float((np.maximum(move[:, i], move[:, i + 1]) - prof[k, i]).max()))
+            assert oracle.is_valid(rb, sc, path, 32).all()
+        assert worst <= 2e-6, worst
+
+
+def test_spines_take_the_largest_safe_step(oracle):
+    """One iteration: the step is the largest t with t * R.col(i+1).|q_e - q|
+    <= profile[i] for every link (expanded bubble), or t * R.col(n).|q_e - q|
+    <= d (plain bubble); t > 1 reaches q_e.  The radii come from the
+    separately pinned enclosing-radii restatement; the bound is evaluated
+    in FP64 here."""
+    w, sc, q, d, prof, q_e = _spine_case(oracle, 64, 32, delta=1.0)
+    rb = w.robot
+    Rm = oracle.enclosing_radii(rb, q).astype(np.float64)
+    adq = np.abs(q_e.astype(np.float64) - q)
+    cov = np.einsum("nij,ni->nj", Rm, adq)                           # (N, n+1): covered distance per point
+    for expanded in (True, False):
+        qn, st = oracle.extend_spines(rb, q, d, prof, q_e, num_iter=1, expanded=expanded)
+        if expanded:
+            with np.errstate(divide="ignore"):
+                t = (prof.astype(np.float64) / cov[:, 1:]).min(1)
+        else:
+            t = d.astype(np.float64) / cov[:, -1]
+        reach = t > 1
+        assert (st[reach] == 3).all() and (st[~reach] == 1).all()
+        assert (qn[reach] == q_e[reach]).all()
+        tn, off = _line_fraction(q[~reach], q_e[~reach], qn[~reach])
+        assert np.abs(tn - t[~reach]).max() <= 1e-5 * max(1.0, t.max())
+        assert off.max() <= 1e-6
+        if expanded:
+            assert (~reach).sum() >= 8     # the case must exercise the step
+
+
+def test_spines_more_iterations_go_further(oracle):
+    """Later iterations continue on the same line with the clearance left
+    over, so NUM_ITER_SPINE = 5 goes at least as far as 1 (and stays safe:
+    test above)."""
+    w, sc, q, d, prof, q_e = _spine_case(oracle, 64, 33)
+    rb = w.robot
+    q1, s1 = oracle.extend_spines(rb, q, d, prof, q_e, num_iter=1)
+    q5, s5 = oracle.extend_spines(rb, q, d, prof, q_e, num_iter=5)
+    t1, _ = _line_fraction(q, q_e, q1)
+    t5, off = _line_fraction(q, q_e, q5)
+    assert off.max() <= 1e-5
+    assert (t5 >= t1 - 1e-6).all()
+    assert (t5 > t1 + 1e-3).mean() > 0.5
+    assert ((s5 == 3) >= (s1 == 3)).all()
+
+
+def test_spines_trapped_and_reached(oracle):
+    """d <= 0 returns (Trapped, q) (RBTConnect.cpp:90-91); clearances far
+    larger than any motion reach q_e exactly."""
+    rb = R.xarm6()
+    rng = np.random.default_rng(34)
+    q = rng.uniform(-1, 1, (8, 6)).astype(np.float32)
+    q_e = rng.uniform(-1, 1, (8, 6)).astype(np.float32)
+    d = np.array([0, -1, 0.5, 0.5, 1e3, 1e3, 0.0, 1e3], np.float32)
+    prof = np.full((8, 6), 1e3, np.float32)
+    prof[2:4] = 0.01
+    qn, st = oracle.extend_spines(rb, q, d, prof, q_e, num_iter=5)
+    for k in (0, 1, 6):
+        assert st[k] == 2 and (qn[k] == q[k]).all()
+    for k in (4, 5, 7):
+        assert st[k] == 3 and (qn[k] == q_e[k]).all()
+    assert (st[2:4] == 1).all()
+
+
+def test_gen_spines_are_collision_free_and_underestimate(oracle):
+    """RGBTConnect::extendGenSpine (RGBTConnect.cpp:71-90) from one root, as
+    a generalized bur: every end state lies on the segment root -> target,
+    the path to it is collision-free, the returned distance is a lower bound
+    of the true distance (underestimation layers) or equal to it (real
+    distance layers), and a spine that still advances has d >= D_CRIT only
+    when it used every layer."""
+    w = W.workload2(4096)
+    sc = oracle.scene_of(_full_tol(w.scene))
+    rb = w.robot
+    d, _, ver, _ = oracle.distance(rb, sc, w.q, 32)
+    root = w.q[np.nonzero((ver == 1) & (d > 0.08))[0][0]]
+    D = rb.num_dofs
+    rng = np.random.default_rng(35)
+    targets = [root + s * np.pi / 2 * np.eye(D)[i] for i in range(D) for s in (-1, 1)]
+    u = rng.normal(size=(20, D))
+    targets += list(root + np.pi * u / np.linalg.norm(u, axis=1, keepdims=True))
+    q_e = np.asarray(targets, np.float32)
+    for real in (False, True):
+        qn, st, dd = oracle.extend_gen_spines(rb, sc, root, q_e, num_layers=5, num_iter=1,
+                                              use_real_dist=real, d_crit=0.03)
+        t, off = _line_fraction(np.repeat(root[None], len(q_e), 0), q_e, qn)
+        assert off.max() <= 1e-5 and (t >= -1e-6).all() and (t <= 1 + 1e-6).all()
+        assert (qn[st == 3] == q_e[st == 3]).all()
+        dt, _, _, _ = oracle.distance(rb, sc, qn, 32)
+        if real:
+            assert (dd == dt).all()
+        else:
+            assert (dd <= dt + 1e-6).all()
+        s = np.linspace(0, 1, 101)
+        for k in range(len(q_e)):
+            path = (root[None] + s[:, None] * (qn[k] - root)[None]).astype(np.float32)
+            assert oracle.is_valid(rb, sc, path, 32).all()
+        # a single layer stops earlier than five (or at the same state)
+        q1, _, _ = oracle.extend_gen_spines(rb, sc, root, q_e, num_layers=1, num_iter=1, use_real_dist=real)
+        t1, _ = _line_fraction(np.repeat(root[None], len(q_e), 0), q_e, q1)
+        assert (t >= t1 - 1e-6).all() and (t > t1 + 1e-3).any()
+
+
+def test_gen_spines_root_in_contact(oracle):
+    """A root in contact (d <= 0) returns (Trapped, root) for every target
+    (RGBTConnect.cpp:74-76)."""
+    w = W.workload2(4096)
+    sc = oracle.scene_of(w.scene)
+    d, _, ver, _ = oracle.distance(w.robot, sc, w.q, 32)
+    root = w.q[np.nonzero(ver == 0)[0][0]]
+    q_e = np.random.default_rng(36).uniform(-1, 1, (5, 6)).astype(np.float32)
+    qn, st, dd = oracle.extend_gen_spines(w.robot, sc, root, q_e)
+    assert (st == 2).all() and (qn == root[None]).all() and (dd == 0).all()
