@@ -248,6 +248,52 @@ def edges_section(eng, cores: int) -> dict:
             "edges_colliding_at_state0": int((got == 0).sum()), "edges_free": int((got == -1).sum())}
 
 
+def spines_section(eng, cores: int) -> dict:
+    """Generalized bur: extendGenSpine from one root towards 65,536 targets."""
+    import numpy as np
+    import torch
+    from rpmplv2_amd import workloads as W
+    O = oracle_module()
+    out = {}
+    for real in (False, True):
+        w = W.workload6(1 << 16, use_real_dist=real)
+        n = len(w.q_e)
+        rid = eng.load_robot(w.robot)
+        eid = eng.create_env(w.scene)
+        dev = torch.device("cuda:0")
+        qr, qe = torch.from_numpy(w.root).to(dev), torch.from_numpy(w.q_e).to(dev)
+        qn = torch.empty_like(qe)
+        st = torch.empty(n, dtype=torch.int32, device=dev)
+        dd = torch.empty(n, dtype=torch.float32, device=dev)
+        kw = dict(num_layers=w.num_layers, num_iter=w.num_iter, use_real_dist=w.use_real_dist, d_crit=w.d_crit)
+        for _ in range(2):
+            eng.extend_gen_spines_device(rid, eid, qr, qe, qn, st, dd, **kw)
+        torch.cuda.synchronize()
+        calls, t0 = 10, time.perf_counter()
+        for _ in range(calls):
+            eng.extend_gen_spines_device(rid, eid, qr, qe, qn, st, dd, **kw)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / calls
+        g_qn, g_st, g_d = qn.cpu().numpy(), st.cpu().numpy(), dd.cpu().numpy()
+        sc = O.scene_of(w.scene)
+        m = 8192
+        t1 = time.perf_counter()
+        o_qn, o_st, o_d = O.extend_gen_spines(w.robot, sc, w.root, w.q_e[:m], w.num_layers, w.num_iter, True,
+                                              w.use_real_dist, w.d_crit, nthreads=cores)
+        cpu_s = time.perf_counter() - t1
+        mism = int((~((g_qn[:m] == o_qn).all(1) & (g_st[:m] == o_st) & (g_d[:m] == o_d))).sum())
+        out["real_distance" if real else "underestimation"] = {
+            "workload": w.name, "spines": n, "num_layers": w.num_layers, "num_iter_spine": w.num_iter,
+            "d_crit": w.d_crit, "ms": ms, "spines_per_s": n / (ms * 1e-3),
+            "status_counts": {"advanced": int((g_st == 1).sum()), "trapped": int((g_st == 2).sum()),
+                              "reached": int((g_st == 3).sum())},
+            "mismatch_vs_oracle": mism, "checked": m,
+            "cpu_baseline": {"value": m / cpu_s, "unit": "spines/s", "cores": cores, "kind": "port",
+                             "sample": f"first {m} targets of the same bur"}}
+        del qr, qe, qn, st, dd
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,6 +397,8 @@ def main():
         out["workload5"] = workload5_section(eng, cores, args.w5_n)
         print("bench: edges", file=sys.stderr, flush=True)
         out["edges"] = edges_section(eng, cores)
+        print("bench: spines", file=sys.stderr, flush=True)
+        out["spines"] = spines_section(eng, cores)
         eng.close()
         print("bench: serving", file=sys.stderr, flush=True)
         from rpmplv2_amd import serving

@@ -111,6 +111,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built: run `python -m rpmplv2_amd.build`")
+        # torch ships its own HIP runtime under the same soname: loaded first,
+        # it is the one librvs binds to, so that the process holds a single
+        # runtime.  Loaded after librvs, it would start a second runtime that
+        # finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, i32, i64, u32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32
         ctx = C.c_void_p

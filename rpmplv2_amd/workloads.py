@@ -313,3 +313,35 @@ def workload5(n: int = 16777216, rotated_semantics: bool = True) -> ConfigBatch:
     rb = R.chain(lengths, radius=0.05, name="chain22", self_collision=True)
     q = rng.uniform(-np.pi, np.pi, size=(n, 22)).astype(np.float32)
     return ConfigBatch("chain22_512obb", rb, sc, q)
+
+
+# ---------------------------------------------------------------- workload 6
+@dataclass
+class BurWorkload:
+    """Generalized spines from one root towards many targets (a generalized
+    bur, RGBTConnect / DRGBT), with the planner's spine settings."""
+    name: str
+    robot: R.Robot
+    scene: Scene
+    root: np.ndarray        # (n,) float32
+    q_e: np.ndarray         # (N, n) float32
+    num_layers: int
+    num_iter: int
+    d_crit: float
+    use_real_dist: bool
+
+
+def workload6(n: int = 65536, use_real_dist: bool = False) -> BurWorkload:
+    """xArm6 in the ground + 16-box scene of workload 2, root at q = 0 (clear
+    of every box by construction, 0.062 m above the ground), targets at
+    DELTA = pi rad in uniform random directions; NUM_LAYERS 5,
+    NUM_ITER_SPINE 1, D_CRIT 0.05 (data/configurations/
+    configuration_rgbtconnect.yaml, configuration_rbtconnect.yaml)."""
+    sc = scene2(np.random.default_rng(1002))
+    rb = R.xarm6(radius=0.05, ground_included=1)
+    rng = np.random.default_rng(1006)
+    root = np.zeros(6, np.float32)
+    u = rng.normal(size=(n, 6))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    q_e = (root + np.pi * u).astype(np.float32)
+    return BurWorkload("xarm6_gbur_16aabb_ground", rb, sc, root, q_e, 5, 1, 0.05, use_real_dist)
