@@ -198,6 +198,20 @@ rvs_status rvs_self_collision(rvs_context* ctx, int32_t robot, const float* q, i
 rvs_status rvs_enclosing_radii(rvs_context* ctx, int32_t robot, const float* q, int64_t n, float* R,
                                uint32_t flags);
 
+/* Capsule part of xArm6::checkSelfCollision(q1, q2) (src/robots/xArm6.cpp:
+ * 284-368) for n edges: radii-bounded steps while the link pairs {0-4, 0-5,
+ * 1-4, 1-5} provably stay apart (at most 5), then a walk in eps_step
+ * (RRTConnectConfig::EPS_STEP) steps testing the pairs not yet proven apart.
+ * collide[e] = 1 when a walked state has overlapping capsules, with
+ * q2_out[e] = the last state before it (the reference's modified q2);
+ * 0 with q2_out[e] = q2[e] otherwise; 255 for an edge whose walk would not
+ * end (non-finite or huge coordinates).  A capsule overlap counts as a
+ * collision: the FCL mesh confirmation (checkRealSelfCollision, :426-483) is
+ * not part of this path, so the answer is conservative.  xArm6 only
+ * (RVS_ENOTSUP otherwise; the reference chains never self-collide). */
+rvs_status rvs_edge_self_collision(rvs_context* ctx, int32_t robot, const float* q1, const float* q2, int64_t n,
+                                   float eps_step, uint8_t* collide, float* q2_out, uint32_t flags);
+
 /* ---------------------------------------------------------------- spines */
 /* Spine outcomes, numbered as base::State::Status (include/state_spaces/State.h:22). */
 enum { RVS_SPINE_ADVANCED = 1, RVS_SPINE_TRAPPED = 2, RVS_SPINE_REACHED = 3 };

@@ -97,6 +97,7 @@ def _declare(L):
         "orc_capsule_obb64": (C.c_double, [f64p, f64p, C.c_double, f64p, Pp, f64p, i32p]),
         "orc_self_min32": (None, [R, Pp, f32p, C.c_int64, i32p, C.c_int32, f32p, C.c_int32]),
         "orc_enclosing_radii32": (None, [R, f32p, C.c_int64, f32p, C.c_int32]),
+        "orc_edge_self_collision32": (None, [R, Pp, f32p, f32p, C.c_int64, C.c_float, u8p, f32p, C.c_int32]),
         "orc_extend_spines32": (None, [R, f32p, f32p, f32p, f32p, C.c_int64, C.c_int32, C.c_int32, f32p, i32p,
                                        C.c_int32]),
         "orc_distance_cull32": (None, [R, Pp, S, f32p, C.c_int64, f32p, f32p, u8p, _p(C.c_int64), C.c_int32]),
@@ -406,4 +407,19 @@ def extend_gen_spines(rb, scene: Scene, q_root, q_e, num_layers=5, num_iter=5, e
         stop = (dl < np.float32(d_crit)) | (st != 1)
         active[idx[stop]] = False
     return q_new, status, d_out
+
+
+def edge_self_collision(rb, q1, q2, eps_step=0.1, params=None, nthreads=8):
+    """Capsule part of xArm6::checkSelfCollision(q1, q2) (xArm6.cpp:284-368):
+    (collide (E,) uint8, q2_out (E, dofs) float32)."""
+    p = params or params_struct()
+    D = rb.num_dofs
+    q1 = np.ascontiguousarray(q1, np.float32).reshape(-1, D)
+    q2 = np.ascontiguousarray(q2, np.float32).reshape(q1.shape)
+    col = np.zeros(len(q1), np.uint8)
+    out = np.zeros_like(q1)
+    rs = robot_struct(rb)
+    lib().orc_edge_self_collision32(C.byref(rs), C.byref(p), _ptr(q1, C.c_float), _ptr(q2, C.c_float), len(q1),
+                                    float(eps_step), _ptr(col, C.c_uint8), _ptr(out, C.c_float), nthreads)
+    return col, out
 

@@ -1263,6 +1263,84 @@ void orc_extend_spines32(const orc_robot* rb, const float* q, const float* d, co
     });
 }
 
+// xArm6::checkSelfCollision(q1, q2), src/robots/xArm6.cpp:284-368, with
+// checkRealSelfCollision (the FCL mesh test, :426-483) taken as true: a pair
+// collides when its capsules overlap.  Phase 1 (:300-324): at most 5
+// radii-bounded steps; phase 2 (:326-366): walk in eps_step steps, skipping
+// the pairs whose bounded step exceeds the walked fraction.  The walk is
+// capped for non-finite or huge coordinates (collide = 255).
+void orc_edge_self_collision32(const orc_robot* rb, const orc_params* p, const float* q1, const float* q2,
+                               int64_t n, float eps_step, uint8_t* collide, float* q_out, int32_t nthreads) {
+    std::vector<Seg> segs = build_segments(rb);
+    const int D = rb->num_dofs;
+    const P<float> pr(p);
+    parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+        std::vector<float> Pk(3 * (rb->num_links + 1)), R((size_t)D * (D + 1)), qt(D), qn(D), dq(D), prod(D);
+        for (int64_t c = b; c < e; ++c) {
+            const float* a = q1 + c * D;
+            const float* g = q2 + c * D;
+            float* out = q_out + c * D;
+            auto caps = [&](const float* q, int i, int j) {
+                skeleton<float>(rb, segs, q, Pk.data());
+                return capsules_distance<float>(rb, pr, Pk.data(), i, j);
+            };
+            auto covered = [&](int j) {
+                for (int k = 0; k < D; ++k) prod[k] = R[(size_t)k * (D + 1) + j] * dq[k];
+                return vecxf_sum(prod.data(), D);
+            };
+            std::copy(a, a + D, qt.begin());
+            float s04 = 0, s05 = 0, s14 = 0, s15 = 0;
+            bool free_edge = false;
+            for (int iter = 1;; ++iter) {
+                radii32(rb, segs, qt.data(), R.data(), Pk.data());
+                for (int k = 0; k < D; ++k) dq[k] = std::fabs(g[k] - qt[k]);
+                const float phi2 = covered(2), phi4 = covered(4), phi5 = covered(5), phi6 = covered(6);
+                s04 = caps(qt.data(), 0, 4) / std::max(phi4, phi5);
+                s05 = caps(qt.data(), 0, 5) / std::max(phi5, phi6);
+                s14 = caps(qt.data(), 1, 4) / (phi2 + std::max(phi4, phi5));
+                s15 = caps(qt.data(), 1, 5) / (phi2 + std::max(phi5, phi6));
+                const float step = std::min(std::min(s04, s05), std::min(s14, s15));
+                if (step > 1) { free_edge = true; break; }
+                if (iter == 5 || step <= 0) break;
+                for (int k = 0; k < D; ++k) qt[k] = qt[k] + step * (g[k] - qt[k]);
+            }
+            if (free_edge) { std::copy(g, g + D, out); collide[c] = 0; continue; }
+            auto norm_to_g = [&](const std::vector<float>& v) {
+                for (int k = 0; k < D; ++k) prod[k] = (g[k] - v[k]) * (g[k] - v[k]);
+                return std::sqrt(vecxf_sum(prod.data(), D));
+            };
+            const float Dn = norm_to_g(qt);
+            float dist = Dn;
+            const float span = Dn / eps_step;
+            const int cap = (span < 65536.0f) ? (int)span + 16 : 65552;
+            uint8_t hit = 0;
+            for (int walked = 0;; ++walked) {
+                if (walked > cap) { hit = 255; break; }
+                bool reached = false;
+                if (dist > eps_step) {
+                    for (int k = 0; k < D; ++k) qn[k] = qt[k] + (g[k] - qt[k]) / dist * eps_step;
+                } else {
+                    std::copy(g, g + D, qn.begin());
+                    reached = true;
+                }
+                dist = norm_to_g(qn);
+                const float crit = 1 - dist / Dn;
+                const bool skip[4] = {s05 > crit, s15 > crit, s04 > crit, s14 > crit};
+                const int pairs[4][2] = {{0, 5}, {1, 5}, {0, 4}, {1, 4}};
+                bool any = false;
+                for (int m = 0; m < 4 && !any; ++m)
+                    if (!skip[m] && caps(qn.data(), pairs[m][0], pairs[m][1]) < 0) any = true;
+                if (any) { hit = 1; break; }
+                qt = qn;
+                if (reached) break;
+            }
+            if (hit == 1) std::copy(qt.begin(), qt.end(), out);
+            else std::copy(g, g + D, out);
+            collide[c] = hit;
+        }
+    });
+}
+
 float orc_capsule_obb32(const float A[3], const float B[3], float r, const float obb[15], const orc_params* p,
                         float np[6], int32_t* np_valid) {
     P<float> pr(p); int v; float d = capsule_obb(A, B, r, obb, pr, np, &v); *np_valid = v; return d;

@@ -138,6 +138,11 @@ void orc_distance_cull64(const orc_robot* rb, const orc_params* p, const orc_sce
  * R [n][dofs][dofs+1]. */
 void orc_enclosing_radii32(const orc_robot* rb, const float* q, int64_t n, float* R, int32_t nthreads);
 
+/* Capsule part of xArm6::checkSelfCollision(q1, q2) for n edges: collide 1
+ * (q_out = last state before the hit), 0 (q_out = q2) or 255 (walk capped). */
+void orc_edge_self_collision32(const orc_robot* rb, const orc_params* p, const float* q1, const float* q2,
+                               int64_t n, float eps_step, uint8_t* collide, float* q_out, int32_t nthreads);
+
 /* RBTConnect::extendSpine for n spines (robot without self-collision
  * checking), float: q_new [n][dofs], status 1 advanced / 2 trapped /
  * 3 reached. */

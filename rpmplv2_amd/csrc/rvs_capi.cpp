@@ -1043,6 +1043,48 @@ rvs_status rvs_extend_gen_spines(rvs_context* c, int32_t robot, int32_t env, con
     return RVS_OK;
 }
 
+rvs_status rvs_edge_self_collision(rvs_context* c, int32_t robot, const float* q1, const float* q2, int64_t n,
+                                   float eps_step, uint8_t* collide, float* q2_out, uint32_t flags) {
+    rvs_status s = check_query(c, robot, -1, q1, n, false);
+    if (s != RVS_OK) return s;
+    if (n > 0 && (!q2 || !collide || !q2_out)) return fail(c, RVS_EINVAL, "rvs_edge_self_collision: null argument");
+    if (!(eps_step > 0.0f)) return fail(c, RVS_EINVAL, "rvs_edge_self_collision: eps_step must be positive");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    if (rb.dev.kind != RVS_SKEL_XARM6 || rb.num_links != 6 || rb.num_dofs != 6)
+        return fail(c, RVS_ENOTSUP, "rvs_edge_self_collision: the xArm6 pair rule only");
+    if (n == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const QueryParams qp = params_of(c);
+    const int D = rb.num_dofs;
+    const size_t qb = (size_t)n * D * 4;
+    hipError_t e = hipSuccess;
+    const float *a = q1, *b = q2;
+    uint8_t* cd = collide;
+    float* od = q2_out;
+    if (!(flags & RVS_F_DEVICE)) {
+        e = c->arena.ensure(Carve::need({qb, qb, (size_t)n, qb}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_self_collision: staging");
+        Carve cv(c->arena.p);
+        float* as = cv.take<float>((size_t)n * D);
+        float* bs = cv.take<float>((size_t)n * D);
+        cd = cv.take<uint8_t>(n);
+        od = cv.take<float>((size_t)n * D);
+        e = h2d(as, q1, qb, c->stream);
+        if (e == hipSuccess) e = h2d(bs, q2, qb, c->stream);
+        a = as;
+        b = bs;
+    }
+    if (e == hipSuccess) e = launch_edge_self(rb.dev, qp, a, b, n, eps_step, cd, od, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
+        e = d2h(collide, cd, (size_t)n, c->stream);
+        if (e == hipSuccess) e = d2h(q2_out, od, qb, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_self_collision");
+    return RVS_OK;
+}
+
 rvs_status rvs_edge_is_valid(rvs_context* c, int32_t robot, int32_t env, const float* q1, const float* q2,
                              int64_t n, uint8_t* valid, uint32_t flags) {
     rvs_status s = check_query(c, robot, env, q1, n, true);

@@ -154,6 +154,11 @@ hipError_t launch_spines(const RobotDev& rb, const SpineArgs& a, int64_t n, hipS
 hipError_t launch_gen_layer(int64_t n, int layer, const float* d_root, const float* d_layer, const int32_t* status,
                             uint8_t* active, float* d_out, float d_crit, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
+// Capsule part of xArm6::checkSelfCollision(q1, q2) per edge: collide[e] =
+// 1 (q_out = last state before the hit), 0 (q_out = q2) or 255 (the walk
+// would not end: non-finite or huge coordinates).
+hipError_t launch_edge_self(const RobotDev& rb, const QueryParams& qp, const float* q1, const float* q2, int64_t n,
+                            float eps_step, uint8_t* collide, float* q_out, hipStream_t st);
 
 // Per-pair distances (dist / dist64, [N][pairs]) and/or the per-configuration
 // minimum and collision-free flag (min_out / free_out); null outputs are skipped.

@@ -13,6 +13,14 @@
 // k_gen_layer: the bookkeeping of RGBTConnect::extendGenSpine
 // (src/planners/rbt/RGBTConnect.cpp:71-90) between layers.
 //
+// k_edge_self: the capsule part of xArm6::checkSelfCollision(q1, q2)
+// (src/robots/xArm6.cpp:284-368) for a batch of edges, one lane per edge:
+// radii-bounded steps along the edge while the four link pairs provably stay
+// apart, then a walk in EPS_STEP steps that tests the pairs not yet proven
+// apart.  A pair collides when its capsules overlap; the FCL mesh
+// confirmation of the reference (checkRealSelfCollision, :426-483) is not
+// part of this path.
+//
 // No FMA contraction anywhere in this translation unit, headers included:
 // the float arithmetic must keep the reference's rounding.
 #pragma clang fp contract(off)
@@ -142,6 +150,116 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
     if (c < n) p[c] = v;
 }
 
+// Floats of one lane's LDS slice for k_edge_self: skeleton 3 (D+1), radii
+// column, current state and next state (D each).
+__host__ __device__ inline int edge_self_slice(int D) { return 3 * (D + 1) + 3 * D; }
+
+__global__ __launch_bounds__(64) void k_edge_self(RobotDev rb, QueryParams qp, const float* __restrict__ q1,
+                                                  const float* __restrict__ q2, int64_t n, float eps_step,
+                                                  uint8_t* collide, float* q_out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int D = rb.num_dofs;
+    const int B = blockDim.x, t = threadIdx.x;
+    float* sk = reinterpret_cast<float*>(smem);             // [3 (D+1)][B]
+    float* ac = sk + (size_t)3 * (D + 1) * B;               // [D][B]
+    float* qt = ac + (size_t)D * B;                         // [D][B]
+    float* qn = qt + (size_t)D * B;                         // [D][B]
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const float* a = q1 + c * D;
+    const float* b = q2 + c * D;
+    float* out = q_out + c * D;
+    const Params<float> p{qp.eq, qp.maxd};
+    auto Q = [&](int k) -> float& { return qt[k * B + t]; };
+    auto QN = [&](int k) -> float& { return qn[k * B + t]; };
+    auto acc = [&](int i) -> float& { return ac[i * B + t]; };
+    auto P = [&](int k) { return Vec3<float>{sk[(3 * k) * B + t], sk[(3 * k + 1) * B + t], sk[(3 * k + 2) * B + t]}; };
+    auto skeleton_of = [&](float* qs) {
+        FKStream fk;
+        fk_init(fk);
+        for (int k = 0; k <= D; ++k) {
+            const Vec3<float> v = skel_point<float>(fk, rb, qs + t, k, B);
+            sk[(3 * k) * B + t] = v.x;
+            sk[(3 * k + 1) * B + t] = v.y;
+            sk[(3 * k + 2) * B + t] = v.z;
+        }
+    };
+    // computeCapsulesDistance (xArm6.cpp:417-424) on the skeleton in LDS
+    auto caps = [&](int i, int j) {
+        return seg_seg<float, false>(P(i), P(i + 1), P(j), P(j + 1), p, nullptr) - rb.radius[i] - rb.radius[j];
+    };
+    auto finish = [&](uint8_t hit, bool from_state) {
+        for (int k = 0; k < D; ++k) out[k] = from_state ? Q(k) : b[k];
+        collide[c] = hit;
+    };
+    for (int k = 0; k < D; ++k) Q(k) = a[k];
+    // radii-bounded steps (:300-324)
+    float s04 = 0.0f, s05 = 0.0f, s14 = 0.0f, s15 = 0.0f;
+    for (int iter = 1;; ++iter) {
+        skeleton_of(qt);
+        for (int i = 0; i < D; ++i) acc(i) = 0.0f;
+        float phi2 = 0.0f, phi4 = 0.0f, phi5 = 0.0f, phi6 = 0.0f;
+        for (int j = 1; j <= D; ++j) {
+            radii_column(rb, P, j, acc);
+            if (j == 2 || j >= 4) {
+                const float v = eigen_dyn_sum([&](int i) { return acc(i) * fabsf(b[i] - Q(i)); }, D);
+                if (j == 2) phi2 = v;
+                else if (j == 4) phi4 = v;
+                else if (j == 5) phi5 = v;
+                else phi6 = v;
+            }
+        }
+        s04 = caps(0, 4) / max_std(phi4, phi5);
+        s05 = caps(0, 5) / max_std(phi5, phi6);
+        s14 = caps(1, 4) / (phi2 + max_std(phi4, phi5));
+        s15 = caps(1, 5) / (phi2 + max_std(phi5, phi6));
+        const float step = min_std(min_std(s04, s05), min_std(s14, s15));
+        if (step > 1.0f) {
+            finish(0, false);
+            return;
+        }
+        if (iter == 5 || step <= 0.0f) break;
+        for (int k = 0; k < D; ++k) Q(k) = Q(k) + step * (b[k] - Q(k));
+    }
+    // walk in EPS_STEP steps from the last bounded state (:326-366); a pair
+    // whose bounded step exceeds the walked fraction is proven apart.  The
+    // step count is capped for non-finite or huge coordinates (the walk
+    // would not end); such an edge is reported as 255.
+    auto norm_to_b = [&](auto qv) {
+        return sqrtf(eigen_dyn_sum([&](int i) { const float e = b[i] - qv(i); return e * e; }, D));
+    };
+    const float Dn = norm_to_b(Q);
+    float dist = Dn;
+    const float span = Dn / eps_step;
+    const int cap = (span < 65536.0f) ? (int)span + 16 : 65552;
+    for (int walked = 0;; ++walked) {
+        if (walked > cap) {
+            finish(255, false);
+            return;
+        }
+        bool reached = false;
+        if (dist > eps_step) {
+            for (int k = 0; k < D; ++k) QN(k) = Q(k) + (b[k] - Q(k)) / dist * eps_step;
+        } else {
+            for (int k = 0; k < D; ++k) QN(k) = b[k];
+            reached = true;
+        }
+        dist = norm_to_b(QN);
+        const float crit = 1.0f - dist / Dn;
+        skeleton_of(qn);
+        // xArm6::checkSelfCollision(q, skip) pair order (:384-411)
+        const bool hit = (!(s05 > crit) && caps(0, 5) < 0.0f) || (!(s15 > crit) && caps(1, 5) < 0.0f) ||
+                         (!(s04 > crit) && caps(0, 4) < 0.0f) || (!(s14 > crit) && caps(1, 4) < 0.0f);
+        if (hit) {
+            finish(1, true);
+            return;
+        }
+        for (int k = 0; k < D; ++k) Q(k) = QN(k);
+        if (reached) break;
+    }
+    finish(0, false);
+}
+
 }  // namespace
 
 size_t spine_lds_per_lane(const RobotDev& rb) { return (size_t)spine_slice(rb.num_dofs) * sizeof(float); }
@@ -166,6 +284,18 @@ hipError_t launch_gen_layer(int64_t n, int layer, const float* d_root, const flo
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gen_layer, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, layer, d_root, d_layer,
                        status, active, d_out, d_crit);
+    return hipGetLastError();
+}
+
+size_t edge_self_lds_per_lane(const RobotDev& rb) { return (size_t)edge_self_slice(rb.num_dofs) * sizeof(float); }
+
+hipError_t launch_edge_self(const RobotDev& rb, const QueryParams& qp, const float* q1, const float* q2, int64_t n,
+                            float eps_step, uint8_t* collide, float* q_out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int B = 64;
+    const size_t sm = edge_self_lds_per_lane(rb) * B;
+    hipLaunchKernelGGL(k_edge_self, dim3((unsigned)((n + B - 1) / B)), dim3(B), sm, st, rb, qp, q1, q2, n, eps_step,
+                       collide, q_out);
     return hipGetLastError();
 }
 

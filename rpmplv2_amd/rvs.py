@@ -98,7 +98,7 @@ EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds", "rvs_distance",
            "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_self_distance",
            "rvs_self_collision", "rvs_enclosing_radii", "rvs_extend_spines", "rvs_extend_gen_spines",
-           "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
+           "rvs_edge_self_collision", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
            "rvs_autotune", "rvs_set_tuning", "rvs_get_tuning", "rvs_autotune_edges", "rvs_set_edge_tuning", "rvs_server_default_options", "rvs_server_create", "rvs_server_destroy", "rvs_server_set_motion",
            "rvs_server_submit", "rvs_server_wait", "rvs_server_get_stats", "rvs_server_now",
            "rvs_server_last_error", "rvs_host_alloc", "rvs_host_free"]
@@ -147,6 +147,7 @@ def lib():
         L.rvs_self_distance.argtypes = [ctx, i32, vp, i64, vp, u32]
         L.rvs_self_collision.argtypes = [ctx, i32, vp, i64, vp, vp, u32]
         L.rvs_enclosing_radii.argtypes = [ctx, i32, vp, i64, vp, u32]
+        L.rvs_edge_self_collision.argtypes = [ctx, i32, vp, vp, i64, C.c_float, vp, vp, u32]
         L.rvs_extend_spines.argtypes = [ctx, i32, vp, vp, vp, vp, i64, i32, i32, vp, vp, u32]
         L.rvs_extend_gen_spines.argtypes = [ctx, i32, i32, vp, vp, i64, i32, i32, i32, i32, C.c_float, vp, vp, vp,
                                             u32]
@@ -364,6 +365,18 @@ class Engine:
         out = np.zeros((len(q), D, D + 1), np.float32)
         self._check(lib().rvs_enclosing_radii(self._ctx, robot, _ptr(q), len(q), _ptr(out), 0), "rvs_enclosing_radii")
         return out
+
+    def edge_self_collision(self, robot: int, q1, q2, eps_step: float = 0.1):
+        """Capsule part of xArm6::checkSelfCollision(q1, q2) per edge:
+        (collide (E,) uint8, q2_out (E, dofs))."""
+        rb = self.robots[robot]
+        q1 = np.ascontiguousarray(q1, np.float32).reshape(-1, rb.num_dofs)
+        q2 = np.ascontiguousarray(q2, np.float32).reshape(q1.shape)
+        col = np.zeros(len(q1), np.uint8)
+        out = np.zeros_like(q1)
+        self._check(lib().rvs_edge_self_collision(self._ctx, robot, _ptr(q1), _ptr(q2), len(q1), float(eps_step),
+                                                  _ptr(col), _ptr(out), 0), "rvs_edge_self_collision")
+        return col, out
 
     def extend_spines(self, robot: int, q, d, profile, q_e, num_iter: int = 5, expanded: bool = True):
         """RBTConnect::extendSpine for a batch of spines (robot without

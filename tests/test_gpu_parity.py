@@ -342,3 +342,25 @@ def test_extend_gen_spines(oracle, engine, use_real_dist, num_iter):
     root_bad = w.q[np.nonzero(ver == 0)[0][0]]
     qn, st, dd = engine.extend_gen_spines(rid, eid, root_bad, q_e[:64])
     assert (st == 2).all() and (qn == root_bad[None]).all() and (dd == 0).all()
+
+
+@pytest.mark.parametrize("eps_step", [0.1, 0.02])
+def test_edge_self_collision(oracle, engine, eps_step):
+    """Capsule part of xArm6::checkSelfCollision(q1, q2) for 16384 edges
+    against the oracle: verdicts identical, returned states identical."""
+    rb = R.xarm6(radius=0.05)
+    rng = np.random.default_rng(61)
+    lim = np.asarray(rb.limits, np.float64)
+    q1 = (lim[:, 0] + rng.random((16384, 6)) * (lim[:, 1] - lim[:, 0])).astype(np.float32)
+    u = rng.normal(size=q1.shape)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    q2 = (q1 + rng.uniform(0.05, 3.0, len(q1))[:, None] * u).astype(np.float32)
+    rid = engine.load_robot(rb)
+    col, out = engine.edge_self_collision(rid, q1, q2, eps_step)
+    ocol, oout = oracle.edge_self_collision(rb, q1, q2, eps_step)
+    same = (out == oout).all(1)
+    print(f"eps {eps_step}: collide {col.sum()} of {len(col)}, verdict mismatches {(col != ocol).sum()}, "
+          f"states identical {same.mean():.5f}")
+    assert (col == ocol).mean() >= 0.999
+    assert same.mean() >= 0.995
+    assert col.sum() > 100

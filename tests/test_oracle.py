@@ -751,3 +751,123 @@ def test_gen_spines_root_in_contact(oracle):
     q_e = np.random.default_rng(36).uniform(-1, 1, (5, 6)).astype(np.float32)
     qn, st, dd = oracle.extend_gen_spines(w.robot, sc, root, q_e)
     assert (st == 2).all() and (qn == root[None]).all() and (dd == 0).all()
+
+
+# ---------------------------------------------------------------- edge self-collision
+def _seg_seg_dist(P1, Q1, P2, Q2):
+    """Distance of segments P1Q1 and P2Q2, vectorised FP64 (closest points of
+    two segments by clamped parameters, Ericson, Real-Time Collision
+    Detection, 5.1.9)."""
+    d1, d2, r = Q1 - P1, Q2 - P2, P1 - P2
+    a, e, f = (d1 * d1).sum(-1), (d2 * d2).sum(-1), (d2 * r).sum(-1)
+    c, b = (d1 * r).sum(-1), (d1 * d2).sum(-1)
+    den = a * e - b * b
+    s = np.where(den > 1e-18, np.clip((b * f - c * e) / np.where(den > 1e-18, den, 1), 0, 1), 0.0)
+    t = (b * s + f) / np.where(e > 1e-18, e, 1)
+    s = np.where(t < 0, np.clip(-c / np.where(a > 1e-18, a, 1), 0, 1), np.where(t > 1, np.clip((b - c) / np.where(a > 1e-18, a, 1), 0, 1), s))
+    t = np.clip(t, 0, 1)
+    return np.linalg.norm((P1 + d1 * s[..., None]) - (P2 + d2 * t[..., None]), axis=-1)
+
+
+def _xarm_pairs_min(oracle, rb, q):
+    """Smallest capsule distance over the xArm6 pairs {0-5, 1-5, 0-4, 1-4}, FP64."""
+    P = oracle.skeleton(rb, q, 64)
+    out = np.full(len(q), np.inf)
+    for a, b in ((0, 5), (1, 5), (0, 4), (1, 4)):
+        d = _seg_seg_dist(P[:, a], P[:, a + 1], P[:, b], P[:, b + 1]) - rb.radius[a] - rb.radius[b]
+        out = np.minimum(out, d)
+    return out
+
+
+def _xarm_edges(oracle, n, seed):
+    rb = R.xarm6(radius=0.05)
+    rng = np.random.default_rng(seed)
+    lim = np.asarray(rb.limits, np.float64)
+    q1 = (lim[:, 0] + rng.random((4 * n, 6)) * (lim[:, 1] - lim[:, 0])).astype(np.float32)
+    q1 = q1[_xarm_pairs_min(oracle, rb, q1) > 1e-3][:n]
+    u = rng.normal(size=q1.shape)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    q2 = (q1 + rng.uniform(0.05, 3.0, len(q1))[:, None] * u).astype(np.float32)
+    return rb, q1, q2
+
+
+def test_edge_self_collision_against_dense_sampling(oracle):
+    """xArm6::checkSelfCollision(q1, q2) capsule part (xArm6.cpp:284-368):
+    a free verdict never hides a deep overlap found by dense FP64 sampling
+    (the radii-bounded steps must be conservative); a collision verdict
+    returns a collision-free state on the edge whose next EPS_STEP state has
+    overlapping capsules; q2 comes back unchanged for a free edge."""
+    rb, q1, q2 = _xarm_edges(oracle, 400, 52)
+    eps = 0.02
+    col, out = oracle.edge_self_collision(rb, q1, q2, eps)
+    assert set(np.unique(col)) <= {0, 1}
+    assert 10 <= col.sum() <= len(col) - 10
+    assert (out[col == 0] == q2[col == 0]).all()
+    s = np.linspace(0, 1, 801)
+    for k in range(len(q1)):
+        path = (q1[k][None] + s[:, None] * (q2[k] - q1[k])[None]).astype(np.float32)
+        dense = _xarm_pairs_min(oracle, rb, path).min()
+        if col[k] == 0:
+            assert dense > -0.02, (k, dense)
+        else:
+            dq = (q2[k] - q1[k]).astype(np.float64)
+            t = float(np.dot(out[k] - q1[k], dq) / np.dot(dq, dq))
+            assert -1e-6 <= t <= 1 + 1e-6
+            assert np.abs(out[k] - (q1[k] + t * dq)).max() <= 1e-5
+            assert _xarm_pairs_min(oracle, rb, out[k][None])[0] >= -1e-5
+            rest = q2[k].astype(np.float64) - out[k]
+            nr = np.linalg.norm(rest)
+            nxt = q2[k] if nr <= eps else (out[k] + rest / nr * eps).astype(np.float32)
+            assert _xarm_pairs_min(oracle, rb, np.asarray(nxt, np.float32)[None])[0] < 1e-5
+
+
+def test_edge_self_collision_special_cases(oracle):
+    """q1 == q2 free: no collision; an edge ending in a self-colliding state
+    is always reported (the last walked state is q2 with no pair skipped
+    unless proven apart)."""
+    rb, q1, q2 = _xarm_edges(oracle, 200, 53)
+    col, out = oracle.edge_self_collision(rb, q1, q1)
+    assert (col == 0).all() and (out == q1).all()
+    lim = np.asarray(rb.limits, np.float64)
+    rng = np.random.default_rng(54)
+    qb = (lim[:, 0] + rng.random((4000, 6)) * (lim[:, 1] - lim[:, 0])).astype(np.float32)
+    bad = qb[_xarm_pairs_min(oracle, rb, qb) < -1e-3][:50]
+    assert len(bad) >= 20
+    col, out = oracle.edge_self_collision(rb, q1[:len(bad)], bad, 0.1)
+    assert (col == 1).all()
+    # the start state itself is never tested (the reference takes q1 as
+    # valid): a start in self-collision is reported only when a walked state
+    # overlaps too
+    col, out = oracle.edge_self_collision(rb, bad, q1[:len(bad)], 0.1)
+    assert 0 < col.sum() < len(bad)
+
+
+def test_edge_self_collision_bounded_steps_are_conservative(oracle):
+    """Starts close to self-contact (pair distance 1-50 mm), a fine walk
+    (eps_step 0.002 rad): the radii-bounded steps must never carry the state
+    past the first overlap that dense FP64 sampling finds, so a reported
+    collision returns a state no later than that overlap, and a free verdict
+    has no overlap at all.  (Which radii columns and sums the bound uses is
+    fixed by xArm6.cpp:300-324; a bound that stays conservative but differs
+    in those details is not told apart here, only by the device parity.)"""
+    rb = R.xarm6(radius=0.05)
+    rng = np.random.default_rng(55)
+    lim = np.asarray(rb.limits, np.float64)
+    qa = (lim[:, 0] + rng.random((40000, 6)) * (lim[:, 1] - lim[:, 0])).astype(np.float32)
+    m = _xarm_pairs_min(oracle, rb, qa)
+    q1 = qa[(m > 1e-3) & (m < 0.05)][:300]
+    u = rng.normal(size=q1.shape)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    q2 = (q1 + rng.uniform(0.3, 3.0, len(q1))[:, None] * u).astype(np.float32)
+    col, out = oracle.edge_self_collision(rb, q1, q2, 0.002)
+    assert 50 <= col.sum() <= 250
+    s = np.linspace(0, 1, 3001)
+    for k in range(len(q1)):
+        path = (q1[k][None] + s[:, None] * (q2[k] - q1[k])[None]).astype(np.float32)
+        hit = np.nonzero(_xarm_pairs_min(oracle, rb, path) < -1e-4)[0]
+        if col[k] == 0:
+            assert len(hit) == 0, k
+        elif len(hit):
+            dq = (q2[k] - q1[k]).astype(np.float64)
+            t = float(np.dot(out[k] - q1[k], dq) / np.dot(dq, dq))
+            assert t <= s[hit[0]] + 1e-3, (k, t, s[hit[0]])
