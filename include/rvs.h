@@ -171,6 +171,31 @@ rvs_status rvs_distance_underestimation(rvs_context* ctx, int32_t robot, int32_t
                                         const float* nearest_ref, int64_t ref_stride, float* d, float* profile,
                                         uint8_t* stored, uint32_t flags);
 
+/* AbstractTrajectory::computeDistanceUnderestimation(q, nearest_points,
+ * delta_t) (src/planners/trajectory/AbstractTrajectory.cpp:272-320): as
+ * rvs_distance_underestimation, with every plane first moved towards the
+ * robot by max_vel[j] * delta_t (max_vel [M], Object::getMaxVel; 0 for
+ * static boxes). */
+rvs_status rvs_distance_underestimation_moving(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n,
+                                               const float* nearest_ref, int64_t ref_stride, const float* max_vel,
+                                               float delta_t, float* d, float* profile, uint8_t* stored,
+                                               uint32_t flags);
+
+/* AbstractTrajectory::isSafe (src/planners/trajectory/AbstractTrajectory.cpp:
+ * 212-256) for num_traj candidate trajectories leaving one current state.
+ * points [num_traj][num_points][dofs] are the positions at t_iter,
+ * t_iter + time_step, ... (TrajectoryConfig::TIME_STEP); q_current [dofs]
+ * with its distance profile [links] and nearest points [M][links][6] from
+ * rvs_distance; max_vel [M] per obstacle.  safe[k] = 1 when trajectory k
+ * is safe.  self_collision = 1 also rejects a point whose self-collision
+ * capsule pairs overlap (robot->checkSelfCollision(q_final) without the FCL
+ * mesh confirmation, so conservative); 0 for robots without self-collision
+ * checking.  Needs one link per revolute joint. */
+rvs_status rvs_trajectory_is_safe(rvs_context* ctx, int32_t robot, int32_t env, const float* points, int64_t num_traj,
+                                  int32_t num_points, const float* q_current, const float* profile_current,
+                                  const float* nearest_current, const float* max_vel, float t_iter, float time_step,
+                                  int32_t self_collision, uint8_t* safe, uint32_t flags);
+
 /* Capsule-capsule distances of the robot's self-collision pairs
  * (xArm6::computeCapsulesDistance, src/robots/xArm6.cpp:417-424), [N][P].
  * A pair with distance < 0 is the capsule part of checkSelfCollision

@@ -97,6 +97,10 @@ def _declare(L):
         "orc_capsule_obb64": (C.c_double, [f64p, f64p, C.c_double, f64p, Pp, f64p, i32p]),
         "orc_self_min32": (None, [R, Pp, f32p, C.c_int64, i32p, C.c_int32, f32p, C.c_int32]),
         "orc_enclosing_radii32": (None, [R, f32p, C.c_int64, f32p, C.c_int32]),
+        "orc_underestimation_moving32": (None, [R, Pp, S, f32p, C.c_int64, f32p, C.c_int64, f32p, C.c_float, f32p,
+                                                f32p, u8p, C.c_int32]),
+        "orc_traj_safe32": (None, [R, Pp, S, f32p, C.c_int64, C.c_int32, f32p, f32p, f32p, f32p, C.c_float,
+                                   C.c_float, i32p, C.c_int32, u8p, C.c_int32]),
         "orc_edge_self_collision32": (None, [R, Pp, f32p, f32p, C.c_int64, C.c_float, u8p, f32p, C.c_int32]),
         "orc_extend_spines32": (None, [R, f32p, f32p, f32p, f32p, C.c_int64, C.c_int32, C.c_int32, f32p, i32p,
                                        C.c_int32]),
@@ -422,4 +426,39 @@ def edge_self_collision(rb, q1, q2, eps_step=0.1, params=None, nthreads=8):
     lib().orc_edge_self_collision32(C.byref(rs), C.byref(p), _ptr(q1, C.c_float), _ptr(q2, C.c_float), len(q1),
                                     float(eps_step), _ptr(col, C.c_uint8), _ptr(out, C.c_float), nthreads)
     return col, out
+
+
+def underestimation_moving(rb, scene: Scene, q, np_ref, max_vel, delta_t, params=None, nthreads=8):
+    """AbstractTrajectory::computeDistanceUnderestimation (planes moved by
+    max_vel * delta_t): (d, profile, stored)."""
+    p = params or params_struct()
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+    n, L, M = len(q), rb.num_links, len(scene.obs)
+    npr = np.ascontiguousarray(np_ref, np.float32)
+    stride = 0 if npr.ndim == 3 else M * L * 6
+    mv = np.ascontiguousarray(max_vel, np.float32).reshape(M)
+    d = np.zeros(n, np.float32); prof = np.zeros((n, L), np.float32); st = np.zeros(n, np.uint8)
+    rs = robot_struct(rb)
+    lib().orc_underestimation_moving32(C.byref(rs), C.byref(p), C.byref(scene.s), _ptr(q, C.c_float), n,
+                                       _ptr(npr, C.c_float), stride, _ptr(mv, C.c_float), float(delta_t),
+                                       _ptr(d, C.c_float), _ptr(prof, C.c_float), _ptr(st, C.c_uint8), nthreads)
+    return d, prof, st
+
+
+def traj_safe(rb, scene: Scene, points, q_current, profile_current, nearest_current, max_vel, t_iter=0.0,
+              time_step=0.01, pairs=None, params=None, nthreads=8):
+    """AbstractTrajectory::isSafe for trajectories points (T, K, dofs): safe (T,) uint8."""
+    p = params or params_struct()
+    pts = np.ascontiguousarray(points, np.float32)
+    T, K = pts.shape[0], pts.shape[1]
+    M = len(scene.obs)
+    pr = np.ascontiguousarray(np.asarray(pairs if pairs is not None else [], np.int32).reshape(-1, 2))
+    args = [np.ascontiguousarray(x, np.float32) for x in (q_current, profile_current, nearest_current)]
+    mv = np.ascontiguousarray(max_vel, np.float32).reshape(M)
+    out = np.zeros(T, np.uint8)
+    rs = robot_struct(rb)
+    lib().orc_traj_safe32(C.byref(rs), C.byref(p), C.byref(scene.s), _ptr(pts, C.c_float), T, K,
+                          *[_ptr(x, C.c_float) for x in args], _ptr(mv, C.c_float), float(t_iter), float(time_step),
+                          _ptr(pr, C.c_int32), len(pr), _ptr(out, C.c_uint8), nthreads)
+    return out
 

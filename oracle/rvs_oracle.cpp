@@ -837,9 +837,12 @@ template <typename T> inline void normalized(const T* v, T* o) {
 // computeDistanceUnderestimation: RealVectorSpace.cpp:402-446 on a fresh
 // state (d_c = -1, so the early return at :405-406 never fires).
 // 'stored' = 0 when the reference returns 0 without touching the state.
+// With max_vel (AbstractTrajectory::computeDistanceUnderestimation,
+// AbstractTrajectory.cpp:272-320) R and O first shift by
+// normalize(R - O) * max_vel[j] * delta_t (:297-300).
 template <typename T>
 void under_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, const T* Pk, const T* npr,
-               T* d_out, T* prof, uint8_t* stored) {
+               T* d_out, T* prof, uint8_t* stored, const T* max_vel = nullptr, T delta_t = 0) {
     const int L = rb->num_links;
     T d_c = pr.maxd;
     for (int i = 0; i < L; ++i) prof[i] = 0;
@@ -851,6 +854,15 @@ void under_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, const T
             if (norm3(O) == pr.maxd) continue;
             T R[3] = { e[0], e[1], e[2] };
             T RO[3], n[3], a[3], b[3];
+            if (max_vel) {
+                sub3(R, O, RO);
+                normalized(RO, n);
+                for (int k = 0; k < 3; ++k) {
+                    const T dl = n[k] * max_vel[j] * delta_t;
+                    R[k] = R[k] + dl;
+                    O[k] = O[k] + dl;
+                }
+            }
             sub3(R, O, RO);
             normalized(RO, n);
             sub3(Pk + 3 * i, O, a);
@@ -1073,6 +1085,82 @@ void orc_underestimation32(const orc_robot* rb, const orc_params* p, const orc_s
             under_one(rb, pr, sc, Pk.data(), np_ref + c * np_stride, &dd, prof.data(), &st);
             d[c] = dd; stored[c] = st;
             if (profile) for (int i = 0; i < L; ++i) profile[c * L + i] = prof[i];
+        }
+    });
+}
+
+void orc_underestimation_moving32(const orc_robot* rb, const orc_params* p, const orc_scene* scn, const float* q,
+                                  int64_t n, const float* np_ref, int64_t np_stride, const float* max_vel,
+                                  float delta_t, float* d, float* profile, uint8_t* stored, int32_t nthreads) {
+    P<float> pr(p);
+    SceneT<float> sc(scn);
+    std::vector<Seg> segs = build_segments(rb);
+    const int L = rb->num_links, D = rb->num_dofs;
+    parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+        std::vector<float> Pk(3 * (L + 1)), prof(L);
+        for (int64_t c = b; c < e; ++c) {
+            skeleton<float>(rb, segs, q + c * D, Pk.data());
+            float dd; uint8_t st;
+            under_one(rb, pr, sc, Pk.data(), np_ref + c * np_stride, &dd, prof.data(), &st, max_vel, delta_t);
+            d[c] = dd; stored[c] = st;
+            if (profile) for (int i = 0; i < L; ++i) profile[c * L + i] = prof[i];
+        }
+    });
+}
+
+// AbstractTrajectory::isSafe, AbstractTrajectory.cpp:212-256, for num_traj
+// trajectories of num_points points leaving q_current.  max_obs_vel is the
+// largest max_vel from 0 (:44-49).  checkSelfCollision(q_final) is the
+// capsule test of the listed pairs when self_collision is set.
+void orc_traj_safe32(const orc_robot* rb, const orc_params* p, const orc_scene* scn, const float* points,
+                     int64_t num_traj, int32_t num_points, const float* q_current, const float* profile_current,
+                     const float* nearest_current, const float* max_vel, float t_iter0, float time_step,
+                     const int32_t* pairs, int32_t npairs, uint8_t* safe, int32_t nthreads) {
+    P<float> pr(p);
+    SceneT<float> sc(scn);
+    std::vector<Seg> segs = build_segments(rb);
+    const int L = rb->num_links, D = rb->num_dofs;
+    float max_obs_vel = 0;
+    for (int j = 0; j < sc.M; ++j)
+        if (max_vel[j] > max_obs_vel) max_obs_vel = max_vel[j];
+    parallel_for(num_traj, nthreads, [&](int64_t b, int64_t e) {
+        std::vector<float> Pk(3 * (L + 1)), R((size_t)D * (D + 1)), qt(D), prof(L), pn(L), dq(D), prod(D);
+        for (int64_t c = b; c < e; ++c) {
+            std::copy(q_current, q_current + D, qt.begin());
+            std::copy(profile_current, profile_current + L, prof.begin());
+            float t_iter = t_iter0, t_init = 0;
+            uint8_t ok = 1;
+            for (int i = 0; i < num_points && ok; ++i) {
+                const float* qf = points + ((int64_t)c * num_points + i) * D;
+                if (npairs > 0) {
+                    skeleton<float>(rb, segs, qf, Pk.data());
+                    for (int m = 0; m < npairs; ++m)
+                        if (capsules_distance<float>(rb, pr, Pk.data(), pairs[2 * m], pairs[2 * m + 1]) < 0) ok = 0;
+                    if (!ok) break;
+                }
+                const float rho_obs = max_obs_vel * (t_iter - t_init);
+                for (int k = 0; k < D; ++k) dq[k] = std::fabs(qf[k] - qt[k]);
+                radii32(rb, segs, qt.data(), R.data(), Pk.data());
+                for (int j = 0; j < D; ++j) {
+                    for (int k = 0; k < D; ++k) prod[k] = R[(size_t)k * (D + 1) + j + 1] * dq[k];
+                    const float rho_robot = vecxf_sum(prod.data(), D);
+                    if (rho_robot + rho_obs >= prof[j]) {
+                        std::copy(qf, qf + D, qt.begin());
+                        skeleton<float>(rb, segs, qt.data(), Pk.data());
+                        float dd; uint8_t st;
+                        under_one(rb, pr, sc, Pk.data(), nearest_current, &dd, pn.data(), &st, max_vel, t_iter);
+                        t_init = t_iter;
+                        if (!st) ok = 0;      // the state keeps d_c = -1
+                        else {
+                            prof = pn;
+                            if (dd <= 0) ok = 0;
+                        }
+                        break;
+                    }
+                }
+                t_iter += time_step;
+            }
+            safe[c] = ok;
         }
     });
 }

@@ -138,6 +138,20 @@ void orc_distance_cull64(const orc_robot* rb, const orc_params* p, const orc_sce
  * R [n][dofs][dofs+1]. */
 void orc_enclosing_radii32(const orc_robot* rb, const float* q, int64_t n, float* R, int32_t nthreads);
 
+/* AbstractTrajectory::computeDistanceUnderestimation(q, np, delta_t): the
+ * planes move by max_vel[j] * delta_t towards the robot first. */
+void orc_underestimation_moving32(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q,
+                                  int64_t n, const float* np_ref, int64_t np_stride, const float* max_vel,
+                                  float delta_t, float* d, float* profile, uint8_t* stored, int32_t nthreads);
+
+/* AbstractTrajectory::isSafe for num_traj trajectories [num_traj][num_points][dofs]
+ * leaving q_current (profile, nearest points [M][L][6]); pairs = the
+ * self-collision pairs to test at every point (npairs 0: none). */
+void orc_traj_safe32(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* points,
+                     int64_t num_traj, int32_t num_points, const float* q_current, const float* profile_current,
+                     const float* nearest_current, const float* max_vel, float t_iter0, float time_step,
+                     const int32_t* pairs, int32_t npairs, uint8_t* safe, int32_t nthreads);
+
 /* Capsule part of xArm6::checkSelfCollision(q1, q2) for n edges: collide 1
  * (q_out = last state before the hit), 0 (q_out = q2) or 255 (walk capped). */
 void orc_edge_self_collision32(const orc_robot* rb, const orc_params* p, const float* q1, const float* q2,

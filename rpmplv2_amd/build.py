@@ -27,7 +27,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librvs.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["rvs_kernels.hip", "rvs_edges.hip", "rvs_fast.hip", "rvs_spines.hip", "rvs_capi.cpp", "rvs_server.cpp"]
+SOURCES = ["rvs_kernels.hip", "rvs_edges.hip", "rvs_fast.hip", "rvs_spines.hip", "rvs_traj.hip", "rvs_capi.cpp", "rvs_server.cpp"]
 # the FP32 bulk pass is the one translation unit that allows FMA contraction
 EXTRA = {"rvs_fast.hip": ["-ffp-contract=fast"]}
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

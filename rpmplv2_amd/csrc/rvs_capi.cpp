@@ -798,6 +798,122 @@ rvs_status rvs_distance_underestimation(rvs_context* c, int32_t robot, int32_t e
     return RVS_OK;
 }
 
+rvs_status rvs_distance_underestimation_moving(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n,
+                                               const float* nearest_ref, int64_t ref_stride, const float* max_vel,
+                                               float delta_t, float* d, float* profile, uint8_t* stored,
+                                               uint32_t flags) {
+    rvs_status s = check_query(c, robot, env, q, n, true);
+    if (s != RVS_OK) return s;
+    if (n > 0 && (!nearest_ref || !d || !max_vel))
+        return fail(c, RVS_EINVAL, "rvs_distance_underestimation_moving: null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
+    const int64_t set = (int64_t)M * L * 6;
+    if (ref_stride != 0 && ref_stride != set)
+        return fail(c, RVS_EINVAL, "rvs_distance_underestimation_moving: ref_stride");
+    if (n == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const QueryParams qp = params_of(c);
+    hipError_t e = hipSuccess;
+    const float* qd = q;
+    const float* npd = nearest_ref;
+    const float* vd = max_vel;
+    float *dd = d, *pd = profile;
+    uint8_t* sd = stored;
+    if (!(flags & RVS_F_DEVICE)) {
+        const size_t nref = ref_stride ? (size_t)n * set : (size_t)set;
+        e = c->arena.ensure(Carve::need({(size_t)n * D * 4, nref * 4, (size_t)M * 4, (size_t)n * 4,
+                                         profile ? (size_t)n * L * 4 : 0, (size_t)n}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_underestimation_moving: staging");
+        Carve cv(c->arena.p);
+        float* qs = cv.take<float>((size_t)n * D);
+        float* ns = cv.take<float>(nref);
+        float* vs = cv.take<float>(M);
+        dd = cv.take<float>(n);
+        pd = profile ? cv.take<float>((size_t)n * L) : nullptr;
+        sd = cv.take<uint8_t>(n);
+        e = h2d(qs, q, (size_t)n * D * 4, c->stream);
+        if (e == hipSuccess) e = h2d(ns, nearest_ref, nref * 4, c->stream);
+        if (e == hipSuccess) e = h2d(vs, max_vel, (size_t)M * 4, c->stream);
+        qd = qs;
+        npd = ns;
+        vd = vs;
+    }
+    if (e == hipSuccess)
+        e = launch_underestimation_moving(rb.dev, ev.dev, qp, qd, n, npd, ref_stride, vd, delta_t, dd, pd, sd,
+                                          c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
+        e = d2h(d, dd, (size_t)n * 4, c->stream);
+        if (e == hipSuccess) e = d2h(profile, pd, (size_t)n * L * 4, c->stream);
+        if (e == hipSuccess) e = d2h(stored, sd, (size_t)n, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_underestimation_moving");
+    return RVS_OK;
+}
+
+rvs_status rvs_trajectory_is_safe(rvs_context* c, int32_t robot, int32_t env, const float* points, int64_t num_traj,
+                                  int32_t num_points, const float* q_current, const float* profile_current,
+                                  const float* nearest_current, const float* max_vel, float t_iter, float time_step,
+                                  int32_t self_collision, uint8_t* safe, uint32_t flags) {
+    rvs_status s = check_query(c, robot, env, points, num_traj, true);
+    if (s != RVS_OK) return s;
+    if (num_points < 0) return fail(c, RVS_EINVAL, "rvs_trajectory_is_safe: negative point count");
+    if (num_traj > 0 && (!q_current || !profile_current || !nearest_current || !max_vel || !safe))
+        return fail(c, RVS_EINVAL, "rvs_trajectory_is_safe: null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    if (rb.num_links != rb.num_dofs)
+        return fail(c, RVS_ENOTSUP, "rvs_trajectory_is_safe: needs one link per revolute joint");
+    if (num_traj == 0) return RVS_OK;
+    (void)hipSetDevice(c->opt.device);
+    const QueryParams qp = params_of(c);
+    const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
+    const size_t pb = (size_t)num_traj * num_points * D * 4, set = (size_t)M * L * 6;
+    TrajArgs a{};
+    a.points = points;
+    a.num_points = num_points;
+    a.num_obj = M;
+    a.q_current = q_current;
+    a.profile_current = profile_current;
+    a.nearest_current = nearest_current;
+    a.max_vel = max_vel;
+    a.t_iter = t_iter;
+    a.time_step = time_step;
+    a.self_collision = self_collision ? 1 : 0;
+    a.safe = safe;
+    hipError_t e = hipSuccess;
+    if (!(flags & RVS_F_DEVICE)) {
+        e = c->arena.ensure(Carve::need({pb, (size_t)D * 4, (size_t)L * 4, set * 4, (size_t)M * 4, (size_t)num_traj}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_trajectory_is_safe: staging");
+        Carve cv(c->arena.p);
+        float* ps = cv.take<float>(pb / 4);
+        float* qs = cv.take<float>(D);
+        float* prs = cv.take<float>(L);
+        float* ns = cv.take<float>(set);
+        float* vs = cv.take<float>(M);
+        a.safe = cv.take<uint8_t>(num_traj);
+        e = h2d(ps, points, pb, c->stream);
+        if (e == hipSuccess) e = h2d(qs, q_current, (size_t)D * 4, c->stream);
+        if (e == hipSuccess) e = h2d(prs, profile_current, (size_t)L * 4, c->stream);
+        if (e == hipSuccess) e = h2d(ns, nearest_current, set * 4, c->stream);
+        if (e == hipSuccess) e = h2d(vs, max_vel, (size_t)M * 4, c->stream);
+        a.points = ps;
+        a.q_current = qs;
+        a.profile_current = prs;
+        a.nearest_current = ns;
+        a.max_vel = vs;
+    }
+    if (e == hipSuccess) e = launch_traj_safe(rb.dev, qp, a, num_traj, c->stream);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(safe, a.safe, (size_t)num_traj, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_trajectory_is_safe");
+    return RVS_OK;
+}
+
 rvs_status rvs_self_distance(rvs_context* c, int32_t robot, const float* q, int64_t n, float* dist, uint32_t flags) {
     rvs_status s = check_query(c, robot, -1, q, n, false);
     if (s != RVS_OK) return s;

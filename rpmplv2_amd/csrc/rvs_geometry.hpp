@@ -566,4 +566,25 @@ __host__ __device__ inline T plane_term(const Vec3<T>& P0, const Vec3<T>& P1, co
     return smin(dot(P0 - O, n), dot(P1 - O, n)) - radius;
 }
 
+// The same term after the plane has moved towards the robot for delta_t at
+// the obstacle's maximal speed (AbstractTrajectory::
+// computeDistanceUnderestimation, AbstractTrajectory.cpp:295-306): both
+// points shift by normalize(R - O) * max_vel * delta_t, then the plane is
+// rebuilt from the shifted points.
+template <typename T>
+__host__ __device__ inline T plane_term_moving(const Vec3<T>& P0, const Vec3<T>& P1, const Vec3<T>& R, const Vec3<T>& O,
+                                               T radius, T max_vel, T delta_t) {
+    const Vec3<T> RO = R - O;
+    const T z = sqnorm(RO);
+    Vec3<T> n = RO;
+    if (z > 0) {
+        const T s = vsqrt(z);
+        n = Vec3<T>{RO.x / s, RO.y / s, RO.z / s};
+    }
+    const Vec3<T> dl{n.x * max_vel * delta_t, n.y * max_vel * delta_t, n.z * max_vel * delta_t};
+    const Vec3<T> R2{R.x + dl.x, R.y + dl.y, R.z + dl.z};
+    const Vec3<T> O2{O.x + dl.x, O.y + dl.y, O.z + dl.z};
+    return plane_term(P0, P1, R2, O2, radius);
+}
+
 }  // namespace rvs

@@ -101,6 +101,11 @@ hipError_t launch_is_valid(const RobotDev& rb, const EnvDev& env, const QueryPar
 hipError_t launch_underestimation(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                   int64_t n, const float* np_ref, int64_t np_stride, float* d, float* profile,
                                   uint8_t* stored, hipStream_t st);
+// The same with the planes moved towards the robot for delta_t at max_vel[j]
+// of each obstacle (AbstractTrajectory::computeDistanceUnderestimation).
+hipError_t launch_underestimation_moving(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                         int64_t n, const float* np_ref, int64_t np_stride, const float* max_vel,
+                                         float delta_t, float* d, float* profile, uint8_t* stored, hipStream_t st);
 // Underestimation for a served batch: configuration c takes nearest-point
 // set r of np_sets ([nseg][M][L][6]) where seg_off[r] <= c < seg_off[r + 1].
 hipError_t launch_underestimation_seg(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
@@ -154,6 +159,24 @@ hipError_t launch_spines(const RobotDev& rb, const SpineArgs& a, int64_t n, hipS
 hipError_t launch_gen_layer(int64_t n, int layer, const float* d_root, const float* d_layer, const int32_t* status,
                             uint8_t* active, float* d_out, float d_crit, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t st);
+
+// A batch of candidate trajectories leaving one current state (device
+// pointers): points [T][num_points][n]; the current state's coordinates,
+// profile [L] and nearest points [M][L][6]; max_vel [M].
+struct TrajArgs {
+    const float* points;
+    int32_t num_points;
+    int32_t num_obj;
+    const float* q_current;
+    const float* profile_current;
+    const float* nearest_current;
+    const float* max_vel;
+    float t_iter;
+    float time_step;
+    int32_t self_collision;
+    uint8_t* safe;
+};
+hipError_t launch_traj_safe(const RobotDev& rb, const QueryParams& qp, const TrajArgs& a, int64_t T, hipStream_t st);
 // Capsule part of xArm6::checkSelfCollision(q1, q2) per edge: collide[e] =
 // 1 (q_out = last state before the hit), 0 (q_out = q2) or 255 (the walk
 // would not end: non-finite or huge coordinates).

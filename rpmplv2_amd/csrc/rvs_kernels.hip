@@ -284,12 +284,15 @@ __global__ __launch_bounds__(kBlock) void k_valid(RobotDev rb, EnvDev env, Query
 // configuration (stride np_stride floats; 0 = one set for all).
 // With seg_off (nseg + 1 non-decreasing offsets), configuration c uses the
 // nearest-point set of its segment r (seg_off[r] <= c < seg_off[r + 1]):
-// requests of a served batch each bring their own set.
+// requests of a served batch each bring their own set.  With max_vel [M]
+// the planes first move towards the robot for delta_t at each obstacle's
+// maximal speed (AbstractTrajectory.cpp:272-320).
 __global__ __launch_bounds__(kBlock) void k_under(RobotDev rb, EnvDev env, QueryParams qp,
                                                   const float* __restrict__ q, int64_t n,
                                                   const float* __restrict__ np_ref, int64_t np_stride, float* d_out,
                                                   float* prof_out, uint8_t* stored,
-                                                  const int64_t* __restrict__ seg_off, int nseg) {
+                                                  const int64_t* __restrict__ seg_off, int nseg,
+                                                  const float* __restrict__ max_vel, float delta_t) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const int M = env.num_obj;
@@ -320,7 +323,8 @@ __global__ __launch_bounds__(kBlock) void k_under(RobotDev rb, EnvDev env, Query
             const Vec3<float> O{e[3], e[4], e[5]};
             if (vnorm(O) == maxd) continue;
             const Vec3<float> R{e[0], e[1], e[2]};
-            const float t = plane_term(P0, P1, R, O, r);
+            const float t = max_vel ? plane_term_moving(P0, P1, R, O, r, max_vel[j], delta_t)
+                                    : plane_term(P0, P1, R, O, r);
             if (t < 0) { neg = true; break; }
             prof = smin(prof, t);
         }
@@ -511,7 +515,16 @@ hipError_t launch_underestimation(const RobotDev& rb, const EnvDev& env, const Q
                                   uint8_t* stored, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_under, grid_for(n, kBlock), dim3(kBlock), 0, st, rb, env, qp, q, n, np_ref, np_stride, d,
-                       profile, stored, nullptr, 0);
+                       profile, stored, nullptr, 0, nullptr, 0.0f);
+    return hipGetLastError();
+}
+
+hipError_t launch_underestimation_moving(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                         int64_t n, const float* np_ref, int64_t np_stride, const float* max_vel,
+                                         float delta_t, float* d, float* profile, uint8_t* stored, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_under, grid_for(n, kBlock), dim3(kBlock), 0, st, rb, env, qp, q, n, np_ref, np_stride, d,
+                       profile, stored, nullptr, 0, max_vel, delta_t);
     return hipGetLastError();
 }
 
@@ -520,7 +533,7 @@ hipError_t launch_underestimation_seg(const RobotDev& rb, const EnvDev& env, con
                                       float* profile, uint8_t* stored, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_under, grid_for(n, kBlock), dim3(kBlock), 0, st, rb, env, qp, q, n, np_sets, (int64_t)0, d,
-                       profile, stored, seg_off, nseg);
+                       profile, stored, seg_off, nseg, nullptr, 0.0f);
     return hipGetLastError();
 }
 

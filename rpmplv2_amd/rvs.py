@@ -96,7 +96,8 @@ RVS_EBUSY = 6
 
 EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error", "rvs_stream", "rvs_last_refined",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds", "rvs_distance",
-           "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_self_distance",
+           "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_distance_underestimation_moving",
+           "rvs_trajectory_is_safe", "rvs_self_distance",
            "rvs_self_collision", "rvs_enclosing_radii", "rvs_extend_spines", "rvs_extend_gen_spines",
            "rvs_edge_self_collision", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
            "rvs_autotune", "rvs_set_tuning", "rvs_get_tuning", "rvs_autotune_edges", "rvs_set_edge_tuning", "rvs_server_default_options", "rvs_server_create", "rvs_server_destroy", "rvs_server_set_motion",
@@ -144,6 +145,10 @@ def lib():
         L.rvs_distance_f64.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, u32]
         L.rvs_is_valid.argtypes = [ctx, i32, i32, vp, i64, vp, u32]
         L.rvs_distance_underestimation.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, vp, vp, u32]
+        L.rvs_distance_underestimation_moving.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, C.c_float, vp, vp, vp,
+                                                          u32]
+        L.rvs_trajectory_is_safe.argtypes = [ctx, i32, i32, vp, i64, i32, vp, vp, vp, vp, C.c_float, C.c_float, i32,
+                                             vp, u32]
         L.rvs_self_distance.argtypes = [ctx, i32, vp, i64, vp, u32]
         L.rvs_self_collision.argtypes = [ctx, i32, vp, i64, vp, vp, u32]
         L.rvs_enclosing_radii.argtypes = [ctx, i32, vp, i64, vp, u32]
@@ -365,6 +370,37 @@ class Engine:
         out = np.zeros((len(q), D, D + 1), np.float32)
         self._check(lib().rvs_enclosing_radii(self._ctx, robot, _ptr(q), len(q), _ptr(out), 0), "rvs_enclosing_radii")
         return out
+
+    def underestimation_moving(self, robot: int, env: int, q, nearest_ref, max_vel, delta_t: float):
+        """AbstractTrajectory::computeDistanceUnderestimation with planes
+        moved for delta_t: (d, profile, stored)."""
+        rb = self.robots[robot]
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
+        n, L = len(q), rb.num_links
+        npr = np.ascontiguousarray(nearest_ref, np.float32)
+        stride = 0 if npr.ndim == 3 else npr[0].size
+        mv = np.ascontiguousarray(max_vel, np.float32)
+        d = np.zeros(n, np.float32)
+        prof = np.zeros((n, L), np.float32)
+        st = np.zeros(n, np.uint8)
+        self._check(lib().rvs_distance_underestimation_moving(self._ctx, robot, env, _ptr(q), n, _ptr(npr), stride,
+                                                              _ptr(mv), float(delta_t), _ptr(d), _ptr(prof), _ptr(st),
+                                                              0), "rvs_distance_underestimation_moving")
+        return d, prof, st
+
+    def trajectory_is_safe(self, robot: int, env: int, points, q_current, profile_current, nearest_current, max_vel,
+                           t_iter: float = 0.0, time_step: float = 0.01, self_collision: bool = False):
+        """AbstractTrajectory::isSafe for trajectories points (T, K, dofs)
+        leaving q_current: safe (T,) uint8."""
+        rb = self.robots[robot]
+        pts = np.ascontiguousarray(points, np.float32)
+        T, K = pts.shape[0], pts.shape[1]
+        args = [np.ascontiguousarray(x, np.float32) for x in (q_current, profile_current, nearest_current, max_vel)]
+        safe = np.zeros(T, np.uint8)
+        self._check(lib().rvs_trajectory_is_safe(self._ctx, robot, env, _ptr(pts), T, K, *[_ptr(x) for x in args],
+                                                 float(t_iter), float(time_step), 1 if self_collision else 0,
+                                                 _ptr(safe), 0), "rvs_trajectory_is_safe")
+        return safe
 
     def edge_self_collision(self, robot: int, q1, q2, eps_step: float = 0.1):
         """Capsule part of xArm6::checkSelfCollision(q1, q2) per edge:

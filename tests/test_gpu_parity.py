@@ -364,3 +364,60 @@ def test_edge_self_collision(oracle, engine, eps_step):
     assert (col == ocol).mean() >= 0.999
     assert same.mean() >= 0.995
     assert col.sum() > 100
+
+
+def _boxes_only(scene):
+    keep = scene.is_ground == 0
+    return W.Scene(scene.dim[keep], scene.pos[keep], scene.quat[keep], scene.is_ground[keep],
+                   scene.min_dist_tol[keep])
+
+
+def test_underestimation_moving(oracle, engine):
+    """Moving-plane underestimation against the oracle, bit for bit."""
+    w = W.workload2(1 << 15)
+    sc = oracle.scene_of(w.scene)
+    root = np.zeros(6, np.float32)
+    _, _, _, np0 = oracle.distance(w.robot, sc, root[None], 32, want_np=True)
+    rng = np.random.default_rng(81)
+    q = (root + rng.uniform(-0.6, 0.6, (1 << 15, 6))).astype(np.float32)
+    mv = rng.uniform(0, 0.5, w.scene.num_obj).astype(np.float32)
+    mv[w.scene.is_ground == 1] = 0
+    rid = engine.load_robot(w.robot)
+    eid = engine.create_env(w.scene)
+    d, prof, st = engine.underestimation_moving(rid, eid, q, np0[0], mv, 0.07)
+    od, oprof, ost = oracle.underestimation_moving(w.robot, sc, q, np0[0], mv, 0.07)
+    print(f"stored {st.mean():.3f}")
+    assert (st == ost).all()
+    assert ulps(d, od).max() <= 2 and (d == od).mean() > 0.999
+    ok = st == 1
+    assert ulps(prof[ok], oprof[ok]).max() <= 2
+
+
+@pytest.mark.parametrize("self_collision", [False, True])
+@pytest.mark.parametrize("v", [0.0, 0.1])
+def test_trajectory_is_safe(oracle, engine, v, self_collision):
+    """AbstractTrajectory::isSafe over 8192 straight candidate trajectories
+    of 60 points from one current state, against the oracle."""
+    w = W.workload2(64)
+    scene = _boxes_only(w.scene)
+    sc = oracle.scene_of(scene)
+    rb = w.robot
+    root = np.zeros(6, np.float32)
+    _, p0, _, np0 = oracle.distance(rb, sc, root[None], 32, want_np=True)
+    rng = np.random.default_rng(82)
+    T, K = 8192, 60
+    u = rng.normal(size=(T, 6))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    reach = rng.uniform(0.1, 2.5, T)
+    s = np.linspace(0, 1, K + 1)[1:]
+    pts = (root[None, None] + (reach[:, None, None] * s[None, :, None]) * u[:, None, :]).astype(np.float32)
+    mv = np.full(scene.num_obj, v, np.float32)
+    rid = engine.load_robot(rb)
+    eid = engine.create_env(scene)
+    safe = engine.trajectory_is_safe(rid, eid, pts, root, p0[0], np0[0], mv, t_iter=0.0, time_step=0.01,
+                                     self_collision=self_collision)
+    osafe = oracle.traj_safe(rb, sc, pts, root, p0[0], np0[0], mv, 0.0, 0.01,
+                             pairs=rb.self_pairs if self_collision else None)
+    print(f"v={v} self={self_collision}: safe {safe.sum()} of {T}, mismatches {(safe != osafe).sum()}")
+    assert (safe == osafe).mean() >= 0.999
+    assert 0 < safe.sum() < T

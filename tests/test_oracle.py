@@ -871,3 +871,114 @@ def test_edge_self_collision_bounded_steps_are_conservative(oracle):
             dq = (q2[k] - q1[k]).astype(np.float64)
             t = float(np.dot(out[k] - q1[k], dq) / np.dot(dq, dq))
             assert t <= s[hit[0]] + 1e-3, (k, t, s[hit[0]])
+
+
+# ---------------------------------------------------------------- moving planes, trajectory safety
+def _boxes_only(scene):
+    keep = scene.is_ground == 0
+    return W.Scene(scene.dim[keep], scene.pos[keep], scene.quat[keep], scene.is_ground[keep],
+                   np.full(int(keep.sum()), W.MAX_DISTANCE, np.float32))
+
+
+def test_underestimation_moving_planes_shift_by_speed_times_time(oracle):
+    """AbstractTrajectory::computeDistanceUnderestimation (:272-320): each
+    plane moves along its normal by max_vel * delta_t, so every link's term
+    drops by exactly that amount (checked per obstacle on a one-box scene,
+    within float rounding); zero speed or zero time gives the static
+    underestimation bit for bit."""
+    w = W.workload2(64)
+    rb = w.robot
+    full = _boxes_only(w.scene)
+    sc = oracle.scene_of(full)
+    root = np.zeros(6, np.float32)
+    _, _, _, np0 = oracle.distance(rb, sc, root[None], 32, want_np=True)
+    rng = np.random.default_rng(71)
+    q = (root + rng.uniform(-0.3, 0.3, (512, 6))).astype(np.float32)
+    d_s, p_s, st_s = oracle.underestimation(rb, sc, q, np0[0])
+    for v, dt in ((0.0, 0.5), (0.7, 0.0)):
+        d_m, p_m, st_m = oracle.underestimation_moving(rb, sc, q, np0[0], np.full(full.num_obj, v), dt)
+        assert (d_m == d_s).all() and (p_m == p_s).all() and (st_m == st_s).all()
+    for j in range(3):
+        one = W.Scene(full.dim[j:j + 1], full.pos[j:j + 1], full.quat[j:j + 1], full.is_ground[j:j + 1],
+                      full.min_dist_tol[j:j + 1])
+        s1 = oracle.scene_of(one)
+        _, _, _, np1 = oracle.distance(rb, s1, root[None], 32, want_np=True)
+        d_s, p_s, st_s = oracle.underestimation(rb, s1, q, np1[0])
+        d_m, p_m, st_m = oracle.underestimation_moving(rb, s1, q, np1[0], [0.4], 0.05)
+        ok = (st_s == 1) & (st_m == 1)
+        assert ok.sum() > 100
+        assert np.abs(p_m[ok] - (p_s[ok] - 0.02)).max() <= 2e-6
+        assert ((st_m == 0) | (st_s == 1)).all()     # moving planes only get closer
+
+
+def _traj_case(oracle, T, K, seed, v):
+    w = W.workload2(64)
+    rb = w.robot
+    scene = _boxes_only(w.scene)
+    sc = oracle.scene_of(scene)
+    root = np.zeros(6, np.float32)
+    d0, p0, _, np0 = oracle.distance(rb, sc, root[None], 32, want_np=True)
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(T, 6))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    reach = rng.uniform(0.2, 2.0, T)
+    s = np.linspace(0, 1, K + 1)[1:]
+    pts = (root[None, None] + (reach[:, None, None] * s[None, :, None]) * u[:, None, :]).astype(np.float32)
+    max_vel = np.full(scene.num_obj, v, np.float32)
+    return rb, scene, sc, root, p0[0], np0[0], pts, max_vel
+
+
+def test_trajectory_safety_implies_clearance_from_moving_obstacles(oracle):
+    """AbstractTrajectory::isSafe (:212-256): a trajectory reported safe keeps
+    every point i at a distance greater than max_vel * t_i from every
+    obstacle at its starting position, t_i = t_iter + i * time_step (bubble
+    points: robot motion plus obstacle motion stay inside the clearance;
+    re-anchored points: the planes moved by max_vel * t_i stay ahead).
+    Checked with real distances at every point, static and moving cases."""
+    for v in (0.0, 0.1):
+        rb, scene, sc, root, p0, np0, pts, mv = _traj_case(oracle, 300, 60, 72, v)
+        safe = oracle.traj_safe(rb, sc, pts, root, p0, np0, mv, t_iter=0.0, time_step=0.01)
+        assert 30 <= safe.sum() <= 270, safe.sum()
+        t = 0.01 * np.arange(pts.shape[1], dtype=np.float64)
+        d, _, _, _ = oracle.distance(rb, sc, pts.reshape(-1, 6), 32)
+        d = d.reshape(pts.shape[:2])
+        margin = d[safe == 1] - v * t[None]
+        assert margin.min() > -1e-5, margin.min()
+        # a start time later in the iteration only shrinks the set of safe trajectories
+        late = oracle.traj_safe(rb, sc, pts, root, p0, np0, mv, t_iter=0.2, time_step=0.01)
+        if v > 0:
+            assert (late <= safe).all() and late.sum() < safe.sum()
+        else:
+            assert (late == safe).all()
+
+
+def test_trajectory_self_collision_points_are_unsafe(oracle):
+    """With self-collision checking, a trajectory through a self-colliding
+    state (capsule pairs) is unsafe; without it that alone does not matter."""
+    rb = R.xarm6(radius=0.05)
+    w = W.workload2(64)
+    scene = _boxes_only(w.scene)
+    sc = oracle.scene_of(scene)
+    root = np.zeros(6, np.float32)
+    _, p0, _, np0 = oracle.distance(rb, sc, root[None], 32, want_np=True)
+    rng = np.random.default_rng(73)
+    lim = np.asarray(rb.limits, np.float64)
+    qb = (lim[:, 0] + rng.random((4000, 6)) * (lim[:, 1] - lim[:, 0])).astype(np.float32)
+    bad = qb[_xarm_pairs_min(oracle, rb, qb) < -1e-3][:16]
+    pts = np.repeat(bad[:, None, :], 3, axis=1)
+    mv = np.zeros(scene.num_obj, np.float32)
+    s1 = oracle.traj_safe(rb, sc, pts, root, p0[0], np0[0], mv, pairs=rb.self_pairs)
+    assert (s1 == 0).all()
+
+
+def test_trajectory_safety_counts_obstacle_motion_while_the_robot_waits(oracle):
+    """A robot that stays at q_current is safe exactly while the obstacles
+    cannot have closed its clearance: with 60 points at 0.01 s, 0.1 m/s
+    obstacles stay 0.059 m short of the 0.1136 m clearance, 0.3 m/s ones do
+    not (rho_obs = max_obs_vel * elapsed time, :230-236)."""
+    rb, scene, sc, root, p0, np0, _, _ = _traj_case(oracle, 1, 60, 74, 0.0)
+    pts = np.repeat(root[None, None], 60, axis=1)
+    assert abs(p0.min() - 0.1136) < 1e-3
+    for v, expect in ((0.1, 1), (0.3, 0)):
+        safe = oracle.traj_safe(rb, sc, pts, root, p0, np0, np.full(scene.num_obj, v, np.float32))
+        assert safe[0] == expect, v
