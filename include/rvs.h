@@ -139,11 +139,36 @@ enum {
 rvs_status rvs_env_create(rvs_context* ctx, const rvs_box* boxes, int32_t num_boxes, int32_t semantics,
                           int32_t* env_id);
 /* In-place move of boxes idx[k] to pos[k] (Object::setPosition,
- * src/environments/Object.cpp:21-26); bounds are recomputed on the GPU. */
+ * src/environments/Object.cpp:21-26); the device bounds tables are
+ * rewritten. */
 rvs_status rvs_env_update(rvs_context* ctx, int32_t env_id, const int32_t* idx, const float* pos, int32_t count,
                           uint32_t flags);
 /* Axis-aligned bounds [num][6] the queries use (host memory). */
 rvs_status rvs_env_bounds(rvs_context* ctx, int32_t env_id, float* aabb, int32_t num);
+
+/* Motion laws of Environment::updateEnvironment (src/environments/
+ * Environment.cpp:194-377).  random_directions draws from std::rand
+ * (Eigen's Random) and has no counterpart here. */
+enum { RVS_MOTION_STRAIGHT = 0, RVS_MOTION_CIRCULAR = 1, RVS_MOTION_TWO_TUNNELS = 2, RVS_MOTION_LIGHT_DIRECTIONS = 3 };
+
+typedef struct rvs_env_motion {
+    int32_t motion_type;     /* RVS_MOTION_* (scenario "testing: motion_type") */
+    float ws_center[3];      /* Environment::getWSCenter (scenario robot WS_center) */
+    float ws_radius;         /* getWSRadius */
+    float base_radius;       /* getBaseRadius */
+    float robot_max_vel;     /* getRobotMaxVel */
+} rvs_env_motion;
+
+/* Sets the motion law of an environment with each box's velocity vel [M][3]
+ * (Object::getVelocity) and maximal speed max_vel [M] (Object::getMaxVel),
+ * host arrays.  Boxes flagged RVS_BOX_DYNAMIC move; the two-tunnels state
+ * starts at sign 1, path length 0 (Environment.cpp:82-83). */
+rvs_status rvs_env_set_motion_law(rvs_context* ctx, int32_t env_id, const rvs_env_motion* law, const float* vel,
+                                  const float* max_vel);
+/* Environment::updateEnvironment(delta_time), 'steps' times, on the GPU;
+ * positions, velocities and every bound the queries read are updated in
+ * place.  pos and vel ([M][3], host, may be NULL) receive the new state. */
+rvs_status rvs_env_step(rvs_context* ctx, int32_t env_id, float delta_time, int32_t steps, float* pos, float* vel);
 
 /* ---------------------------------------------------------------- queries */
 /* RealVectorSpace::computeDistance(q, compute_again = true)

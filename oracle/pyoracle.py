@@ -101,6 +101,8 @@ def _declare(L):
                                                 f32p, u8p, C.c_int32]),
         "orc_traj_safe32": (None, [R, Pp, S, f32p, C.c_int64, C.c_int32, f32p, f32p, f32p, f32p, C.c_float,
                                    C.c_float, i32p, C.c_int32, u8p, C.c_int32]),
+        "orc_env_step32": (None, [C.c_int32, f32p, C.c_float, C.c_float, C.c_float, C.c_int32, f32p, f32p, f32p,
+                                  u8p, f32p, i32p, C.c_float, C.c_int32]),
         "orc_edge_self_collision32": (None, [R, Pp, f32p, f32p, C.c_int64, C.c_float, u8p, f32p, C.c_int32]),
         "orc_extend_spines32": (None, [R, f32p, f32p, f32p, f32p, C.c_int64, C.c_int32, C.c_int32, f32p, i32p,
                                        C.c_int32]),
@@ -461,4 +463,29 @@ def traj_safe(rb, scene: Scene, points, q_current, profile_current, nearest_curr
                           *[_ptr(x, C.c_float) for x in args], _ptr(mv, C.c_float), float(t_iter), float(time_step),
                           _ptr(pr, C.c_int32), len(pr), _ptr(out, C.c_uint8), nthreads)
     return out
+
+
+class MotionState:
+    """Positions, velocities and two-tunnels state of a scene's boxes under
+    Environment::updateEnvironment (Environment.cpp:194-377)."""
+
+    def __init__(self, law, pos, vel, max_vel, dynamic, ws_center=(0, 0, 0), ws_radius=1.0, base_radius=0.1,
+                 robot_max_vel=1.0):
+        self.law = int(law)
+        self.pos = np.array(pos, np.float32).reshape(-1, 3)
+        self.vel = np.array(vel, np.float32).reshape(-1, 3)
+        M = len(self.pos)
+        self.max_vel = np.ascontiguousarray(max_vel, np.float32).reshape(M)
+        self.dynamic = np.ascontiguousarray(dynamic, np.uint8).reshape(M)
+        self.path_len = np.zeros(M, np.float32)
+        self.sign = np.ones(M, np.int32)
+        self.ws = np.ascontiguousarray(ws_center, np.float32).reshape(3)
+        self.ws_radius, self.base_radius, self.robot_max_vel = ws_radius, base_radius, robot_max_vel
+
+    def step(self, dt, steps=1):
+        lib().orc_env_step32(self.law, _ptr(self.ws, C.c_float), self.ws_radius, self.base_radius,
+                             self.robot_max_vel, len(self.pos), _ptr(self.pos, C.c_float), _ptr(self.vel, C.c_float),
+                             _ptr(self.max_vel, C.c_float), _ptr(self.dynamic, C.c_uint8),
+                             _ptr(self.path_len, C.c_float), _ptr(self.sign, C.c_int32), float(dt), int(steps))
+        return self.pos, self.vel
 

@@ -1165,6 +1165,61 @@ void orc_traj_safe32(const orc_robot* rb, const orc_params* p, const orc_scene* 
     });
 }
 
+// Environment::updateEnvironment(delta_time), Environment.cpp:194-377, for
+// the deterministic laws (0 straight :225-238, 1 circular :241-257, 2 two
+// tunnels :260-287, 3 light directions :330-376), 'steps' times, in float as
+// fcl::Vector3f computes (fixed-size norms and dots x + (y + z)).  pos, vel
+// [M][3], path_len [M], sign [M] are updated in place; boxes with
+// dynamic[j] = 0 stay.
+void orc_env_step32(int32_t law, const float ws[3], float ws_radius, float base_radius, float robot_max_vel,
+                    int32_t M, float* pos, float* vel, const float* max_vel, const uint8_t* dynamic,
+                    float* path_len, int32_t* sign, float dt, int32_t steps) {
+    auto norm3v = [](const float* a) { return std::sqrt(a[0] * a[0] + (a[1] * a[1] + a[2] * a[2])); };
+    for (int j = 0; j < M; ++j) {
+        if (!dynamic[j]) continue;
+        float* p = pos + 3 * j;
+        float* v = vel + 3 * j;
+        const float vmax = max_vel[j];
+        for (int s = 0; s < steps; ++s) {
+            if (law == 0) {
+                p[0] -= dt * vmax;
+            } else if (law == 1) {
+                const float phi = std::atan2(p[1], p[0]);
+                const float dir[3] = {-std::sin(phi), std::cos(phi), 0.0f};
+                for (int k = 0; k < 3; ++k) v[k] = dir[k] * vmax;
+                for (int k = 0; k < 3; ++k) p[k] += v[k] * dt;
+            } else if (law == 2) {
+                if (path_len[j] > 0.2f) {
+                    sign[j] *= -1;
+                    path_len[j] = -0.2f;
+                }
+                const float unit[3] = {p[1] > 0 ? 1.0f : 0.0f, p[1] > 0 ? 0.0f : 1.0f, 0.0f};
+                for (int k = 0; k < 3; ++k) v[k] = unit[k] * (float)sign[j] * vmax;
+                path_len[j] += norm3v(v) * dt;
+                for (int k = 0; k < 3; ++k) p[k] += v[k] * dt;
+            } else {
+                const float tol = std::max(norm3v(v) / robot_max_vel, base_radius);
+                float pn[3], n[3], rel[3];
+                for (int k = 0; k < 3; ++k) pn[k] = p[k] + v[k] * dt;
+                for (int k = 0; k < 3; ++k) rel[k] = pn[k] - ws[k];
+                bool change = true;
+                if (pn[2] < 0) { n[0] = 0; n[1] = 0; n[2] = 1; }
+                else if (norm3v(rel) > ws_radius) { n[0] = -pn[0]; n[1] = -pn[1]; n[2] = -(pn[2] - ws[2]); }
+                else if (std::sqrt(pn[0] * pn[0] + pn[1] * pn[1]) < tol && pn[2] < ws[2]) { n[0] = pn[0]; n[1] = pn[1]; n[2] = 0; }
+                else if (norm3v(rel) < tol) { n[0] = pn[0]; n[1] = pn[1]; n[2] = pn[2] - ws[2]; }
+                else { for (int k = 0; k < 3; ++k) p[k] = pn[k]; change = false; }
+                if (change) {
+                    float dlt[3];
+                    for (int k = 0; k < 3; ++k) dlt[k] = pn[k] - p[k];
+                    const float t = dot3(dlt, n) / sqn3(n);
+                    for (int k = 0; k < 3; ++k) p[k] = 2 * pn[k] - p[k] - 2 * t * n[k];
+                    for (int k = 0; k < 3; ++k) v[k] = (p[k] - pn[k]) / dt;
+                }
+            }
+        }
+    }
+}
+
 void orc_self_distance32(const orc_robot* rb, const orc_params* p, const float* q, int64_t n,
                          const int32_t* pairs, int32_t npairs, float* dist, int32_t nthreads) {
     P<float> pr(p);

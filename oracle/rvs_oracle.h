@@ -152,6 +152,12 @@ void orc_traj_safe32(const orc_robot* rb, const orc_params* p, const orc_scene* 
                      const float* nearest_current, const float* max_vel, float t_iter0, float time_step,
                      const int32_t* pairs, int32_t npairs, uint8_t* safe, int32_t nthreads);
 
+/* Environment::updateEnvironment(dt) 'steps' times for the deterministic
+ * motion laws (0 straight, 1 circular, 2 two tunnels, 3 light directions). */
+void orc_env_step32(int32_t law, const float ws[3], float ws_radius, float base_radius, float robot_max_vel,
+                    int32_t M, float* pos, float* vel, const float* max_vel, const uint8_t* dynamic,
+                    float* path_len, int32_t* sign, float dt, int32_t steps);
+
 /* Capsule part of xArm6::checkSelfCollision(q1, q2) for n edges: collide 1
  * (q_out = last state before the hit), 0 (q_out = q2) or 255 (walk capped). */
 void orc_edge_self_collision32(const orc_robot* rb, const orc_params* p, const float* q1, const float* q2,

@@ -404,6 +404,95 @@ rvs_status rvs_env_update(rvs_context* c, int32_t env_id, const int32_t* idx, co
     return RVS_OK;
 }
 
+rvs_status rvs_env_set_motion_law(rvs_context* c, int32_t env_id, const rvs_env_motion* law, const float* vel,
+                                  const float* max_vel) {
+    if (!c || !law || !vel || !max_vel) return fail(c, RVS_EINVAL, "rvs_env_set_motion_law: null argument");
+    if (law->motion_type < RVS_MOTION_STRAIGHT || law->motion_type > RVS_MOTION_LIGHT_DIRECTIONS)
+        return fail(c, RVS_EINVAL, "rvs_env_set_motion_law: unknown motion type");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!env_ok(c, env_id)) return fail(c, RVS_EINVAL, "rvs_env_set_motion_law: bad env id");
+    EnvRec& r = c->envs[env_id];
+    const int M = (int)r.boxes.size();
+    (void)hipSetDevice(c->opt.device);
+    std::vector<float> pos(3 * M), v(vel, vel + 3 * (size_t)M), mv(max_vel, max_vel + M), half(3 * M), rad(M),
+        ext(3 * M), plen(M, 0.0f);
+    std::vector<int32_t> sign(M, 1);
+    std::vector<uint8_t> dyn(M), ident(M);
+    for (int j = 0; j < M; ++j) {
+        const rvs_box& b = r.boxes[j];
+        for (int k = 0; k < 3; ++k) {
+            pos[3 * j + k] = b.pos[k];
+            half[3 * j + k] = 0.5f * b.dim[k];
+        }
+        const float m0 = -half[3 * j], m1 = -half[3 * j + 1], m2 = -half[3 * j + 2];
+        rad[j] = std::sqrt(m0 * m0 + (m1 * m1 + m2 * m2));
+        ident[j] = box_identity(b) ? 1 : 0;
+        dyn[j] = (b.flags & RVS_BOX_DYNAMIC) ? 1 : 0;
+        obb_extent(&r.obb[15 * (size_t)j], &ext[3 * j]);
+    }
+    EnvStepDev m{};
+    m.law = law->motion_type;
+    m.num_obj = M;
+    for (int k = 0; k < 3; ++k) m.ws[k] = law->ws_center[k];
+    m.ws_radius = law->ws_radius;
+    m.base_radius = law->base_radius;
+    m.robot_max_vel = law->robot_max_vel;
+    hipError_t e = upload(r.allocs, pos.data(), pos.size(), const_cast<const float**>(&m.pos));
+    if (e == hipSuccess) e = upload(r.allocs, v.data(), v.size(), const_cast<const float**>(&m.vel));
+    if (e == hipSuccess) e = upload(r.allocs, plen.data(), plen.size(), const_cast<const float**>(&m.path_len));
+    if (e == hipSuccess) e = upload(r.allocs, sign.data(), sign.size(), const_cast<const int32_t**>(&m.sign));
+    if (e == hipSuccess) e = upload(r.allocs, mv.data(), mv.size(), &m.max_vel);
+    if (e == hipSuccess) e = upload(r.allocs, dyn.data(), dyn.size(), &m.dynamic);
+    if (e == hipSuccess) e = upload(r.allocs, half.data(), half.size(), &m.half);
+    if (e == hipSuccess) e = upload(r.allocs, rad.data(), rad.size(), &m.rad);
+    if (e == hipSuccess) e = upload(r.allocs, ident.data(), ident.size(), &m.ident);
+    if (e == hipSuccess) e = upload(r.allocs, ext.data(), ext.size(), &m.ext);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_env_set_motion_law");
+    r.step = m;
+    r.has_motion = true;
+    return RVS_OK;
+}
+
+rvs_status rvs_env_step(rvs_context* c, int32_t env_id, float delta_time, int32_t steps, float* pos, float* vel) {
+    if (!c) return RVS_EINVAL;
+    if (steps < 0) return fail(c, RVS_EINVAL, "rvs_env_step: negative step count");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!env_ok(c, env_id)) return fail(c, RVS_EINVAL, "rvs_env_step: bad env id");
+    EnvRec& r = c->envs[env_id];
+    if (!r.has_motion) return fail(c, RVS_EINVAL, "rvs_env_step: no motion law set");
+    const int M = (int)r.boxes.size();
+    (void)hipSetDevice(c->opt.device);
+    float* aabb = const_cast<float*>(r.dev.aabb);
+    float* obb = const_cast<float*>(r.dev.obb);
+    float* cull = (r.dev.cull == r.dev.aabb) ? nullptr : const_cast<float*>(r.dev.cull);
+    hipError_t e = launch_env_step(r.step, delta_time, steps, aabb, obb, cull, c->stream);
+    // keep the host copies (bounds, box positions) in step with the device
+    std::vector<float> p(3 * (size_t)M);
+    if (e == hipSuccess) e = hipMemcpyAsync(p.data(), r.step.pos, p.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && vel) e = hipMemcpyAsync(vel, r.step.vel, p.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(r.aabb.data(), r.dev.aabb, r.aabb.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(r.obb.data(), r.dev.obb, r.obb.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && cull)
+        e = hipMemcpyAsync(r.cull.data(), r.dev.cull, r.cull.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_env_step");
+    for (int j = 0; j < M; ++j) {
+        for (int k = 0; k < 3; ++k) r.boxes[j].pos[k] = p[3 * j + k];
+        if (!cull) {   // the host copy of the oriented boxes' bounds
+            float ext[3];
+            obb_extent(&r.obb[15 * (size_t)j], ext);
+            for (int k = 0; k < 3; ++k) {
+                r.cull[6 * j + k] = r.obb[15 * j + k] - ext[k];
+                r.cull[6 * j + 3 + k] = r.obb[15 * j + k] + ext[k];
+            }
+        }
+    }
+    if (pos) std::memcpy(pos, p.data(), p.size() * 4);
+    return RVS_OK;
+}
+
 rvs_status rvs_env_bounds(rvs_context* c, int32_t env_id, float* aabb, int32_t num) {
     if (!c || !aabb) return fail(c, RVS_EINVAL, "rvs_env_bounds: null argument");
     std::lock_guard<std::mutex> lk(c->mu);

@@ -49,6 +49,8 @@ struct EnvRec {
     std::vector<rvs_box> boxes;
     std::vector<float> aabb, obb, cull;   // host copies of the device tables
     int semantics = RVS_ENV_AABB;
+    bool has_motion = false;              // rvs_env_set_motion_law was called
+    EnvStepDev step{};                    // motion-law state on the device
 };
 
 }  // namespace rvs

@@ -129,6 +129,31 @@ struct MotionDev {
 hipError_t launch_env_motion(const MotionDev& m, int64_t tick, float tick_seconds, float* aabb, float* obb,
                              float* cull, hipStream_t st);
 
+// Motion-law state of an environment's boxes (device pointers, [M] or [M][3]).
+struct EnvStepDev {
+    int32_t law;            // RVS_MOTION_*
+    int32_t num_obj;
+    float ws[3];            // workspace centre
+    float ws_radius;
+    float base_radius;
+    float robot_max_vel;
+    float* pos;
+    float* vel;
+    float* path_len;        // two tunnels
+    int32_t* sign;          // two tunnels
+    const float* max_vel;
+    const uint8_t* dynamic; // 1 = label "dynamic_obstacle"
+    const float* half;
+    const float* rad;
+    const uint8_t* ident;
+    const float* ext;
+};
+// Environment::updateEnvironment(delta_time) 'steps' times, then the AABB
+// [M][6], oriented-box centres [M][15] and (when cull is not null) the
+// broad-phase bounds [M][6] at the final positions.
+hipError_t launch_env_step(const EnvStepDev& m, float delta_time, int32_t steps, float* aabb, float* obb, float* cull,
+                           hipStream_t st);
+
 // AbstractRobot::computeEnclosingRadii for N configurations: R [N][n][n+1]
 // (row i = starting skeleton point, column j = final point).
 hipError_t launch_enclosing_radii(const RobotDev& rb, const float* q, int64_t n, float* R, hipStream_t st);

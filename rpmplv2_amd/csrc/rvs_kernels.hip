@@ -444,6 +444,97 @@ __global__ void k_env_motion(MotionDev m, int64_t tick, float tick_seconds, floa
     }
 }
 
+// Environment::updateEnvironment (Environment.cpp:194-377) for the
+// deterministic motion laws, one lane per box, all steps in one launch, in
+// float as fcl::Vector3f computes them (fixed-size norms and dots sum as
+// x + (y + z)); boxes without the dynamic label keep their place.
+//   straight (:225-238): x -= delta_time * max_vel
+//   circular (:241-257): velocity tangent to the circle about the z axis
+//   two tunnels (:260-287): back and forth 0.2 m legs, along x above y = 0
+//     and along y below
+//   light directions (:330-376): mirror reflection off the ground, the
+//     workspace sphere and the keep-out zones around the robot base
+__global__ void k_env_step(EnvStepDev m, float dt, int32_t steps, float* aabb, float* obb, float* cull) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m.num_obj) return;
+    float p[3] = {m.pos[3 * j], m.pos[3 * j + 1], m.pos[3 * j + 2]};
+    float v[3] = {m.vel[3 * j], m.vel[3 * j + 1], m.vel[3 * j + 2]};
+    if (m.dynamic[j]) {
+        const float vmax = m.max_vel[j];
+        float plen = m.path_len[j];
+        int32_t sg = m.sign[j];
+        auto norm3 = [](float x, float y, float z) { return sqrtf(x * x + (y * y + z * z)); };
+        for (int s = 0; s < steps; ++s) {
+            if (m.law == 0) {
+                p[0] = p[0] - dt * vmax;
+            } else if (m.law == 1) {
+                const float phi = atan2f(p[1], p[0]);
+                v[0] = -sinf(phi) * vmax;
+                v[1] = cosf(phi) * vmax;
+                v[2] = 0.0f * vmax;
+                for (int k = 0; k < 3; ++k) p[k] = p[k] + v[k] * dt;
+            } else if (m.law == 2) {
+                if (plen > 0.2f) {
+                    sg = -sg;
+                    plen = -0.2f;
+                }
+                const float sv = (float)sg;
+                const int axis = p[1] > 0.0f ? 0 : 1;
+                for (int k = 0; k < 3; ++k) v[k] = ((k == axis) ? 1.0f : 0.0f) * sv * vmax;
+                plen = plen + norm3(v[0], v[1], v[2]) * dt;
+                for (int k = 0; k < 3; ++k) p[k] = p[k] + v[k] * dt;
+            } else {
+                const float sp = norm3(v[0], v[1], v[2]);
+                const float tol = max_std(sp / m.robot_max_vel, m.base_radius);
+                float pn[3];
+                for (int k = 0; k < 3; ++k) pn[k] = p[k] + v[k] * dt;
+                float n[3];
+                bool change = true;
+                const float dws = norm3(pn[0] - m.ws[0], pn[1] - m.ws[1], pn[2] - m.ws[2]);
+                if (pn[2] < 0.0f) {
+                    n[0] = 0.0f; n[1] = 0.0f; n[2] = 1.0f;
+                } else if (dws > m.ws_radius) {
+                    n[0] = -pn[0]; n[1] = -pn[1]; n[2] = -(pn[2] - m.ws[2]);
+                } else if (sqrtf(pn[0] * pn[0] + pn[1] * pn[1]) < tol && pn[2] < m.ws[2]) {
+                    n[0] = pn[0]; n[1] = pn[1]; n[2] = 0.0f;
+                } else if (dws < tol) {
+                    n[0] = pn[0]; n[1] = pn[1]; n[2] = pn[2] - m.ws[2];
+                } else {
+                    for (int k = 0; k < 3; ++k) p[k] = pn[k];
+                    change = false;
+                }
+                if (change) {
+                    const float dd = (pn[0] - p[0]) * n[0] + ((pn[1] - p[1]) * n[1] + (pn[2] - p[2]) * n[2]);
+                    const float t = dd / (n[0] * n[0] + (n[1] * n[1] + n[2] * n[2]));
+                    for (int k = 0; k < 3; ++k) p[k] = 2.0f * pn[k] - p[k] - 2.0f * t * n[k];
+                    for (int k = 0; k < 3; ++k) v[k] = (p[k] - pn[k]) / dt;
+                }
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            m.pos[3 * j + k] = p[k];
+            m.vel[3 * j + k] = v[k];
+        }
+        m.path_len[j] = plen;
+        m.sign[j] = sg;
+    }
+    const float* hr = m.half + 3 * j;
+    for (int k = 0; k < 3; ++k) {
+        if (m.ident[j]) {
+            aabb[6 * j + k] = -hr[k] + p[k];
+            aabb[6 * j + 3 + k] = hr[k] + p[k];
+        } else {
+            aabb[6 * j + k] = p[k] - m.rad[j];
+            aabb[6 * j + 3 + k] = p[k] + m.rad[j];
+        }
+        obb[15 * j + k] = p[k];
+        if (cull) {
+            cull[6 * j + k] = p[k] - m.ext[3 * j + k];
+            cull[6 * j + 3 + k] = p[k] + m.ext[3 * j + k];
+        }
+    }
+}
+
 inline dim3 grid_for(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
 
 }  // namespace
@@ -546,6 +637,14 @@ hipError_t launch_enclosing_radii(const RobotDev& rb, const float* q, int64_t n,
     while (B > 32 && per * B > 64 * 1024) B /= 2;
     const size_t sm = per * B;
     hipLaunchKernelGGL(k_radii, grid_for(n, B), dim3(B), sm, st, rb, q, n, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_env_step(const EnvStepDev& m, float delta_time, int32_t steps, float* aabb, float* obb, float* cull,
+                           hipStream_t st) {
+    if (m.num_obj <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_env_step, dim3((m.num_obj + 255) / 256), dim3(256), 0, st, m, delta_time, steps, aabb, obb,
+                       cull);
     return hipGetLastError();
 }
 

@@ -60,6 +60,14 @@ class Box(C.Structure):
                 ("flags", C.c_uint32), ("min_dist_tol", C.c_float)]
 
 
+class EnvMotion(C.Structure):
+    _fields_ = [("motion_type", C.c_int32), ("ws_center", C.c_float * 3), ("ws_radius", C.c_float),
+                ("base_radius", C.c_float), ("robot_max_vel", C.c_float)]
+
+
+MOTION_STRAIGHT, MOTION_CIRCULAR, MOTION_TWO_TUNNELS, MOTION_LIGHT_DIRECTIONS = range(4)
+
+
 class ServerOptions(C.Structure):
     _fields_ = [("memory_budget", C.c_int64), ("num_streams", C.c_int32), ("max_batch_configs", C.c_int32),
                 ("tick_seconds", C.c_float), ("precision", C.c_int32), ("launches_per_stream", C.c_int32)]
@@ -95,7 +103,8 @@ REQ_UNDERESTIMATION = 1
 RVS_EBUSY = 6
 
 EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error", "rvs_stream", "rvs_last_refined",
-           "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds", "rvs_distance",
+           "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds",
+           "rvs_env_set_motion_law", "rvs_env_step", "rvs_distance",
            "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_distance_underestimation_moving",
            "rvs_trajectory_is_safe", "rvs_self_distance",
            "rvs_self_collision", "rvs_enclosing_radii", "rvs_extend_spines", "rvs_extend_gen_spines",
@@ -141,6 +150,8 @@ def lib():
         L.rvs_env_create.argtypes = [ctx, C.POINTER(Box), i32, i32, C.POINTER(i32)]
         L.rvs_env_update.argtypes = [ctx, i32, vp, vp, i32, u32]
         L.rvs_env_bounds.argtypes = [ctx, i32, vp, i32]
+        L.rvs_env_set_motion_law.argtypes = [ctx, i32, C.POINTER(EnvMotion), vp, vp]
+        L.rvs_env_step.argtypes = [ctx, i32, C.c_float, i32, vp, vp]
         L.rvs_distance.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, vp, u32]
         L.rvs_distance_f64.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, u32]
         L.rvs_is_valid.argtypes = [ctx, i32, i32, vp, i64, vp, u32]
@@ -224,7 +235,8 @@ def boxes_of(scene: W.Scene):
             arr[j].pos[i] = scene.pos[j, i]
         for i in range(4):
             arr[j].quat_wxyz[i] = scene.quat[j, i]
-        arr[j].flags = BOX_GROUND if scene.is_ground[j] else 0
+        arr[j].flags = (BOX_GROUND if scene.is_ground[j] else 0) | \
+            (BOX_DYNAMIC if scene.dynamic is not None and scene.dynamic[j] else 0)
         arr[j].min_dist_tol = scene.min_dist_tol[j]
     return arr
 
@@ -370,6 +382,24 @@ class Engine:
         out = np.zeros((len(q), D, D + 1), np.float32)
         self._check(lib().rvs_enclosing_radii(self._ctx, robot, _ptr(q), len(q), _ptr(out), 0), "rvs_enclosing_radii")
         return out
+
+    def set_motion_law(self, env: int, motion_type: int, vel, max_vel, ws_center=(0.0, 0.0, 0.0),
+                       ws_radius: float = 1.0, base_radius: float = 0.1, robot_max_vel: float = 1.0):
+        """Motion law of Environment::updateEnvironment for an environment."""
+        m = EnvMotion(motion_type, (C.c_float * 3)(*ws_center), ws_radius, base_radius, robot_max_vel)
+        v = np.ascontiguousarray(vel, np.float32)
+        mv = np.ascontiguousarray(max_vel, np.float32)
+        self._check(lib().rvs_env_set_motion_law(self._ctx, env, C.byref(m), _ptr(v), _ptr(mv)),
+                    "rvs_env_set_motion_law")
+
+    def env_step(self, env: int, delta_time: float, steps: int = 1):
+        """Environment::updateEnvironment(delta_time), steps times: (pos, vel) (M, 3)."""
+        M = self.envs[env].num_obj
+        pos = np.zeros((M, 3), np.float32)
+        vel = np.zeros((M, 3), np.float32)
+        self._check(lib().rvs_env_step(self._ctx, env, float(delta_time), int(steps), _ptr(pos), _ptr(vel)),
+                    "rvs_env_step")
+        return pos, vel
 
     def underestimation_moving(self, robot: int, env: int, q, nearest_ref, max_vel, delta_t: float):
         """AbstractTrajectory::computeDistanceUnderestimation with planes

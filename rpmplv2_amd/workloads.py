@@ -35,6 +35,7 @@ class Scene:
     is_ground: np.ndarray    # (M,) uint8
     min_dist_tol: np.ndarray # (M,) float32
     aabb: np.ndarray = field(init=False)
+    dynamic: Optional[np.ndarray] = None   # (M,) uint8, 1 = label "dynamic_obstacle"
 
     def __post_init__(self):
         self.dim = np.ascontiguousarray(self.dim, np.float32).reshape(-1, 3)

@@ -421,3 +421,49 @@ def test_trajectory_is_safe(oracle, engine, v, self_collision):
     print(f"v={v} self={self_collision}: safe {safe.sum()} of {T}, mismatches {(safe != osafe).sum()}")
     assert (safe == osafe).mean() >= 0.999
     assert 0 < safe.sum() < T
+
+
+@pytest.mark.parametrize("law", [0, 1, 2, 3])
+def test_env_motion_laws(oracle, engine, law):
+    """Environment::updateEnvironment on the GPU against the oracle: 300
+    steps of 10 ms; the bounds the queries read follow the boxes, and a query
+    on the moved environment equals one on a fresh environment built at the
+    new positions."""
+    rng = np.random.default_rng(95 + law)
+    n = 24
+    pos = np.zeros((0, 3))
+    while len(pos) < n:
+        c = np.stack([rng.uniform(-0.8, 0.8, 4 * n), rng.uniform(-0.8, 0.8, 4 * n), rng.uniform(0.05, 1.0, 4 * n)], 1)
+        rr = np.linalg.norm(c - np.array([0, 0, 0.267]), axis=1)
+        c = c[(rr < 0.95) & (rr > 0.35) & (np.hypot(c[:, 0], c[:, 1]) > 0.35)]
+        pos = np.concatenate([pos, c])[:n]
+    dim = rng.uniform(0.05, 0.2, (n, 3))
+    quat = np.tile([1.0, 0, 0, 0], (n, 1))
+    quat[::3] = [0.9238795, 0.0, 0.0, 0.3826834]        # some rotated boxes
+    dyn = np.ones(n, np.uint8)
+    dyn[::4] = 0
+    scene = W.Scene(dim, pos, quat, np.zeros(n, np.uint8), np.full(n, W.MAX_DISTANCE), dynamic=dyn)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    vel, vmax = (0.5 * u).astype(np.float32), np.full(n, 0.5, np.float32)
+    eid = engine.create_env(scene)
+    kw = dict(ws_center=(0, 0, 0.267), ws_radius=1.0, base_radius=0.15, robot_max_vel=3.0)
+    engine.set_motion_law(eid, law, vel, vmax, **kw)
+    m = oracle.MotionState(law, scene.pos, vel, vmax, dyn, **kw)
+    gp, gv = engine.env_step(eid, 0.01, 150)
+    gp, gv = engine.env_step(eid, 0.01, 150)
+    op, ov = m.step(0.01, 300)
+    err = np.abs(gp - op).max()
+    print(f"law {law}: max position difference {err:.3g}, identical {(gp == op).all()}")
+    if law == 1:
+        assert err < 1e-4            # device and host sin / cos / atan2 may differ in the last bit
+    else:
+        assert (gp == op).all() and (gv == ov).all()
+    moved = W.Scene(dim, gp, quat, np.zeros(n, np.uint8), np.full(n, W.MAX_DISTANCE))
+    assert (engine.env_bounds(eid) == moved.aabb).all()
+    rb = R.xarm6()
+    rid = engine.load_robot(rb)
+    q = W.workload2(4096).q
+    d1, p1, v1, _ = engine.distance(rid, eid, q)
+    d2, p2, v2, _ = engine.distance(rid, engine.create_env(moved), q)
+    assert (d1 == d2).all() and (p1 == p2).all() and (v1 == v2).all()

@@ -982,3 +982,89 @@ def test_trajectory_safety_counts_obstacle_motion_while_the_robot_waits(oracle):
     for v, expect in ((0.1, 1), (0.3, 0)):
         safe = oracle.traj_safe(rb, sc, pts, root, p0, np0, np.full(scene.num_obj, v, np.float32))
         assert safe[0] == expect, v
+
+
+# ---------------------------------------------------------------- obstacle motion laws
+def _motion(oracle, law, n=32, seed=90, speed=0.3):
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((0, 3))
+    while len(pos) < n:     # inside the workspace ball, above the ground, away from the base
+        c = np.stack([rng.uniform(-0.8, 0.8, 4 * n), rng.uniform(-0.8, 0.8, 4 * n), rng.uniform(0.05, 1.0, 4 * n)], 1)
+        rr = np.linalg.norm(c - np.array([0, 0, 0.267]), axis=1)
+        c = c[(rr < 0.95) & (rr > 0.35) & (np.hypot(c[:, 0], c[:, 1]) > 0.35)]
+        pos = np.concatenate([pos, c])[:n]
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    dyn = np.ones(n, np.uint8)
+    dyn[::5] = 0
+    return oracle.MotionState(law, pos, speed * u, np.full(n, speed), dyn, ws_center=(0, 0, 0.267), ws_radius=1.0,
+                              base_radius=0.15, robot_max_vel=3.0)
+
+
+def test_motion_straight_and_static(oracle):
+    """moveStraight (Environment.cpp:225-238): x decreases by max_vel * dt per
+    step, y and z stay; boxes without the dynamic label never move."""
+    m = _motion(oracle, 0)
+    p0 = m.pos.copy()
+    m.step(0.01, 250)
+    dyn = m.dynamic == 1
+    assert (m.pos[~dyn] == p0[~dyn]).all()
+    assert np.abs(m.pos[dyn, 0] - (p0[dyn, 0] - 250 * 0.01 * 0.3)).max() < 1e-5
+    assert (m.pos[dyn, 1:] == p0[dyn, 1:]).all()
+
+
+def test_motion_circular_euler_steps(oracle):
+    """moveCircular (:241-257) takes explicit tangent steps about the z axis:
+    |p_xy|^2 grows by (v dt)^2 per step and the angle by atan(v dt / r)."""
+    m = _motion(oracle, 1, speed=0.5)
+    p0 = m.pos.astype(np.float64).copy()
+    dyn = m.dynamic == 1
+    N, h = 400, 0.5 * 0.01
+    m.step(0.01, N)
+    r0 = np.hypot(p0[dyn, 0], p0[dyn, 1])
+    r = np.hypot(m.pos[dyn, 0], m.pos[dyn, 1]).astype(np.float64)
+    assert np.abs(r ** 2 - (r0 ** 2 + N * h * h)).max() < 2e-4
+    rk = np.sqrt(r0[:, None] ** 2 + np.arange(N)[None] * h * h)
+    ang = np.arctan2(p0[dyn, 1], p0[dyn, 0]) + np.arctan(h / rk).sum(1)
+    got = np.arctan2(m.pos[dyn, 1], m.pos[dyn, 0]).astype(np.float64)
+    assert np.abs(np.angle(np.exp(1j * (got - ang)))).max() < 1e-3
+    assert (m.pos[dyn, 2] == p0[dyn, 2].astype(np.float32)).all()
+
+
+def test_motion_two_tunnels_legs(oracle):
+    """moveTwoTunnels (:260-287): boxes above y = 0 shuttle along x, boxes
+    below along y, turning after each 0.2 m leg (then 0.4 m legs), so the
+    displacement stays within 0.2 m (+ one step) and reaches both ends."""
+    rng = np.random.default_rng(91)
+    n = 16
+    pos = np.stack([rng.uniform(-1, 1, n), np.where(np.arange(n) % 2 == 0, 0.6, -0.8) + rng.uniform(-0.1, 0.1, n),
+                    rng.uniform(0.2, 1.0, n)], 1)
+    m = oracle.MotionState(2, pos, np.zeros((n, 3)), np.full(n, 0.25), np.ones(n, np.uint8))
+    p0 = m.pos.copy()
+    lo, hi = np.zeros((n, 3)), np.zeros((n, 3))
+    for _ in range(300):
+        m.step(0.01, 1)
+        dl = (m.pos - p0).astype(np.float64)
+        lo, hi = np.minimum(lo, dl), np.maximum(hi, dl)
+    up = p0[:, 1] > 0
+    assert (lo[up, 1:] == 0).all() and (hi[up, 1:] == 0).all()
+    assert (lo[~up][:, [0, 2]] == 0).all() and (hi[~up][:, [0, 2]] == 0).all()
+    ax = np.where(up, 0, 1)
+    span_lo, span_hi = lo[np.arange(n), ax], hi[np.arange(n), ax]
+    assert (span_hi <= 0.2 + 2 * 0.0025 + 1e-6).all() and (span_lo >= -0.2 - 2 * 0.0025 - 1e-6).all()
+    assert (span_hi > 0.19).all() and (span_lo < -0.19).all()
+
+
+def test_motion_light_directions_reflects(oracle):
+    """moveLightDirections (:330-376): mirror reflection keeps each box's
+    speed, keeps it above the ground and inside the workspace sphere (up to
+    one step), and out of the base zone."""
+    m = _motion(oracle, 3, n=64, speed=0.6)
+    dyn = m.dynamic == 1
+    s0 = np.linalg.norm(m.vel[dyn], axis=1)
+    for _ in range(2000):
+        m.step(0.01, 1)
+        assert (m.pos[dyn, 2] > -1e-5).all()
+        assert (np.linalg.norm(m.pos[dyn] - np.array([0, 0, 0.267]), axis=1) <= 1.0 + 0.006 + 1e-5).all()
+    s = np.linalg.norm(m.vel[dyn], axis=1)
+    assert np.abs(s - s0).max() < 1e-4
