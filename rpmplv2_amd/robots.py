@@ -98,6 +98,16 @@ def planar_2dof(radius: float | Sequence[float] = 0.025) -> Robot:
     return Robot("planar_2dof", SKEL_CHAIN, joints, 2, [0.0, 0.0]).with_radius(radius)
 
 
+def planar_10dof(radius: float | Sequence[float] = 0.025) -> Robot:
+    """The repository's planar_10dof robot: ten revolute joints about z,
+    0.2 m apart along x, tool 0.2 m past the last joint."""
+    joints = [Joint(JOINT_REVOLUTE, (0, 0, 0), axis=(0, 0, 1), lower=-3.14159, upper=3.14159)]
+    for _ in range(9):
+        joints.append(Joint(JOINT_REVOLUTE, (0.2, 0, 0), axis=(0, 0, 1), lower=-3.14159, upper=3.14159))
+    joints.append(Joint(JOINT_FIXED, (0.2, 0, 0)))
+    return Robot("planar_10dof", SKEL_CHAIN, joints, 10, [0.0] * 10).with_radius(radius)
+
+
 _SPATIAL_L = {10: 0.1, 14: 0.0692, 18: 0.05294, 22: 0.042857}
 
 

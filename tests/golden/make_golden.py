@@ -17,7 +17,10 @@
    freeze the oracle's behaviour; its correctness is pinned by the closed
    forms, brute force and the known answers above (tests/test_oracle.py).
 
-Usage: python tests/golden/make_golden.py   (needs /root/reference for part 1)
+3. scenarios/*.yaml - data of a few scenario files and random-scenario
+   runs (comments removed) for the scenario loader.
+
+Usage: python tests/golden/make_golden.py [--no-vectors]   (parts 1 and 3 need /root/reference)
 """
 from __future__ import annotations
 
@@ -91,6 +94,45 @@ def reference_scenarios():
         json.dump(out, f, indent=0)
 
 
+SCENARIO_FILES = [("planar_2dof", "scenario_test"), ("planar_2dof", "scenario1"), ("planar_10dof", "scenario_test"),
+                  ("spatial_10dof", "scenario1"), ("xarm6", "scenario_test"), ("xarm6", "scenario1"),
+                  ("xarm6", "scenario2"), ("spatial_10dof", "scenario_random_obstacles")]
+
+
+def scenario_files():
+    """3. scenarios/*.yaml - the data of a few scenario files (comments
+    removed, layout and tabs kept) and the first runs of one
+    random_scenarios.yaml, for the scenario loader's tests."""
+    out = os.path.join(HERE, "scenarios")
+    os.makedirs(out, exist_ok=True)
+
+    def strip(text):
+        lines = []
+        for ln in text.splitlines():
+            k = ln.find("#")
+            ln = (ln[:k] if k >= 0 else ln).rstrip()
+            if ln.strip():
+                lines.append(ln)
+        return "\n".join(lines) + "\n"
+
+    for d, name in SCENARIO_FILES:
+        with open(os.path.join(REF, d, name, name + ".yaml")) as f:
+            text = f.read()
+        with open(os.path.join(out, f"{d}_{name}.yaml"), "w") as f:
+            f.write(strip(text))
+    with open(os.path.join(REF, "spatial_10dof", "scenario_random_obstacles", "random_scenarios.yaml")) as f:
+        text = strip(f.read())
+    keep, runs = [], 0
+    for ln in text.splitlines():
+        if ln.startswith("  run_"):
+            runs += 1
+        if runs > 5:
+            break
+        keep.append(ln)
+    with open(os.path.join(out, "spatial_10dof_random_scenarios.yaml"), "w") as f:
+        f.write("\n".join(keep) + "\n")
+
+
 def oracle_vectors():
     sys.path.insert(0, ROOT)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -134,5 +176,7 @@ def oracle_vectors():
 if __name__ == "__main__":
     if os.path.isdir(REF):
         reference_scenarios()
-    oracle_vectors()
+        scenario_files()
+    if "--no-vectors" not in sys.argv:
+        oracle_vectors()
     print("golden fixtures written to", HERE)
