@@ -656,7 +656,7 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
             if (e == hipSuccess)
                 e = (prec == RVS_PREC_FP64) ? launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream)
-                                            : launch_distance_ref32(rb.dev, ev.dev, qp, qd, n, out, c->stream);
+                                            : launch_distance_ref32_auto(rb.dev, ev.dev, qp, qd, n, out, c->wave_max, c->stream);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
         }
     }
@@ -749,6 +749,14 @@ rvs_status rvs_set_tuning(rvs_context* c, int32_t robot, int32_t env, const rvs_
         return fail(c, RVS_EINVAL, "rvs_set_tuning: variant 0/1, block 64/128/256, blocks_per_cu >= 0");
     std::lock_guard<std::mutex> lk(c->tuning_mu);
     c->tuning[{robot, env}] = FastTuning{t->variant, t->block, t->blocks_per_cu};
+    return RVS_OK;
+}
+
+rvs_status rvs_set_small_batch(rvs_context* c, int64_t max_configs) {
+    if (!c) return RVS_EINVAL;
+    if (max_configs < 0) return fail(c, RVS_EINVAL, "rvs_set_small_batch: negative size");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->wave_max = max_configs;
     return RVS_OK;
 }
 
@@ -1179,7 +1187,7 @@ rvs_status rvs_extend_gen_spines(rvs_context* c, int32_t robot, int32_t env, con
     const int D = rb.num_dofs, L = rb.num_links, M = ev.dev.num_obj;
     const bool host = !(flags & RVS_F_DEVICE);
     const size_t qb = (size_t)n * D * 4, set = (size_t)M * L * 6;
-    hipError_t e = c->arena.ensure(Carve::need({4, (size_t)L * 4, set * 4, (size_t)n * 4, (size_t)n * L * 4, (size_t)n,
+    hipError_t e = c->arena.ensure(Carve::need({4, (size_t)L * 4, set * 4, (size_t)n * 4, (size_t)n * L * 4,
                                                 (size_t)n, host ? (size_t)D * 4 : 0, host ? qb : 0, host ? qb : 0,
                                                 host ? (size_t)n * 4 : 0, host ? (size_t)n * 4 : 0}));
     if (e != hipSuccess) return hip_fail(c, e, "rvs_extend_gen_spines: staging");
@@ -1189,7 +1197,6 @@ rvs_status rvs_extend_gen_spines(rvs_context* c, int32_t robot, int32_t env, con
     float* np_root = cv.take<float>(set);
     float* d_layer = cv.take<float>(n);
     float* p_layer = cv.take<float>((size_t)n * L);
-    uint8_t* stored = cv.take<uint8_t>(n);
     uint8_t* active = cv.take<uint8_t>(n);
     const float* qr = q_root;
     const float* qe = q_e;
@@ -1212,12 +1219,32 @@ rvs_status rvs_extend_gen_spines(rvs_context* c, int32_t robot, int32_t env, con
     ro.d = d_root;
     ro.profile = p_root;
     ro.nearest = np_root;
-    if (e == hipSuccess) e = launch_distance_ref32(rb.dev, ev.dev, qp, qr, 1, ro, c->stream);
-    if (e == hipSuccess) e = launch_fill_u8(active, n, 1, c->stream);
+    if (e == hipSuccess) e = launch_distance_ref32_auto(rb.dev, ev.dev, qp, qr, 1, ro, c->wave_max, c->stream);
+    if (!use_real_dist) {
+        // every layer of a spine in one lane
+        GenSpineArgs g{};
+        g.q_root = qr;
+        g.d_root = d_root;
+        g.profile_root = p_root;
+        g.np_root = np_root;
+        g.q_e = qe;
+        g.q_new = qn;
+        g.status = st;
+        g.d_out = dd;
+        g.num_obj = M;
+        g.num_layers = num_layers;
+        g.num_iter = num_iter;
+        g.expanded = use_expanded_bubble ? 1 : 0;
+        g.d_crit = d_crit;
+        g.maxd = qp.maxd;
+        if (e == hipSuccess) e = launch_gen_spines_under(rb.dev, g, n, c->stream);
+    } else if (e == hipSuccess) {
+        e = launch_fill_u8(active, n, 1, c->stream);
+    }
     DistOut lo{};
     lo.d = d_layer;
     lo.profile = p_layer;
-    for (int layer = 0; layer < num_layers && e == hipSuccess; ++layer) {
+    for (int layer = 0; use_real_dist && layer < num_layers && e == hipSuccess; ++layer) {
         SpineArgs a{};
         a.q = layer == 0 ? qr : qn;
         a.q_stride = layer == 0 ? 0 : D;
@@ -1233,9 +1260,7 @@ rvs_status rvs_extend_gen_spines(rvs_context* c, int32_t robot, int32_t env, con
         a.expanded = use_expanded_bubble ? 1 : 0;
         e = launch_spines(rb.dev, a, n, c->stream);
         if (e != hipSuccess) break;
-        e = use_real_dist ? launch_distance_ref32(rb.dev, ev.dev, qp, qn, n, lo, c->stream)
-                          : launch_underestimation(rb.dev, ev.dev, qp, qn, n, np_root, 0, d_layer, p_layer, stored,
-                                                   c->stream);
+        e = launch_distance_ref32_auto(rb.dev, ev.dev, qp, qn, n, lo, c->wave_max, c->stream);
         if (e == hipSuccess) e = launch_gen_layer(n, layer, d_root, d_layer, st, active, dd, d_crit, c->stream);
     }
     if (e == hipSuccess && host) {

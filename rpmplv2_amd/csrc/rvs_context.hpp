@@ -73,6 +73,9 @@ struct rvs_context {
     // (lane refill when links x obstacles >= 1024), 0 = one lane per
     // configuration, 1 = lane refill
     int32_t fast_variant = -1;
+    // reference-arithmetic distance batches up to this many configurations
+    // run one wave per configuration (latency) instead of one lane
+    int64_t wave_max = 16384;
     int32_t edge_waves_per_cu = 16;   // persistent first-collision kernel: resident waves per CU
     int32_t edge_spl = 1;             // ... and slots per lane in one 64-lane item
     std::map<std::pair<int32_t, int32_t>, rvs::FastTuning> tuning;   // (robot, env) -> autotuned launch shape

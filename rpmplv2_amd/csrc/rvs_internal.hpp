@@ -94,6 +94,17 @@ hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st);
 
 hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                  int64_t n, const DistOut& out, hipStream_t st);
+// The same results with one wave per configuration (lanes over obstacles):
+// the low-latency form for small batches.
+hipError_t launch_distance_ref32_waves(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                       int64_t n, const DistOut& out, hipStream_t st);
+// Either form by batch size: waves up to wave_max configurations.
+inline hipError_t launch_distance_ref32_auto(const RobotDev& rb, const EnvDev& env, const QueryParams& qp,
+                                             const float* q, int64_t n, const DistOut& out, int64_t wave_max,
+                                             hipStream_t st) {
+    return n <= wave_max ? launch_distance_ref32_waves(rb, env, qp, q, n, out, st)
+                         : launch_distance_ref32(rb, env, qp, q, n, out, st);
+}
 hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                 int64_t n, const DistOut& out, hipStream_t st);
 hipError_t launch_is_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
@@ -178,6 +189,26 @@ struct SpineArgs {
     int32_t expanded;
 };
 hipError_t launch_spines(const RobotDev& rb, const SpineArgs& a, int64_t n, hipStream_t st);
+// Generalized spines with underestimation layers, fused (device pointers):
+// root state, its distance [1], profile [L] and nearest points [M][L][6];
+// targets q_e [N][n]; outputs q_new [N][n], status [N], d_out [N].
+struct GenSpineArgs {
+    const float* q_root;
+    const float* d_root;
+    const float* profile_root;
+    const float* np_root;
+    const float* q_e;
+    float* q_new;
+    int32_t* status;
+    float* d_out;
+    int32_t num_obj;
+    int32_t num_layers;
+    int32_t num_iter;
+    int32_t expanded;
+    float d_crit;
+    float maxd;
+};
+hipError_t launch_gen_spines_under(const RobotDev& rb, const GenSpineArgs& g, int64_t n, hipStream_t st);
 // Between layers of generalized spines: d_out = d_layer and deactivation of
 // spines with d_layer < d_crit or status != advanced (layer 0: a root in
 // contact stops every spine with the root's distance).

@@ -157,20 +157,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // reduction order cannot change a bit of the result.
 constexpr int kRefineWaves = 4;   // waves per workgroup of the refinement kernel (they share the staged obstacles)
 
-template <bool OBB>
+// With idx null the kernel serves a direct call: wave k takes configuration
+// k < n (latency-bound small batches: one wave per configuration instead of
+// one lane).  With nearest points every pair is evaluated (no broad phase)
+// and its points stored; a collision leaves NaN points, as the lane kernel.
+template <bool OBB, bool WANT_NP>
 __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev rb, EnvDev env, QueryParams qp,
                                                                        const float* __restrict__ q, DistOut out,
                                                                        const int32_t* __restrict__ idx,
-                                                                       const uint32_t* __restrict__ count) {
+                                                                       const uint32_t* __restrict__ count,
+                                                                       int64_t n_direct) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<float> s = stage_env<float, OBB>(env, smem);
     const int M = env.num_obj, L = rb.num_links;
     const int t = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
     const Params<float> p{qp.eq, qp.maxd};
-    const uint32_t total = *count;
-    for (uint32_t k = blockIdx.x * kRefineWaves + wave; k < total; k += gridDim.x * kRefineWaves) {
-        const int64_t c = idx[k];
+    const int64_t total = idx ? (int64_t)*count : n_direct;
+    for (int64_t k = (int64_t)blockIdx.x * kRefineWaves + wave; k < total; k += (int64_t)gridDim.x * kRefineWaves) {
+        const int64_t c = idx ? (int64_t)idx[k] : k;
         const float* qc = q + c * rb.num_dofs;
         FKStream fk;
         fk_init(fk);
@@ -188,24 +193,43 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
             seg_bounds(P0, P1, slo, shi);
             for (int j = t; j < M; j += 64) {
                 float dt;
+                NP<float> np;
                 // broad phase against this lane's running minimum, an upper
                 // bound of the link's minimum over all lanes
-                if (cull_pair(env.cull + 6 * j, slo, shi, cull_bound(lo, r, p.eq))) continue;
+                if (!WANT_NP && cull_pair(env.cull + 6 * j, slo, shi, cull_bound(lo, r, p.eq))) continue;
                 if (s.gnd[j] && i < rb.ground_included) {
                     dt = p.maxd;
+                    if (WANT_NP) {
+                        np.r = Vec3<float>{0.0f, 0.0f, 0.0f};
+                        np.o = Vec3<float>{0.0f, 0.0f, -p.maxd};
+                        np.valid = true;
+                    }
                 } else if (OBB) {
                     const float* x = s.xf + 12 * j;
                     const Box<float> bx = s.box[j];
-                    dt = capsule_box<float, false>(to_box(x, P0), to_box(x, P1), r, bx, p, nullptr, nullptr, nullptr);
+                    dt = capsule_box<float, WANT_NP>(to_box(x, P0), to_box(x, P1), r, bx, p, &np, nullptr, nullptr);
+                    if (WANT_NP && np.valid) {
+                        np.r = from_box(x, np.r);
+                        np.o = from_box(x, np.o);
+                    }
                 } else {
                     const Box<float> bx = s.box[j];
-                    dt = capsule_box<float, false>(P0, P1, r, bx, p, nullptr, nullptr, nullptr);
+                    dt = capsule_box<float, WANT_NP>(P0, P1, r, bx, p, &np, nullptr, nullptr);
                 }
                 if (dt > s.tol[j]) dt = p.maxd;
                 lo = smin(lo, dt);
                 if (dt <= 0 && first == INT_MAX) {
                     first = j;
                     first_v = dt;
+                }
+                if (WANT_NP) {
+                    float* e = out.nearest + ((c * M + j) * L + i) * 6;
+                    if (np.valid) {
+                        e[0] = np.r.x; e[1] = np.r.y; e[2] = np.r.z;
+                        e[3] = np.o.x; e[4] = np.o.y; e[5] = np.o.z;
+                    } else {
+                        for (int k2 = 0; k2 < 6; ++k2) e[k2] = NAN;
+                    }
                 }
             }
             for (int o = 32; o > 0; o >>= 1) {
@@ -223,6 +247,13 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
             if (hit) break;
             d = smin(d, prof);
             P0 = P1;
+        }
+        if (WANT_NP && hit) {
+            // the points this wave stored for the pairs so far must land
+            // before the NaN fill of the same addresses by other lanes
+            __builtin_amdgcn_s_waitcnt(0);
+            float* e = out.nearest + c * (int64_t)M * L * 6;
+            for (int k2 = t; k2 < M * L * 6; k2 += 64) e[k2] = NAN;
         }
         if (t == 0) {
             for (int k2 = i + 1; k2 < L; ++k2)
@@ -569,6 +600,32 @@ hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const Que
     return hipGetLastError();
 }
 
+hipError_t launch_distance_ref32_waves(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                       int64_t n, const DistOut& out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)std::min<int64_t>((n + kRefineWaves - 1) / kRefineWaves, 65536);
+    const size_t sm = staged_bytes<float>(env);
+    const dim3 b(64 * kRefineWaves);
+    const int32_t* no_idx = nullptr;
+    const uint32_t* no_count = nullptr;
+    if (env.use_obb) {
+        if (out.nearest)
+            hipLaunchKernelGGL((k_distance_refine<true, true>), dim3(grid), b, sm, st, rb, env, qp, q, out, no_idx,
+                               no_count, n);
+        else
+            hipLaunchKernelGGL((k_distance_refine<true, false>), dim3(grid), b, sm, st, rb, env, qp, q, out, no_idx,
+                               no_count, n);
+    } else {
+        if (out.nearest)
+            hipLaunchKernelGGL((k_distance_refine<false, true>), dim3(grid), b, sm, st, rb, env, qp, q, out, no_idx,
+                               no_count, n);
+        else
+            hipLaunchKernelGGL((k_distance_refine<false, false>), dim3(grid), b, sm, st, rb, env, qp, q, out, no_idx,
+                               no_count, n);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                   int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
                                   hipStream_t st) {
@@ -579,9 +636,11 @@ hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const Q
     const size_t sm = staged_bytes<float>(env);
     const dim3 b(64 * kRefineWaves);
     if (env.use_obb)
-        hipLaunchKernelGGL(k_distance_refine<true>, dim3(grid), b, sm, st, rb, env, qp, q, out, idx, count);
+        hipLaunchKernelGGL((k_distance_refine<true, false>), dim3(grid), b, sm, st, rb, env, qp, q, out, idx, count,
+                           (int64_t)0);
     else
-        hipLaunchKernelGGL(k_distance_refine<false>, dim3(grid), b, sm, st, rb, env, qp, q, out, idx, count);
+        hipLaunchKernelGGL((k_distance_refine<false, false>), dim3(grid), b, sm, st, rb, env, qp, q, out, idx, count,
+                           (int64_t)0);
     return hipGetLastError();
 }
 

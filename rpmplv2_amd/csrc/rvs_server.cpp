@@ -290,7 +290,7 @@ hipError_t issue(rvs_server* s, Batch& b, const Layout& lay) {
             if (e == hipSuccess) e = launch_distance_fast(rb, env, s->qp, q, N, out, idx, count, ft, s->ctx->num_cus, st);
             if (e == hipSuccess) e = launch_distance_refine(rb, env, s->qp, q, N, out, idx, count, st);
         } else if (e == hipSuccess) {
-            e = launch_distance_ref32(rb, env, s->qp, q, N, out, st);
+            e = launch_distance_ref32_auto(rb, env, s->qp, q, N, out, s->ctx->wave_max, st);
         }
     } else {
         // nearest points of every request's first configuration, then the
@@ -315,7 +315,7 @@ hipError_t issue(rvs_server* s, Batch& b, const Layout& lay) {
         o1.d = df;
         o1.nearest = nps;
         o1.verdict = vf;
-        if (e == hipSuccess) e = launch_distance_ref32(rb, env, s->qp, qf, lay.nreq, o1, st);
+        if (e == hipSuccess) e = launch_distance_ref32_auto(rb, env, s->qp, qf, lay.nreq, o1, s->ctx->wave_max, st);
         if (e == hipSuccess)
             e = launch_underestimation_seg(rb, env, s->qp, q, N, nps, seg, lay.nreq, d, prof, flag, st);
     }

@@ -10,8 +10,10 @@
 // shrinks the clearances for the next iteration.  Skeleton, radii column,
 // state and clearances of a lane live in its LDS slice.
 //
-// k_gen_layer: the bookkeeping of RGBTConnect::extendGenSpine
-// (src/planners/rbt/RGBTConnect.cpp:71-90) between layers.
+// k_gen_spines_under: RGBTConnect::extendGenSpine (src/planners/rbt/
+// RGBTConnect.cpp:71-90) with underestimation layers, all layers of a spine
+// in one lane.  k_gen_layer: the bookkeeping between layers when each
+// layer's distance is a separate launch (real distances).
 //
 // k_edge_self: the capsule part of xArm6::checkSelfCollision(q1, q2)
 // (src/robots/xArm6.cpp:284-368) for a batch of edges, one lane per edge:
@@ -45,87 +47,176 @@ constexpr int32_t kAdvanced = 1, kTrapped = 2, kReached = 3;
 // losses (D each).
 __host__ __device__ inline int spine_slice(int D) { return 6 * (D + 1) + 3 * D; }
 
+// One lane's view of its LDS slice.
+struct SpineLane {
+    float* sk0;
+    float* sk;
+    float* ac;
+    float* qt;
+    float* rp;
+    int B, t;
+    __device__ SpineLane(float* base, int D, int B_, int t_) : B(B_), t(t_) {
+        sk0 = base;
+        sk = sk0 + (size_t)3 * (D + 1) * B;
+        ac = sk + (size_t)3 * (D + 1) * B;
+        qt = ac + (size_t)D * B;
+        rp = qt + (size_t)D * B;
+    }
+    __device__ float& Q(int k) const { return qt[k * B + t]; }
+    __device__ float& RP(int k) const { return rp[k * B + t]; }
+    __device__ float& acc(int i) const { return ac[i * B + t]; }
+    __device__ Vec3<float> P0(int k) const {
+        return Vec3<float>{sk0[(3 * k) * B + t], sk0[(3 * k + 1) * B + t], sk0[(3 * k + 2) * B + t]};
+    }
+    __device__ Vec3<float> P(int k) const {
+        return Vec3<float>{sk[(3 * k) * B + t], sk[(3 * k + 1) * B + t], sk[(3 * k + 2) * B + t]};
+    }
+    __device__ void set_P(float* s, int k, const Vec3<float>& v) const {
+        s[(3 * k) * B + t] = v.x;
+        s[(3 * k + 1) * B + t] = v.y;
+        s[(3 * k + 2) * B + t] = v.z;
+    }
+};
+
+// extendSpine from the state in the lane's Q (distance dq, link profile
+// pc(i)) towards qe; Q holds q_new afterwards.  Returns the status.
+template <typename PC>
+__device__ int32_t spine_run(const RobotDev& rb, const SpineLane& s, float dq, PC pc, const float* qe,
+                             int num_iter, int expanded) {
+    const int D = rb.num_dofs;
+    if (dq <= 0.0f) return kTrapped;
+    {
+        FKStream fk;
+        fk_init(fk);
+        for (int k = 0; k <= D; ++k) {
+            const Vec3<float> v = skel_point<float>(fk, rb, s.qt + s.t, k, s.B);
+            s.set_P(s.sk0, k, v);
+            s.set_P(s.sk, k, v);
+        }
+    }
+    for (int k = 0; k < D; ++k) s.RP(k) = 0.0f;
+    float rho = 0.0f;
+    int32_t status = kAdvanced;
+    auto P = [&](int k) { return s.P(k); };
+    auto acc = [&](int i) -> float& { return s.acc(i); };
+    for (int it = 1;; ++it) {
+        for (int i = 0; i < D; ++i) s.acc(i) = 0.0f;
+        // covered distance of skeleton point j: R.col(j).dot(|q_e - q_temp|)
+        auto covered = [&]() {
+            return eigen_dyn_sum([&](int i) { return s.acc(i) * fabsf(qe[i] - s.Q(i)); }, D);
+        };
+        float step = INFINITY;
+        for (int j = 1; j <= D; ++j) {
+            radii_column(rb, P, j, acc);
+            if (expanded) step = min_std(step, (pc(j - 1) - s.RP(j - 1)) / covered());
+        }
+        if (!expanded) step = (dq - rho) / covered();
+        if (step > 1.0f) {
+            for (int k = 0; k < D; ++k) s.Q(k) = qe[k];
+            status = kReached;
+        } else {
+            for (int k = 0; k < D; ++k) s.Q(k) = s.Q(k) + step * (qe[k] - s.Q(k));
+        }
+        if (it >= num_iter || status != kAdvanced) break;
+        // moved distance of every skeleton point since the start state
+        FKStream fk;
+        fk_init(fk);
+        s.set_P(s.sk, 0, skel_point<float>(fk, rb, s.qt + s.t, 0, s.B));
+        float prev = 0.0f;
+        for (int k = 1; k <= D; ++k) {
+            const Vec3<float> v = skel_point<float>(fk, rb, s.qt + s.t, k, s.B);
+            s.set_P(s.sk, k, v);
+            const float rk = norm_dyn3(s.P0(k) - v);
+            s.RP(k - 1) = max_std(prev, rk);
+            prev = rk;
+            rho = max_std(rho, rk);
+        }
+    }
+    return status;
+}
+
 __global__ __launch_bounds__(64) void k_spines(RobotDev rb, SpineArgs a, int64_t n) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int D = rb.num_dofs;
-    const int B = blockDim.x, t = threadIdx.x;
-    float* base = reinterpret_cast<float*>(smem);
-    float* sk0 = base;                                  // [3 (D+1)][B]
-    float* sk = sk0 + (size_t)3 * (D + 1) * B;          // [3 (D+1)][B]
-    float* ac = sk + (size_t)3 * (D + 1) * B;           // [D][B]
-    float* qt = ac + (size_t)D * B;                     // [D][B]
-    float* rp = qt + (size_t)D * B;                     // [D][B]
+    const SpineLane s(reinterpret_cast<float*>(smem), D, blockDim.x, threadIdx.x);
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     if (a.active && !a.active[c]) return;
     const float* qc = a.q + c * a.q_stride;
     const float* pc = a.profile + c * a.profile_stride;
-    const float* qe = a.q_e + c * D;
-    const float dq = a.d[c * a.d_stride];
-    float* qn = a.q_new + c * D;
-    auto Q = [&](int k) -> float& { return qt[k * B + t]; };
-    auto RP = [&](int k) -> float& { return rp[k * B + t]; };
-    auto acc = [&](int i) -> float& { return ac[i * B + t]; };
-    auto P0 = [&](int k) { return Vec3<float>{sk0[(3 * k) * B + t], sk0[(3 * k + 1) * B + t], sk0[(3 * k + 2) * B + t]}; };
-    auto P = [&](int k) { return Vec3<float>{sk[(3 * k) * B + t], sk[(3 * k + 1) * B + t], sk[(3 * k + 2) * B + t]}; };
-    auto set_P = [&](float* s, int k, const Vec3<float>& v) {
-        s[(3 * k) * B + t] = v.x;
-        s[(3 * k + 1) * B + t] = v.y;
-        s[(3 * k + 2) * B + t] = v.z;
-    };
     // q_new may alias q (in-place layers): read the whole state first
-    for (int k = 0; k < D; ++k) Q(k) = qc[k];
-    int32_t status = kAdvanced;
-    if (dq <= 0.0f) {
-        status = kTrapped;
-    } else {
-        {
+    for (int k = 0; k < D; ++k) s.Q(k) = qc[k];
+    const int32_t status =
+        spine_run(rb, s, a.d[c * a.d_stride], [&](int i) { return pc[i]; }, a.q_e + c * D, a.num_iter, a.expanded);
+    float* qn = a.q_new + c * D;
+    for (int k = 0; k < D; ++k) qn[k] = s.Q(k);
+    a.status[c] = status;
+}
+
+// All layers of generalized spines with underestimation distances in one
+// lane (RGBTConnect::extendGenSpine with use_real_dist = false): spine,
+// computeDistanceUnderestimation of its end state against the root's
+// nearest points, stop below D_CRIT or when the spine did not advance.  The
+// profile of the current layer's start state sits in the lane's slice after
+// the spine part.
+__global__ __launch_bounds__(64) void k_gen_spines_under(RobotDev rb, GenSpineArgs g, int64_t n) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int D = rb.num_dofs, L = rb.num_links;
+    const int B = blockDim.x, t = threadIdx.x;
+    float* base = reinterpret_cast<float*>(smem);
+    const SpineLane s(base, D, B, t);
+    float* pr = base + (size_t)spine_slice(D) * B;    // [L][B]
+    auto PR = [&](int k) -> float& { return pr[k * B + t]; };
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const float* qe = g.q_e + c * D;
+    const float d0 = g.d_root[0];
+    for (int k = 0; k < D; ++k) s.Q(k) = g.q_root[k];
+    for (int k = 0; k < L; ++k) PR(k) = g.profile_root[k];
+    int32_t status = kTrapped;
+    float d_out = d0;
+    float dq = d0;
+    if (d0 > 0.0f) {
+        for (int layer = 0; layer < g.num_layers; ++layer) {
+            status = spine_run(rb, s, dq, [&](int i) { return PR(i); }, qe, g.num_iter, g.expanded);
+            // computeDistanceUnderestimation(q_new, root nearest points)
             FKStream fk;
             fk_init(fk);
-            for (int k = 0; k <= D; ++k) {
-                const Vec3<float> v = skel_point<float>(fk, rb, qt + t, k, B);
-                set_P(sk0, k, v);
-                set_P(sk, k, v);
+            Vec3<float> P0 = skel_point<float>(fk, rb, s.qt + t, 0, B);
+            float d = g.maxd;
+            bool neg = false;
+            for (int i = 0; i < L && !neg; ++i) {
+                const Vec3<float> P1 = skel_point<float>(fk, rb, s.qt + t, i + 1, B);
+                float prof = g.maxd;
+                for (int j = 0; j < g.num_obj; ++j) {
+                    const float* e = g.np_root + ((int64_t)j * L + i) * 6;
+                    const Vec3<float> O{e[3], e[4], e[5]};
+                    if (vnorm(O) == g.maxd) continue;
+                    const Vec3<float> R{e[0], e[1], e[2]};
+                    const float tm = plane_term(P0, P1, R, O, rb.radius[i]);
+                    if (tm < 0.0f) {
+                        neg = true;
+                        break;
+                    }
+                    prof = min_std(prof, tm);
+                }
+                PR(i) = prof;
+                d = min_std(d, prof);
+                P0 = P1;
             }
-        }
-        for (int k = 0; k < D; ++k) RP(k) = 0.0f;
-        float rho = 0.0f;
-        for (int it = 1;; ++it) {
-            for (int i = 0; i < D; ++i) acc(i) = 0.0f;
-            // covered distance of skeleton point j: R.col(j).dot(|q_e - q_temp|)
-            auto covered = [&]() {
-                return eigen_dyn_sum([&](int i) { return acc(i) * fabsf(qe[i] - Q(i)); }, D);
-            };
-            float step = INFINITY;
-            for (int j = 1; j <= D; ++j) {
-                radii_column(rb, P, j, acc);
-                if (a.expanded) step = min_std(step, (pc[j - 1] - RP(j - 1)) / covered());
+            if (neg) {   // returned early: no profile stored
+                d = 0.0f;
+                for (int k = 0; k < L; ++k) PR(k) = 0.0f;
             }
-            if (!a.expanded) step = (dq - rho) / covered();
-            if (step > 1.0f) {
-                for (int k = 0; k < D; ++k) Q(k) = qe[k];
-                status = kReached;
-            } else {
-                for (int k = 0; k < D; ++k) Q(k) = Q(k) + step * (qe[k] - Q(k));
-            }
-            if (it >= a.num_iter || status != kAdvanced) break;
-            // moved distance of every skeleton point since the start state
-            FKStream fk;
-            fk_init(fk);
-            set_P(sk, 0, skel_point<float>(fk, rb, qt + t, 0, B));
-            float prev = 0.0f;
-            for (int k = 1; k <= D; ++k) {
-                const Vec3<float> v = skel_point<float>(fk, rb, qt + t, k, B);
-                set_P(sk, k, v);
-                const float rk = norm_dyn3(P0(k) - v);
-                RP(k - 1) = max_std(prev, rk);
-                prev = rk;
-                rho = max_std(rho, rk);
-            }
+            d_out = d;
+            dq = d;
+            if (d < g.d_crit || status != kAdvanced) break;
         }
     }
-    for (int k = 0; k < D; ++k) qn[k] = Q(k);
-    a.status[c] = status;
+    float* qn = g.q_new + c * D;
+    for (int k = 0; k < D; ++k) qn[k] = s.Q(k);
+    g.status[c] = status;
+    g.d_out[c] = d_out;
 }
 
 // After the distance of layer 'layer': the reference leaves the loop when
@@ -276,6 +367,15 @@ hipError_t launch_spines(const RobotDev& rb, const SpineArgs& a, int64_t n, hipS
     const int B = spine_block(rb);
     const size_t sm = spine_lds_per_lane(rb) * B;
     hipLaunchKernelGGL(k_spines, dim3((unsigned)((n + B - 1) / B)), dim3(B), sm, st, rb, a, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_spines_under(const RobotDev& rb, const GenSpineArgs& g, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const size_t per = spine_lds_per_lane(rb) + (size_t)rb.num_links * sizeof(float);
+    int B = 64;
+    while (B > 16 && per * B > 64 * 1024) B /= 2;
+    hipLaunchKernelGGL(k_gen_spines_under, dim3((unsigned)((n + B - 1) / B)), dim3(B), per * B, st, rb, g, n);
     return hipGetLastError();
 }
 

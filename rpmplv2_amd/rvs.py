@@ -109,7 +109,7 @@ EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error",
            "rvs_trajectory_is_safe", "rvs_self_distance",
            "rvs_self_collision", "rvs_enclosing_radii", "rvs_extend_spines", "rvs_extend_gen_spines",
            "rvs_edge_self_collision", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
-           "rvs_autotune", "rvs_set_tuning", "rvs_get_tuning", "rvs_autotune_edges", "rvs_set_edge_tuning", "rvs_server_default_options", "rvs_server_create", "rvs_server_destroy", "rvs_server_set_motion",
+           "rvs_autotune", "rvs_set_tuning", "rvs_get_tuning", "rvs_set_small_batch", "rvs_autotune_edges", "rvs_set_edge_tuning", "rvs_server_default_options", "rvs_server_create", "rvs_server_destroy", "rvs_server_set_motion",
            "rvs_server_submit", "rvs_server_wait", "rvs_server_get_stats", "rvs_server_now",
            "rvs_server_last_error", "rvs_host_alloc", "rvs_host_free"]
 
@@ -172,6 +172,7 @@ def lib():
         L.rvs_autotune_edges.argtypes = [ctx, i32, vp, vp, vp, vp, i64, i32, C.c_float, u32, C.POINTER(EdgeTuning)]
         L.rvs_set_edge_tuning.argtypes = [ctx, C.POINTER(EdgeTuning)]
         L.rvs_get_tuning.argtypes = [ctx, i32, i32, C.POINTER(Tuning)]
+        L.rvs_set_small_batch.argtypes = [ctx, i64]
         L.rvs_server_default_options.argtypes = [C.POINTER(ServerOptions)]
         L.rvs_server_default_options.restype = None
         L.rvs_server_create.argtypes = [ctx, C.POINTER(ServerOptions), C.POINTER(C.c_void_p)]
@@ -382,6 +383,10 @@ class Engine:
         out = np.zeros((len(q), D, D + 1), np.float32)
         self._check(lib().rvs_enclosing_radii(self._ctx, robot, _ptr(q), len(q), _ptr(out), 0), "rvs_enclosing_radii")
         return out
+
+    def set_small_batch(self, max_configs: int):
+        """Reference-arithmetic batches up to max_configs run one wave per configuration."""
+        self._check(lib().rvs_set_small_batch(self._ctx, int(max_configs)), "rvs_set_small_batch")
 
     def set_motion_law(self, env: int, motion_type: int, vel, max_vel, ws_center=(0.0, 0.0, 0.0),
                        ws_radius: float = 1.0, base_radius: float = 0.1, robot_max_vel: float = 1.0):

@@ -467,3 +467,22 @@ def test_env_motion_laws(oracle, engine, law):
     d1, p1, v1, _ = engine.distance(rid, eid, q)
     d2, p2, v2, _ = engine.distance(rid, engine.create_env(moved), q)
     assert (d1 == d2).all() and (p1 == p2).all() and (v1 == v2).all()
+
+
+@pytest.mark.parametrize("n", [1, 7, 300, 5000])
+def test_small_batches_one_wave_per_configuration(oracle, engine, n):
+    """Small reference-arithmetic batches run one wave per configuration;
+    results (distances, profiles, verdicts, nearest points) are those of the
+    one-lane-per-configuration kernel bit for bit."""
+    w = W.workload2(n)
+    rid = engine.load_robot(w.robot)
+    eid = engine.create_env(w.scene)
+    engine.set_small_batch(1 << 20)
+    a = engine.distance(rid, eid, w.q, want_nearest=True)
+    engine.set_small_batch(0)
+    b = engine.distance(rid, eid, w.q, want_nearest=True)
+    engine.set_small_batch(16384)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y, equal_nan=True)
+    od, oprof, over, onp = oracle.distance(w.robot, oracle.scene_of(w.scene), w.q, 32, want_np=True)
+    assert (a[2] == over).all() and ulps(a[0], od).max() <= 2
