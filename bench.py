@@ -56,7 +56,7 @@ PEAK_HBM_GBPS = 8000.0
 # two `rocprofv3 --pmc` runs: FETCH_SIZE 12,586.2 KiB x 2.0 + WRITE_SIZE
 # 57,044.2 KiB x 1.0, the factors being what the same counters read for a
 # 256 MiB copy (profiles/k_distance_fast_xarm6_1M.md).
-TRAFFIC_BYTES = (2.0 * 12586.2 + 1.0 * 57044.2) * 1024
+TRAFFIC_BYTES = (2.0 * 12620.8 + 58652.1) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
 
 
 def pair_count(rb, scene) -> int:
