@@ -965,6 +965,8 @@ rvs_status rvs_trajectory_is_safe(rvs_context* c, int32_t robot, int32_t env, co
     const EnvRec& ev = c->envs[env];
     if (rb.num_links != rb.num_dofs)
         return fail(c, RVS_ENOTSUP, "rvs_trajectory_is_safe: needs one link per revolute joint");
+    if (traj_lds_bytes(rb.dev) > kLdsLimit) return fail(c, RVS_ERANGE, "rvs_trajectory_is_safe: robot too long");
+    if (num_traj > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_trajectory_is_safe: too many trajectories");
     if (num_traj == 0) return RVS_OK;
     (void)hipSetDevice(c->opt.device);
     const QueryParams qp = params_of(c);

@@ -233,6 +233,7 @@ struct TrajArgs {
     uint8_t* safe;
 };
 hipError_t launch_traj_safe(const RobotDev& rb, const QueryParams& qp, const TrajArgs& a, int64_t T, hipStream_t st);
+size_t traj_lds_bytes(const RobotDev& rb);   // LDS of one trajectory's workgroup
 // Capsule part of xArm6::checkSelfCollision(q1, q2) per edge: collide[e] =
 // 1 (q_out = last state before the hit), 0 (q_out = q2) or 255 (the walk
 // would not end: non-finite or huge coordinates).
