@@ -294,6 +294,68 @@ def spines_section(eng, cores: int) -> dict:
     return out
 
 
+def bubble_section(eng, cores: int) -> dict:
+    """Trajectory safety (8,192 straight 60-point trajectories, obstacles at
+    0.1 m/s) and xArm6 edge self-collision (16,384 edges), timed on the GPU
+    and on the oracle."""
+    import numpy as np
+    from rpmplv2_amd import robots as R, workloads as W
+    O = oracle_module()
+    out = {}
+    w = W.workload2(64)
+    keep = w.scene.is_ground == 0
+    scene = W.Scene(w.scene.dim[keep], w.scene.pos[keep], w.scene.quat[keep], w.scene.is_ground[keep],
+                    w.scene.min_dist_tol[keep])
+    rid, eid = eng.load_robot(w.robot), eng.create_env(scene)
+    root = np.zeros(6, np.float32)
+    _, p0, _, np0 = eng.distance(rid, eid, root[None], want_nearest=True)
+    rng = np.random.default_rng(82)
+    T, K = 8192, 60
+    u = rng.normal(size=(T, 6))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    s = np.linspace(0, 1, K + 1)[1:]
+    pts = (rng.uniform(0.1, 2.5, T)[:, None, None] * s[None, :, None] * u[:, None, :]).astype(np.float32)
+    mv = np.full(scene.num_obj, 0.1, np.float32)
+    eng.trajectory_is_safe(rid, eid, pts, root, p0[0], np0[0], mv, self_collision=True)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        safe = eng.trajectory_is_safe(rid, eid, pts, root, p0[0], np0[0], mv, self_collision=True)
+    ms = (time.perf_counter() - t0) * 1e3 / 5
+    sc = O.scene_of(scene)
+    t1 = time.perf_counter()
+    osafe = O.traj_safe(w.robot, sc, pts, root, p0[0], np0[0], mv, pairs=w.robot.self_pairs, nthreads=cores)
+    cpu_s = time.perf_counter() - t1
+    out["trajectories"] = {"workload": "xarm6_16aabb_8192traj_60pts", "trajectories": T, "points": K,
+                           "ms": ms, "trajectories_per_s": T / (ms * 1e-3), "safe": int(safe.sum()),
+                           "mismatch_vs_oracle": int((safe != osafe).sum()),
+                           "cpu_baseline": {"value": T / cpu_s, "unit": "trajectories/s", "cores": cores,
+                                            "kind": "port", "sample": "the same 8192 trajectories"},
+                           "note": "host arrays: includes the 11.8 MB point upload"}
+    rb = R.xarm6(radius=0.05)
+    lim = np.asarray(rb.limits, np.float64)
+    rng = np.random.default_rng(61)
+    q1 = (lim[:, 0] + rng.random((16384, 6)) * (lim[:, 1] - lim[:, 0])).astype(np.float32)
+    u = rng.normal(size=q1.shape)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    q2 = (q1 + rng.uniform(0.05, 3.0, len(q1))[:, None] * u).astype(np.float32)
+    rid = eng.load_robot(rb)
+    eng.edge_self_collision(rid, q1, q2, 0.1)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        col, qo = eng.edge_self_collision(rid, q1, q2, 0.1)
+    ms = (time.perf_counter() - t0) * 1e3 / 5
+    t1 = time.perf_counter()
+    ocol, oqo = O.edge_self_collision(rb, q1, q2, 0.1, nthreads=cores)
+    cpu_s = time.perf_counter() - t1
+    out["edge_self_collision"] = {"workload": "xarm6_16384edges_0.05-3rad_eps0.1", "edges": len(q1), "ms": ms,
+                                  "edges_per_s": len(q1) / (ms * 1e-3), "colliding": int(col.sum()),
+                                  "mismatch_vs_oracle": int(((col != ocol) | (qo != oqo).any(1)).sum()),
+                                  "cpu_baseline": {"value": len(q1) / cpu_s, "unit": "edges/s", "cores": cores,
+                                                   "kind": "port", "sample": "the same 16384 edges"},
+                                  "note": "host arrays: includes the uploads and read-back"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -399,6 +461,8 @@ def main():
         out["edges"] = edges_section(eng, cores)
         print("bench: spines", file=sys.stderr, flush=True)
         out["spines"] = spines_section(eng, cores)
+        print("bench: trajectories, edge self-collision", file=sys.stderr, flush=True)
+        out["bubble"] = bubble_section(eng, cores)
         eng.close()
         print("bench: serving", file=sys.stderr, flush=True)
         from rpmplv2_amd import serving
