@@ -147,6 +147,64 @@ class RealVectorSpaceGPU {
         return free_flag == 0;
     }
 
+    // AbstractRobot::computeEnclosingRadii(q): R [dofs][dofs+1], row-major
+    // (row i = starting skeleton point, column j = final point).
+    std::vector<float> computeEnclosingRadii(const std::shared_ptr<State>& q) {
+        std::vector<float> R((size_t)num_dofs_ * (num_dofs_ + 1));
+        check(rvs_enclosing_radii(ctx_, robot_, coords(q), 1, R.data(), 0), "rvs_enclosing_radii");
+        return R;
+    }
+
+    // RBTConnect::extendSpine(q, q_e) (RBTConnect.cpp:85-150) for a robot
+    // without self-collision checking: (status RVS_SPINE_*, q_new); q gets its
+    // distance first when it has none (:88-89); a trapped spine returns q.
+    std::pair<int32_t, std::shared_ptr<State>> extendSpine(const std::shared_ptr<State>& q,
+                                                           const std::shared_ptr<State>& q_e, int32_t num_iter = 5,
+                                                           bool use_expanded_bubble = true) {
+        if (q->getDistance() < 0) computeDistance(q);
+        const float d = q->getDistance();
+        std::vector<float> prof = q->getDistanceProfile();
+        prof.resize((size_t)num_links_, 0.0f);
+        std::vector<float> qn((size_t)num_dofs_);
+        int32_t st = 0;
+        check(rvs_extend_spines(ctx_, robot_, coords(q), &d, prof.data(), coords(q_e), 1, num_iter,
+                                use_expanded_bubble ? 1 : 0, qn.data(), &st, 0),
+              "rvs_extend_spines");
+        if (st == RVS_SPINE_TRAPPED) return {st, q};
+        return {st, std::make_shared<State>(std::move(qn))};
+    }
+
+    // Capsule part of xArm6::checkSelfCollision(q1, q2) (xArm6.cpp:284-368):
+    // true on a capsule overlap along the edge, with q2 replaced by the last
+    // state before it (the reference's modified q2).
+    bool checkSelfCollision(const std::shared_ptr<State>& q1, std::shared_ptr<State>& q2, float eps_step = 0.1f) {
+        uint8_t hit = 0;
+        std::vector<float> out((size_t)num_dofs_);
+        check(rvs_edge_self_collision(ctx_, robot_, coords(q1), coords(q2), 1, eps_step, &hit, out.data(), 0),
+              "rvs_edge_self_collision");
+        if (hit == 255) throw Error(RVS_ERANGE, "checkSelfCollision: the edge walk would not end");
+        if (hit) q2 = std::make_shared<State>(std::move(out));
+        return hit != 0;
+    }
+
+    // AbstractTrajectory::isSafe(pos_points, q_current, t_iter, time_step)
+    // (AbstractTrajectory.cpp:212-256) for one trajectory of num_points
+    // positions [num_points][dofs]; q_current needs its real distance
+    // (computed here when missing) and max_vel holds one speed per obstacle.
+    bool isSafe(const float* points, int32_t num_points, const std::shared_ptr<State>& q_current,
+                const std::vector<float>& max_vel, float t_iter, float time_step = 0.01f,
+                bool self_collision = false) {
+        if ((int32_t)max_vel.size() != num_objects_) throw Error(RVS_EINVAL, "isSafe: one max_vel per obstacle");
+        if (!q_current->getNearestPoints() || !q_current->getIsRealDistance()) computeDistance(q_current, true);
+        if (!q_current->getNearestPoints()) return false;   // the current state is in collision
+        uint8_t safe = 0;
+        check(rvs_trajectory_is_safe(ctx_, robot_, env_, points, 1, num_points, coords(q_current),
+                                     q_current->getDistanceProfile().data(), q_current->getNearestPoints()->data(),
+                                     max_vel.data(), t_iter, time_step, self_collision ? 1 : 0, &safe, 0),
+              "rvs_trajectory_is_safe");
+        return safe != 0;
+    }
+
     // ---------------------------------------------------- batches
     // Arrays are host memory, or device memory with RVS_F_DEVICE in flags.
     void isValid(const float* q, int64_t n, uint8_t* valid, uint32_t flags = 0) {
@@ -169,6 +227,40 @@ class RealVectorSpaceGPU {
     }
     void checkSelfCollision(const float* q, int64_t n, float* min_dist, uint8_t* free_flag, uint32_t flags = 0) {
         check(rvs_self_collision(ctx_, robot_, q, n, min_dist, free_flag, flags), "rvs_self_collision");
+    }
+
+    void computeEnclosingRadii(const float* q, int64_t n, float* R, uint32_t flags = 0) {
+        check(rvs_enclosing_radii(ctx_, robot_, q, n, R, flags), "rvs_enclosing_radii");
+    }
+    void extendSpines(const float* q, const float* d, const float* profile, const float* q_e, int64_t n,
+                      int32_t num_iter, bool use_expanded_bubble, float* q_new, int32_t* status, uint32_t flags = 0) {
+        check(rvs_extend_spines(ctx_, robot_, q, d, profile, q_e, n, num_iter, use_expanded_bubble ? 1 : 0, q_new,
+                                status, flags),
+              "rvs_extend_spines");
+    }
+    void extendGenSpines(const float* q_root, const float* q_e, int64_t n, int32_t num_layers, int32_t num_iter,
+                         bool use_expanded_bubble, bool use_real_dist, float d_crit, float* q_new, int32_t* status,
+                         float* d, uint32_t flags = 0) {
+        check(rvs_extend_gen_spines(ctx_, robot_, env_, q_root, q_e, n, num_layers, num_iter,
+                                    use_expanded_bubble ? 1 : 0, use_real_dist ? 1 : 0, d_crit, q_new, status, d,
+                                    flags),
+              "rvs_extend_gen_spines");
+    }
+    void checkSelfCollision(const float* q1, const float* q2, int64_t n, float eps_step, uint8_t* collide,
+                            float* q2_out, uint32_t flags = 0) {
+        check(rvs_edge_self_collision(ctx_, robot_, q1, q2, n, eps_step, collide, q2_out, flags),
+              "rvs_edge_self_collision");
+    }
+
+    // Environment::updateEnvironment(delta_time), steps times, with the
+    // motion law set by setMotionLaw; pos / vel [M][3] may be null.
+    void setMotionLaw(const rvs_env_motion& law, const std::vector<float>& vel, const std::vector<float>& max_vel) {
+        if (vel.size() != (size_t)num_objects_ * 3 || max_vel.size() != (size_t)num_objects_)
+            throw Error(RVS_EINVAL, "setMotionLaw: vel [M][3] and max_vel [M]");
+        check(rvs_env_set_motion_law(ctx_, env_, &law, vel.data(), max_vel.data()), "rvs_env_set_motion_law");
+    }
+    void updateEnvironment(float delta_time, int32_t steps = 1, float* pos = nullptr, float* vel = nullptr) {
+        check(rvs_env_step(ctx_, env_, delta_time, steps, pos, vel), "rvs_env_step");
     }
 
     // Move obstacles in place (Environment::updateEnvironment's position

@@ -93,6 +93,45 @@ int main(int argc, char** argv) {
     expect(std::memcmp(bd.data(), d.data(), n * sizeof(float)) == 0, "batch distance == single-state distance");
     expect(std::memcmp(bprof.data(), prof.data(), prof.size() * sizeof(float)) == 0, "batch profile == single-state");
     expect(std::memcmp(bvalid.data(), valid.data(), (size_t)n) == 0, "batch isValid == single-state isValid");
+    // bubble routines: single-state methods agree with the batched calls
+    if (L == D) {
+        std::vector<float> R1, Rb((size_t)n * D * (D + 1));
+        ss.computeEnclosingRadii(q.data(), n, Rb.data());
+        std::vector<float> qe((size_t)n * D), qn_b((size_t)n * D);
+        std::vector<int32_t> st_b(n);
+        for (int64_t k = 0; k < n; ++k)
+            for (int i = 0; i < D; ++i) qe[k * D + i] = q[((k + 1) % n) * D + i];
+        ss.extendSpines(q.data(), bd.data(), bprof.data(), qe.data(), n, 1, true, qn_b.data(), st_b.data());
+        bool radii_ok = true, spine_ok = true;
+        for (int64_t k = 0; k < n; ++k) {
+            auto s = std::make_shared<rvs::State>(std::vector<float>(q.begin() + k * D, q.begin() + (k + 1) * D));
+            R1 = ss.computeEnclosingRadii(s);
+            radii_ok = radii_ok && std::memcmp(R1.data(), Rb.data() + k * D * (D + 1), R1.size() * 4) == 0;
+            auto e = std::make_shared<rvs::State>(std::vector<float>(qe.begin() + k * D, qe.begin() + (k + 1) * D));
+            auto res = ss.extendSpine(s, e, 1, true);   // computes s's distance first
+            spine_ok = spine_ok && res.first == st_b[k] &&
+                       std::memcmp(res.second->getCoord().data(), qn_b.data() + k * D, D * 4) == 0;
+            if (res.first == RVS_SPINE_TRAPPED) spine_ok = spine_ok && res.second == s;
+        }
+        expect(radii_ok, "computeEnclosingRadii(state) == batched radii");
+        expect(spine_ok, "extendSpine(state) == batched spines");
+    }
+    if (kind == RVS_SKEL_XARM6 && D == 6) {
+        std::vector<uint8_t> col(n);
+        std::vector<float> q2o((size_t)n * D);
+        std::vector<float> q2((size_t)n * D);
+        for (int64_t k = 0; k < n; ++k)
+            for (int i = 0; i < D; ++i) q2[k * D + i] = q[((k + 1) % n) * D + i];
+        ss.checkSelfCollision(q.data(), q2.data(), n, 0.1f, col.data(), q2o.data());
+        bool ok = true;
+        for (int64_t k = 0; k < n; ++k) {
+            auto a = std::make_shared<rvs::State>(std::vector<float>(q.begin() + k * D, q.begin() + (k + 1) * D));
+            auto b = std::make_shared<rvs::State>(std::vector<float>(q2.begin() + k * D, q2.begin() + (k + 1) * D));
+            const bool hit = ss.checkSelfCollision(a, b, 0.1f);
+            ok = ok && hit == (col[k] == 1) && std::memcmp(b->getCoord().data(), q2o.data() + k * D, D * 4) == 0;
+        }
+        expect(ok, "checkSelfCollision(q1, q2) == batched edge self-collision");
+    }
     // errors surface as rvs::Error
     bool threw = false;
     try {
