@@ -56,7 +56,7 @@ PEAK_HBM_GBPS = 8000.0
 # two `rocprofv3 --pmc` runs: FETCH_SIZE 12,586.2 KiB x 2.0 + WRITE_SIZE
 # 57,044.2 KiB x 1.0, the factors being what the same counters read for a
 # 256 MiB copy (profiles/k_distance_fast_xarm6_1M.md).
-TRAFFIC_BYTES = (2.0 * 12620.8 + 58652.1) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
+TRAFFIC_BYTES = (2.0 * 127854.1 + 123701.4) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
 
 
 def pair_count(rb, scene) -> int:
@@ -120,13 +120,15 @@ def timed_distance(eng, rid, eid, q, d, prof, ver, prec, steps, warmup):
         refined.append(eng.last_refined())
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
-    bulk, refine = [], []
+    bulk, refine, order = [], [], []
     for _ in range(max(3, steps // 4) if steps > 1 else 1):
         eng.distance_device(rid, eid, q, d, prof, ver, precision=prec, timing=True)
         st = eng.last_timing()
         bulk.append(st[0])
         refine.append(st[1] if len(st) > 1 else 0.0)
+        order.append(st[2] if len(st) > 2 else 0.0)
     torch.cuda.synchronize()
+    timed_distance.order_ms = float(np.mean(order))
     return ms, float(np.mean(bulk)), float(np.mean(refine)), int(np.mean(refined))
 
 
@@ -390,6 +392,7 @@ def main():
 
     ms_step, bulk_ms, refine_ms, refined = timed_distance(eng, rid, eid, q, d, prof, ver, rvs.PREC_FP32_REFINE,
                                                           args.steps, args.warmup)
+    order_ms = timed_distance.order_ms
     value = n / (ms_step * 1e-3)
     d_gpu, prof_gpu, ver_gpu = d.cpu().numpy(), prof.cpu().numpy(), ver.cpu().numpy()
     ms_ref32, k_ref32, _, _ = timed_distance(eng, rid, eid, q, d, prof, ver, rvs.PREC_REF32, max(3, args.steps // 2), 2)
@@ -436,6 +439,8 @@ def main():
         "tuning": tune,
         "refine": {"configs": refined, "fraction": refined / n, "kernel": "k_distance_refine",
                    "kernel_ms": refine_ms},
+        "walk_order": {"kernels": "k_coh_hist, k_coh_scan, k_coh_scatter (bucket by the first three joint angles)",
+                       "ms": order_ms},
         "ref32_exact": {"configs_per_s": n / (ms_ref32 * 1e-3), "kernel_ms": k_ref32},
         "fp64": {"configs_per_s": n / (ms_fp64 * 1e-3), "kernel_ms": k_fp64},
     }

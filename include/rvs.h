@@ -330,6 +330,10 @@ typedef struct rvs_tuning {
     int32_t variant;        /* 0: one lane per configuration; 1: persistent lane refill */
     int32_t block;          /* threads per workgroup: 64, 128 or 256 */
     int32_t blocks_per_cu;  /* lane refill: resident workgroups per CU (0 = as many as fit) */
+    int32_t coherence_bits; /* walk order (0..18): configurations sorted by their first
+                               min(dofs, 18 / bits) joint angles, 2^bits bins each, so that a wave's
+                               lanes hold similar poses; 0 = the order given.  Results do not depend
+                               on it. */
     float ms;               /* bulk-pass time on the sample (rvs_autotune) */
 } rvs_tuning;
 
@@ -465,8 +469,9 @@ int64_t rvs_last_refined(const rvs_context* ctx);
 /* Stage durations of the last call made with RVS_F_TIMING, in ms, measured
  * with HIP events on the context's stream.  rvs_distance reports
  * [0] the bulk kernel (k_distance_fast for RVS_PREC_FP32_REFINE, the exact
- * kernel otherwise) and [1] the refinement kernel (0 when there is none).
- * Returns the number of stages written. */
+ * kernel otherwise), [1] the refinement kernel (0 when there is none) and,
+ * for RVS_PREC_FP32_REFINE, [2] the walk-order kernels (coherence_bits of
+ * rvs_tuning; 0 when off).  Returns the number of stages written. */
 int32_t rvs_last_timing(const rvs_context* ctx, float* stage_ms, int32_t max_stages);
 /* Library version string. */
 const char* rvs_version(void);

@@ -704,7 +704,28 @@ struct CullCount {
     int64_t evaluated = 0, culled = 0;
 };
 
-// computeDistance: RealVectorSpace.cpp:318-392 (fresh state, compute_again)
+// Filtered distance of one (link, obstacle) pair as the reference's loop
+// computes it (ground skipped for the first ground_included links, values
+// beyond the obstacle's tolerance replaced by MAX_DISTANCE).
+template <typename T>
+T pair_filtered(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, const T* A, const T* B, int i, int j) {
+    T np[6];
+    int v = 1;
+    T d_tmp;
+    if (sc.ground[j] && i < rb->ground_included) d_tmp = pr.maxd;
+    else if (!sc.obb.empty()) d_tmp = capsule_obb(A, B, (T)rb->radius[i], &sc.obb[15 * j], pr, np, &v);
+    else d_tmp = capsule_box(A, B, (T)rb->radius[i], &sc.obs[6 * j], pr, np, &v, nullptr);
+    if (d_tmp > sc.tol[j]) d_tmp = pr.maxd;
+    return d_tmp;
+}
+
+// computeDistance: RealVectorSpace.cpp:318-392 (fresh state, compute_again).
+// With cull (and no nearest points) this is the engine's bulk-pass walk
+// instead: pairs whose bounding-box gap exceeds max(running link minimum + r,
+// EQUALITY_THRESHOLD) + 1e-5 are skipped, and each link first evaluates the
+// obstacle that held the previous link's minimum (warm start); if that pair
+// is a hit, the pairs before it (gap bound of a running minimum of 0) are
+// searched for an earlier hit.  Its results must equal the plain loop's.
 template <typename T>
 void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, const T* Pk,
                   T* d_out, T* prof, T* np_out /* M*L*6 or null */, uint8_t* verdict,
@@ -713,16 +734,49 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
     T d_c = pr.maxd;
     for (int i = 0; i < L; ++i) prof[i] = 0;
     if (np_out) for (int k = 0; k < sc.M * L * 6; ++k) np_out[k] = std::numeric_limits<T>::quiet_NaN();
+    int warm = -1;
     for (int i = 0; i < L; ++i) {
         prof[i] = pr.maxd;
         const T* A = Pk + 3 * i;
         const T* B = Pk + 3 * (i + 1);
         T r = (T)rb->radius[i];
+        int best = -1;
+        const bool walk = cull && !np_out;
+        if (walk && warm >= 0) {
+            cull->evaluated++;
+            const T dw = pair_filtered(rb, pr, sc, A, B, i, warm);
+            if (dw <= 0) {
+                T first = dw;
+                for (int j = 0; j < warm; ++j) {
+                    T lo[3], hi[3];
+                    cull_box(sc, j, lo, hi);
+                    if (seg_box_gap(A, B, lo, hi) > std::max(T(0) + r, pr.eq) + T(1e-5)) {
+                        cull->culled++;
+                        continue;
+                    }
+                    cull->evaluated++;
+                    const T dj = pair_filtered(rb, pr, sc, A, B, i, j);
+                    if (dj <= 0) {
+                        first = dj;
+                        break;
+                    }
+                }
+                prof[i] = first;
+                *d_out = 0;
+                *verdict = 0;
+                return;
+            }
+            if (dw < prof[i]) {
+                prof[i] = dw;
+                best = warm;
+            }
+        }
         for (int j = 0; j < sc.M; ++j) {
             T d_tmp;
             T np[6];
             int v = 1;
-            if (cull && !np_out) {
+            if (walk) {
+                if (j == warm) continue;
                 T lo[3], hi[3];
                 cull_box(sc, j, lo, hi);
                 if (seg_box_gap(A, B, lo, hi) > std::max(prof[i] + r, pr.eq) + T(1e-5)) {
@@ -730,6 +784,15 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
                     continue;
                 }
                 cull->evaluated++;
+                d_tmp = pair_filtered(rb, pr, sc, A, B, i, j);
+                if (d_tmp < prof[i]) best = j;
+                prof[i] = stdmin(prof[i], d_tmp);
+                if (prof[i] <= 0) {
+                    *d_out = 0;
+                    *verdict = 0;
+                    return;
+                }
+                continue;
             }
             if (sc.ground[j] && i < rb->ground_included) {
                 d_tmp = pr.maxd;
@@ -749,6 +812,7 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
             }
             if (np_out) std::memcpy(np_out + 6 * (j * L + i), np, sizeof(np));
         }
+        warm = best;
         d_c = stdmin(d_c, prof[i]);
     }
     *d_out = d_c;

@@ -202,7 +202,7 @@ rvs_status rvs_create(const rvs_options* opt, rvs_context** out) {
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, c->opt.device);
     if (const char* v = std::getenv("RVS_FAST_VARIANT")) c->fast_variant = std::atoi(v);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
     if (e != hipSuccess) { delete c; return RVS_EDEVICE; }
     *out = c;
     return RVS_OK;
@@ -217,7 +217,7 @@ void rvs_destroy(rvs_context* c) {
     for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     if (c->count_host) (void)hipHostFree(c->count_host);
     c->arena.release();
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < 4; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -538,7 +538,7 @@ constexpr size_t kMaxGraphs = 8;
 
 hipError_t fp32_graph_run(rvs_context* c, int32_t robot, int32_t env, const RobotDev& rb, const EnvDev& ev,
                           const QueryParams& qp, const float* q, int64_t n, const DistOut& out, int32_t* ridx,
-                          uint32_t* rcount, const FastTuning& ft, uint32_t* refined) {
+                          uint32_t* rcount, uint32_t* coh, const FastTuning& ft, uint32_t* refined) {
     hipError_t e = hipSuccess;
     if (!c->count_host) {
         e = hipHostMalloc(reinterpret_cast<void**>(&c->count_host), 256, hipHostMallocDefault);
@@ -548,7 +548,7 @@ hipError_t fp32_graph_run(rvs_context* c, int32_t robot, int32_t env, const Robo
     for (auto& g : c->graphs)
         if (g.robot == robot && g.env == env && g.q == q && g.d == out.d && g.profile == out.profile &&
             g.verdict == out.verdict && g.arena == c->arena.p && g.n == n && g.t.variant == ft.variant &&
-            g.t.block == ft.block && g.t.blocks_per_cu == ft.blocks_per_cu) {
+            g.t.block == ft.block && g.t.blocks_per_cu == ft.blocks_per_cu && g.t.coh_bits == ft.coh_bits) {
             hit = &g;
             break;
         }
@@ -565,7 +565,10 @@ hipError_t fp32_graph_run(rvs_context* c, int32_t robot, int32_t env, const Robo
         e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return e;
         hipError_t le = launch_zero_u32(rcount, 64, c->stream);
-        if (le == hipSuccess) le = launch_distance_fast(rb, ev, qp, q, n, out, ridx, rcount, ft, c->num_cus, c->stream);
+        const int32_t* perm = nullptr;
+        if (le == hipSuccess) le = launch_coherence_order(rb, q, n, ft, coh, &perm, c->stream);
+        if (le == hipSuccess)
+            le = launch_distance_fast(rb, ev, qp, q, n, out, ridx, rcount, ft, perm, c->num_cus, c->stream);
         if (le == hipSuccess) le = launch_distance_refine(rb, ev, qp, q, n, out, ridx, rcount, c->stream);
         if (le == hipSuccess)
             le = hipMemcpyAsync(c->count_host, rcount, 4, hipMemcpyDeviceToHost, c->stream);
@@ -610,22 +613,26 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     hipError_t e = hipSuccess;
     // refinement list (FP32_REFINE) lives at the start of the arena
     const size_t refine_bytes = (prec == RVS_PREC_FP32_REFINE) ? (size_t)n * 4 + 256 : 0;
+    const size_t coh_need = (prec == RVS_PREC_FP32_REFINE) ? coh_bytes(n, D) : 0;
     int32_t* ridx = nullptr;
     uint32_t* rcount = nullptr;
+    uint32_t* coh = nullptr;
     if (flags & RVS_F_DEVICE) {
-        e = c->arena.ensure(Carve::need({refine_bytes}));
+        e = c->arena.ensure(Carve::need({refine_bytes, coh_need}));
         if (e != hipSuccess) return hip_fail(c, e, "rvs_distance: staging");
         Carve cv(c->arena.p);
         if (refine_bytes) { rcount = cv.take<uint32_t>(64); ridx = cv.take<int32_t>(n); }
+        if (coh_need) coh = cv.take<uint32_t>(coh_need / 4);
         out.d = d; out.profile = profile; out.nearest = nearest; out.verdict = verdict;
     } else {
-        const size_t need = Carve::need({refine_bytes, (size_t)n * D * 4, (size_t)n * 4,
+        const size_t need = Carve::need({refine_bytes, coh_need, (size_t)n * D * 4, (size_t)n * 4,
                                          profile ? (size_t)n * L * 4 : 0, nearest ? (size_t)n * M * L * 24 : 0,
                                          (size_t)n});
         e = c->arena.ensure(need);
         if (e != hipSuccess) return hip_fail(c, e, "rvs_distance: staging");
         Carve cv(c->arena.p);
         if (refine_bytes) { rcount = cv.take<uint32_t>(64); ridx = cv.take<int32_t>(n); }
+        if (coh_need) coh = cv.take<uint32_t>(coh_need / 4);
         float* qs = cv.take<float>((size_t)n * D);
         out.d = cv.take<float>(n);
         out.profile = profile ? cv.take<float>((size_t)n * L) : nullptr;
@@ -641,13 +648,17 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
                            !(flags & RVS_F_EAGER);
         if (graph) {
             const FastTuning ft = c->tuning_for(robot, env, L, M);
-            e = fp32_graph_run(c, robot, env, rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, ft, &refined);
+            e = fp32_graph_run(c, robot, env, rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, coh, ft, &refined);
         } else if (prec == RVS_PREC_FP32_REFINE) {
             e = launch_zero_u32(rcount, 64, c->stream);
             const FastTuning ft = c->tuning_for(robot, env, L, M);
+            const int32_t* perm = nullptr;
+            if (e == hipSuccess && timing) e = hipEventRecord(c->ev[3], c->stream);
+            if (e == hipSuccess) e = launch_coherence_order(rb.dev, qd, n, ft, coh, &perm, c->stream);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
             if (e == hipSuccess)
-                e = launch_distance_fast(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, ft, c->num_cus, c->stream);
+                e = launch_distance_fast(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, ft, perm, c->num_cus,
+                                         c->stream);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
             if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, c->stream);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[2], c->stream);
@@ -670,10 +681,13 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     if (e != hipSuccess) return hip_fail(c, e, "rvs_distance");
     c->last_refined = (prec == RVS_PREC_FP32_REFINE) ? (int64_t)refined : 0;
     if (timing) {
-        c->nstages = 2;
-        c->stage_ms[0] = c->stage_ms[1] = 0;
+        c->nstages = prec == RVS_PREC_FP32_REFINE ? 3 : 2;
+        c->stage_ms[0] = c->stage_ms[1] = c->stage_ms[2] = 0;
         (void)hipEventElapsedTime(&c->stage_ms[0], c->ev[0], c->ev[1]);
-        if (prec == RVS_PREC_FP32_REFINE) (void)hipEventElapsedTime(&c->stage_ms[1], c->ev[1], c->ev[2]);
+        if (prec == RVS_PREC_FP32_REFINE) {
+            (void)hipEventElapsedTime(&c->stage_ms[1], c->ev[1], c->ev[2]);
+            (void)hipEventElapsedTime(&c->stage_ms[2], c->ev[3], c->ev[0]);
+        }
     }
     return RVS_OK;
 }
@@ -695,7 +709,7 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
     const int64_t m = std::min<int64_t>(n, 1 << 21);
     const int L = rb.num_links, D = rb.num_dofs;
     hipError_t e = c->arena.ensure(Carve::need({(size_t)m * D * 4, (size_t)m * 4, (size_t)m * L * 4, (size_t)m,
-                                                (size_t)m * 4, 256}));
+                                                (size_t)m * 4, 256, coh_bytes(m, D)}));
     if (e != hipSuccess) return hip_fail(c, e, "rvs_autotune: staging");
     Carve cv(c->arena.p);
     float* qs = cv.take<float>((size_t)m * D);
@@ -705,50 +719,69 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
     out.verdict = cv.take<uint8_t>(m);
     int32_t* idx = cv.take<int32_t>(m);
     uint32_t* count = cv.take<uint32_t>(64);
+    uint32_t* coh = cv.take<uint32_t>(coh_bytes(m, D) / 4);
     e = hipMemcpyAsync(qs, q, (size_t)m * D * 4, (flags & RVS_F_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                        c->stream);
     const QueryParams qp = params_of(c);
     // candidates: one lane per configuration at three workgroup sizes; lane
     // refill at three workgroup sizes, with the occupancy limit and half of it
     std::vector<FastTuning> cand;
-    for (int b : {64, 128, 256}) cand.push_back(FastTuning{0, b, 0});
-    for (int b : {64, 128, 256}) {
-        const int occ = std::max(1, fast_pool_blocks_per_cu(ev.dev, b));
-        cand.push_back(FastTuning{1, b, occ});
-        if (occ > 1) cand.push_back(FastTuning{1, b, (occ + 1) / 2});
+    for (int cb : {0, 5}) {   // walk order: as given, or sorted by the first joint angles
+        for (int b : {64, 128, 256}) cand.push_back(FastTuning{0, b, 0, cb});
+        for (int b : {64, 128, 256}) {
+            const int occ = std::max(1, fast_pool_blocks_per_cu(ev.dev, b));
+            cand.push_back(FastTuning{1, b, occ, cb});
+            if (occ > 1) cand.push_back(FastTuning{1, b, (occ + 1) / 2, cb});
+        }
     }
     FastTuning win{};
     float win_ms = 1e30f;
-    for (const FastTuning& t : cand) {
+    auto try_shape = [&](const FastTuning& t) -> hipError_t {
         float best_ms = 1e30f;
         for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {   // first run warms up
             e = launch_zero_u32(count, 64, c->stream);
+            const int32_t* perm = nullptr;
             if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->stream);
-            if (e == hipSuccess) e = launch_distance_fast(rb.dev, ev.dev, qp, qs, m, out, idx, count, t, c->num_cus, c->stream);
+            if (e == hipSuccess) e = launch_coherence_order(rb.dev, qs, m, t, coh, &perm, c->stream);
+            if (e == hipSuccess)
+                e = launch_distance_fast(rb.dev, ev.dev, qp, qs, m, out, idx, count, t, perm, c->num_cus, c->stream);
             if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->stream);
             if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
             float ms = 0;
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
             if (rep > 0) best_ms = std::min(best_ms, ms);
         }
-        if (e != hipSuccess) return hip_fail(c, e, "rvs_autotune");
-        if (best_ms < win_ms) { win_ms = best_ms; win = t; }
+        if (e == hipSuccess && best_ms < win_ms) {
+            win_ms = best_ms;
+            win = t;
+        }
+        return e;
+    };
+    for (const FastTuning& t : cand)
+        if (try_shape(t) != hipSuccess) return hip_fail(c, e, "rvs_autotune");
+    // then the winning shape with other walk-order key widths
+    if (win.coh_bits > 0) {
+        const FastTuning base = win;
+        for (int cb : {3, 4, 6})
+            if (try_shape(FastTuning{base.variant, base.block, base.blocks_per_cu, cb}) != hipSuccess)
+                return hip_fail(c, e, "rvs_autotune");
     }
     {
         std::lock_guard<std::mutex> tk(c->tuning_mu);
         c->tuning[{robot, env}] = win;
     }
-    if (best) *best = rvs_tuning{win.variant, win.block, win.blocks_per_cu, win_ms};
+    if (best) *best = rvs_tuning{win.variant, win.block, win.blocks_per_cu, win.coh_bits, win_ms};
     return RVS_OK;
 }
 
 rvs_status rvs_set_tuning(rvs_context* c, int32_t robot, int32_t env, const rvs_tuning* t) {
     if (!c || !t) return fail(c, RVS_EINVAL, "rvs_set_tuning: null argument");
     if ((t->variant != 0 && t->variant != 1) || (t->block != 64 && t->block != 128 && t->block != 256) ||
-        t->blocks_per_cu < 0)
-        return fail(c, RVS_EINVAL, "rvs_set_tuning: variant 0/1, block 64/128/256, blocks_per_cu >= 0");
+        t->blocks_per_cu < 0 || t->coherence_bits < 0 || t->coherence_bits > kCohKeyBits)
+        return fail(c, RVS_EINVAL,
+                    "rvs_set_tuning: variant 0/1, block 64/128/256, blocks_per_cu >= 0, coherence_bits 0..18");
     std::lock_guard<std::mutex> lk(c->tuning_mu);
-    c->tuning[{robot, env}] = FastTuning{t->variant, t->block, t->blocks_per_cu};
+    c->tuning[{robot, env}] = FastTuning{t->variant, t->block, t->blocks_per_cu, t->coherence_bits};
     return RVS_OK;
 }
 
@@ -769,7 +802,7 @@ rvs_status rvs_get_tuning(rvs_context* c, int32_t robot, int32_t env, rvs_tuning
         if (env_ok(c, env)) M = c->envs[env].dev.num_obj;
     }
     const FastTuning f = c->tuning_for(robot, env, L, M);
-    *t = rvs_tuning{f.variant, f.block, f.blocks_per_cu, 0.0f};
+    *t = rvs_tuning{f.variant, f.block, f.blocks_per_cu, f.coh_bits, 0.0f};
     return RVS_OK;
 }
 

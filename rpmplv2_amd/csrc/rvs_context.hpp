@@ -65,8 +65,8 @@ struct rvs_context {
     std::vector<rvs::EnvRec> envs;
     rvs::DevBuf arena;
     int64_t last_refined = 0;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    float stage_ms[2] = {0, 0};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    float stage_ms[3] = {0, 0, 0};
     int32_t nstages = 0;
     int32_t num_cus = 0;
     // FP32 bulk pass variant when no tuning is recorded: -1 = by walk length
@@ -97,7 +97,8 @@ struct rvs_context {
         auto it = tuning.find({robot, env});
         if (it != tuning.end()) return it->second;
         const int32_t v = fast_variant >= 0 ? fast_variant : ((int64_t)links * objects >= 1024 ? 1 : 0);
-        return rvs::FastTuning{v, 256, 0};
+        // one lane per configuration: 64-thread workgroups measured fastest
+        return rvs::FastTuning{v, v == 0 ? 64 : 256, 0, 5};
     }
 };
 

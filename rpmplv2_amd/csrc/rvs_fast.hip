@@ -68,14 +68,16 @@ struct FKf {
     int seg, qi;
 };
 
-__device__ __attribute__((always_inline)) inline void fkf_advance(FKf& s, const SegDev* seg, const float* q, int upto) {
+// Joint k of the configuration is q[k * qstride].
+__device__ __attribute__((always_inline)) inline void fkf_advance(FKf& s, const SegDev* seg, const float* q, int upto,
+                                                                 int qstride) {
     while (s.seg <= upto) {
         const SegDev& g = seg[s.seg];
         const Mat3f R = loadf(g.R);
         Mat3f SM;
         if (g.rev) {
             float st, ct;
-            sincosf(q[s.qi++], &st, &ct);
+            sincosf(q[(s.qi++) * qstride], &st, &ct);
             SM = mulf(rot2f(g.axis, st, ct), R);
         } else {
             SM = R;
@@ -94,10 +96,10 @@ __device__ __attribute__((always_inline)) inline void fkf_advance(FKf& s, const 
 }
 
 __device__ __attribute__((always_inline)) inline Vec3<float> skelf_point(FKf& s, const RobotDev& rb, const float* q,
-                                                                       int k) {
+                                                                       int k, int qstride = 1) {
     const PointDev pt = rb.pts[k];
     if (pt.frame < 0) return Vec3<float>{0.0f, 0.0f, 0.0f};
-    fkf_advance(s, rb.seg, q, pt.frame);
+    fkf_advance(s, rb.seg, q, pt.frame, qstride);
     const float w0 = pt.axis == 0 ? 1.0f : 0.0f, w1 = pt.axis == 1 ? 1.0f : 0.0f, w2 = pt.axis == 2 ? 1.0f : 0.0f;
     const Mat3f& M = s.M;
     const float ax = M.m0 * w0 + M.m1 * w1 + M.m2 * w2;
@@ -143,6 +145,70 @@ __device__ __attribute__((always_inline)) inline float fast_pair(const Staged<fl
     return dt;
 }
 
+// Smallest filtered distance of one link over the obstacles, with the
+// reference's early exit: the first pair in obstacle order whose distance is
+// <= 0 ends the link (hit) and its distance is the link's value.  The pair
+// that held the previous link's minimum (warm, -1 = none) is evaluated first:
+// neighbouring links are usually nearest to the same obstacle, so the running
+// minimum starts close to its final value and the broad phase skips most of
+// the other pairs.  If the warm pair is a hit, the pairs before it in
+// obstacle order are searched for an earlier one; a bound of running minimum
+// 0 never skips a pair whose exact distance can be <= 0.  Otherwise the
+// obstacles are walked in order as usual, so the result (the minimum, or the
+// first hit's distance) is the in-order one.  On return warm holds this
+// link's nearest obstacle.  Called by the whole wave; lanes with active false
+// evaluate nothing.  Both bulk kernels walk a configuration's pairs through
+// this function, so they evaluate the same pairs, flag the same
+// configurations for the exact pass and return the same bits.
+template <bool OBB>
+__device__ __attribute__((always_inline)) inline float link_min(const Staged<float>& s, const float* __restrict__ cull,
+                                                                int M, bool gnd_skip, const Vec3<float>& P0,
+                                                                const Vec3<float>& P1, float r, const Params<float>& p,
+                                                                float band, bool active, int& warm, bool& hit,
+                                                                bool& uncertain) {
+    Vec3<float> slo, shi;
+    seg_bounds(P0, P1, slo, shi);
+    float prof = p.maxd;
+    int best = -1;
+    const int w = active ? warm : -1;
+    hit = false;
+    if (w >= 0) {
+        const float dt = (s.gnd[w] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, w, P0, P1, r, p, band, uncertain);
+        if (dt <= 0) {
+            float first = dt;
+            const float b0 = cull_bound(0.0f, r, p.eq);
+            for (int j = 0; j < w; ++j) {
+                if (cull_pair(cull + 6 * j, slo, shi, b0)) continue;
+                const float dj = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+                if (dj <= 0) {
+                    first = dj;
+                    break;
+                }
+            }
+            prof = first;
+            hit = true;
+        } else if (dt < prof) {
+            prof = dt;
+            best = w;
+        }
+    }
+    bool go = active && !hit;
+    for (int j = 0; j < M; ++j) {
+        if (go && j != w && !cull_pair(cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
+            const float dt = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+            if (dt < prof) best = j;
+            prof = smin(prof, dt);
+            if (prof <= 0) {
+                hit = true;
+                go = false;
+            }
+        }
+        if (__ballot(go) == 0) break;
+    }
+    warm = best;
+    return prof;
+}
+
 // Wave-aggregated append of configuration c for every lane with flag set
 // (called by the whole wave).
 __device__ __attribute__((always_inline)) inline void append_flagged(bool flag, int64_t c, int32_t* idx,
@@ -161,17 +227,34 @@ __device__ __attribute__((always_inline)) inline void append_flagged(bool flag, 
     }
 }
 
-template <bool OBB>
+// With ROWS (a walk order is given) each lane copies its configuration's
+// joint values into its LDS slice in one burst before the walk and collects
+// its profile row there, writing it out in one burst at the end: the rows of
+// a wave's lanes are scattered in memory, and reading or writing them a
+// joint or a link at a time, far apart in time, costs a cache line each.
+template <bool OBB, bool ROWS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
-    uint32_t* count) {
+    uint32_t* count, const int32_t* __restrict__ perm, size_t rows_off) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<float> s = stage_env<float, OBB>(env, smem);
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    // with a walk order, lane g takes configuration perm[g]
+    const int64_t c = perm ? (int64_t)perm[g] : g;
     const int M = env.num_obj;
     const int L = rb.num_links;
-    const float* qc = q + c * rb.num_dofs;
+    const int D = rb.num_dofs;
+    const int B = blockDim.x, t = threadIdx.x;
+    float* qrow = reinterpret_cast<float*>(smem + rows_off) + t;   // [D] at stride B
+    float* prow = qrow + (size_t)D * B;                             // [L] at stride B
+    const float* qc = q + c * D;
+    int qs = 1;
+    if (ROWS) {
+        for (int k = 0; k < D; ++k) qrow[k * B] = qc[k];
+        qc = qrow;
+        qs = B;
+    }
     const Params<float> p{qp.eq, qp.maxd};
     const float band = qp.band;
 
@@ -180,31 +263,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     fk.px = fk.py = fk.pz = 0.0f;
     fk.seg = 0;
     fk.qi = 0;
-    Vec3<float> P0 = skelf_point(fk, rb, qc, 0);
+    Vec3<float> P0 = skelf_point(fk, rb, qc, 0, qs);
     float d = p.maxd;
     bool hit = false, uncertain = false;
-    int i = 0;
+    int i = 0, warm = -1;
     for (; i < L; ++i) {
-        const Vec3<float> P1 = skelf_point(fk, rb, qc, i + 1);
+        const Vec3<float> P1 = skelf_point(fk, rb, qc, i + 1, qs);
         if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
         const float r = rb.radius[i];
-        float prof = p.maxd;
-        Vec3<float> slo, shi;
-        seg_bounds(P0, P1, slo, shi);
-        for (int j = 0; j < M; ++j) {
-            if (cull_pair(env.cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) continue;
-            const float dt = (s.gnd[j] && i < rb.ground_included) ? p.maxd
-                                                                   : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
-            prof = smin(prof, dt);
-            if (prof <= 0) { hit = true; break; }
-        }
-        if (out.profile) out.profile[c * L + i] = prof;
+        const float prof = link_min<OBB>(s, env.cull, M, i < rb.ground_included, P0, P1, r, p, band, true, warm, hit,
+                                         uncertain);
+        if (ROWS) prow[i * B] = prof;
+        else if (out.profile) out.profile[c * L + i] = prof;
         if (hit) break;
         d = smin(d, prof);
         P0 = P1;
     }
-    for (int k = i + 1; k < L; ++k)
-        if (out.profile) out.profile[c * L + k] = 0.0f;
+    if (out.profile) {
+        if (ROWS) {
+            for (int k = 0; k < L; ++k) out.profile[c * L + k] = k <= i ? prow[k * B] : 0.0f;
+        } else {
+            for (int k = i + 1; k < L; ++k) out.profile[c * L + k] = 0.0f;
+        }
+    }
     if (hit) d = 0.0f;
     if (out.d) out.d[c] = d;
     if (out.verdict) out.verdict[c] = hit ? 0 : 1;
@@ -224,7 +305,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 template <bool OBB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_pool(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
-    uint32_t* count, uint32_t* next) {
+    uint32_t* count, uint32_t* next, const int32_t* __restrict__ perm) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<float> s = stage_env<float, OBB>(env, smem);
     const int M = env.num_obj;
@@ -237,6 +318,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     int64_t c = -1;           // configuration of this lane, -1 = none
     bool drained = false;     // the counter has passed n
     int i = 0;                // current link
+    int warm = -1;            // nearest obstacle of the previous link
     const float* qc = q;
     FKf fk;
     Vec3<float> P0{0.0f, 0.0f, 0.0f};
@@ -255,7 +337,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const uint64_t below = (lane == 0) ? 0ull : (nm & (~0ull >> (64 - lane)));
                 const int64_t cand = (int64_t)base + __popcll(below);
                 if (cand < n) {
-                    c = cand;
+                    c = perm ? (int64_t)perm[cand] : cand;
                     qc = q + c * D;
                     fk.M = Mat3f{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
                     fk.px = fk.py = fk.pz = 0.0f;
@@ -263,6 +345,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     fk.qi = 0;
                     P0 = skelf_point(fk, rb, qc, 0);
                     i = 0;
+                    warm = -1;
                     d = p.maxd;
                     uncertain = false;
                 } else {
@@ -279,19 +362,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
             r = rb.radius[i];
         }
-        const bool gnd_skip = i < rb.ground_included;
-        float prof = p.maxd;
-        bool active = busy, hit = false;
-        Vec3<float> slo, shi;
-        seg_bounds(P0, P1, slo, shi);
-        for (int j = 0; j < M; ++j) {
-            if (active && !cull_pair(env.cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
-                const float dt = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
-                prof = smin(prof, dt);
-                if (prof <= 0) { hit = true; active = false; }
-            }
-            if (__ballot(active) == 0) break;
-        }
+        bool hit = false;
+        const float prof = link_min<OBB>(s, env.cull, M, i < rb.ground_included, P0, P1, r, p, band, busy, warm, hit,
+                                         uncertain);
         bool finished = false;
         if (busy) {
             if (out.profile) out.profile[c * L + i] = prof;
@@ -317,6 +390,177 @@ __global__ void k_zero_u32(uint32_t* p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// Coherence order.  The key of a configuration is its first
+// min(dofs, 18 / bits) joint angles, each wrapped to [-pi, pi) and cut into
+// 2^bits bins (the first joint in the highest bits).  The configurations are sorted by key in two
+// passes without global atomics: a counting sort of tiles by the key's top
+// (up to) 8 bits into 256 buckets (tile histograms, a scan of each bucket's
+// row of tile counts, a scatter ranked by LDS atomics), then every chunk of at
+// most kCohChunk configurations of a bucket is sorted by the remaining low
+// bits (up to 10) in LDS.  Neighbours in the resulting order hold similar
+// poses of the base links, so the lanes of a wave take the same branches of
+// the case tree and the same broad-phase decisions more often.  The order
+// within a key depends on atomics; a configuration's result does not depend
+// on which lane computes it.
+__device__ inline uint32_t coh_key(const float* qc, int nj, int bits) {
+    const int nb = 1 << bits;
+    uint32_t key = 0;
+    for (int k = 0; k < nj; ++k) {
+        float t = (qc[k] + 3.14159265f) * 0.159154943f;   // revolutions from -pi
+        t = t - floorf(t);
+        int b = (t >= 0.0f && t < 1.0f) ? (int)(t * (float)nb) : 0;   // non-finite angles: bin 0
+        b = b < nb ? b : nb - 1;
+        key = key * (uint32_t)nb + (uint32_t)b;
+    }
+    return key;
+}
+
+// Exclusive prefix sum of v over the NT threads of a workgroup; total = the
+// sum over all of them.  Every thread of the workgroup must call it.
+template <int NT>
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t& total) {
+    constexpr int NW = NT / 64;
+    __shared__ uint32_t wsum[NW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const uint32_t sk = wsum[k];
+        before += k < w ? sk : 0u;
+        all += sk;
+    }
+    __syncthreads();
+    total = all;
+    return before + x - v;
+}
+
+constexpr int kCohThreads = 256;                      // threads of the tile kernels (one per bucket in a row)
+constexpr int kCohPer = kCohTile / kCohThreads;       // configurations per thread of a tile
+constexpr int kCohSortThreads = 1024;                 // threads of the low-bit sort (one per low key)
+static_assert(kCohBuckets == kCohThreads, "one thread per bucket");
+static_assert((1 << kCohLoBits) <= kCohSortThreads, "one thread per low key");
+
+// Pass 1: keys of a tile of kCohTile configurations and the histogram of
+// their buckets (key >> lo), stored bucket-major: hist[bucket * tiles + tile].
+__global__ __launch_bounds__(kCohThreads) void k_coh_hist(const float* __restrict__ q, int64_t n, int D, int nj,
+                                                          int bits, int lo, int64_t tiles, uint32_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[kCohBuckets];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kCohTile;
+    for (int k = 0; k < kCohPer; ++k) {
+        const int64_t c = base + k * kCohThreads + threadIdx.x;
+        if (c < n) {
+            const uint32_t key = coh_key(q + c * D, nj, bits);
+            keys[c] = key;
+            atomicAdd(&cnt[key >> lo], 1u);
+        }
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Pass 2: one workgroup per bucket turns its row of tile counts into
+// exclusive offsets within the bucket and stores the bucket's size.
+__global__ __launch_bounds__(kCohThreads) void k_coh_scan(uint32_t* __restrict__ hist, int64_t tiles,
+                                                          uint32_t* __restrict__ sizes) {
+    uint32_t* row = hist + (int64_t)blockIdx.x * tiles;
+    uint32_t carry = 0;
+    for (int64_t s0 = 0; s0 < tiles; s0 += kCohThreads) {
+        const int64_t i = s0 + threadIdx.x;
+        const uint32_t v = i < tiles ? row[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<kCohThreads>(v, tot);
+        if (i < tiles) row[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) sizes[blockIdx.x] = carry;
+}
+
+// Pass 3: every configuration of a tile goes to its bucket (bucket start +
+// the tile's offset in the bucket + its rank among the tile's members); the
+// low key bits travel with it.  With lo == 0 this is the final order.
+__global__ __launch_bounds__(kCohThreads) void k_coh_bucket(const uint32_t* __restrict__ keys, int64_t n, int lo,
+                                                            const uint32_t* __restrict__ hist, int64_t tiles,
+                                                            const uint32_t* __restrict__ sizes,
+                                                            uint16_t* __restrict__ key_lo, int32_t* __restrict__ dst) {
+    __shared__ uint32_t next[kCohBuckets];
+    uint32_t tot;
+    const uint32_t start = block_exclusive_scan<kCohThreads>(sizes[threadIdx.x], tot);
+    next[threadIdx.x] = start + hist[(int64_t)threadIdx.x * tiles + blockIdx.x];
+    __syncthreads();
+    const uint32_t mask = (1u << lo) - 1u;
+    const int64_t base = (int64_t)blockIdx.x * kCohTile;
+    for (int k = 0; k < kCohPer; ++k) {
+        const int64_t c = base + k * kCohThreads + threadIdx.x;
+        if (c < n) {
+            const uint32_t key = keys[c];
+            const uint32_t pos = atomicAdd(&next[key >> lo], 1u);
+            if (lo) key_lo[pos] = (uint16_t)(key & mask);
+            dst[pos] = (int32_t)c;
+        }
+    }
+}
+
+// Pass 4: workgroup j sorts chunk j of the buckets' chunks (each bucket cut
+// into chunks of at most kCohChunk configurations, buckets in order) by the
+// low key bits with an LDS counting sort.  Workgroups beyond the last chunk
+// return at once; the grid is 256 + ceil(n / kCohChunk), which bounds the
+// number of chunks.
+__global__ __launch_bounds__(kCohSortThreads) void k_coh_lowsort(const uint32_t* __restrict__ sizes, int lo,
+                                                                 const uint16_t* __restrict__ key_lo,
+                                                                 const int32_t* __restrict__ idx,
+                                                                 int32_t* __restrict__ perm) {
+    __shared__ uint32_t cnt[kCohSortThreads];
+    __shared__ uint32_t first_chunk[kCohBuckets + 1];
+    __shared__ uint32_t bstart[kCohBuckets], bsize[kCohBuckets];
+    const int t = threadIdx.x;
+    const uint32_t size = t < kCohBuckets ? sizes[t] : 0u;
+    const uint32_t chunks = (size + kCohChunk - 1) / kCohChunk;
+    uint32_t total, total_chunks;
+    const uint32_t start = block_exclusive_scan<kCohSortThreads>(size, total);
+    const uint32_t cfirst = block_exclusive_scan<kCohSortThreads>(chunks, total_chunks);
+    if (t < kCohBuckets) {
+        first_chunk[t] = cfirst;
+        bstart[t] = start;
+        bsize[t] = size;
+    }
+    if (t == 0) first_chunk[kCohBuckets] = total_chunks;
+    __syncthreads();
+    const uint32_t j = blockIdx.x;
+    if (j >= total_chunks) return;   // uniform over the workgroup
+    // the bucket of chunk j: the largest b with first_chunk[b] <= j (never empty)
+    int b = 0;
+    for (int step = kCohBuckets / 2; step > 0; step >>= 1)
+        if (first_chunk[b + step] <= j) b += step;
+    const uint32_t begin = bstart[b] + (j - first_chunk[b]) * (uint32_t)kCohChunk;
+    const uint32_t bend = bstart[b] + bsize[b];
+    const uint32_t end = bend < begin + (uint32_t)kCohChunk ? bend : begin + (uint32_t)kCohChunk;
+    const int nb = 1 << lo;
+    if (t < nb) cnt[t] = 0;
+    __syncthreads();
+    for (uint32_t i = begin + t; i < end; i += kCohSortThreads) atomicAdd(&cnt[key_lo[i]], 1u);
+    __syncthreads();
+    const uint32_t v = t < nb ? cnt[t] : 0u;
+    uint32_t dummy;
+    const uint32_t ex = block_exclusive_scan<kCohSortThreads>(v, dummy);
+    if (t < nb) cnt[t] = ex;
+    __syncthreads();
+    for (uint32_t i = begin + t; i < end; i += kCohSortThreads) {
+        const uint32_t pos = begin + atomicAdd(&cnt[key_lo[i]], 1u);
+        perm[pos] = idx[i];
+    }
+}
+
 }  // namespace
 
 int fast_pool_blocks_per_cu(const EnvDev& env, int block) {
@@ -334,9 +578,35 @@ hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n, const FastTuning& t, uint32_t* coh,
+                                  const int32_t** perm, hipStream_t st) {
+    *perm = nullptr;
+    const int bits = std::min<int>(t.coh_bits, kCohKeyBits);
+    if (n <= 0 || !coh || bits <= 0 || rb.num_dofs <= 0) return hipSuccess;
+    const int nj = std::min(rb.num_dofs, kCohKeyBits / bits);
+    if (n > INT32_MAX) return hipErrorInvalidValue;
+    const int keybits = nj * bits;
+    const int lo = std::max(0, keybits - kCohHiBits);
+    const CohLayout L = coh_layout(coh, n);
+    const unsigned tiles = (unsigned)L.tiles;
+    hipLaunchKernelGGL(k_coh_hist, dim3(tiles), dim3(kCohThreads), 0, st, q, n, rb.num_dofs, nj, bits, lo, L.tiles,
+                       L.keys, L.hist);
+    hipLaunchKernelGGL(k_coh_scan, dim3(kCohBuckets), dim3(kCohThreads), 0, st, L.hist, L.tiles, L.sizes);
+    hipLaunchKernelGGL(k_coh_bucket, dim3(tiles), dim3(kCohThreads), 0, st, L.keys, n, lo, L.hist, L.tiles, L.sizes,
+                       L.key_lo, lo ? L.idx : L.perm);
+    if (lo) {
+        const unsigned grid = (unsigned)(kCohBuckets + (n + kCohChunk - 1) / kCohChunk);
+        hipLaunchKernelGGL(k_coh_lowsort, dim3(grid), dim3(kCohSortThreads), 0, st, L.sizes, lo, L.key_lo, L.idx,
+                           L.perm);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) *perm = L.perm;
+    return e;
+}
+
 hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                 int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
-                                int num_cus, hipStream_t st) {
+                                const int32_t* perm, int num_cus, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const int B = (t.block == 64 || t.block == 128) ? t.block : kBlock;
     const size_t sm = staged_bytes<float>(env);
@@ -348,17 +618,32 @@ hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const Que
         const dim3 g((unsigned)std::max<int64_t>(1, blocks)), b(B);
         if (env.use_obb)
             hipLaunchKernelGGL(k_distance_fast_pool<true>, g, b, sm, st, rb, env, qp, q, n, out, idx, count,
-                               count + 32);
+                               count + 32, perm);
         else
             hipLaunchKernelGGL(k_distance_fast_pool<false>, g, b, sm, st, rb, env, qp, q, n, out, idx, count,
-                               count + 32);
+                               count + 32, perm);
         return hipGetLastError();
     }
     const dim3 g((unsigned)((n + B - 1) / B)), b(B);
-    if (env.use_obb)
-        hipLaunchKernelGGL(k_distance_fast<true>, g, b, sm, st, rb, env, qp, q, n, out, idx, count);
-    else
-        hipLaunchKernelGGL(k_distance_fast<false>, g, b, sm, st, rb, env, qp, q, n, out, idx, count);
+    // rows staged through LDS when a walk order scatters them and the slices fit
+    const size_t rows_off = (sm + 15) & ~size_t(15);
+    const size_t rows_sm = rows_off + (size_t)(rb.num_dofs + rb.num_links) * 4 * B;
+    const bool rows = perm && rows_sm <= 64 * 1024;
+    if (env.use_obb) {
+        if (rows)
+            hipLaunchKernelGGL((k_distance_fast<true, true>), g, b, rows_sm, st, rb, env, qp, q, n, out, idx, count,
+                               perm, rows_off);
+        else
+            hipLaunchKernelGGL((k_distance_fast<true, false>), g, b, sm, st, rb, env, qp, q, n, out, idx, count, perm,
+                               rows_off);
+    } else {
+        if (rows)
+            hipLaunchKernelGGL((k_distance_fast<false, true>), g, b, rows_sm, st, rb, env, qp, q, n, out, idx, count,
+                               perm, rows_off);
+        else
+            hipLaunchKernelGGL((k_distance_fast<false, false>), g, b, sm, st, rb, env, qp, q, n, out, idx, count,
+                               perm, rows_off);
+    }
     return hipGetLastError();
 }
 

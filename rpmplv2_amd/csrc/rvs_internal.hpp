@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "rvs_fk.hpp"
 
 namespace rvs {
@@ -78,11 +80,60 @@ struct FastTuning {
     int32_t variant;        // 0: one lane per configuration; 1: persistent lane refill
     int32_t block;          // threads per workgroup: 64, 128 or 256
     int32_t blocks_per_cu;  // lane refill: resident workgroups per CU, 0 = the occupancy limit
+    int32_t coh_bits;       // walk order: 2^bits bins per joint angle of the first 18 / bits joints, 0 = off
 };
+// Walk-order sort: keys of up to kCohKeyBits bits (bits per joint angle x
+// min(dofs, kCohKeyBits / bits) joints); the top kCohHiBits select one of kCohBuckets buckets,
+// the rest (at most kCohLoBits) order each chunk of a bucket.  (A sorted copy
+// of the configurations instead of the permutation's gather was measured
+// slower.)
+constexpr int kCohKeyBits = 18;
+constexpr int kCohHiBits = 8;
+constexpr int kCohLoBits = kCohKeyBits - kCohHiBits;
+constexpr int kCohBuckets = 1 << kCohHiBits;
+constexpr int kCohTile = 4096;     // configurations per workgroup of the bucket passes
+constexpr int kCohChunk = 16384;   // at most this many configurations per low-bit sort
+inline int64_t coh_tiles(int64_t n) { return (n + kCohTile - 1) / kCohTile; }
+inline size_t coh_words(size_t w) { return (w + 63) & ~size_t(63); }   // 256-byte pieces
+// Pieces of the walk-order scratch (256-byte aligned base).
+struct CohLayout {
+    uint32_t* hist;     // kCohBuckets x tiles tile counts, then offsets
+    uint32_t* sizes;    // kCohBuckets bucket sizes
+    uint32_t* keys;     // n keys
+    int32_t* idx;       // n configuration indices in bucket order
+    uint16_t* key_lo;   // n low key bits in bucket order
+    int32_t* perm;      // n: the walk order
+    int64_t tiles;
+};
+inline size_t coh_bytes(int64_t n, int dofs) {
+    (void)dofs;
+    const size_t m = (size_t)std::max<int64_t>(n, 0);
+    return 4 * (coh_words(kCohBuckets * (size_t)coh_tiles(n)) + coh_words(kCohBuckets) + 3 * coh_words(m) +
+                coh_words((m + 1) / 2));
+}
+inline CohLayout coh_layout(uint32_t* base, int64_t n) {
+    CohLayout L;
+    const size_t m = (size_t)std::max<int64_t>(n, 0);
+    L.tiles = coh_tiles(n);
+    L.hist = base;
+    L.sizes = L.hist + coh_words(kCohBuckets * (size_t)L.tiles);
+    L.keys = L.sizes + coh_words(kCohBuckets);
+    L.idx = reinterpret_cast<int32_t*>(L.keys + coh_words(m));
+    L.perm = L.idx + coh_words(m);
+    L.key_lo = reinterpret_cast<uint16_t*>(L.perm + coh_words(m));
+    return L;
+}
+// The walk order of the bulk pass: with t.coh_bits > 0 and coh scratch the
+// configurations are bucketed by their first joint angles (so that the lanes
+// of a wave hold similar poses) and *perm points at the order in coh;
+// otherwise *perm is null (the order given).
+hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n, const FastTuning& t, uint32_t* coh,
+                                  const int32_t** perm, hipStream_t st);
 // The lane-refill variant uses count[32] as its configuration counter
-// (expected zero, like count[0]).
+// (expected zero, like count[0]).  perm: the walk order (may be null).
 hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                 int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
+                                const int32_t* perm,
                                 int num_cus, hipStream_t st);
 // Resident workgroups per CU of the lane-refill kernel for this obstacle set.
 int fast_pool_blocks_per_cu(const EnvDev& env, int block);

@@ -264,6 +264,110 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
     }
 }
 
+// Exact recomputation for small obstacle sets (links x obstacles <=
+// kPairsMax, fewer than 64 links): one wave per configuration evaluates all
+// of its (link, obstacle) pairs at once, lane t taking pairs t, t + 64, ...
+// of pair = link * M + obstacle, instead of one link per round.  Every lane
+// runs the same forward kinematics and lane k keeps skeleton point k; a pair
+// fetches its end points by lane shuffles.  The filtered pair distances go to
+// the wave's LDS row, lane i then walks link i's obstacles in order as the
+// reference's loop does (running std::min from MAX_DISTANCE, stop at the
+// first value <= 0), and the links combine in order: the first link with a
+// hit ends the configuration (later links 0, distance 0), otherwise the
+// distance is the smallest link value (a minimum of non-NaN floats, exact in
+// any order).  Every pair is evaluated (no broad phase, no early exit), which
+// changes no result: the per-link walk over the stored values is the
+// reference's loop.  The workgroup's waves step through the configurations
+// together so that the LDS row can be handed over with a workgroup barrier.
+constexpr int kPairsMax = 256;
+
+__host__ __device__ inline size_t pairs_row_off(const EnvDev& env) {
+    return (staged_bytes<float>(env) + 15) & ~size_t(15);
+}
+
+inline bool refine_pairs_fit(const RobotDev& rb, const EnvDev& env) {
+    return rb.num_links < 64 && (int64_t)rb.num_links * env.num_obj <= kPairsMax;
+}
+
+template <bool OBB>
+__global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine_pairs(RobotDev rb, EnvDev env,
+                                                                             QueryParams qp,
+                                                                             const float* __restrict__ q, DistOut out,
+                                                                             const int32_t* __restrict__ idx,
+                                                                             const uint32_t* __restrict__ count,
+                                                                             int64_t n_direct) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<float> s = stage_env<float, OBB>(env, smem);
+    const int M = env.num_obj, L = rb.num_links;
+    const int npairs = L * M;
+    const int t = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    float* row = reinterpret_cast<float*>(smem + pairs_row_off(env)) + wave * kPairsMax;
+    const Params<float> p{qp.eq, qp.maxd};
+    const int64_t total = idx ? (int64_t)*count : n_direct;
+    for (int64_t k0 = (int64_t)blockIdx.x * kRefineWaves; k0 < total; k0 += (int64_t)gridDim.x * kRefineWaves) {
+        const int64_t k = k0 + wave;
+        const bool live = k < total;   // uniform over the wave
+        int64_t c = 0;
+        if (live) {
+            c = idx ? (int64_t)idx[k] : k;
+            const float* qc = q + c * rb.num_dofs;
+            FKStream fk;
+            fk_init(fk);
+            Vec3<float> mine{0.0f, 0.0f, 0.0f};
+            for (int pt = 0; pt <= L; ++pt) {
+                const Vec3<float> P = skel_point<float>(fk, rb, qc, pt);
+                if (t == pt) mine = P;
+            }
+            for (int base = 0; base < npairs; base += 64) {   // uniform rounds: the shuffles see every lane
+                const int pp = base + t;
+                const int i = pp < npairs ? pp / M : 0;
+                const int j = pp - i * M;
+                const Vec3<float> A{__shfl(mine.x, i, 64), __shfl(mine.y, i, 64), __shfl(mine.z, i, 64)};
+                const Vec3<float> B{__shfl(mine.x, i + 1, 64), __shfl(mine.y, i + 1, 64), __shfl(mine.z, i + 1, 64)};
+                if (pp < npairs) {
+                    float dt;
+                    if (s.gnd[j] && i < rb.ground_included) {
+                        dt = p.maxd;
+                    } else if (OBB) {
+                        const float* x = s.xf + 12 * j;
+                        dt = capsule_box<float, false>(to_box(x, A), to_box(x, B), rb.radius[i], s.box[j], p, nullptr,
+                                                       nullptr, nullptr);
+                    } else {
+                        dt = capsule_box<float, false>(A, B, rb.radius[i], s.box[j], p, nullptr, nullptr, nullptr);
+                    }
+                    if (dt > s.tol[j]) dt = p.maxd;
+                    row[pp] = dt;
+                }
+            }
+        }
+        __syncthreads();
+        if (live) {
+            float prof = p.maxd;
+            bool hit = false;
+            if (t < L) {
+                for (int j = 0; j < M; ++j) {
+                    prof = smin(prof, row[t * M + j]);
+                    if (prof <= 0) {
+                        hit = true;
+                        break;
+                    }
+                }
+            }
+            const uint64_t hm = __ballot(hit);
+            const int first_hit = hm ? __ffsll((unsigned long long)hm) - 1 : L;
+            float d = (t < first_hit && t < L) ? prof : p.maxd;
+            for (int o = 32; o > 0; o >>= 1) d = smin(d, __shfl_xor(d, o, 64));
+            if (t < L && out.profile) out.profile[c * L + t] = t <= first_hit ? prof : 0.0f;
+            if (t == 0) {
+                if (out.d) out.d[c] = hm ? 0.0f : d;
+                if (out.verdict) out.verdict[c] = hm ? 0 : 1;
+            }
+        }
+        __syncthreads();   // the row is rewritten by the next round
+    }
+}
+
 template <bool WANT_NP, bool OBB>
 __global__ __launch_bounds__(kBlock) void k_distance64(RobotDev rb, EnvDev env, QueryParams qp,
                                                        const float* __restrict__ q, int64_t n, DistOut out) {
@@ -608,6 +712,16 @@ hipError_t launch_distance_ref32_waves(const RobotDev& rb, const EnvDev& env, co
     const dim3 b(64 * kRefineWaves);
     const int32_t* no_idx = nullptr;
     const uint32_t* no_count = nullptr;
+    if (!out.nearest && refine_pairs_fit(rb, env)) {
+        const size_t smp = pairs_row_off(env) + (size_t)kRefineWaves * kPairsMax * sizeof(float);
+        if (env.use_obb)
+            hipLaunchKernelGGL((k_distance_refine_pairs<true>), dim3(grid), b, smp, st, rb, env, qp, q, out, no_idx,
+                               no_count, n);
+        else
+            hipLaunchKernelGGL((k_distance_refine_pairs<false>), dim3(grid), b, smp, st, rb, env, qp, q, out, no_idx,
+                               no_count, n);
+        return hipGetLastError();
+    }
     if (env.use_obb) {
         if (out.nearest)
             hipLaunchKernelGGL((k_distance_refine<true, true>), dim3(grid), b, sm, st, rb, env, qp, q, out, no_idx,
@@ -635,6 +749,16 @@ hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const Q
     const unsigned grid = (unsigned)std::min<int64_t>((n + kRefineWaves - 1) / kRefineWaves, 2048);
     const size_t sm = staged_bytes<float>(env);
     const dim3 b(64 * kRefineWaves);
+    if (refine_pairs_fit(rb, env)) {
+        const size_t smp = pairs_row_off(env) + (size_t)kRefineWaves * kPairsMax * sizeof(float);
+        if (env.use_obb)
+            hipLaunchKernelGGL((k_distance_refine_pairs<true>), dim3(grid), b, smp, st, rb, env, qp, q, out, idx, count,
+                               (int64_t)0);
+        else
+            hipLaunchKernelGGL((k_distance_refine_pairs<false>), dim3(grid), b, smp, st, rb, env, qp, q, out, idx,
+                               count, (int64_t)0);
+        return hipGetLastError();
+    }
     if (env.use_obb)
         hipLaunchKernelGGL((k_distance_refine<true, false>), dim3(grid), b, sm, st, rb, env, qp, q, out, idx, count,
                            (int64_t)0);

@@ -1,6 +1,7 @@
 """Small driver for rocprofv3 runs of the distance kernels.
 
     python -m rpmplv2_amd.profile_driver [--workload 2|5|bur|traj|edgeself] [--n N] [--calls K] [--calibrate]
+                                         [--tuning VARIANT,BLOCK,BLOCKS_PER_CU,COHERENCE_BITS]
 
 Builds the seeded workload, optionally runs a 256 MiB elementwise copy first
 (the calibration point for FETCH_SIZE / WRITE_SIZE), then calls the engine K
@@ -8,7 +9,9 @@ times on device-resident inputs: rvs_distance per precision (workloads 2 and
 5), generalized spines from one root with underestimation and real-distance
 layers (bur: 65,536 targets), trajectory safety (traj: 8,192 straight
 60-point trajectories) or xArm6 edge self-collision (edgeself: 16,384
-edges).  Used as the profiled
+edges).  --tuning records a launch shape for the FP32 bulk pass (as
+rvs_set_tuning does), so that a profile can use the shape an autotuned run
+chose.  Used as the profiled
 program in `rocprofv3 --kernel-trace --stats -- python3 -m ...` and
 `rocprofv3 --pmc ... -- python3 -m ...` runs.
 """
@@ -24,6 +27,7 @@ def main():
     ap.add_argument("--calls", type=int, default=3)
     ap.add_argument("--calibrate", action="store_true")
     ap.add_argument("--precisions", default="ref32,fp32r,fp64")
+    ap.add_argument("--tuning", default="")
     args = ap.parse_args()
 
     import torch
@@ -45,6 +49,9 @@ def main():
     eng = rvs.Engine()
     rid = eng.load_robot(w.robot)
     eid = eng.create_env(w.scene, rvs.ENV_OBB if args.workload == "5" else rvs.ENV_AABB)
+    if args.tuning:
+        v, b, bpc, cb = (int(x) for x in args.tuning.split(","))
+        eng.set_tuning(rid, eid, v, b, bpc, cb)
     n, L = len(w.q), w.robot.num_links
     q = torch.from_numpy(w.q).to(dev)
     d = torch.empty(n, device=dev)

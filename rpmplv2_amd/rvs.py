@@ -91,7 +91,8 @@ class ServerStats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("variant", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("ms", C.c_float)]
+    _fields_ = [("variant", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("coherence_bits", C.c_int32), ("ms", C.c_float)]
 
 
 class EdgeTuning(C.Structure):
@@ -518,16 +519,19 @@ class Engine:
             q = np.ascontiguousarray(q, np.float32).reshape(-1, self.robots[robot].num_dofs)
         self._check(lib().rvs_autotune(self._ctx, robot, env, _ptr(q), q.shape[0], F_DEVICE if dev else 0,
                                        C.byref(t)), "rvs_autotune")
-        return {"variant": t.variant, "block": t.block, "blocks_per_cu": t.blocks_per_cu, "ms": t.ms}
+        return {"variant": t.variant, "block": t.block, "blocks_per_cu": t.blocks_per_cu,
+                "coherence_bits": t.coherence_bits, "ms": t.ms}
 
-    def set_tuning(self, robot: int, env: int, variant: int, block: int = 256, blocks_per_cu: int = 0):
-        t = Tuning(variant, block, blocks_per_cu, 0.0)
+    def set_tuning(self, robot: int, env: int, variant: int, block: int = 256, blocks_per_cu: int = 0,
+                   coherence_bits: int = 0):
+        t = Tuning(variant, block, blocks_per_cu, coherence_bits, 0.0)
         self._check(lib().rvs_set_tuning(self._ctx, robot, env, C.byref(t)), "rvs_set_tuning")
 
     def get_tuning(self, robot: int, env: int) -> dict:
         t = Tuning()
         self._check(lib().rvs_get_tuning(self._ctx, robot, env, C.byref(t)), "rvs_get_tuning")
-        return {"variant": t.variant, "block": t.block, "blocks_per_cu": t.blocks_per_cu}
+        return {"variant": t.variant, "block": t.block, "blocks_per_cu": t.blocks_per_cu,
+                "coherence_bits": t.coherence_bits}
 
     # ---- edges
     def edge_is_valid(self, robot: int, env: int, q1, q2):

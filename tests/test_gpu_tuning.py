@@ -1,5 +1,5 @@
-"""Autotuner of the FP32 bulk pass: every launch shape gives the same
-per-configuration results (the shape only changes which lane evaluates a
+"""Autotuner of the FP32 bulk pass: every launch shape and walk order gives
+the same per-configuration results (they only change which lane evaluates a
 configuration), and rvs_autotune keeps one of the candidate shapes."""
 import numpy as np
 import pytest
@@ -20,16 +20,18 @@ def test_launch_shapes_agree(engine, wl):
     rid = engine.load_robot(w.robot)
     eid = engine.create_env(w.scene, sem)
     ref = None
-    for variant in (0, 1):
-        for block in (64, 128, 256):
-            for per_cu in ((0,) if variant == 0 else (0, 1)):
-                engine.set_tuning(rid, eid, variant, block, per_cu)
-                d, prof, ver, _ = engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
-                if ref is None:
-                    ref = (d, prof, ver)
-                else:
-                    assert np.array_equal(ver, ref[2]), (variant, block, per_cu)
-                    assert np.array_equal(d, ref[0]) and np.array_equal(prof, ref[1]), (variant, block, per_cu)
+    for coh in (0, 5, 3):
+        for variant in (0, 1):
+            for block in (64, 128, 256):
+                for per_cu in ((0,) if variant == 0 else (0, 1)):
+                    engine.set_tuning(rid, eid, variant, block, per_cu, coh)
+                    d, prof, ver, _ = engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
+                    if ref is None:
+                        ref = (d, prof, ver)
+                    else:
+                        shape = (variant, block, per_cu, coh)
+                        assert np.array_equal(ver, ref[2]), shape
+                        assert np.array_equal(d, ref[0]) and np.array_equal(prof, ref[1]), shape
     best = engine.autotune(rid, eid, w.q)
     print(wl, "autotune:", best)
     assert best["variant"] in (0, 1) and best["block"] in (64, 128, 256) and best["ms"] > 0
@@ -67,3 +69,47 @@ def test_edge_launch_shapes_agree(oracle, engine):
     got = engine.edge_batches(eid, batches)
     for g, e in zip(got, expect):
         assert np.array_equal(g, e)
+
+
+@pytest.mark.parametrize("case", ["xarm6_ragged", "xarm6_one_pose", "chain2", "chain1"])
+def test_walk_order_covers_every_configuration(engine, case):
+    """The walk-order sort hands every configuration to exactly one lane at
+    every key width: outputs start as NaN (verdict 255) and must equal the
+    unordered pass's.  Cases: a ragged batch (70,001 configurations, not a
+    whole number of tiles), 65,536 copies of one pose (one bucket cut into
+    four chunks), and 2- and 1-joint chains (up to 9 and 18 bits per
+    joint)."""
+    import torch
+    rng = np.random.default_rng(7)
+    if case.startswith("xarm6"):
+        w = W.workload2(70001 if case == "xarm6_ragged" else 65536)
+        rb, scene, q = w.robot, w.scene, w.q
+        if case == "xarm6_one_pose":
+            q = np.repeat(q[:1], len(q), axis=0)
+        widths = (0, 1, 2, 3, 4, 5, 6, 18)
+    else:
+        nd = 2 if case == "chain2" else 1
+        rb = R.chain([0.3] * nd, radius=0.02)
+        scene = W._boxes(rng, 6, 0.05, 0.1, 0.8)
+        q = rng.uniform(-4.0, 4.0, (50000, nd)).astype(np.float32)
+        widths = (0, 3, 9, 18)
+    rid = engine.load_robot(rb)
+    eid = engine.create_env(scene)
+    dev = torch.device("cuda:0")
+    qd = torch.from_numpy(np.ascontiguousarray(q)).to(dev)
+    n, L = qd.shape[0], rb.num_links
+    ref = None
+    for cb in widths:
+        for variant in (0, 1):
+            engine.set_tuning(rid, eid, variant, 128, 0, cb)
+            d = torch.full((n,), float("nan"), device=dev)
+            prof = torch.full((n, L), float("nan"), device=dev)
+            ver = torch.full((n,), 255, dtype=torch.uint8, device=dev)
+            engine.distance_device(rid, eid, qd, d, prof, ver, precision=rvs.PREC_FP32_REFINE)
+            got = (d.cpu().numpy(), prof.cpu().numpy(), ver.cpu().numpy())
+            assert not np.isnan(got[0]).any() and not np.isnan(got[1]).any() and (got[2] != 255).all(), (cb, variant)
+            if ref is None:
+                ref = got
+            else:
+                assert np.array_equal(got[2], ref[2]), (cb, variant)
+                assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (cb, variant)

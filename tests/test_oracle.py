@@ -504,13 +504,25 @@ def test_self_min_matches_pairs(oracle):
 
 
 # ---------------------------------------------------------------- broad phase
-@pytest.mark.parametrize("name", ["workload1", "workload2", "workload5_obb", "workload5_aabb", "chain_ground"])
+@pytest.mark.parametrize("name", ["workload1", "workload2", "workload5_obb", "workload5_aabb", "chain_ground",
+                                  "dense_overlap"])
 def test_broad_phase_changes_nothing(oracle, name):
-    """Skipping pairs whose bounding-box gap exceeds max(link minimum + r,
-    EQUALITY_THRESHOLD) + margin leaves distances, profiles and verdicts
-    bit-identical (both arithmetics), and skips a real share of the pairs."""
+    """The bulk pass's walk (pairs whose bounding-box gap exceeds max(link
+    minimum + r, EQUALITY_THRESHOLD) + margin skipped, each link started at
+    the previous link's nearest obstacle) leaves distances, profiles and
+    verdicts bit-identical to the plain loop (both arithmetics), and skips a
+    real share of the pairs.  dense_overlap packs overlapping boxes around
+    the arm so that links often cross several boxes: there the warm pair is
+    often a hit with an earlier hit before it in obstacle order, whose value
+    (0 for a crossing, -r for an inside end point) the profile must take."""
     oriented = False
-    if name == "workload1":
+    if name == "dense_overlap":
+        rng = np.random.default_rng(21)
+        rb = R.chain([0.1] + [0.25] * 5, radius=0.03)
+        sc = W._boxes(rng, 24, 0.1, 0.4, 0.9, (0, 0, 0.5))
+        q = rng.uniform(-np.pi, np.pi, (8192, 6)).astype(np.float32)
+        w = W.ConfigBatch("dense", rb, sc, q)
+    elif name == "workload1":
         w = W.workload1(8192)
     elif name == "workload2":
         w = W.workload2(16384)
@@ -528,6 +540,9 @@ def test_broad_phase_changes_nothing(oracle, name):
         dc, profc, verc, (ev, cu) = oracle.distance_cull(w.robot, sc, w.q, prec)
         assert np.array_equal(d, dc) and np.array_equal(prof, profc) and np.array_equal(ver, verc)
         assert cu > 0.2 * (ev + cu)
+        if name == "dense_overlap":   # the case the warm start must handle is present
+            hit_links = (prof <= 0).sum()
+            assert hit_links > 1000 and len(np.unique(prof[prof <= 0])) >= 2
 
 
 # ---------------------------------------------------------------- enclosing radii
