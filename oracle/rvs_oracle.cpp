@@ -722,10 +722,11 @@ T pair_filtered(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, const 
 // computeDistance: RealVectorSpace.cpp:318-392 (fresh state, compute_again).
 // With cull (and no nearest points) this is the engine's bulk-pass walk
 // instead: pairs whose bounding-box gap exceeds max(running link minimum + r,
-// EQUALITY_THRESHOLD) + 1e-5 are skipped, and each link first evaluates the
-// obstacle that held the previous link's minimum (warm start); if that pair
-// is a hit, the pairs before it (gap bound of a running minimum of 0) are
-// searched for an earlier hit.  Its results must equal the plain loop's.
+// EQUALITY_THRESHOLD) + 1e-5 are skipped, and (in scenes of at most 128
+// obstacles) each link first evaluates the obstacle that held the previous
+// link's minimum (warm start); if that pair is a hit, the pairs before it
+// (gap bound of a running minimum of 0) are searched for an earlier hit.
+// Its results must equal the plain loop's.
 template <typename T>
 void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, const T* Pk,
                   T* d_out, T* prof, T* np_out /* M*L*6 or null */, uint8_t* verdict,
@@ -742,12 +743,13 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
         T r = (T)rb->radius[i];
         int best = -1;
         const bool walk = cull && !np_out;
-        if (walk && warm >= 0) {
+        const int w = (walk && sc.M <= 128) ? warm : -1;   // warm start only for scenes of <= 128 obstacles
+        if (w >= 0) {
             cull->evaluated++;
-            const T dw = pair_filtered(rb, pr, sc, A, B, i, warm);
+            const T dw = pair_filtered(rb, pr, sc, A, B, i, w);
             if (dw <= 0) {
                 T first = dw;
-                for (int j = 0; j < warm; ++j) {
+                for (int j = 0; j < w; ++j) {
                     T lo[3], hi[3];
                     cull_box(sc, j, lo, hi);
                     if (seg_box_gap(A, B, lo, hi) > std::max(T(0) + r, pr.eq) + T(1e-5)) {
@@ -768,7 +770,7 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
             }
             if (dw < prof[i]) {
                 prof[i] = dw;
-                best = warm;
+                best = w;
             }
         }
         for (int j = 0; j < sc.M; ++j) {
@@ -776,7 +778,7 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
             T np[6];
             int v = 1;
             if (walk) {
-                if (j == warm) continue;
+                if (j == w) continue;
                 T lo[3], hi[3];
                 cull_box(sc, j, lo, hi);
                 if (seg_box_gap(A, B, lo, hi) > std::max(prof[i] + r, pr.eq) + T(1e-5)) {

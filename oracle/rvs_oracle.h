@@ -128,9 +128,9 @@ void orc_self_min32(const orc_robot* rb, const orc_params* p, const float* q, in
 
 /* computeDistance walked as the engine's bulk pass walks it: pairs whose
  * bounding-box gap exceeds max(running link minimum + r, EQUALITY_THRESHOLD)
- * + 1e-5 are skipped, and each link starts at the obstacle that held the
- * previous link's minimum (a hit there triggers a search for an earlier hit
- * in obstacle order).  Results equal orc_distance32/64; counts = {pairs
+ * + 1e-5 are skipped, and (scenes of at most 128 obstacles) each link starts
+ * at the obstacle that held the previous link's minimum (a hit there triggers
+ * a search for an earlier hit in obstacle order).  Results equal orc_distance32/64; counts = {pairs
  * evaluated, pairs skipped}. */
 void orc_distance_cull32(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q, int64_t n,
                          float* d, float* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads);

@@ -159,8 +159,14 @@ __device__ __attribute__((always_inline)) inline float fast_pair(const Staged<fl
 // link's nearest obstacle.  Called by the whole wave; lanes with active false
 // evaluate nothing.  Both bulk kernels walk a configuration's pairs through
 // this function, so they evaluate the same pairs, flag the same
-// configurations for the exact pass and return the same bits.
-template <bool OBB>
+// configurations for the exact pass and return the same bits.  Scenes with
+// more than kWarmMaxObjects obstacles walk in order only (WARM false, chosen
+// at launch): there the walk's bounding-box tests dominate, and the warm pair
+// (a different obstacle in every lane) costs a fully divergent evaluation per
+// link.
+constexpr int kWarmMaxObjects = 128;
+
+template <bool OBB, bool WARM>
 __device__ __attribute__((always_inline)) inline float link_min(const Staged<float>& s, const float* __restrict__ cull,
                                                                 int M, bool gnd_skip, const Vec3<float>& P0,
                                                                 const Vec3<float>& P1, float r, const Params<float>& p,
@@ -169,9 +175,24 @@ __device__ __attribute__((always_inline)) inline float link_min(const Staged<flo
     Vec3<float> slo, shi;
     seg_bounds(P0, P1, slo, shi);
     float prof = p.maxd;
+    hit = false;
+    if (!WARM) {   // the plain in-order walk
+        bool go = active;
+        for (int j = 0; j < M; ++j) {
+            if (go && !cull_pair(cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
+                const float dt = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+                prof = smin(prof, dt);
+                if (prof <= 0) {
+                    hit = true;
+                    go = false;
+                }
+            }
+            if (__ballot(go) == 0) break;
+        }
+        return prof;
+    }
     int best = -1;
     const int w = active ? warm : -1;
-    hit = false;
     if (w >= 0) {
         const float dt = (s.gnd[w] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, w, P0, P1, r, p, band, uncertain);
         if (dt <= 0) {
@@ -232,7 +253,7 @@ __device__ __attribute__((always_inline)) inline void append_flagged(bool flag, 
 // its profile row there, writing it out in one burst at the end: the rows of
 // a wave's lanes are scattered in memory, and reading or writing them a
 // joint or a link at a time, far apart in time, costs a cache line each.
-template <bool OBB, bool ROWS>
+template <bool OBB, bool ROWS, bool WARM>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
     uint32_t* count, const int32_t* __restrict__ perm, size_t rows_off) {
@@ -271,8 +292,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const Vec3<float> P1 = skelf_point(fk, rb, qc, i + 1, qs);
         if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
         const float r = rb.radius[i];
-        const float prof = link_min<OBB>(s, env.cull, M, i < rb.ground_included, P0, P1, r, p, band, true, warm, hit,
-                                         uncertain);
+        const float prof = link_min<OBB, WARM>(s, env.cull, M, i < rb.ground_included, P0, P1, r, p, band, true, warm,
+                                               hit, uncertain);
         if (ROWS) prow[i * B] = prof;
         else if (out.profile) out.profile[c * L + i] = prof;
         if (hit) break;
@@ -302,7 +323,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // and link).  Lanes therefore sit on different links of different
 // configurations; per configuration the pair order, the early exit and every
 // result are those of the classic kernel.
-template <bool OBB>
+template <bool OBB, bool WARM>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_pool(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
     uint32_t* count, uint32_t* next, const int32_t* __restrict__ perm) {
@@ -363,8 +384,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             r = rb.radius[i];
         }
         bool hit = false;
-        const float prof = link_min<OBB>(s, env.cull, M, i < rb.ground_included, P0, P1, r, p, band, busy, warm, hit,
-                                         uncertain);
+        const float prof = link_min<OBB, WARM>(s, env.cull, M, i < rb.ground_included, P0, P1, r, p, band, busy, warm,
+                                               hit, uncertain);
         bool finished = false;
         if (busy) {
             if (out.profile) out.profile[c * L + i] = prof;
@@ -565,11 +586,15 @@ __global__ __launch_bounds__(kCohSortThreads) void k_coh_lowsort(const uint32_t*
 
 int fast_pool_blocks_per_cu(const EnvDev& env, int block) {
     int nb = 0;
-    const hipError_t e =
-        env.use_obb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<true>, block,
-                                                                    staged_bytes<float>(env))
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<false>, block,
-                                                                    staged_bytes<float>(env));
+    const size_t sm = staged_bytes<float>(env);
+    const bool warm = env.num_obj <= kWarmMaxObjects;
+    hipError_t e;
+    if (env.use_obb)
+        e = warm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<true, true>, block, sm)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<true, false>, block, sm);
+    else
+        e = warm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<false, true>, block, sm)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<false, false>, block, sm);
     return e == hipSuccess ? nb : 0;
 }
 
@@ -604,10 +629,12 @@ hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n,
     return e;
 }
 
-hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
-                                const int32_t* perm, int num_cus, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
+namespace {
+
+template <bool OBB, bool WARM>
+hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
+                             const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
+                             const int32_t* perm, int num_cus, hipStream_t st) {
     const int B = (t.block == 64 || t.block == 128) ? t.block : kBlock;
     const size_t sm = staged_bytes<float>(env);
     if (t.variant == 1) {
@@ -616,35 +643,35 @@ hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const Que
         per_cu = std::max(1, per_cu);
         const int64_t blocks = std::min<int64_t>((int64_t)num_cus * per_cu, (n + B - 1) / B);
         const dim3 g((unsigned)std::max<int64_t>(1, blocks)), b(B);
-        if (env.use_obb)
-            hipLaunchKernelGGL(k_distance_fast_pool<true>, g, b, sm, st, rb, env, qp, q, n, out, idx, count,
-                               count + 32, perm);
-        else
-            hipLaunchKernelGGL(k_distance_fast_pool<false>, g, b, sm, st, rb, env, qp, q, n, out, idx, count,
-                               count + 32, perm);
+        hipLaunchKernelGGL((k_distance_fast_pool<OBB, WARM>), g, b, sm, st, rb, env, qp, q, n, out, idx, count,
+                           count + 32, perm);
         return hipGetLastError();
     }
     const dim3 g((unsigned)((n + B - 1) / B)), b(B);
     // rows staged through LDS when a walk order scatters them and the slices fit
     const size_t rows_off = (sm + 15) & ~size_t(15);
     const size_t rows_sm = rows_off + (size_t)(rb.num_dofs + rb.num_links) * 4 * B;
-    const bool rows = perm && rows_sm <= 64 * 1024;
-    if (env.use_obb) {
-        if (rows)
-            hipLaunchKernelGGL((k_distance_fast<true, true>), g, b, rows_sm, st, rb, env, qp, q, n, out, idx, count,
-                               perm, rows_off);
-        else
-            hipLaunchKernelGGL((k_distance_fast<true, false>), g, b, sm, st, rb, env, qp, q, n, out, idx, count, perm,
-                               rows_off);
-    } else {
-        if (rows)
-            hipLaunchKernelGGL((k_distance_fast<false, true>), g, b, rows_sm, st, rb, env, qp, q, n, out, idx, count,
-                               perm, rows_off);
-        else
-            hipLaunchKernelGGL((k_distance_fast<false, false>), g, b, sm, st, rb, env, qp, q, n, out, idx, count,
-                               perm, rows_off);
-    }
+    if (perm && rows_sm <= 64 * 1024)
+        hipLaunchKernelGGL((k_distance_fast<OBB, true, WARM>), g, b, rows_sm, st, rb, env, qp, q, n, out, idx, count,
+                           perm, rows_off);
+    else
+        hipLaunchKernelGGL((k_distance_fast<OBB, false, WARM>), g, b, sm, st, rb, env, qp, q, n, out, idx, count,
+                           perm, rows_off);
     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
+                                int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
+                                const int32_t* perm, int num_cus, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const bool warm = env.num_obj <= kWarmMaxObjects;
+    if (env.use_obb)
+        return warm ? launch_fast_shape<true, true>(rb, env, qp, q, n, out, idx, count, t, perm, num_cus, st)
+                    : launch_fast_shape<true, false>(rb, env, qp, q, n, out, idx, count, t, perm, num_cus, st);
+    return warm ? launch_fast_shape<false, true>(rb, env, qp, q, n, out, idx, count, t, perm, num_cus, st)
+                : launch_fast_shape<false, false>(rb, env, qp, q, n, out, idx, count, t, perm, num_cus, st);
 }
 
 }  // namespace rvs
