@@ -187,6 +187,22 @@ rvs_status rvs_distance_f64(rvs_context* ctx, int32_t robot, int32_t env, const 
 rvs_status rvs_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n, uint8_t* valid,
                         uint32_t flags);
 
+/* Stream-ordered forms of rvs_distance (no nearest points) and rvs_is_valid
+ * for device arrays (flags must hold RVS_F_DEVICE; no RVS_F_TIMING): the
+ * work is enqueued on 'stream' (a hipStream_t; NULL = the context's stream)
+ * and the call returns at once.  'done' (a hipEvent_t, may be NULL) is
+ * recorded on the stream after the work.  The arrays must stay valid until
+ * then.  Asynchronous calls run in call order on the device (each waits for
+ * the previous one, whatever its stream); any other call on the context
+ * first waits on the host for the asynchronous calls in flight.
+ * rvs_last_refined does not report asynchronous calls.  The reference's
+ * interface has no asynchronous form (RealVectorSpace.h:45-49 are blocking
+ * calls); these serve callers that overlap queries with their own GPU work. */
+rvs_status rvs_distance_async(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n, float* d,
+                              float* profile, uint8_t* verdict, uint32_t flags, void* stream, void* done);
+rvs_status rvs_is_valid_async(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n,
+                              uint8_t* valid, uint32_t flags, void* stream, void* done);
+
 /* RealVectorSpace::computeDistanceUnderestimation(q, nearest_points)
  * (RealVectorSpace.cpp:402-446) on fresh states.  nearest_ref is [M][L][6]
  * shared by all configurations (ref_stride = 0) or one set per configuration

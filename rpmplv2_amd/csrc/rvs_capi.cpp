@@ -203,6 +203,7 @@ rvs_status rvs_create(const rvs_options* opt, rvs_context** out) {
     if (const char* v = std::getenv("RVS_FAST_VARIANT")) c->fast_variant = std::atoi(v);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->async_ev, hipEventDisableTiming);
     if (e != hipSuccess) { delete c; return RVS_EDEVICE; }
     *out = c;
     return RVS_OK;
@@ -212,6 +213,7 @@ void rvs_destroy(rvs_context* c) {
     if (!c) return;
     (void)hipSetDevice(c->opt.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->drain_async();
     for (auto& r : c->robots) for (void* p : r.allocs) (void)hipFree(p);
     for (auto& e : c->envs) for (void* p : e.allocs) (void)hipFree(p);
     for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
@@ -219,6 +221,7 @@ void rvs_destroy(rvs_context* c) {
     c->arena.release();
     for (int k = 0; k < 4; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    if (c->async_ev) (void)hipEventDestroy(c->async_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -236,7 +239,7 @@ int32_t rvs_last_timing(const rvs_context* c, float* stage_ms, int32_t max_stage
 
 rvs_status rvs_robot_load(rvs_context* c, const rvs_robot_desc* d, int32_t* robot_id) {
     if (!c || !d || !robot_id || !d->joints) return fail(c, RVS_EINVAL, "rvs_robot_load: null argument");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     if (d->num_joints <= 0 || d->num_joints > kMaxSeg) return fail(c, RVS_ERANGE, "rvs_robot_load: joint count");
     if (d->num_links <= 0 || d->num_links > kMaxLinks) return fail(c, RVS_ERANGE, "rvs_robot_load: link count");
     if (!d->capsule_radius) return fail(c, RVS_EINVAL, "rvs_robot_load: capsule radii missing");
@@ -329,7 +332,7 @@ rvs_status rvs_env_create(rvs_context* c, const rvs_box* boxes, int32_t num, int
     if (!c || !env_id || (num > 0 && !boxes)) return fail(c, RVS_EINVAL, "rvs_env_create: null argument");
     if (num < 0 || num > kMaxLdsObstacles) return fail(c, RVS_ERANGE, "rvs_env_create: at most 1024 boxes");
     if (semantics != RVS_ENV_AABB && semantics != RVS_ENV_OBB) return fail(c, RVS_EINVAL, "rvs_env_create: semantics");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     EnvRec r;
     r.semantics = semantics;
     r.boxes.assign(boxes, boxes + num);
@@ -374,7 +377,7 @@ rvs_status rvs_env_update(rvs_context* c, int32_t env_id, const int32_t* idx, co
                           uint32_t flags) {
     if (!c || (count > 0 && (!idx || !pos))) return fail(c, RVS_EINVAL, "rvs_env_update: null argument");
     if (flags & RVS_F_DEVICE) return fail(c, RVS_ENOTSUP, "rvs_env_update: host arrays only");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     if (!env_ok(c, env_id)) return fail(c, RVS_EINVAL, "rvs_env_update: bad env id");
     EnvRec& r = c->envs[env_id];
     for (int k = 0; k < count; ++k)
@@ -409,7 +412,7 @@ rvs_status rvs_env_set_motion_law(rvs_context* c, int32_t env_id, const rvs_env_
     if (!c || !law || !vel || !max_vel) return fail(c, RVS_EINVAL, "rvs_env_set_motion_law: null argument");
     if (law->motion_type < RVS_MOTION_STRAIGHT || law->motion_type > RVS_MOTION_LIGHT_DIRECTIONS)
         return fail(c, RVS_EINVAL, "rvs_env_set_motion_law: unknown motion type");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     if (!env_ok(c, env_id)) return fail(c, RVS_EINVAL, "rvs_env_set_motion_law: bad env id");
     EnvRec& r = c->envs[env_id];
     const int M = (int)r.boxes.size();
@@ -456,7 +459,7 @@ rvs_status rvs_env_set_motion_law(rvs_context* c, int32_t env_id, const rvs_env_
 rvs_status rvs_env_step(rvs_context* c, int32_t env_id, float delta_time, int32_t steps, float* pos, float* vel) {
     if (!c) return RVS_EINVAL;
     if (steps < 0) return fail(c, RVS_EINVAL, "rvs_env_step: negative step count");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     if (!env_ok(c, env_id)) return fail(c, RVS_EINVAL, "rvs_env_step: bad env id");
     EnvRec& r = c->envs[env_id];
     if (!r.has_motion) return fail(c, RVS_EINVAL, "rvs_env_step: no motion law set");
@@ -495,7 +498,7 @@ rvs_status rvs_env_step(rvs_context* c, int32_t env_id, float delta_time, int32_
 
 rvs_status rvs_env_bounds(rvs_context* c, int32_t env_id, float* aabb, int32_t num) {
     if (!c || !aabb) return fail(c, RVS_EINVAL, "rvs_env_bounds: null argument");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     if (!env_ok(c, env_id)) return fail(c, RVS_EINVAL, "rvs_env_bounds: bad env id");
     const EnvRec& r = c->envs[env_id];
     if (num != (int)r.boxes.size()) return fail(c, RVS_EINVAL, "rvs_env_bounds: size mismatch");
@@ -595,7 +598,7 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
                         float* profile, float* nearest, uint8_t* verdict, uint32_t flags) {
     rvs_status s = check_query(c, robot, env, q, n, true);
     if (s != RVS_OK) return s;
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
     int prec = prec_of(c, flags);
@@ -692,12 +695,98 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     return RVS_OK;
 }
 
+namespace {
+
+// Stream-ordered start of an asynchronous call: after every earlier one.
+hipError_t async_begin(rvs_context* c, hipStream_t st) {
+    return c->async_pending ? hipStreamWaitEvent(st, c->async_ev, 0) : hipSuccess;
+}
+
+// End of an asynchronous call: the context's event and the caller's.
+hipError_t async_end(rvs_context* c, hipStream_t st, void* done) {
+    hipError_t e = hipEventRecord(c->async_ev, st);
+    if (e == hipSuccess) c->async_pending = true;
+    if (e == hipSuccess && done) e = hipEventRecord(static_cast<hipEvent_t>(done), st);
+    return e;
+}
+
+}  // namespace
+
+rvs_status rvs_distance_async(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n, float* d,
+                              float* profile, uint8_t* verdict, uint32_t flags, void* stream, void* done) {
+    rvs_status s = check_query(c, robot, env, q, n, true);
+    if (s != RVS_OK) return s;
+    if (!(flags & RVS_F_DEVICE)) return fail(c, RVS_EINVAL, "rvs_distance_async: device arrays only (RVS_F_DEVICE)");
+    if (flags & RVS_F_TIMING) return fail(c, RVS_EINVAL, "rvs_distance_async: no timing");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    const int prec = prec_of(c, flags);
+    s = env_fits(c, ev.dev, prec == RVS_PREC_FP64, "rvs_distance_async");
+    if (s != RVS_OK) return s;
+    if (prec == RVS_PREC_FP32_REFINE && n > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_distance_async: batch too large");
+    (void)hipSetDevice(c->opt.device);
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
+    const QueryParams qp = params_of(c);
+    DistOut out{};
+    out.d = d; out.profile = profile; out.verdict = verdict;
+    hipError_t e = async_begin(c, st);
+    if (e == hipSuccess && n > 0) {
+        if (prec == RVS_PREC_FP32_REFINE) {
+            const size_t need = Carve::need({(size_t)n * 4 + 256, coh_bytes(n, D)});
+            if (need > c->arena.cap) c->drain_async();   // growing frees the block earlier calls use
+            e = c->arena.ensure(need);
+            if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_async: staging");
+            Carve cv(c->arena.p);
+            uint32_t* rcount = cv.take<uint32_t>(64);
+            int32_t* ridx = cv.take<int32_t>(n);
+            uint32_t* coh = cv.take<uint32_t>(coh_bytes(n, D) / 4);
+            const FastTuning ft = c->tuning_for(robot, env, L, M);
+            const int32_t* perm = nullptr;
+            e = launch_zero_u32(rcount, 64, st);
+            if (e == hipSuccess) e = launch_coherence_order(rb.dev, q, n, ft, coh, &perm, st);
+            if (e == hipSuccess)
+                e = launch_distance_fast(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, ft, perm, c->num_cus, st);
+            if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, st);
+        } else if (prec == RVS_PREC_FP64) {
+            e = launch_distance_fp64(rb.dev, ev.dev, qp, q, n, out, st);
+        } else {
+            e = launch_distance_ref32_auto(rb.dev, ev.dev, qp, q, n, out, c->wave_max, st);
+        }
+    }
+    if (e == hipSuccess) e = async_end(c, st, done);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_async");
+    return RVS_OK;
+}
+
+rvs_status rvs_is_valid_async(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n, uint8_t* valid,
+                              uint32_t flags, void* stream, void* done) {
+    rvs_status s = check_query(c, robot, env, q, n, true);
+    if (s != RVS_OK) return s;
+    if (n > 0 && !valid) return fail(c, RVS_EINVAL, "rvs_is_valid_async: null output");
+    if (!(flags & RVS_F_DEVICE)) return fail(c, RVS_EINVAL, "rvs_is_valid_async: device arrays only (RVS_F_DEVICE)");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    const int fp64 = prec_of(c, flags) == RVS_PREC_FP64;
+    s = env_fits(c, ev.dev, fp64, "rvs_is_valid_async");
+    if (s != RVS_OK) return s;
+    (void)hipSetDevice(c->opt.device);
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    hipError_t e = async_begin(c, st);
+    if (e == hipSuccess && n > 0) e = launch_is_valid(rb.dev, ev.dev, params_of(c), q, n, valid, fp64, st);
+    if (e == hipSuccess) e = async_end(c, st, done);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_is_valid_async");
+    return RVS_OK;
+}
+
 rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n, uint32_t flags,
                         rvs_tuning* best) {
     rvs_status s = check_query(c, robot, env, q, n, true);
     if (s != RVS_OK) return s;
     if (n <= 0) return fail(c, RVS_EINVAL, "rvs_autotune: empty sample");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
     s = env_fits(c, ev.dev, false, "rvs_autotune");
@@ -788,7 +877,7 @@ rvs_status rvs_set_tuning(rvs_context* c, int32_t robot, int32_t env, const rvs_
 rvs_status rvs_set_small_batch(rvs_context* c, int64_t max_configs) {
     if (!c) return RVS_EINVAL;
     if (max_configs < 0) return fail(c, RVS_EINVAL, "rvs_set_small_batch: negative size");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     c->wave_max = max_configs;
     return RVS_OK;
 }
@@ -797,7 +886,7 @@ rvs_status rvs_get_tuning(rvs_context* c, int32_t robot, int32_t env, rvs_tuning
     if (!c || !t) return fail(c, RVS_EINVAL, "rvs_get_tuning: null argument");
     int32_t L = 0, M = 0;
     {
-        std::lock_guard<std::mutex> lk(c->mu);
+        CtxLock lk(c);
         if (robot_ok(c, robot)) L = c->robots[robot].num_links;
         if (env_ok(c, env)) M = c->envs[env].dev.num_obj;
     }
@@ -810,7 +899,7 @@ rvs_status rvs_distance_f64(rvs_context* c, int32_t robot, int32_t env, const fl
                             double* profile, uint8_t* verdict, uint32_t flags) {
     rvs_status s = check_query(c, robot, env, q, n, true);
     if (s != RVS_OK) return s;
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
     s = env_fits(c, ev.dev, true, "rvs_distance_f64");
@@ -852,7 +941,7 @@ rvs_status rvs_is_valid(rvs_context* c, int32_t robot, int32_t env, const float*
     rvs_status s = check_query(c, robot, env, q, n, true);
     if (s != RVS_OK) return s;
     if (n > 0 && !valid) return fail(c, RVS_EINVAL, "rvs_is_valid: null output");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
     const int fp64 = prec_of(c, flags) == RVS_PREC_FP64;
@@ -887,7 +976,7 @@ rvs_status rvs_distance_underestimation(rvs_context* c, int32_t robot, int32_t e
     rvs_status s = check_query(c, robot, env, q, n, true);
     if (s != RVS_OK) return s;
     if (n > 0 && (!nearest_ref || !d)) return fail(c, RVS_EINVAL, "rvs_distance_underestimation: null argument");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
     const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
@@ -936,7 +1025,7 @@ rvs_status rvs_distance_underestimation_moving(rvs_context* c, int32_t robot, in
     if (s != RVS_OK) return s;
     if (n > 0 && (!nearest_ref || !d || !max_vel))
         return fail(c, RVS_EINVAL, "rvs_distance_underestimation_moving: null argument");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
     const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
@@ -993,7 +1082,7 @@ rvs_status rvs_trajectory_is_safe(rvs_context* c, int32_t robot, int32_t env, co
     if (num_points < 0) return fail(c, RVS_EINVAL, "rvs_trajectory_is_safe: negative point count");
     if (num_traj > 0 && (!q_current || !profile_current || !nearest_current || !max_vel || !safe))
         return fail(c, RVS_EINVAL, "rvs_trajectory_is_safe: null argument");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
     if (rb.num_links != rb.num_dofs)
@@ -1049,7 +1138,7 @@ rvs_status rvs_trajectory_is_safe(rvs_context* c, int32_t robot, int32_t env, co
 rvs_status rvs_self_distance(rvs_context* c, int32_t robot, const float* q, int64_t n, float* dist, uint32_t flags) {
     rvs_status s = check_query(c, robot, -1, q, n, false);
     if (s != RVS_OK) return s;
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     if (n == 0 || rb.num_pairs == 0) return RVS_OK;
     if (!dist) return fail(c, RVS_EINVAL, "rvs_self_distance: null output");
@@ -1080,7 +1169,7 @@ rvs_status rvs_self_collision(rvs_context* c, int32_t robot, const float* q, int
     rvs_status s = check_query(c, robot, -1, q, n, false);
     if (s != RVS_OK) return s;
     if (n > 0 && !min_dist && !free_flag) return fail(c, RVS_EINVAL, "rvs_self_collision: no output");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     if (n == 0) return RVS_OK;
     (void)hipSetDevice(c->opt.device);
@@ -1114,7 +1203,7 @@ rvs_status rvs_enclosing_radii(rvs_context* c, int32_t robot, const float* q, in
     rvs_status s = check_query(c, robot, -1, q, n, false);
     if (s != RVS_OK) return s;
     if (n > 0 && !R) return fail(c, RVS_EINVAL, "rvs_enclosing_radii: null output");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     if (rb.num_links != rb.num_dofs)
         return fail(c, RVS_ENOTSUP, "rvs_enclosing_radii: needs one link per revolute joint");
@@ -1149,7 +1238,7 @@ rvs_status rvs_extend_spines(rvs_context* c, int32_t robot, const float* q, cons
     if (n > 0 && (!d || !q_e || !q_new || !status || (use_expanded_bubble && !profile)))
         return fail(c, RVS_EINVAL, "rvs_extend_spines: null argument");
     if (num_iter < 1) return fail(c, RVS_EINVAL, "rvs_extend_spines: num_iter must be at least 1");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     if (rb.num_links != rb.num_dofs)
         return fail(c, RVS_ENOTSUP, "rvs_extend_spines: needs one link per revolute joint");
@@ -1209,7 +1298,7 @@ rvs_status rvs_extend_gen_spines(rvs_context* c, int32_t robot, int32_t env, con
     if (n > 0 && (!q_root || !q_new || !status || !d)) return fail(c, RVS_EINVAL, "rvs_extend_gen_spines: null argument");
     if (num_layers < 1 || num_iter < 1)
         return fail(c, RVS_EINVAL, "rvs_extend_gen_spines: num_layers and num_iter must be at least 1");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
     if (rb.num_links != rb.num_dofs)
@@ -1314,7 +1403,7 @@ rvs_status rvs_edge_self_collision(rvs_context* c, int32_t robot, const float* q
     if (s != RVS_OK) return s;
     if (n > 0 && (!q2 || !collide || !q2_out)) return fail(c, RVS_EINVAL, "rvs_edge_self_collision: null argument");
     if (!(eps_step > 0.0f)) return fail(c, RVS_EINVAL, "rvs_edge_self_collision: eps_step must be positive");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     if (rb.dev.kind != RVS_SKEL_XARM6 || rb.num_links != 6 || rb.num_dofs != 6)
         return fail(c, RVS_ENOTSUP, "rvs_edge_self_collision: the xArm6 pair rule only");
@@ -1355,7 +1444,7 @@ rvs_status rvs_edge_is_valid(rvs_context* c, int32_t robot, int32_t env, const f
     rvs_status s = check_query(c, robot, env, q1, n, true);
     if (s != RVS_OK) return s;
     if (n > 0 && (!q2 || !valid)) return fail(c, RVS_EINVAL, "rvs_edge_is_valid: null argument");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     const RobotRec& rb = c->robots[robot];
     const EnvRec& ev = c->envs[env];
     if (prec_of(c, flags) == RVS_PREC_FP64) return fail(c, RVS_ENOTSUP, "rvs_edge_is_valid: reference arithmetic only");
@@ -1482,7 +1571,7 @@ rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* 
     if (num_edges > 0 && (!robots || !q1 || !dir || !K || !first))
         return fail(c, RVS_EINVAL, "rvs_edge_first_collision: null argument");
     if (num_edges > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: too many edges");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     if (!env_ok(c, env)) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: bad env id");
     const EnvRec& ev = c->envs[env];
     if (prec_of(c, flags) == RVS_PREC_FP64)
@@ -1536,7 +1625,7 @@ rvs_status rvs_set_edge_tuning(rvs_context* c, const rvs_edge_tuning* t) {
     if (!c || !t) return fail(c, RVS_EINVAL, "rvs_set_edge_tuning: null argument");
     if (t->waves_per_cu < 1 || t->waves_per_cu > 64 || t->slots_per_lane < 1 || t->slots_per_lane > 16)
         return fail(c, RVS_EINVAL, "rvs_set_edge_tuning: waves_per_cu 1..64, slots_per_lane 1..16");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     c->edge_waves_per_cu = t->waves_per_cu;
     c->edge_spl = t->slots_per_lane;
     return RVS_OK;
@@ -1548,7 +1637,7 @@ rvs_status rvs_autotune_edges(rvs_context* c, int32_t env, const int32_t* robots
     if (!c) return RVS_EINVAL;
     if (num_edges <= 0 || num_edges > INT32_MAX || !robots || !q1 || !dir || !K)
         return fail(c, RVS_EINVAL, "rvs_autotune_edges: bad arguments");
-    std::lock_guard<std::mutex> lk(c->mu);
+    CtxLock lk(c);
     if (!env_ok(c, env)) return fail(c, RVS_EINVAL, "rvs_autotune_edges: bad env id");
     const EnvRec& ev = c->envs[env];
     EdgePlan pl;

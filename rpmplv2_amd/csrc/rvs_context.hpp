@@ -65,6 +65,18 @@ struct rvs_context {
     std::vector<rvs::EnvRec> envs;
     rvs::DevBuf arena;
     int64_t last_refined = 0;
+    // Asynchronous calls (rvs_*_async) enqueue on a caller's stream and
+    // return; async_ev is recorded after each of them (each first waits for
+    // the previous one, so the event covers them all).  Every other call
+    // waits for it on the host before it touches the arena or the tables.
+    hipEvent_t async_ev = nullptr;
+    bool async_pending = false;
+    void drain_async() {
+        if (async_pending) {
+            (void)hipEventSynchronize(async_ev);
+            async_pending = false;
+        }
+    }
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     float stage_ms[3] = {0, 0, 0};
     int32_t nstages = 0;
@@ -103,6 +115,14 @@ struct rvs_context {
 };
 
 namespace rvs {
+
+// The context's lock for a synchronous call: it also waits for the
+// asynchronous calls still in flight, which may use the arena and read the
+// robot and obstacle tables.
+struct CtxLock {
+    std::lock_guard<std::mutex> lk;
+    explicit CtxLock(rvs_context* c) : lk(c->mu) { c->drain_async(); }
+};
 
 // Sub-allocation of one device block, 256-byte aligned pieces.
 struct Carve {

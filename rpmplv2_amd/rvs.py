@@ -106,7 +106,8 @@ RVS_EBUSY = 6
 EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error", "rvs_stream", "rvs_last_refined",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds",
            "rvs_env_set_motion_law", "rvs_env_step", "rvs_distance",
-           "rvs_distance_f64", "rvs_is_valid", "rvs_distance_underestimation", "rvs_distance_underestimation_moving",
+           "rvs_distance_f64", "rvs_is_valid", "rvs_distance_async", "rvs_is_valid_async",
+           "rvs_distance_underestimation", "rvs_distance_underestimation_moving",
            "rvs_trajectory_is_safe", "rvs_self_distance",
            "rvs_self_collision", "rvs_enclosing_radii", "rvs_extend_spines", "rvs_extend_gen_spines",
            "rvs_edge_self_collision", "rvs_edge_is_valid", "rvs_edge_first_collision", "rvs_last_timing",
@@ -115,6 +116,21 @@ EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error",
            "rvs_server_last_error", "rvs_host_alloc", "rvs_host_free"]
 
 _lib = None
+
+
+def _stream_handle(stream):
+    """hipStream_t of a torch.cuda.Stream (None stays None)."""
+    return None if stream is None else C.c_void_p(stream.cuda_stream)
+
+
+def _event_handle(event):
+    """hipEvent_t of a torch.cuda.Event, created by a first record if torch
+    has not created it yet (None stays None)."""
+    if event is None:
+        return None
+    if not event.cuda_event:
+        event.record()
+    return C.c_void_p(event.cuda_event)
 
 
 def lib():
@@ -156,6 +172,8 @@ def lib():
         L.rvs_distance.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, vp, u32]
         L.rvs_distance_f64.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, u32]
         L.rvs_is_valid.argtypes = [ctx, i32, i32, vp, i64, vp, u32]
+        L.rvs_distance_async.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, u32, vp, vp]
+        L.rvs_is_valid_async.argtypes = [ctx, i32, i32, vp, i64, vp, u32, vp, vp]
         L.rvs_distance_underestimation.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, vp, vp, u32]
         L.rvs_distance_underestimation_moving.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, C.c_float, vp, vp, vp,
                                                           u32]
@@ -505,6 +523,23 @@ class Engine:
               | (F_EAGER if eager else 0))
         self._check(lib().rvs_distance(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d), _ptr(profile),
                                        _ptr(nearest), _ptr(verdict), fl), "rvs_distance")
+
+    def distance_async(self, robot: int, env: int, q, d, profile=None, verdict=None, precision: Optional[int] = None,
+                       stream=None, done=None):
+        """rvs_distance_async on device tensors: enqueued on `stream` (a
+        torch.cuda.Stream; None = the context's stream), returns at once;
+        `done` (a torch.cuda.Event, may be None) is recorded after the work."""
+        fl = F_DEVICE | (F_PREC(precision) if precision is not None else 0)
+        self._check(lib().rvs_distance_async(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d), _ptr(profile),
+                                             _ptr(verdict), fl, _stream_handle(stream), _event_handle(done)),
+                    "rvs_distance_async")
+
+    def is_valid_async(self, robot: int, env: int, q, valid, precision: Optional[int] = None, stream=None,
+                       done=None):
+        """rvs_is_valid_async on device tensors (see distance_async)."""
+        fl = F_DEVICE | (F_PREC(precision) if precision is not None else 0)
+        self._check(lib().rvs_is_valid_async(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(valid), fl,
+                                             _stream_handle(stream), _event_handle(done)), "rvs_is_valid_async")
 
     def distance_f64_device(self, robot: int, env: int, q, d, profile=None, verdict=None, timing: bool = False):
         fl = F_DEVICE | (F_TIMING if timing else 0)
