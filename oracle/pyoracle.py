@@ -329,8 +329,9 @@ def capsule_obb(A, B, r, obb, precision=32, params=None):
 
 
 def distance_cull(rb, scene: Scene, q, precision=32, params=None, nthreads=8):
-    """computeDistance with the broad phase; returns (d, profile, verdict,
-    (pairs evaluated, pairs skipped))."""
+    """computeDistance walked as the engine's bulk pass walks it (warm start,
+    grouped broad phase); returns (d, profile, verdict, (pairs evaluated,
+    bounding-box tests))."""
     p = params or params_struct()
     q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
     n, L = len(q), rb.num_links

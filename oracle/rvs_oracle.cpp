@@ -701,8 +701,58 @@ template <typename T> void cull_box(const SceneT<T>& sc, int j, T* lo, T* hi) {
 }
 
 struct CullCount {
-    int64_t evaluated = 0, culled = 0;
+    int64_t evaluated = 0, tests = 0;
 };
+
+// The engine's broad-phase walk order (restated): obstacles sorted stably by
+// the Morton code, 10 bits per axis over the scene's extent, of their cull
+// box centres (in double; non-finite centres get code 0), in groups of 4
+// whose union boxes are tested before their members.
+constexpr int kGroup = 4;
+template <typename T> struct WalkOrder {
+    std::vector<int> order;
+    std::vector<T> glo, ghi;   // [groups][3]
+};
+template <typename T> WalkOrder<T> walk_order(const SceneT<T>& sc) {
+    const int M = sc.M;
+    WalkOrder<T> wo;
+    std::vector<T> lo(3 * (size_t)M), hi(3 * (size_t)M);
+    std::vector<double> cen(3 * (size_t)M);
+    double blo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, bhi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int j = 0; j < M; ++j) {
+        cull_box(sc, j, &lo[3 * j], &hi[3 * j]);
+        for (int k = 0; k < 3; ++k) {
+            const double c = 0.5 * ((double)lo[3 * j + k] + (double)hi[3 * j + k]);
+            cen[3 * j + k] = c;
+            if (std::isfinite(c)) { blo[k] = std::min(blo[k], c); bhi[k] = std::max(bhi[k], c); }
+        }
+    }
+    std::vector<uint64_t> key(M);
+    wo.order.resize(M);
+    for (int j = 0; j < M; ++j) {
+        uint64_t code = 0;
+        for (int k = 0; k < 3; ++k) {
+            const double c = cen[3 * j + k];
+            uint64_t u = 0;
+            if (std::isfinite(c) && bhi[k] > blo[k])
+                u = (uint64_t)std::min(1023.0, std::max(0.0, std::floor((c - blo[k]) / (bhi[k] - blo[k]) * 1024.0)));
+            for (int b = 0; b < 10; ++b) code |= ((u >> b) & 1u) << (3 * b + k);
+        }
+        key[j] = code;
+        wo.order[j] = j;
+    }
+    std::stable_sort(wo.order.begin(), wo.order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    const int ng = (M + kGroup - 1) / kGroup;
+    wo.glo.assign(3 * (size_t)ng, std::numeric_limits<T>::infinity());
+    wo.ghi.assign(3 * (size_t)ng, -std::numeric_limits<T>::infinity());
+    for (int g = 0; g < ng; ++g)
+        for (int m = g * kGroup; m < std::min(M, (g + 1) * kGroup); ++m)
+            for (int k = 0; k < 3; ++k) {
+                wo.glo[3 * g + k] = std::min(wo.glo[3 * g + k], lo[3 * wo.order[m] + k]);
+                wo.ghi[3 * g + k] = std::max(wo.ghi[3 * g + k], hi[3 * wo.order[m] + k]);
+            }
+    return wo;
+}
 
 // Filtered distance of one (link, obstacle) pair as the reference's loop
 // computes it (ground skipped for the first ground_included links, values
@@ -720,17 +770,18 @@ T pair_filtered(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, const 
 }
 
 // computeDistance: RealVectorSpace.cpp:318-392 (fresh state, compute_again).
-// With cull (and no nearest points) this is the engine's bulk-pass walk
-// instead: pairs whose bounding-box gap exceeds max(running link minimum + r,
-// EQUALITY_THRESHOLD) + 1e-5 are skipped, and (in scenes of at most 128
-// obstacles) each link first evaluates the obstacle that held the previous
-// link's minimum (warm start); if that pair is a hit, the pairs before it
-// (gap bound of a running minimum of 0) are searched for an earlier hit.
-// Its results must equal the plain loop's.
+// With walk (and no nearest points) this is the engine's bulk-pass walk
+// instead: each link first evaluates the obstacle that held the previous
+// link's minimum (warm start), then walks the obstacles in the walk order,
+// skipping a group of 4 whose union box, or an obstacle whose cull box, lies
+// farther from the segment's bounding box than max(running link minimum + r,
+// EQUALITY_THRESHOLD) + 1e-5; a hit found this way is replaced by the first
+// hit in obstacle order, searched among the obstacles before it with the
+// bound of a running minimum of 0.  Its results must equal the plain loop's.
 template <typename T>
 void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, const T* Pk,
                   T* d_out, T* prof, T* np_out /* M*L*6 or null */, uint8_t* verdict,
-                  CullCount* cull = nullptr) {
+                  CullCount* cull = nullptr, const WalkOrder<T>* wo = nullptr) {
     const int L = rb->num_links;
     T d_c = pr.maxd;
     for (int i = 0; i < L; ++i) prof[i] = 0;
@@ -741,61 +792,62 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
         const T* A = Pk + 3 * i;
         const T* B = Pk + 3 * (i + 1);
         T r = (T)rb->radius[i];
-        int best = -1;
-        const bool walk = cull && !np_out;
-        const int w = (walk && sc.M <= 128) ? warm : -1;   // warm start only for scenes of <= 128 obstacles
-        if (w >= 0) {
-            cull->evaluated++;
-            const T dw = pair_filtered(rb, pr, sc, A, B, i, w);
-            if (dw <= 0) {
-                T first = dw;
-                for (int j = 0; j < w; ++j) {
+        if (cull && wo && !np_out) {
+            auto far = [&](const T* lo, const T* hi, T bound) {
+                if (seg_box_gap(A, B, lo, hi) > std::max(bound + r, pr.eq) + T(1e-5)) return true;
+                return false;
+            };
+            int best = -1, jh = -1;
+            T vh = 0;
+            const int w = warm;
+            if (w >= 0) {
+                cull->evaluated++;
+                const T dw = pair_filtered(rb, pr, sc, A, B, i, w);
+                if (dw <= 0) { jh = w; vh = dw; }
+                else if (dw < prof[i]) { prof[i] = dw; best = w; }
+            }
+            const int ng = (int)wo->glo.size() / 3;
+            for (int g = 0; g < ng && jh < 0; ++g) {
+                cull->tests++;
+                if (far(&wo->glo[3 * g], &wo->ghi[3 * g], prof[i])) continue;
+                for (int m = g * kGroup; m < std::min(sc.M, (g + 1) * kGroup) && jh < 0; ++m) {
+                    const int j = wo->order[m];
+                    if (j == w) continue;
                     T lo[3], hi[3];
                     cull_box(sc, j, lo, hi);
-                    if (seg_box_gap(A, B, lo, hi) > std::max(T(0) + r, pr.eq) + T(1e-5)) {
-                        cull->culled++;
-                        continue;
-                    }
+                    cull->tests++;
+                    if (far(lo, hi, prof[i])) continue;
+                    cull->evaluated++;
+                    const T dt = pair_filtered(rb, pr, sc, A, B, i, j);
+                    if (dt < prof[i]) best = j;
+                    prof[i] = stdmin(prof[i], dt);
+                    if (prof[i] <= 0) { jh = j; vh = prof[i]; }
+                }
+            }
+            if (jh >= 0) {
+                T first = vh;
+                for (int j = 0; j < jh; ++j) {
+                    T lo[3], hi[3];
+                    cull_box(sc, j, lo, hi);
+                    cull->tests++;
+                    if (far(lo, hi, T(0))) continue;
                     cull->evaluated++;
                     const T dj = pair_filtered(rb, pr, sc, A, B, i, j);
-                    if (dj <= 0) {
-                        first = dj;
-                        break;
-                    }
+                    if (dj <= 0) { first = dj; break; }
                 }
                 prof[i] = first;
                 *d_out = 0;
                 *verdict = 0;
                 return;
             }
-            if (dw < prof[i]) {
-                prof[i] = dw;
-                best = w;
-            }
+            warm = best;
+            d_c = stdmin(d_c, prof[i]);
+            continue;
         }
         for (int j = 0; j < sc.M; ++j) {
             T d_tmp;
             T np[6];
             int v = 1;
-            if (walk) {
-                if (j == w) continue;
-                T lo[3], hi[3];
-                cull_box(sc, j, lo, hi);
-                if (seg_box_gap(A, B, lo, hi) > std::max(prof[i] + r, pr.eq) + T(1e-5)) {
-                    cull->culled++;
-                    continue;
-                }
-                cull->evaluated++;
-                d_tmp = pair_filtered(rb, pr, sc, A, B, i, j);
-                if (d_tmp < prof[i]) best = j;
-                prof[i] = stdmin(prof[i], d_tmp);
-                if (prof[i] <= 0) {
-                    *d_out = 0;
-                    *verdict = 0;
-                    return;
-                }
-                continue;
-            }
             if (sc.ground[j] && i < rb->ground_included) {
                 d_tmp = pr.maxd;
                 np[0] = np[1] = np[2] = 0; np[3] = 0; np[4] = 0; np[5] = -pr.maxd;
@@ -814,7 +866,6 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
             }
             if (np_out) std::memcpy(np_out + 6 * (j * L + i), np, sizeof(np));
         }
-        warm = best;
         d_c = stdmin(d_c, prof[i]);
     }
     *d_out = d_c;
@@ -855,6 +906,8 @@ void distance_batch(const orc_robot* rb, const orc_params* p, const orc_scene* s
     SceneT<T> sc(scn);
     std::vector<Seg> segs = build_segments(rb);
     const int L = rb->num_links, D = rb->num_dofs;
+    WalkOrder<T> wo;
+    if (counts) wo = walk_order(sc);
     std::mutex mu;
     parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
         std::vector<T> Pk(3 * (L + 1));
@@ -864,7 +917,7 @@ void distance_batch(const orc_robot* rb, const orc_params* p, const orc_scene* s
             skeleton<T>(rb, segs, q + c * D, Pk.data());
             T dd; uint8_t v;
             distance_one(rb, pr, sc, Pk.data(), &dd, prof.data(), np ? np + c * (int64_t)sc.M * L * 6 : nullptr, &v,
-                         counts ? &cc : nullptr);
+                         counts ? &cc : nullptr, counts ? &wo : nullptr);
             if (d) d[c] = dd;
             if (profile) for (int i = 0; i < L; ++i) profile[c * L + i] = prof[i];
             if (verdict) verdict[c] = v;
@@ -872,7 +925,7 @@ void distance_batch(const orc_robot* rb, const orc_params* p, const orc_scene* s
         if (counts) {
             std::lock_guard<std::mutex> lk(mu);
             counts[0] += cc.evaluated;
-            counts[1] += cc.culled;
+            counts[1] += cc.tests;
         }
     });
 }

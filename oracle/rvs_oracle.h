@@ -126,12 +126,14 @@ double orc_capsule_obb64(const double A[3], const double B[3], double r, const d
 void orc_self_min32(const orc_robot* rb, const orc_params* p, const float* q, int64_t n, const int32_t* pairs,
                     int32_t npairs, float* out, int32_t nthreads);
 
-/* computeDistance walked as the engine's bulk pass walks it: pairs whose
- * bounding-box gap exceeds max(running link minimum + r, EQUALITY_THRESHOLD)
- * + 1e-5 are skipped, and (scenes of at most 128 obstacles) each link starts
- * at the obstacle that held the previous link's minimum (a hit there triggers
- * a search for an earlier hit in obstacle order).  Results equal orc_distance32/64; counts = {pairs
- * evaluated, pairs skipped}. */
+/* computeDistance walked as the engine's bulk pass walks it: each link starts
+ * at the obstacle that held the previous link's minimum, then walks the
+ * obstacles in the engine's walk order (Morton order of the cull box centres,
+ * groups of 4 tested by their union box first), skipping boxes farther from
+ * the segment's bounding box than max(running link minimum + r,
+ * EQUALITY_THRESHOLD) + 1e-5; a hit found out of obstacle order is replaced
+ * by the first one in order.  Results equal orc_distance32/64; counts =
+ * {pairs evaluated, bounding-box tests (groups and obstacles)}. */
 void orc_distance_cull32(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q, int64_t n,
                          float* d, float* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads);
 void orc_distance_cull64(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q, int64_t n,

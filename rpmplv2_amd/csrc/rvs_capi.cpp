@@ -158,6 +158,53 @@ void obb_extent(const float o[15], float ext[3]) {
     }
 }
 
+void group_order(const float* cull, int M, int32_t* order) {
+    std::vector<double> cen(3 * (size_t)M);
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int j = 0; j < M; ++j)
+        for (int k = 0; k < 3; ++k) {
+            const double c = 0.5 * ((double)cull[6 * j + k] + (double)cull[6 * j + 3 + k]);
+            cen[3 * j + k] = c;
+            if (std::isfinite(c)) {
+                lo[k] = std::min(lo[k], c);
+                hi[k] = std::max(hi[k], c);
+            }
+        }
+    std::vector<uint64_t> key(M);
+    for (int j = 0; j < M; ++j) {
+        uint64_t code = 0;
+        for (int k = 0; k < 3; ++k) {
+            const double c = cen[3 * j + k];
+            uint64_t u = 0;
+            if (std::isfinite(c) && hi[k] > lo[k])
+                u = (uint64_t)std::min(1023.0, std::max(0.0, std::floor((c - lo[k]) / (hi[k] - lo[k]) * 1024.0)));
+            for (int b = 0; b < 10; ++b) code |= ((u >> b) & 1u) << (3 * b + k);
+        }
+        key[j] = code;
+        order[j] = j;
+    }
+    std::stable_sort(order, order + M, [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+}
+
+void group_bounds(const float* cull, const int32_t* order, int M, float* gcull) {
+    for (int g = 0; g < num_groups(M); ++g) {
+        float lo[3] = {HUGE_VALF, HUGE_VALF, HUGE_VALF}, hi[3] = {-HUGE_VALF, -HUGE_VALF, -HUGE_VALF};
+        bool finite = true;
+        for (int m = g * kGroupSize; m < std::min(M, (g + 1) * kGroupSize); ++m) {
+            const float* b = cull + 6 * (size_t)order[m];
+            for (int k = 0; k < 3; ++k) {
+                finite = finite && std::isfinite(b[k]) && std::isfinite(b[3 + k]);
+                lo[k] = std::min(lo[k], b[k]);
+                hi[k] = std::max(hi[k], b[3 + k]);
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            gcull[6 * g + k] = finite ? lo[k] : -HUGE_VALF;
+            gcull[6 * g + 3 + k] = finite ? hi[k] : HUGE_VALF;
+        }
+    }
+}
+
 QueryParams params_of(const rvs_context* c) {
     QueryParams qp;
     qp.eq = c->opt.equality_threshold;
@@ -360,6 +407,18 @@ rvs_status rvs_env_create(rvs_context* c, const rvs_box* boxes, int32_t num, int
     if (e == hipSuccess) e = upload(r.allocs, fl.data(), fl.size(), &r.dev.flags);
     if (e == hipSuccess) e = upload(r.allocs, r.obb.data(), r.obb.size(), &r.dev.obb);
     if (e == hipSuccess) e = upload(r.allocs, r.cull.data(), r.cull.size(), &r.dev.cull);
+    // the grouped walk of the broad phase over the boxes the kernels bound
+    // each obstacle by: the AABB itself (reference semantics) or the oriented
+    // box's tight world AABB
+    const float* cull_src = (semantics == RVS_ENV_AABB) ? r.aabb.data() : r.cull.data();
+    r.gorder.resize(num);
+    r.gcull.resize((size_t)num_groups(num) * 6);
+    if (num > 0) {
+        group_order(cull_src, num, r.gorder.data());
+        group_bounds(cull_src, r.gorder.data(), num, r.gcull.data());
+    }
+    if (e == hipSuccess) e = upload(r.allocs, r.gorder.data(), r.gorder.size(), &r.dev.gorder);
+    if (e == hipSuccess) e = upload(r.allocs, r.gcull.data(), r.gcull.size(), &r.dev.gcull);
     if (e != hipSuccess) {
         for (void* p : r.allocs) (void)hipFree(p);
         return hip_fail(c, e, "rvs_env_create");
@@ -402,6 +461,11 @@ rvs_status rvs_env_update(rvs_context* c, int32_t env_id, const int32_t* idx, co
         e = hipMemcpyAsync((void*)r.dev.obb, r.obb.data(), r.obb.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && r.dev.cull != r.dev.aabb)
         e = hipMemcpyAsync((void*)r.dev.cull, r.cull.data(), r.cull.size() * 4, hipMemcpyHostToDevice, c->stream);
+    // group boxes of the broad phase (the walk order stays)
+    group_bounds(r.dev.cull == r.dev.aabb ? r.aabb.data() : r.cull.data(), r.gorder.data(), (int)r.boxes.size(),
+                 r.gcull.data());
+    if (e == hipSuccess && !r.gcull.empty())
+        e = hipMemcpyAsync((void*)r.dev.gcull, r.gcull.data(), r.gcull.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_env_update");
     return RVS_OK;
@@ -469,6 +533,8 @@ rvs_status rvs_env_step(rvs_context* c, int32_t env_id, float delta_time, int32_
     float* obb = const_cast<float*>(r.dev.obb);
     float* cull = (r.dev.cull == r.dev.aabb) ? nullptr : const_cast<float*>(r.dev.cull);
     hipError_t e = launch_env_step(r.step, delta_time, steps, aabb, obb, cull, c->stream);
+    if (e == hipSuccess && M > 0)
+        e = launch_group_bounds(r.dev.cull, r.dev.gorder, M, const_cast<float*>(r.dev.gcull), c->stream);
     // keep the host copies (bounds, box positions) in step with the device
     std::vector<float> p(3 * (size_t)M);
     if (e == hipSuccess) e = hipMemcpyAsync(p.data(), r.step.pos, p.size() * 4, hipMemcpyDeviceToHost, c->stream);

@@ -48,6 +48,8 @@ struct EnvRec {
     std::vector<void*> allocs;
     std::vector<rvs_box> boxes;
     std::vector<float> aabb, obb, cull;   // host copies of the device tables
+    std::vector<int32_t> gorder;          // broad-phase walk order (host copy)
+    std::vector<float> gcull;             // group boxes of the walk (host staging)
     int semantics = RVS_ENV_AABB;
     bool has_motion = false;              // rvs_env_set_motion_law was called
     EnvStepDev step{};                    // motion-law state on the device

@@ -147,36 +147,38 @@ __device__ __attribute__((always_inline)) inline float fast_pair(const Staged<fl
 
 // Smallest filtered distance of one link over the obstacles, with the
 // reference's early exit: the first pair in obstacle order whose distance is
-// <= 0 ends the link (hit) and its distance is the link's value.  The pair
-// that held the previous link's minimum (warm, -1 = none) is evaluated first:
-// neighbouring links are usually nearest to the same obstacle, so the running
-// minimum starts close to its final value and the broad phase skips most of
-// the other pairs.  If the warm pair is a hit, the pairs before it in
-// obstacle order are searched for an earlier one; a bound of running minimum
-// 0 never skips a pair whose exact distance can be <= 0.  Otherwise the
-// obstacles are walked in order as usual, so the result (the minimum, or the
-// first hit's distance) is the in-order one.  On return warm holds this
-// link's nearest obstacle.  Called by the whole wave; lanes with active false
-// evaluate nothing.  Both bulk kernels walk a configuration's pairs through
-// this function, so they evaluate the same pairs, flag the same
-// configurations for the exact pass and return the same bits.  Scenes with
-// more than kWarmMaxObjects obstacles walk in order only (WARM false, chosen
-// at launch): there the walk's bounding-box tests dominate, and the warm pair
-// (a different obstacle in every lane) costs a fully divergent evaluation per
-// link.
-constexpr int kWarmMaxObjects = 128;
-
-template <bool OBB, bool WARM>
-__device__ __attribute__((always_inline)) inline float link_min(const Staged<float>& s, const float* __restrict__ cull,
-                                                                int M, bool gnd_skip, const Vec3<float>& P0,
+// <= 0 ends the link (hit) and its distance is the link's value.
+//
+// GROUPED (the walk of every scene with group tables): the pair that held
+// the previous link's minimum (warm, -1 = none) is evaluated first, since
+// neighbouring links are usually nearest to the same obstacle; then the
+// obstacles are walked in the scene's Morton order, kGroupSize at a time, a
+// group's union box tested before its members.  With a running minimum near
+// its final value most groups fail the gap test at once.  The minimum of the
+// evaluated pairs is the link's minimum (a skipped pair's distance exceeds
+// the running minimum).  A hit found this way need not be the first in
+// obstacle order, so the pairs before it are searched for an earlier one
+// with the bound of a running minimum of 0, which never skips a pair whose
+// exact distance can be <= 0: the value is the in-order one.  On return warm
+// holds this link's nearest obstacle.  Otherwise (no group tables) the
+// obstacles are walked in order.
+//
+// Called by the whole wave; lanes with active false evaluate nothing.  Both
+// bulk kernels walk a configuration's pairs through this function, so they
+// evaluate the same pairs, flag the same configurations for the exact pass
+// and return the same bits.
+template <bool OBB, bool GROUPED>
+__device__ __attribute__((always_inline)) inline float link_min(const Staged<float>& s, const EnvDev& env, int M,
+                                                                bool gnd_skip, const Vec3<float>& P0,
                                                                 const Vec3<float>& P1, float r, const Params<float>& p,
                                                                 float band, bool active, int& warm, bool& hit,
                                                                 bool& uncertain) {
+    const float* __restrict__ cull = env.cull;
     Vec3<float> slo, shi;
     seg_bounds(P0, P1, slo, shi);
     float prof = p.maxd;
     hit = false;
-    if (!WARM) {   // the plain in-order walk
+    if (!GROUPED) {   // the plain in-order walk
         bool go = active;
         for (int j = 0; j < M; ++j) {
             if (go && !cull_pair(cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
@@ -192,39 +194,54 @@ __device__ __attribute__((always_inline)) inline float link_min(const Staged<flo
         return prof;
     }
     int best = -1;
+    int jh = -1;         // a hit found out of obstacle order, and its value
+    float vh = 0.0f;
     const int w = active ? warm : -1;
     if (w >= 0) {
         const float dt = (s.gnd[w] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, w, P0, P1, r, p, band, uncertain);
         if (dt <= 0) {
-            float first = dt;
-            const float b0 = cull_bound(0.0f, r, p.eq);
-            for (int j = 0; j < w; ++j) {
-                if (cull_pair(cull + 6 * j, slo, shi, b0)) continue;
-                const float dj = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
-                if (dj <= 0) {
-                    first = dj;
-                    break;
-                }
-            }
-            prof = first;
-            hit = true;
+            jh = w;
+            vh = dt;
         } else if (dt < prof) {
             prof = dt;
             best = w;
         }
     }
-    bool go = active && !hit;
-    for (int j = 0; j < M; ++j) {
-        if (go && j != w && !cull_pair(cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
-            const float dt = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
-            if (dt < prof) best = j;
-            prof = smin(prof, dt);
-            if (prof <= 0) {
-                hit = true;
-                go = false;
+    bool go = active && jh < 0;
+    const int ng = num_groups(M);
+    for (int g = 0; g < ng; ++g) {
+        if (go && !cull_pair(env.gcull + 6 * g, slo, shi, cull_bound(prof, r, p.eq))) {
+            const int m1 = min(M, (g + 1) * kGroupSize);
+            for (int m = g * kGroupSize; m < m1; ++m) {
+                const int j = env.gorder[m];
+                if (go && j != w && !cull_pair(cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
+                    const float dt =
+                        (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+                    if (dt < prof) best = j;
+                    prof = smin(prof, dt);
+                    if (prof <= 0) {
+                        jh = j;
+                        vh = prof;
+                        go = false;
+                    }
+                }
             }
         }
         if (__ballot(go) == 0) break;
+    }
+    if (jh >= 0) {   // the first hit in obstacle order
+        float first = vh;
+        const float b0 = cull_bound(0.0f, r, p.eq);
+        for (int j = 0; j < jh; ++j) {
+            if (cull_pair(cull + 6 * j, slo, shi, b0)) continue;
+            const float dj = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+            if (dj <= 0) {
+                first = dj;
+                break;
+            }
+        }
+        prof = first;
+        hit = true;
     }
     warm = best;
     return prof;
@@ -253,7 +270,7 @@ __device__ __attribute__((always_inline)) inline void append_flagged(bool flag, 
 // its profile row there, writing it out in one burst at the end: the rows of
 // a wave's lanes are scattered in memory, and reading or writing them a
 // joint or a link at a time, far apart in time, costs a cache line each.
-template <bool OBB, bool ROWS, bool WARM>
+template <bool OBB, bool ROWS, bool GROUPED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
     uint32_t* count, const int32_t* __restrict__ perm, size_t rows_off) {
@@ -292,8 +309,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const Vec3<float> P1 = skelf_point(fk, rb, qc, i + 1, qs);
         if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
         const float r = rb.radius[i];
-        const float prof = link_min<OBB, WARM>(s, env.cull, M, i < rb.ground_included, P0, P1, r, p, band, true, warm,
-                                               hit, uncertain);
+        const float prof = link_min<OBB, GROUPED>(s, env, M, i < rb.ground_included, P0, P1, r, p, band, true, warm,
+                                                  hit, uncertain);
         if (ROWS) prow[i * B] = prof;
         else if (out.profile) out.profile[c * L + i] = prof;
         if (hit) break;
@@ -323,7 +340,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // and link).  Lanes therefore sit on different links of different
 // configurations; per configuration the pair order, the early exit and every
 // result are those of the classic kernel.
-template <bool OBB, bool WARM>
+template <bool OBB, bool GROUPED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_pool(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
     uint32_t* count, uint32_t* next, const int32_t* __restrict__ perm) {
@@ -384,8 +401,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             r = rb.radius[i];
         }
         bool hit = false;
-        const float prof = link_min<OBB, WARM>(s, env.cull, M, i < rb.ground_included, P0, P1, r, p, band, busy, warm,
-                                               hit, uncertain);
+        const float prof = link_min<OBB, GROUPED>(s, env, M, i < rb.ground_included, P0, P1, r, p, band, busy, warm,
+                                                  hit, uncertain);
         bool finished = false;
         if (busy) {
             if (out.profile) out.profile[c * L + i] = prof;
@@ -463,23 +480,25 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t& total) {
     return before + x - v;
 }
 
-constexpr int kCohThreads = 256;                      // threads of the tile kernels (one per bucket in a row)
-constexpr int kCohPer = kCohTile / kCohThreads;       // configurations per thread of a tile
+constexpr int kCohThreads = 256;                      // threads of the per-bucket scan
+constexpr int kCohTileThreads = 1024;                 // threads of the tile kernels (the first 256: one per bucket)
+constexpr int kCohPer = kCohTile / kCohTileThreads;   // configurations per thread of a tile
 constexpr int kCohSortThreads = 1024;                 // threads of the low-bit sort (one per low key)
-static_assert(kCohBuckets == kCohThreads, "one thread per bucket");
+static_assert(kCohBuckets <= kCohTileThreads, "one thread per bucket");
 static_assert((1 << kCohLoBits) <= kCohSortThreads, "one thread per low key");
 
 // Pass 1: keys of a tile of kCohTile configurations and the histogram of
 // their buckets (key >> lo), stored bucket-major: hist[bucket * tiles + tile].
-__global__ __launch_bounds__(kCohThreads) void k_coh_hist(const float* __restrict__ q, int64_t n, int D, int nj,
-                                                          int bits, int lo, int64_t tiles, uint32_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kCohTileThreads) void k_coh_hist(const float* __restrict__ q, int64_t n, int D, int nj,
+                                                              int bits, int lo, int64_t tiles,
+                                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ hist) {
     __shared__ uint32_t cnt[kCohBuckets];
-    cnt[threadIdx.x] = 0;
+    const int t = threadIdx.x;
+    if (t < kCohBuckets) cnt[t] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kCohTile;
     for (int k = 0; k < kCohPer; ++k) {
-        const int64_t c = base + k * kCohThreads + threadIdx.x;
+        const int64_t c = base + k * kCohTileThreads + t;
         if (c < n) {
             const uint32_t key = coh_key(q + c * D, nj, bits);
             keys[c] = key;
@@ -487,7 +506,7 @@ __global__ __launch_bounds__(kCohThreads) void k_coh_hist(const float* __restric
         }
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+    if (t < kCohBuckets) hist[(int64_t)t * tiles + blockIdx.x] = cnt[t];
 }
 
 // Pass 2: one workgroup per bucket turns its row of tile counts into
@@ -510,19 +529,21 @@ __global__ __launch_bounds__(kCohThreads) void k_coh_scan(uint32_t* __restrict__
 // Pass 3: every configuration of a tile goes to its bucket (bucket start +
 // the tile's offset in the bucket + its rank among the tile's members); the
 // low key bits travel with it.  With lo == 0 this is the final order.
-__global__ __launch_bounds__(kCohThreads) void k_coh_bucket(const uint32_t* __restrict__ keys, int64_t n, int lo,
-                                                            const uint32_t* __restrict__ hist, int64_t tiles,
-                                                            const uint32_t* __restrict__ sizes,
-                                                            uint16_t* __restrict__ key_lo, int32_t* __restrict__ dst) {
+__global__ __launch_bounds__(kCohTileThreads) void k_coh_bucket(const uint32_t* __restrict__ keys, int64_t n,
+                                                                int lo, const uint32_t* __restrict__ hist,
+                                                                int64_t tiles, const uint32_t* __restrict__ sizes,
+                                                                uint16_t* __restrict__ key_lo,
+                                                                int32_t* __restrict__ dst) {
     __shared__ uint32_t next[kCohBuckets];
+    const int t = threadIdx.x;
     uint32_t tot;
-    const uint32_t start = block_exclusive_scan<kCohThreads>(sizes[threadIdx.x], tot);
-    next[threadIdx.x] = start + hist[(int64_t)threadIdx.x * tiles + blockIdx.x];
+    const uint32_t start = block_exclusive_scan<kCohTileThreads>(t < kCohBuckets ? sizes[t] : 0u, tot);
+    if (t < kCohBuckets) next[t] = start + hist[(int64_t)t * tiles + blockIdx.x];
     __syncthreads();
     const uint32_t mask = (1u << lo) - 1u;
     const int64_t base = (int64_t)blockIdx.x * kCohTile;
     for (int k = 0; k < kCohPer; ++k) {
-        const int64_t c = base + k * kCohThreads + threadIdx.x;
+        const int64_t c = base + k * kCohTileThreads + t;
         if (c < n) {
             const uint32_t key = keys[c];
             const uint32_t pos = atomicAdd(&next[key >> lo], 1u);
@@ -587,7 +608,7 @@ __global__ __launch_bounds__(kCohSortThreads) void k_coh_lowsort(const uint32_t*
 int fast_pool_blocks_per_cu(const EnvDev& env, int block) {
     int nb = 0;
     const size_t sm = staged_bytes<float>(env);
-    const bool warm = env.num_obj <= kWarmMaxObjects;
+    const bool warm = env.gorder && env.gcull;
     hipError_t e;
     if (env.use_obb)
         e = warm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_pool<true, true>, block, sm)
@@ -614,10 +635,10 @@ hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n,
     const int lo = std::max(0, keybits - kCohHiBits);
     const CohLayout L = coh_layout(coh, n);
     const unsigned tiles = (unsigned)L.tiles;
-    hipLaunchKernelGGL(k_coh_hist, dim3(tiles), dim3(kCohThreads), 0, st, q, n, rb.num_dofs, nj, bits, lo, L.tiles,
+    hipLaunchKernelGGL(k_coh_hist, dim3(tiles), dim3(kCohTileThreads), 0, st, q, n, rb.num_dofs, nj, bits, lo, L.tiles,
                        L.keys, L.hist);
     hipLaunchKernelGGL(k_coh_scan, dim3(kCohBuckets), dim3(kCohThreads), 0, st, L.hist, L.tiles, L.sizes);
-    hipLaunchKernelGGL(k_coh_bucket, dim3(tiles), dim3(kCohThreads), 0, st, L.keys, n, lo, L.hist, L.tiles, L.sizes,
+    hipLaunchKernelGGL(k_coh_bucket, dim3(tiles), dim3(kCohTileThreads), 0, st, L.keys, n, lo, L.hist, L.tiles, L.sizes,
                        L.key_lo, lo ? L.idx : L.perm);
     if (lo) {
         const unsigned grid = (unsigned)(kCohBuckets + (n + kCohChunk - 1) / kCohChunk);
@@ -631,7 +652,7 @@ hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n,
 
 namespace {
 
-template <bool OBB, bool WARM>
+template <bool OBB, bool GROUPED>
 hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
                              const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
                              const int32_t* perm, int num_cus, hipStream_t st) {
@@ -643,7 +664,7 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
         per_cu = std::max(1, per_cu);
         const int64_t blocks = std::min<int64_t>((int64_t)num_cus * per_cu, (n + B - 1) / B);
         const dim3 g((unsigned)std::max<int64_t>(1, blocks)), b(B);
-        hipLaunchKernelGGL((k_distance_fast_pool<OBB, WARM>), g, b, sm, st, rb, env, qp, q, n, out, idx, count,
+        hipLaunchKernelGGL((k_distance_fast_pool<OBB, GROUPED>), g, b, sm, st, rb, env, qp, q, n, out, idx, count,
                            count + 32, perm);
         return hipGetLastError();
     }
@@ -652,10 +673,10 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
     const size_t rows_off = (sm + 15) & ~size_t(15);
     const size_t rows_sm = rows_off + (size_t)(rb.num_dofs + rb.num_links) * 4 * B;
     if (perm && rows_sm <= 64 * 1024)
-        hipLaunchKernelGGL((k_distance_fast<OBB, true, WARM>), g, b, rows_sm, st, rb, env, qp, q, n, out, idx, count,
+        hipLaunchKernelGGL((k_distance_fast<OBB, true, GROUPED>), g, b, rows_sm, st, rb, env, qp, q, n, out, idx, count,
                            perm, rows_off);
     else
-        hipLaunchKernelGGL((k_distance_fast<OBB, false, WARM>), g, b, sm, st, rb, env, qp, q, n, out, idx, count,
+        hipLaunchKernelGGL((k_distance_fast<OBB, false, GROUPED>), g, b, sm, st, rb, env, qp, q, n, out, idx, count,
                            perm, rows_off);
     return hipGetLastError();
 }
@@ -666,7 +687,7 @@ hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const Que
                                 int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
                                 const int32_t* perm, int num_cus, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    const bool warm = env.num_obj <= kWarmMaxObjects;
+    const bool warm = env.gorder && env.gcull;   // the grouped walk when the scene has group tables
     if (env.use_obb)
         return warm ? launch_fast_shape<true, true>(rb, env, qp, q, n, out, idx, count, t, perm, num_cus, st)
                     : launch_fast_shape<true, false>(rb, env, qp, q, n, out, idx, count, t, perm, num_cus, st);

@@ -112,17 +112,19 @@ __host__ __device__ __attribute__((always_inline)) inline void fk_init(FKStream&
     s.qi = 0;
 }
 
-// Apply segments until frame 'upto' is the current frame.  Joint k of the
-// configuration is q[k * qstride].
-__host__ __device__ __attribute__((always_inline)) inline void fk_advance(FKStream& s, const SegDev* seg,
-                                                                        const float* q, int upto, int qstride = 1) {
+// Apply segments until frame 'upto' is the current frame.  sc(k, &s, &c)
+// gives the sine and cosine of joint k (in double, of the joint's float
+// value).
+template <class SinCos>
+__host__ __device__ __attribute__((always_inline)) inline void fk_advance_sc(FKStream& s, const SegDev* seg, int upto,
+                                                                           const SinCos& sc) {
     while (s.seg <= upto) {
         const SegDev& g = seg[s.seg];
         const Mat3 R = mat_load(g.R);
         Mat3 SM;
         if (g.rev) {
             double st, ct;
-            sincos((double)q[(s.qi++) * qstride], &st, &ct);
+            sc(s.qi++, &st, &ct);
             SM = mat_mul(rot2(g.axis, st, ct), R);
         } else {
             SM = R;
@@ -140,13 +142,20 @@ __host__ __device__ __attribute__((always_inline)) inline void fk_advance(FKStre
     }
 }
 
-// Skeleton point k (points must be requested in increasing k).
-template <typename T>
-__host__ __device__ __attribute__((always_inline)) inline Vec3<T> skel_point(FKStream& s, const RobotDev& rb,
-                                                                           const float* q, int k, int qstride = 1) {
+// The same with the joint values read from q: joint k is q[k * qstride].
+__host__ __device__ __attribute__((always_inline)) inline void fk_advance(FKStream& s, const SegDev* seg,
+                                                                        const float* q, int upto, int qstride = 1) {
+    fk_advance_sc(s, seg, upto, [&](int k, double* st, double* ct) { sincos((double)q[k * qstride], st, ct); });
+}
+
+// Skeleton point k from the frames of an FK stream advanced by sc (points
+// must be requested in increasing k).
+template <typename T, class SinCos>
+__host__ __device__ __attribute__((always_inline)) inline Vec3<T> skel_point_sc(FKStream& s, const RobotDev& rb, int k,
+                                                                              const SinCos& sc) {
     const PointDev pt = rb.pts[k];
     if (pt.frame < 0) return Vec3<T>{T(0), T(0), T(0)};
-    fk_advance(s, rb.seg, q, pt.frame, qstride);
+    fk_advance_sc(s, rb.seg, pt.frame, sc);
     double x = s.px, y = s.py, z = s.pz;
     // column 'axis' of the frame, picked with one-hot weights (exact: the
     // other two terms are signed zeros) rather than an indexed load, which
@@ -169,6 +178,15 @@ __host__ __device__ __attribute__((always_inline)) inline Vec3<T> skel_point(FKS
         P.z = P.z - f * (T)az;
     }
     return P;
+}
+
+// Skeleton point k with the joint values read from q (joint m is
+// q[m * qstride]; points must be requested in increasing k).
+template <typename T>
+__host__ __device__ __attribute__((always_inline)) inline Vec3<T> skel_point(FKStream& s, const RobotDev& rb,
+                                                                           const float* q, int k, int qstride = 1) {
+    return skel_point_sc<T>(s, rb, k,
+                            [&](int m, double* st, double* ct) { sincos((double)q[m * qstride], st, ct); });
 }
 
 }  // namespace rvs

@@ -23,7 +23,25 @@ struct EnvDev {
     const float* obb;         // [M][15] center(3), rotation rows(9), half extents(3)
     const float* cull;        // [M][6] box containing the obstacle for the broad phase:
                               // the AABB (reference semantics) or the OBB's tight world AABB
+    const int32_t* gorder;    // [M] broad-phase walk order (obstacles by Morton code of their cull box
+                              // centres); null = obstacle order
+    const float* gcull;       // [ceil(M / kGroupSize)][6] union of the cull boxes of each group of
+                              // kGroupSize consecutive entries of gorder
 };
+
+// Obstacles per group of the broad phase's grouped walk.
+constexpr int kGroupSize = 4;
+__host__ __device__ inline int num_groups(int M) { return (M + kGroupSize - 1) / kGroupSize; }
+
+// Walk order of the grouped broad phase: obstacles sorted (stably) by the
+// Morton code, 10 bits per axis over the scene's extent, of their cull box
+// centres (computed in double).  Non-finite centres go to code 0.
+void group_order(const float* cull, int M, int32_t* order);
+// Union of the cull boxes of every group of kGroupSize consecutive entries of
+// order; a group with a non-finite member bound gets an infinite box.
+void group_bounds(const float* cull, const int32_t* order, int M, float* gcull);
+// The same on the device (after the obstacles moved).
+hipError_t launch_group_bounds(const float* cull, const int32_t* order, int M, float* gcull, hipStream_t st);
 
 struct QueryParams {
     float eq;       // EQUALITY_THRESHOLD

@@ -267,9 +267,10 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
 // Exact recomputation for small obstacle sets (links x obstacles <=
 // kPairsMax, fewer than 64 links): one wave per configuration evaluates all
 // of its (link, obstacle) pairs at once, lane t taking pairs t, t + 64, ...
-// of pair = link * M + obstacle, instead of one link per round.  Every lane
-// runs the same forward kinematics and lane k keeps skeleton point k; a pair
-// fetches its end points by lane shuffles.  The filtered pair distances go to
+// of pair = link * M + obstacle, instead of one link per round.  Lane m
+// computes joint m's sine and cosine, every lane runs the same forward
+// kinematics on them and lane k keeps skeleton point k; a pair fetches its
+// end points by lane shuffles.  The filtered pair distances go to
 // the wave's LDS row, lane i then walks link i's obstacles in order as the
 // reference's loop does (running std::min from MAX_DISTANCE, stop at the
 // first value <= 0), and the links combine in order: the first link with a
@@ -286,7 +287,7 @@ __host__ __device__ inline size_t pairs_row_off(const EnvDev& env) {
 }
 
 inline bool refine_pairs_fit(const RobotDev& rb, const EnvDev& env) {
-    return rb.num_links < 64 && (int64_t)rb.num_links * env.num_obj <= kPairsMax;
+    return rb.num_links < 64 && rb.num_dofs <= 64 && (int64_t)rb.num_links * env.num_obj <= kPairsMax;
 }
 
 template <bool OBB>
@@ -312,11 +313,20 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine_pairs(Rob
         if (live) {
             c = idx ? (int64_t)idx[k] : k;
             const float* qc = q + c * rb.num_dofs;
+            // lane m computes the sine and cosine of joint m (the same
+            // function of the same value as a serial stream); the FK stream
+            // then takes them by shuffles
+            double my_s = 0.0, my_c = 1.0;
+            if (t < rb.num_dofs) sincos((double)qc[t], &my_s, &my_c);
+            const auto sc = [&](int m, double* st, double* ct) {
+                *st = __shfl(my_s, m, 64);
+                *ct = __shfl(my_c, m, 64);
+            };
             FKStream fk;
             fk_init(fk);
             Vec3<float> mine{0.0f, 0.0f, 0.0f};
             for (int pt = 0; pt <= L; ++pt) {
-                const Vec3<float> P = skel_point<float>(fk, rb, qc, pt);
+                const Vec3<float> P = skel_point_sc<float>(fk, rb, pt, sc);
                 if (t == pt) mine = P;
             }
             for (int base = 0; base < npairs; base += 64) {   // uniform rounds: the shuffles see every lane
@@ -589,6 +599,30 @@ __global__ void k_env_motion(MotionDev m, int64_t tick, float tick_seconds, floa
 //     and along y below
 //   light directions (:330-376): mirror reflection off the ground, the
 //     workspace sphere and the keep-out zones around the robot base
+// Group boxes of the broad-phase walk after the obstacles moved: thread g
+// takes the union of the cull boxes of entries g * kGroupSize .. of order
+// (the rule of group_bounds on the host).
+__global__ void k_group_bounds(const float* __restrict__ cull, const int32_t* __restrict__ order, int M,
+                               float* __restrict__ gcull) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= num_groups(M)) return;
+    float lo[3] = {HUGE_VALF, HUGE_VALF, HUGE_VALF}, hi[3] = {-HUGE_VALF, -HUGE_VALF, -HUGE_VALF};
+    bool finite = true;
+    const int m1 = min(M, (g + 1) * kGroupSize);
+    for (int m = g * kGroupSize; m < m1; ++m) {
+        const float* b = cull + 6 * order[m];
+        for (int k = 0; k < 3; ++k) {
+            finite = finite && isfinite(b[k]) && isfinite(b[3 + k]);
+            lo[k] = b[k] < lo[k] ? b[k] : lo[k];
+            hi[k] = hi[k] < b[3 + k] ? b[3 + k] : hi[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        gcull[6 * g + k] = finite ? lo[k] : -HUGE_VALF;
+        gcull[6 * g + 3 + k] = finite ? hi[k] : HUGE_VALF;
+    }
+}
+
 __global__ void k_env_step(EnvStepDev m, float dt, int32_t steps, float* aabb, float* obb, float* cull) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m.num_obj) return;
@@ -820,6 +854,13 @@ hipError_t launch_enclosing_radii(const RobotDev& rb, const float* q, int64_t n,
     while (B > 32 && per * B > 64 * 1024) B /= 2;
     const size_t sm = per * B;
     hipLaunchKernelGGL(k_radii, grid_for(n, B), dim3(B), sm, st, rb, q, n, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_bounds(const float* cull, const int32_t* order, int M, float* gcull, hipStream_t st) {
+    const int ng = num_groups(M);
+    if (ng <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_group_bounds, dim3((ng + 63) / 64), dim3(64), 0, st, cull, order, M, gcull);
     return hipGetLastError();
 }
 

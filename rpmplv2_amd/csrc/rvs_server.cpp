@@ -127,10 +127,13 @@ struct Req {
 struct Motion {
     MotionDev dev{};
     std::vector<void*> allocs;
+    const int32_t* gorder = nullptr;   // the environment's broad-phase walk order
+    bool use_obb = false;
     struct Slot {
         float* aabb = nullptr;
         float* obb = nullptr;
         float* cull = nullptr;   // broad-phase bounds of oriented boxes
+        float* gcull = nullptr;  // group boxes of the broad-phase walk
         int64_t tick = -1;
         int refs = 0;
         hipEvent_t ready = nullptr;
@@ -244,6 +247,7 @@ hipError_t issue(rvs_server* s, Batch& b, const Layout& lay) {
         env.aabb = sl.aabb;
         env.obb = sl.obb;
         env.cull = env.use_obb ? sl.cull : sl.aabb;
+        env.gcull = sl.gcull;
         e = hipStreamWaitEvent(st, sl.ready, 0);
     }
     Carve cv(s->pool.base() + b.off);
@@ -372,6 +376,8 @@ void tick_motion(rvs_server* s) {
         if (k < 0) continue;   // every other slot still read by running launches: retry next loop
         Motion::Slot& sl = m.slot[k];
         hipError_t e = launch_env_motion(m.dev, tick, s->opt.tick_seconds, sl.aabb, sl.obb, sl.cull, s->upd);
+        if (e == hipSuccess && m.dev.num_obj > 0)
+            e = launch_group_bounds(m.use_obb ? sl.cull : sl.aabb, m.gorder, m.dev.num_obj, sl.gcull, s->upd);
         if (e == hipSuccess) e = hipEventRecord(sl.ready, s->upd);
         if (e != hipSuccess) {
             s->err = std::string("obstacle update: ") + hipGetErrorString(e);   // caller holds s->mu
@@ -642,10 +648,14 @@ rvs_status rvs_server_set_motion(rvs_server* s, int32_t env, const float* vel) {
     if (!s || !vel) return RVS_EINVAL;
     rvs_context* c = s->ctx;
     std::vector<rvs_box> boxes;
+    const int32_t* gorder = nullptr;
+    bool use_obb = false;
     {
         std::lock_guard<std::mutex> lk(c->mu);
         if (env < 0 || env >= (int32_t)c->envs.size()) return sfail(s, RVS_EINVAL, "rvs_server_set_motion: bad env id");
         boxes = c->envs[env].boxes;
+        gorder = c->envs[env].dev.gorder;
+        use_obb = c->envs[env].dev.use_obb != 0;
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -666,6 +676,8 @@ rvs_status rvs_server_set_motion(rvs_server* s, int32_t env, const float* vel) {
         rad[j] = std::sqrt(m0 * m0 + (m1 * m1 + m2 * m2));
     }
     auto m = std::make_unique<Motion>();
+    m->gorder = gorder;
+    m->use_obb = use_obb;
     (void)hipSetDevice(c->opt.device);
     hipError_t e = hipSuccess;
     auto put = [&](const void* src, size_t bytes, const void** dst) {
@@ -696,6 +708,10 @@ rvs_status rvs_server_set_motion(rvs_server* s, int32_t env, const float* vel) {
         if (e == hipSuccess) e = hipMalloc(&cb, std::max<size_t>((size_t)M * 24, 4));
         if (e == hipSuccess) m->allocs.push_back(cb);
         sl.cull = static_cast<float*>(cb);
+        void* gb = nullptr;
+        if (e == hipSuccess) e = hipMalloc(&gb, std::max<size_t>((size_t)num_groups(M) * 24, 4));
+        if (e == hipSuccess) m->allocs.push_back(gb);
+        sl.gcull = static_cast<float*>(gb);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
         sl.aabb = static_cast<float*>(a);
         sl.obb = static_cast<float*>(o);
