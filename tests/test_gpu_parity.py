@@ -467,6 +467,31 @@ def test_env_motion_laws(oracle, engine, law):
     d1, p1, v1, _ = engine.distance(rid, eid, q)
     d2, p2, v2, _ = engine.distance(rid, engine.create_env(moved), q)
     assert (d1 == d2).all() and (p1 == p2).all() and (v1 == v2).all()
+    # the FP32 path's broad phase reads group boxes that the step recomputed
+    d3, _, v3, _ = engine.distance(rid, eid, q, precision=rvs.PREC_FP32_REFINE)
+    od, _, ov, _ = oracle.distance(rb, oracle.scene_of(moved), q, 32)
+    assert (v3 == ov).all() and np.abs(d3 - od).max() <= 1e-5
+
+
+def test_fp32_refine_after_env_update(oracle, engine):
+    """Boxes moved in place (rvs_env_update) far across the scene: the FP32
+    path's grouped broad phase keeps its walk order but rebuilds its group
+    boxes, so verdicts stay identical to the oracle's on the moved scene and
+    distances within 1e-5 m."""
+    w = W.workload2(1 << 16)
+    rid = engine.load_robot(w.robot)
+    eid = engine.create_env(w.scene)
+    engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
+    rng = np.random.default_rng(41)
+    idx = np.arange(1, len(w.scene.dim), dtype=np.int32)          # every box but the ground
+    pos = w.scene.pos.copy()
+    pos[idx] = pos[idx][rng.permutation(len(idx))] + rng.uniform(-0.2, 0.2, (len(idx), 3))
+    engine.update_env(eid, idx, pos[idx])
+    moved = W.Scene(w.scene.dim, pos, w.scene.quat, w.scene.is_ground, w.scene.min_dist_tol)
+    d, _, v, _ = engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
+    od, _, ov, _ = oracle.distance(w.robot, oracle.scene_of(moved), w.q, 32)
+    print(f"after update: collision-free {v.mean():.3f}, verdict mismatches {(v != ov).sum()}")
+    assert (v == ov).all() and np.abs(d - od).max() <= 1e-5
 
 
 @pytest.mark.parametrize("n", [1, 7, 300, 5000])
