@@ -78,3 +78,16 @@ def test_async_rejects_host_arrays_and_timing(engine):
     st = rvs.lib().rvs_distance_async(engine._ctx, rid, eid, C.c_void_p(qd.data_ptr()), len(q),
                                       C.c_void_p(dd.data_ptr()), None, None, rvs.F_DEVICE | rvs.F_TIMING, None, None)
     assert st == rvs.RVS_EINVAL
+
+
+def test_async_empty_batch(engine):
+    """An empty batch enqueues nothing but still records the caller's event."""
+    dev = torch.device("cuda:0")
+    w = W.workload2(8)
+    rid, eid = engine.load_robot(w.robot), engine.create_env(w.scene)
+    q = torch.empty((0, 6), device=dev)
+    d = torch.empty(0, device=dev)
+    done = torch.cuda.Event()
+    engine.distance_async(rid, eid, q, d, precision=rvs.PREC_FP32_REFINE, stream=torch.cuda.Stream(), done=done)
+    done.synchronize()
+    assert engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)[2].shape == (8,)

@@ -147,6 +147,30 @@ def test_empty_and_single(oracle, engine):
     assert ver[0] == over[0] and ulps(d, od).max() <= 2
 
 
+@pytest.mark.parametrize("boxes", [0, 1, 5])
+def test_fp32_refine_small_scenes_and_nan(oracle, engine, boxes):
+    """The FP32 path (walk order, grouped broad phase) on scenes of 0, 1 and
+    5 boxes (no group, one partial group, a full and a partial group), with
+    non-finite joint values among the configurations: verdicts equal the
+    oracle's, NaN distances where the oracle has them, the others within
+    1e-5 m."""
+    rng = np.random.default_rng(60 + boxes)
+    rb = R.xarm6()
+    sc = W._boxes(rng, boxes, 0.1, 0.4, 0.8, (0, 0, 0.4)) if boxes else W.Scene(
+        np.zeros((0, 3)), np.zeros((0, 3)), np.zeros((0, 4)), np.zeros(0, np.uint8), np.zeros(0))
+    q = W.workload2(50000).q.copy()
+    q[::97, 1] = np.nan
+    q[::89, 0] = np.inf
+    rid = engine.load_robot(rb)
+    eid = engine.create_env(sc)
+    d, prof, ver, _ = engine.distance(rid, eid, q, precision=rvs.PREC_FP32_REFINE)
+    od, oprof, over, _ = oracle.distance(rb, oracle.scene_of(sc), q, 32)
+    assert (ver == over).all()
+    assert np.array_equal(np.isnan(d), np.isnan(od))
+    m = ~np.isnan(d)
+    assert np.abs(d[m] - od[m]).max() <= 1e-5
+
+
 def test_nan_configurations(oracle, engine):
     w = W.workload2(256)
     q = w.q.copy()
