@@ -188,7 +188,8 @@ rvs_status rvs_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const floa
                         uint32_t flags);
 
 /* Stream-ordered forms of rvs_distance (no nearest points) and rvs_is_valid
- * for device arrays (flags must hold RVS_F_DEVICE; no RVS_F_TIMING): the
+ * (and, below, of the underestimation and self-collision queries) for device
+ * arrays (flags must hold RVS_F_DEVICE; no RVS_F_TIMING): the
  * work is enqueued on 'stream' (a hipStream_t; NULL = the context's stream)
  * and the call returns at once.  'done' (a hipEvent_t, may be NULL) is
  * recorded on the stream after the work.  The arrays must stay valid until
@@ -202,6 +203,14 @@ rvs_status rvs_distance_async(rvs_context* ctx, int32_t robot, int32_t env, cons
                               float* profile, uint8_t* verdict, uint32_t flags, void* stream, void* done);
 rvs_status rvs_is_valid_async(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n,
                               uint8_t* valid, uint32_t flags, void* stream, void* done);
+/* The same stream-ordered form of rvs_distance_underestimation and of
+ * rvs_self_collision (device arrays). */
+rvs_status rvs_distance_underestimation_async(rvs_context* ctx, int32_t robot, int32_t env, const float* q,
+                                              int64_t n, const float* nearest_ref, int64_t ref_stride, float* d,
+                                              float* profile, uint8_t* stored, uint32_t flags, void* stream,
+                                              void* done);
+rvs_status rvs_self_collision_async(rvs_context* ctx, int32_t robot, const float* q, int64_t n, float* min_dist,
+                                    uint8_t* free_flag, uint32_t flags, void* stream, void* done);
 
 /* RealVectorSpace::computeDistanceUnderestimation(q, nearest_points)
  * (RealVectorSpace.cpp:402-446) on fresh states.  nearest_ref is [M][L][6]

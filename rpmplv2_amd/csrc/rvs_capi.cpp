@@ -847,6 +847,48 @@ rvs_status rvs_is_valid_async(rvs_context* c, int32_t robot, int32_t env, const 
     return RVS_OK;
 }
 
+rvs_status rvs_distance_underestimation_async(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n,
+                                              const float* nearest_ref, int64_t ref_stride, float* d, float* profile,
+                                              uint8_t* stored, uint32_t flags, void* stream, void* done) {
+    rvs_status s = check_query(c, robot, env, q, n, true);
+    if (s != RVS_OK) return s;
+    if (n > 0 && (!nearest_ref || !d)) return fail(c, RVS_EINVAL, "rvs_distance_underestimation_async: null argument");
+    if (!(flags & RVS_F_DEVICE))
+        return fail(c, RVS_EINVAL, "rvs_distance_underestimation_async: device arrays only (RVS_F_DEVICE)");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    const int64_t set = (int64_t)ev.dev.num_obj * rb.num_links * 6;
+    if (ref_stride != 0 && ref_stride != set)
+        return fail(c, RVS_EINVAL, "rvs_distance_underestimation_async: ref_stride");
+    (void)hipSetDevice(c->opt.device);
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    hipError_t e = async_begin(c, st);
+    if (e == hipSuccess && n > 0)
+        e = launch_underestimation(rb.dev, ev.dev, params_of(c), q, n, nearest_ref, ref_stride, d, profile, stored, st);
+    if (e == hipSuccess) e = async_end(c, st, done);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_underestimation_async");
+    return RVS_OK;
+}
+
+rvs_status rvs_self_collision_async(rvs_context* c, int32_t robot, const float* q, int64_t n, float* min_dist,
+                                    uint8_t* free_flag, uint32_t flags, void* stream, void* done) {
+    rvs_status s = check_query(c, robot, -1, q, n, false);
+    if (s != RVS_OK) return s;
+    if (n > 0 && !min_dist && !free_flag) return fail(c, RVS_EINVAL, "rvs_self_collision_async: no output");
+    if (!(flags & RVS_F_DEVICE)) return fail(c, RVS_EINVAL, "rvs_self_collision_async: device arrays only (RVS_F_DEVICE)");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    (void)hipSetDevice(c->opt.device);
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    hipError_t e = async_begin(c, st);
+    if (e == hipSuccess && n > 0)
+        e = launch_self_distance(rb.dev, params_of(c), q, n, nullptr, nullptr, min_dist, free_flag, st);
+    if (e == hipSuccess) e = async_end(c, st, done);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_self_collision_async");
+    return RVS_OK;
+}
+
 rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n, uint32_t flags,
                         rvs_tuning* best) {
     rvs_status s = check_query(c, robot, env, q, n, true);

@@ -107,6 +107,7 @@ EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds",
            "rvs_env_set_motion_law", "rvs_env_step", "rvs_distance",
            "rvs_distance_f64", "rvs_is_valid", "rvs_distance_async", "rvs_is_valid_async",
+           "rvs_distance_underestimation_async", "rvs_self_collision_async",
            "rvs_distance_underestimation", "rvs_distance_underestimation_moving",
            "rvs_trajectory_is_safe", "rvs_self_distance",
            "rvs_self_collision", "rvs_enclosing_radii", "rvs_extend_spines", "rvs_extend_gen_spines",
@@ -174,6 +175,8 @@ def lib():
         L.rvs_is_valid.argtypes = [ctx, i32, i32, vp, i64, vp, u32]
         L.rvs_distance_async.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, u32, vp, vp]
         L.rvs_is_valid_async.argtypes = [ctx, i32, i32, vp, i64, vp, u32, vp, vp]
+        L.rvs_distance_underestimation_async.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, vp, vp, u32, vp, vp]
+        L.rvs_self_collision_async.argtypes = [ctx, i32, vp, i64, vp, vp, u32, vp, vp]
         L.rvs_distance_underestimation.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, vp, vp, u32]
         L.rvs_distance_underestimation_moving.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, C.c_float, vp, vp, vp,
                                                           u32]
@@ -540,6 +543,22 @@ class Engine:
         fl = F_DEVICE | (F_PREC(precision) if precision is not None else 0)
         self._check(lib().rvs_is_valid_async(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(valid), fl,
                                              _stream_handle(stream), _event_handle(done)), "rvs_is_valid_async")
+
+    def underestimation_async(self, robot: int, env: int, q, nearest_ref, d, profile=None, stored=None,
+                              per_config: bool = False, stream=None, done=None):
+        """rvs_distance_underestimation_async on device tensors (see
+        distance_async); per_config: one nearest-point set per configuration."""
+        rb, M = self.robots[robot], self.envs[env].num_obj
+        stride = M * rb.num_links * 6 if per_config else 0
+        self._check(lib().rvs_distance_underestimation_async(
+            self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(nearest_ref), stride, _ptr(d), _ptr(profile),
+            _ptr(stored), F_DEVICE, _stream_handle(stream), _event_handle(done)), "rvs_distance_underestimation_async")
+
+    def self_collision_async(self, robot: int, q, min_dist=None, free=None, stream=None, done=None):
+        """rvs_self_collision_async on device tensors (see distance_async)."""
+        self._check(lib().rvs_self_collision_async(self._ctx, robot, _ptr(q), q.shape[0], _ptr(min_dist), _ptr(free),
+                                                   F_DEVICE, _stream_handle(stream), _event_handle(done)),
+                    "rvs_self_collision_async")
 
     def distance_f64_device(self, robot: int, env: int, q, d, profile=None, verdict=None, timing: bool = False):
         fl = F_DEVICE | (F_TIMING if timing else 0)

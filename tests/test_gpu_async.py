@@ -91,3 +91,32 @@ def test_async_empty_batch(engine):
     engine.distance_async(rid, eid, q, d, precision=rvs.PREC_FP32_REFINE, stream=torch.cuda.Stream(), done=done)
     done.synchronize()
     assert engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)[2].shape == (8,)
+
+
+def test_underestimation_and_self_collision_async(engine):
+    """The stream-ordered underestimation and self-collision queries equal
+    the blocking calls bit for bit."""
+    dev = torch.device("cuda:0")
+    w = W.workload2(1 << 14)
+    rid, eid = engine.load_robot(w.robot), engine.create_env(w.scene)
+    q = w.q
+    _, _, _, npo = engine.distance(rid, eid, q[:1], precision=rvs.PREC_REF32, want_nearest=True)
+    ref = npo[0]
+    d0, p0, s0 = engine.underestimation(rid, eid, q, ref)
+    m0, f0 = engine.self_collision(rid, q)
+    qd = torch.from_numpy(q).to(dev)
+    refd = torch.from_numpy(np.ascontiguousarray(ref)).to(dev)
+    n, L = len(q), w.robot.num_links
+    d = torch.full((n,), float("nan"), device=dev)
+    prof = torch.full((n, L), float("nan"), device=dev)
+    st = torch.full((n,), 255, dtype=torch.uint8, device=dev)
+    m = torch.full((n,), float("nan"), device=dev)
+    f = torch.full((n,), 255, dtype=torch.uint8, device=dev)
+    s = torch.cuda.Stream()
+    done = torch.cuda.Event()
+    engine.underestimation_async(rid, eid, qd, refd, d, prof, st, stream=s)
+    engine.self_collision_async(rid, qd, m, f, stream=s, done=done)
+    done.synchronize()
+    assert np.array_equal(d.cpu().numpy(), d0) and np.array_equal(st.cpu().numpy(), s0)
+    assert np.array_equal(prof.cpu().numpy(), p0)
+    assert np.array_equal(m.cpu().numpy(), m0) and np.array_equal(f.cpu().numpy(), f0)
