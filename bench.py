@@ -219,6 +219,49 @@ def workload5_section(eng, cores: int, n: int) -> dict:
     }
 
 
+def workload1_section(eng, cores: int) -> dict:
+    """Planar 2-DOF arm, 4 boxes, 65,536 configurations: the FP64 drop-in
+    (rvs_distance_f64 on device arrays), the reference arithmetic and the
+    FP32 path, with the oracle's FP64 loop on the host cores beside them."""
+    import numpy as np
+    import torch
+    from rpmplv2_amd import rvs, workloads as W
+    O = oracle_module()
+    w = W.workload1(65536)
+    rid, eid = eng.load_robot(w.robot), eng.create_env(w.scene)
+    n, L = len(w.q), w.robot.num_links
+    dev = torch.device("cuda:0")
+    q = torch.from_numpy(w.q).to(dev)
+    d64 = torch.empty(n, dtype=torch.float64, device=dev)
+    p64 = torch.empty((n, L), dtype=torch.float64, device=dev)
+    v = torch.empty(n, dtype=torch.uint8, device=dev)
+    for _ in range(3):
+        eng.distance_f64_device(rid, eid, q, d64, p64, v)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        eng.distance_f64_device(rid, eid, q, d64, p64, v)
+    torch.cuda.synchronize()
+    ms64 = (time.perf_counter() - t0) * 1e3 / 20
+    got_d, got_v = d64.cpu().numpy(), v.cpu().numpy()
+    d = torch.empty(n, device=dev)
+    prof = torch.empty((n, L), device=dev)
+    ms32, _, _, _ = timed_distance(eng, rid, eid, q, d, prof, v, rvs.PREC_REF32, 20, 3)
+    msr, _, _, _ = timed_distance(eng, rid, eid, q, d, prof, v, rvs.PREC_FP32_REFINE, 20, 3)
+    sc = O.scene_of(w.scene)
+    O.distance(w.robot, sc, w.q[:4096], 64, nthreads=cores)
+    t = time.perf_counter()
+    od, _, ov, _ = O.distance(w.robot, sc, w.q, 64, nthreads=cores)
+    cpu_s = time.perf_counter() - t
+    return {"workload": "planar2dof_4boxes", "configs": n,
+            "fp64": {"configs_per_s": n / (ms64 * 1e-3), "ms": ms64,
+                     "verdict_mismatch_vs_oracle_fp64": int((got_v != ov).sum()),
+                     "max_abs_dist_err_vs_oracle_fp64": float(np.abs(got_d - od).max())},
+            "ref32": {"configs_per_s": n / (ms32 * 1e-3), "ms": ms32},
+            "fp32_refine": {"configs_per_s": n / (msr * 1e-3), "ms": msr},
+            "cpu_oracle_fp64": {"configs_per_s": n / cpu_s, "cores": cores}}
+
+
 def edges_section(eng, cores: int) -> dict:
     import numpy as np
     import torch
@@ -469,6 +512,8 @@ def main():
             "collision_free_fraction": float(ver_gpu.mean()),
         }
     if not args.quick:
+        print("bench: workload 1", file=sys.stderr, flush=True)
+        out["workload1"] = workload1_section(eng, cores)
         print("bench: workload 5", file=sys.stderr, flush=True)
         out["workload5"] = workload5_section(eng, cores, args.w5_n)
         print("bench: edges", file=sys.stderr, flush=True)
