@@ -147,6 +147,28 @@ def test_empty_and_single(oracle, engine):
     assert ver[0] == over[0] and ulps(d, od).max() <= 2
 
 
+@pytest.mark.parametrize("links,boxes", [(16, 16), (16, 17), (63, 4)])
+def test_refinement_forms_at_their_limits(oracle, engine, links, boxes):
+    """The exact refinement evaluates all pairs of a configuration at once up
+    to 256 (link, obstacle) pairs and fewer than 64 links (16 x 16, 63 x 4),
+    and link by link beyond (16 x 17): both give the oracle's reference
+    arithmetic, on the FP32 path's flagged configurations and as the
+    small-batch form of reference-arithmetic calls."""
+    rng = np.random.default_rng(70 + links + boxes)
+    rb = R.chain(rng.uniform(0.03, 0.06, links), radius=0.02)
+    sc = W._boxes(rng, boxes, 0.05, 0.2, 0.9, (0, 0, 0.3))
+    q = rng.uniform(-1.0, 1.0, (20000, links)).astype(np.float32)
+    rid = engine.load_robot(rb)
+    eid = engine.create_env(sc)
+    osc = oracle.scene_of(sc)
+    od, oprof, over, _ = oracle.distance(rb, osc, q, 32)
+    d, prof, ver, _ = engine.distance(rid, eid, q, precision=rvs.PREC_FP32_REFINE)
+    print(f"{links} links x {boxes} boxes: collision-free {ver.mean():.3f}, refined {engine.last_refined()}")
+    assert (ver == over).all() and np.abs(d - od).max() <= 1e-5
+    d2, prof2, ver2, _ = engine.distance(rid, eid, q[:3000], precision=rvs.PREC_REF32)   # one wave per configuration
+    assert (ver2 == over[:3000]).all() and ulps(d2, od[:3000]).max() <= 2 and ulps(prof2, oprof[:3000]).max() <= 2
+
+
 @pytest.mark.parametrize("boxes", [0, 1, 5])
 def test_fp32_refine_small_scenes_and_nan(oracle, engine, boxes):
     """The FP32 path (walk order, grouped broad phase) on scenes of 0, 1 and
