@@ -191,7 +191,9 @@ def workload5_section(eng, cores: int, n: int) -> dict:
                 "refined": nref_o, "collision_free_fraction": free_o, "tuning": tune,
                 "verdict_mismatch_vs_ref32_subsample": int((ver_o != ov32).sum()),
                 "max_abs_dist_err_vs_ref64_subsample": float(np.abs(d_o - od64).max()),
-                "roofline": {"bound": "valu", "kernel": "k_distance_fast_pool<OBB> / k_distance_fast<OBB>",
+                "roofline": {"bound": "valu",
+                             "kernel": {0: "k_distance_fast<OBB>", 1: "k_distance_fast_pool<OBB>",
+                                        2: "k_distance_fast_chunks<OBB>"}[tune["variant"]],
                              "dense_flop_per_config": flop_obb_dense,
                              "dense_equivalent_tflops": flop_obb_dense * n / (bulk_o * 1e-3) / 1e12,
                              "dense_note": "dense nominal work (100 flop per FK joint + 196 per capsule-OBB pair) "
@@ -459,7 +461,7 @@ def main():
                         + ev_pairs * FLOP_CAPSULE_AABB)
     achieved = dense_flop_cfg * n / (bulk_ms * 1e-3) / 1e12
     hbm_bytes = n * (w.robot.num_dofs * 4 + 4 + L * 4 + 1)
-    kernel = "k_distance_fast_pool" if tune["variant"] == 1 else "k_distance_fast"
+    kernel = {0: "k_distance_fast", 1: "k_distance_fast_pool", 2: "k_distance_fast_chunks"}[tune["variant"]]
     out = {
         "metric": "distance queries/sec (xArm6, 16 boxes) at verdict-exact",
         "value": value,

@@ -352,9 +352,10 @@ rvs_status rvs_edge_first_collision(rvs_context* ctx, int32_t env, const int32_t
  * whole in LDS (every lane walks every obstacle of its current link, so a
  * smaller obstacle tile would only add reloads); it must fit 64 KB. */
 typedef struct rvs_tuning {
-    int32_t variant;        /* 0: one lane per configuration; 1: persistent lane refill */
+    int32_t variant;        /* 0: one lane per configuration; 1: persistent lane refill; 2: one lane per
+                               configuration in persistent waves over 64-configuration chunks */
     int32_t block;          /* threads per workgroup: 64, 128 or 256 */
-    int32_t blocks_per_cu;  /* lane refill: resident workgroups per CU (0 = as many as fit) */
+    int32_t blocks_per_cu;  /* variants 1 and 2: resident workgroups per CU (0 = as many as fit) */
     int32_t coherence_bits; /* walk order (0..18): configurations sorted by their first
                                min(dofs, 18 / bits) joint angles, 2^bits bins each, so that a wave's
                                lanes hold similar poses; 0 = the order given.  Results do not depend

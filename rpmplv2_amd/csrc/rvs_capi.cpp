@@ -920,11 +920,13 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
     e = hipMemcpyAsync(qs, q, (size_t)m * D * 4, (flags & RVS_F_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                        c->stream);
     const QueryParams qp = params_of(c);
-    // candidates: one lane per configuration at three workgroup sizes; lane
+    // candidates: one lane per configuration at three workgroup sizes, as one
+    // workgroup per 64..256 configurations and in persistent waves; lane
     // refill at three workgroup sizes, with the occupancy limit and half of it
     std::vector<FastTuning> cand;
     for (int cb : {0, 5}) {   // walk order: as given, or sorted by the first joint angles
         for (int b : {64, 128, 256}) cand.push_back(FastTuning{0, b, 0, cb});
+        for (int b : {64, 128, 256}) cand.push_back(FastTuning{2, b, 0, cb});
         for (int b : {64, 128, 256}) {
             const int occ = std::max(1, fast_pool_blocks_per_cu(ev.dev, b));
             cand.push_back(FastTuning{1, b, occ, cb});
@@ -973,10 +975,10 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
 
 rvs_status rvs_set_tuning(rvs_context* c, int32_t robot, int32_t env, const rvs_tuning* t) {
     if (!c || !t) return fail(c, RVS_EINVAL, "rvs_set_tuning: null argument");
-    if ((t->variant != 0 && t->variant != 1) || (t->block != 64 && t->block != 128 && t->block != 256) ||
+    if (t->variant < 0 || t->variant > 2 || (t->block != 64 && t->block != 128 && t->block != 256) ||
         t->blocks_per_cu < 0 || t->coherence_bits < 0 || t->coherence_bits > kCohKeyBits)
         return fail(c, RVS_EINVAL,
-                    "rvs_set_tuning: variant 0/1, block 64/128/256, blocks_per_cu >= 0, coherence_bits 0..18");
+                    "rvs_set_tuning: variant 0/1/2, block 64/128/256, blocks_per_cu >= 0, coherence_bits 0..18");
     std::lock_guard<std::mutex> lk(c->tuning_mu);
     c->tuning[{robot, env}] = FastTuning{t->variant, t->block, t->blocks_per_cu, t->coherence_bits};
     return RVS_OK;

@@ -265,34 +265,28 @@ __device__ __attribute__((always_inline)) inline void append_flagged(bool flag, 
     }
 }
 
-// With ROWS (a walk order is given) each lane copies its configuration's
-// joint values into its LDS slice in one burst before the walk and collects
-// its profile row there, writing it out in one burst at the end: the rows of
-// a wave's lanes are scattered in memory, and reading or writing them a
-// joint or a link at a time, far apart in time, costs a cache line each.
-// Held to 96 VGPRs (5 waves per SIMD): the walk-order form needs no scratch
-// at that budget and ran 8% faster than at 4 waves (6 waves spill and were
-// slower).
+// One configuration of the bulk pass for this lane (configuration c; active
+// false: the lane has none and only takes part in the wave's collective
+// steps).  With ROWS (a walk order is given) the lane copies the
+// configuration's joint values into its LDS slice in one burst before the
+// walk and collects its profile row there, writing it out in one burst at the
+// end: the rows of a wave's lanes are scattered in memory, and reading or
+// writing them a joint or a link at a time, far apart in time, costs a cache
+// line each.
 template <bool OBB, bool ROWS, bool GROUPED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_distance_fast(
-    RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
-    uint32_t* count, const int32_t* __restrict__ perm, size_t rows_off) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<float> s = stage_env<float, OBB>(env, smem);
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    // with a walk order, lane g takes configuration perm[g]
-    const int64_t c = perm ? (int64_t)perm[g] : g;
+__device__ __attribute__((always_inline)) inline void fast_one(const Staged<float>& s, const RobotDev& rb,
+                                                               const EnvDev& env, const QueryParams& qp,
+                                                               const float* __restrict__ q, int64_t c, bool active,
+                                                               const DistOut& out, int32_t* idx, uint32_t* count,
+                                                               float* qrow, float* prow, int B) {
     const int M = env.num_obj;
     const int L = rb.num_links;
     const int D = rb.num_dofs;
-    const int B = blockDim.x, t = threadIdx.x;
-    float* qrow = reinterpret_cast<float*>(smem + rows_off) + t;   // [D] at stride B
-    float* prow = qrow + (size_t)D * B;                             // [L] at stride B
-    const float* qc = q + c * D;
+    const float* qc = q + (active ? c : 0) * D;
     int qs = 1;
     if (ROWS) {
-        for (int k = 0; k < D; ++k) qrow[k * B] = qc[k];
+        if (active)
+            for (int k = 0; k < D; ++k) qrow[k * B] = qc[k];
         qc = qrow;
         qs = B;
     }
@@ -312,25 +306,75 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         const Vec3<float> P1 = skelf_point(fk, rb, qc, i + 1, qs);
         if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
         const float r = rb.radius[i];
-        const float prof = link_min<OBB, GROUPED>(s, env, M, i < rb.ground_included, P0, P1, r, p, band, true, warm,
+        const float prof = link_min<OBB, GROUPED>(s, env, M, i < rb.ground_included, P0, P1, r, p, band, active, warm,
                                                   hit, uncertain);
         if (ROWS) prow[i * B] = prof;
-        else if (out.profile) out.profile[c * L + i] = prof;
+        else if (active && out.profile) out.profile[c * L + i] = prof;
         if (hit) break;
         d = smin(d, prof);
         P0 = P1;
     }
-    if (out.profile) {
-        if (ROWS) {
-            for (int k = 0; k < L; ++k) out.profile[c * L + k] = k <= i ? prow[k * B] : 0.0f;
-        } else {
-            for (int k = i + 1; k < L; ++k) out.profile[c * L + k] = 0.0f;
+    if (active) {
+        if (out.profile) {
+            if (ROWS) {
+                for (int k = 0; k < L; ++k) out.profile[c * L + k] = k <= i ? prow[k * B] : 0.0f;
+            } else {
+                for (int k = i + 1; k < L; ++k) out.profile[c * L + k] = 0.0f;
+            }
         }
+        if (hit) d = 0.0f;
+        if (out.d) out.d[c] = d;
+        if (out.verdict) out.verdict[c] = hit ? 0 : 1;
     }
-    if (hit) d = 0.0f;
-    if (out.d) out.d[c] = d;
-    if (out.verdict) out.verdict[c] = hit ? 0 : 1;
-    append_flagged(uncertain, c, idx, count);
+    append_flagged(active && uncertain, c, idx, count);
+}
+
+// One lane per configuration: lane g takes configuration perm[g] of the walk
+// order (g itself without one).  Held to 96 VGPRs (5 waves per SIMD): the
+// walk-order form needs no scratch at that budget and ran 8% faster than at
+// 4 waves (6 waves spill and were slower).
+template <bool OBB, bool ROWS, bool GROUPED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_distance_fast(
+    RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
+    uint32_t* count, const int32_t* __restrict__ perm, size_t rows_off) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<float> s = stage_env<float, OBB>(env, smem);
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int64_t c = perm ? (int64_t)perm[g] : g;
+    const int B = blockDim.x;
+    float* qrow = reinterpret_cast<float*>(smem + rows_off) + threadIdx.x;   // [D] at stride B
+    float* prow = qrow + (size_t)rb.num_dofs * B;                          // [L] at stride B
+    fast_one<OBB, ROWS, GROUPED>(s, rb, env, qp, q, c, true, out, idx, count, qrow, prow, B);
+}
+
+// The same with persistent waves: each wave takes 64-configuration chunks of
+// the walk order from a global counter (next, expected zero) until none is
+// left, so a workgroup stages the obstacles once and the launch ends with
+// one chunk's imbalance instead of a partial last round of workgroups.
+// Every wave leaves once the counter has passed n.  Held to 128 VGPRs (4
+// waves per SIMD): at 96 the loop's extra state spills (76 bytes per lane).
+template <bool OBB, bool ROWS, bool GROUPED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_chunks(
+    RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
+    uint32_t* count, const int32_t* __restrict__ perm, size_t rows_off, uint32_t* next) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<float> s = stage_env<float, OBB>(env, smem);
+    const int B = blockDim.x;
+    const int lane = __lane_id();
+    float* qrow = reinterpret_cast<float*>(smem + rows_off) + threadIdx.x;   // [D] at stride B
+    float* prow = qrow + (size_t)rb.num_dofs * B;                          // [L] at stride B
+    for (;;) {
+        uint32_t chunk = 0;
+        if (lane == 0) chunk = atomicAdd(next, 1u);
+        chunk = __shfl(chunk, 0, 64);
+        const int64_t g0 = (int64_t)chunk * 64;
+        if (g0 >= n) break;   // uniform over the wave
+        const int64_t g = g0 + lane;
+        const bool active = g < n;
+        const int64_t c = active ? (perm ? (int64_t)perm[g] : g) : 0;
+        fast_one<OBB, ROWS, GROUPED>(s, rb, env, qp, q, c, active, out, idx, count, qrow, prow, B);
+    }
 }
 
 // The same pass with lane refill.  The classic kernel keeps a lane on one
@@ -671,11 +715,35 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
                            count + 32, perm);
         return hipGetLastError();
     }
-    const dim3 g((unsigned)((n + B - 1) / B)), b(B);
     // rows staged through LDS when a walk order scatters them and the slices fit
     const size_t rows_off = (sm + 15) & ~size_t(15);
     const size_t rows_sm = rows_off + (size_t)(rb.num_dofs + rb.num_links) * 4 * B;
-    if (perm && rows_sm <= 64 * 1024)
+    const bool rows = perm && rows_sm <= 64 * 1024;
+    if (t.variant == 2) {
+        // persistent waves over 64-configuration chunks; count[32] is the chunk counter
+        const size_t smc = rows ? rows_sm : sm;
+        int per_cu = t.blocks_per_cu;
+        if (per_cu <= 0) {
+            const hipError_t oe =
+                rows ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_distance_fast_chunks<OBB, true, GROUPED>,
+                                                                    B, smc)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_distance_fast_chunks<OBB, false, GROUPED>,
+                                                                    B, smc);
+            if (oe != hipSuccess) per_cu = 1;
+        }
+        per_cu = std::max(1, per_cu);
+        const int64_t blocks = std::min<int64_t>((int64_t)num_cus * per_cu, (n + B - 1) / B);
+        const dim3 g((unsigned)std::max<int64_t>(1, blocks)), b(B);
+        if (rows)
+            hipLaunchKernelGGL((k_distance_fast_chunks<OBB, true, GROUPED>), g, b, smc, st, rb, env, qp, q, n, out, idx,
+                               count, perm, rows_off, count + 32);
+        else
+            hipLaunchKernelGGL((k_distance_fast_chunks<OBB, false, GROUPED>), g, b, smc, st, rb, env, qp, q, n, out,
+                               idx, count, perm, rows_off, count + 32);
+        return hipGetLastError();
+    }
+    const dim3 g((unsigned)((n + B - 1) / B)), b(B);
+    if (rows)
         hipLaunchKernelGGL((k_distance_fast<OBB, true, GROUPED>), g, b, rows_sm, st, rb, env, qp, q, n, out, idx, count,
                            perm, rows_off);
     else

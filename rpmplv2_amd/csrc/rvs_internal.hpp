@@ -95,7 +95,8 @@ hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const Quer
 // pass expects to be zero).
 // Launch shape of the FP32 bulk pass (chosen by the autotuner).
 struct FastTuning {
-    int32_t variant;        // 0: one lane per configuration; 1: persistent lane refill
+    int32_t variant;        // 0: one lane per configuration; 1: persistent lane refill;
+                            // 2: one lane per configuration in persistent waves over 64-configuration chunks
     int32_t block;          // threads per workgroup: 64, 128 or 256
     int32_t blocks_per_cu;  // lane refill: resident workgroups per CU, 0 = the occupancy limit
     int32_t coh_bits;       // walk order: 2^bits bins per joint angle of the first 18 / bits joints, 0 = off
