@@ -53,10 +53,10 @@ FLOP_BROAD_PHASE = 15
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBPS = 8000.0
 # HBM bytes per launch of the FP32 bulk kernel on the headline workload from
-# two `rocprofv3 --pmc` runs: FETCH_SIZE 129,363.4 KiB x 2.0 + WRITE_SIZE
-# 123,926.9 KiB x 1.0, the factors being what the same counters read for a
+# two `rocprofv3 --pmc` runs: FETCH_SIZE 131,681.3 KiB x 2.0 + WRITE_SIZE
+# 123,943.0 KiB x 1.0, the factors being what the same counters read for a
 # 256 MiB copy (profiles/k_distance_fast_xarm6_1M.md).
-TRAFFIC_BYTES = (2.0 * 129363.4 + 123926.9) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
+TRAFFIC_BYTES = (2.0 * 131681.3 + 123943.0) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
 
 
 def pair_count(rb, scene) -> int:
