@@ -14,6 +14,7 @@
 //   face crossing, margin < band  the crossing test could differ
 //   |d - tol| < band              the min_dist_tol filter could differ
 //   sin^2 < 1e-6, raw < 0.01      a near-parallel segment solve close by
+//   zero-length segment and edge  the reference's 0/0 candidates
 //   non-finite skeleton           NaN / Inf configurations
 // (inside points and face crossings are screened by how far the comparison
 // that decided them was from flipping, since their results 0 and -r carry no
@@ -139,7 +140,8 @@ __device__ __attribute__((always_inline)) inline float fast_pair(const Staged<fl
     // (0 or -r) is exact unless the decision is marginal
     if (info.inside) uncertain |= info.depth < band;
     else if (info.hit) uncertain |= info.hit_margin < band;
-    else uncertain |= (info.raw < p.eq + band) || (fabsf(dt) < band) || (info.sin2 < 1e-6f && info.raw < 0.01f);
+    else uncertain |= (info.raw < p.eq + band) || (fabsf(dt) < band) || (info.sin2 < 1e-6f && info.raw < 0.01f) ||
+                      (info.sin2 < -0.5f);
     if (tol < p.maxd && fabsf(dt - tol) < band) uncertain = true;
     if (dt > tol) dt = p.maxd;
     return dt;

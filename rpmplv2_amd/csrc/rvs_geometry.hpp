@@ -16,7 +16,9 @@
 // compiled without FMA contraction gives results bit-identical to the
 // reference's float arithmetic (T = float) or the same algorithm in FP64
 // (T = double); FastMath (hardware reciprocal and square root, contraction
-// allowed by its translation unit) is the relaxed FP32 bulk pass.
+// allowed by its translation unit) is the relaxed FP32 bulk pass, which also
+// measures box edges with seg_edge, the segment-segment solve reduced for an
+// axis-aligned edge (same cases, different rounding).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -53,12 +55,14 @@ template <typename T> __host__ __device__ inline T sqnorm(const Vec3<T>& a) { re
 __host__ __device__ inline float vsqrt(float x) { return sqrtf(x); }
 __host__ __device__ inline double vsqrt(double x) { return sqrt(x); }
 struct ExactMath {
+    static constexpr bool kAxisEdges = false;   // box edges through the general seg_seg
     template <typename T> __host__ __device__ static T div(T a, T b) { return a / b; }
     template <typename T> __host__ __device__ static T root(T x) { return vsqrt(x); }
 };
 
 // Relaxed FP32: v_rcp_f32 / v_sqrt_f32 (about 1 ulp), device only.
 struct FastMath {
+    static constexpr bool kAxisEdges = true;    // box edges through seg_edge
     __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static float root(float x) { return __builtin_amdgcn_sqrtf(x); }
 };
@@ -169,6 +173,57 @@ __host__ __device__ inline T seg_seg_sin2(const Vec3<T>& A, const Vec3<T>& B, co
     const T uu = sqnorm(u), vv = sqnorm(v), uv = dot(u, v);
     const T den = uu * vv;
     return den > 0 ? MP::div(den - uv * uv, den) : T(1);
+}
+
+// The same distance of AB to an axis-aligned box edge, for the relaxed bulk
+// pass: the edge runs from corner C along axis ax for a length L (D = C + L e),
+// so every product with D - C is one component.  The interior solution and the
+// four clamped candidates are the reference's (parameters clamped to [0, 1]
+// instead of choosing the end point: the same point up to rounding); results
+// differ from seg_seg by rounding only.  sin2 receives the squared sine of the
+// angle between the segments (1 when either has zero length, as seg_seg_sin2).
+// A zero-length AB or edge alone gives the reference's value: its NaN
+// candidates are the projections of the point, never below the point's
+// distance to the other segment, which the remaining candidates hold.  When
+// both have zero length the reference returns MAX_DISTANCE and this routine
+// the point distance: sin2 is then -1, for the caller to send the pair to the
+// exact pass.
+template <class MP>
+__host__ __device__ inline float seg_edge(const Vec3<float>& A, const Vec3<float>& B, const Vec3<float>& C, int ax, float L,
+                                 const Params<float>& p, float& sin2) {
+    const Vec3<float> BA = B - A, AC = A - C;
+    const float ex = ax == 0 ? 1.0f : 0.0f, ey = ax == 1 ? 1.0f : 0.0f, ez = ax == 2 ? 1.0f : 0.0f;
+    const float ba = BA[ax], ac = AC[ax];
+    const float alpha1 = sqnorm(BA);
+    const float g = -dot(AC, BA);         // (A - C).(A - B)
+    const float w = alpha1 - ba * ba;     // alpha1 sin^2
+    const float ia = MP::div(1.0f, alpha1);
+    sin2 = (alpha1 > 0.0f && L > 0.0f) ? w * ia : ((alpha1 > 0.0f || L > 0.0f) ? 1.0f : -1.0f);
+    // interior: the point sL along the edge from C, t along AB
+    const float sL = MP::div(alpha1 * ac + ba * g, w);
+    const float t = (g + ba * sL) * ia;
+    if (t > 0.0f && t < 1.0f && sL > 0.0f && sL < L) {
+        // (C + sL e) - (A + t BA)
+        const float dx = sL * ex - (AC.x + t * BA.x);
+        const float dy = sL * ey - (AC.y + t * BA.y);
+        const float dz = sL * ez - (AC.z + t * BA.z);
+        const float d = MP::root(dx * dx + (dy * dy + dz * dz));
+        return d < p.eq ? 0.0f : d;
+    }
+    // C and D projected on AB, A and B projected on the edge, all clamped
+    const float o0 = fminf(fmaxf(g * ia, 0.0f), 1.0f);
+    const float o1 = fminf(fmaxf((g + L * ba) * ia, 0.0f), 1.0f);
+    const float x0 = AC.x + o0 * BA.x, y0 = AC.y + o0 * BA.y, z0 = AC.z + o0 * BA.z;
+    const float x1 = AC.x + o1 * BA.x - L * ex, y1 = AC.y + o1 * BA.y - L * ey, z1 = AC.z + o1 * BA.z - L * ez;
+    const float ca = fminf(fmaxf(ac, 0.0f), L), bc = ba + ac, cb = fminf(fmaxf(bc, 0.0f), L);
+    const float xa = AC.x - ca * ex, ya = AC.y - ca * ey, za = AC.z - ca * ez;
+    const float xb = AC.x + BA.x - cb * ex, yb = AC.y + BA.y - cb * ey, zb = AC.z + BA.z - cb * ez;
+    float d = p.maxd;
+    d = smin(d, MP::root(x0 * x0 + (y0 * y0 + z0 * z0)));
+    d = smin(d, MP::root(x1 * x1 + (y1 * y1 + z1 * z1)));
+    d = smin(d, MP::root(xa * xa + (ya * ya + za * za)));
+    d = smin(d, MP::root(xb * xb + (yb * yb + zb * zb)));
+    return d;
 }
 
 // ---------------------------------------------------------------- seg-pt
@@ -434,10 +489,20 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
     } else {
         for (int e = 0; e < nedges; ++e) {
             const int ed = list_edge(kind, param, e);
-            const Vec3<T> C = b.corner(ed & 7), D = b.corner(ed >> 3);
             NP<T> t;
-            const T d_tmp = seg_seg<T, WANT_NP, MP>(A, B, C, D, p, &t);
-            if (info) info->sin2 = smin(info->sin2, seg_seg_sin2<T, MP>(A, B, C, D));
+            T d_tmp;
+            if constexpr (MP::kAxisEdges && !WANT_NP) {
+                // the corners differ in one bit: the edge's axis
+                const int c0 = ed & 7, c1 = ed >> 3, bit = c0 ^ c1;
+                const int ax = bit == 1 ? 0 : (bit == 2 ? 1 : 2);
+                T s2;
+                d_tmp = seg_edge<MP>(A, B, b.corner(c0 & c1), ax, b.hi_at(ax) - b.lo_at(ax), p, s2);
+                if (info) info->sin2 = smin(info->sin2, s2);
+            } else {
+                const Vec3<T> C = b.corner(ed & 7), D = b.corner(ed >> 3);
+                d_tmp = seg_seg<T, WANT_NP, MP>(A, B, C, D, p, &t);
+                if (info) info->sin2 = smin(info->sin2, seg_seg_sin2<T, MP>(A, B, C, D));
+            }
             if (direct) {
                 d_c = d_tmp;
                 if (WANT_NP) *np = t;
