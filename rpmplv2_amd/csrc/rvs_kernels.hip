@@ -154,7 +154,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // whose distance is <= 0, which then becomes the profile value (the running
 // minimum is positive up to that pair).  A minimum of floats is exact in any
 // order, and only the first non-positive value can reach the profile, so the
-// reduction order cannot change a bit of the result.
+// reduction order cannot change a bit of the result.  Without nearest points
+// the obstacles go in rounds of 64 (lane t takes obstacle j0 + t), the wave's
+// minimum after each round bounds the broad phase of the next, and the link
+// ends after the first round with a hit.
 constexpr int kRefineWaves = 4;   // waves per workgroup of the refinement kernel (they share the staged obstacles)
 
 // With idx null the kernel serves a direct call: wave k takes configuration
@@ -191,12 +194,9 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
             float first_v = 0.0f;
             Vec3<float> slo, shi;
             seg_bounds(P0, P1, slo, shi);
-            for (int j = t; j < M; j += 64) {
+            // filtered distance of (link i, obstacle j), and its nearest points
+            auto pair = [&](int j, NP<float>& np) -> float {
                 float dt;
-                NP<float> np;
-                // broad phase against this lane's running minimum, an upper
-                // bound of the link's minimum over all lanes
-                if (!WANT_NP && cull_pair(env.cull + 6 * j, slo, shi, cull_bound(lo, r, p.eq))) continue;
                 if (s.gnd[j] && i < rb.ground_included) {
                     dt = p.maxd;
                     if (WANT_NP) {
@@ -216,13 +216,18 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
                     const Box<float> bx = s.box[j];
                     dt = capsule_box<float, WANT_NP>(P0, P1, r, bx, p, &np, nullptr, nullptr);
                 }
-                if (dt > s.tol[j]) dt = p.maxd;
-                lo = smin(lo, dt);
-                if (dt <= 0 && first == INT_MAX) {
-                    first = j;
-                    first_v = dt;
-                }
-                if (WANT_NP) {
+                return dt > s.tol[j] ? p.maxd : dt;
+            };
+            if (WANT_NP) {
+                // every pair, its points stored
+                for (int j = t; j < M; j += 64) {
+                    NP<float> np;
+                    const float dt = pair(j, np);
+                    lo = smin(lo, dt);
+                    if (dt <= 0 && first == INT_MAX) {
+                        first = j;
+                        first_v = dt;
+                    }
                     float* e = out.nearest + ((c * M + j) * L + i) * 6;
                     if (np.valid) {
                         e[0] = np.r.x; e[1] = np.r.y; e[2] = np.r.z;
@@ -230,6 +235,27 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
                     } else {
                         for (int k2 = 0; k2 < 6; ++k2) e[k2] = NAN;
                     }
+                }
+            } else {
+                // rounds of 64 obstacles.  The broad phase bound is the
+                // wave's minimum over the earlier rounds (an upper bound of
+                // the link's minimum; positive while no pair has hit, so a
+                // pair that can be <= 0 is never skipped), and the first round
+                // with a hit ends the link: later rounds hold only larger
+                // obstacle indices, so the first hit is among this round's.
+                for (int j0 = 0; j0 < M; j0 += 64) {
+                    const int j = j0 + t;
+                    if (j < M && !cull_pair(env.cull + 6 * j, slo, shi, cull_bound(lo, r, p.eq))) {
+                        NP<float> np;
+                        const float dt = pair(j, np);
+                        lo = smin(lo, dt);
+                        if (dt <= 0) {
+                            first = j;
+                            first_v = dt;
+                        }
+                    }
+                    if (__ballot(first != INT_MAX)) break;
+                    for (int o = 32; o > 0; o >>= 1) lo = smin(lo, __shfl_xor(lo, o, 64));
                 }
             }
             for (int o = 32; o > 0; o >>= 1) {

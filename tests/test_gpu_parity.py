@@ -557,3 +557,26 @@ def test_small_batches_one_wave_per_configuration(oracle, engine, n):
         assert np.array_equal(x, y, equal_nan=True)
     od, oprof, over, onp = oracle.distance(w.robot, oracle.scene_of(w.scene), w.q, 32, want_np=True)
     assert (a[2] == over).all() and ulps(a[0], od).max() <= 2
+
+
+@pytest.mark.parametrize("oriented", [True, False])
+def test_small_batches_many_obstacles(oracle, engine, oriented):
+    """One wave per configuration over 512 boxes: the obstacles go in rounds
+    of 64 with the wave's running minimum as the broad-phase bound, and a
+    link ends after the first round with a hit.  Results equal the
+    one-lane-per-configuration kernel's bit for bit (configurations free,
+    colliding at link 0 and colliding further out, first hits beyond the
+    first round)."""
+    w = W.workload5(300)
+    rid = engine.load_robot(w.robot)
+    eid = engine.create_env(w.scene, rvs.ENV_OBB if oriented else rvs.ENV_AABB)
+    engine.set_small_batch(1 << 20)
+    a = engine.distance(rid, eid, w.q)
+    engine.set_small_batch(0)
+    b = engine.distance(rid, eid, w.q)
+    engine.set_small_batch(16384)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y, equal_nan=True)
+    od, oprof, over, _ = oracle.distance(w.robot, oracle.scene_of(w.scene, oriented=oriented), w.q, 32)
+    print(f"oriented={oriented}: collision-free {a[2].mean():.3f}")
+    assert (a[2] == over).all() and ulps(a[0], od).max() <= 2 and ulps(a[1], oprof).max() <= 2
