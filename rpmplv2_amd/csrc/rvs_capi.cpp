@@ -38,6 +38,8 @@ rvs_status hip_fail(rvs_context* c, hipError_t e, const char* what) {
 
 // Workgroup LDS the kernels may ask for.
 constexpr size_t kLdsLimit = 64 * 1024;
+// exact-path batches of at least this many configurations run in the walk order
+constexpr int64_t kExactSortMin = 65536;
 
 // The obstacle set must fit the LDS image of the kernels (oriented boxes
 // take 48 more bytes each than axis-aligned ones).
@@ -682,7 +684,12 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     hipError_t e = hipSuccess;
     // refinement list (FP32_REFINE) lives at the start of the arena
     const size_t refine_bytes = (prec == RVS_PREC_FP32_REFINE) ? (size_t)n * 4 + 256 : 0;
-    const size_t coh_need = (prec == RVS_PREC_FP32_REFINE) ? coh_bytes(n, D) : 0;
+    // the exact paths walk large batches in the walk order too (lanes of a
+    // wave then hold similar poses and take the same branches); below this
+    // size the sort costs more than it saves
+    const bool sort_exact = prec != RVS_PREC_FP32_REFINE && n >= kExactSortMin && n <= INT32_MAX &&
+                            (prec == RVS_PREC_FP64 || n > c->wave_max);
+    const size_t coh_need = (prec == RVS_PREC_FP32_REFINE || sort_exact) ? coh_bytes(n, D) : 0;
     int32_t* ridx = nullptr;
     uint32_t* rcount = nullptr;
     uint32_t* coh = nullptr;
@@ -734,9 +741,13 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
             if (e == hipSuccess) e = d2h(&refined, rcount, 4, c->stream);
         } else {
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
+            const int32_t* perm = nullptr;
+            if (e == hipSuccess && sort_exact)
+                e = launch_coherence_order(rb.dev, qd, n, c->tuning_for(robot, env, L, M), coh, &perm, c->stream);
             if (e == hipSuccess)
-                e = (prec == RVS_PREC_FP64) ? launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream)
-                                            : launch_distance_ref32_auto(rb.dev, ev.dev, qp, qd, n, out, c->wave_max, c->stream);
+                e = (prec == RVS_PREC_FP64)
+                        ? launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream, perm)
+                        : launch_distance_ref32_auto(rb.dev, ev.dev, qp, qd, n, out, c->wave_max, c->stream, perm);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
         }
     }

@@ -162,8 +162,10 @@ hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const Q
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st);
 
 
+// One lane per configuration; with perm, lane g takes configuration perm[g]
+// (a walk order from launch_coherence_order).
 hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                 int64_t n, const DistOut& out, hipStream_t st);
+                                 int64_t n, const DistOut& out, hipStream_t st, const int32_t* perm = nullptr);
 // The same results with one wave per configuration (lanes over obstacles):
 // the low-latency form for small batches.
 hipError_t launch_distance_ref32_waves(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
@@ -171,12 +173,12 @@ hipError_t launch_distance_ref32_waves(const RobotDev& rb, const EnvDev& env, co
 // Either form by batch size: waves up to wave_max configurations.
 inline hipError_t launch_distance_ref32_auto(const RobotDev& rb, const EnvDev& env, const QueryParams& qp,
                                              const float* q, int64_t n, const DistOut& out, int64_t wave_max,
-                                             hipStream_t st) {
+                                             hipStream_t st, const int32_t* perm = nullptr) {
     return n <= wave_max ? launch_distance_ref32_waves(rb, env, qp, q, n, out, st)
-                         : launch_distance_ref32(rb, env, qp, q, n, out, st);
+                         : launch_distance_ref32(rb, env, qp, q, n, out, st, perm);
 }
 hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                int64_t n, const DistOut& out, hipStream_t st);
+                                int64_t n, const DistOut& out, hipStream_t st, const int32_t* perm = nullptr);
 hipError_t launch_is_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
                            uint8_t* valid, int fp64, hipStream_t st);
 hipError_t launch_underestimation(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,

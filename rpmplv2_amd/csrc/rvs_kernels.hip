@@ -49,16 +49,14 @@ template <typename T, bool WANT_NP, bool OBB>
 __device__ __attribute__((always_inline)) inline void distance_body(const RobotDev& rb, const EnvDev& env,
                                                                     const QueryParams& qp, const float* __restrict__ q,
                                                                     int64_t n, const DistOut& out,
-                                                                    const int32_t* idx = nullptr,
-                                                                    const uint32_t* count = nullptr) {
+                                                                    const int32_t* __restrict__ perm) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<T> s = stage_env<T, OBB>(env, smem);
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx) {
-        if (c >= (int64_t)*count) return;
-        c = idx[c];
-    }
     if (c >= n) return;
+    // lane g takes configuration perm[g] of the walk order (similar poses in
+    // one wave): the results do not depend on the order
+    if (perm) c = perm[c];
     const int M = env.num_obj;
     const int L = rb.num_links;
     const float* qc = q + c * rb.num_dofs;
@@ -140,8 +138,9 @@ __device__ __attribute__((always_inline)) inline void distance_body(const RobotD
 // kernel keeps its natural allocation (a cap would spill).
 template <bool WANT_NP, bool OBB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_distance32(
-    RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out) {
-    distance_body<float, WANT_NP, OBB>(rb, env, qp, q, n, out);
+    RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out,
+    const int32_t* __restrict__ perm) {
+    distance_body<float, WANT_NP, OBB>(rb, env, qp, q, n, out, perm);
 }
 
 // Exact recomputation of the configurations the FP32 bulk pass flagged: one
@@ -406,8 +405,9 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine_pairs(Rob
 
 template <bool WANT_NP, bool OBB>
 __global__ __launch_bounds__(kBlock) void k_distance64(RobotDev rb, EnvDev env, QueryParams qp,
-                                                       const float* __restrict__ q, int64_t n, DistOut out) {
-    distance_body<double, WANT_NP, OBB>(rb, env, qp, q, n, out);
+                                                       const float* __restrict__ q, int64_t n, DistOut out,
+                                                       const int32_t* __restrict__ perm) {
+    distance_body<double, WANT_NP, OBB>(rb, env, qp, q, n, out, perm);
 }
 
 // ---------------------------------------------------------------- isValid
@@ -735,31 +735,31 @@ inline dim3 grid_for(int64_t n, int block) { return dim3((unsigned)((n + block -
 }  // namespace
 
 hipError_t launch_distance_ref32(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                 int64_t n, const DistOut& out, hipStream_t st) {
+                                 int64_t n, const DistOut& out, hipStream_t st, const int32_t* perm) {
     if (n <= 0) return hipSuccess;
     const size_t sm = staged_bytes<float>(env);
     const dim3 g = grid_for(n, kBlock), b(kBlock);
     if (env.use_obb) {
-        if (out.nearest) hipLaunchKernelGGL((k_distance32<true, true>), g, b, sm, st, rb, env, qp, q, n, out);
-        else hipLaunchKernelGGL((k_distance32<false, true>), g, b, sm, st, rb, env, qp, q, n, out);
+        if (out.nearest) hipLaunchKernelGGL((k_distance32<true, true>), g, b, sm, st, rb, env, qp, q, n, out, perm);
+        else hipLaunchKernelGGL((k_distance32<false, true>), g, b, sm, st, rb, env, qp, q, n, out, perm);
     } else {
-        if (out.nearest) hipLaunchKernelGGL((k_distance32<true, false>), g, b, sm, st, rb, env, qp, q, n, out);
-        else hipLaunchKernelGGL((k_distance32<false, false>), g, b, sm, st, rb, env, qp, q, n, out);
+        if (out.nearest) hipLaunchKernelGGL((k_distance32<true, false>), g, b, sm, st, rb, env, qp, q, n, out, perm);
+        else hipLaunchKernelGGL((k_distance32<false, false>), g, b, sm, st, rb, env, qp, q, n, out, perm);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
-                                int64_t n, const DistOut& out, hipStream_t st) {
+                                int64_t n, const DistOut& out, hipStream_t st, const int32_t* perm) {
     if (n <= 0) return hipSuccess;
     const size_t sm = staged_bytes<double>(env);
     const dim3 g = grid_for(n, kBlock), b(kBlock);
     if (env.use_obb) {
-        if (out.nearest) hipLaunchKernelGGL((k_distance64<true, true>), g, b, sm, st, rb, env, qp, q, n, out);
-        else hipLaunchKernelGGL((k_distance64<false, true>), g, b, sm, st, rb, env, qp, q, n, out);
+        if (out.nearest) hipLaunchKernelGGL((k_distance64<true, true>), g, b, sm, st, rb, env, qp, q, n, out, perm);
+        else hipLaunchKernelGGL((k_distance64<false, true>), g, b, sm, st, rb, env, qp, q, n, out, perm);
     } else {
-        if (out.nearest) hipLaunchKernelGGL((k_distance64<true, false>), g, b, sm, st, rb, env, qp, q, n, out);
-        else hipLaunchKernelGGL((k_distance64<false, false>), g, b, sm, st, rb, env, qp, q, n, out);
+        if (out.nearest) hipLaunchKernelGGL((k_distance64<true, false>), g, b, sm, st, rb, env, qp, q, n, out, perm);
+        else hipLaunchKernelGGL((k_distance64<false, false>), g, b, sm, st, rb, env, qp, q, n, out, perm);
     }
     return hipGetLastError();
 }

@@ -540,11 +540,12 @@ def test_fp32_refine_after_env_update(oracle, engine):
     assert (v == ov).all() and np.abs(d - od).max() <= 1e-5
 
 
-@pytest.mark.parametrize("n", [1, 7, 300, 5000])
+@pytest.mark.parametrize("n", [1, 7, 300, 5000, 70000])
 def test_small_batches_one_wave_per_configuration(oracle, engine, n):
     """Small reference-arithmetic batches run one wave per configuration;
     results (distances, profiles, verdicts, nearest points) are those of the
-    one-lane-per-configuration kernel bit for bit."""
+    one-lane-per-configuration kernel bit for bit (which walks batches of
+    65,536 configurations or more in the walk order)."""
     w = W.workload2(n)
     rid = engine.load_robot(w.robot)
     eid = engine.create_env(w.scene)
