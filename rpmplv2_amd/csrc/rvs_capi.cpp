@@ -1075,16 +1075,29 @@ rvs_status rvs_is_valid(rvs_context* c, int32_t robot, int32_t env, const float*
     hipError_t e = hipSuccess;
     const float* qd = q;
     uint8_t* vd = valid;
+    // large batches run in the walk order (as rvs_distance's exact paths)
+    const size_t coh_need = (n >= kExactSortMin && n <= INT32_MAX) ? coh_bytes(n, D) : 0;
+    uint32_t* coh = nullptr;
     if (!(flags & RVS_F_DEVICE)) {
-        e = c->arena.ensure(Carve::need({(size_t)n * D * 4, (size_t)n}));
+        e = c->arena.ensure(Carve::need({coh_need, (size_t)n * D * 4, (size_t)n}));
         if (e != hipSuccess) return hip_fail(c, e, "rvs_is_valid: staging");
         Carve cv(c->arena.p);
+        if (coh_need) coh = cv.take<uint32_t>(coh_need / 4);
         float* qs = cv.take<float>((size_t)n * D);
         vd = cv.take<uint8_t>(n);
         e = h2d(qs, q, (size_t)n * D * 4, c->stream);
         qd = qs;
+    } else if (coh_need) {
+        e = c->arena.ensure(Carve::need({coh_need}));
+        if (e != hipSuccess) return hip_fail(c, e, "rvs_is_valid: staging");
+        Carve cv(c->arena.p);
+        coh = cv.take<uint32_t>(coh_need / 4);
     }
-    if (e == hipSuccess) e = launch_is_valid(rb.dev, ev.dev, qp, qd, n, vd, fp64, c->stream);
+    const int32_t* perm = nullptr;
+    if (e == hipSuccess && coh)
+        e = launch_coherence_order(rb.dev, qd, n, c->tuning_for(robot, env, rb.num_links, ev.dev.num_obj), coh, &perm,
+                                   c->stream);
+    if (e == hipSuccess) e = launch_is_valid(rb.dev, ev.dev, qp, qd, n, vd, fp64, c->stream, perm);
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(valid, vd, (size_t)n, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_is_valid");

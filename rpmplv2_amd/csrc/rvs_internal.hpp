@@ -180,7 +180,7 @@ inline hipError_t launch_distance_ref32_auto(const RobotDev& rb, const EnvDev& e
 hipError_t launch_distance_fp64(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                 int64_t n, const DistOut& out, hipStream_t st, const int32_t* perm = nullptr);
 hipError_t launch_is_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
-                           uint8_t* valid, int fp64, hipStream_t st);
+                           uint8_t* valid, int fp64, hipStream_t st, const int32_t* perm = nullptr);
 hipError_t launch_underestimation(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                   int64_t n, const float* np_ref, int64_t np_stride, float* d, float* profile,
                                   uint8_t* stored, hipStream_t st);

@@ -416,11 +416,13 @@ __global__ __launch_bounds__(kBlock) void k_distance64(RobotDev rb, EnvDev env, 
 // first hit.
 template <typename T, bool OBB>
 __global__ __launch_bounds__(kBlock) void k_valid(RobotDev rb, EnvDev env, QueryParams qp,
-                                                  const float* __restrict__ q, int64_t n, uint8_t* valid) {
+                                                  const float* __restrict__ q, int64_t n, uint8_t* valid,
+                                                  const int32_t* __restrict__ perm) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<T> s = stage_env<T, OBB>(env, smem);
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
+    if (perm) c = perm[c];   // walk order: the verdicts do not depend on it
     const int M = env.num_obj;
     const int L = rb.num_links;
     const float* qc = q + c * rb.num_dofs;
@@ -829,17 +831,17 @@ hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const Q
 }
 
 hipError_t launch_is_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
-                           uint8_t* valid, int fp64, hipStream_t st) {
+                           uint8_t* valid, int fp64, hipStream_t st, const int32_t* perm) {
     if (n <= 0) return hipSuccess;
     const dim3 g = grid_for(n, kBlock), b(kBlock);
     if (fp64) {
         const size_t sm = staged_bytes<double>(env);
-        if (env.use_obb) hipLaunchKernelGGL((k_valid<double, true>), g, b, sm, st, rb, env, qp, q, n, valid);
-        else hipLaunchKernelGGL((k_valid<double, false>), g, b, sm, st, rb, env, qp, q, n, valid);
+        if (env.use_obb) hipLaunchKernelGGL((k_valid<double, true>), g, b, sm, st, rb, env, qp, q, n, valid, perm);
+        else hipLaunchKernelGGL((k_valid<double, false>), g, b, sm, st, rb, env, qp, q, n, valid, perm);
     } else {
         const size_t sm = staged_bytes<float>(env);
-        if (env.use_obb) hipLaunchKernelGGL((k_valid<float, true>), g, b, sm, st, rb, env, qp, q, n, valid);
-        else hipLaunchKernelGGL((k_valid<float, false>), g, b, sm, st, rb, env, qp, q, n, valid);
+        if (env.use_obb) hipLaunchKernelGGL((k_valid<float, true>), g, b, sm, st, rb, env, qp, q, n, valid, perm);
+        else hipLaunchKernelGGL((k_valid<float, false>), g, b, sm, st, rb, env, qp, q, n, valid, perm);
     }
     return hipGetLastError();
 }
