@@ -1032,13 +1032,24 @@ rvs_status rvs_distance_f64(rvs_context* c, int32_t robot, int32_t env, const fl
     DistOut out{};
     const float* qd = q;
     hipError_t e = hipSuccess;
+    // large batches run in the walk order (as rvs_distance's exact paths)
+    const size_t coh_need = (n >= kExactSortMin && n <= INT32_MAX) ? coh_bytes(n, D) : 0;
+    uint32_t* coh = nullptr;
     if (flags & RVS_F_DEVICE) {
         out.d64 = d; out.profile64 = profile; out.verdict = verdict;
+        if (coh_need) {
+            e = c->arena.ensure(Carve::need({coh_need}));
+            if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_f64: staging");
+            Carve cv(c->arena.p);
+            coh = cv.take<uint32_t>(coh_need / 4);
+        }
     } else {
-        const size_t need = Carve::need({(size_t)n * D * 4, (size_t)n * 8, profile ? (size_t)n * L * 8 : 0, (size_t)n});
+        const size_t need = Carve::need({coh_need, (size_t)n * D * 4, (size_t)n * 8, profile ? (size_t)n * L * 8 : 0,
+                                         (size_t)n});
         e = c->arena.ensure(need);
         if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_f64: staging");
         Carve cv(c->arena.p);
+        if (coh_need) coh = cv.take<uint32_t>(coh_need / 4);
         float* qs = cv.take<float>((size_t)n * D);
         out.d64 = cv.take<double>(n);
         out.profile64 = profile ? cv.take<double>((size_t)n * L) : nullptr;
@@ -1046,7 +1057,10 @@ rvs_status rvs_distance_f64(rvs_context* c, int32_t robot, int32_t env, const fl
         e = h2d(qs, q, (size_t)n * D * 4, c->stream);
         qd = qs;
     }
-    if (e == hipSuccess) e = launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream);
+    const int32_t* perm = nullptr;
+    if (e == hipSuccess && coh)
+        e = launch_coherence_order(rb.dev, qd, n, c->tuning_for(robot, env, L, ev.dev.num_obj), coh, &perm, c->stream);
+    if (e == hipSuccess) e = launch_distance_fp64(rb.dev, ev.dev, qp, qd, n, out, c->stream, perm);
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
         e = d2h(d, out.d64, (size_t)n * 8, c->stream);
         if (e == hipSuccess) e = d2h(profile, out.profile64, (size_t)n * L * 8, c->stream);
