@@ -789,6 +789,27 @@ hipError_t async_end(rvs_context* c, hipStream_t st, void* done) {
 
 }  // namespace
 
+namespace {
+
+// Walk order for a large exact-path batch of an async call (nullptr below
+// kExactSortMin): the sort's scratch comes from the arena, which async calls
+// use one after another (async_begin orders each after the previous one).
+hipError_t async_exact_order(rvs_context* c, int32_t robot, int32_t env, const RobotRec& rb, const EnvRec& ev,
+                             const float* q, int64_t n, hipStream_t st, const int32_t** perm) {
+    *perm = nullptr;
+    if (n < kExactSortMin || n > INT32_MAX) return hipSuccess;
+    const size_t need = Carve::need({coh_bytes(n, rb.num_dofs)});
+    if (need > c->arena.cap) c->drain_async();   // growing frees the block earlier calls use
+    hipError_t e = c->arena.ensure(need);
+    if (e != hipSuccess) return e;
+    Carve cv(c->arena.p);
+    uint32_t* coh = cv.take<uint32_t>(coh_bytes(n, rb.num_dofs) / 4);
+    return launch_coherence_order(rb.dev, q, n, c->tuning_for(robot, env, rb.num_links, ev.dev.num_obj), coh, perm,
+                                  st);
+}
+
+}  // namespace
+
 rvs_status rvs_distance_async(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n, float* d,
                               float* profile, uint8_t* verdict, uint32_t flags, void* stream, void* done) {
     rvs_status s = check_query(c, robot, env, q, n, true);
@@ -826,10 +847,13 @@ rvs_status rvs_distance_async(rvs_context* c, int32_t robot, int32_t env, const 
             if (e == hipSuccess)
                 e = launch_distance_fast(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, ft, perm, c->num_cus, st);
             if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, st);
-        } else if (prec == RVS_PREC_FP64) {
-            e = launch_distance_fp64(rb.dev, ev.dev, qp, q, n, out, st);
         } else {
-            e = launch_distance_ref32_auto(rb.dev, ev.dev, qp, q, n, out, c->wave_max, st);
+            const int32_t* perm = nullptr;
+            if (prec == RVS_PREC_FP64 || n > c->wave_max) e = async_exact_order(c, robot, env, rb, ev, q, n, st, &perm);
+            if (e == hipSuccess)
+                e = prec == RVS_PREC_FP64 ? launch_distance_fp64(rb.dev, ev.dev, qp, q, n, out, st, perm)
+                                          : launch_distance_ref32_auto(rb.dev, ev.dev, qp, q, n, out, c->wave_max, st,
+                                                                       perm);
         }
     }
     if (e == hipSuccess) e = async_end(c, st, done);
@@ -852,7 +876,9 @@ rvs_status rvs_is_valid_async(rvs_context* c, int32_t robot, int32_t env, const 
     (void)hipSetDevice(c->opt.device);
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
     hipError_t e = async_begin(c, st);
-    if (e == hipSuccess && n > 0) e = launch_is_valid(rb.dev, ev.dev, params_of(c), q, n, valid, fp64, st);
+    const int32_t* perm = nullptr;
+    if (e == hipSuccess && n > 0) e = async_exact_order(c, robot, env, rb, ev, q, n, st, &perm);
+    if (e == hipSuccess && n > 0) e = launch_is_valid(rb.dev, ev.dev, params_of(c), q, n, valid, fp64, st, perm);
     if (e == hipSuccess) e = async_end(c, st, done);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_is_valid_async");
     return RVS_OK;
