@@ -446,6 +446,14 @@ def main():
     d_gpu, prof_gpu, ver_gpu = d.cpu().numpy(), prof.cpu().numpy(), ver.cpu().numpy()
     ms_ref32, k_ref32, _, _ = timed_distance(eng, rid, eid, q, d, prof, ver, rvs.PREC_REF32, max(3, args.steps // 2), 2)
     ms_fp64, k_fp64, _, _ = timed_distance(eng, rid, eid, q, d, prof, ver, rvs.PREC_FP64, max(3, args.steps // 2), 2)
+    # end to end from host arrays (pageable memory): the configurations go up
+    # over PCIe and d, profile and verdict come back inside each call
+    for _ in range(2):
+        eng.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        eng.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
+    ms_h2h = (time.perf_counter() - t0) * 1e3 / 5
     del q, d, prof, ver
     torch.cuda.empty_cache()
 
@@ -497,6 +505,9 @@ def main():
                        "ms": order_ms},
         "ref32_exact": {"configs_per_s": n / (ms_ref32 * 1e-3), "kernel_ms": k_ref32},
         "fp64": {"configs_per_s": n / (ms_fp64 * 1e-3), "kernel_ms": k_fp64},
+        "host_to_host": {"configs_per_s": n / (ms_h2h * 1e-3), "ms": ms_h2h,
+                         "note": "FP32 + refinement from pageable host arrays through rvs_distance (24 B per "
+                                 "configuration up, 29 B down over PCIe inside each call); not value"},
     }
     cores = host_cores()
     if not args.no_cpu:
