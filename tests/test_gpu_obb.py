@@ -150,3 +150,26 @@ def test_self_collision(oracle, engine, w5):
     om2 = oracle.self_min(xa, q2)
     assert ulps(m2, om2).max() <= 2
     assert np.array_equal(f2, (om2 >= 0).astype(np.uint8))
+
+
+@pytest.mark.parametrize("kind", [rvs.ENV_OBB, rvs.ENV_AABB])
+def test_walk_order_exact_paths(engine, kind):
+    """Batches of 65,536 configurations or more run the exact paths in the
+    walk order; the same configurations in two batches of 32,768 run in the
+    given order (same one-lane kernels).  Distances, profiles and verdicts
+    (reference arithmetic and FP64) and isValid verdicts agree bit for bit."""
+    w = W.workload5(65536)
+    rid = engine.load_robot(w.robot)
+    eid = engine.create_env(w.scene, kind)
+    parts = (slice(0, 32768), slice(32768, 65536))
+    big = engine.distance(rid, eid, w.q)
+    small = [engine.distance(rid, eid, w.q[s]) for s in parts]
+    for k in range(3):
+        assert np.array_equal(big[k], np.concatenate([h[k] for h in small]), equal_nan=True)
+    big64 = engine.distance_f64(rid, eid, w.q)
+    small64 = [engine.distance_f64(rid, eid, w.q[s]) for s in parts]
+    for k in range(3):
+        assert np.array_equal(big64[k], np.concatenate([h[k] for h in small64]), equal_nan=True)
+    v = engine.is_valid(rid, eid, w.q)
+    assert np.array_equal(v, np.concatenate([engine.is_valid(rid, eid, w.q[s]) for s in parts]))
+    assert 0 < big[2].sum() or kind == rvs.ENV_AABB
