@@ -10,7 +10,8 @@ restated in Box.cpp:3-15 / Object.cpp:21-26 of the reference.
 Workloads (sizes as quoted; each function takes a scale-down factor for tests):
   1  planar 2-DOF, r = 0, 4 boxes in a 2 m disk, 65,536 configurations
   2  xArm6, r = 0.05, ground box + 16 boxes in a 1 m ball at (0,0,0.4),
-     boxes that overlap the start pose (q = 0) rejected, 1,048,576 configurations
+     boxes that make the start pose (q = 0) collide rejected, 1,048,576
+     configurations
   3  chains of 10/14/18/22 DOF with random link lengths, r = 0.04, self-collision
      on, 64 boxes in a 1.5 m ball, 4,096 discretised edges per chain
   4  serving stream (see serving.py)
@@ -156,16 +157,54 @@ def _shoemake(rng, n):
     return np.stack([w, x, y, z], 1)
 
 
-def _overlaps_pose(rb: R.Robot, aabb: np.ndarray, q) -> bool:
-    """Conservative start-pose test: the box's bounds overlap the bounds of a
-    capsule of the pose (each capsule's segment bounds grown by its radius)."""
+def seg_box_distance(A, B, lo, hi) -> float:
+    """Exact distance from segment AB to the box [lo, hi] (float64).  The
+    squared distance of P(t) = A + t (B - A) to the box is convex and
+    piecewise quadratic in t, its pieces split where a coordinate crosses a
+    slab bound; the minimum is at a breakpoint or at a piece's vertex."""
+    A = np.asarray(A, np.float64)
+    B = np.asarray(B, np.float64)
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    D = B - A
+    ts = [0.0, 1.0]
+    for k in range(3):
+        if D[k] != 0:
+            for v in (lo[k], hi[k]):
+                t = (v - A[k]) / D[k]
+                if 0 < t < 1:
+                    ts.append(t)
+    ts.sort()
+
+    def d2(t):
+        e = np.maximum(np.maximum(lo - (A + t * D), (A + t * D) - hi), 0.0)
+        return float(e @ e)
+
+    best = min(d2(t) for t in ts)
+    for a, b in zip(ts[:-1], ts[1:]):
+        P = A + 0.5 * (a + b) * D
+        num = den = 0.0
+        for k in range(3):       # axes outside the slab on this piece: e_k = c_k + D_k t
+            if P[k] < lo[k]:
+                c = A[k] - lo[k]
+            elif P[k] > hi[k]:
+                c = A[k] - hi[k]
+            else:
+                continue
+            num += c * D[k]
+            den += D[k] * D[k]
+        if den > 0:
+            best = min(best, d2(min(max(-num / den, a), b)))
+    return float(np.sqrt(best))
+
+
+def collides_at_pose(rb: R.Robot, aabb: np.ndarray, q) -> bool:
+    """Some capsule of the pose q lies closer to the box than its radius:
+    the geometric form of isValid(q) failing for one box (the collision tree
+    of RealVectorSpace.cpp:277-309 reports the same except in the corner
+    regions of its index quirk, CollisionAndDistance.cpp:69-72)."""
     P = R.skeleton_np(rb, q)[0]
-    for i in range(rb.num_links):
-        lo = np.minimum(P[i], P[i + 1]) - rb.radius[i]
-        hi = np.maximum(P[i], P[i + 1]) + rb.radius[i]
-        if np.all(aabb[:3] <= hi) and np.all(aabb[3:] >= lo):
-            return True
-    return False
+    return any(seg_box_distance(P[i], P[i + 1], aabb[:3], aabb[3:]) < rb.radius[i] for i in range(rb.num_links))
 
 
 def uniform_configs(rng, n, limits) -> np.ndarray:
@@ -204,7 +243,7 @@ def scene2(rng=None, num_boxes: int = 16) -> Scene:
     kept: List[Scene] = []
     while len(kept) < num_boxes:
         b = _boxes(rng, 1, 0.05, 0.3, 1.0, (0, 0, 0.4))
-        if _overlaps_pose(rb, b.aabb[0], q0):
+        if collides_at_pose(rb, b.aabb[0], q0):     # CommonFunctions.h:105-110: isValid(start) fails
             continue
         kept.append(b)
     boxes = kept[0]
@@ -245,6 +284,79 @@ def workload3(edges_per_robot: int = 4096, dofs=(10, 14, 18, 22)) -> EdgeWorkloa
         lengths = rng.uniform(0.1, 0.3, n)
         rb = R.chain(lengths, radius=0.04, name=f"chain{n}", self_collision=True)
         q1 = rng.uniform(-np.pi, np.pi, size=(edges_per_robot, n)).astype(np.float32)
+        u = rng.normal(size=(edges_per_robot, n))
+        u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+        K = np.rint(np.exp(rng.uniform(np.log(8), np.log(256), edges_per_robot))).astype(np.int32)
+        batches.append(EdgeBatch(rb, q1, u, K))
+    return EdgeWorkload(sc, batches)
+
+
+def _capsule_overlaps_box(P: np.ndarray, radius, aabb: np.ndarray, margin: float = 0.0) -> bool:
+    """Conservative test: some capsule of the skeleton P ((L + 1) x 3,
+    radii per link) has its bounds grown by radius + margin overlapping the
+    box's bounds."""
+    for i in range(len(P) - 1):
+        g = radius[i] + margin
+        lo = np.minimum(P[i], P[i + 1]) - g
+        hi = np.maximum(P[i], P[i + 1]) + g
+        if np.all(aabb[:3] <= hi) and np.all(aabb[3:] >= lo):
+            return True
+    return False
+
+
+def workload3_planner(edges_per_robot: int = 4096, dofs=(10, 14, 18, 22), spread: float = 0.35,
+                      oracle=None) -> EdgeWorkload:
+    """Edges as a tree-based planner checks them: RRTConnect::extend checks
+    only edges that leave a tree node, and tree nodes are collision-free
+    (RRTConnect.cpp:92-105 -> RealVectorSpace.cpp:261-275; the scene and
+    start rule of CommonFunctions.h:77-113, which re-draws every random
+    obstacle that makes the start invalid).  As specified, workload 3 has no
+    such state: one of its 64 boxes overlaps the chains' fixed base link, so
+    every configuration collides at state 0.
+
+    Robots: workload 3's four chains (10/14/18/22 DOF, r = 0.04,
+    self-collision on).  Scene: 64 boxes drawn as in workload 3 (sides
+    U[0.1, 0.5], centres uniform in a 1.5 m ball), each re-drawn while its
+    bounds come within 5 cm of the bounds of a capsule of any chain's home
+    pose (all joints 0: the chain straight up from the origin).  Starts: joint 1 uniform in [-pi, pi],
+    joints 2..n uniform in [-spread, spread] rad around the home pose,
+    re-drawn until the start is collision-free against the boxes and itself
+    (the oracle's own edge predicate, a one-state edge).  Edges: as in
+    workload 3 (a uniform direction in joint space, K = round(exp U[ln 8,
+    ln 256]) states 0.05 rad apart).  The start states are the only change
+    to the distribution; uniform starts in [-pi, pi]^n are collision-free
+    with probability below 1e-6 on the 22-DOF chain (6e-4 for its
+    self-collision alone).
+
+    `oracle` is the module that judges the starts (pyoracle): the generator
+    is test and benchmark infrastructure."""
+    if oracle is None:
+        raise ValueError("workload3_planner needs the oracle module to judge start states")
+    robots = [b.robot for b in workload3(16, dofs).batches]
+    homes = [R.skeleton_np(rb, np.zeros(rb.num_dofs, np.float32))[0] for rb in robots]
+    rng = np.random.default_rng(1013)
+    kept: List[Scene] = []
+    while len(kept) < 64:
+        b = _boxes(rng, 1, 0.1, 0.5, 1.5)
+        if any(_capsule_overlaps_box(P, rb.radius, b.aabb[0], 0.05) for P, rb in zip(homes, robots)):
+            continue
+        kept.append(b)
+    sc = kept[0]
+    for b in kept[1:]:
+        sc = sc.concat(b)
+    osc = oracle.scene_of(sc)
+    batches = []
+    for rb in robots:
+        n = rb.num_dofs
+        starts = np.zeros((0, n), np.float32)
+        while len(starts) < edges_per_robot:
+            m = 4 * edges_per_robot
+            q = np.empty((m, n), np.float32)
+            q[:, 0] = rng.uniform(-np.pi, np.pi, m)
+            q[:, 1:] = rng.uniform(-spread, spread, (m, n - 1))
+            first = oracle.edge_first_collision(rb, osc, q, np.zeros_like(q), np.ones(m, np.int32), 0.05)
+            starts = np.concatenate([starts, q[first < 0]])
+        q1 = np.ascontiguousarray(starts[:edges_per_robot])
         u = rng.normal(size=(edges_per_robot, n))
         u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
         K = np.rint(np.exp(rng.uniform(np.log(8), np.log(256), edges_per_robot))).astype(np.int32)

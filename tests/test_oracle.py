@@ -147,18 +147,187 @@ def test_collision_tree_rules(oracle):
     assert not oracle.collision_capsule_box([1.2, -5, 0.5], [1.3, 5, 0.5], r, obs)
     # a segment crossing the box
     assert oracle.collision_capsule_box([-1, 0.5, 0.5], [2, 0.5, 0.5], r, obs)
-    # statistical agreement with the exact distance test
+    # exact agreement with the exact distance test outside a 1e-3 m band.
+    # On the unit cube every test of the tree is exact except the corner
+    # tests with the index quirk, and each quirked corner point lies at least
+    # 1 - r from the corner square it is tested in, so none of them can fire:
+    # any disagreement is a restatement error
     rng = np.random.default_rng(6)
-    agree = n = 0
-    for _ in range(5000):
+    n = 0
+    for _ in range(20000):
         A = rng.uniform(-0.5, 1.5, 3).astype(np.float32)
         B = (A + rng.uniform(-0.5, 0.5, 3)).astype(np.float32)
         true = oracle.true_seg_box(A.astype(np.float64), B.astype(np.float64), obs.astype(np.float64))
         if abs(true - r) < 1e-3:
             continue
         n += 1
-        agree += oracle.collision_capsule_box(A, B, r, obs) == (true < r)
-    assert agree / n > 0.97
+        assert oracle.collision_capsule_box(A, B, r, obs) == (true < r), (A, B, true)
+    assert n > 19000
+
+
+# Segments of a planar arm's first link at q0 (A at the origin, B at
+# (cos q0, sin q0, 0)) whose crossing of the box face x = xmin lands where one
+# of the quirked corner tests of collisionCapsuleToRectangle fires although
+# the capsule is farther than r from the box.
+# (name, box, q0, exact distance)
+QUIRK_CASES = [
+    ("e", (0.5, -1.0, 0.03, 1.0, 0.01, 1.0), float(np.arctan(0.11)), 0.05386),
+    ("c", (0.5, -0.1, 0.048, 1.0, 0.005, 1.0), float(np.arctan(-0.24)), 0.05179),
+    ("f", (0.5, -1.0, -0.035, 1.5, -0.045, -0.03), 0.0, 0.05408),
+]
+
+
+@pytest.mark.parametrize("name,obs,q0,true", QUIRK_CASES)
+def test_collision_tree_corner_index_quirk(oracle, name, obs, q0, true):
+    """Hand trace of collisionCapsuleToBox (CollisionAndDistance.cpp:4-42)
+    into collisionCapsuleToRectangle (:46-111) for a capsule of r = 0.05 that
+    is 0.052-0.054 m from the box, so the exact answer is "free":
+      :7       r_new = 0.05 * sqrt(3) / 3 = 0.0289
+      :9-13    neither end point is inside the box grown by r_new (A.x = 0 <
+               0.5 - 0.0289; B lies beyond y or z of the box by more)
+      :14-17   no coordinate has both end points beyond one face by r
+      :19-20   A.x = 0 < x_min = 0.5: the rectangle of face 0 (coord 0)
+      :54      rec = (y_min, z_min, y_max, z_max), the (min1, min2, max1,
+               max2) layout; A_rec, B_rec = (y, z) of A and B
+      :57-58   t = (0.5 - 0) / (B.x - 0) in (0, 1); M = (y, z) where the
+               segment crosses x = 0.5, here (0.5 tan q0, 0)
+      :64-65   M lies inside the rectangle grown by r on both axes
+      :67-68   but not in its y span (a) nor its z span (b): M sits in a
+               corner square
+    case e, :71: M(0) > rec(1) = z_min and M(1) < rec(2) = y_max and
+               |M - (rec(1), rec(2))| = |(0.055 - 0.03, 0 - 0.01)| = 0.027 < r
+               -> true.  The corner M is near is (y_max, z_min) at 0.054.
+    case c, :69: M(0) < rec(0) = y_min and M(1) < rec(2) = y_max and
+               |M - (rec(0), rec(2))| = |(-0.12 + 0.1, 0 - 0.005)| = 0.021 < r
+               -> true.  The corner M is near is (y_min, z_min) at 0.052.
+    case f, :72: M = (0, 0); M(0) > rec(1) = z_min and M(1) > rec(3) = z_max
+               and |M - (rec(1), rec(3))| = |(0.035, 0.03)| = 0.046 < r
+               -> true.  The corner M is near is (y_max, z_max) at 0.054.
+    A restatement that tests the geometric corners, (rec(0), rec(1)),
+    (rec(2), rec(1)), (rec(2), rec(3)), answers "free" in all three (every
+    other test of the tree is exact, so nothing else reports a hit at a
+    distance above r), and so does the distance tree."""
+    r = np.float32(0.05)
+    obs = np.asarray(obs, np.float32)
+    A = np.zeros(3, np.float32)
+    B = np.array([np.cos(q0), np.sin(q0), 0.0], np.float32)
+    t = oracle.true_seg_box(A.astype(np.float64), B.astype(np.float64), obs.astype(np.float64))
+    assert abs(t - true) < 1e-4 and t > r
+    assert oracle.collision_capsule_box(A, B, r, obs)
+    assert oracle.collision_capsule_box(A.astype(np.float64), B.astype(np.float64), float(r),
+                                        obs.astype(np.float64), 64)
+    assert oracle.capsule_box(A, B, r, obs, 32)[0] > 0
+    if name == "f":
+        # z_min only moves the quirked corner point, not the box's nearest
+        # point: with z_min = -0.045 that point is 0.054 from M and the
+        # verdict follows the geometry
+        o2 = obs.copy()
+        o2[2] = -0.045
+        assert abs(oracle.true_seg_box(A.astype(np.float64), B.astype(np.float64), o2.astype(np.float64)) - t) < 1e-12
+        assert not oracle.collision_capsule_box(A, B, r, o2)
+
+
+# (branch, A, B, exact distance) on the unit cube with r = 0.05
+FALLBACK_CASES = [
+    # :19-20 A.x = -0.09 < 0: face 0, rec = (y, z) limits; A_rec = (0.47,
+    # 0.28) inside, B_rec = (0.47, -0.46) below; :57 t = 0.45, M = (0.47,
+    # -0.053) not within r of the rectangle (:64-65); :89 only A projects:
+    # :91 min(checkCases(B_rec), |A - A_proj| = 0.09); checkCases :134-139
+    # B_rec(1) < rec(1): the edge z = 0 of face 0, at 0.0138 < r -> true
+    ("only A, edge", (-0.09, 0.47, 0.28), (0.11, 0.47, -0.46), 0.013828),
+    # :19-20 face 0; A_rec = (0.74, 1.07) above, B_rec = (0.17, 0.52) inside;
+    # :57 t < 0, :77 min(|A - A_proj| = 0.01, ...) < r goes on; :95 only B:
+    # :97 checkCases(A_rec) :140-145 A_rec(1) > rec(3): the edge z = 1 of
+    # face 0, at 0.020 < r -> true
+    ("only B, edge", (-0.01, 0.74, 1.07), (-0.09, 0.17, 0.52), 0.019972),
+    # :22-23 A.x = 1.03 > 1: face 3; A_rec = (0.49, 1.02), B_rec = (0.88,
+    # 1.24), neither inside; :57 t = 0.5, M = (0.685, 1.13) beyond z_max + r;
+    # :100 no projections: :102 checkCases(A_rec) :140-145 the edge z = 1 of
+    # face 3 at 0.036 < r -> true
+    ("none, A's edges", (1.03, 0.49, 1.02), (0.97, 0.88, 1.24), 0.036055),
+    # :22-23 face 3; A_rec = (0.54, 1.0) (on the open rectangle's border:
+    # outside), B_rec = (1.06, 0.73); :57 t = 2, :77 min(0.04, 0.02) < r goes
+    # on; :100 no projections: checkCases(A_rec) finds no edge (:121-132,
+    # :134-145 all false), :105 checkCases(B_rec) :127-132 B_rec(0) > rec(2):
+    # the edge y = 1 of face 3 at 0.022 < r -> true
+    ("none, B's edges", (1.04, 0.54, 1.0), (1.02, 1.06, 0.73), 0.022291),
+    # :28-29 A.y = 1.11 > 1: face 4, rec = (x, z) limits; both projections
+    # inside; :57 t = 1.375; :77 min(0.11, 0.03) < r goes on; :82-86 both
+    # project: min(|A - A_proj|, |B - B_proj|) = 0.03 < r -> true
+    ("both", (0.66, 1.11, 0.91), (0.44, 1.03, 0.70), 0.03),
+    # the same with B 0.06 from the face: :77 min(0.11, 0.06) > r -> false
+    ("both, far", (0.66, 1.11, 0.91), (0.44, 1.06, 0.70), 0.06),
+    # only A, the edge 0.0917 away (A 0.2 above z = 1 of face 5, B 0.03
+    # above it beyond x = 1): :89-93 min(0.0917, 0.2) > r -> false
+    ("only A, edge far", (0.5, 0.5, 1.2), (1.3, 0.5, 1.03), 0.091695),
+]
+
+
+@pytest.mark.parametrize("branch,A,B,true", FALLBACK_CASES)
+def test_collision_tree_fallback_branches(oracle, branch, A, B, true):
+    """The radius > 0 block of collisionCapsuleToRectangle (:80-108) and
+    checkCases (:113-148), one hand-traced case per branch (trace in the
+    comments of FALLBACK_CASES); the expected verdict is the exact geometry's
+    (true distance < r), since these branches measure true distances."""
+    r = np.float32(0.05)
+    obs = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    A = np.asarray(A, np.float32)
+    B = np.asarray(B, np.float32)
+    t = oracle.true_seg_box(A.astype(np.float64), B.astype(np.float64), obs.astype(np.float64))
+    assert abs(t - true) < 1e-5
+    assert oracle.collision_capsule_box(A, B, r, obs) == (t < r)
+
+
+# (A, B, expected d, case) on the unit cube with r = 0.05
+CONTACT_CASES = [
+    # :411-418 A strictly inside: d_c = 0 and compute returns before :447,
+    # so the distance is 0, not -r
+    ((0.5, 0.5, 0.5), (2, 2, 2), 0.0, "inside"),
+    # A projects on z_max, B on z_min (:462-471); num_proj = 1 (:420, the
+    # largest count on one face); idx_point = 1 (:423, equal distances);
+    # :427 idx_coord = 2; :440-441 checkEdges(A, 1): projections(2, 1),
+    # :519-521 collisionCapsuleToRectangle(A, B, 0, obs, 2) crosses z = 0
+    # inside the face -> :525 d_c = 0; :447 d_c -= r -> -r
+    ((0.5, 0.5, 1.5), (0.5, 0.5, -0.5), -0.05, "crossing"),
+    # A projects on y_min, B on x_max, both 0.5 away; idx_point = 1,
+    # checkEdges(A, 1) :489-497 face 3 not crossed (the crossing y = 0 is on
+    # the rectangle's border); getLineSegments (:548-553, A.y < y_min): the
+    # edge x = 1, y = 0; :584-587 its distance is 0 -> d_c = 0; :447 -> -r
+    ((0.5, -0.5, 0.5), (1.5, 0.5, 0.5), -0.05, "edge"),
+    # no projections, the segment through the box centre: checkOtherCases,
+    # all six faces (:739-746); the crossing -> 0 -> -r
+    ((1.5, -0.5, 0.5), (-0.5, 1.5, 0.5), -0.05, "through"),
+    # projections on x_max only, both 0.5 and 1.5 away, num_proj = 2: d_c =
+    # 0.5 (:424), :447 -> 0.5 - r
+    ((1.5, 0.5, 0.5), (2.5, 0.5, 0.5), np.float32(0.5) - np.float32(0.05), "face"),
+]
+
+
+@pytest.mark.parametrize("A,B,want,case", CONTACT_CASES)
+def test_distance_tree_contact_outputs(oracle, A, B, want, case):
+    """The distance tree's outputs at contact (Appendix of the reference
+    semantics): an end point strictly inside gives 0 (no radius subtracted,
+    CollisionAndDistance.cpp:414-418), a crossing or an edge touched gives
+    exactly -r (:447 after :525 / :584-587), otherwise d - r."""
+    obs = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    d32 = oracle.capsule_box(np.asarray(A, np.float32), np.asarray(B, np.float32), np.float32(0.05), obs, 32)[0]
+    assert d32 == np.float32(want), (case, d32)
+    d64 = oracle.capsule_box(np.asarray(A, np.float64), np.asarray(B, np.float64), 0.05, obs.astype(np.float64), 64)[0]
+    assert abs(d64 - float(want)) < 1e-7, (case, d64)
+
+
+def test_distance_tree_face_plane_nan(oracle):
+    """An end point projecting on a face with the other end point inside that
+    face's span and on its plane: getLineSegments (:542-574) selects no edge
+    and distanceToMoreLineSegments reads a matrix that was never set; the
+    restatement returns NaN with status bit 0 (path 9), which the loops of
+    RealVectorSpace drop as std::min does.  One rounding of the segment off
+    the plane makes it a crossing (-r)."""
+    obs = np.array([0.2, 0.0, -0.1, 0.8, 0.5, 0.1], np.float32)
+    d, _, _, path = oracle.capsule_box(np.float32([0, 0, 0]), np.float32([1, 0, 0]), np.float32(0.02), obs, 32)
+    assert np.isnan(d) and path == 9
+    d2 = oracle.capsule_box(np.float32([0, 0, 0]), np.float32([1, 1e-7, 0]), np.float32(0.02), obs, 32)[0]
+    assert d2 == np.float32(-0.02)
 
 
 # ---------------------------------------------------------------- kinematics
@@ -1086,3 +1255,61 @@ def test_motion_light_directions_reflects(oracle):
         assert (np.linalg.norm(m.pos[dyn] - np.array([0, 0, 0.267]), axis=1) <= 1.0 + 0.006 + 1e-5).all()
     s = np.linalg.norm(m.vel[dyn], axis=1)
     assert np.abs(s - s0).max() < 1e-4
+
+
+# ---------------------------------------------------------------- workload generators
+def test_seg_box_distance_is_exact(oracle):
+    """The workload generators' segment-box distance equals the oracle's
+    independent exact one."""
+    rng = np.random.default_rng(31)
+    for _ in range(2000):
+        A = rng.uniform(-1, 1, 3)
+        B = A + rng.uniform(-1, 1, 3)
+        lo = rng.uniform(-0.5, 0.2, 3)
+        hi = lo + rng.uniform(0.05, 0.6, 3)
+        assert abs(W.seg_box_distance(A, B, lo, hi) - oracle.true_seg_box(A, B, np.concatenate([lo, hi]))) < 1e-12
+
+
+def test_workload2_boxes_rejected_by_is_valid_at_start(oracle):
+    """Workload 2 re-draws a box while it makes the start pose (q = 0)
+    collide, as CommonFunctions.h:105-110 does with isValid(start): every
+    candidate the generator draws is accepted exactly when the oracle's
+    isValid(q = 0) against that box is true, and the accepted boxes are the
+    scene's."""
+    rb = R.xarm6()
+    q0 = np.zeros((1, 6), np.float32)
+    rng = np.random.default_rng(1002)
+    kept = []
+    drawn = 0
+    while len(kept) < 16:
+        b = W._boxes(rng, 1, 0.05, 0.3, 1.0, (0, 0, 0.4))
+        drawn += 1
+        reject = W.collides_at_pose(rb, b.aabb[0], q0[0])
+        assert reject == (oracle.is_valid(rb, oracle.scene_of(b), q0)[0] == 0)
+        if not reject:
+            kept.append(b.aabb[0])
+    sc = W.scene2()
+    assert np.array_equal(sc.aabb[1:], np.asarray(kept))
+    assert oracle.is_valid(rb, oracle.scene_of(sc), q0)[0] == 1
+    # a box on the start pose is rejected
+    hit = W.Scene([[0.1, 0.1, 0.1]], [[0.0, 0.0, 0.3]], [[1, 0, 0, 0]], [0], [W.MAX_DISTANCE])
+    assert W.collides_at_pose(rb, hit.aabb[0], q0[0])
+    assert oracle.is_valid(rb, oracle.scene_of(hit), q0)[0] == 0
+
+
+def test_workload3_planner_starts_are_valid(oracle):
+    """The planner form of workload 3: every edge starts at a state the
+    oracle finds collision-free (boxes and self-collision), the scene keeps
+    every chain's home pose free, and edges reach far past their start."""
+    w = W.workload3_planner(256, oracle=oracle)
+    sc = oracle.scene_of(w.scene)
+    assert w.scene.num_obj == 64
+    for b in w.batches:
+        n = b.robot.num_dofs
+        assert oracle.edge_first_collision(b.robot, sc, b.q1, b.dir, np.ones(len(b.q1), np.int32), b.step).max() < 0
+        home = np.zeros((1, n), np.float32)
+        assert oracle.edge_first_collision(b.robot, sc, home, home, np.ones(1, np.int32), b.step)[0] < 0
+        f = oracle.edge_first_collision(b.robot, sc, b.q1, b.dir, b.K, b.step)
+        assert (f != 0).all() and (f < 0).mean() > 0.1 and np.median(f[f >= 0]) >= 5
+    with pytest.raises(ValueError):
+        W.workload3_planner(4)
