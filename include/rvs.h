@@ -8,7 +8,9 @@
  *
  * Conventions
  *   - Plain C types only; every call returns rvs_status; no exceptions cross
- *     the ABI.  rvs_last_error() describes the last failure of a context.
+ *     the ABI.  rvs_last_error() describes the last failure of a context;
+ *     the string is the calling thread's own copy, valid until that thread
+ *     calls rvs_last_error (or rvs_server_last_error) again.
  *   - Buffers are caller-owned.  With RVS_F_DEVICE every array argument is a
  *     device pointer on the context's GPU; otherwise arrays are host memory
  *     and the call stages them through the context's device arena.  No
@@ -129,8 +131,10 @@ enum {
     /* Build extension: rotated boxes are queried as oriented boxes (the
      * capsule is moved into the box frame; nearest points come back in the
      * world frame).  Accepted by every environment query.  The obstacle set is
-     * staged in 64 KB of LDS: up to 851 oriented boxes for float queries and
-     * 624 for RVS_PREC_FP64 (RVS_ERANGE beyond). */
+     * staged in the workgroup LDS the device reports
+     * (hipDeviceAttributeMaxSharedMemoryPerBlock): with 64 KB, up to 851
+     * oriented boxes for float queries and 624 for RVS_PREC_FP64 (RVS_ERANGE
+     * beyond); environments hold at most 1024 boxes. */
     RVS_ENV_OBB = 1
 };
 
@@ -348,14 +352,19 @@ rvs_status rvs_edge_first_collision(rvs_context* ctx, int32_t env, const int32_t
  * Launch shape of the RVS_PREC_FP32_REFINE bulk pass for one (robot,
  * environment): configurations per wavefront (one per lane for the whole
  * walk, or persistent lane refill that hands a lane the next configuration
- * after each link) and workgroup size.  The obstacle set is always staged
- * whole in LDS (every lane walks every obstacle of its current link, so a
- * smaller obstacle tile would only add reloads); it must fit 64 KB. */
+ * after each link, or persistent lane refill with every lane walking its own
+ * (link, obstacle) pairs so that a wave evaluates one pair per lane per
+ * step) and workgroup size.  The obstacle set is always staged whole in LDS
+ * (every lane walks every obstacle of its current link, so a smaller
+ * obstacle tile would only add reloads); it must fit the device's workgroup
+ * LDS.  Results are the same bit for bit with every shape. */
 typedef struct rvs_tuning {
     int32_t variant;        /* 0: one lane per configuration; 1: persistent lane refill; 2: one lane per
-                               configuration in persistent waves over 64-configuration chunks */
-    int32_t block;          /* threads per workgroup: 64, 128 or 256 */
-    int32_t blocks_per_cu;  /* variants 1 and 2: resident workgroups per CU (0 = as many as fit) */
+                               configuration in persistent waves over 64-configuration chunks;
+                               3: persistent lane refill, each lane walking its own pairs (needs the
+                               broad-phase tables in LDS beside the obstacles; runs as 1 otherwise) */
+    int32_t block;          /* threads per workgroup: 64, 128 or 256; variant 3: 256, 512 or 1024 */
+    int32_t blocks_per_cu;  /* variants 1, 2 and 3: resident workgroups per CU (0 = as many as fit) */
     int32_t coherence_bits; /* walk order (0..18): configurations sorted by their first
                                min(dofs, 18 / bits) joint angles, 2^bits bins each, so that a wave's
                                lanes hold similar poses; 0 = the order given.  Results do not depend

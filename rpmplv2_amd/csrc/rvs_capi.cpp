@@ -27,7 +27,10 @@ using namespace rvs;
 namespace {
 
 rvs_status fail(rvs_context* c, rvs_status s, const std::string& msg) {
-    if (c) c->err = msg;
+    if (c) {
+        std::lock_guard<std::mutex> lk(c->err_mu);
+        c->err = msg;
+    }
     return s;
 }
 
@@ -36,8 +39,6 @@ rvs_status hip_fail(rvs_context* c, hipError_t e, const char* what) {
                 std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Workgroup LDS the kernels may ask for.
-constexpr size_t kLdsLimit = 64 * 1024;
 // exact-path batches of at least this many configurations run in the walk order
 constexpr int64_t kExactSortMin = 65536;
 
@@ -45,7 +46,8 @@ constexpr int64_t kExactSortMin = 65536;
 // take 48 more bytes each than axis-aligned ones).
 rvs_status env_fits(rvs_context* c, const EnvDev& env, bool fp64, const char* what) {
     const size_t b = fp64 ? staged_bytes<double>(env) : staged_bytes<float>(env);
-    if (b > kLdsLimit) return fail(c, RVS_ERANGE, std::string(what) + ": obstacle set too large for LDS staging");
+    if (b > device_lds_limit(c->opt.device))
+        return fail(c, RVS_ERANGE, std::string(what) + ": obstacle set too large for LDS staging");
     return RVS_OK;
 }
 
@@ -275,7 +277,14 @@ void rvs_destroy(rvs_context* c) {
     delete c;
 }
 
-const char* rvs_last_error(const rvs_context* c) { return c ? c->err.c_str() : "null context"; }
+const char* rvs_last_error(const rvs_context* c) {
+    if (!c) return "null context";
+    // a copy per calling thread: valid until this thread's next call
+    thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(c->err_mu);
+    copy = c->err;
+    return copy.c_str();
+}
 void* rvs_stream(rvs_context* c) { return c ? (void*)c->stream : nullptr; }
 int64_t rvs_last_refined(const rvs_context* c) { return c ? c->last_refined : 0; }
 
@@ -969,6 +978,12 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
             cand.push_back(FastTuning{1, b, occ, cb});
             if (occ > 1) cand.push_back(FastTuning{1, b, (occ + 1) / 2, cb});
         }
+        for (int b : {256, 512, 1024}) {   // per-lane walks, when the scene's tables fit LDS
+            const int occ = fast_lanes_blocks_per_cu(ev.dev, b);
+            if (occ <= 0) continue;
+            cand.push_back(FastTuning{3, b, occ, cb});
+            if (occ > 1) cand.push_back(FastTuning{3, b, (occ + 1) / 2, cb});
+        }
     }
     FastTuning win{};
     float win_ms = 1e30f;
@@ -1012,10 +1027,13 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
 
 rvs_status rvs_set_tuning(rvs_context* c, int32_t robot, int32_t env, const rvs_tuning* t) {
     if (!c || !t) return fail(c, RVS_EINVAL, "rvs_set_tuning: null argument");
-    if (t->variant < 0 || t->variant > 2 || (t->block != 64 && t->block != 128 && t->block != 256) ||
-        t->blocks_per_cu < 0 || t->coherence_bits < 0 || t->coherence_bits > kCohKeyBits)
+    const bool block_ok = t->variant == 3 ? (t->block == 256 || t->block == 512 || t->block == 1024)
+                                          : (t->block == 64 || t->block == 128 || t->block == 256);
+    if (t->variant < 0 || t->variant > 3 || !block_ok || t->blocks_per_cu < 0 || t->coherence_bits < 0 ||
+        t->coherence_bits > kCohKeyBits)
         return fail(c, RVS_EINVAL,
-                    "rvs_set_tuning: variant 0/1/2, block 64/128/256, blocks_per_cu >= 0, coherence_bits 0..18");
+                    "rvs_set_tuning: variant 0/1/2/3, block 64/128/256 (variant 3: 256/512/1024), "
+                    "blocks_per_cu >= 0, coherence_bits 0..18");
     std::lock_guard<std::mutex> lk(c->tuning_mu);
     c->tuning[{robot, env}] = FastTuning{t->variant, t->block, t->blocks_per_cu, t->coherence_bits};
     return RVS_OK;
@@ -1261,7 +1279,8 @@ rvs_status rvs_trajectory_is_safe(rvs_context* c, int32_t robot, int32_t env, co
     const EnvRec& ev = c->envs[env];
     if (rb.num_links != rb.num_dofs)
         return fail(c, RVS_ENOTSUP, "rvs_trajectory_is_safe: needs one link per revolute joint");
-    if (traj_lds_bytes(rb.dev) > kLdsLimit) return fail(c, RVS_ERANGE, "rvs_trajectory_is_safe: robot too long");
+    if (traj_lds_bytes(rb.dev) > device_lds_limit(c->opt.device))
+        return fail(c, RVS_ERANGE, "rvs_trajectory_is_safe: robot too long");
     if (num_traj > INT32_MAX) return fail(c, RVS_ERANGE, "rvs_trajectory_is_safe: too many trajectories");
     if (num_traj == 0) return RVS_OK;
     (void)hipSetDevice(c->opt.device);
@@ -1625,7 +1644,7 @@ rvs_status rvs_edge_is_valid(rvs_context* c, int32_t robot, int32_t env, const f
     if (n == 0) return RVS_OK;
     if (c->opt.num_checks <= 0) return fail(c, RVS_EINVAL, "rvs_edge_is_valid: num_checks must be positive");
     const int D = rb.num_dofs;
-    if (((staged_bytes<float>(ev.dev) + 15) & ~size_t(15)) + (size_t)D * 256 * 4 > kLdsLimit)
+    if (((staged_bytes<float>(ev.dev) + 15) & ~size_t(15)) + (size_t)D * 256 * 4 > device_lds_limit(c->opt.device))
         return fail(c, RVS_ERANGE, "rvs_edge_is_valid: obstacle set too large for LDS staging");
     (void)hipSetDevice(c->opt.device);
     const QueryParams qp = params_of(c);
@@ -1755,7 +1774,7 @@ rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* 
     rvs_status s = plan_edges(c, robots, K, num_edges, row_stride, pl);
     if (s != RVS_OK) return s;
     const size_t lds = edge_first_lds_bytes(pl.gr, ev.dev);
-    if (lds > kLdsLimit) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: robots too large for LDS staging");
+    if (lds > device_lds_limit(c->opt.device)) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: robots too large for LDS staging");
     (void)hipSetDevice(c->opt.device);
     const size_t rows = (size_t)num_edges * row_stride;
     hipError_t e = c->arena.ensure(Carve::need({pl.order.size() * 4, pl.rounds.size() * 4, 256,
@@ -1817,7 +1836,7 @@ rvs_status rvs_autotune_edges(rvs_context* c, int32_t env, const int32_t* robots
     EdgePlan pl;
     rvs_status s = plan_edges(c, robots, K, num_edges, row_stride, pl);
     if (s != RVS_OK) return s;
-    if (edge_first_lds_bytes(pl.gr, ev.dev) > kLdsLimit)
+    if (edge_first_lds_bytes(pl.gr, ev.dev) > device_lds_limit(c->opt.device))
         return fail(c, RVS_ERANGE, "rvs_autotune_edges: robots too large for LDS staging");
     (void)hipSetDevice(c->opt.device);
     const size_t rows = (size_t)num_edges * row_stride;

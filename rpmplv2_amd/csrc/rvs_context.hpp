@@ -62,6 +62,9 @@ struct rvs_context {
     rvs_options opt{};
     hipStream_t stream = nullptr;
     std::mutex mu;
+    // the last failure's message; err_mu guards it alone, since argument
+    // checks fail before mu is taken
+    mutable std::mutex err_mu;
     std::string err;
     std::vector<rvs::RobotRec> robots;
     std::vector<rvs::EnvRec> envs;

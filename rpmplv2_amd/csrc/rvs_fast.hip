@@ -26,6 +26,8 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "rvs_fk.hpp"
 #include "rvs_geometry.hpp"
@@ -141,7 +143,12 @@ __device__ __attribute__((always_inline)) inline float fast_pair(const Staged<fl
     if (info.inside) uncertain |= info.depth < band;
     else if (info.hit) uncertain |= info.hit_margin < band;
     else uncertain |= (info.raw < p.eq + band) || (fabsf(dt) < band) || (info.sin2 < 1e-6f && info.raw < 0.01f) ||
-                      (info.sin2 < -0.5f);
+                      (info.sin2 < -0.5f) || (dt != dt);
+    // dt is NaN when an end point's projection lies exactly on a face plane
+    // with the other end point inside that face's span (the reference then
+    // reads nearest points it never set); the minimum drops NaN, so the pair
+    // would vanish unseen, and the reference arithmetic, one rounding away,
+    // may land on a crossing instead: the exact pass decides it
     if (tol < p.maxd && fabsf(dt - tol) < band) uncertain = true;
     if (dt > tol) dt = p.maxd;
     return dt;
@@ -472,6 +479,189 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
 }
 
+// The grouped walk with every lane on its own cursor.  In the kernels above
+// a wave walks the obstacles together and evaluates obstacle j whenever one
+// of its lanes needs it, so a wave pays for the union of its lanes' pairs and
+// runs as long as its longest configuration.  Here each lane runs the walk
+// of link_min<OBB, true> as a state machine over its own configurations
+// (taken from a global counter as lanes free up) and the wave advances all
+// lanes together one pair at a time:
+//   1. a lane at the start of a link computes the link's end point (FK);
+//   2. a lane without a pair scans its own walk position, at most kLaneScan
+//      broad-phase tests per round (group boxes first, as the walk does),
+//      until it finds its next pair or reaches the end of the link's walk;
+//   3. every lane that found a pair evaluates it in the same step (each its
+//      own obstacle, read from LDS);
+//   4. each lane folds the result into its link as link_min does.
+// Per configuration the pairs, their order, every screening flag and every
+// result are those of link_min, so all launch shapes agree bit for bit.
+// Phases of a lane's link: the warm pair, the walk, and the search for the
+// first hit in obstacle order before a hit found out of order.
+constexpr int kLaneScan = 8;
+enum : int { kPhNewLink = 0, kPhWarm = 1, kPhWalk = 2, kPhSearch = 3 };
+
+template <bool OBB>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_lanes(
+    RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
+    uint32_t* count, uint32_t* next, const int32_t* __restrict__ perm, size_t walk_off) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<float> s = stage_env<float, OBB>(env, smem);
+    const WalkTables wt = stage_walk(env, smem + walk_off);
+    const int M = env.num_obj;
+    const int L = rb.num_links;
+    const int D = rb.num_dofs;
+    const Params<float> p{qp.eq, qp.maxd};
+    const float band = qp.band;
+    const int lane = __lane_id();
+
+    int64_t c = -1;          // configuration of this lane, -1 = none
+    bool drained = false;    // the counter has passed n
+    const float* qc = q;
+    FKf fk;
+    Vec3<float> P0{0.0f, 0.0f, 0.0f}, P1{0.0f, 0.0f, 0.0f};
+    float r = 0.0f, prof = 0.0f, d = 0.0f, vh = 0.0f;
+    int i = 0, warm = -1, best = -1, jh = -1, cur = 0, phase = kPhNewLink;
+    bool uncertain = false;
+    for (;;) {
+        // refill the lanes without a configuration
+        const bool need = c < 0 && !drained;
+        const uint64_t nm = __ballot(need);
+        if (nm) {
+            const int leader = __ffsll((unsigned long long)nm) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(next, (uint32_t)__popcll(nm));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const uint64_t below = (lane == 0) ? 0ull : (nm & (~0ull >> (64 - lane)));
+                const int64_t cand = (int64_t)base + __popcll(below);
+                if (cand < n) {
+                    c = perm ? (int64_t)perm[cand] : cand;
+                    qc = q + c * D;
+                    fk.M = Mat3f{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+                    fk.px = fk.py = fk.pz = 0.0f;
+                    fk.seg = 0;
+                    fk.qi = 0;
+                    P0 = skelf_point(fk, rb, qc, 0);
+                    i = 0;
+                    warm = -1;
+                    d = p.maxd;
+                    uncertain = false;
+                    phase = kPhNewLink;
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        const bool busy = c >= 0;
+        if (__ballot(busy) == 0) break;
+        // 1. the link's end point
+        if (busy && phase == kPhNewLink) {
+            P1 = skelf_point(fk, rb, qc, i + 1);
+            if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
+            r = rb.radius[i];
+            prof = p.maxd;
+            best = -1;
+            jh = -1;
+            cur = 0;
+            phase = warm >= 0 ? kPhWarm : kPhWalk;
+        }
+        Vec3<float> slo, shi;
+        seg_bounds(P0, P1, slo, shi);
+        // 2. this lane's next pair
+        int task = -1;
+        bool link_done = false;
+        if (busy) {
+            if (phase == kPhWarm) {
+                task = warm;
+            } else if (phase == kPhWalk) {
+                const float b = cull_bound(prof, r, p.eq);
+                for (int k = 0; k < kLaneScan && cur < M; ++k) {
+                    if ((cur & (kGroupSize - 1)) == 0 && cull_pair(wt.gcull + 6 * (cur / kGroupSize), slo, shi, b)) {
+                        cur += kGroupSize;
+                        continue;
+                    }
+                    const int j = wt.gorder[cur++];
+                    if (j != warm && !cull_pair(wt.cull + 6 * j, slo, shi, b)) {
+                        task = j;
+                        break;
+                    }
+                }
+                if (task < 0 && cur >= M) link_done = true;
+            } else if (phase == kPhSearch) {
+                const float b0 = cull_bound(0.0f, r, p.eq);
+                for (int k = 0; k < kLaneScan && cur < jh; ++k) {
+                    const int j = cur++;
+                    if (!cull_pair(wt.cull + 6 * j, slo, shi, b0)) {
+                        task = j;
+                        break;
+                    }
+                }
+                if (task < 0 && cur >= jh) link_done = true;   // no earlier hit: the value stays vh
+            }
+        }
+        // 3. the pairs of every lane that found one, in one step
+        const bool gnd_skip = i < rb.ground_included;
+        float dt = 0.0f;
+        if (task >= 0) dt = (s.gnd[task] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, task, P0, P1, r, p, band, uncertain);
+        // 4. fold the result into the link
+        if (task >= 0) {
+            if (phase == kPhWarm) {
+                if (dt <= 0) {
+                    jh = warm;
+                    vh = dt;
+                    phase = kPhSearch;
+                } else {
+                    if (dt < prof) {
+                        prof = dt;
+                        best = warm;
+                    }
+                    phase = kPhWalk;
+                }
+                cur = 0;
+            } else if (phase == kPhWalk) {
+                if (dt < prof) best = task;
+                prof = smin(prof, dt);
+                if (prof <= 0) {
+                    jh = task;
+                    vh = prof;
+                    phase = kPhSearch;
+                    cur = 0;
+                }
+            } else if (dt <= 0) {   // the first hit in obstacle order
+                vh = dt;
+                link_done = true;
+            }
+        }
+        bool finished = false;
+        if (link_done) {
+            const bool hit = phase == kPhSearch;
+            const float lp = hit ? vh : prof;
+            if (out.profile) out.profile[c * L + i] = lp;
+            if (hit) {
+                for (int k = i + 1; k < L; ++k)
+                    if (out.profile) out.profile[c * L + k] = 0.0f;
+                finished = true;
+            } else {
+                d = smin(d, lp);
+                warm = best;
+                if (i == L - 1) {
+                    finished = true;
+                } else {
+                    ++i;
+                    P0 = P1;
+                    phase = kPhNewLink;
+                }
+            }
+            if (finished) {
+                if (out.d) out.d[c] = hit ? 0.0f : d;
+                if (out.verdict) out.verdict[c] = hit ? 0 : 1;
+            }
+        }
+        append_flagged(finished && uncertain, c, idx, count);
+        if (finished) c = -1;
+    }
+}
+
 __global__ void k_zero_u32(uint32_t* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -668,6 +858,46 @@ int fast_pool_blocks_per_cu(const EnvDev& env, int block) {
     return e == hipSuccess ? nb : 0;
 }
 
+size_t device_lds_limit(int dev) {
+    static std::mutex mu;
+    static std::map<int, size_t> cache;
+    if (dev < 0 && hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    int v = 0;
+    const size_t lim = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0
+                           ? (size_t)v
+                           : (size_t)64 * 1024;
+    cache[dev] = lim;
+    return lim;
+}
+
+namespace {
+
+// LDS of the per-lane-walk kernel: the staged obstacles, then (16-byte
+// aligned) the broad-phase tables; 0 when it cannot run on this obstacle set.
+size_t lanes_lds(const EnvDev& env, size_t* walk_off) {
+    if (!env.gorder || !env.gcull) return 0;
+    const size_t off = (staged_bytes<float>(env) + 15) & ~size_t(15);
+    const size_t total = off + walk_tables_bytes(env.num_obj);
+    if (total > device_lds_limit()) return 0;
+    if (walk_off) *walk_off = off;
+    return total;
+}
+
+}  // namespace
+
+int fast_lanes_blocks_per_cu(const EnvDev& env, int block) {
+    const size_t sm = lanes_lds(env, nullptr);
+    if (!sm) return 0;
+    int nb = 0;
+    const hipError_t e =
+        env.use_obb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_lanes<true>, block, sm)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_lanes<false>, block, sm);
+    return e == hipSuccess ? nb : 0;
+}
+
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_u32, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
     return hipGetLastError();
@@ -705,9 +935,24 @@ template <bool OBB, bool GROUPED>
 hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
                              const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
                              const int32_t* perm, int num_cus, hipStream_t st) {
-    const int B = (t.block == 64 || t.block == 128) ? t.block : kBlock;
     const size_t sm = staged_bytes<float>(env);
-    if (t.variant == 1) {
+    if (t.variant == 3 && GROUPED) {
+        // persistent per-lane walks; count[32] is the configuration counter
+        size_t walk_off = 0;
+        const size_t lsm = lanes_lds(env, &walk_off);
+        if (lsm) {
+            const int BL = (t.block == 512 || t.block == 1024) ? t.block : 256;
+            int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : fast_lanes_blocks_per_cu(env, BL);
+            per_cu = std::max(1, per_cu);
+            const int64_t blocks = std::min<int64_t>((int64_t)num_cus * per_cu, (n + BL - 1) / BL);
+            const dim3 g((unsigned)std::max<int64_t>(1, blocks)), b(BL);
+            hipLaunchKernelGGL((k_distance_fast_lanes<OBB>), g, b, lsm, st, rb, env, qp, q, n, out, idx, count,
+                               count + 32, perm, walk_off);
+            return hipGetLastError();
+        }
+    }
+    const int B = (t.block == 64 || t.block == 128) ? t.block : kBlock;
+    if (t.variant == 1 || t.variant == 3) {
         // persistent lane-refill variant; count[32] is its configuration counter
         int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : fast_pool_blocks_per_cu(env, B);
         per_cu = std::max(1, per_cu);
@@ -720,7 +965,7 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
     // rows staged through LDS when a walk order scatters them and the slices fit
     const size_t rows_off = (sm + 15) & ~size_t(15);
     const size_t rows_sm = rows_off + (size_t)(rb.num_dofs + rb.num_links) * 4 * B;
-    const bool rows = perm && rows_sm <= 64 * 1024;
+    const bool rows = perm && rows_sm <= device_lds_limit();
     if (t.variant == 2) {
         // persistent waves over 64-configuration chunks; count[32] is the chunk counter
         const size_t smc = rows ? rows_sm : sm;

@@ -96,8 +96,9 @@ hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const Quer
 // Launch shape of the FP32 bulk pass (chosen by the autotuner).
 struct FastTuning {
     int32_t variant;        // 0: one lane per configuration; 1: persistent lane refill;
-                            // 2: one lane per configuration in persistent waves over 64-configuration chunks
-    int32_t block;          // threads per workgroup: 64, 128 or 256
+                            // 2: one lane per configuration in persistent waves over 64-configuration chunks;
+                            // 3: persistent lane refill with every lane walking its own pairs
+    int32_t block;          // threads per workgroup: 64, 128 or 256 (variant 3: 256, 512 or 1024)
     int32_t blocks_per_cu;  // lane refill: resident workgroups per CU, 0 = the occupancy limit
     int32_t coh_bits;       // walk order: 2^bits bins per joint angle of the first 18 / bits joints, 0 = off
 };
@@ -148,14 +149,23 @@ inline CohLayout coh_layout(uint32_t* base, int64_t n) {
 // otherwise *perm is null (the order given).
 hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n, const FastTuning& t, uint32_t* coh,
                                   const int32_t** perm, hipStream_t st);
-// The lane-refill variant uses count[32] as its configuration counter
-// (expected zero, like count[0]).  perm: the walk order (may be null).
+// The persistent variants (1: lane refill, 2: chunks, 3: per-lane walks)
+// use count[32] as their work counter (expected zero, like count[0]).
+// perm: the walk order (may be null).  Variant 3 needs the scene's group
+// tables and its broad-phase tables in LDS; without them it runs as
+// variant 1.
 hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                 int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
                                 const int32_t* perm,
                                 int num_cus, hipStream_t st);
 // Resident workgroups per CU of the lane-refill kernel for this obstacle set.
 int fast_pool_blocks_per_cu(const EnvDev& env, int block);
+// The same for the per-lane-walk kernel (0 when it does not run on this
+// obstacle set: no group tables, or its tables exceed the LDS limit).
+int fast_lanes_blocks_per_cu(const EnvDev& env, int block);
+// Workgroup LDS of device dev, or of the current device when dev < 0
+// (hipDeviceAttributeMaxSharedMemoryPerBlock, read once per device).
+size_t device_lds_limit(int dev = -1);
 hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                   int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
                                   hipStream_t st);

@@ -45,7 +45,6 @@ namespace {
 using Clock = std::chrono::steady_clock;
 constexpr size_t kGranule = 256;
 constexpr int kMotionSlots = 4;
-constexpr size_t kLds = 64 * 1024;
 // Requests of at most this many configurations travel through the server's
 // pinned staging memory: one copy per launch each way instead of one per
 // request and array (large requests are copied directly).
@@ -750,7 +749,8 @@ rvs_status rvs_server_submit(rvs_server* s, const rvs_request* req, uint64_t* ti
         r->dofs = c->robots[req->robot].num_dofs;
         r->links = c->robots[req->robot].num_links;
     }
-    if (staged_bytes<float>(r->env) > kLds) return sfail(s, RVS_ERANGE, "rvs_server_submit: obstacle set too large");
+    if (staged_bytes<float>(r->env) > device_lds_limit(s->ctx->opt.device))
+        return sfail(s, RVS_ERANGE, "rvs_server_submit: obstacle set too large");
     if (req->n > s->opt.max_batch_configs || req->n > INT32_MAX)
         return sfail(s, RVS_ERANGE, "rvs_server_submit: request larger than max_batch_configs");
     Layout lay;
@@ -813,8 +813,11 @@ double rvs_server_now(rvs_server* s) { return s ? s->now() : 0.0; }
 
 const char* rvs_server_last_error(rvs_server* s) {
     if (!s) return "null server";
+    // a copy per calling thread: the server thread may replace the message
+    thread_local std::string copy;
     std::lock_guard<std::mutex> lk(s->mu);
-    return s->err.c_str();
+    copy = s->err;
+    return copy.c_str();
 }
 
 void* rvs_host_alloc(int64_t bytes) {
