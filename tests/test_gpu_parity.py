@@ -193,6 +193,94 @@ def test_fp32_refine_small_scenes_and_nan(oracle, engine, boxes):
     assert np.abs(d[m] - od[m]).max() <= 1e-5
 
 
+def test_fp32_screen_sends_nan_pairs_to_the_exact_pass(oracle, engine):
+    """A link lying in the plane of a box face, with one end point projecting
+    onto the opposite face and the other end point inside that face's span,
+    makes the case tree return NaN (the reference reads nearest points it
+    never set, CollisionAndDistance.cpp:476-574): the pair drops out of the
+    minimum.  One rounding away the segment crosses the face and the pair is
+    a hit.  Planar 2-DOF at q0 = 0 lies on y = 0, the box's lo.y face: the
+    FP32 bulk pass must flag every such configuration for the exact pass,
+    and the verdicts must be the reference arithmetic's on both sides of the
+    discontinuity."""
+    sc = W.Scene([[0.6, 0.5, 0.2]], [[0.5, 0.25, 0.0]], [[1, 0, 0, 0]], [0], [W.MAX_DISTANCE])
+    assert sc.aabb[0, 1] == 0.0
+    rb = R.planar_2dof(radius=0.02)
+    osc = oracle.scene_of(sc)
+    rid = engine.load_robot(rb)
+    eid = engine.create_env(sc)
+    # q0 = 0 exactly, any q1: link 0 takes the NaN path (the oracle drops it)
+    q_on = np.zeros((256, 2), np.float32)
+    q_on[:, 1] = np.linspace(-3.0, 3.0, 256, dtype=np.float32)
+    od, oprof, over, _ = oracle.distance(rb, osc, q_on, 32)
+    assert (oprof[:, 0] == W.MAX_DISTANCE).all()          # the NaN pair left link 0 at MAX_DISTANCE
+    d, prof, ver, _ = engine.distance(rid, eid, q_on, precision=rvs.PREC_FP32_REFINE)
+    refined = engine.last_refined()
+    print(f"on the face plane: refined {refined} of {len(q_on)}, free {ver.mean():.3f}")
+    assert refined == len(q_on)
+    assert (ver == over).all() and ulps(d, od).max() <= 2 and ulps(prof, oprof).max() <= 2
+    # one configuration alone
+    d1, _, v1, _ = engine.distance(rid, eid, q_on[:1], precision=rvs.PREC_FP32_REFINE)
+    assert engine.last_refined() == 1 and v1[0] == over[0] and ulps(d1, od[:1]).max() <= 2
+    # across the discontinuity: q0 a few float steps either side of 0
+    k = np.arange(-2000, 2001, dtype=np.float64)
+    q0 = np.concatenate([k * 1e-10, k * 1e-8, np.float32(np.pi) * np.ones(1)]).astype(np.float32)
+    q = np.stack([q0, np.tile(np.float32([0.0, 0.7, -1.3]), len(q0) // 3 + 1)[:len(q0)]], 1)
+    od, oprof, over, _ = oracle.distance(rb, osc, q, 32)
+    d, prof, ver, _ = engine.distance(rid, eid, q, precision=rvs.PREC_FP32_REFINE)
+    print(f"across it: refined {engine.last_refined()} of {len(q)}, free {ver.mean():.3f}, "
+          f"oracle NaN pairs {(oprof[:, 0] == W.MAX_DISTANCE).sum()}")
+    assert (ver == over).all()
+    assert np.abs(d - od).max() <= 1e-5 and np.abs(prof - oprof).max() <= 1e-5
+    assert 0 < over.mean() < 1
+
+
+# A planar arm's first link crossing the face x = 0.5 of a box where one of
+# the quirked corner tests of collisionCapsuleToRectangle
+# (CollisionAndDistance.cpp:69, :71, :72) reports a hit although the capsule
+# is 0.052-0.054 m from the box (r = 0.05); q1 turns the second link away.
+# (box, q0, q1)
+QUIRK_SCENES = [
+    ((0.5, -1.0, 0.03, 1.0, 0.01, 1.0), float(np.arctan(0.11)), 1.5708),
+    ((0.5, -0.1, 0.048, 1.0, 0.005, 1.0), float(np.arctan(-0.24)), -1.5708),
+    ((0.5, -1.0, -0.035, 1.5, -0.045, -0.03), 0.0, 1.5708),
+]
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_collision_tree_corner_quirk(oracle, engine, case):
+    """isValid(q) on the quirked corner tests: the device reports the
+    collision the reference's tree reports (FP32 and FP64 trees), and the
+    distance paths report the positive distance the reference's
+    computeDistance returns for the same configuration; around it, q0 swept
+    across the edge of the quirk's region."""
+    box, q0, q1 = QUIRK_SCENES[case]
+    lo, hi = np.asarray(box[:3]), np.asarray(box[3:])
+    sc = W.Scene([hi - lo], [(hi + lo) / 2], [[1, 0, 0, 0]], [0], [W.MAX_DISTANCE])
+    assert np.abs(sc.aabb[0] - np.asarray(box)).max() < 1e-6
+    rb = R.planar_2dof(radius=0.05)
+    osc = oracle.scene_of(sc)
+    q = np.stack([np.float32(q0) + np.linspace(-0.02, 0.02, 4001, dtype=np.float32),
+                  np.full(4001, q1, np.float32)], 1).astype(np.float32)
+    q[2000] = (q0, q1)
+    rid = engine.load_robot(rb)
+    eid = engine.create_env(sc)
+    assert (engine.env_bounds(eid) == sc.aabb).all()
+    ov = oracle.is_valid(rb, osc, q)
+    ov64 = oracle.is_valid(rb, osc, q, 64)
+    od, oprof, over, _ = oracle.distance(rb, osc, q, 32)
+    assert ov[2000] == 0 and over[2000] == 1             # the tree's quirk hit; computeDistance: free
+    v = engine.is_valid(rid, eid, q)
+    v64 = engine.is_valid(rid, eid, q, precision=rvs.PREC_FP64)
+    print(f"case {case}: tree collisions {int((ov == 0).sum())}, distance collisions {int((over == 0).sum())}, "
+          f"tree hits at positive distance {int(((ov == 0) & (over == 1)).sum())}")
+    assert (v == ov).all() and (v64 == ov64).all()
+    d, prof, ver, _ = engine.distance(rid, eid, q)
+    assert (ver == over).all() and ulps(d, od).max() <= 2
+    d3, _, ver3, _ = engine.distance(rid, eid, q, precision=rvs.PREC_FP32_REFINE)
+    assert (ver3 == over).all() and np.abs(d3 - od).max() <= 1e-5
+
+
 def test_nan_configurations(oracle, engine):
     w = W.workload2(256)
     q = w.q.copy()
@@ -235,6 +323,26 @@ def test_edge_first_collision_workload3(oracle, engine):
         assert np.array_equal(f, of)
 
 
+def test_edge_first_collision_planner_workload(oracle, engine):
+    """Workload 3's four chains (10/14/18/22 DOF, self-collision on) with
+    edges that start at collision-free states, as a planner's tree nodes
+    are, in one persistent launch: the first colliding state of every edge
+    equals the oracle's, for each (waves per CU, slots per lane) shape; the
+    edges walk far past state 0 and many run to their last state."""
+    w = W.workload3_planner(1024, oracle=oracle)
+    eid = engine.create_env(w.scene)
+    sc = oracle.scene_of(w.scene)
+    expect = [oracle.edge_first_collision(b.robot, sc, b.q1, b.dir, b.K, b.step) for b in w.batches]
+    allf = np.concatenate(expect)
+    print(f"free {(allf < 0).mean():.3f}, median first collision {np.median(allf[allf >= 0])}")
+    assert (allf != 0).all() and 0.1 < (allf < 0).mean() < 0.6 and np.median(allf[allf >= 0]) >= 5
+    for wpc, spl in ((4, 2), (16, 1), (32, 4)):
+        engine.set_edge_tuning(wpc, spl)
+        got = engine.edge_batches(eid, w.batches)
+        for b, g, e in zip(w.batches, got, expect):
+            assert np.array_equal(g, e), (b.robot.name, wpc, spl, int((g != e).sum()))
+
+
 def test_edge_first_collision_sparse_scene(oracle, engine):
     # few, small obstacles so many edges run to their last state
     rng = np.random.default_rng(11)
@@ -266,21 +374,22 @@ def _engine_with_variant(v):
 
 @pytest.fixture(scope="module")
 def engines_by_variant():
-    """Engines whose FP32 bulk pass is the lane-refill ("pool") or the
-    one-lane-per-configuration ("classic") kernel."""
-    e = {"pool": _engine_with_variant("1"), "classic": _engine_with_variant("0")}
+    """Engines whose FP32 bulk pass is the lane-refill ("pool"), the
+    one-lane-per-configuration ("classic") or the per-lane-walk ("lanes")
+    kernel."""
+    e = {"pool": _engine_with_variant("1"), "classic": _engine_with_variant("0"), "lanes": _engine_with_variant("3")}
     yield e
     for x in e.values():
         x.close()
 
 
-@pytest.mark.parametrize("variant", ["pool", "classic"])
+@pytest.mark.parametrize("variant", ["pool", "classic", "lanes"])
 @pytest.mark.parametrize("wl", ["workload1", "workload2"])
 def test_fp32_refine_verdict_exact(oracle, engines_by_variant, wl, variant):
     """FP32 bulk pass + exact refinement: verdicts identical to the reference
     arithmetic, distances within 1e-5 m of the FP64 oracle (and profiles of
     collision-free configurations too), for the lane-refill and the
-    one-lane-per-configuration bulk kernels."""
+    one-lane-per-configuration and the per-lane-walk bulk kernels."""
     engine = engines_by_variant[variant]
     w = W.workload1(65536) if wl == "workload1" else W.workload2(1 << 18)
     rid = engine.load_robot(w.robot)
@@ -341,9 +450,8 @@ def _spine_inputs(which, n, seed):
 @pytest.mark.parametrize("which", ["xarm6", "chain22", "planar2"])
 @pytest.mark.parametrize("num_iter,expanded", [(1, True), (5, True), (5, False)])
 def test_extend_spines(oracle, engine, which, num_iter, expanded):
-    """RBTConnect::extendSpine batch against the oracle: statuses identical,
-    end states identical bit for bit except where the device and host double
-    sine / cosine differ in the last bit (then within 1e-5 rad)."""
+    """RBTConnect::extendSpine batch against the oracle: statuses and end
+    states identical bit for bit on every spine."""
     rb, q, d, prof, q_e = _spine_inputs(which, 8192, 41)
     rid = engine.load_robot(rb)
     qn, st = engine.extend_spines(rid, q, d, prof, q_e, num_iter=num_iter, expanded=expanded)
@@ -352,10 +460,8 @@ def test_extend_spines(oracle, engine, which, num_iter, expanded):
     print(f"{which} iter={num_iter} expanded={expanded}: status {np.bincount(st, minlength=4)[1:]}, "
           f"status mismatches {(st != ost).sum()}, states identical {same.mean():.5f}, "
           f"max diff {np.abs(qn - oqn).max():.3g}")
-    assert (st == ost).mean() >= 0.999
-    assert same.mean() >= 0.99
-    ok = st == ost
-    assert np.abs(qn[ok] - oqn[ok]).max() <= 1e-5
+    bad = np.nonzero((st != ost) | ~same)[0]
+    assert len(bad) == 0, f"spines differing from the oracle: {bad[:16].tolist()}"
     assert (st == 2).any() and (st == 1).any()
 
 
@@ -380,10 +486,8 @@ def test_extend_gen_spines(oracle, engine, use_real_dist, num_iter):
     same = (qn == oqn).all(1) & (dd == odd)
     print(f"real={use_real_dist} iter={num_iter}: status {np.bincount(st, minlength=4)[1:]}, "
           f"status mismatches {(st != ost).sum()}, identical {same.mean():.5f}")
-    assert (st == ost).mean() >= 0.999
-    assert same.mean() >= 0.99
-    ok = st == ost
-    assert np.abs(qn[ok] - oqn[ok]).max() <= 1e-4
+    bad = np.nonzero((st != ost) | ~same)[0]
+    assert len(bad) == 0, f"spines differing from the oracle: {bad[:16].tolist()}"
     # the root in contact: every target trapped at the root with d = 0
     root_bad = w.q[np.nonzero(ver == 0)[0][0]]
     qn, st, dd = engine.extend_gen_spines(rid, eid, root_bad, q_e[:64])
@@ -407,8 +511,8 @@ def test_edge_self_collision(oracle, engine, eps_step):
     same = (out == oout).all(1)
     print(f"eps {eps_step}: collide {col.sum()} of {len(col)}, verdict mismatches {(col != ocol).sum()}, "
           f"states identical {same.mean():.5f}")
-    assert (col == ocol).mean() >= 0.999
-    assert same.mean() >= 0.995
+    bad = np.nonzero((col != ocol) | ~same)[0]
+    assert len(bad) == 0, f"edges differing from the oracle: {bad[:16].tolist()}"
     assert col.sum() > 100
 
 
@@ -465,7 +569,8 @@ def test_trajectory_is_safe(oracle, engine, v, self_collision):
     osafe = oracle.traj_safe(rb, sc, pts, root, p0[0], np0[0], mv, 0.0, 0.01,
                              pairs=rb.self_pairs if self_collision else None)
     print(f"v={v} self={self_collision}: safe {safe.sum()} of {T}, mismatches {(safe != osafe).sum()}")
-    assert (safe == osafe).mean() >= 0.999
+    bad = np.nonzero(safe != osafe)[0]
+    assert len(bad) == 0, f"trajectories differing from the oracle: {bad[:16].tolist()}"
     assert 0 < safe.sum() < T
 
 

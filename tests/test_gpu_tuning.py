@@ -1,5 +1,5 @@
 """Autotuner of the FP32 bulk pass: every launch shape (one lane per
-configuration, lane refill, persistent chunks) and walk order gives the same
+configuration, lane refill, persistent chunks, per-lane walks) and walk order gives the same
 per-configuration results (they only change which lane evaluates a
 configuration), and rvs_autotune keeps one of the candidate shapes."""
 import numpy as np
@@ -22,8 +22,8 @@ def test_launch_shapes_agree(engine, wl):
     eid = engine.create_env(w.scene, sem)
     ref = None
     for coh in (0, 5, 3):
-        for variant in (0, 1, 2):
-            for block in (64, 128, 256):
+        for variant in (0, 1, 2, 3):
+            for block in ((256, 512, 1024) if variant == 3 else (64, 128, 256)):
                 for per_cu in ((0,) if variant == 0 else (0, 1)):
                     engine.set_tuning(rid, eid, variant, block, per_cu, coh)
                     d, prof, ver, _ = engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
@@ -35,7 +35,7 @@ def test_launch_shapes_agree(engine, wl):
                         assert np.array_equal(d, ref[0]) and np.array_equal(prof, ref[1]), shape
     best = engine.autotune(rid, eid, w.q)
     print(wl, "autotune:", best)
-    assert best["variant"] in (0, 1, 2) and best["block"] in (64, 128, 256) and best["ms"] > 0
+    assert best["variant"] in (0, 1, 2, 3) and best["block"] in (64, 128, 256, 512, 1024) and best["ms"] > 0
     assert engine.get_tuning(rid, eid)["variant"] == best["variant"]
     d, prof, ver, _ = engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
     assert np.array_equal(ver, ref[2]) and np.array_equal(d, ref[0])
