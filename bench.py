@@ -69,9 +69,31 @@ def pair_count(rb, scene) -> int:
     return pairs
 
 
+# Host threads for the CPU baseline: the CPU share of one GPU on the
+# benchmark machines is 16 cores, whatever the process's affinity reports.
+HOST_THREAD_CAP = 16
+
+
+def host_info() -> dict:
+    """The host the CPU baseline runs on: CPU model, logical CPUs, the
+    process's affinity and the threads used."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "affinity": affinity,
+            "threads": max(1, min(affinity, HOST_THREAD_CAP)),
+            "thread_cap": f"{HOST_THREAD_CAP} (the CPU share of one GPU on this machine class)"}
+
+
 def host_cores() -> int:
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    return max(1, min(cores or 1, 16))
+    return host_info()["threads"]
 
 
 def oracle_module():
@@ -100,7 +122,7 @@ def cpu_baseline(w, cores: int):
     cb = {"value": len(q) / t_all, "unit": "configs/s", "cores": cores, "kind": "port",
           "sample": f"all {len(q)} configurations of the timed batch, median of 3 passes on {cores} threads "
                     f"({t_all:.2f} s each); 1-thread rate on the first 65536",
-          "value_1thread": len(q1) / t_one}
+          "value_1thread": len(q1) / t_one, "host": host_info()}
     return cb, res
 
 
