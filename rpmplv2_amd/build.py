@@ -1,6 +1,7 @@
 """Builds librvs.so (the gfx950 engine) in-tree with hipcc.
 
-    python -m rpmplv2_amd.build          # compile rpmplv2_amd/librvs.so
+    python -m rpmplv2_amd.build               # compile rpmplv2_amd/librvs.so
+    python -m rpmplv2_amd.build --sanitized   # and the host-sanitized variant
 
 Compile flags:
   --offload-arch=gfx950                    MI355X only
@@ -66,5 +67,53 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+# Host-code sanitizers (AddressSanitizer and UndefinedBehaviorSanitizer) for
+# the C ABI and the request server: their host code is instrumented, the
+# kernels are the normal objects (device code is not instrumented).  The
+# library leaves the sanitizer runtime to the executables that load it, which
+# link it statically: the C++ host-class test and the serving replay driver.
+SAN_DIR = os.path.join(BUILD_DIR, "san")
+SAN_LIB = os.path.join(SAN_DIR, "librvs_san.so")
+SAN_HOST = "-Xarch_host -fsanitize=address -Xarch_host -fsanitize=undefined".split()
+SAN_EXE_FLAGS = ["-O1", "-g", "-std=c++17", "-fno-omit-frame-pointer", "-fsanitize=address,undefined"]
+CLANGXX = os.environ.get("RVS_CLANGXX", "/opt/rocm/llvm/bin/clang++")
+
+
+def build_sanitized(verbose: bool = False) -> str:
+    """librvs_san.so plus the sanitized host programs
+    (rpmplv2_amd/_build/san/rvs_serve_replay_san,
+    tests/host/_build/space_test_san); needs build() first (kernel objects)."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        if src.endswith(".cpp"):
+            obj = os.path.join(SAN_DIR, os.path.splitext(src)[0] + ".o")
+            cmd = [HIPCC, "-x", "hip"] + COMMON + ["-O1", "-g", "-fno-omit-frame-pointer"] + SAN_HOST + \
+                  ["-c", os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd))
+            _run(cmd)
+        else:
+            obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+    tmp = SAN_LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, SAN_LIB)
+    inc = ["-I", os.path.join(ROOT, "include")]
+    exe = os.path.join(SAN_DIR, "rvs_serve_replay_san")
+    _run([CLANGXX] + SAN_EXE_FLAGS + inc + ["-pthread", os.path.join(CSRC, "serve_replay.cpp"), "-o", exe + ".tmp",
+                                           SAN_LIB, "-Wl,-rpath," + SAN_DIR])
+    os.replace(exe + ".tmp", exe)
+    host_out = os.path.join(ROOT, "tests", "host", "_build")
+    os.makedirs(host_out, exist_ok=True)
+    exe = os.path.join(host_out, "space_test_san")
+    _run([CLANGXX] + SAN_EXE_FLAGS + inc + [os.path.join(ROOT, "tests", "host", "space_test.cpp"), "-o", exe + ".tmp",
+                                           SAN_LIB, "-Wl,-rpath," + SAN_DIR])
+    os.replace(exe + ".tmp", exe)
+    return SAN_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    if "--sanitized" in sys.argv:
+        print(build_sanitized(verbose="-v" in sys.argv))

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 #include "rvs_fk.hpp"
 #include "rvs_geometry.hpp"
@@ -479,34 +480,47 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
 }
 
-// The grouped walk with every lane on its own cursor.  In the kernels above
-// a wave walks the obstacles together and evaluates obstacle j whenever one
-// of its lanes needs it, so a wave pays for the union of its lanes' pairs and
-// runs as long as its longest configuration.  Here each lane runs the walk
-// of link_min<OBB, true> as a state machine over its own configurations
-// (taken from a global counter as lanes free up) and the wave advances all
-// lanes together one pair at a time:
+// The grouped walk with every lane on its own cursor, its broad phase run
+// ahead of its pair evaluations.  In the kernels above a wave walks the
+// obstacles together and evaluates obstacle j whenever one of its lanes
+// needs it, so a wave pays for the union of its lanes' pairs; with many
+// obstacles (512 boxes: about 37 broad-phase tests per evaluated pair) the
+// lanes of a wave then need different pairs at different times and few lanes
+// do useful work per instruction.  Here each lane runs the walk of
+// link_min<OBB, true> on its own configurations (taken from a global counter
+// as lanes free up), in rounds:
 //   1. a lane at the start of a link computes the link's end point (FK);
-//   2. a lane without a pair scans its own walk position, at most kLaneScan
-//      broad-phase tests per round (group boxes first, as the walk does),
-//      until it finds its next pair or reaches the end of the link's walk;
-//   3. every lane that found a pair evaluates it in the same step (each its
-//      own obstacle, read from LDS);
+//   2. scan: every lane with walk left tests its own next entries (group box
+//      first, as the walk does, then members) with its current bound and
+//      queues the members that pass (an LDS ring of kLaneQueue per lane),
+//      until nearly every lane holds a pair (at least kLaneScanMin steps);
+//   3. each lane takes its queue's pairs in walk order, tests each again with
+//      its bound of now, and evaluates the first that passes; all lanes
+//      evaluate their pairs in the same step (each its own obstacle);
 //   4. each lane folds the result into its link as link_min does.
-// Per configuration the pairs, their order, every screening flag and every
-// result are those of link_min, so all launch shapes agree bit for bit.
-// Phases of a lane's link: the warm pair, the walk, and the search for the
-// first hit in obstacle order before a hit found out of order.
-constexpr int kLaneScan = 8;
+// The scan's bound is never tighter than the plain walk's at the same entry
+// (the bound only shrinks), a member's box lies inside its group's box, and
+// every queued pair is tested again with exactly the bound the plain walk
+// has when it reaches it (all pairs before it in walk order have been
+// evaluated or dropped by then).  So per configuration the pairs, their
+// order, every screening flag and every result are those of link_min, and
+// all launch shapes agree bit for bit.  Phases of a lane's link: the warm
+// pair, the walk, and the search for the first hit in obstacle order before
+// a hit found out of order (queued the same way, bound fixed).
+constexpr int kLaneQueue = 8;       // pending pairs per lane (power of 2)
+constexpr int kLaneScanMin = 2;     // broad-phase steps per round at least (every lane with walk left advances)
+constexpr int kLaneScanMax = 48;    // and at most
 enum : int { kPhNewLink = 0, kPhWarm = 1, kPhWalk = 2, kPhSearch = 3 };
 
 template <bool OBB>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_lanes(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
-    uint32_t* count, uint32_t* next, const int32_t* __restrict__ perm, size_t walk_off) {
+    uint32_t* count, uint32_t* next, const int32_t* __restrict__ perm, size_t walk_off, size_t queue_off) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<float> s = stage_env<float, OBB>(env, smem);
     const WalkTables wt = stage_walk(env, smem + walk_off);
+    const int B = blockDim.x;
+    int16_t* const ring = reinterpret_cast<int16_t*>(smem + queue_off) + threadIdx.x;   // slot k: ring[k * B]
     const int M = env.num_obj;
     const int L = rb.num_links;
     const int D = rb.num_dofs;
@@ -521,6 +535,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     Vec3<float> P0{0.0f, 0.0f, 0.0f}, P1{0.0f, 0.0f, 0.0f};
     float r = 0.0f, prof = 0.0f, d = 0.0f, vh = 0.0f;
     int i = 0, warm = -1, best = -1, jh = -1, cur = 0, phase = kPhNewLink;
+    int qh = 0, qn = 0;      // queue head and length
     bool uncertain = false;
     for (;;) {
         // refill the lanes without a configuration
@@ -553,7 +568,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
             }
         }
         const bool busy = c >= 0;
-        if (__ballot(busy) == 0) break;
+        const uint64_t busy_mask = __ballot(busy);
+        if (busy_mask == 0) break;
         // 1. the link's end point
         if (busy && phase == kPhNewLink) {
             P1 = skelf_point(fk, rb, qc, i + 1);
@@ -563,43 +579,54 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
             best = -1;
             jh = -1;
             cur = 0;
+            qh = 0;
+            qn = 0;
             phase = warm >= 0 ? kPhWarm : kPhWalk;
         }
         Vec3<float> slo, shi;
         seg_bounds(P0, P1, slo, shi);
-        // 2. this lane's next pair
-        int task = -1;
-        bool link_done = false;
-        if (busy) {
-            if (phase == kPhWarm) {
-                task = warm;
-            } else if (phase == kPhWalk) {
-                const float b = cull_bound(prof, r, p.eq);
-                for (int k = 0; k < kLaneScan && cur < M; ++k) {
-                    if ((cur & (kGroupSize - 1)) == 0 && cull_pair(wt.gcull + 6 * (cur / kGroupSize), slo, shi, b)) {
+        const bool walk = busy && phase == kPhWalk;
+        const bool search = busy && phase == kPhSearch;
+        const int lim = search ? jh : M;
+        const float bw = cull_bound(prof, r, p.eq);                 // the walk's bound now
+        const float bs = search ? cull_bound(0.0f, r, p.eq) : bw;   // the scan's bound
+        // 2. scan ahead
+        const int need_pair = __popcll(busy_mask);
+        for (int k = 0; k < kLaneScanMax; ++k) {
+            const bool scanning = (walk || search) && cur < lim && qn < kLaneQueue;
+            const bool starving = scanning && qn == 0;
+            if (__ballot(scanning) == 0) break;
+            if (k >= kLaneScanMin && __popcll(__ballot(starving)) * 8 <= need_pair) break;
+            if (scanning) {
+                int j = -1;
+                if (walk) {
+                    if ((cur & (kGroupSize - 1)) == 0 && cull_pair(wt.gcull + 6 * (cur / kGroupSize), slo, shi, bs)) {
                         cur += kGroupSize;
-                        continue;
+                    } else {
+                        const int jj = wt.gorder[cur++];
+                        if (jj != warm && !cull_pair(wt.cull + 6 * jj, slo, shi, bs)) j = jj;
                     }
-                    const int j = wt.gorder[cur++];
-                    if (j != warm && !cull_pair(wt.cull + 6 * j, slo, shi, b)) {
-                        task = j;
-                        break;
-                    }
+                } else {
+                    const int jj = cur++;
+                    if (!cull_pair(wt.cull + 6 * jj, slo, shi, bs)) j = jj;
                 }
-                if (task < 0 && cur >= M) link_done = true;
-            } else if (phase == kPhSearch) {
-                const float b0 = cull_bound(0.0f, r, p.eq);
-                for (int k = 0; k < kLaneScan && cur < jh; ++k) {
-                    const int j = cur++;
-                    if (!cull_pair(wt.cull + 6 * j, slo, shi, b0)) {
-                        task = j;
-                        break;
-                    }
+                if (j >= 0) {
+                    ring[((qh + qn) & (kLaneQueue - 1)) * B] = (int16_t)j;
+                    ++qn;
                 }
-                if (task < 0 && cur >= jh) link_done = true;   // no earlier hit: the value stays vh
             }
         }
-        // 3. the pairs of every lane that found one, in one step
+        // 3. this lane's next pair
+        int task = -1;
+        if (busy && phase == kPhWarm) task = warm;
+        while ((walk || search) && task < 0 && qn > 0) {
+            const int j = ring[(qh & (kLaneQueue - 1)) * B];
+            ++qh;
+            --qn;
+            if (search || !cull_pair(wt.cull + 6 * j, slo, shi, bw)) task = j;
+        }
+        bool link_done = (walk || search) && task < 0 && qn == 0 && cur >= lim;   // search: no earlier hit, vh stays
+        // the pairs of every lane that has one, in one step
         const bool gnd_skip = i < rb.ground_included;
         float dt = 0.0f;
         if (task >= 0) dt = (s.gnd[task] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, task, P0, P1, r, p, band, uncertain);
@@ -626,6 +653,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                     vh = prof;
                     phase = kPhSearch;
                     cur = 0;
+                    qh = 0;
+                    qn = 0;
                 }
             } else if (dt <= 0) {   // the first hit in obstacle order
                 vh = dt;
@@ -875,21 +904,24 @@ size_t device_lds_limit(int dev) {
 
 namespace {
 
-// LDS of the per-lane-walk kernel: the staged obstacles, then (16-byte
-// aligned) the broad-phase tables; 0 when it cannot run on this obstacle set.
-size_t lanes_lds(const EnvDev& env, size_t* walk_off) {
-    if (!env.gorder || !env.gcull) return 0;
-    const size_t off = (staged_bytes<float>(env) + 15) & ~size_t(15);
-    const size_t total = off + walk_tables_bytes(env.num_obj);
+// LDS of the per-lane-walk kernel for workgroups of `block` threads: the
+// staged obstacles, then (16-byte aligned) the broad-phase tables, then the
+// lanes' pair queues; 0 when it cannot run on this obstacle set.
+size_t lanes_lds(const EnvDev& env, int block, size_t* walk_off, size_t* queue_off) {
+    if (!env.gorder || !env.gcull || env.num_obj > 32767) return 0;
+    const size_t woff = (staged_bytes<float>(env) + 15) & ~size_t(15);
+    const size_t qoff = (woff + walk_tables_bytes(env.num_obj) + 15) & ~size_t(15);
+    const size_t total = qoff + (size_t)kLaneQueue * block * sizeof(int16_t);
     if (total > device_lds_limit()) return 0;
-    if (walk_off) *walk_off = off;
+    if (walk_off) *walk_off = woff;
+    if (queue_off) *queue_off = qoff;
     return total;
 }
 
 }  // namespace
 
 int fast_lanes_blocks_per_cu(const EnvDev& env, int block) {
-    const size_t sm = lanes_lds(env, nullptr);
+    const size_t sm = lanes_lds(env, block, nullptr, nullptr);
     if (!sm) return 0;
     int nb = 0;
     const hipError_t e =
@@ -931,6 +963,24 @@ hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n,
 
 namespace {
 
+// Whether a kernel with LDS row slices keeps as many resident workgroups per
+// CU as its form without them (answers cached per kernel, block and sizes).
+template <typename KRows, typename KPlain>
+bool rows_keep_occupancy(KRows krows, KPlain kplain, int block, size_t rows_sm, size_t sm) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, size_t, size_t>, bool> cache;
+    const auto key = std::make_tuple(reinterpret_cast<const void*>(krows), block, rows_sm, sm);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int with = 0, without = 0;
+    const bool ok = hipOccupancyMaxActiveBlocksPerMultiprocessor(&with, krows, block, rows_sm) == hipSuccess &&
+                    hipOccupancyMaxActiveBlocksPerMultiprocessor(&without, kplain, block, sm) == hipSuccess &&
+                    with >= without;
+    cache[key] = ok;
+    return ok;
+}
+
 template <bool OBB, bool GROUPED>
 hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
                              const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
@@ -938,16 +988,16 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
     const size_t sm = staged_bytes<float>(env);
     if (t.variant == 3 && GROUPED) {
         // persistent per-lane walks; count[32] is the configuration counter
-        size_t walk_off = 0;
-        const size_t lsm = lanes_lds(env, &walk_off);
+        const int BL = (t.block == 512 || t.block == 1024) ? t.block : 256;
+        size_t walk_off = 0, queue_off = 0;
+        const size_t lsm = lanes_lds(env, BL, &walk_off, &queue_off);
         if (lsm) {
-            const int BL = (t.block == 512 || t.block == 1024) ? t.block : 256;
             int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : fast_lanes_blocks_per_cu(env, BL);
             per_cu = std::max(1, per_cu);
             const int64_t blocks = std::min<int64_t>((int64_t)num_cus * per_cu, (n + BL - 1) / BL);
             const dim3 g((unsigned)std::max<int64_t>(1, blocks)), b(BL);
             hipLaunchKernelGGL((k_distance_fast_lanes<OBB>), g, b, lsm, st, rb, env, qp, q, n, out, idx, count,
-                               count + 32, perm, walk_off);
+                               count + 32, perm, walk_off, queue_off);
             return hipGetLastError();
         }
     }
@@ -962,10 +1012,16 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
                            count + 32, perm);
         return hipGetLastError();
     }
-    // rows staged through LDS when a walk order scatters them and the slices fit
+    // rows staged through LDS when a walk order scatters them, the slices fit,
+    // and they cost no resident workgroups (on workload-5-sized obstacle sets
+    // the slices would cut the workgroups per CU to one: 3.4x slower)
     const size_t rows_off = (sm + 15) & ~size_t(15);
     const size_t rows_sm = rows_off + (size_t)(rb.num_dofs + rb.num_links) * 4 * B;
-    const bool rows = perm && rows_sm <= device_lds_limit();
+    const bool rows = perm && rows_sm <= device_lds_limit() &&
+                      (t.variant == 2 ? rows_keep_occupancy(k_distance_fast_chunks<OBB, true, GROUPED>,
+                                                            k_distance_fast_chunks<OBB, false, GROUPED>, B, rows_sm, sm)
+                                      : rows_keep_occupancy(k_distance_fast<OBB, true, GROUPED>,
+                                                            k_distance_fast<OBB, false, GROUPED>, B, rows_sm, sm));
     if (t.variant == 2) {
         // persistent waves over 64-configuration chunks; count[32] is the chunk counter
         const size_t smc = rows ? rows_sm : sm;
