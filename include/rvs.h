@@ -205,6 +205,20 @@ rvs_status rvs_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const floa
  * calls); these serve callers that overlap queries with their own GPU work. */
 rvs_status rvs_distance_async(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n, float* d,
                               float* profile, uint8_t* verdict, uint32_t flags, void* stream, void* done);
+/* rvs_distance_async that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it).  Its
+ * scratch (refinement list and counters, walk-order sort) is a fixed device
+ * array of 20 MiB that each GPU the library runs on holds until the process
+ * ends: at most 1048576 configurations per call (RVS_ERANGE beyond).  Same
+ * arguments, results bit-identical to rvs_distance_async.  Every call on one
+ * GPU shares that GPU's array, whatever its context: calls that overlap on
+ * one GPU (two contexts, or a call enqueued on a stream that runs beside
+ * another such call) race on it; calls on one context are ordered like
+ * every asynchronous call. */
+rvs_status rvs_distance_async_static(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n,
+                                     float* d, float* profile, uint8_t* verdict, uint32_t flags, void* stream,
+                                     void* done);
 rvs_status rvs_is_valid_async(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n,
                               uint8_t* valid, uint32_t flags, void* stream, void* done);
 /* The same stream-ordered form of rvs_distance_underestimation and of

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
@@ -41,6 +42,38 @@ rvs_status hip_fail(rvs_context* c, hipError_t e, const char* what) {
 
 // exact-path batches of at least this many configurations run in the walk order
 constexpr int64_t kExactSortMin = 65536;
+
+// Scratch of rvs_distance_async_static: the refinement list and counters and
+// the walk-order sort of one call of up to kStaticMaxConfigs configurations
+// (19.1 MB at that size), in a device array of fixed size that every GPU the
+// library runs on holds until the process ends.  Host code never passes the
+// array's name to a kernel: it is a host-side stand-in, not a device
+// address.  rvs_create looks up the array's address on its device once
+// (static_scratch_addr) and the entry point passes that address.
+constexpr int64_t kStaticMaxConfigs = int64_t(1) << 20;
+constexpr size_t kStaticScratchBytes = size_t(20) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_addr;   // device -> address of that device's scratch
+
+// The address of the current device's scratch (device dev), looked up once
+// per device; called with dev current.
+hipError_t static_scratch_resolve(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_addr.count(dev)) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[dev] = ptr;
+    return hipSuccess;
+}
+
+void* static_scratch_addr(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_addr.find(dev);
+    return it == g_scratch_addr.end() ? nullptr : it->second;
+}
 
 // The obstacle set must fit the LDS image of the kernels (oriented boxes
 // take 48 more bytes each than axis-aligned ones).
@@ -255,6 +288,8 @@ rvs_status rvs_create(const rvs_options* opt, rvs_context** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->async_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = static_scratch_resolve(c->opt.device);
+    if (e == hipSuccess) (void)device_lds_limit(c->opt.device);   // read once here, not inside a query
     if (e != hipSuccess) { delete c; return RVS_EDEVICE; }
     *out = c;
     return RVS_OK;
@@ -867,6 +902,64 @@ rvs_status rvs_distance_async(rvs_context* c, int32_t robot, int32_t env, const 
     }
     if (e == hipSuccess) e = async_end(c, st, done);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_async");
+    return RVS_OK;
+}
+
+rvs_status rvs_distance_async_static(rvs_context* c, int32_t robot, int32_t env, const float* q, int64_t n,
+                                     float* d, float* profile, uint8_t* verdict, uint32_t flags, void* stream,
+                                     void* done) {
+    rvs_status s = check_query(c, robot, env, q, n, true);
+    if (s != RVS_OK) return s;
+    if (!(flags & RVS_F_DEVICE))
+        return fail(c, RVS_EINVAL, "rvs_distance_async_static: device arrays only (RVS_F_DEVICE)");
+    if (flags & RVS_F_TIMING) return fail(c, RVS_EINVAL, "rvs_distance_async_static: no timing");
+    if (n > kStaticMaxConfigs)
+        return fail(c, RVS_ERANGE, "rvs_distance_async_static: at most 1048576 configurations per call "
+                                   "(the size of its static scratch)");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const RobotRec& rb = c->robots[robot];
+    const EnvRec& ev = c->envs[env];
+    const int prec = prec_of(c, flags);
+    s = env_fits(c, ev.dev, prec == RVS_PREC_FP64, "rvs_distance_async_static");
+    if (s != RVS_OK) return s;
+    void* base = static_scratch_addr(c->opt.device);
+    if (!base) return fail(c, RVS_EDEVICE, "rvs_distance_async_static: no scratch address for this device");
+    const int L = rb.num_links, D = rb.num_dofs, M = ev.dev.num_obj;
+    const bool refine = prec == RVS_PREC_FP32_REFINE;
+    const bool sort = refine || (n >= kExactSortMin && (prec == RVS_PREC_FP64 || n > c->wave_max));
+    const size_t need = Carve::need({refine ? (size_t)n * 4 + 256 : 0, sort ? coh_bytes(n, D) : 0});
+    if (need > kStaticScratchBytes)
+        return fail(c, RVS_ERANGE, "rvs_distance_async_static: batch needs more scratch than the static array holds");
+    (void)hipSetDevice(c->opt.device);
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const QueryParams qp = params_of(c);
+    DistOut out{};
+    out.d = d; out.profile = profile; out.verdict = verdict;
+    const FastTuning ft = c->tuning_for(robot, env, L, M);
+    hipError_t e = async_begin(c, st);
+    if (e == hipSuccess && n > 0) {
+        Carve cv(base);
+        uint32_t* rcount = refine ? cv.take<uint32_t>(64) : nullptr;
+        int32_t* ridx = refine ? cv.take<int32_t>(n) : nullptr;
+        uint32_t* coh = sort ? cv.take<uint32_t>(coh_bytes(n, D) / 4) : nullptr;
+        const int32_t* perm = nullptr;
+        if (refine) {
+            // counters cleared by a kernel: the scratch keeps the last call's contents
+            e = launch_zero_u32(rcount, 64, st);
+            if (e == hipSuccess) e = launch_coherence_order(rb.dev, q, n, ft, coh, &perm, st);
+            if (e == hipSuccess)
+                e = launch_distance_fast(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, ft, perm, c->num_cus, st);
+            if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, st);
+        } else {
+            if (sort) e = launch_coherence_order(rb.dev, q, n, ft, coh, &perm, st);
+            if (e == hipSuccess)
+                e = prec == RVS_PREC_FP64 ? launch_distance_fp64(rb.dev, ev.dev, qp, q, n, out, st, perm)
+                                          : launch_distance_ref32_auto(rb.dev, ev.dev, qp, q, n, out, c->wave_max, st,
+                                                                       perm);
+        }
+    }
+    if (e == hipSuccess) e = async_end(c, st, done);
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_async_static");
     return RVS_OK;
 }
 

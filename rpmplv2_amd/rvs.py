@@ -106,7 +106,7 @@ RVS_EBUSY = 6
 EXPORTS = ["rvs_default_options", "rvs_create", "rvs_destroy", "rvs_last_error", "rvs_stream", "rvs_last_refined",
            "rvs_version", "rvs_robot_load", "rvs_env_create", "rvs_env_update", "rvs_env_bounds",
            "rvs_env_set_motion_law", "rvs_env_step", "rvs_distance",
-           "rvs_distance_f64", "rvs_is_valid", "rvs_distance_async", "rvs_is_valid_async",
+           "rvs_distance_f64", "rvs_is_valid", "rvs_distance_async", "rvs_distance_async_static", "rvs_is_valid_async",
            "rvs_distance_underestimation_async", "rvs_self_collision_async",
            "rvs_distance_underestimation", "rvs_distance_underestimation_moving",
            "rvs_trajectory_is_safe", "rvs_self_distance",
@@ -174,6 +174,7 @@ def lib():
         L.rvs_distance_f64.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, u32]
         L.rvs_is_valid.argtypes = [ctx, i32, i32, vp, i64, vp, u32]
         L.rvs_distance_async.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, u32, vp, vp]
+        L.rvs_distance_async_static.argtypes = [ctx, i32, i32, vp, i64, vp, vp, vp, u32, vp, vp]
         L.rvs_is_valid_async.argtypes = [ctx, i32, i32, vp, i64, vp, u32, vp, vp]
         L.rvs_distance_underestimation_async.argtypes = [ctx, i32, i32, vp, i64, vp, i64, vp, vp, vp, u32, vp, vp]
         L.rvs_self_collision_async.argtypes = [ctx, i32, vp, i64, vp, vp, u32, vp, vp]
@@ -536,6 +537,16 @@ class Engine:
         self._check(lib().rvs_distance_async(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d), _ptr(profile),
                                              _ptr(verdict), fl, _stream_handle(stream), _event_handle(done)),
                     "rvs_distance_async")
+
+    def distance_async_static(self, robot: int, env: int, q, d, profile=None, verdict=None,
+                              precision: Optional[int] = None, stream=None, done=None):
+        """rvs_distance_async_static on device tensors: rvs_distance_async
+        with its scratch in the library's fixed device array (no allocation;
+        at most 1,048,576 configurations)."""
+        fl = F_DEVICE | (F_PREC(precision) if precision is not None else 0)
+        self._check(lib().rvs_distance_async_static(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(d),
+                                                    _ptr(profile), _ptr(verdict), fl, _stream_handle(stream),
+                                                    _event_handle(done)), "rvs_distance_async_static")
 
     def is_valid_async(self, robot: int, env: int, q, valid, precision: Optional[int] = None, stream=None,
                        done=None):
