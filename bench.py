@@ -225,7 +225,8 @@ def workload5_section(eng, cores: int, n: int) -> dict:
                 "max_abs_dist_err_vs_ref64_subsample": float(np.abs(d_o - od64).max()),
                 "roofline": {"bound": "valu",
                              "kernel": {0: "k_distance_fast<OBB>", 1: "k_distance_fast_pool<OBB>",
-                                        2: "k_distance_fast_chunks<OBB>"}[tune["variant"]],
+                                        2: "k_distance_fast_chunks<OBB>", 3: "k_distance_fast_lanes<OBB>",
+                                        4: "k_distance_fast_links<OBB>"}[tune["variant"]],
                              "dense_flop_per_config": flop_obb_dense,
                              "dense_equivalent_tflops": flop_obb_dense * n / (bulk_o * 1e-3) / 1e12,
                              "dense_note": "dense nominal work (100 flop per FK joint + 196 per capsule-OBB pair) "
@@ -536,7 +537,8 @@ def main():
                         + ev_pairs * FLOP_CAPSULE_AABB)
     achieved = dense_flop_cfg * n / (bulk_ms * 1e-3) / 1e12
     hbm_bytes = n * (w.robot.num_dofs * 4 + 4 + L * 4 + 1)
-    kernel = {0: "k_distance_fast", 1: "k_distance_fast_pool", 2: "k_distance_fast_chunks"}[tune["variant"]]
+    kernel = {0: "k_distance_fast", 1: "k_distance_fast_pool", 2: "k_distance_fast_chunks", 3: "k_distance_fast_lanes",
+              4: "k_distance_fast_links"}[tune["variant"]]
     out = {
         "metric": "distance queries/sec (xArm6, 16 boxes) at verdict-exact",
         "value": value,

@@ -366,19 +366,28 @@ rvs_status rvs_edge_first_collision(rvs_context* ctx, int32_t env, const int32_t
  * Launch shape of the RVS_PREC_FP32_REFINE bulk pass for one (robot,
  * environment): configurations per wavefront (one per lane for the whole
  * walk, or persistent lane refill that hands a lane the next configuration
- * after each link, or persistent lane refill with every lane walking its own
- * (link, obstacle) pairs so that a wave evaluates one pair per lane per
- * step) and workgroup size.  The obstacle set is always staged whole in LDS
- * (every lane walks every obstacle of its current link, so a smaller
- * obstacle tile would only add reloads); it must fit the device's workgroup
- * LDS.  Results are the same bit for bit with every shape. */
+ * after each link, or persistent lane refill with the wave walking the
+ * broad phase together and every lane queueing its own (link, obstacle)
+ * pairs, so that a wave evaluates one pair per lane per step, or one launch
+ * per link over the configurations still alive at it, in the walk order,
+ * with the same queues) and workgroup size.  The obstacle set is always staged whole in LDS (every lane walks
+ * every obstacle of its current link, so a smaller obstacle tile would only
+ * add reloads); it must fit the device's workgroup LDS.  Variants 0, 1 and 2
+ * give the same results bit for bit.  Variants 3 and 4 evaluate pairs with a
+ * branch-free form of the same segment-box distance: the same verdicts, and
+ * profile distances within FP32 rounding (a few 1e-7 m) of the other
+ * variants'; the screening flags, and with them the refined set, can differ. */
 typedef struct rvs_tuning {
     int32_t variant;        /* 0: one lane per configuration; 1: persistent lane refill; 2: one lane per
                                configuration in persistent waves over 64-configuration chunks;
-                               3: persistent lane refill, each lane walking its own pairs (needs the
-                               broad-phase tables in LDS beside the obstacles; runs as 1 otherwise) */
-    int32_t block;          /* threads per workgroup: 64, 128 or 256; variant 3: 256, 512 or 1024 */
-    int32_t blocks_per_cu;  /* variants 1, 2 and 3: resident workgroups per CU (0 = as many as fit) */
+                               3: persistent lane refill, each lane queueing its own pairs (needs the
+                               broad-phase group tables, and the cull boxes and pair queues in LDS
+                               beside the obstacles; runs as 1 otherwise); 4: one launch per link
+                               over the configurations alive at it, each lane queueing its own pairs
+                               (the needs of 3 and 16 bytes of scratch per configuration; runs as 3
+                               in rvs_distance_async_static and served requests) */
+    int32_t block;          /* threads per workgroup: 64, 128 or 256; variants 3 and 4: 256, 512 or 1024 */
+    int32_t blocks_per_cu;  /* variants 1 to 4: resident workgroups per CU (0 = as many as fit) */
     int32_t coherence_bits; /* walk order (0..18): configurations sorted by their first
                                min(dofs, 18 / bits) joint angles, 2^bits bins each, so that a wave's
                                lanes hold similar poses; 0 = the order given.  Results do not depend

@@ -106,8 +106,10 @@ def _declare(L):
         "orc_edge_self_collision32": (None, [R, Pp, f32p, f32p, C.c_int64, C.c_float, u8p, f32p, C.c_int32]),
         "orc_extend_spines32": (None, [R, f32p, f32p, f32p, f32p, C.c_int64, C.c_int32, C.c_int32, f32p, i32p,
                                        C.c_int32]),
-        "orc_distance_cull32": (None, [R, Pp, S, f32p, C.c_int64, f32p, f32p, u8p, _p(C.c_int64), C.c_int32]),
-        "orc_distance_cull64": (None, [R, Pp, S, f32p, C.c_int64, f64p, f64p, u8p, _p(C.c_int64), C.c_int32]),
+        "orc_distance_cull32": (None, [R, Pp, S, f32p, C.c_int64, f32p, f32p, u8p, _p(C.c_int64), C.c_int32,
+                                       C.c_int32]),
+        "orc_distance_cull64": (None, [R, Pp, S, f32p, C.c_int64, f64p, f64p, u8p, _p(C.c_int64), C.c_int32,
+                                       C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -328,10 +330,12 @@ def capsule_obb(A, B, r, obb, precision=32, params=None):
     return d, np_, v.value
 
 
-def distance_cull(rb, scene: Scene, q, precision=32, params=None, nthreads=8):
+def distance_cull(rb, scene: Scene, q, precision=32, params=None, nthreads=8, search_from_hit=False):
     """computeDistance walked as the engine's bulk pass walks it (warm start,
-    grouped broad phase); returns (d, profile, verdict, (pairs evaluated,
-    bounding-box tests))."""
+    grouped broad phase); the first hit in obstacle order searched before the
+    hit in obstacle order, or (search_from_hit) in the walk positions after
+    it; returns (d, profile, verdict, (pairs evaluated, bounding-box
+    tests))."""
     p = params or params_struct()
     q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
     n, L = len(q), rb.num_links
@@ -341,7 +345,7 @@ def distance_cull(rb, scene: Scene, q, precision=32, params=None, nthreads=8):
     rs = robot_struct(rb)
     fn = lib().orc_distance_cull32 if precision == 32 else lib().orc_distance_cull64
     fn(C.byref(rs), C.byref(p), C.byref(scene.s), _ptr(q, C.c_float), n, _ptr(d, ct), _ptr(prof, ct),
-       _ptr(ver, C.c_uint8), _ptr(counts, C.c_int64), nthreads)
+       _ptr(ver, C.c_uint8), _ptr(counts, C.c_int64), nthreads, 1 if search_from_hit else 0)
     return d, prof, ver, (int(counts[0]), int(counts[1]))
 
 

@@ -702,6 +702,7 @@ template <typename T> void cull_box(const SceneT<T>& sc, int j, T* lo, T* hi) {
 
 struct CullCount {
     int64_t evaluated = 0, tests = 0;
+    bool from_hit = false;   // the first-hit search continues the walk after the hit (else: obstacle order)
 };
 
 // The engine's broad-phase walk order (restated): obstacles sorted stably by
@@ -797,7 +798,7 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
                 if (seg_box_gap(A, B, lo, hi) > std::max(bound + r, pr.eq) + T(1e-5)) return true;
                 return false;
             };
-            int best = -1, jh = -1;
+            int best = -1, jh = -1, mhit = -1;   // mhit: walk position of the hit (-1: the warm pair)
             T vh = 0;
             const int w = warm;
             if (w >= 0) {
@@ -821,19 +822,44 @@ void distance_one(const orc_robot* rb, const P<T>& pr, const SceneT<T>& sc, cons
                     const T dt = pair_filtered(rb, pr, sc, A, B, i, j);
                     if (dt < prof[i]) best = j;
                     prof[i] = stdmin(prof[i], dt);
-                    if (prof[i] <= 0) { jh = j; vh = prof[i]; }
+                    if (prof[i] <= 0) { jh = j; vh = prof[i]; mhit = m; }
                 }
             }
             if (jh >= 0) {
                 T first = vh;
-                for (int j = 0; j < jh; ++j) {
-                    T lo[3], hi[3];
-                    cull_box(sc, j, lo, hi);
-                    cull->tests++;
-                    if (far(lo, hi, T(0))) continue;
-                    cull->evaluated++;
-                    const T dj = pair_filtered(rb, pr, sc, A, B, i, j);
-                    if (dj <= 0) { first = dj; break; }
+                if (cull->from_hit) {
+                    // every walk position before the hit held a pair evaluated
+                    // above 0 or culled with a bound at least that of a minimum
+                    // of 0: the search walks the positions after it (all of
+                    // them after a warm hit) for a smaller index with a
+                    // distance <= 0
+                    int fj = jh;
+                    const int m0 = mhit + 1;
+                    for (int g = m0 / kGroup; g < ng; ++g) {
+                        cull->tests++;
+                        if (far(&wo->glo[3 * g], &wo->ghi[3 * g], T(0))) continue;
+                        for (int m = std::max(g * kGroup, m0); m < std::min(sc.M, (g + 1) * kGroup); ++m) {
+                            const int j = wo->order[m];
+                            if (j == w || j >= fj) continue;
+                            T lo[3], hi[3];
+                            cull_box(sc, j, lo, hi);
+                            cull->tests++;
+                            if (far(lo, hi, T(0))) continue;
+                            cull->evaluated++;
+                            const T dj = pair_filtered(rb, pr, sc, A, B, i, j);
+                            if (dj <= 0) { fj = j; first = dj; }
+                        }
+                    }
+                } else {
+                    for (int j = 0; j < jh; ++j) {
+                        T lo[3], hi[3];
+                        cull_box(sc, j, lo, hi);
+                        cull->tests++;
+                        if (far(lo, hi, T(0))) continue;
+                        cull->evaluated++;
+                        const T dj = pair_filtered(rb, pr, sc, A, B, i, j);
+                        if (dj <= 0) { first = dj; break; }
+                    }
                 }
                 prof[i] = first;
                 *d_out = 0;
@@ -901,7 +927,8 @@ void parallel_for(int64_t n, int nthreads, F&& fn) {
 
 template <typename T>
 void distance_batch(const orc_robot* rb, const orc_params* p, const orc_scene* scn, const float* q, int64_t n,
-                    T* d, T* profile, T* np, uint8_t* verdict, int nthreads, int64_t* counts = nullptr) {
+                    T* d, T* profile, T* np, uint8_t* verdict, int nthreads, int64_t* counts = nullptr,
+                    bool from_hit = false) {
     P<T> pr(p);
     SceneT<T> sc(scn);
     std::vector<Seg> segs = build_segments(rb);
@@ -913,6 +940,7 @@ void distance_batch(const orc_robot* rb, const orc_params* p, const orc_scene* s
         std::vector<T> Pk(3 * (L + 1));
         std::vector<T> prof(L);
         CullCount cc;
+        cc.from_hit = from_hit;
         for (int64_t c = b; c < e; ++c) {
             skeleton<T>(rb, segs, q + c * D, Pk.data());
             T dd; uint8_t v;
@@ -1434,14 +1462,16 @@ void orc_edge_first_collision32(const orc_robot* rb, const orc_params* p, const 
 }
 
 void orc_distance_cull32(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q, int64_t n,
-                         float* d, float* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads) {
+                         float* d, float* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads,
+                         int32_t search) {
     counts[0] = counts[1] = 0;
-    distance_batch<float>(rb, p, sc, q, n, d, profile, nullptr, verdict, nthreads, counts);
+    distance_batch<float>(rb, p, sc, q, n, d, profile, nullptr, verdict, nthreads, counts, search != 0);
 }
 void orc_distance_cull64(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q, int64_t n,
-                         double* d, double* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads) {
+                         double* d, double* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads,
+                         int32_t search) {
     counts[0] = counts[1] = 0;
-    distance_batch<double>(rb, p, sc, q, n, d, profile, nullptr, verdict, nthreads, counts);
+    distance_batch<double>(rb, p, sc, q, n, d, profile, nullptr, verdict, nthreads, counts, search != 0);
 }
 
 // AbstractRobot::computeEnclosingRadii, float skeleton (ref32):

@@ -132,12 +132,17 @@ void orc_self_min32(const orc_robot* rb, const orc_params* p, const float* q, in
  * groups of 4 tested by their union box first), skipping boxes farther from
  * the segment's bounding box than max(running link minimum + r,
  * EQUALITY_THRESHOLD) + 1e-5; a hit found out of obstacle order is replaced
- * by the first one in order.  Results equal orc_distance32/64; counts =
- * {pairs evaluated, bounding-box tests (groups and obstacles)}. */
+ * by the first one in order, searched (search 0) among the obstacles before
+ * it in obstacle order, as the bulk kernels of tuning variants 0-3 do, or
+ * (search 1) among the walk positions after it, as the link-major kernel
+ * does.  Results equal orc_distance32/64; counts = {pairs evaluated,
+ * bounding-box tests (groups and obstacles)}. */
 void orc_distance_cull32(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q, int64_t n,
-                         float* d, float* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads);
+                         float* d, float* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads,
+                         int32_t search);
 void orc_distance_cull64(const orc_robot* rb, const orc_params* p, const orc_scene* sc, const float* q, int64_t n,
-                         double* d, double* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads);
+                         double* d, double* profile, uint8_t* verdict, int64_t counts[2], int32_t nthreads,
+                         int32_t search);
 
 /* AbstractRobot::computeEnclosingRadii (xArm6 / spatial chains), float:
  * R [n][dofs][dofs+1]. */

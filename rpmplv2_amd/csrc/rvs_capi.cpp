@@ -653,7 +653,7 @@ constexpr size_t kMaxGraphs = 8;
 
 hipError_t fp32_graph_run(rvs_context* c, int32_t robot, int32_t env, const RobotDev& rb, const EnvDev& ev,
                           const QueryParams& qp, const float* q, int64_t n, const DistOut& out, int32_t* ridx,
-                          uint32_t* rcount, uint32_t* coh, const FastTuning& ft, uint32_t* refined) {
+                          uint32_t* rcount, uint32_t* coh, uint32_t* links, const FastTuning& ft, uint32_t* refined) {
     hipError_t e = hipSuccess;
     if (!c->count_host) {
         e = hipHostMalloc(reinterpret_cast<void**>(&c->count_host), 256, hipHostMallocDefault);
@@ -683,7 +683,7 @@ hipError_t fp32_graph_run(rvs_context* c, int32_t robot, int32_t env, const Robo
         const int32_t* perm = nullptr;
         if (le == hipSuccess) le = launch_coherence_order(rb, q, n, ft, coh, &perm, c->stream);
         if (le == hipSuccess)
-            le = launch_distance_fast(rb, ev, qp, q, n, out, ridx, rcount, ft, perm, c->num_cus, c->stream);
+            le = launch_distance_fast(rb, ev, qp, q, n, out, ridx, rcount, ft, perm, links, c->num_cus, c->stream);
         if (le == hipSuccess) le = launch_distance_refine(rb, ev, qp, q, n, out, ridx, rcount, c->stream);
         if (le == hipSuccess)
             le = hipMemcpyAsync(c->count_host, rcount, 4, hipMemcpyDeviceToHost, c->stream);
@@ -734,18 +734,22 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
     const bool sort_exact = prec != RVS_PREC_FP32_REFINE && n >= kExactSortMin && n <= INT32_MAX &&
                             (prec == RVS_PREC_FP64 || n > c->wave_max);
     const size_t coh_need = (prec == RVS_PREC_FP32_REFINE || sort_exact) ? coh_bytes(n, D) : 0;
+    const FastTuning ft = c->tuning_for(robot, env, L, M);
+    const size_t links_need = (prec == RVS_PREC_FP32_REFINE && ft.variant == 4) ? fast_links_bytes(n) : 0;
     int32_t* ridx = nullptr;
     uint32_t* rcount = nullptr;
     uint32_t* coh = nullptr;
+    uint32_t* links = nullptr;
     if (flags & RVS_F_DEVICE) {
-        e = c->arena.ensure(Carve::need({refine_bytes, coh_need}));
+        e = c->arena.ensure(Carve::need({refine_bytes, coh_need, links_need}));
         if (e != hipSuccess) return hip_fail(c, e, "rvs_distance: staging");
         Carve cv(c->arena.p);
         if (refine_bytes) { rcount = cv.take<uint32_t>(64); ridx = cv.take<int32_t>(n); }
         if (coh_need) coh = cv.take<uint32_t>(coh_need / 4);
+        if (links_need) links = cv.take<uint32_t>(links_need / 4);
         out.d = d; out.profile = profile; out.nearest = nearest; out.verdict = verdict;
     } else {
-        const size_t need = Carve::need({refine_bytes, coh_need, (size_t)n * D * 4, (size_t)n * 4,
+        const size_t need = Carve::need({refine_bytes, coh_need, links_need, (size_t)n * D * 4, (size_t)n * 4,
                                          profile ? (size_t)n * L * 4 : 0, nearest ? (size_t)n * M * L * 24 : 0,
                                          (size_t)n});
         e = c->arena.ensure(need);
@@ -753,6 +757,7 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
         Carve cv(c->arena.p);
         if (refine_bytes) { rcount = cv.take<uint32_t>(64); ridx = cv.take<int32_t>(n); }
         if (coh_need) coh = cv.take<uint32_t>(coh_need / 4);
+        if (links_need) links = cv.take<uint32_t>(links_need / 4);
         float* qs = cv.take<float>((size_t)n * D);
         out.d = cv.take<float>(n);
         out.profile = profile ? cv.take<float>((size_t)n * L) : nullptr;
@@ -767,17 +772,16 @@ rvs_status rvs_distance(rvs_context* c, int32_t robot, int32_t env, const float*
         const bool graph = prec == RVS_PREC_FP32_REFINE && (flags & RVS_F_DEVICE) && !timing &&
                            !(flags & RVS_F_EAGER);
         if (graph) {
-            const FastTuning ft = c->tuning_for(robot, env, L, M);
-            e = fp32_graph_run(c, robot, env, rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, coh, ft, &refined);
+            e = fp32_graph_run(c, robot, env, rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, coh, links, ft,
+                               &refined);
         } else if (prec == RVS_PREC_FP32_REFINE) {
             e = launch_zero_u32(rcount, 64, c->stream);
-            const FastTuning ft = c->tuning_for(robot, env, L, M);
             const int32_t* perm = nullptr;
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[3], c->stream);
             if (e == hipSuccess) e = launch_coherence_order(rb.dev, qd, n, ft, coh, &perm, c->stream);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
             if (e == hipSuccess)
-                e = launch_distance_fast(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, ft, perm, c->num_cus,
+                e = launch_distance_fast(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, ft, perm, links, c->num_cus,
                                          c->stream);
             if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
             if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, qd, n, out, ridx, rcount, c->stream);
@@ -876,7 +880,9 @@ rvs_status rvs_distance_async(rvs_context* c, int32_t robot, int32_t env, const 
     hipError_t e = async_begin(c, st);
     if (e == hipSuccess && n > 0) {
         if (prec == RVS_PREC_FP32_REFINE) {
-            const size_t need = Carve::need({(size_t)n * 4 + 256, coh_bytes(n, D)});
+            const FastTuning ft = c->tuning_for(robot, env, L, M);
+            const size_t links_need = ft.variant == 4 ? fast_links_bytes(n) : 0;
+            const size_t need = Carve::need({(size_t)n * 4 + 256, coh_bytes(n, D), links_need});
             if (need > c->arena.cap) c->drain_async();   // growing frees the block earlier calls use
             e = c->arena.ensure(need);
             if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_async: staging");
@@ -884,12 +890,13 @@ rvs_status rvs_distance_async(rvs_context* c, int32_t robot, int32_t env, const 
             uint32_t* rcount = cv.take<uint32_t>(64);
             int32_t* ridx = cv.take<int32_t>(n);
             uint32_t* coh = cv.take<uint32_t>(coh_bytes(n, D) / 4);
-            const FastTuning ft = c->tuning_for(robot, env, L, M);
+            uint32_t* links = links_need ? cv.take<uint32_t>(links_need / 4) : nullptr;
             const int32_t* perm = nullptr;
             e = launch_zero_u32(rcount, 64, st);
             if (e == hipSuccess) e = launch_coherence_order(rb.dev, q, n, ft, coh, &perm, st);
             if (e == hipSuccess)
-                e = launch_distance_fast(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, ft, perm, c->num_cus, st);
+                e = launch_distance_fast(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, ft, perm, links, c->num_cus,
+                                         st);
             if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, st);
         } else {
             const int32_t* perm = nullptr;
@@ -947,8 +954,10 @@ rvs_status rvs_distance_async_static(rvs_context* c, int32_t robot, int32_t env,
             // counters cleared by a kernel: the scratch keeps the last call's contents
             e = launch_zero_u32(rcount, 64, st);
             if (e == hipSuccess) e = launch_coherence_order(rb.dev, q, n, ft, coh, &perm, st);
+            // no links scratch in the static array: a link-major tuning runs as variant 3
             if (e == hipSuccess)
-                e = launch_distance_fast(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, ft, perm, c->num_cus, st);
+                e = launch_distance_fast(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, ft, perm, nullptr, c->num_cus,
+                                         st);
             if (e == hipSuccess) e = launch_distance_refine(rb.dev, ev.dev, qp, q, n, out, ridx, rcount, st);
         } else {
             if (sort) e = launch_coherence_order(rb.dev, q, n, ft, coh, &perm, st);
@@ -1045,7 +1054,7 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
     const int64_t m = std::min<int64_t>(n, 1 << 21);
     const int L = rb.num_links, D = rb.num_dofs;
     hipError_t e = c->arena.ensure(Carve::need({(size_t)m * D * 4, (size_t)m * 4, (size_t)m * L * 4, (size_t)m,
-                                                (size_t)m * 4, 256, coh_bytes(m, D)}));
+                                                (size_t)m * 4, 256, coh_bytes(m, D), fast_links_bytes(m)}));
     if (e != hipSuccess) return hip_fail(c, e, "rvs_autotune: staging");
     Carve cv(c->arena.p);
     float* qs = cv.take<float>((size_t)m * D);
@@ -1056,6 +1065,7 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
     int32_t* idx = cv.take<int32_t>(m);
     uint32_t* count = cv.take<uint32_t>(64);
     uint32_t* coh = cv.take<uint32_t>(coh_bytes(m, D) / 4);
+    uint32_t* links = cv.take<uint32_t>(fast_links_bytes(m) / 4);
     e = hipMemcpyAsync(qs, q, (size_t)m * D * 4, (flags & RVS_F_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                        c->stream);
     const QueryParams qp = params_of(c);
@@ -1077,6 +1087,10 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
             cand.push_back(FastTuning{3, b, occ, cb});
             if (occ > 1) cand.push_back(FastTuning{3, b, (occ + 1) / 2, cb});
         }
+        for (int b : {256, 512, 1024}) {   // link-major passes, under the same condition
+            const int occ = fast_links_blocks_per_cu(ev.dev, b);
+            if (occ > 0) cand.push_back(FastTuning{4, b, occ, cb});
+        }
     }
     FastTuning win{};
     float win_ms = 1e30f;
@@ -1088,7 +1102,8 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
             if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->stream);
             if (e == hipSuccess) e = launch_coherence_order(rb.dev, qs, m, t, coh, &perm, c->stream);
             if (e == hipSuccess)
-                e = launch_distance_fast(rb.dev, ev.dev, qp, qs, m, out, idx, count, t, perm, c->num_cus, c->stream);
+                e = launch_distance_fast(rb.dev, ev.dev, qp, qs, m, out, idx, count, t, perm, links, c->num_cus,
+                                         c->stream);
             if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->stream);
             if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
             float ms = 0;
@@ -1120,12 +1135,12 @@ rvs_status rvs_autotune(rvs_context* c, int32_t robot, int32_t env, const float*
 
 rvs_status rvs_set_tuning(rvs_context* c, int32_t robot, int32_t env, const rvs_tuning* t) {
     if (!c || !t) return fail(c, RVS_EINVAL, "rvs_set_tuning: null argument");
-    const bool block_ok = t->variant == 3 ? (t->block == 256 || t->block == 512 || t->block == 1024)
+    const bool block_ok = t->variant >= 3 ? (t->block == 256 || t->block == 512 || t->block == 1024)
                                           : (t->block == 64 || t->block == 128 || t->block == 256);
-    if (t->variant < 0 || t->variant > 3 || !block_ok || t->blocks_per_cu < 0 || t->coherence_bits < 0 ||
+    if (t->variant < 0 || t->variant > 4 || !block_ok || t->blocks_per_cu < 0 || t->coherence_bits < 0 ||
         t->coherence_bits > kCohKeyBits)
         return fail(c, RVS_EINVAL,
-                    "rvs_set_tuning: variant 0/1/2/3, block 64/128/256 (variant 3: 256/512/1024), "
+                    "rvs_set_tuning: variant 0..4, block 64/128/256 (variants 3 and 4: 256/512/1024), "
                     "blocks_per_cu >= 0, coherence_bits 0..18");
     std::lock_guard<std::mutex> lk(c->tuning_mu);
     c->tuning[{robot, env}] = FastTuning{t->variant, t->block, t->blocks_per_cu, t->coherence_bits};

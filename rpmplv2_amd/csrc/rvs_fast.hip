@@ -155,6 +155,26 @@ __device__ __attribute__((always_inline)) inline float fast_pair(const Staged<fl
     return dt;
 }
 
+// One (link, obstacle) pair of the per-lane walk: fast_pair with the flat
+// evaluation (the same filtering and tolerance screening).
+template <bool OBB>
+__device__ __attribute__((always_inline)) inline float flat_pair(const Staged<float>& s, int j, const Vec3<float>& P0,
+                                                                 const Vec3<float>& P1, float r, const Params<float>& p,
+                                                                 float band, bool& uncertain) {
+    const Box<float> bx = s.box[j];
+    float dt;
+    if (OBB) {
+        const float* x = s.xf + 12 * j;
+        dt = flat_capsule_box<FastMath>(to_box(x, P0), to_box(x, P1), r, bx, p, band, uncertain);
+    } else {
+        dt = flat_capsule_box<FastMath>(P0, P1, r, bx, p, band, uncertain);
+    }
+    const float tol = s.tol[j];
+    if (tol < p.maxd && fabsf(dt - tol) < band) uncertain = true;
+    if (dt > tol) dt = p.maxd;
+    return dt;
+}
+
 // Smallest filtered distance of one link over the obstacles, with the
 // reference's early exit: the first pair in obstacle order whose distance is
 // <= 0 ends the link (hit) and its distance is the link's value.
@@ -480,50 +500,54 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
 }
 
-// The grouped walk with every lane on its own cursor, its broad phase run
-// ahead of its pair evaluations.  In the kernels above a wave walks the
-// obstacles together and evaluates obstacle j whenever one of its lanes
-// needs it, so a wave pays for the union of its lanes' pairs; with many
-// obstacles (512 boxes: about 37 broad-phase tests per evaluated pair) the
-// lanes of a wave then need different pairs at different times and few lanes
-// do useful work per instruction.  Here each lane runs the walk of
-// link_min<OBB, true> on its own configurations (taken from a global counter
-// as lanes free up), in rounds:
-//   1. a lane at the start of a link computes the link's end point (FK);
-//   2. scan: every lane with walk left tests its own next entries (group box
-//      first, as the walk does, then members) with its current bound and
-//      queues the members that pass (an LDS ring of kLaneQueue per lane),
-//      until nearly every lane holds a pair (at least kLaneScanMin steps);
-//   3. each lane takes its queue's pairs in walk order, tests each again with
-//      its bound of now, and evaluates the first that passes; all lanes
-//      evaluate their pairs in the same step (each its own obstacle);
-//   4. each lane folds the result into its link as link_min does.
-// The scan's bound is never tighter than the plain walk's at the same entry
-// (the bound only shrinks), a member's box lies inside its group's box, and
-// every queued pair is tested again with exactly the bound the plain walk
-// has when it reaches it (all pairs before it in walk order have been
-// evaluated or dropped by then).  So per configuration the pairs, their
-// order, every screening flag and every result are those of link_min, and
-// all launch shapes agree bit for bit.  Phases of a lane's link: the warm
-// pair, the walk, and the search for the first hit in obstacle order before
-// a hit found out of order (queued the same way, bound fixed).
-constexpr int kLaneQueue = 8;       // pending pairs per lane (power of 2)
-constexpr int kLaneScanMin = 2;     // broad-phase steps per round at least (every lane with walk left advances)
-constexpr int kLaneScanMax = 48;    // and at most
-enum : int { kPhNewLink = 0, kPhWarm = 1, kPhWalk = 2, kPhSearch = 3 };
+// The grouped walk with lane refill and packed pair evaluations (tuning
+// variant 3).  In the kernels above a lane evaluates a pair where the wave's
+// walk meets it, so a wave whose lanes need different obstacles (lanes on
+// different poses, as lane refill makes them) pays one evaluation per
+// obstacle any lane needs.  Here, per link step (every lane on the current
+// link of its own configuration, taken from a global counter as lanes free
+// up):
+//   1. every lane evaluates its warm pair (its previous link's nearest
+//      obstacle), all lanes in one step;
+//   2. the wave walks the groups in the scene's walk order together (a
+//      group's box and members are read once for all lanes); each lane tests
+//      them against its bound and queues the members that pass (an LDS ring
+//      of kLaneQueue per lane);
+//   3. whenever enough lanes hold queued pairs (and at the end), every lane
+//      takes its next queued pair, tests it again against its bound of now
+//      and evaluates it if it passes: one pair per lane per step, each its
+//      own obstacle, with the flat evaluation (no case tree, so the lanes do
+//      not split across branches);
+//   4. a lane whose pair ends its link (distance <= 0) stops walking; the
+//      first hit in obstacle order is then found among the obstacles before
+//      it that the bound of a running minimum of 0 does not rule out, queued
+//      and evaluated the same way, the smallest index with distance <= 0
+//      winning.
+// The walk evaluates exactly the pairs of link_min's walk in its order (the
+// queueing bound is never tighter than the walk's, a member's box lies
+// inside its group's, and every pair is tested again with exactly the bound
+// the walk has when it reaches it); the search evaluates every candidate
+// before the hit instead of stopping at the first, which can only add
+// screening flags.  The link's value, the warm pair of the next link and the
+// verdict are those of link_min.
+constexpr int kLaneQueue = 16;         // pending pairs per lane (power of 2, > kGroupSize)
+constexpr int kDrainLanes = 40;        // an evaluation step once this many lanes hold a pair
 
 template <bool OBB>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_lanes(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
-    uint32_t* count, uint32_t* next, const int32_t* __restrict__ perm, size_t walk_off, size_t queue_off) {
+    uint32_t* count, uint32_t* next, const int32_t* __restrict__ perm, size_t cull_off, size_t queue_off) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<float> s = stage_env<float, OBB>(env, smem);
-    const WalkTables wt = stage_walk(env, smem + walk_off);
+    float* const lcull = reinterpret_cast<float*>(smem + cull_off);   // [M][6] cull boxes, for the queued pairs
+    for (int k = threadIdx.x; k < 6 * env.num_obj; k += blockDim.x) lcull[k] = env.cull[k];
+    const Staged<float> s = stage_env<float, OBB>(env, smem);   // ends with a barrier
     const int B = blockDim.x;
     int16_t* const ring = reinterpret_cast<int16_t*>(smem + queue_off) + threadIdx.x;   // slot k: ring[k * B]
     const int M = env.num_obj;
     const int L = rb.num_links;
     const int D = rb.num_dofs;
+    const int ng = num_groups(M);
+    const float* __restrict__ cull = env.cull;
     const Params<float> p{qp.eq, qp.maxd};
     const float band = qp.band;
     const int lane = __lane_id();
@@ -532,10 +556,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     bool drained = false;    // the counter has passed n
     const float* qc = q;
     FKf fk;
-    Vec3<float> P0{0.0f, 0.0f, 0.0f}, P1{0.0f, 0.0f, 0.0f};
-    float r = 0.0f, prof = 0.0f, d = 0.0f, vh = 0.0f;
-    int i = 0, warm = -1, best = -1, jh = -1, cur = 0, phase = kPhNewLink;
-    int qh = 0, qn = 0;      // queue head and length
+    Vec3<float> P0{0.0f, 0.0f, 0.0f};
+    float d = 0.0f;
+    int i = 0, warm = -1;
     bool uncertain = false;
     for (;;) {
         // refill the lanes without a configuration
@@ -561,109 +584,124 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                     warm = -1;
                     d = p.maxd;
                     uncertain = false;
-                    phase = kPhNewLink;
                 } else {
                     drained = true;
                 }
             }
         }
         const bool busy = c >= 0;
-        const uint64_t busy_mask = __ballot(busy);
-        if (busy_mask == 0) break;
-        // 1. the link's end point
-        if (busy && phase == kPhNewLink) {
+        if (__ballot(busy) == 0) break;
+        // the link's end point
+        Vec3<float> P1 = P0;
+        float r = 0.0f;
+        if (busy) {
             P1 = skelf_point(fk, rb, qc, i + 1);
             if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
             r = rb.radius[i];
-            prof = p.maxd;
-            best = -1;
-            jh = -1;
-            cur = 0;
-            qh = 0;
-            qn = 0;
-            phase = warm >= 0 ? kPhWarm : kPhWalk;
         }
         Vec3<float> slo, shi;
         seg_bounds(P0, P1, slo, shi);
-        const bool walk = busy && phase == kPhWalk;
-        const bool search = busy && phase == kPhSearch;
-        const int lim = search ? jh : M;
-        const float bw = cull_bound(prof, r, p.eq);                 // the walk's bound now
-        const float bs = search ? cull_bound(0.0f, r, p.eq) : bw;   // the scan's bound
-        // 2. scan ahead
-        const int need_pair = __popcll(busy_mask);
-        for (int k = 0; k < kLaneScanMax; ++k) {
-            const bool scanning = (walk || search) && cur < lim && qn < kLaneQueue;
-            const bool starving = scanning && qn == 0;
-            if (__ballot(scanning) == 0) break;
-            if (k >= kLaneScanMin && __popcll(__ballot(starving)) * 8 <= need_pair) break;
-            if (scanning) {
-                int j = -1;
-                if (walk) {
-                    if ((cur & (kGroupSize - 1)) == 0 && cull_pair(wt.gcull + 6 * (cur / kGroupSize), slo, shi, bs)) {
-                        cur += kGroupSize;
-                    } else {
-                        const int jj = wt.gorder[cur++];
-                        if (jj != warm && !cull_pair(wt.cull + 6 * jj, slo, shi, bs)) j = jj;
-                    }
-                } else {
-                    const int jj = cur++;
-                    if (!cull_pair(wt.cull + 6 * jj, slo, shi, bs)) j = jj;
-                }
-                if (j >= 0) {
-                    ring[((qh + qn) & (kLaneQueue - 1)) * B] = (int16_t)j;
-                    ++qn;
-                }
+        const bool gnd_skip = i < rb.ground_included;
+        auto eval = [&](int j) {
+            return (s.gnd[j] && gnd_skip) ? p.maxd : flat_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+        };
+        // 1. the warm pair
+        float prof = p.maxd, vh = 0.0f;
+        int best = -1, jh = -1;
+        const int w = busy ? warm : -1;
+        if (w >= 0) {
+            const float dw = eval(w);
+            if (dw <= 0) {
+                jh = w;
+                vh = dw;
+            } else if (dw < prof) {
+                prof = dw;
+                best = w;
             }
         }
-        // 3. this lane's next pair
-        int task = -1;
-        if (busy && phase == kPhWarm) task = warm;
-        while ((walk || search) && task < 0 && qn > 0) {
-            const int j = ring[(qh & (kLaneQueue - 1)) * B];
-            ++qh;
-            --qn;
-            if (search || !cull_pair(wt.cull + 6 * j, slo, shi, bw)) task = j;
-        }
-        bool link_done = (walk || search) && task < 0 && qn == 0 && cur >= lim;   // search: no earlier hit, vh stays
-        // the pairs of every lane that has one, in one step
-        const bool gnd_skip = i < rb.ground_included;
-        float dt = 0.0f;
-        if (task >= 0) dt = (s.gnd[task] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, task, P0, P1, r, p, band, uncertain);
-        // 4. fold the result into the link
-        if (task >= 0) {
-            if (phase == kPhWarm) {
-                if (dt <= 0) {
-                    jh = warm;
-                    vh = dt;
-                    phase = kPhSearch;
-                } else {
-                    if (dt < prof) {
-                        prof = dt;
-                        best = warm;
-                    }
-                    phase = kPhWalk;
-                }
-                cur = 0;
-            } else if (phase == kPhWalk) {
+        // 2-3. the walk, its pairs evaluated from the queues
+        bool go = busy && jh < 0;
+        int qh = 0, qn = 0;
+        auto walk_step = [&]() {   // one queued pair per lane
+            int task = -1;
+            while (qn > 0 && task < 0) {
+                const int j = ring[(qh & (kLaneQueue - 1)) * B];
+                ++qh;
+                --qn;
+                if (!cull_pair(lcull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) task = j;
+            }
+            if (task >= 0) {
+                const float dt = eval(task);
                 if (dt < prof) best = task;
                 prof = smin(prof, dt);
                 if (prof <= 0) {
                     jh = task;
                     vh = prof;
-                    phase = kPhSearch;
-                    cur = 0;
-                    qh = 0;
+                    go = false;
                     qn = 0;
                 }
-            } else if (dt <= 0) {   // the first hit in obstacle order
-                vh = dt;
-                link_done = true;
             }
+        };
+        for (int g = 0; g < ng; ++g) {
+            if (__ballot(go) == 0) break;
+            if (go && !cull_pair(env.gcull + 6 * g, slo, shi, cull_bound(prof, r, p.eq))) {
+                const int m1 = min(M, (g + 1) * kGroupSize);
+                for (int m = g * kGroupSize; m < m1; ++m) {
+                    const int j = env.gorder[m];
+                    if (go && j != w && !cull_pair(cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
+                        ring[((qh + qn) & (kLaneQueue - 1)) * B] = (int16_t)j;
+                        ++qn;
+                    }
+                }
+            }
+            // evaluate once enough lanes hold pairs; before the next group,
+            // every queue must have room for a whole group
+            if (__popcll(__ballot(qn > 0)) >= kDrainLanes) walk_step();
+            while (__ballot(qn > kLaneQueue - kGroupSize)) walk_step();
         }
+        while (__ballot(qn > 0)) walk_step();
+        // 4. the first hit in obstacle order before the hit found
+        bool srch = busy && jh >= 0;
+        if (__ballot(srch)) {
+            const float b0 = cull_bound(0.0f, r, p.eq);
+            int fj = jh;
+            float fv = vh;
+            auto search_step = [&]() {
+                if (qn > 0) {
+                    const int j = ring[(qh & (kLaneQueue - 1)) * B];
+                    ++qh;
+                    --qn;
+                    if (j < fj) {
+                        const float dt = eval(j);
+                        if (dt <= 0) {
+                            fj = j;
+                            fv = dt;
+                        }
+                    }
+                }
+            };
+            qh = qn = 0;
+            for (int g = 0; g < ng; ++g) {
+                if (srch && !cull_pair(env.gcull + 6 * g, slo, shi, b0)) {
+                    const int m1 = min(M, (g + 1) * kGroupSize);
+                    for (int m = g * kGroupSize; m < m1; ++m) {
+                        const int j = env.gorder[m];
+                        if (srch && j < fj && !cull_pair(cull + 6 * j, slo, shi, b0)) {
+                            ring[((qh + qn) & (kLaneQueue - 1)) * B] = (int16_t)j;
+                            ++qn;
+                        }
+                    }
+                }
+                if (__popcll(__ballot(qn > 0)) >= kDrainLanes) search_step();
+                while (__ballot(qn > kLaneQueue - kGroupSize)) search_step();
+            }
+            while (__ballot(qn > 0)) search_step();
+            if (srch) vh = fv;
+        }
+        // the link's value
         bool finished = false;
-        if (link_done) {
-            const bool hit = phase == kPhSearch;
+        if (busy) {
+            const bool hit = jh >= 0;
             const float lp = hit ? vh : prof;
             if (out.profile) out.profile[c * L + i] = lp;
             if (hit) {
@@ -678,7 +716,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                 } else {
                     ++i;
                     P0 = P1;
-                    phase = kPhNewLink;
                 }
             }
             if (finished) {
@@ -688,6 +725,201 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         }
         append_flagged(finished && uncertain, c, idx, count);
         if (finished) c = -1;
+    }
+}
+
+// Read-only tables indexed by wave-uniform values: read through the
+// constant address space, so that they are fetched with scalar loads into
+// SGPRs (the tables are not written while a kernel reads them).
+template <typename T>
+using const_ptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __attribute__((always_inline)) inline const_ptr<T> as_const(const T* p) {
+    return (const_ptr<T>)(p);
+}
+
+// The link-major walk (tuning variant 4): one launch per link index i over
+// the configurations still alive at link i, 64 consecutive ones per wave in
+// the walk order, so that the lanes of a wave hold one link of similar poses
+// and take the broad phase's branches together.  Per configuration:
+//   1. the link's end points, from the joint angles (the same compositions
+//      in the same order as the incremental walk, so the same bits);
+//   2. the warm pair (the previous link's nearest obstacle), all lanes in one
+//      step;
+//   3. the wave walks the groups in the scene's walk order together (a
+//      group's box and members read once, with scalar loads, for all lanes);
+//      each lane queues the members that pass its bound (an LDS ring of
+//      kLaneQueue per lane); whenever enough lanes hold queued pairs every
+//      lane takes its next one, tests it again against its bound of now and
+//      evaluates it if it passes (the flat evaluation, one pair per lane per
+//      step, each its own obstacle);
+//   4. once a lane's pair ends the link (distance <= 0), the rest of its walk
+//      is the search for the first hit in obstacle order: the positions after
+//      the hit, with the bound of a running minimum of 0, obstacles of a
+//      smaller index than the smallest hit found.  Every position before the
+//      hit held a pair evaluated above 0 or culled with a bound at least that
+//      of a minimum of 0, which never culls a pair whose distance can be
+//      <= 0: none of them can be an earlier hit.
+// The walk evaluates exactly the pairs of link_min's walk up to its hit, in
+// its order (the queueing bound is never tighter than the walk's, a member's
+// box lies inside its group's, and every pair is tested again with exactly
+// the bound the walk has when it reaches it); the link's value (the value at
+// the first hit in obstacle order, or the minimum), the warm pair of the next
+// link and the verdict are those of link_min.  A configuration that goes on
+// keeps its running minimum, warm pair and screening flag in st_d / st_w and
+// is appended to the next link's list; the lists are in no particular order
+// between waves (each configuration's results do not depend on it).
+template <bool OBB>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_links(
+    RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
+    uint32_t* count, int link, const int32_t* __restrict__ in, const uint32_t* __restrict__ in_count,
+    int32_t* __restrict__ next_list, uint32_t* next_count, float* __restrict__ st_d, uint32_t* __restrict__ st_w,
+    size_t cull_off, size_t queue_off) {
+    // configurations at this link: pass 0 takes all n (in = the walk order, or null)
+    const int64_t cnt = link == 0 ? n : (int64_t)*in_count;
+    const int B = blockDim.x;
+    if ((int64_t)blockIdx.x * B >= cnt) return;   // no chunk for this workgroup (uniform)
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* const lcull = reinterpret_cast<float*>(smem + cull_off);   // [M][6] cull boxes, for the queued pairs
+    for (int k = threadIdx.x; k < 6 * env.num_obj; k += B) lcull[k] = env.cull[k];
+    const Staged<float> s = stage_env<float, OBB>(env, smem);   // ends with a barrier
+    int16_t* const ring = reinterpret_cast<int16_t*>(smem + queue_off) + threadIdx.x;   // slot k: ring[k * B]
+    const const_ptr<float> gcull = as_const(env.gcull);
+    const const_ptr<float> ccull = as_const(env.cull);
+    const const_ptr<int32_t> gorder = as_const(env.gorder);
+    const int M = env.num_obj;
+    const int L = rb.num_links;
+    const int D = rb.num_dofs;
+    const int ng = num_groups(M);
+    const Params<float> p{qp.eq, qp.maxd};
+    const float band = qp.band;
+    const int i = link;
+    const float r = rb.radius[i];
+    const bool gnd_skip = i < rb.ground_included;
+    const float b0 = cull_bound(0.0f, r, p.eq);
+    const int lane = __lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * B + threadIdx.x) >> 6;
+    const int64_t waves = ((int64_t)gridDim.x * B) >> 6;
+
+    for (int64_t base = wave * 64; base < cnt; base += waves * 64) {
+        const int64_t e = base + lane;
+        const bool busy = e < cnt;
+        int64_t c = 0;
+        float d = p.maxd;
+        int warm = -1;
+        bool uncertain = false;
+        Vec3<float> P0{0.0f, 0.0f, 0.0f}, P1{0.0f, 0.0f, 0.0f};
+        if (busy) {
+            c = in ? (int64_t)in[e] : e;
+            if (i > 0) {
+                d = st_d[c];
+                const uint32_t sw = st_w[c];
+                warm = (int)(sw >> 1) - 1;
+                uncertain = (sw & 1u) != 0;
+            }
+            const float* qc = q + c * D;
+            FKf fk;
+            fk.M = Mat3f{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+            fk.px = fk.py = fk.pz = 0.0f;
+            fk.seg = 0;
+            fk.qi = 0;
+            P0 = skelf_point(fk, rb, qc, i);
+            P1 = skelf_point(fk, rb, qc, i + 1);
+            if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
+        }
+        Vec3<float> slo, shi;
+        seg_bounds(P0, P1, slo, shi);
+        auto eval = [&](int j) {
+            return (s.gnd[j] && gnd_skip) ? p.maxd : flat_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+        };
+        // 2. the warm pair
+        float prof = p.maxd, vh = 0.0f;
+        int best = -1, jh = -1;   // jh: the smallest hit index found (-1: walking)
+        const int w = busy ? warm : -1;
+        if (w >= 0) {
+            const float dw = eval(w);
+            if (dw <= 0) {
+                jh = w;
+                vh = dw;
+            } else if (dw < prof) {
+                prof = dw;
+                best = w;
+            }
+        }
+        // 3-4. the walk, then the search, its pairs evaluated from the queues
+        int qh = 0, qn = 0;
+        auto step = [&]() {   // one queued pair per lane
+            int task = -1;
+            while (qn > 0 && task < 0) {
+                const int j = ring[(qh & (kLaneQueue - 1)) * B];
+                ++qh;
+                --qn;
+                const bool pass = jh < 0 ? !cull_pair(lcull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))
+                                         : (j < jh && !cull_pair(lcull + 6 * j, slo, shi, b0));
+                if (pass) task = j;
+            }
+            if (task >= 0) {
+                const float dt = eval(task);
+                if (jh < 0) {
+                    if (dt < prof) best = task;
+                    prof = smin(prof, dt);
+                    if (prof <= 0) {
+                        jh = task;
+                        vh = prof;
+                    }
+                } else if (dt <= 0) {
+                    jh = task;
+                    vh = dt;
+                }
+            }
+        };
+        for (int g = 0; g < ng; ++g) {
+            const float gb[6] = {gcull[6 * g], gcull[6 * g + 1], gcull[6 * g + 2],
+                                 gcull[6 * g + 3], gcull[6 * g + 4], gcull[6 * g + 5]};
+            const float bg = jh < 0 ? cull_bound(prof, r, p.eq) : b0;
+            const bool gpass = busy && !cull_pair(gb, slo, shi, bg);
+            if (__ballot(gpass)) {
+                const int m1 = min(M, (g + 1) * kGroupSize);
+                for (int m = g * kGroupSize; m < m1; ++m) {
+                    const int j = gorder[m];
+                    const float cb[6] = {ccull[6 * j], ccull[6 * j + 1], ccull[6 * j + 2],
+                                         ccull[6 * j + 3], ccull[6 * j + 4], ccull[6 * j + 5]};
+                    if (gpass && j != w && (jh < 0 || j < jh) && !cull_pair(cb, slo, shi, bg)) {
+                        ring[((qh + qn) & (kLaneQueue - 1)) * B] = (int16_t)j;
+                        ++qn;
+                    }
+                }
+            }
+            // evaluate once enough lanes hold pairs; before the next group,
+            // every queue must have room for a whole group
+            if (__popcll(__ballot(qn > 0)) >= kDrainLanes) step();
+            while (__ballot(qn > kLaneQueue - kGroupSize)) step();
+        }
+        while (__ballot(qn > 0)) step();
+        // the link's value
+        bool finished = false;
+        if (busy) {
+            const bool hit = jh >= 0;
+            const float lp = hit ? vh : prof;
+            if (out.profile) out.profile[c * L + i] = lp;
+            if (hit) {
+                if (out.profile)
+                    for (int k = i + 1; k < L; ++k) out.profile[c * L + k] = 0.0f;
+                finished = true;
+            } else {
+                d = smin(d, lp);
+                finished = i == L - 1;
+            }
+            if (finished) {
+                if (out.d) out.d[c] = hit ? 0.0f : d;
+                if (out.verdict) out.verdict[c] = hit ? 0 : 1;
+            } else {
+                st_d[c] = d;
+                st_w[c] = ((uint32_t)(best + 1) << 1) | (uncertain ? 1u : 0u);
+            }
+        }
+        append_flagged(busy && finished && uncertain, c, idx, count);
+        append_flagged(busy && !finished, c, next_list, next_count);
     }
 }
 
@@ -905,15 +1137,15 @@ size_t device_lds_limit(int dev) {
 namespace {
 
 // LDS of the per-lane-walk kernel for workgroups of `block` threads: the
-// staged obstacles, then (16-byte aligned) the broad-phase tables, then the
+// staged obstacles, then (16-byte aligned) a copy of the cull boxes, then the
 // lanes' pair queues; 0 when it cannot run on this obstacle set.
-size_t lanes_lds(const EnvDev& env, int block, size_t* walk_off, size_t* queue_off) {
+size_t lanes_lds(const EnvDev& env, int block, size_t* cull_off, size_t* queue_off) {
     if (!env.gorder || !env.gcull || env.num_obj > 32767) return 0;
-    const size_t woff = (staged_bytes<float>(env) + 15) & ~size_t(15);
-    const size_t qoff = (woff + walk_tables_bytes(env.num_obj) + 15) & ~size_t(15);
+    const size_t coff = (staged_bytes<float>(env) + 15) & ~size_t(15);
+    const size_t qoff = coff + (size_t)env.num_obj * 6 * sizeof(float);
     const size_t total = qoff + (size_t)kLaneQueue * block * sizeof(int16_t);
     if (total > device_lds_limit()) return 0;
-    if (walk_off) *walk_off = woff;
+    if (cull_off) *cull_off = coff;
     if (queue_off) *queue_off = qoff;
     return total;
 }
@@ -927,6 +1159,16 @@ int fast_lanes_blocks_per_cu(const EnvDev& env, int block) {
     const hipError_t e =
         env.use_obb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_lanes<true>, block, sm)
                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_lanes<false>, block, sm);
+    return e == hipSuccess ? nb : 0;
+}
+
+int fast_links_blocks_per_cu(const EnvDev& env, int block) {
+    const size_t sm = lanes_lds(env, block, nullptr, nullptr);
+    if (!sm) return 0;
+    int nb = 0;
+    const hipError_t e =
+        env.use_obb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_links<true>, block, sm)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_distance_fast_links<false>, block, sm);
     return e == hipSuccess ? nb : 0;
 }
 
@@ -984,25 +1226,47 @@ bool rows_keep_occupancy(KRows krows, KPlain kplain, int block, size_t rows_sm, 
 template <bool OBB, bool GROUPED>
 hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q, int64_t n,
                              const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
-                             const int32_t* perm, int num_cus, hipStream_t st) {
+                             const int32_t* perm, uint32_t* links, int num_cus, hipStream_t st) {
     const size_t sm = staged_bytes<float>(env);
-    if (t.variant == 3 && GROUPED) {
+    if (t.variant == 4 && GROUPED && links && n <= INT32_MAX) {
+        // one launch per link over the configurations alive at it
+        const int BL = (t.block == 512 || t.block == 1024) ? t.block : 256;
+        size_t cull_off = 0, queue_off = 0;
+        const size_t lsm = lanes_lds(env, BL, &cull_off, &queue_off);
+        if (lsm) {
+            const LinksLayout ll = links_layout(links, n);
+            int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : fast_links_blocks_per_cu(env, BL);
+            per_cu = std::max(1, per_cu);
+            const int64_t blocks = std::min<int64_t>((int64_t)num_cus * per_cu, (n + BL - 1) / BL);
+            const dim3 g((unsigned)std::max<int64_t>(1, blocks)), b(BL);
+            hipError_t e = launch_zero_u32(ll.count, kMaxLinks + 1, st);
+            for (int i = 0; i < rb.num_links && e == hipSuccess; ++i) {
+                const int32_t* in = i == 0 ? perm : ll.list[(i - 1) & 1];
+                hipLaunchKernelGGL((k_distance_fast_links<OBB>), g, b, lsm, st, rb, env, qp, q, n, out, idx, count, i,
+                                   in, ll.count + i, ll.list[i & 1], ll.count + i + 1, ll.d, ll.w, cull_off,
+                                   queue_off);
+                e = hipGetLastError();
+            }
+            return e;
+        }
+    }
+    if ((t.variant == 3 || t.variant == 4) && GROUPED) {
         // persistent per-lane walks; count[32] is the configuration counter
         const int BL = (t.block == 512 || t.block == 1024) ? t.block : 256;
-        size_t walk_off = 0, queue_off = 0;
-        const size_t lsm = lanes_lds(env, BL, &walk_off, &queue_off);
+        size_t cull_off = 0, queue_off = 0;
+        const size_t lsm = lanes_lds(env, BL, &cull_off, &queue_off);
         if (lsm) {
             int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : fast_lanes_blocks_per_cu(env, BL);
             per_cu = std::max(1, per_cu);
             const int64_t blocks = std::min<int64_t>((int64_t)num_cus * per_cu, (n + BL - 1) / BL);
             const dim3 g((unsigned)std::max<int64_t>(1, blocks)), b(BL);
             hipLaunchKernelGGL((k_distance_fast_lanes<OBB>), g, b, lsm, st, rb, env, qp, q, n, out, idx, count,
-                               count + 32, perm, walk_off, queue_off);
+                               count + 32, perm, cull_off, queue_off);
             return hipGetLastError();
         }
     }
     const int B = (t.block == 64 || t.block == 128) ? t.block : kBlock;
-    if (t.variant == 1 || t.variant == 3) {
+    if (t.variant == 1 || t.variant == 3 || t.variant == 4) {
         // persistent lane-refill variant; count[32] is its configuration counter
         int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : fast_pool_blocks_per_cu(env, B);
         per_cu = std::max(1, per_cu);
@@ -1059,14 +1323,14 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
 
 hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                 int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
-                                const int32_t* perm, int num_cus, hipStream_t st) {
+                                const int32_t* perm, uint32_t* links, int num_cus, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const bool warm = env.gorder && env.gcull;   // the grouped walk when the scene has group tables
     if (env.use_obb)
-        return warm ? launch_fast_shape<true, true>(rb, env, qp, q, n, out, idx, count, t, perm, num_cus, st)
-                    : launch_fast_shape<true, false>(rb, env, qp, q, n, out, idx, count, t, perm, num_cus, st);
-    return warm ? launch_fast_shape<false, true>(rb, env, qp, q, n, out, idx, count, t, perm, num_cus, st)
-                : launch_fast_shape<false, false>(rb, env, qp, q, n, out, idx, count, t, perm, num_cus, st);
+        return warm ? launch_fast_shape<true, true>(rb, env, qp, q, n, out, idx, count, t, perm, links, num_cus, st)
+                    : launch_fast_shape<true, false>(rb, env, qp, q, n, out, idx, count, t, perm, links, num_cus, st);
+    return warm ? launch_fast_shape<false, true>(rb, env, qp, q, n, out, idx, count, t, perm, links, num_cus, st)
+                : launch_fast_shape<false, false>(rb, env, qp, q, n, out, idx, count, t, perm, links, num_cus, st);
 }
 
 }  // namespace rvs

@@ -519,6 +519,135 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
     return d_c - radius;
 }
 
+// Capsule-box distance of the relaxed bulk pass without the case tree: a
+// fixed sequence of operations, so that lanes evaluating different pairs in
+// one step (the per-lane walk) do not split across the tree's branches.
+// Away from contact the reference's case tree returns the exact segment-box
+// distance minus r; at contact it returns 0 when an end point lies inside
+// the box (CollisionAndDistance.cpp:414-418; a point counts when its
+// coordinates are all within the box and one strictly) and -r when the
+// segment crosses it (:447 after :525 or :584-587).  Here:
+//   inside: the larger of the end points' depths (the smallest distance of
+//     a coordinate to its slab bound) is > 0: the value is 0, uncertain
+//     within band of the surface;
+//   crossing: the segment meets the box shrunk by band on every side (then
+//     it crosses the box whatever the rounding): the value is -r;
+//   otherwise the exact distance raw of the segment to the box, minus r,
+//     uncertain when raw < EQ + band (the snap to -r, a touching crossing,
+//     the tree's face-plane path), |raw - r| < band, the segment nearly
+//     parallel to a box axis within 1 cm (the reference's segment solves
+//     are ill-conditioned there) or of zero length (the reference's
+//     projections are 0/0 there).
+// raw: the squared distance of P(t) = A + t D to the box is convex with
+// derivative g(t) = sum_k D_k (P_k(t) - clamp(P_k(t))), continuous,
+// piecewise linear and non-decreasing, with breaks where a coordinate meets a
+// slab bound.  Its root lies between the largest break point t_a with
+// g <= 0 and the smallest t_b with g > 0 (0 and 1 included), and g is linear
+// between them: t* = -sum c_k D_k / sum D_k^2 over the axes outside their
+// slab there (c_k = A_k - bound), clamped to [t_a, t_b].
+template <class MP>
+__host__ __device__ inline float flat_capsule_box(const Vec3<float>& A, const Vec3<float>& B, float r,
+                                                  const Box<float>& b, const Params<float>& p, float band,
+                                                  bool& uncertain) {
+    const float dA = fminf(fminf(fminf(A.x - b.lo[0], b.hi[0] - A.x), fminf(A.y - b.lo[1], b.hi[1] - A.y)),
+                           fminf(A.z - b.lo[2], b.hi[2] - A.z));
+    const float dB = fminf(fminf(fminf(B.x - b.lo[0], b.hi[0] - B.x), fminf(B.y - b.lo[1], b.hi[1] - B.y)),
+                           fminf(B.z - b.lo[2], b.hi[2] - B.z));
+    const float din = fmaxf(dA, dB);
+    const float Dx = B.x - A.x, Dy = B.y - A.y, Dz = B.z - A.z;
+    const float dd = Dx * Dx + (Dy * Dy + Dz * Dz);
+    const float ix = MP::div(1.0f, Dx), iy = MP::div(1.0f, Dy), iz = MP::div(1.0f, Dz);
+    // crossing of the box shrunk by band (slab clipping; an axis the segment
+    // does not move along must lie inside the shrunk slab)
+    float t0 = 0.0f, t1 = 1.0f;
+    bool in_slabs = true;
+    {
+        const float lx = b.lo[0] + band, hx = b.hi[0] - band;
+        const float ly = b.lo[1] + band, hy = b.hi[1] - band;
+        const float lz = b.lo[2] + band, hz = b.hi[2] - band;
+        if (Dx != 0.0f) {
+            const float a = (lx - A.x) * ix, c = (hx - A.x) * ix;
+            t0 = fmaxf(t0, fminf(a, c));
+            t1 = fminf(t1, fmaxf(a, c));
+        } else {
+            in_slabs = in_slabs && A.x > lx && A.x < hx;
+        }
+        if (Dy != 0.0f) {
+            const float a = (ly - A.y) * iy, c = (hy - A.y) * iy;
+            t0 = fmaxf(t0, fminf(a, c));
+            t1 = fminf(t1, fmaxf(a, c));
+        } else {
+            in_slabs = in_slabs && A.y > ly && A.y < hy;
+        }
+        if (Dz != 0.0f) {
+            const float a = (lz - A.z) * iz, c = (hz - A.z) * iz;
+            t0 = fmaxf(t0, fminf(a, c));
+            t1 = fminf(t1, fmaxf(a, c));
+        } else {
+            in_slabs = in_slabs && A.z > lz && A.z < hz;
+        }
+    }
+    const bool deep = in_slabs && t0 <= t1 && b.hi[0] - b.lo[0] > 2.0f * band && b.hi[1] - b.lo[1] > 2.0f * band &&
+                      b.hi[2] - b.lo[2] > 2.0f * band;
+    // derivative of half the squared distance at t
+    auto g = [&](float t) {
+        const float px = A.x + t * Dx, py = A.y + t * Dy, pz = A.z + t * Dz;
+        const float ex = px - fminf(fmaxf(px, b.lo[0]), b.hi[0]);
+        const float ey = py - fminf(fmaxf(py, b.lo[1]), b.hi[1]);
+        const float ez = pz - fminf(fmaxf(pz, b.lo[2]), b.hi[2]);
+        return Dx * ex + (Dy * ey + Dz * ez);
+    };
+    float ta = 0.0f, tb = 1.0f;   // the bracket
+    const float g0 = g(0.0f), g1 = g(1.0f);
+    auto consider = [&](float t) {
+        if (t > 0.0f && t < 1.0f) {
+            const float gt = g(t);
+            if (gt <= 0.0f) ta = fmaxf(ta, t);
+            else tb = fminf(tb, t);
+        }
+    };
+    if (Dx != 0.0f) { consider((b.lo[0] - A.x) * ix); consider((b.hi[0] - A.x) * ix); }
+    if (Dy != 0.0f) { consider((b.lo[1] - A.y) * iy); consider((b.hi[1] - A.y) * iy); }
+    if (Dz != 0.0f) { consider((b.lo[2] - A.z) * iz); consider((b.hi[2] - A.z) * iz); }
+    float ts;
+    if (g0 >= 0.0f) {
+        ts = 0.0f;
+    } else if (g1 <= 0.0f) {
+        ts = 1.0f;
+    } else {
+        // the axes outside their slab on (ta, tb), at its midpoint
+        const float tm = 0.5f * (ta + tb);
+        const float px = A.x + tm * Dx, py = A.y + tm * Dy, pz = A.z + tm * Dz;
+        const float cx = px < b.lo[0] ? A.x - b.lo[0] : (px > b.hi[0] ? A.x - b.hi[0] : 0.0f);
+        const float cy = py < b.lo[1] ? A.y - b.lo[1] : (py > b.hi[1] ? A.y - b.hi[1] : 0.0f);
+        const float cz = pz < b.lo[2] ? A.z - b.lo[2] : (pz > b.hi[2] ? A.z - b.hi[2] : 0.0f);
+        const float wx = px < b.lo[0] || px > b.hi[0] ? Dx : 0.0f;
+        const float wy = py < b.lo[1] || py > b.hi[1] ? Dy : 0.0f;
+        const float wz = pz < b.lo[2] || pz > b.hi[2] ? Dz : 0.0f;
+        const float num = cx * wx + (cy * wy + cz * wz);
+        const float den = wx * wx + (wy * wy + wz * wz);
+        ts = den > 0.0f ? fminf(fmaxf(-MP::div(num, den), ta), tb) : ta;
+    }
+    const float px = A.x + ts * Dx, py = A.y + ts * Dy, pz = A.z + ts * Dz;
+    const float ex = px - fminf(fmaxf(px, b.lo[0]), b.hi[0]);
+    const float ey = py - fminf(fmaxf(py, b.lo[1]), b.hi[1]);
+    const float ez = pz - fminf(fmaxf(pz, b.lo[2]), b.hi[2]);
+    const float raw = MP::root(ex * ex + (ey * ey + ez * ez));
+    // screening and the value
+    if (din > 0.0f) {
+        uncertain |= din < band;
+        return 0.0f;
+    }
+    // an end point within band of the surface: the reference's inside test
+    // (which also counts points on a face) could go either way
+    uncertain |= din > -band;
+    if (deep) return -r;
+    const float dt = raw - r;
+    const float smin2 = fminf(fminf(Dy * Dy + Dz * Dz, Dx * Dx + Dz * Dz), Dx * Dx + Dy * Dy);   // dd sin^2 to the nearest axis
+    uncertain |= (raw < p.eq + band) || (fabsf(dt) < band) || !(dd > 0.0f) || (raw < 0.01f && smin2 < 1e-6f * dd);
+    return dt;
+}
+
 // ---------------------------------------------------------------- collision
 // checkCases, CollisionAndDistance.cpp:113-148: distance of AB to the one or
 // two rectangle edges on the side of point 'pt' (2-D coordinates u, v).

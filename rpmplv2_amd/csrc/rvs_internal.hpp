@@ -97,8 +97,9 @@ hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const Quer
 struct FastTuning {
     int32_t variant;        // 0: one lane per configuration; 1: persistent lane refill;
                             // 2: one lane per configuration in persistent waves over 64-configuration chunks;
-                            // 3: persistent lane refill with every lane walking its own pairs
-    int32_t block;          // threads per workgroup: 64, 128 or 256 (variant 3: 256, 512 or 1024)
+                            // 3: persistent lane refill with every lane queueing its own pairs;
+                            // 4: one launch per link over the configurations alive at it
+    int32_t block;          // threads per workgroup: 64, 128 or 256 (variants 3 and 4: 256, 512 or 1024)
     int32_t blocks_per_cu;  // lane refill: resident workgroups per CU, 0 = the occupancy limit
     int32_t coh_bits;       // walk order: 2^bits bins per joint angle of the first 18 / bits joints, 0 = off
 };
@@ -149,20 +150,46 @@ inline CohLayout coh_layout(uint32_t* base, int64_t n) {
 // otherwise *perm is null (the order given).
 hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n, const FastTuning& t, uint32_t* coh,
                                   const int32_t** perm, hipStream_t st);
-// The persistent variants (1: lane refill, 2: chunks, 3: per-lane walks)
+// Scratch of the link-major bulk pass (tuning variant 4), 256-byte aligned:
+// two lists of the configurations alive at a link (used in turn), each
+// configuration's running minimum and (warm pair + 1) << 1 | screening flag
+// between links, and the lists' sizes per link.
+struct LinksLayout {
+    int32_t* list[2];
+    float* d;
+    uint32_t* w;
+    uint32_t* count;   // kMaxLinks + 1
+};
+inline size_t fast_links_bytes(int64_t n) {
+    const size_t m = (size_t)std::max<int64_t>(n, 0);
+    return 4 * (4 * coh_words(m) + coh_words(kMaxLinks + 1));
+}
+inline LinksLayout links_layout(uint32_t* base, int64_t n) {
+    const size_t m = coh_words((size_t)std::max<int64_t>(n, 0));
+    LinksLayout L;
+    L.list[0] = reinterpret_cast<int32_t*>(base);
+    L.list[1] = L.list[0] + m;
+    L.d = reinterpret_cast<float*>(L.list[1] + m);
+    L.w = reinterpret_cast<uint32_t*>(L.d + m);
+    L.count = L.w + m;
+    return L;
+}
+// The persistent variants (1: lane refill, 2: chunks, 3: per-lane queues)
 // use count[32] as their work counter (expected zero, like count[0]).
-// perm: the walk order (may be null).  Variant 3 needs the scene's group
-// tables and its broad-phase tables in LDS; without them it runs as
-// variant 1.
+// perm: the walk order (may be null).  Variants 3 and 4 need the scene's
+// group tables and their queues in LDS, and variant 4 the links scratch of
+// fast_links_bytes(n) bytes (links, may be null); without them variant 4
+// runs as 3, and 3 as 1.
 hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                 int64_t n, const DistOut& out, int32_t* idx, uint32_t* count, const FastTuning& t,
-                                const int32_t* perm,
-                                int num_cus, hipStream_t st);
+                                const int32_t* perm, uint32_t* links, int num_cus, hipStream_t st);
 // Resident workgroups per CU of the lane-refill kernel for this obstacle set.
 int fast_pool_blocks_per_cu(const EnvDev& env, int block);
 // The same for the per-lane-walk kernel (0 when it does not run on this
 // obstacle set: no group tables, or its tables exceed the LDS limit).
 int fast_lanes_blocks_per_cu(const EnvDev& env, int block);
+// The same for the link-major kernel.
+int fast_links_blocks_per_cu(const EnvDev& env, int block);
 // Workgroup LDS of device dev, or of the current device when dev < 0
 // (hipDeviceAttributeMaxSharedMemoryPerBlock, read once per device).
 size_t device_lds_limit(int dev = -1);

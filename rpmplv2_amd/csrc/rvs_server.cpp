@@ -291,7 +291,9 @@ hipError_t issue(rvs_server* s, Batch& b, const Layout& lay) {
             const FastTuning ft = s->ctx->tuning_for(h.r.robot, h.r.env, L, M);
             if (e == hipSuccess) e = launch_zero_u32(count, 64, st);
             if (e == hipSuccess)
-                e = launch_distance_fast(rb, env, s->qp, q, N, out, idx, count, ft, nullptr, s->ctx->num_cus, st);   // order given
+                // order given; no links scratch (a link-major tuning runs as variant 3)
+                e = launch_distance_fast(rb, env, s->qp, q, N, out, idx, count, ft, nullptr, nullptr, s->ctx->num_cus,
+                                         st);
             if (e == hipSuccess) e = launch_distance_refine(rb, env, s->qp, q, N, out, idx, count, st);
         } else if (e == hipSuccess) {
             e = launch_distance_ref32_auto(rb, env, s->qp, q, N, out, s->ctx->wave_max, st);

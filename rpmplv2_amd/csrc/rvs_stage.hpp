@@ -70,35 +70,6 @@ __device__ inline Staged<T> stage_env(const EnvDev& env, unsigned char* smem) {
     return s;
 }
 
-// LDS image of the grouped broad phase's tables, for kernels whose lanes
-// walk the obstacles independently (every lane reads its own entry):
-//   cull   [M][6]   box containing each obstacle
-//   gcull  [G][6]   union box of each group of kGroupSize walk-order entries
-//   gorder [M]      walk order (obstacle index per position)
-struct WalkTables {
-    const float* cull;
-    const float* gcull;
-    const int16_t* gorder;
-};
-
-__host__ __device__ inline size_t walk_tables_bytes(int M) {
-    return (size_t)M * 6 * sizeof(float) + (size_t)num_groups(M) * 6 * sizeof(float) + (size_t)M * sizeof(int16_t);
-}
-
-// Copy the tables into LDS at base (whole workgroup participates; the
-// scene must have group tables, and M <= 32767).
-__device__ inline WalkTables stage_walk(const EnvDev& env, unsigned char* base) {
-    const int M = env.num_obj, G = num_groups(M);
-    float* cull = reinterpret_cast<float*>(base);
-    float* gcull = cull + 6 * M;
-    int16_t* gorder = reinterpret_cast<int16_t*>(gcull + 6 * G);
-    for (int k = threadIdx.x; k < 6 * M; k += blockDim.x) cull[k] = env.cull[k];
-    for (int k = threadIdx.x; k < 6 * G; k += blockDim.x) gcull[k] = env.gcull[k];
-    for (int k = threadIdx.x; k < M; k += blockDim.x) gorder[k] = (int16_t)env.gorder[k];
-    __syncthreads();
-    return WalkTables{cull, gcull, gorder};
-}
-
 // World point -> box frame: local_k = R[0][k] (x - c_x) + R[1][k] (y - c_y)
 // + R[2][k] (z - c_z), summed left to right (x = centre, rotation rows).
 template <typename T>

@@ -41,7 +41,8 @@ int run(long n, uint64_t seed, const char* tag) {
     orc_params op{1e-4f, 1000.0f, 10};
     Params<T> p{(T)op.eq_threshold, (T)op.max_distance};
     long bad_cb = 0, bad_np = 0, bad_col = 0, bad_ss = 0, bad_sp = 0, bad_pl = 0, bad_ax = 0, flag_ax = 0;
-    double max_ax = 0;
+    long bad_flat = 0, flag_flat = 0;
+    double max_ax = 0, max_flat = 0;
     long paths[10] = {0};
     for (long it = 0; it < n; ++it) {
         // box
@@ -108,6 +109,28 @@ int run(long n, uint64_t seed, const char* tag) {
                 if (e > max_ax) max_ax = e;
             }
         }
+        if constexpr (sizeof(T) == 4) {
+            // the flat evaluation of the per-lane walk: where it does not flag
+            // the pair, the contact outputs (0 inside, -r crossing) are the
+            // reference's exactly and other values within 2e-6 m; every NaN of
+            // the tree is flagged
+            bool unc = false;
+            const float gf = flat_capsule_box<AxisMath>(a, bb, r, b, p, 1e-4f, unc);
+            if (unc) {
+                flag_flat++;
+            } else {
+                const bool contact = od == 0.0f || od == -r;
+                const double e = std::fabs((double)gf - (double)od);
+                const bool ok = !std::isnan(od) && (contact ? gf == od : e <= 2e-6);
+                if (!ok) {
+                    if (bad_flat < 5)
+                        printf("[%s] flat mismatch it=%ld mode=%d path=%d flat=%.9g oracle=%.9g r=%.9g\n", tag, it, mode,
+                               opath, (double)gf, (double)od, (double)r);
+                    bad_flat++;
+                }
+                if (!contact && e > max_flat) max_flat = e;
+            }
+        }
         // collision tree
         T rn = (T)((double)r * std::sqrt(3.0) / 3.0);
         int oc;
@@ -138,8 +161,9 @@ int run(long n, uint64_t seed, const char* tag) {
            bad_np, bad_col, bad_ss, bad_sp);
     for (int k = 0; k < 10; ++k) printf(" %ld", paths[k]);
     if (sizeof(T) == 4) printf(" axis_edges=%ld (flagged %ld, max error %.3g m)", bad_ax, flag_ax, max_ax);
+    if (sizeof(T) == 4) printf(" flat=%ld (flagged %ld, max error %.3g m)", bad_flat, flag_flat, max_flat);
     printf("\n");
-    return (bad_cb || bad_np || bad_col || bad_ss || bad_sp || bad_ax) ? 1 : 0;
+    return (bad_cb || bad_np || bad_col || bad_ss || bad_sp || bad_ax || bad_flat) ? 1 : 0;
 }
 
 int main(int argc, char** argv) {

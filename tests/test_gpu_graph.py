@@ -10,7 +10,7 @@ from rpmplv2_amd import rvs, workloads as W
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 3, 4])
 def test_graph_replays_match_eager(engine, variant):
     w = W.workload2(1 << 16)
     rid = engine.load_robot(w.robot)

@@ -375,21 +375,21 @@ def _engine_with_variant(v):
 @pytest.fixture(scope="module")
 def engines_by_variant():
     """Engines whose FP32 bulk pass is the lane-refill ("pool"), the
-    one-lane-per-configuration ("classic") or the per-lane-walk ("lanes")
-    kernel."""
-    e = {"pool": _engine_with_variant("1"), "classic": _engine_with_variant("0"), "lanes": _engine_with_variant("3")}
+    one-lane-per-configuration ("classic"), the lane-refill with per-lane
+    pair queues ("lanes") or the link-major ("links") kernel."""
+    e = {"pool": _engine_with_variant("1"), "classic": _engine_with_variant("0"), "lanes": _engine_with_variant("3"),
+         "links": _engine_with_variant("4")}
     yield e
     for x in e.values():
         x.close()
 
 
-@pytest.mark.parametrize("variant", ["pool", "classic", "lanes"])
+@pytest.mark.parametrize("variant", ["pool", "classic", "lanes", "links"])
 @pytest.mark.parametrize("wl", ["workload1", "workload2"])
 def test_fp32_refine_verdict_exact(oracle, engines_by_variant, wl, variant):
     """FP32 bulk pass + exact refinement: verdicts identical to the reference
     arithmetic, distances within 1e-5 m of the FP64 oracle (and profiles of
-    collision-free configurations too), for the lane-refill and the
-    one-lane-per-configuration and the per-lane-walk bulk kernels."""
+    collision-free configurations too), for every bulk kernel."""
     engine = engines_by_variant[variant]
     w = W.workload1(65536) if wl == "workload1" else W.workload2(1 << 18)
     rid = engine.load_robot(w.robot)
@@ -408,6 +408,35 @@ def test_fp32_refine_verdict_exact(oracle, engines_by_variant, wl, variant):
     bad = ver == 0
     assert np.abs(prof[bad] - oprof[bad]).max() <= 1e-5
     assert refined < 0.05 * len(w.q)
+
+
+@pytest.mark.parametrize("variant", ["pool", "classic", "lanes", "links"])
+@pytest.mark.parametrize("scene", ["dense_overlap", "dense_overlap_wide"])
+def test_fp32_refine_first_hit_in_obstacle_order(oracle, engines_by_variant, scene, variant):
+    """Overlapping boxes make links cross several boxes, so the walk often
+    finds a hit that is not the first in obstacle order; the first one's
+    value (0 for a crossing, about -r for an inside end point) is the
+    profile's.  Every bulk kernel (obstacle-order search before the hit, or
+    the link-major kernel's search of the walk positions after it) gives the
+    oracle's verdicts, and its profiles within 1e-5 m, with non-finite joint
+    values among the configurations."""
+    engine = engines_by_variant[variant]
+    wide = scene == "dense_overlap_wide"
+    rng = np.random.default_rng(23 if wide else 21)
+    rb = R.chain([0.1] + [0.25] * 5, radius=0.03)
+    sc = W._boxes(rng, 48, 0.2, 0.5, 0.9, (0, 0, 0.5)) if wide else W._boxes(rng, 24, 0.1, 0.4, 0.9, (0, 0, 0.5))
+    q = rng.uniform(-np.pi, np.pi, (65536, 6)).astype(np.float32)
+    q[::101, 2] = np.nan
+    rid = engine.load_robot(rb)
+    eid = engine.create_env(sc)
+    print(f"{scene} {variant}: {len(q)} configurations", flush=True)
+    d, prof, ver, _ = engine.distance(rid, eid, q, precision=rvs.PREC_FP32_REFINE)
+    od, oprof, over, _ = oracle.distance(rb, oracle.scene_of(sc), q, 32)
+    assert (ver == over).all()
+    assert np.array_equal(np.isnan(prof), np.isnan(oprof))
+    m = ~np.isnan(prof)
+    assert np.abs(prof[m] - oprof[m]).max() <= 1e-5
+    assert (oprof <= 0).sum() > 10000 and len(np.unique(oprof[oprof <= 0])) > 100
 
 
 @pytest.mark.parametrize("which", ["xarm6", "chain22", "planar2"])

@@ -1,7 +1,9 @@
 """Autotuner of the FP32 bulk pass: every launch shape (one lane per
-configuration, lane refill, persistent chunks, per-lane walks) and walk order gives the same
+configuration, lane refill, persistent chunks) and walk order gives the same
 per-configuration results (they only change which lane evaluates a
-configuration), and rvs_autotune keeps one of the candidate shapes."""
+configuration); the per-lane walks, which evaluate pairs without the case
+tree, give the same verdicts and distances within FP32 rounding; and
+rvs_autotune keeps one of the candidate shapes."""
 import numpy as np
 import pytest
 
@@ -20,24 +22,37 @@ def test_launch_shapes_agree(engine, wl):
         sem = rvs.ENV_OBB
     rid = engine.load_robot(w.robot)
     eid = engine.create_env(w.scene, sem)
-    ref = None
+    # bit-identical within a pair-evaluation form and search: the case tree
+    # (variants 0-2); the flat form with the restarted search (3) and with
+    # the search continued from the hit (4), whose screening flags differ
+    refs = {}
+    form_of = lambda v: "tree" if v < 3 else v
     for coh in (0, 5, 3):
-        for variant in (0, 1, 2, 3):
-            for block in ((256, 512, 1024) if variant == 3 else (64, 128, 256)):
+        for variant in (0, 1, 2, 3, 4):
+            for block in ((256, 512, 1024) if variant >= 3 else (64, 128, 256)):
                 for per_cu in ((0,) if variant == 0 else (0, 1)):
                     engine.set_tuning(rid, eid, variant, block, per_cu, coh)
                     d, prof, ver, _ = engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
-                    if ref is None:
-                        ref = (d, prof, ver)
+                    form = form_of(variant)
+                    shape = (variant, block, per_cu, coh)
+                    if form not in refs:
+                        refs[form] = (d, prof, ver)
                     else:
-                        shape = (variant, block, per_cu, coh)
+                        ref = refs[form]
                         assert np.array_equal(ver, ref[2]), shape
                         assert np.array_equal(d, ref[0]) and np.array_equal(prof, ref[1]), shape
+    # across forms: the same verdicts, distances and profiles within FP32 rounding
+    d0, p0, v0 = refs["tree"]
+    for form in (3, 4):
+        d1, p1, v1 = refs[form]
+        assert np.array_equal(v0, v1), form
+        assert np.abs(d0 - d1).max() <= 1e-6 and np.abs(p0 - p1).max() <= 1e-6, form
     best = engine.autotune(rid, eid, w.q)
     print(wl, "autotune:", best)
-    assert best["variant"] in (0, 1, 2, 3) and best["block"] in (64, 128, 256, 512, 1024) and best["ms"] > 0
+    assert best["variant"] in (0, 1, 2, 3, 4) and best["block"] in (64, 128, 256, 512, 1024) and best["ms"] > 0
     assert engine.get_tuning(rid, eid)["variant"] == best["variant"]
     d, prof, ver, _ = engine.distance(rid, eid, w.q, precision=rvs.PREC_FP32_REFINE)
+    ref = refs[form_of(best["variant"])]
     assert np.array_equal(ver, ref[2]) and np.array_equal(d, ref[0])
 
 

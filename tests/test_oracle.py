@@ -674,7 +674,7 @@ def test_self_min_matches_pairs(oracle):
 
 # ---------------------------------------------------------------- broad phase
 @pytest.mark.parametrize("name", ["workload1", "workload2", "workload5_obb", "workload5_aabb", "chain_ground",
-                                  "dense_overlap"])
+                                  "dense_overlap", "dense_overlap_wide"])
 def test_broad_phase_changes_nothing(oracle, name):
     """The bulk pass's walk (each link started at the previous link's nearest
     obstacle, then the obstacles in Morton order, groups of 4 and single
@@ -685,12 +685,20 @@ def test_broad_phase_changes_nothing(oracle, name):
     share of the pairs.  dense_overlap packs overlapping boxes around the arm
     so that links often cross several boxes: there a hit found out of order
     often has an earlier hit before it in obstacle order, whose value (0 for
-    a crossing, -r for an inside end point) the profile must take."""
+    a crossing, -r for an inside end point) the profile must take.  Both
+    searches for it: in obstacle order before the hit (tuning variants 0-3)
+    and in the walk positions after the hit (the link-major variant 4).
+    dense_overlap_wide (48 larger boxes) often holds several earlier hits of
+    different values after the hit in walk order, of which the smallest index
+    must win: mistakes planted in the second search (skipping the position
+    after the hit, keeping the first hit found instead of the smallest index,
+    searching only the hit's group) change 12 to 501 of its profiles."""
     oriented = False
-    if name == "dense_overlap":
-        rng = np.random.default_rng(21)
+    if name in ("dense_overlap", "dense_overlap_wide"):
+        wide = name == "dense_overlap_wide"
+        rng = np.random.default_rng(23 if wide else 21)
         rb = R.chain([0.1] + [0.25] * 5, radius=0.03)
-        sc = W._boxes(rng, 24, 0.1, 0.4, 0.9, (0, 0, 0.5))
+        sc = W._boxes(rng, 48, 0.2, 0.5, 0.9, (0, 0, 0.5)) if wide else W._boxes(rng, 24, 0.1, 0.4, 0.9, (0, 0, 0.5))
         q = rng.uniform(-np.pi, np.pi, (8192, 6)).astype(np.float32)
         w = W.ConfigBatch("dense", rb, sc, q)
     elif name == "workload1":
@@ -708,10 +716,11 @@ def test_broad_phase_changes_nothing(oracle, name):
     sc = oracle.scene_of(w.scene, oriented=oriented)
     for prec in (32, 64):
         d, prof, ver, _ = oracle.distance(w.robot, sc, w.q, prec)
-        dc, profc, verc, (ev, tests) = oracle.distance_cull(w.robot, sc, w.q, prec)
-        assert np.array_equal(d, dc) and np.array_equal(prof, profc) and np.array_equal(ver, verc)
-        pairs = len(w.q) * w.robot.num_links * len(w.scene.dim)
-        assert ev < 0.8 * pairs and tests < pairs
+        for from_hit in (False, True):
+            dc, profc, verc, (ev, tests) = oracle.distance_cull(w.robot, sc, w.q, prec, search_from_hit=from_hit)
+            assert np.array_equal(d, dc) and np.array_equal(prof, profc) and np.array_equal(ver, verc), from_hit
+            pairs = len(w.q) * w.robot.num_links * len(w.scene.dim)
+            assert ev < 0.8 * pairs and (from_hit or tests < pairs)
         if name == "dense_overlap":   # the case the warm start must handle is present
             hit_links = (prof <= 0).sum()
             assert hit_links > 1000 and len(np.unique(prof[prof <= 0])) >= 2
