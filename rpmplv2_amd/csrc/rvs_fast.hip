@@ -157,6 +157,37 @@ __device__ __attribute__((always_inline)) inline float fast_pair(const Staged<fl
 
 // One (link, obstacle) pair of the per-lane walk: fast_pair with the flat
 // evaluation (the same filtering and tolerance screening).
+// Read-only tables indexed by wave-uniform values: read through the
+// constant address space, so that they are fetched with scalar loads into
+// SGPRs (the tables are not written while a kernel reads them).
+template <typename T>
+using const_ptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __attribute__((always_inline)) inline const_ptr<T> as_const(const T* p) {
+    return (const_ptr<T>)(p);
+}
+
+// Broad-phase test of the bulk pass: the gap of cull_pair against a squared
+// bound given (kept by the caller while its running minimum is unchanged),
+// each axis's gap as max(lo - shi, slo - hi, 0), summed x, y, z with fused
+// multiply-adds.  For finite end points it decides as cull_pair does; with a
+// non-finite end point it may skip pairs cull_pair would not, but such a
+// configuration is sent to the exact pass, which recomputes it whole.
+template <class P>
+__device__ __attribute__((always_inline)) inline bool cull_fast(P cb, const Vec3<float>& slo, const Vec3<float>& shi,
+                                                                float bound2) {
+    const float gx = fmaxf(fmaxf(cb[0] - shi.x, slo.x - cb[3]), 0.0f);
+    const float gy = fmaxf(fmaxf(cb[1] - shi.y, slo.y - cb[4]), 0.0f);
+    const float gz = fmaxf(fmaxf(cb[2] - shi.z, slo.z - cb[5]), 0.0f);
+    return fmaf(gz, gz, fmaf(gy, gy, gx * gx)) > bound2;
+}
+
+// Square of cull_bound(prof, r, eq).
+__device__ __attribute__((always_inline)) inline float cull_bound2(float prof, float r, float eq) {
+    const float b = fmaxf(prof + r, eq) + kCullMargin;
+    return b * b;
+}
+
 template <bool OBB>
 __device__ __attribute__((always_inline)) inline float flat_pair(const Staged<float>& s, int j, const Vec3<float>& P0,
                                                                  const Vec3<float>& P1, float r, const Params<float>& p,
@@ -203,17 +234,22 @@ __device__ __attribute__((always_inline)) inline float link_min(const Staged<flo
                                                                 const Vec3<float>& P1, float r, const Params<float>& p,
                                                                 float band, bool active, int& warm, bool& hit,
                                                                 bool& uncertain) {
-    const float* __restrict__ cull = env.cull;
+    // the tables are read with wave-uniform indices (scalar loads)
+    const const_ptr<float> cull = as_const(env.cull);
+    const const_ptr<float> gcull = as_const(env.gcull);
+    const const_ptr<int32_t> gorder = as_const(env.gorder);
     Vec3<float> slo, shi;
     seg_bounds(P0, P1, slo, shi);
     float prof = p.maxd;
+    float bb = cull_bound2(prof, r, p.eq);   // squared bound of the running minimum
     hit = false;
     if (!GROUPED) {   // the plain in-order walk
         bool go = active;
         for (int j = 0; j < M; ++j) {
-            if (go && !cull_pair(cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
+            if (go && !cull_fast(cull + 6 * j, slo, shi, bb)) {
                 const float dt = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
                 prof = smin(prof, dt);
+                bb = cull_bound2(prof, r, p.eq);
                 if (prof <= 0) {
                     hit = true;
                     go = false;
@@ -235,20 +271,22 @@ __device__ __attribute__((always_inline)) inline float link_min(const Staged<flo
         } else if (dt < prof) {
             prof = dt;
             best = w;
+            bb = cull_bound2(prof, r, p.eq);
         }
     }
     bool go = active && jh < 0;
     const int ng = num_groups(M);
     for (int g = 0; g < ng; ++g) {
-        if (go && !cull_pair(env.gcull + 6 * g, slo, shi, cull_bound(prof, r, p.eq))) {
+        if (go && !cull_fast(gcull + 6 * g, slo, shi, bb)) {
             const int m1 = min(M, (g + 1) * kGroupSize);
             for (int m = g * kGroupSize; m < m1; ++m) {
-                const int j = env.gorder[m];
-                if (go && j != w && !cull_pair(cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
+                const int j = gorder[m];
+                if (go && j != w && !cull_fast(cull + 6 * j, slo, shi, bb)) {
                     const float dt =
                         (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
                     if (dt < prof) best = j;
                     prof = smin(prof, dt);
+                    bb = cull_bound2(prof, r, p.eq);
                     if (prof <= 0) {
                         jh = j;
                         vh = prof;
@@ -261,9 +299,9 @@ __device__ __attribute__((always_inline)) inline float link_min(const Staged<flo
     }
     if (jh >= 0) {   // the first hit in obstacle order
         float first = vh;
-        const float b0 = cull_bound(0.0f, r, p.eq);
+        const float bb0 = cull_bound2(0.0f, r, p.eq);
         for (int j = 0; j < jh; ++j) {
-            if (cull_pair(cull + 6 * j, slo, shi, b0)) continue;
+            if (cull_fast(cull + 6 * j, slo, shi, bb0)) continue;
             const float dj = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
             if (dj <= 0) {
                 first = dj;
@@ -547,7 +585,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     const int L = rb.num_links;
     const int D = rb.num_dofs;
     const int ng = num_groups(M);
-    const float* __restrict__ cull = env.cull;
+    const const_ptr<float> cull = as_const(env.cull);
+    const const_ptr<float> gcull = as_const(env.gcull);
+    const const_ptr<int32_t> gorder = as_const(env.gorder);
     const Params<float> p{qp.eq, qp.maxd};
     const float band = qp.band;
     const int lane = __lane_id();
@@ -628,7 +668,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                 const int j = ring[(qh & (kLaneQueue - 1)) * B];
                 ++qh;
                 --qn;
-                if (!cull_pair(lcull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) task = j;
+                if (!cull_fast(lcull + 6 * j, slo, shi, cull_bound2(prof, r, p.eq))) task = j;
             }
             if (task >= 0) {
                 const float dt = eval(task);
@@ -644,11 +684,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         };
         for (int g = 0; g < ng; ++g) {
             if (__ballot(go) == 0) break;
-            if (go && !cull_pair(env.gcull + 6 * g, slo, shi, cull_bound(prof, r, p.eq))) {
+            if (go && !cull_fast(gcull + 6 * g, slo, shi, cull_bound2(prof, r, p.eq))) {
                 const int m1 = min(M, (g + 1) * kGroupSize);
                 for (int m = g * kGroupSize; m < m1; ++m) {
-                    const int j = env.gorder[m];
-                    if (go && j != w && !cull_pair(cull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))) {
+                    const int j = gorder[m];
+                    if (go && j != w && !cull_fast(cull + 6 * j, slo, shi, cull_bound2(prof, r, p.eq))) {
                         ring[((qh + qn) & (kLaneQueue - 1)) * B] = (int16_t)j;
                         ++qn;
                     }
@@ -663,7 +703,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         // 4. the first hit in obstacle order before the hit found
         bool srch = busy && jh >= 0;
         if (__ballot(srch)) {
-            const float b0 = cull_bound(0.0f, r, p.eq);
+            const float bb0 = cull_bound2(0.0f, r, p.eq);
             int fj = jh;
             float fv = vh;
             auto search_step = [&]() {
@@ -682,11 +722,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
             };
             qh = qn = 0;
             for (int g = 0; g < ng; ++g) {
-                if (srch && !cull_pair(env.gcull + 6 * g, slo, shi, b0)) {
+                if (srch && !cull_fast(gcull + 6 * g, slo, shi, bb0)) {
                     const int m1 = min(M, (g + 1) * kGroupSize);
                     for (int m = g * kGroupSize; m < m1; ++m) {
-                        const int j = env.gorder[m];
-                        if (srch && j < fj && !cull_pair(cull + 6 * j, slo, shi, b0)) {
+                        const int j = gorder[m];
+                        if (srch && j < fj && !cull_fast(cull + 6 * j, slo, shi, bb0)) {
                             ring[((qh + qn) & (kLaneQueue - 1)) * B] = (int16_t)j;
                             ++qn;
                         }
@@ -726,16 +766,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         append_flagged(finished && uncertain, c, idx, count);
         if (finished) c = -1;
     }
-}
-
-// Read-only tables indexed by wave-uniform values: read through the
-// constant address space, so that they are fetched with scalar loads into
-// SGPRs (the tables are not written while a kernel reads them).
-template <typename T>
-using const_ptr = const __attribute__((address_space(4))) T*;
-template <typename T>
-__device__ __attribute__((always_inline)) inline const_ptr<T> as_const(const T* p) {
-    return (const_ptr<T>)(p);
 }
 
 // The link-major walk (tuning variant 4): one launch per link index i over
@@ -796,7 +826,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     const int i = link;
     const float r = rb.radius[i];
     const bool gnd_skip = i < rb.ground_included;
-    const float b0 = cull_bound(0.0f, r, p.eq);
+    const float bb0 = cull_bound2(0.0f, r, p.eq);
     const int lane = __lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * B + threadIdx.x) >> 6;
     const int64_t waves = ((int64_t)gridDim.x * B) >> 6;
@@ -846,17 +876,20 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                 best = w;
             }
         }
+        // the lane's squared bound (running minimum, or 0 once searching;
+        // an idle lane's culls everything) and the obstacles it still takes
+        // (all while walking, those below the smallest hit once searching)
+        float bb = !busy ? -1.0f : (jh < 0 ? cull_bound2(prof, r, p.eq) : bb0);
+        int lim = jh < 0 ? M : jh;
         // 3-4. the walk, then the search, its pairs evaluated from the queues
-        int qh = 0, qn = 0;
+        int qh = 0, qt = 0, qn = 0;   // ring head, tail and length
         auto step = [&]() {   // one queued pair per lane
             int task = -1;
             while (qn > 0 && task < 0) {
-                const int j = ring[(qh & (kLaneQueue - 1)) * B];
-                ++qh;
+                const int j = ring[qh * B];
+                qh = (qh + 1) & (kLaneQueue - 1);
                 --qn;
-                const bool pass = jh < 0 ? !cull_pair(lcull + 6 * j, slo, shi, cull_bound(prof, r, p.eq))
-                                         : (j < jh && !cull_pair(lcull + 6 * j, slo, shi, b0));
-                if (pass) task = j;
+                if (j < lim && !cull_fast(lcull + 6 * j, slo, shi, bb)) task = j;
             }
             if (task >= 0) {
                 const float dt = eval(task);
@@ -866,26 +899,31 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                     if (prof <= 0) {
                         jh = task;
                         vh = prof;
+                        lim = task;
+                        bb = bb0;
+                    } else {
+                        bb = cull_bound2(prof, r, p.eq);
                     }
                 } else if (dt <= 0) {
                     jh = task;
                     vh = dt;
+                    lim = task;
                 }
             }
         };
         for (int g = 0; g < ng; ++g) {
             const float gb[6] = {gcull[6 * g], gcull[6 * g + 1], gcull[6 * g + 2],
                                  gcull[6 * g + 3], gcull[6 * g + 4], gcull[6 * g + 5]};
-            const float bg = jh < 0 ? cull_bound(prof, r, p.eq) : b0;
-            const bool gpass = busy && !cull_pair(gb, slo, shi, bg);
+            const bool gpass = !cull_fast(gb, slo, shi, bb);
             if (__ballot(gpass)) {
                 const int m1 = min(M, (g + 1) * kGroupSize);
                 for (int m = g * kGroupSize; m < m1; ++m) {
                     const int j = gorder[m];
                     const float cb[6] = {ccull[6 * j], ccull[6 * j + 1], ccull[6 * j + 2],
                                          ccull[6 * j + 3], ccull[6 * j + 4], ccull[6 * j + 5]};
-                    if (gpass && j != w && (jh < 0 || j < jh) && !cull_pair(cb, slo, shi, bg)) {
-                        ring[((qh + qn) & (kLaneQueue - 1)) * B] = (int16_t)j;
+                    if (gpass && j != w && j < lim && !cull_fast(cb, slo, shi, bb)) {
+                        ring[qt * B] = (int16_t)j;
+                        qt = (qt + 1) & (kLaneQueue - 1);
                         ++qn;
                     }
                 }
