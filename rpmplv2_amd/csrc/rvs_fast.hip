@@ -768,55 +768,193 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     }
 }
 
+// Wave-wide minimum / maximum of a float (non-finite lanes contribute
+// nothing when given as +-infinity by the caller).
+__device__ __attribute__((always_inline)) inline float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __attribute__((always_inline)) inline float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Morton code (10 bits per axis, x lowest) of a point quantized over the
+// box lo + [0, 1024 / scale) per axis (box = lo[3], scale[3]); points
+// outside are clamped, non-finite coordinates give 0.
+__device__ __attribute__((always_inline)) inline uint32_t point_key(const Vec3<float>& P, const float* box) {
+    const float c[3] = {P.x, P.y, P.z};
+    uint32_t key = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float t = (c[a] - box[a]) * box[3 + a];
+        const uint32_t u = (t >= 0.0f) ? (uint32_t)fminf(t, 1023.0f) : 0u;   // NaN compares false: 0
+#pragma unroll
+        for (int b = 0; b < 10; ++b) key |= ((u >> b) & 1u) << (3 * b + a);
+    }
+    return key;
+}
+
+// Set-up of the link-major walk (one workgroup of kBoxThreads):
+//   box[0..5]: the box quantizing the survivors' sort keys, the union of the
+//     scene's group boxes (finite coordinates only) as lo[3], 1024 /
+//     extent[3] (0 for an empty extent);
+//   warm0: the first link's warm pair, the obstacle whose cull box is nearest
+//     to configuration 0's first skeleton point (smallest index on ties; -1
+//     without obstacles).  Any obstacle gives the walk the same results; a
+//     near one gives the first link a tight bound from its first pair.
+constexpr int kBoxThreads = 256;
+__global__ __launch_bounds__(kBoxThreads) void k_links_setup(RobotDev rb, EnvDev env, const float* __restrict__ q,
+                                                             float* __restrict__ box, int32_t* __restrict__ warm0) {
+    __shared__ float red[6][kBoxThreads];
+    __shared__ float bd[kBoxThreads];
+    __shared__ int bj[kBoxThreads];
+    const int ng = num_groups(env.num_obj);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int g = threadIdx.x; g < ng; g += kBoxThreads)
+        for (int a = 0; a < 3; ++a) {
+            const float l = env.gcull[6 * g + a], h = env.gcull[6 * g + 3 + a];
+            if (isfinite(l)) lo[a] = fminf(lo[a], l);
+            if (isfinite(h)) hi[a] = fmaxf(hi[a], h);
+        }
+    FKf fk;
+    fk.M = Mat3f{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+    fk.px = fk.py = fk.pz = 0.0f;
+    fk.seg = 0;
+    fk.qi = 0;
+    const Vec3<float> P = skelf_point(fk, rb, q, 0);
+    float best = INFINITY;
+    int bestj = -1;
+    for (int j = threadIdx.x; j < env.num_obj; j += kBoxThreads) {
+        const float* cb = env.cull + 6 * j;
+        const float gx = fmaxf(fmaxf(cb[0] - P.x, P.x - cb[3]), 0.0f);
+        const float gy = fmaxf(fmaxf(cb[1] - P.y, P.y - cb[4]), 0.0f);
+        const float gz = fmaxf(fmaxf(cb[2] - P.z, P.z - cb[5]), 0.0f);
+        const float g2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
+        if (g2 < best) {   // ascending j per thread: the first of equals stays
+            best = g2;
+            bestj = j;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        red[a][threadIdx.x] = lo[a];
+        red[3 + a][threadIdx.x] = hi[a];
+    }
+    bd[threadIdx.x] = best;
+    bj[threadIdx.x] = bestj;
+    __syncthreads();
+    for (int w = kBoxThreads / 2; w > 0; w >>= 1) {
+        const int t = threadIdx.x;
+        if (t < w) {
+            for (int a = 0; a < 3; ++a) {
+                red[a][t] = fminf(red[a][t], red[a][t + w]);
+                red[3 + a][t] = fmaxf(red[3 + a][t], red[3 + a][t + w]);
+            }
+            const float d2 = bd[t + w];
+            const int j2 = bj[t + w];
+            if (j2 >= 0 && (bj[t] < 0 || d2 < bd[t] || (d2 == bd[t] && j2 < bj[t]))) {
+                bd[t] = d2;
+                bj[t] = j2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) {
+        const int a = threadIdx.x;
+        const float l = red[a][0], h = red[3 + a][0];
+        const bool ok = l < h;   // false for an empty scene (infinities)
+        box[a] = ok ? l : 0.0f;
+        box[3 + a] = ok ? 1024.0f / (h - l) : 0.0f;
+    }
+    if (threadIdx.x == 0) *warm0 = bj[0];
+}
+
+// In-place ascending bitonic sort of blockDim.x (a power of 2) 64-bit
+// entries in LDS, one a thread; the whole workgroup calls it.
+__device__ inline void block_sort(uint64_t* v) {
+    const int n = blockDim.x, t = threadIdx.x;
+    for (int k = 2; k <= n; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int u = t ^ j;
+            if (u > t) {
+                const uint64_t a = v[t], b = v[u];
+                if ((a > b) == ((t & k) == 0)) {
+                    v[t] = b;
+                    v[u] = a;
+                }
+            }
+            __syncthreads();
+        }
+}
+
 // The link-major walk (tuning variant 4): one launch per link index i over
-// the configurations still alive at link i, 64 consecutive ones per wave in
-// the walk order, so that the lanes of a wave hold one link of similar poses
-// and take the broad phase's branches together.  Per configuration:
+// the configurations still alive at link i.  A workgroup takes rounds of
+// blockDim.x consecutive entries of the link's list from a counter, 64 per
+// wave, so that the lanes of a wave hold the same link of nearby
+// configurations.  Per configuration:
 //   1. the link's end points, from the joint angles (the same compositions
 //      in the same order as the incremental walk, so the same bits);
-//   2. the warm pair (the previous link's nearest obstacle), all lanes in one
-//      step;
-//   3. the wave walks the groups in the scene's walk order together (a
+//   2. the warm pair (the previous link's nearest obstacle; for the first
+//      link the obstacle nearest to the first skeleton point), all lanes in
+//      one step;
+//   3. the wave's candidate groups: 64 groups at a time, one per lane, tested
+//      against the union of the lanes' segment boxes with the largest of
+//      their bounds (a bound only shrinks during the walk, so a group failing
+//      it fails every lane's own test: skipping it changes nothing);
+//   4. the wave walks the candidate groups in the walk order together (a
 //      group's box and members read once, with scalar loads, for all lanes);
 //      each lane queues the members that pass its bound (an LDS ring of
 //      kLaneQueue per lane); whenever enough lanes hold queued pairs every
 //      lane takes its next one, tests it again against its bound of now and
 //      evaluates it if it passes (the flat evaluation, one pair per lane per
 //      step, each its own obstacle);
-//   4. once a lane's pair ends the link (distance <= 0), the rest of its walk
+//   5. once a lane's pair ends the link (distance <= 0), the rest of its walk
 //      is the search for the first hit in obstacle order: the positions after
 //      the hit, with the bound of a running minimum of 0, obstacles of a
 //      smaller index than the smallest hit found.  Every position before the
 //      hit held a pair evaluated above 0 or culled with a bound at least that
 //      of a minimum of 0, which never culls a pair whose distance can be
 //      <= 0: none of them can be an earlier hit.
-// The walk evaluates exactly the pairs of link_min's walk up to its hit, in
-// its order (the queueing bound is never tighter than the walk's, a member's
-// box lies inside its group's, and every pair is tested again with exactly
-// the bound the walk has when it reaches it); the link's value (the value at
-// the first hit in obstacle order, or the minimum), the warm pair of the next
-// link and the verdict are those of link_min.  A configuration that goes on
-// keeps its running minimum, warm pair and screening flag in st_d / st_w and
-// is appended to the next link's list; the lists are in no particular order
-// between waves (each configuration's results do not depend on it).
+// The link's value (the value at the first hit in obstacle order, or the
+// minimum over the pairs, which no culled pair can lower) and the verdict
+// are those of link_min whatever the warm pair; the walk evaluates the pairs
+// of link_min's walk up to its hit in its order when their warm pairs agree
+// (the queueing bound is never tighter than the walk's, a member's box lies
+// inside its group's, and every pair is tested again with exactly the bound
+// the walk has when it reaches it).  A configuration that goes on keeps its
+// running minimum, warm pair and screening flag in st_d / st_w; a round's
+// survivors are sorted by the Morton code (over box) of their next link's
+// start point and appended to the next link's list together, so that the
+// next launch's waves again hold nearby segments.  The lists' order between
+// rounds depends on timing; no configuration's results do.  (Rounds of one
+// 64-entry chunk per wave measured fastest: a window twice as long per
+// round, 5% slower on workload 5; four times, a smaller share of LDS left
+// for resident workgroups, 60% slower.)
 template <bool OBB>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_links(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
     uint32_t* count, int link, const int32_t* __restrict__ in, const uint32_t* __restrict__ in_count,
-    int32_t* __restrict__ next_list, uint32_t* next_count, float* __restrict__ st_d, uint32_t* __restrict__ st_w,
-    size_t cull_off, size_t queue_off) {
-    // configurations at this link: pass 0 takes all n (in = the walk order, or null)
+    uint32_t* cursor, int32_t* __restrict__ next_list, uint32_t* next_count, float* __restrict__ st_d,
+    uint32_t* __restrict__ st_w, const float* __restrict__ box, const int32_t* __restrict__ warm0, size_t cull_off,
+    size_t queue_off, size_t sort_off) {
+    // configurations at this link: the first takes all n (in = the walk order, or null)
     const int64_t cnt = link == 0 ? n : (int64_t)*in_count;
     const int B = blockDim.x;
-    if ((int64_t)blockIdx.x * B >= cnt) return;   // no chunk for this workgroup (uniform)
+    if ((int64_t)blockIdx.x * B >= cnt) return;   // no round for this workgroup (uniform)
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t round_id, out_base;
     float* const lcull = reinterpret_cast<float*>(smem + cull_off);   // [M][6] cull boxes, for the queued pairs
     for (int k = threadIdx.x; k < 6 * env.num_obj; k += B) lcull[k] = env.cull[k];
     const Staged<float> s = stage_env<float, OBB>(env, smem);   // ends with a barrier
     int16_t* const ring = reinterpret_cast<int16_t*>(smem + queue_off) + threadIdx.x;   // slot k: ring[k * B]
+    uint64_t* const sorted = reinterpret_cast<uint64_t*>(smem + sort_off);   // [B] a round's survivors
     const const_ptr<float> gcull = as_const(env.gcull);
     const const_ptr<float> ccull = as_const(env.cull);
     const const_ptr<int32_t> gorder = as_const(env.gorder);
+    const float kbox[6] = {box[0], box[1], box[2], box[3], box[4], box[5]};
+    const int first_warm = link == 0 ? *warm0 : -1;
     const int M = env.num_obj;
     const int L = rb.num_links;
     const int D = rb.num_dofs;
@@ -828,15 +966,17 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     const bool gnd_skip = i < rb.ground_included;
     const float bb0 = cull_bound2(0.0f, r, p.eq);
     const int lane = __lane_id();
-    const int64_t wave = ((int64_t)blockIdx.x * B + threadIdx.x) >> 6;
-    const int64_t waves = ((int64_t)gridDim.x * B) >> 6;
 
-    for (int64_t base = wave * 64; base < cnt; base += waves * 64) {
-        const int64_t e = base + lane;
+    for (;;) {
+        // the workgroup's next round
+        if (threadIdx.x == 0) round_id = atomicAdd(cursor, 1u);
+        __syncthreads();
+        const int64_t e = (int64_t)round_id * B + threadIdx.x;
+        if ((int64_t)round_id * B >= cnt) break;   // uniform: every thread read the same round
         const bool busy = e < cnt;
         int64_t c = 0;
         float d = p.maxd;
-        int warm = -1;
+        int warm = first_warm;
         bool uncertain = false;
         Vec3<float> P0{0.0f, 0.0f, 0.0f}, P1{0.0f, 0.0f, 0.0f};
         if (busy) {
@@ -876,12 +1016,18 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                 best = w;
             }
         }
-        // the lane's squared bound (running minimum, or 0 once searching;
-        // an idle lane's culls everything) and the obstacles it still takes
+        // the lane's squared bound (running minimum, or 0 once searching; an
+        // idle lane's culls everything) and the obstacles it still takes
         // (all while walking, those below the smallest hit once searching)
         float bb = !busy ? -1.0f : (jh < 0 ? cull_bound2(prof, r, p.eq) : bb0);
         int lim = jh < 0 ? M : jh;
-        // 3-4. the walk, then the search, its pairs evaluated from the queues
+        // 3. the union of the lanes' segment boxes and the largest bound
+        const Vec3<float> ulo{wave_min(busy ? slo.x : INFINITY), wave_min(busy ? slo.y : INFINITY),
+                              wave_min(busy ? slo.z : INFINITY)};
+        const Vec3<float> uhi{wave_max(busy ? shi.x : -INFINITY), wave_max(busy ? shi.y : -INFINITY),
+                              wave_max(busy ? shi.z : -INFINITY)};
+        const float ubb = wave_max(bb);
+        // 4-5. the walk, then the search, its pairs evaluated from the queues
         int qh = 0, qt = 0, qn = 0;   // ring head, tail and length
         auto step = [&]() {   // one queued pair per lane
             int task = -1;
@@ -911,27 +1057,34 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                 }
             }
         };
-        for (int g = 0; g < ng; ++g) {
-            const float gb[6] = {gcull[6 * g], gcull[6 * g + 1], gcull[6 * g + 2],
-                                 gcull[6 * g + 3], gcull[6 * g + 4], gcull[6 * g + 5]};
-            const bool gpass = !cull_fast(gb, slo, shi, bb);
-            if (__ballot(gpass)) {
-                const int m1 = min(M, (g + 1) * kGroupSize);
-                for (int m = g * kGroupSize; m < m1; ++m) {
-                    const int j = gorder[m];
-                    const float cb[6] = {ccull[6 * j], ccull[6 * j + 1], ccull[6 * j + 2],
-                                         ccull[6 * j + 3], ccull[6 * j + 4], ccull[6 * j + 5]};
-                    if (gpass && j != w && j < lim && !cull_fast(cb, slo, shi, bb)) {
-                        ring[qt * B] = (int16_t)j;
-                        qt = (qt + 1) & (kLaneQueue - 1);
-                        ++qn;
+        for (int g0 = 0; g0 < ng; g0 += 64) {
+            const int gl = g0 + lane;
+            const bool cand = gl < ng && !cull_fast(env.gcull + 6 * gl, ulo, uhi, ubb);
+            uint64_t mask = __ballot(cand);
+            while (mask) {
+                const int g = g0 + __ffsll((unsigned long long)mask) - 1;
+                mask &= mask - 1;
+                const float gb[6] = {gcull[6 * g], gcull[6 * g + 1], gcull[6 * g + 2],
+                                     gcull[6 * g + 3], gcull[6 * g + 4], gcull[6 * g + 5]};
+                const bool gpass = !cull_fast(gb, slo, shi, bb);
+                if (__ballot(gpass)) {
+                    const int m1 = min(M, (g + 1) * kGroupSize);
+                    for (int m = g * kGroupSize; m < m1; ++m) {
+                        const int j = gorder[m];
+                        const float cb[6] = {ccull[6 * j], ccull[6 * j + 1], ccull[6 * j + 2],
+                                             ccull[6 * j + 3], ccull[6 * j + 4], ccull[6 * j + 5]};
+                        if (gpass && j != w && j < lim && !cull_fast(cb, slo, shi, bb)) {
+                            ring[qt * B] = (int16_t)j;
+                            qt = (qt + 1) & (kLaneQueue - 1);
+                            ++qn;
+                        }
                     }
                 }
+                // evaluate once enough lanes hold pairs; before the next
+                // group, every queue must have room for a whole group
+                if (__popcll(__ballot(qn > 0)) >= kDrainLanes) step();
+                while (__ballot(qn > kLaneQueue - kGroupSize)) step();
             }
-            // evaluate once enough lanes hold pairs; before the next group,
-            // every queue must have room for a whole group
-            if (__popcll(__ballot(qn > 0)) >= kDrainLanes) step();
-            while (__ballot(qn > kLaneQueue - kGroupSize)) step();
         }
         while (__ballot(qn > 0)) step();
         // the link's value
@@ -957,7 +1110,23 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
             }
         }
         append_flagged(busy && finished && uncertain, c, idx, count);
-        append_flagged(busy && !finished, c, next_list, next_count);
+        // survivors, with the sort key of their next link's start point
+        const bool keep = busy && !finished;
+        sorted[threadIdx.x] = keep ? ((uint64_t)point_key(P1, kbox) << 32) | (uint64_t)(uint32_t)c : ~0ull;
+        // the round's survivors, sorted, appended together
+        __syncthreads();
+        block_sort(sorted);
+        if (threadIdx.x == 0) {
+            int k = 0;   // survivors come first (the others hold the largest key)
+            for (int h = B; h > 0; h >>= 1)
+                if (k + h <= B && sorted[k + h - 1] != ~0ull) k += h;
+            out_base = k ? atomicAdd(next_count, (uint32_t)k) : 0u;
+            round_id = (uint32_t)k;
+        }
+        __syncthreads();
+        const int nkeep = (int)round_id;
+        if ((int)threadIdx.x < nkeep) next_list[out_base + threadIdx.x] = (int32_t)(uint32_t)sorted[threadIdx.x];
+        __syncthreads();   // round_id and the sort buffer are written again next round
     }
 }
 
@@ -1188,6 +1357,24 @@ size_t lanes_lds(const EnvDev& env, int block, size_t* cull_off, size_t* queue_o
     return total;
 }
 
+// LDS of the link-major kernel: that of the per-lane-queue kernel, then
+// (16-byte aligned) the sort buffer of a round's survivors (8 bytes per
+// thread); 0 when it cannot run on this obstacle set
+// (kStaticLds: room for the kernel's own shared variables).
+constexpr size_t kStaticLds = 1024;
+size_t links_lds(const EnvDev& env, int block, size_t* cull_off, size_t* queue_off, size_t* sort_off) {
+    size_t c = 0, qo = 0;
+    const size_t base = lanes_lds(env, block, &c, &qo);
+    if (!base) return 0;
+    const size_t so = (base + 15) & ~size_t(15);
+    const size_t total = so + (size_t)8 * block;
+    if (total + kStaticLds > device_lds_limit()) return 0;
+    if (cull_off) *cull_off = c;
+    if (queue_off) *queue_off = qo;
+    if (sort_off) *sort_off = so;
+    return total;
+}
+
 }  // namespace
 
 int fast_lanes_blocks_per_cu(const EnvDev& env, int block) {
@@ -1201,7 +1388,7 @@ int fast_lanes_blocks_per_cu(const EnvDev& env, int block) {
 }
 
 int fast_links_blocks_per_cu(const EnvDev& env, int block) {
-    const size_t sm = lanes_lds(env, block, nullptr, nullptr);
+    const size_t sm = links_lds(env, block, nullptr, nullptr, nullptr);
     if (!sm) return 0;
     int nb = 0;
     const hipError_t e =
@@ -1269,20 +1456,24 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
     if (t.variant == 4 && GROUPED && links && n <= INT32_MAX) {
         // one launch per link over the configurations alive at it
         const int BL = (t.block == 512 || t.block == 1024) ? t.block : 256;
-        size_t cull_off = 0, queue_off = 0;
-        const size_t lsm = lanes_lds(env, BL, &cull_off, &queue_off);
+        size_t cull_off = 0, queue_off = 0, sort_off = 0;
+        const size_t lsm = links_lds(env, BL, &cull_off, &queue_off, &sort_off);
         if (lsm) {
             const LinksLayout ll = links_layout(links, n);
             int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : fast_links_blocks_per_cu(env, BL);
             per_cu = std::max(1, per_cu);
             const int64_t blocks = std::min<int64_t>((int64_t)num_cus * per_cu, (n + BL - 1) / BL);
             const dim3 g((unsigned)std::max<int64_t>(1, blocks)), b(BL);
-            hipError_t e = launch_zero_u32(ll.count, kMaxLinks + 1, st);
+            hipError_t e = launch_zero_u32(ll.count, kLinksCounters, st);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_links_setup, dim3(1), dim3(kBoxThreads), 0, st, rb, env, q, ll.box, ll.warm0);
+                e = hipGetLastError();
+            }
             for (int i = 0; i < rb.num_links && e == hipSuccess; ++i) {
                 const int32_t* in = i == 0 ? perm : ll.list[(i - 1) & 1];
                 hipLaunchKernelGGL((k_distance_fast_links<OBB>), g, b, lsm, st, rb, env, qp, q, n, out, idx, count, i,
-                                   in, ll.count + i, ll.list[i & 1], ll.count + i + 1, ll.d, ll.w, cull_off,
-                                   queue_off);
+                                   in, ll.count + i, ll.cursor + i, ll.list[i & 1], ll.count + i + 1, ll.d, ll.w,
+                                   ll.box, ll.warm0, cull_off, queue_off, sort_off);
                 e = hipGetLastError();
             }
             return e;

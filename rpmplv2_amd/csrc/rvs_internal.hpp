@@ -151,18 +151,25 @@ inline CohLayout coh_layout(uint32_t* base, int64_t n) {
 hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n, const FastTuning& t, uint32_t* coh,
                                   const int32_t** perm, hipStream_t st);
 // Scratch of the link-major bulk pass (tuning variant 4), 256-byte aligned:
-// two lists of the configurations alive at a link (used in turn), each
-// configuration's running minimum and (warm pair + 1) << 1 | screening flag
-// between links, and the lists' sizes per link.
+// two lists of the configurations alive at a link, used in turn, each
+// configuration's running minimum
+// and (warm pair + 1) << 1 | screening flag between links, the lists' sizes
+// per link and the round counters per link (kLinksCounters words, cleared
+// together), the box that quantizes the sort keys and the first link's warm
+// pair.
+constexpr int kLinksCounters = 2 * kMaxLinks + 2;
 struct LinksLayout {
     int32_t* list[2];
     float* d;
     uint32_t* w;
-    uint32_t* count;   // kMaxLinks + 1
+    uint32_t* count;    // [kMaxLinks + 1] list sizes, then
+    uint32_t* cursor;   // [kMaxLinks + 1] round counters
+    float* box;         // lo[3], 1024 / extent[3]
+    int32_t* warm0;
 };
 inline size_t fast_links_bytes(int64_t n) {
     const size_t m = (size_t)std::max<int64_t>(n, 0);
-    return 4 * (4 * coh_words(m) + coh_words(kMaxLinks + 1));
+    return 4 * (4 * coh_words(m) + coh_words(kLinksCounters) + coh_words(8));
 }
 inline LinksLayout links_layout(uint32_t* base, int64_t n) {
     const size_t m = coh_words((size_t)std::max<int64_t>(n, 0));
@@ -172,6 +179,9 @@ inline LinksLayout links_layout(uint32_t* base, int64_t n) {
     L.d = reinterpret_cast<float*>(L.list[1] + m);
     L.w = reinterpret_cast<uint32_t*>(L.d + m);
     L.count = L.w + m;
+    L.cursor = L.count + kMaxLinks + 1;
+    L.box = reinterpret_cast<float*>(L.count + coh_words(kLinksCounters));
+    L.warm0 = reinterpret_cast<int32_t*>(L.box + 6);
     return L;
 }
 // The persistent variants (1: lane refill, 2: chunks, 3: per-lane queues)
