@@ -373,10 +373,13 @@ rvs_status rvs_edge_first_collision(rvs_context* ctx, int32_t env, const int32_t
  * with the same queues) and workgroup size.  The obstacle set is always staged whole in LDS (every lane walks
  * every obstacle of its current link, so a smaller obstacle tile would only
  * add reloads); it must fit the device's workgroup LDS.  Variants 0, 1 and 2
- * give the same results bit for bit.  Variants 3 and 4 evaluate pairs with a
- * branch-free form of the same segment-box distance: the same verdicts, and
- * profile distances within FP32 rounding (a few 1e-7 m) of the other
- * variants'; the screening flags, and with them the refined set, can differ. */
+ * evaluate the same pairs and give the same results bit for bit.  Variants 3
+ * and 4 search for the first hit in obstacle order among other candidates
+ * (more of them, or the walk positions after the hit), so that their
+ * screening flags, and with them the refined set, can differ: the same
+ * verdicts, and distances within FP32 rounding (a few 1e-7 m) of the other
+ * variants'.  Each variant's results do not depend on its block size,
+ * workgroups per CU or walk order. */
 typedef struct rvs_tuning {
     int32_t variant;        /* 0: one lane per configuration; 1: persistent lane refill; 2: one lane per
                                configuration in persistent waves over 64-configuration chunks;

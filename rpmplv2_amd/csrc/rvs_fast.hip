@@ -121,42 +121,6 @@ __device__ __attribute__((always_inline)) inline Vec3<float> skelf_point(FKf& s,
     return P;
 }
 
-// One (link, obstacle) pair of the bulk pass: filtered distance, and the
-// screening rules that send the configuration to the exact pass.
-template <bool OBB>
-__device__ __attribute__((always_inline)) inline float fast_pair(const Staged<float>& s, int j, const Vec3<float>& P0,
-                                                                 const Vec3<float>& P1, float r, const Params<float>& p,
-                                                                 float band, bool& uncertain) {
-    const Box<float> bx = s.box[j];
-    PairInfo<float> info{p.maxd, p.maxd, -1.0f, 1.0f, false, false};
-    float dt;
-    if (OBB) {
-        // oriented box: solved in the box frame (the frame change adds a few
-        // ulp of the lever arm, far inside the band)
-        const float* x = s.xf + 12 * j;
-        dt = capsule_box<float, false, FastMath>(to_box(x, P0), to_box(x, P1), r, bx, p, nullptr, &info, nullptr);
-    } else {
-        dt = capsule_box<float, false, FastMath>(P0, P1, r, bx, p, nullptr, &info, nullptr);
-    }
-    const float tol = s.tol[j];
-    // inside points and crossings are decided by comparisons: the result
-    // (0 or -r) is exact unless the decision is marginal
-    if (info.inside) uncertain |= info.depth < band;
-    else if (info.hit) uncertain |= info.hit_margin < band;
-    else uncertain |= (info.raw < p.eq + band) || (fabsf(dt) < band) || (info.sin2 < 1e-6f && info.raw < 0.01f) ||
-                      (info.sin2 < -0.5f) || (dt != dt);
-    // dt is NaN when an end point's projection lies exactly on a face plane
-    // with the other end point inside that face's span (the reference then
-    // reads nearest points it never set); the minimum drops NaN, so the pair
-    // would vanish unseen, and the reference arithmetic, one rounding away,
-    // may land on a crossing instead: the exact pass decides it
-    if (tol < p.maxd && fabsf(dt - tol) < band) uncertain = true;
-    if (dt > tol) dt = p.maxd;
-    return dt;
-}
-
-// One (link, obstacle) pair of the per-lane walk: fast_pair with the flat
-// evaluation (the same filtering and tolerance screening).
 // Read-only tables indexed by wave-uniform values: read through the
 // constant address space, so that they are fetched with scalar loads into
 // SGPRs (the tables are not written while a kernel reads them).
@@ -188,6 +152,16 @@ __device__ __attribute__((always_inline)) inline float cull_bound2(float prof, f
     return b * b;
 }
 
+// One (link, obstacle) pair of the bulk pass: the filtered distance by the
+// case-tree-free segment-box evaluation (flat_capsule_box, in the box frame
+// for oriented boxes; the frame change adds a few ulp of the lever arm, far
+// inside the band) and the screening rules that send the configuration to
+// the exact pass: those of flat_capsule_box, which flag every pair the
+// reference's tree returns NaN for (a segment in a face plane) and every
+// marginal contact decision, and a value within band of the obstacle's
+// tolerance.  A fixed sequence of operations keeps a wave's lanes together
+// whatever pairs they evaluate (workload 2, one lane per configuration:
+// 0.214 ms per 1,048,576 configurations against 0.266 ms for the tree).
 template <bool OBB>
 __device__ __attribute__((always_inline)) inline float flat_pair(const Staged<float>& s, int j, const Vec3<float>& P0,
                                                                  const Vec3<float>& P1, float r, const Params<float>& p,
@@ -247,7 +221,7 @@ __device__ __attribute__((always_inline)) inline float link_min(const Staged<flo
         bool go = active;
         for (int j = 0; j < M; ++j) {
             if (go && !cull_fast(cull + 6 * j, slo, shi, bb)) {
-                const float dt = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+                const float dt = (s.gnd[j] && gnd_skip) ? p.maxd : flat_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
                 prof = smin(prof, dt);
                 bb = cull_bound2(prof, r, p.eq);
                 if (prof <= 0) {
@@ -264,7 +238,7 @@ __device__ __attribute__((always_inline)) inline float link_min(const Staged<flo
     float vh = 0.0f;
     const int w = active ? warm : -1;
     if (w >= 0) {
-        const float dt = (s.gnd[w] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, w, P0, P1, r, p, band, uncertain);
+        const float dt = (s.gnd[w] && gnd_skip) ? p.maxd : flat_pair<OBB>(s, w, P0, P1, r, p, band, uncertain);
         if (dt <= 0) {
             jh = w;
             vh = dt;
@@ -283,7 +257,7 @@ __device__ __attribute__((always_inline)) inline float link_min(const Staged<flo
                 const int j = gorder[m];
                 if (go && j != w && !cull_fast(cull + 6 * j, slo, shi, bb)) {
                     const float dt =
-                        (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+                        (s.gnd[j] && gnd_skip) ? p.maxd : flat_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
                     if (dt < prof) best = j;
                     prof = smin(prof, dt);
                     bb = cull_bound2(prof, r, p.eq);
@@ -302,7 +276,7 @@ __device__ __attribute__((always_inline)) inline float link_min(const Staged<flo
         const float bb0 = cull_bound2(0.0f, r, p.eq);
         for (int j = 0; j < jh; ++j) {
             if (cull_fast(cull + 6 * j, slo, shi, bb0)) continue;
-            const float dj = (s.gnd[j] && gnd_skip) ? p.maxd : fast_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
+            const float dj = (s.gnd[j] && gnd_skip) ? p.maxd : flat_pair<OBB>(s, j, P0, P1, r, p, band, uncertain);
             if (dj <= 0) {
                 first = dj;
                 break;

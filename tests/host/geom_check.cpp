@@ -69,6 +69,9 @@ int run(long n, uint64_t seed, const char* tag) {
             B[k] = hi[k] + (T)R.u(-1e-4, 1e-4);
         } else if (mode == 4) {  // degenerate segment
             for (int m = 0; m < 3; ++m) B[m] = A[m];
+        } else if (mode == 5) {  // segment lying in a face plane (the tree's NaN paths)
+            int k = R.i(3);
+            A[k] = B[k] = R.i(2) ? lo[k] : hi[k];
         }
         T r = (T)(R.i(4) == 0 ? 0.0 : R.u(0, 0.1));
         T obs[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};

@@ -22,11 +22,11 @@ def test_launch_shapes_agree(engine, wl):
         sem = rvs.ENV_OBB
     rid = engine.load_robot(w.robot)
     eid = engine.create_env(w.scene, sem)
-    # bit-identical within a pair-evaluation form and search: the case tree
-    # (variants 0-2); the flat form with the restarted search (3) and with
-    # the search continued from the hit (4), whose screening flags differ
+    # bit-identical within a first-hit search: in obstacle order (variants
+    # 0-2), among all candidates in walk order (3), from the hit on (4); the
+    # searches evaluate different pairs, so their screening flags differ
     refs = {}
-    form_of = lambda v: "tree" if v < 3 else v
+    form_of = lambda v: "ordered" if v < 3 else v
     for coh in (0, 5, 3):
         for variant in (0, 1, 2, 3, 4):
             for block in ((256, 512, 1024) if variant >= 3 else (64, 128, 256)):
@@ -42,7 +42,7 @@ def test_launch_shapes_agree(engine, wl):
                         assert np.array_equal(ver, ref[2]), shape
                         assert np.array_equal(d, ref[0]) and np.array_equal(prof, ref[1]), shape
     # across forms: the same verdicts, distances and profiles within FP32 rounding
-    d0, p0, v0 = refs["tree"]
+    d0, p0, v0 = refs["ordered"]
     for form in (3, 4):
         d1, p1, v1 = refs[form]
         assert np.array_equal(v0, v1), form
