@@ -51,12 +51,58 @@ FLOP_CAPSULE_OBB = 196
 FLOP_CAPSULE_CAPSULE = 60
 FLOP_BROAD_PHASE = 15
 PEAK_FP32_TFLOPS = 157.3
+# the FP32 vector rate without packed FP32 instructions, which this build
+# turns off (one FP32 FMA per lane per cycle instead of two)
+PEAK_FP32_UNPACKED_TFLOPS = 78.6
 PEAK_HBM_GBPS = 8000.0
+# FP32 operations per configuration that the bulk kernels execute, from
+# rocprofv3 counters on the same workloads: SQ_INSTS_VALU_FLOPS_FP32 (2 FMA +
+# ADD + MUL + TRANS, in wave instructions) x lanes active per VALU
+# instruction (SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU), divided by the
+# configurations of the profiled call.  roofline.achieved = this x the
+# configurations of a launch / the launch time measured here.
+COUNTER_FLOP_PER_CONFIG = {
+    ("xarm6_16aabb_ground", "k_distance_fast"): (2537.2, "profiles/k_distance_fast_xarm6_1M.md"),
+    ("chain22_512rotated_boxes", "k_distance_fast_links"):
+        (24940.0, "profiles/k_distance_fast_links_chain22_512obb.md"),
+}
 # HBM bytes per launch of the FP32 bulk kernel on the headline workload from
-# two `rocprofv3 --pmc` runs: FETCH_SIZE 131,681.3 KiB x 2.0 + WRITE_SIZE
-# 123,943.0 KiB x 1.0, the factors being what the same counters read for a
+# two `rocprofv3 --pmc` runs: FETCH_SIZE 130,624.1 KiB x 2.0 + WRITE_SIZE
+# 124,050.6 KiB x 1.0, the factors being what the same counters read for a
 # 256 MiB copy (profiles/k_distance_fast_xarm6_1M.md).
-TRAFFIC_BYTES = (2.0 * 131123.0 + 123917.2) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
+TRAFFIC_BYTES = (2.0 * 130624.1 + 124050.6) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
+BULK_KERNELS = {0: "k_distance_fast", 1: "k_distance_fast_pool", 2: "k_distance_fast_chunks", 3: "k_distance_fast_lanes",
+                4: "k_distance_fast_links"}
+
+
+def valu_roofline(workload: str, kernel: str, n: int, kernel_ms: float, evaluated_flop_cfg: float,
+                  dense_flop_cfg: float) -> dict:
+    """roofline of an FP32 bulk kernel: achieved = FP32 operations executed
+    per configuration (counters, when the kernel has a profile on this
+    workload; else the evaluated-work model) x n / kernel time."""
+    entry = COUNTER_FLOP_PER_CONFIG.get((workload, kernel))
+    if entry:
+        flop_cfg, src = entry
+        accounting = ("FP32 operations executed per configuration from rocprofv3 counters (" + src +
+                      "): SQ_INSTS_VALU_FLOPS_FP32 x lanes active per VALU instruction")
+    else:
+        flop_cfg = evaluated_flop_cfg
+        accounting = "evaluated-work model (no counter profile of this kernel on this workload)"
+    achieved = flop_cfg * n / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS, "kernel": kernel, "kernel_ms": kernel_ms,
+            "flop_per_config": flop_cfg, "flop_accounting": accounting,
+            "peak_unpacked": PEAK_FP32_UNPACKED_TFLOPS, "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
+            "peak_kind": "FP32 vector (VALU) with packed FP32; peak_unpacked without it, as this build runs; "
+                         "the path has no MFMA work",
+            "evaluated_flop_per_config": evaluated_flop_cfg,
+            "evaluated_tflops": evaluated_flop_cfg * n / (kernel_ms * 1e-3) / 1e12,
+            "evaluated_note": "model: 100 flop per FK joint + 15 per broad-phase test + 160 (AABB) or 196 (OBB) per "
+                              "evaluated pair, the counts from the oracle's restatement of the walk",
+            "dense_flop_per_config": dense_flop_cfg,
+            "dense_equivalent_tflops": dense_flop_cfg * n / (kernel_ms * 1e-3) / 1e12,
+            "dense_note": "nominal work of every FK joint and every (link, obstacle) pair per second; the broad "
+                          "phase skips most pairs, so it is no achieved rate"}
 
 
 def pair_count(rb, scene) -> int:
@@ -223,23 +269,13 @@ def workload5_section(eng, cores: int, n: int) -> dict:
                 "refined": nref_o, "collision_free_fraction": free_o, "tuning": tune,
                 "verdict_mismatch_vs_ref32_subsample": int((ver_o != ov32).sum()),
                 "max_abs_dist_err_vs_ref64_subsample": float(np.abs(d_o - od64).max()),
-                "roofline": {"bound": "valu",
-                             "kernel": {0: "k_distance_fast<OBB>", 1: "k_distance_fast_pool<OBB>",
-                                        2: "k_distance_fast_chunks<OBB>", 3: "k_distance_fast_lanes<OBB>",
-                                        4: "k_distance_fast_links<OBB>"}[tune["variant"]],
-                             "dense_flop_per_config": flop_obb_dense,
-                             "dense_equivalent_tflops": flop_obb_dense * n / (bulk_o * 1e-3) / 1e12,
-                             "dense_note": "dense nominal work (100 flop per FK joint + 196 per capsule-OBB pair) "
-                                           "per second; it exceeds the VALU peak because the broad phase skips "
-                                           "98% of the pairs, so the roofline fraction uses the evaluated work",
-                             "pairs_evaluated_per_config": ev_p / m, "box_tests_per_config": tests_p / m,
-                             "counts_from": "oracle restatement of the bulk pass's walk on the 65,536-configuration "
-                                            "subsample",
-                             "flop_per_config": flop_obb,
-                             "achieved_tflops": flop_obb * n / (bulk_o * 1e-3) / 1e12,
-                             "frac": flop_obb * n / (bulk_o * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
-                             "algorithmic_bytes_per_config": bytes_cfg,
-                             "hbm_GBps_algorithmic": bytes_cfg * n / (bulk_o * 1e-3) / 1e9}},
+                "roofline": dict(valu_roofline("chain22_512rotated_boxes", BULK_KERNELS[tune["variant"]], n, bulk_o,
+                                               flop_obb, flop_obb_dense),
+                                 pairs_evaluated_per_config=ev_p / m, box_tests_per_config=tests_p / m,
+                                 counts_from="oracle restatement of the bulk pass's walk on the "
+                                             "65,536-configuration subsample",
+                                 algorithmic_bytes_per_config=bytes_cfg,
+                                 hbm_GBps_algorithmic=bytes_cfg * n / (bulk_o * 1e-3) / 1e9)},
         "aabb_of_obb_reference_semantics": {
             "configs_per_s": n / (ms_a * 1e-3), "ms": ms_a, "bulk_ms": bulk_a, "refine_ms": ref_a,
             "refined": nref_a, "collision_free_fraction": free_a,
@@ -339,19 +375,20 @@ def _edge_run(eng, O, w, cores: int) -> dict:
         flop_eval += float(ev.sum()) * flop_state
         off += m
     nominal = int(K.sum())
-    achieved = flop_eval / (kms * 1e-3) / 1e12
+    dense = flop_eval / (kms * 1e-3) / 1e12
     return {"edges": len(ids), "states_nominal": nominal, "states_evaluated": states_eval,
             "ms": ms, "kernel_ms": kms, "edges_per_s": len(ids) / (ms * 1e-3),
             "states_evaluated_per_s": states_eval / (ms * 1e-3),
             "states_nominal_per_s": nominal / (ms * 1e-3), "tuning": tune,
             "first_collision_mismatch_vs_oracle": mism,
             "edges_colliding_at_state0": int((got == 0).sum()), "edges_free": int((got == -1).sum()),
-            "roofline": {"bound": "valu", "kernel": "k_edge_first", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                         "flop_accounting": "dense nominal work of every state a search reaches (edge start to "
-                                            "its first collision or its end): 100 per FK joint + 160 per "
-                                            "capsule-box pair + 60 per self pair; the kernel's broad phase "
-                                            "and early exit evaluate less, so this is a dense-equivalent rate"}}
+            "roofline": {"bound": "valu", "kernel": "k_edge_first", "achieved": None, "peak": PEAK_FP32_TFLOPS,
+                         "unit": "TFLOP/s", "dense_equivalent_tflops": dense,
+                         "dense_frac": dense / PEAK_FP32_TFLOPS,
+                         "dense_note": "nominal work of every state a search reaches (edge start to its first "
+                                       "collision or its end): 100 per FK joint + 160 per capsule-box pair + 60 "
+                                       "per self pair; the kernel's broad phase and early exit evaluate less, and "
+                                       "no counter profile of this kernel is recorded, so no achieved rate"}}
 
 
 def edges_section(eng, cores: int) -> dict:
@@ -525,20 +562,24 @@ def main():
     del q, d, prof, ver
     torch.cuda.empty_cache()
 
-    # algorithmic work per launch: the nominal dense figure per configuration
-    # (every FK joint and every (link, obstacle) pair) x the configurations
-    # of the launch.  The work the kernel actually executes after its broad
-    # phase (pairs reaching the broad phase and pairs evaluated, counted by
-    # the oracle's restatement of the bulk pass's walk) is reported beside it.
+    # work per launch: the FP32 operations the kernel executes (counters of
+    # the same kernel on this workload) x the configurations of the launch;
+    # beside it the evaluated-work model (pairs reaching the broad phase and
+    # pairs evaluated, counted by the oracle's restatement of the bulk pass's
+    # walk) and the nominal dense figure
     O = oracle_module()
     _, _, _, (ev_pairs, box_tests) = O.distance_cull(w.robot, O.scene_of(w.scene), w.q, 32, nthreads=host_cores())
     dense_flop_cfg = w.robot.num_dofs * FLOP_FK_JOINT + pair_count(w.robot, w.scene) * FLOP_CAPSULE_AABB
-    eval_flop_launch = (n * w.robot.num_dofs * FLOP_FK_JOINT + box_tests * FLOP_BROAD_PHASE
-                        + ev_pairs * FLOP_CAPSULE_AABB)
-    achieved = dense_flop_cfg * n / (bulk_ms * 1e-3) / 1e12
+    eval_flop_cfg = (n * w.robot.num_dofs * FLOP_FK_JOINT + box_tests * FLOP_BROAD_PHASE
+                     + ev_pairs * FLOP_CAPSULE_AABB) / n
     hbm_bytes = n * (w.robot.num_dofs * 4 + 4 + L * 4 + 1)
-    kernel = {0: "k_distance_fast", 1: "k_distance_fast_pool", 2: "k_distance_fast_chunks", 3: "k_distance_fast_lanes",
-              4: "k_distance_fast_links"}[tune["variant"]]
+    roof = valu_roofline("xarm6_16aabb_ground", BULK_KERNELS[tune["variant"]], n, bulk_ms, eval_flop_cfg,
+                         dense_flop_cfg)
+    roof.update({"traffic": TRAFFIC_BYTES, "pairs_evaluated_per_config": ev_pairs / n,
+                 "box_tests_per_config": box_tests / n, "algorithmic_bytes": hbm_bytes,
+                 "hbm_GBps_algorithmic": hbm_bytes / (bulk_ms * 1e-3) / 1e9,
+                 "traffic_note": "FETCH_SIZE x 2 + WRITE_SIZE per launch of k_distance_fast on this workload "
+                                 "(profiles/k_distance_fast_xarm6_1M.md)"})
     out = {
         "metric": "distance queries/sec (xArm6, 16 boxes) at verdict-exact",
         "value": value,
@@ -555,16 +596,7 @@ def main():
         "config": {"workload": "xarm6_16aabb_ground", "configs_per_step": n, "links": L,
                    "objects": w.scene.num_obj, "capsule_radius": 0.05,
                    "precision": "fp32 bulk + exact near-contact refinement (band 1e-4 m)"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES,
-                     "kernel": kernel, "kernel_ms": bulk_ms, "flop_per_config": dense_flop_cfg,
-                     "flop_accounting": "dense nominal work: 100 flop per FK joint + 160 per capsule-box pair",
-                     "pairs_evaluated_per_config": ev_pairs / n, "box_tests_per_config": box_tests / n,
-                     "evaluated_flop_per_config": eval_flop_launch / n,
-                     "evaluated_tflops": eval_flop_launch / (bulk_ms * 1e-3) / 1e12,
-                     "peak_kind": "FP32 vector (VALU); the path has no MFMA work",
-                     "algorithmic_bytes": hbm_bytes,
-                     "hbm_GBps_algorithmic": hbm_bytes / (bulk_ms * 1e-3) / 1e9},
+        "roofline": roof,
         "tuning": tune,
         "refine": {"configs": refined, "fraction": refined / n,
                    "kernel": "k_distance_refine_pairs" if L * w.scene.num_obj <= 256 else "k_distance_refine",
