@@ -410,6 +410,48 @@ def test_fp32_refine_verdict_exact(oracle, engines_by_variant, wl, variant):
     assert refined < 0.05 * len(w.q)
 
 
+@pytest.mark.parametrize("variant", ["lanes", "links"])
+def test_fp32_refine_queued_kernels_ragged(oracle, engines_by_variant, variant):
+    """The per-lane-queue and link-major bulk kernels on ragged shapes: batch
+    sizes that leave partial waves, workgroups and rounds (1, 63, 65, 1,000,
+    70,001 configurations), robots of 1, 2 and 22 links (the link-major
+    kernel launches once per link), scenes of 1, 5 and 40 boxes (a partial
+    group, a group and a partial one, 10 groups), a ground box skipped for
+    the first links, and non-finite joint values: verdicts equal the
+    oracle's, distances and profiles within 1e-5 m, NaN where the oracle has
+    NaN."""
+    engine = engines_by_variant[variant]
+    # (robot, boxes, side lo / hi, placement radius, centre): the 1-link
+    # chain's link is fixed along z, crossed by its box in every configuration
+    cases = [
+        (R.chain([0.4], radius=0.03), 1, 0.05, 0.08, 0.01, (0.04, 0.0, 0.2)),
+        (R.chain([0.3, 0.3], radius=0.03), 5, 0.05, 0.15, 0.5, (0.0, 0.0, 0.0)),
+        (R.spatial(10, radius=0.025, ground_included=2), 40, 0.05, 0.2, 2.0, (0.0, 0.0, 0.3)),
+        (W.workload5(16).robot, 40, 0.1, 0.3, 1.5, (0.0, 0.0, 0.0)),
+    ]
+    for k, (rb, nbox, lo, hi, rad, cen) in enumerate(cases):
+        rng = np.random.default_rng(71 + k)
+        sc = W._boxes(rng, nbox, lo, hi, rad, cen)
+        if rb.ground_included:
+            sc = W.ground_box().concat(sc)
+        rid = engine.load_robot(rb)
+        eid = engine.create_env(sc)
+        osc = oracle.scene_of(sc)
+        for n in (1, 63, 65, 1000, 70001):
+            q = rng.uniform(-np.pi, np.pi, (n, rb.num_dofs)).astype(np.float32)
+            if n > 100:
+                q[::37, rb.num_dofs // 2] = np.nan
+            print(f"{variant}: {rb.num_links} links, {sc.num_obj} boxes, {n} configurations", flush=True)
+            d, prof, ver, _ = engine.distance(rid, eid, q, precision=rvs.PREC_FP32_REFINE)
+            od, oprof, over, _ = oracle.distance(rb, osc, q, 32)
+            assert (ver == over).all(), (rb.num_links, nbox, n)
+            assert np.array_equal(np.isnan(prof), np.isnan(oprof)) and np.array_equal(np.isnan(d), np.isnan(od))
+            m = ~np.isnan(prof)
+            assert np.abs(prof[m] - oprof[m]).max() <= 1e-5, (rb.num_links, nbox, n)
+            m = ~np.isnan(d)
+            assert np.abs(d[m] - od[m]).max() <= 1e-5, (rb.num_links, nbox, n)
+
+
 @pytest.mark.parametrize("variant", ["pool", "classic", "lanes", "links"])
 @pytest.mark.parametrize("scene", ["dense_overlap", "dense_overlap_wide"])
 def test_fp32_refine_first_hit_in_obstacle_order(oracle, engines_by_variant, scene, variant):
