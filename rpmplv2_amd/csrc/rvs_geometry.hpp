@@ -559,33 +559,22 @@ __host__ __device__ inline float flat_capsule_box(const Vec3<float>& A, const Ve
     const float ix = MP::div(1.0f, Dx), iy = MP::div(1.0f, Dy), iz = MP::div(1.0f, Dz);
     // crossing of the box shrunk by band (slab clipping; an axis the segment
     // does not move along must lie inside the shrunk slab)
+    // (selects, not branches: an axis the segment does not move along gives
+    // infinite or NaN slab parameters, which the selects then ignore)
     float t0 = 0.0f, t1 = 1.0f;
     bool in_slabs = true;
     {
-        const float lx = b.lo[0] + band, hx = b.hi[0] - band;
-        const float ly = b.lo[1] + band, hy = b.hi[1] - band;
-        const float lz = b.lo[2] + band, hz = b.hi[2] - band;
-        if (Dx != 0.0f) {
-            const float a = (lx - A.x) * ix, c = (hx - A.x) * ix;
-            t0 = fmaxf(t0, fminf(a, c));
-            t1 = fminf(t1, fmaxf(a, c));
-        } else {
-            in_slabs = in_slabs && A.x > lx && A.x < hx;
-        }
-        if (Dy != 0.0f) {
-            const float a = (ly - A.y) * iy, c = (hy - A.y) * iy;
-            t0 = fmaxf(t0, fminf(a, c));
-            t1 = fminf(t1, fmaxf(a, c));
-        } else {
-            in_slabs = in_slabs && A.y > ly && A.y < hy;
-        }
-        if (Dz != 0.0f) {
-            const float a = (lz - A.z) * iz, c = (hz - A.z) * iz;
-            t0 = fmaxf(t0, fminf(a, c));
-            t1 = fminf(t1, fmaxf(a, c));
-        } else {
-            in_slabs = in_slabs && A.z > lz && A.z < hz;
-        }
+        auto slab = [&](float D, float inv, float a0, float lo, float hi) {
+            const float l = lo + band, h = hi - band;
+            const float a = (l - a0) * inv, c = (h - a0) * inv;
+            const bool moves = D != 0.0f;
+            t0 = moves ? fmaxf(t0, fminf(a, c)) : t0;
+            t1 = moves ? fminf(t1, fmaxf(a, c)) : t1;
+            in_slabs = in_slabs && (moves || (a0 > l && a0 < h));
+        };
+        slab(Dx, ix, A.x, b.lo[0], b.hi[0]);
+        slab(Dy, iy, A.y, b.lo[1], b.hi[1]);
+        slab(Dz, iz, A.z, b.lo[2], b.hi[2]);
     }
     const bool deep = in_slabs && t0 <= t1 && b.hi[0] - b.lo[0] > 2.0f * band && b.hi[1] - b.lo[1] > 2.0f * band &&
                       b.hi[2] - b.lo[2] > 2.0f * band;
@@ -599,16 +588,21 @@ __host__ __device__ inline float flat_capsule_box(const Vec3<float>& A, const Ve
     };
     float ta = 0.0f, tb = 1.0f;   // the bracket
     const float g0 = g(0.0f), g1 = g(1.0f);
+    // every break point evaluated, its bracket update selected (a break
+    // point outside (0, 1), or infinite or NaN for an axis the segment does
+    // not move along, changes nothing)
     auto consider = [&](float t) {
-        if (t > 0.0f && t < 1.0f) {
-            const float gt = g(t);
-            if (gt <= 0.0f) ta = fmaxf(ta, t);
-            else tb = fminf(tb, t);
-        }
+        const bool in = t > 0.0f && t < 1.0f;
+        const float gt = g(in ? t : 0.0f);
+        ta = (in && gt <= 0.0f) ? fmaxf(ta, t) : ta;
+        tb = (in && !(gt <= 0.0f)) ? fminf(tb, t) : tb;
     };
-    if (Dx != 0.0f) { consider((b.lo[0] - A.x) * ix); consider((b.hi[0] - A.x) * ix); }
-    if (Dy != 0.0f) { consider((b.lo[1] - A.y) * iy); consider((b.hi[1] - A.y) * iy); }
-    if (Dz != 0.0f) { consider((b.lo[2] - A.z) * iz); consider((b.hi[2] - A.z) * iz); }
+    consider((b.lo[0] - A.x) * ix);
+    consider((b.hi[0] - A.x) * ix);
+    consider((b.lo[1] - A.y) * iy);
+    consider((b.hi[1] - A.y) * iy);
+    consider((b.lo[2] - A.z) * iz);
+    consider((b.hi[2] - A.z) * iz);
     float ts;
     if (g0 >= 0.0f) {
         ts = 0.0f;
