@@ -1,26 +1,27 @@
 // FP32 bulk pass of RVS_PREC_FP32_REFINE (gfx950).
 //
-// The same case tree and early-exit loop as the reference-exact kernel, in
+// The reference-exact kernel's early-exit loop over (link, obstacle) pairs in
 // relaxed single precision: kinematics in float, FMA contraction (this
 // translation unit is compiled with -ffp-contract=fast), hardware reciprocal
-// and square root.  Its results differ from the reference arithmetic by a few
-// 1e-7 m, so every (link, obstacle) pair it evaluates is screened and the
-// configuration is sent to the exact refinement pass when any pair is
-// uncertain:
-//   |d - r| < band                the verdict sign could differ
+// and square root, and the capsule-box distance by flat_capsule_box, a
+// branch-free form of the reference's case tree (rvs_geometry.hpp).  Its
+// results differ from the reference arithmetic by a few 1e-7 m, so every pair
+// it evaluates is screened and the configuration is sent to the exact
+// refinement pass when any pair is uncertain:
+//   end point within band of the box surface  the inside test could differ
 //   raw distance < EQ + band      the reference snaps distances below
-//                                 EQUALITY_THRESHOLD to 0 (then -r)
-//   end point inside, depth < band  the inside test could differ
-//   face crossing, margin < band  the crossing test could differ
+//                                 EQUALITY_THRESHOLD to 0 (then -r), touches
+//                                 a face, or takes its NaN path
+//   |d - r| < band                the verdict sign could differ
 //   |d - tol| < band              the min_dist_tol filter could differ
-//   sin^2 < 1e-6, raw < 0.01      a near-parallel segment solve close by
-//   zero-length segment and edge  the reference's 0/0 candidates
+//   near-axis-parallel, raw < 0.01  an ill-conditioned segment solve close by
+//   zero-length segment           the reference's 0/0 projections
 //   non-finite skeleton           NaN / Inf configurations
-// (inside points and face crossings are screened by how far the comparison
-// that decided them was from flipping, since their results 0 and -r carry no
-// rounding.)  Configurations outside all of these get verdicts identical to the reference
-// (a sign decided with margin >= band cannot flip under a 1e-6 m error), and
-// the same early-exit pair, so the same partial profile up to rounding.
+// A crossing is reported only when the segment meets the box shrunk by band
+// on every side, so it is a crossing whatever the rounding.  Configurations
+// outside all of these get verdicts identical to the reference (a sign
+// decided with margin >= band cannot flip under a 1e-6 m error), and the same
+// early-exit pair, so the same partial profile up to rounding.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>

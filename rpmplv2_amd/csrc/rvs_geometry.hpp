@@ -16,9 +16,9 @@
 // compiled without FMA contraction gives results bit-identical to the
 // reference's float arithmetic (T = float) or the same algorithm in FP64
 // (T = double); FastMath (hardware reciprocal and square root, contraction
-// allowed by its translation unit) is the relaxed FP32 bulk pass, which also
-// measures box edges with seg_edge, the segment-segment solve reduced for an
-// axis-aligned edge (same cases, different rounding).
+// allowed by its translation unit) is the relaxed FP32 bulk pass, which runs
+// flat_capsule_box, a branch-free form of the capsule-box distance, instead of
+// the case tree.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -55,14 +55,12 @@ template <typename T> __host__ __device__ inline T sqnorm(const Vec3<T>& a) { re
 __host__ __device__ inline float vsqrt(float x) { return sqrtf(x); }
 __host__ __device__ inline double vsqrt(double x) { return sqrt(x); }
 struct ExactMath {
-    static constexpr bool kAxisEdges = false;   // box edges through the general seg_seg
     template <typename T> __host__ __device__ static T div(T a, T b) { return a / b; }
     template <typename T> __host__ __device__ static T root(T x) { return vsqrt(x); }
 };
 
 // Relaxed FP32: v_rcp_f32 / v_sqrt_f32 (about 1 ulp), device only.
 struct FastMath {
-    static constexpr bool kAxisEdges = true;    // box edges through seg_edge
     __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static float root(float x) { return __builtin_amdgcn_sqrtf(x); }
 };
@@ -163,67 +161,6 @@ __host__ __device__ inline T seg_seg(const Vec3<T>& A, const Vec3<T>& B, const V
         }
     }
     return d_c;
-}
-
-// Squared sine of the angle between AB and CD, from the same quantities the
-// reference divides by; small values flag an ill-conditioned interior solve.
-template <typename T, class MP = ExactMath>
-__host__ __device__ inline T seg_seg_sin2(const Vec3<T>& A, const Vec3<T>& B, const Vec3<T>& C, const Vec3<T>& D) {
-    const Vec3<T> u = B - A, v = D - C;
-    const T uu = sqnorm(u), vv = sqnorm(v), uv = dot(u, v);
-    const T den = uu * vv;
-    return den > 0 ? MP::div(den - uv * uv, den) : T(1);
-}
-
-// The same distance of AB to an axis-aligned box edge, for the relaxed bulk
-// pass: the edge runs from corner C along axis ax for a length L (D = C + L e),
-// so every product with D - C is one component.  The interior solution and the
-// four clamped candidates are the reference's (parameters clamped to [0, 1]
-// instead of choosing the end point: the same point up to rounding); results
-// differ from seg_seg by rounding only.  sin2 receives the squared sine of the
-// angle between the segments (1 when either has zero length, as seg_seg_sin2).
-// A zero-length AB or edge alone gives the reference's value: its NaN
-// candidates are the projections of the point, never below the point's
-// distance to the other segment, which the remaining candidates hold.  When
-// both have zero length the reference returns MAX_DISTANCE and this routine
-// the point distance: sin2 is then -1, for the caller to send the pair to the
-// exact pass.
-template <class MP>
-__host__ __device__ inline float seg_edge(const Vec3<float>& A, const Vec3<float>& B, const Vec3<float>& C, int ax, float L,
-                                 const Params<float>& p, float& sin2) {
-    const Vec3<float> BA = B - A, AC = A - C;
-    const float ex = ax == 0 ? 1.0f : 0.0f, ey = ax == 1 ? 1.0f : 0.0f, ez = ax == 2 ? 1.0f : 0.0f;
-    const float ba = BA[ax], ac = AC[ax];
-    const float alpha1 = sqnorm(BA);
-    const float g = -dot(AC, BA);         // (A - C).(A - B)
-    const float w = alpha1 - ba * ba;     // alpha1 sin^2
-    const float ia = MP::div(1.0f, alpha1);
-    sin2 = (alpha1 > 0.0f && L > 0.0f) ? w * ia : ((alpha1 > 0.0f || L > 0.0f) ? 1.0f : -1.0f);
-    // interior: the point sL along the edge from C, t along AB
-    const float sL = MP::div(alpha1 * ac + ba * g, w);
-    const float t = (g + ba * sL) * ia;
-    if (t > 0.0f && t < 1.0f && sL > 0.0f && sL < L) {
-        // (C + sL e) - (A + t BA)
-        const float dx = sL * ex - (AC.x + t * BA.x);
-        const float dy = sL * ey - (AC.y + t * BA.y);
-        const float dz = sL * ez - (AC.z + t * BA.z);
-        const float d = MP::root(dx * dx + (dy * dy + dz * dz));
-        return d < p.eq ? 0.0f : d;
-    }
-    // C and D projected on AB, A and B projected on the edge, all clamped
-    const float o0 = fminf(fmaxf(g * ia, 0.0f), 1.0f);
-    const float o1 = fminf(fmaxf((g + L * ba) * ia, 0.0f), 1.0f);
-    const float x0 = AC.x + o0 * BA.x, y0 = AC.y + o0 * BA.y, z0 = AC.z + o0 * BA.z;
-    const float x1 = AC.x + o1 * BA.x - L * ex, y1 = AC.y + o1 * BA.y - L * ey, z1 = AC.z + o1 * BA.z - L * ez;
-    const float ca = fminf(fmaxf(ac, 0.0f), L), bc = ba + ac, cb = fminf(fmaxf(bc, 0.0f), L);
-    const float xa = AC.x - ca * ex, ya = AC.y - ca * ey, za = AC.z - ca * ez;
-    const float xb = AC.x + BA.x - cb * ex, yb = AC.y + BA.y - cb * ey, zb = AC.z + BA.z - cb * ez;
-    float d = p.maxd;
-    d = smin(d, MP::root(x0 * x0 + (y0 * y0 + z0 * z0)));
-    d = smin(d, MP::root(x1 * x1 + (y1 * y1 + z1 * z1)));
-    d = smin(d, MP::root(xa * xa + (ya * ya + za * za)));
-    d = smin(d, MP::root(xb * xb + (yb * yb + zb * zb)));
-    return d;
 }
 
 // ---------------------------------------------------------------- seg-pt
@@ -327,40 +264,9 @@ __host__ __device__ inline int face_side_edges(const Box<T>& b, int face, const 
 // dereference a missing nearest-point matrix (end point projections more than
 // MAX_DISTANCE away, or an end point exactly on a face plane with the other
 // inside the face's span); the distance is then NaN.
-// Side information for the FP32 bulk pass's refinement trigger.
-template <typename T>
-struct PairInfo {
-    T raw;         // segment-box distance before the radius is subtracted
-    T depth;       // end point inside the box: its distance to the nearest face
-    T hit_margin;  // face crossing: distance of the decision from flipping
-    T sin2;        // smallest squared sine over the segment-segment solves
-    bool inside;
-    bool hit;
-};
-
-// How far the crossing of face 'face' is from being decided the other way:
-// the smaller of the crossing point's distances to the rectangle's sides and
-// the end points' distances to the face plane (negative when not crossed).
-template <typename T, class MP = ExactMath>
-__host__ __device__ inline T face_margin(const Vec3<T>& A, const Vec3<T>& B, const Box<T>& b, int face) {
-    const int c = face % 3;
-    const int o1 = (c == 0) ? 1 : 0, o2 = (c == 2) ? 1 : 2;
-    const T v = b.f(face);
-    const T den = B[c] - A[c];
-    if (den == 0) return T(-1);
-    const T t = MP::div(v - A[c], den);
-    if (!(t > 0 && t < 1)) return T(-1);
-    const T m0 = A[o1] + t * (B[o1] - A[o1]);
-    const T m1 = A[o2] + t * (B[o2] - A[o2]);
-    T mg = smin(smin(m0 - b.lo_at(o1), b.hi_at(o1) - m0), smin(m1 - b.lo_at(o2), b.hi_at(o2) - m1));
-    const T ea = v - A[c], eb = B[c] - v;
-    mg = smin(mg, smin(ea < 0 ? -ea : ea, eb < 0 ? -eb : eb));
-    return mg;
-}
-
 template <typename T, bool WANT_NP, class MP = ExactMath>
 __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T radius, const Box<T>& b,
-                                         const Params<T>& p, NP<T>* np, PairInfo<T>* info, uint32_t* status) {
+                                         const Params<T>& p, NP<T>* np, uint32_t* status) {
     // --- projectionLineSegOnSide x3 (lines 411-413, 450-474); bit f of the
     // masks is face f (xmin, ymin, zmin, xmax, ymax, zmax)
     int pmA = 0, pmB = 0;
@@ -375,16 +281,7 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
             int& pm = i ? pmB : pmA;
             T& dist = i ? distB : distA;
             if (X[o1] >= b.lo[o1] && X[o1] <= b.hi[o1] && X[o2] >= b.lo[o2] && X[o2] <= b.hi[o2]) {
-                if (X[k] > b.lo[k] && X[k] < b.hi[k]) {
-                    inside = true;
-                    if (info) {
-                        // distance of the inside point to the box surface
-                        T dm = smin(smin(X.x - b.lo[0], b.hi[0] - X.x), smin(X.y - b.lo[1], b.hi[1] - X.y));
-                        dm = smin(dm, smin(X.z - b.lo[2], b.hi[2] - X.z));
-                        info->depth = info->inside ? smin(info->depth, dm) : dm;
-                        info->inside = true;
-                    }
-                }
+                if (X[k] > b.lo[k] && X[k] < b.hi[k]) inside = true;
                 else if (X[k] <= b.lo[k]) { pm |= 1 << k; dist = b.lo[k] - X[k]; }
                 else if (X[k] >= b.hi[k]) { pm |= 8 << k; dist = X[k] - b.hi[k]; }
             }
@@ -392,7 +289,6 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
     }
     if (inside) {
         if (WANT_NP) np->valid = false;
-        if (info) info->raw = T(0);
         return T(0);
     }
 
@@ -469,14 +365,6 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
     else if (nfaces == 6)
         for (int f = 0; f < 6; ++f) hit = hit || face_crossed<T, MP>(A, B, b, f, p);
 
-    if (info && hit) {
-        T mg = T(-1);
-        if (nfaces == 1) mg = face_margin<T, MP>(A, B, b, face1);
-        else
-            for (int f = 0; f < 6; ++f) { const T m = face_margin<T, MP>(A, B, b, f); mg = (m > mg) ? m : mg; }
-        info->hit = true;
-        info->hit_margin = mg;
-    }
     if (hit) {
         d_c = T(0);
     } else if (no_edge) {
@@ -490,19 +378,7 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
         for (int e = 0; e < nedges; ++e) {
             const int ed = list_edge(kind, param, e);
             NP<T> t;
-            T d_tmp;
-            if constexpr (MP::kAxisEdges && !WANT_NP) {
-                // the corners differ in one bit: the edge's axis
-                const int c0 = ed & 7, c1 = ed >> 3, bit = c0 ^ c1;
-                const int ax = bit == 1 ? 0 : (bit == 2 ? 1 : 2);
-                T s2;
-                d_tmp = seg_edge<MP>(A, B, b.corner(c0 & c1), ax, b.hi_at(ax) - b.lo_at(ax), p, s2);
-                if (info) info->sin2 = smin(info->sin2, s2);
-            } else {
-                const Vec3<T> C = b.corner(ed & 7), D = b.corner(ed >> 3);
-                d_tmp = seg_seg<T, WANT_NP, MP>(A, B, C, D, p, &t);
-                if (info) info->sin2 = smin(info->sin2, seg_seg_sin2<T, MP>(A, B, C, D));
-            }
+            const T d_tmp = seg_seg<T, WANT_NP, MP>(A, B, b.corner(ed & 7), b.corner(ed >> 3), p, &t);
             if (direct) {
                 d_c = d_tmp;
                 if (WANT_NP) *np = t;
@@ -515,7 +391,6 @@ __host__ __device__ inline T capsule_box(const Vec3<T>& A, const Vec3<T>& B, T r
             }
         }
     }
-    if (info) info->raw = d_c;
     return d_c - radius;
 }
 

@@ -90,14 +90,14 @@ __device__ __attribute__((always_inline)) inline void distance_body(const RobotD
                 // oriented box: the pair is solved in the box frame
                 const float* x = s.xf + 12 * j;
                 const Box<T> bx = s.box[j];
-                dt = capsule_box<T, WANT_NP>(to_box(x, P0), to_box(x, P1), r, bx, p, &np, nullptr, nullptr);
+                dt = capsule_box<T, WANT_NP>(to_box(x, P0), to_box(x, P1), r, bx, p, &np, nullptr);
                 if (WANT_NP && np.valid) {
                     np.r = from_box(x, np.r);
                     np.o = from_box(x, np.o);
                 }
             } else {
                 const Box<T> bx = s.box[j];
-                dt = capsule_box<T, WANT_NP>(P0, P1, r, bx, p, &np, nullptr, nullptr);
+                dt = capsule_box<T, WANT_NP>(P0, P1, r, bx, p, &np, nullptr);
             }
             if (dt > s.tol[j]) dt = p.maxd;
             prof = smin(prof, dt);
@@ -206,14 +206,14 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
                 } else if (OBB) {
                     const float* x = s.xf + 12 * j;
                     const Box<float> bx = s.box[j];
-                    dt = capsule_box<float, WANT_NP>(to_box(x, P0), to_box(x, P1), r, bx, p, &np, nullptr, nullptr);
+                    dt = capsule_box<float, WANT_NP>(to_box(x, P0), to_box(x, P1), r, bx, p, &np, nullptr);
                     if (WANT_NP && np.valid) {
                         np.r = from_box(x, np.r);
                         np.o = from_box(x, np.o);
                     }
                 } else {
                     const Box<float> bx = s.box[j];
-                    dt = capsule_box<float, WANT_NP>(P0, P1, r, bx, p, &np, nullptr, nullptr);
+                    dt = capsule_box<float, WANT_NP>(P0, P1, r, bx, p, &np, nullptr);
                 }
                 return dt > s.tol[j] ? p.maxd : dt;
             };
@@ -367,9 +367,9 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine_pairs(Rob
                     } else if (OBB) {
                         const float* x = s.xf + 12 * j;
                         dt = capsule_box<float, false>(to_box(x, A), to_box(x, B), rb.radius[i], s.box[j], p, nullptr,
-                                                       nullptr, nullptr);
+                                                       nullptr);
                     } else {
-                        dt = capsule_box<float, false>(A, B, rb.radius[i], s.box[j], p, nullptr, nullptr, nullptr);
+                        dt = capsule_box<float, false>(A, B, rb.radius[i], s.box[j], p, nullptr, nullptr);
                     }
                     if (dt > s.tol[j]) dt = p.maxd;
                     row[pp] = dt;

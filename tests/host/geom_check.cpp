@@ -17,12 +17,11 @@ using namespace rvs;
 static bool same(float a, float b) { return (std::isnan(a) && std::isnan(b)) || std::memcmp(&a, &b, 4) == 0 || (a == b); }
 static bool same(double a, double b) { return (std::isnan(a) && std::isnan(b)) || (a == b); }
 
-// IEEE arithmetic through the relaxed bulk pass's axis-aligned edge routine
-// (seg_edge): checks that routine's algebra and its use in capsule_box on the
-// host, to rounding (the bulk pass itself uses hardware reciprocal and square
-// root, which exist on the GPU only).
+// IEEE arithmetic through the relaxed bulk pass's branch-free capsule-box form
+// (flat_capsule_box): checks its algebra and screening on the host, to
+// rounding (the bulk pass itself uses hardware reciprocal and square root,
+// which exist on the GPU only).
 struct AxisMath {
-    static constexpr bool kAxisEdges = true;
     template <typename U> static U div(U a, U b) { return a / b; }
     template <typename U> static U root(U x) { return vsqrt(x); }
 };
@@ -40,9 +39,9 @@ int run(long n, uint64_t seed, const char* tag) {
     Rng<T> R(seed);
     orc_params op{1e-4f, 1000.0f, 10};
     Params<T> p{(T)op.eq_threshold, (T)op.max_distance};
-    long bad_cb = 0, bad_np = 0, bad_col = 0, bad_ss = 0, bad_sp = 0, bad_pl = 0, bad_ax = 0, flag_ax = 0;
+    long bad_cb = 0, bad_np = 0, bad_col = 0, bad_ss = 0, bad_sp = 0, bad_pl = 0;
     long bad_flat = 0, flag_flat = 0;
-    double max_ax = 0, max_flat = 0;
+    double max_flat = 0;
     long paths[10] = {0};
     for (long it = 0; it < n; ++it) {
         // box
@@ -86,7 +85,7 @@ int run(long n, uint64_t seed, const char* tag) {
         if (opath >= 0 && opath < 10) paths[opath]++;
         NP<T> np{};
         uint32_t st = 0;
-        T gd = capsule_box<T, true>(a, bb, r, b, p, &np, nullptr, &st);
+        T gd = capsule_box<T, true>(a, bb, r, b, p, &np, &st);
         if (!same(gd, od)) {
             if (bad_cb < 5) printf("[%s] capsule_box mismatch it=%ld path=%d gpu=%.9g oracle=%.9g\n", tag, it, opath, (double)gd, (double)od);
             bad_cb++;
@@ -95,22 +94,6 @@ int run(long n, uint64_t seed, const char* tag) {
             T g6[6] = {np.r.x, np.r.y, np.r.z, np.o.x, np.o.y, np.o.z};
             for (int k = 0; k < 6 && ok; ++k) ok = same(g6[k], onp[k]);
             if (!ok) { if (bad_np < 5) printf("[%s] nearest-point mismatch it=%ld path=%d\n", tag, it, opath); bad_np++; }
-        }
-        if constexpr (sizeof(T) == 4) {
-            // the same pair with box edges through seg_edge: equal to rounding,
-            // except where the exact pass takes over (a distance that
-            // EQUALITY_THRESHOLD snapping may decide, or the sin2 = -1 flag)
-            PairInfo<float> info{p.maxd, p.maxd, -1.0f, 1.0f, false, false};
-            const float ge = capsule_box<float, false, AxisMath>(a, bb, r, b, p, nullptr, &info, nullptr);
-            const double e = std::fabs((double)ge - (double)od);
-            if (info.sin2 < -0.5f) flag_ax++;
-            else if (!(std::isnan(ge) && std::isnan(od)) && info.raw >= p.eq + 1e-5f) {
-                if (!(e <= 2e-6)) {
-                    if (bad_ax < 5) printf("[%s] axis-edge mismatch it=%ld path=%d fast=%.9g oracle=%.9g\n", tag, it, opath, (double)ge, (double)od);
-                    bad_ax++;
-                }
-                if (e > max_ax) max_ax = e;
-            }
         }
         if constexpr (sizeof(T) == 4) {
             // the flat evaluation of the per-lane walk: where it does not flag
@@ -163,10 +146,9 @@ int run(long n, uint64_t seed, const char* tag) {
     printf("[%s] pairs=%ld capsule_box=%ld nearest=%ld collision=%ld seg_seg=%ld seg_pt=%ld paths:", tag, n, bad_cb,
            bad_np, bad_col, bad_ss, bad_sp);
     for (int k = 0; k < 10; ++k) printf(" %ld", paths[k]);
-    if (sizeof(T) == 4) printf(" axis_edges=%ld (flagged %ld, max error %.3g m)", bad_ax, flag_ax, max_ax);
     if (sizeof(T) == 4) printf(" flat=%ld (flagged %ld, max error %.3g m)", bad_flat, flag_flat, max_flat);
     printf("\n");
-    return (bad_cb || bad_np || bad_col || bad_ss || bad_sp || bad_ax || bad_flat) ? 1 : 0;
+    return (bad_cb || bad_np || bad_col || bad_ss || bad_sp || bad_flat) ? 1 : 0;
 }
 
 int main(int argc, char** argv) {

@@ -73,7 +73,6 @@ def test_device_geometry_source_matches_oracle_on_host():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "capsule_box=0 nearest=0 collision=0 seg_seg=0 seg_pt=0" in r.stdout
-    assert "axis_edges=0 " in r.stdout
     assert "flat=0 " in r.stdout
 
 
