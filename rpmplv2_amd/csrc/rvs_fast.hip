@@ -846,6 +846,9 @@ __global__ __launch_bounds__(kBoxThreads) void k_links_setup(RobotDev rb, EnvDev
     if (threadIdx.x == 0) *warm0 = bj[0];
 }
 
+// A round's sort ranks (one word per thread) reuse the queue rings.
+static_assert(kLaneQueue * sizeof(int16_t) >= sizeof(uint32_t), "queue rings hold a word per thread");
+
 // In-place ascending bitonic sort of blockDim.x (a power of 2) 64-bit
 // entries in LDS, one a thread; the whole workgroup calls it.
 __device__ inline void block_sort(uint64_t* v) {
@@ -869,8 +872,10 @@ __device__ inline void block_sort(uint64_t* v) {
 // blockDim.x consecutive entries of the link's list from a counter, 64 per
 // wave, so that the lanes of a wave hold the same link of nearby
 // configurations.  Per configuration:
-//   1. the link's end points, from the joint angles (the same compositions
-//      in the same order as the incremental walk, so the same bits);
+//   1. the link's end points: the forward-kinematics walk goes on from the
+//      frame the previous link's launch left in the list entry (the same
+//      compositions in the same order as the incremental walk, so the same
+//      bits);
 //   2. the warm pair (the previous link's nearest obstacle; for the first
 //      link the obstacle nearest to the first skeleton point), all lanes in
 //      one step;
@@ -899,9 +904,9 @@ __device__ inline void block_sort(uint64_t* v) {
 // (the queueing bound is never tighter than the walk's, a member's box lies
 // inside its group's, and every pair is tested again with exactly the bound
 // the walk has when it reaches it).  A configuration that goes on keeps its
-// running minimum, warm pair and screening flag in st_d / st_w; a round's
-// survivors are sorted by the Morton code (over box) of their next link's
-// start point and appended to the next link's list together, so that the
+// running minimum, warm pair, screening flag and walk state in its entry
+// of the next link's list; a round's survivors are sorted by the Morton code (over box) of
+// their next link's start point and appended to that list together, so that the
 // next launch's waves again hold nearby segments.  The lists' order between
 // rounds depends on timing; no configuration's results do.  (Rounds of one
 // 64-entry chunk per wave measured fastest: a window twice as long per
@@ -910,11 +915,12 @@ __device__ inline void block_sort(uint64_t* v) {
 template <bool OBB>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_links(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
-    uint32_t* count, int link, const int32_t* __restrict__ in, const uint32_t* __restrict__ in_count,
-    uint32_t* cursor, int32_t* __restrict__ next_list, uint32_t* next_count, float* __restrict__ st_d,
-    uint32_t* __restrict__ st_w, const float* __restrict__ box, const int32_t* __restrict__ warm0, size_t cull_off,
-    size_t queue_off, size_t sort_off) {
-    // configurations at this link: the first takes all n (in = the walk order, or null)
+    uint32_t* count, int link, const int32_t* __restrict__ perm, const LinkEntry* __restrict__ in,
+    const uint32_t* __restrict__ in_count, uint32_t* cursor, LinkEntry* __restrict__ next_list, uint32_t* next_count,
+    const float* __restrict__ box, const int32_t* __restrict__ warm0, size_t cull_off, size_t queue_off,
+    size_t sort_off) {
+    // configurations at this link: the first takes all n (in the walk order
+    // perm, or the order given), the others the previous link's list
     const int64_t cnt = link == 0 ? n : (int64_t)*in_count;
     const int B = blockDim.x;
     if ((int64_t)blockIdx.x * B >= cnt) return;   // no round for this workgroup (uniform)
@@ -954,20 +960,30 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         int warm = first_warm;
         bool uncertain = false;
         Vec3<float> P0{0.0f, 0.0f, 0.0f}, P1{0.0f, 0.0f, 0.0f};
+        FKf fk;
+        fk.M = Mat3f{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+        fk.px = fk.py = fk.pz = 0.0f;
+        fk.seg = 0;
+        fk.qi = 0;
         if (busy) {
-            c = in ? (int64_t)in[e] : e;
             if (i > 0) {
-                d = st_d[c];
-                const uint32_t sw = st_w[c];
-                warm = (int)(sw >> 1) - 1;
-                uncertain = (sw & 1u) != 0;
+                // the state the previous link left, its walk included (the
+                // walk goes on from the same floats, so the same bits)
+                const LinkEntry en = in[e];
+                c = en.c;
+                d = en.d;
+                warm = (int)(en.w >> 1) - 1;
+                uncertain = (en.w & 1u) != 0;
+                fk.M = Mat3f{en.m[0], en.m[1], en.m[2], en.m[3], en.m[4], en.m[5], en.m[6], en.m[7], en.m[8]};
+                fk.px = en.p[0];
+                fk.py = en.p[1];
+                fk.pz = en.p[2];
+                fk.seg = (int)(en.walk & 0xffffu);
+                fk.qi = (int)(en.walk >> 16);
+            } else {
+                c = perm ? (int64_t)perm[e] : e;
             }
             const float* qc = q + c * D;
-            FKf fk;
-            fk.M = Mat3f{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
-            fk.px = fk.py = fk.pz = 0.0f;
-            fk.seg = 0;
-            fk.qi = 0;
             P0 = skelf_point(fk, rb, qc, i);
             P1 = skelf_point(fk, rb, qc, i + 1);
             if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
@@ -1079,29 +1095,40 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
             if (finished) {
                 if (out.d) out.d[c] = hit ? 0.0f : d;
                 if (out.verdict) out.verdict[c] = hit ? 0 : 1;
-            } else {
-                st_d[c] = d;
-                st_w[c] = ((uint32_t)(best + 1) << 1) | (uncertain ? 1u : 0u);
             }
         }
         append_flagged(busy && finished && uncertain, c, idx, count);
         // survivors, with the sort key of their next link's start point
         const bool keep = busy && !finished;
-        sorted[threadIdx.x] = keep ? ((uint64_t)point_key(P1, kbox) << 32) | (uint64_t)(uint32_t)c : ~0ull;
-        // the round's survivors, sorted, appended together
+        sorted[threadIdx.x] = keep ? ((uint64_t)point_key(P1, kbox) << 32) | (uint64_t)threadIdx.x : ~0ull;
+        // the round's survivors, sorted, appended together: each thread
+        // learns its place in the sorted order (every wave has left its walk,
+        // so the queue rings hold the ranks) and writes its own entry there
         __syncthreads();
         block_sort(sorted);
+        uint32_t* const rank = reinterpret_cast<uint32_t*>(smem + queue_off);
+        const uint64_t sk = sorted[threadIdx.x];
+        if (sk != ~0ull) rank[(uint32_t)sk] = threadIdx.x;
         if (threadIdx.x == 0) {
             int k = 0;   // survivors come first (the others hold the largest key)
             for (int h = B; h > 0; h >>= 1)
                 if (k + h <= B && sorted[k + h - 1] != ~0ull) k += h;
             out_base = k ? atomicAdd(next_count, (uint32_t)k) : 0u;
-            round_id = (uint32_t)k;
         }
         __syncthreads();
-        const int nkeep = (int)round_id;
-        if ((int)threadIdx.x < nkeep) next_list[out_base + threadIdx.x] = (int32_t)(uint32_t)sorted[threadIdx.x];
-        __syncthreads();   // round_id and the sort buffer are written again next round
+        if (keep) {
+            LinkEntry en;
+            en.c = (int32_t)c;
+            en.d = d;
+            en.w = ((uint32_t)(best + 1) << 1) | (uncertain ? 1u : 0u);
+            en.walk = (uint32_t)fk.seg | ((uint32_t)fk.qi << 16);
+            en.m[0] = fk.M.m0; en.m[1] = fk.M.m1; en.m[2] = fk.M.m2;
+            en.m[3] = fk.M.m3; en.m[4] = fk.M.m4; en.m[5] = fk.M.m5;
+            en.m[6] = fk.M.m6; en.m[7] = fk.M.m7; en.m[8] = fk.M.m8;
+            en.p[0] = fk.px; en.p[1] = fk.py; en.p[2] = fk.pz;
+            next_list[out_base + rank[threadIdx.x]] = en;
+        }
+        __syncthreads();   // round_id, the sort buffer and the rings are written again next round
     }
 }
 
@@ -1445,10 +1472,10 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
                 e = hipGetLastError();
             }
             for (int i = 0; i < rb.num_links && e == hipSuccess; ++i) {
-                const int32_t* in = i == 0 ? perm : ll.list[(i - 1) & 1];
+                const LinkEntry* in = i == 0 ? nullptr : ll.list[(i - 1) & 1];
                 hipLaunchKernelGGL((k_distance_fast_links<OBB>), g, b, lsm, st, rb, env, qp, q, n, out, idx, count, i,
-                                   in, ll.count + i, ll.cursor + i, ll.list[i & 1], ll.count + i + 1, ll.d, ll.w,
-                                   ll.box, ll.warm0, cull_off, queue_off, sort_off);
+                                   perm, in, ll.count + i, ll.cursor + i, ll.list[i & 1], ll.count + i + 1, ll.box,
+                                   ll.warm0, cull_off, queue_off, sort_off);
                 e = hipGetLastError();
             }
             return e;

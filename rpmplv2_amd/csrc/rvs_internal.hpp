@@ -151,17 +151,26 @@ inline CohLayout coh_layout(uint32_t* base, int64_t n) {
 hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n, const FastTuning& t, uint32_t* coh,
                                   const int32_t** perm, hipStream_t st);
 // Scratch of the link-major bulk pass (tuning variant 4), 256-byte aligned:
-// two lists of the configurations alive at a link, used in turn, each
-// configuration's running minimum
-// and (warm pair + 1) << 1 | screening flag between links, the lists' sizes
-// per link and the round counters per link (kLinksCounters words, cleared
-// together), the box that quantizes the sort keys and the first link's warm
-// pair.
+// two lists of the configurations alive at a link, used in turn, whose
+// entries carry each configuration's state between links (read and written
+// in list order, so the state moves in whole lines): its running minimum,
+// (warm pair + 1) << 1 | screening flag and the forward-kinematics walk up to
+// the link's end point, so that the next link goes on from there; then the
+// lists' sizes per link and the round counters per link (kLinksCounters
+// words, cleared together), the box that quantizes the sort keys and the
+// first link's warm pair.
+struct alignas(16) LinkEntry {
+    int32_t c;       // configuration
+    float d;         // running minimum over the links before
+    uint32_t w;      // (warm pair + 1) << 1 | screening flag
+    uint32_t walk;   // the walk's next segment | next joint << 16
+    float m[9];      // the walk's frame: orientation (row major)
+    float p[3];      // and origin
+};
+static_assert(sizeof(LinkEntry) == 64, "LinkEntry is four 16-byte pieces");
 constexpr int kLinksCounters = 2 * kMaxLinks + 2;
 struct LinksLayout {
-    int32_t* list[2];
-    float* d;
-    uint32_t* w;
+    LinkEntry* list[2];
     uint32_t* count;    // [kMaxLinks + 1] list sizes, then
     uint32_t* cursor;   // [kMaxLinks + 1] round counters
     float* box;         // lo[3], 1024 / extent[3]
@@ -169,16 +178,14 @@ struct LinksLayout {
 };
 inline size_t fast_links_bytes(int64_t n) {
     const size_t m = (size_t)std::max<int64_t>(n, 0);
-    return 4 * (4 * coh_words(m) + coh_words(kLinksCounters) + coh_words(8));
+    return 4 * (2 * coh_words(16 * m) + coh_words(kLinksCounters) + coh_words(8));
 }
 inline LinksLayout links_layout(uint32_t* base, int64_t n) {
-    const size_t m = coh_words((size_t)std::max<int64_t>(n, 0));
+    const size_t lw = coh_words(16 * (size_t)std::max<int64_t>(n, 0));
     LinksLayout L;
-    L.list[0] = reinterpret_cast<int32_t*>(base);
-    L.list[1] = L.list[0] + m;
-    L.d = reinterpret_cast<float*>(L.list[1] + m);
-    L.w = reinterpret_cast<uint32_t*>(L.d + m);
-    L.count = L.w + m;
+    L.list[0] = reinterpret_cast<LinkEntry*>(base);
+    L.list[1] = reinterpret_cast<LinkEntry*>(base + lw);
+    L.count = base + 2 * lw;
     L.cursor = L.count + kMaxLinks + 1;
     L.box = reinterpret_cast<float*>(L.count + coh_words(kLinksCounters));
     L.warm0 = reinterpret_cast<int32_t*>(L.box + 6);
