@@ -64,8 +64,13 @@ PEAK_HBM_GBPS = 8000.0
 COUNTER_FLOP_PER_CONFIG = {
     ("xarm6_16aabb_ground", "k_distance_fast"): (2617.7, "profiles/k_distance_fast_xarm6_1M.md"),
     ("chain22_512rotated_boxes", "k_distance_fast_links"):
-        (24310.0, "profiles/k_distance_fast_links_chain22_512obb.md"),
+        (15429.2, "profiles/k_distance_fast_links_chain22_512obb.md"),
 }
+# HBM bytes per configuration of the link-major bulk pass on the 22-DOF chain
+# with 512 rotated boxes (all 22 launches of a call): FETCH_SIZE 1,735,085.4
+# KiB x 2.0 + WRITE_SIZE 1,899,255.9 KiB x 1.0 per call on 2,097,152
+# configurations (profiles/k_distance_fast_links_chain22_512obb.md).
+LINKS_TRAFFIC_PER_CONFIG = (2.0 * 1735085.4 + 1899255.9) * 1024 / 2097152
 # HBM bytes per launch of the FP32 bulk kernel on the headline workload from
 # two `rocprofv3 --pmc` runs: FETCH_SIZE 130,780.8 KiB x 2.0 + WRITE_SIZE
 # 124,071.9 KiB x 1.0, the factors being what the same counters read for a
@@ -275,7 +280,12 @@ def workload5_section(eng, cores: int, n: int) -> dict:
                                  counts_from="oracle restatement of the bulk pass's walk on the "
                                              "65,536-configuration subsample",
                                  algorithmic_bytes_per_config=bytes_cfg,
-                                 hbm_GBps_algorithmic=bytes_cfg * n / (bulk_o * 1e-3) / 1e9)},
+                                 hbm_GBps_algorithmic=bytes_cfg * n / (bulk_o * 1e-3) / 1e9,
+                                 **({"traffic_bytes_per_config": LINKS_TRAFFIC_PER_CONFIG,
+                                     "traffic_GBps": LINKS_TRAFFIC_PER_CONFIG * n / (bulk_o * 1e-3) / 1e9,
+                                     "traffic_note": "FETCH_SIZE x 2 + WRITE_SIZE over the 22 link launches, per "
+                                                     "configuration (profiles/k_distance_fast_links_chain22_512obb.md)"}
+                                    if tune["variant"] == 4 else {}))},
         "aabb_of_obb_reference_semantics": {
             "configs_per_s": n / (ms_a * 1e-3), "ms": ms_a, "bulk_ms": bulk_a, "refine_ms": ref_a,
             "refined": nref_a, "collision_free_fraction": free_a,
