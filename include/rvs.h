@@ -51,8 +51,12 @@ typedef enum rvs_precision {
      * EQUALITY_THRESHOLD snap, a marginal inside/crossing decision, near a
      * min_dist_tol, a near-parallel segment solve, or non-finite input) is
      * recomputed with the RVS_PREC_REF32 arithmetic (FP64 kinematics,
-     * reference-order FP32 geometry).  Verdicts are those of
-     * RVS_PREC_REF32; distances stay within 1e-5 m of RVS_PREC_FP64. */
+     * reference-order FP32 geometry) from the first link with an uncertain
+     * pair on (from link 0 in batches above 2^25 configurations or without
+     * a profile output); the links before it keep the bulk pass's values,
+     * decided with the same margin as an unflagged configuration's.
+     * Verdicts are those of RVS_PREC_REF32; distances stay within 1e-5 m of
+     * RVS_PREC_FP64. */
     RVS_PREC_FP32_REFINE = 2
 } rvs_precision;
 

@@ -243,7 +243,7 @@ void group_bounds(const float* cull, const int32_t* order, int M, float* gcull) 
 }
 
 QueryParams params_of(const rvs_context* c) {
-    QueryParams qp;
+    QueryParams qp{};
     qp.eq = c->opt.equality_threshold;
     qp.maxd = c->opt.max_distance;
     qp.band = c->opt.band;

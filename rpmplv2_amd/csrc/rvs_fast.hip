@@ -344,13 +344,14 @@ __device__ __attribute__((always_inline)) inline void fast_one(const Staged<floa
     Vec3<float> P0 = skelf_point(fk, rb, qc, 0, qs);
     float d = p.maxd;
     bool hit = false, uncertain = false;
-    int i = 0, warm = -1;
+    int i = 0, warm = -1, uk = -1;   // uk: the first link with an uncertain decision
     for (; i < L; ++i) {
         const Vec3<float> P1 = skelf_point(fk, rb, qc, i + 1, qs);
         if (!isfinite(P1.x + P1.y + P1.z)) uncertain = true;
         const float r = rb.radius[i];
         const float prof = link_min<OBB, GROUPED>(s, env, M, i < rb.ground_included, P0, P1, r, p, band, active, warm,
                                                   hit, uncertain);
+        if (uncertain && uk < 0) uk = i;
         if (ROWS) prow[i * B] = prof;
         else if (active && out.profile) out.profile[c * L + i] = prof;
         if (hit) break;
@@ -369,7 +370,7 @@ __device__ __attribute__((always_inline)) inline void fast_one(const Staged<floa
         if (out.d) out.d[c] = d;
         if (out.verdict) out.verdict[c] = hit ? 0 : 1;
     }
-    append_flagged(active && uncertain, c, idx, count);
+    append_flagged(active && uncertain, flag_entry(c, uk, qp.flag_link != 0), idx, count);
 }
 
 // One lane per configuration: lane g takes configuration perm[g] of the walk
@@ -447,6 +448,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     bool drained = false;     // the counter has passed n
     int i = 0;                // current link
     int warm = -1;            // nearest obstacle of the previous link
+    int uk = -1;              // the first link with an uncertain decision
     const float* qc = q;
     FKf fk;
     Vec3<float> P0{0.0f, 0.0f, 0.0f};
@@ -474,6 +476,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     P0 = skelf_point(fk, rb, qc, 0);
                     i = 0;
                     warm = -1;
+                    uk = -1;
                     d = p.maxd;
                     uncertain = false;
                 } else {
@@ -495,6 +498,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                   hit, uncertain);
         bool finished = false;
         if (busy) {
+            if (uncertain && uk < 0) uk = i;
             if (out.profile) out.profile[c * L + i] = prof;
             if (!hit) d = smin(d, prof);
             if (hit || i == L - 1) {
@@ -508,7 +512,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 P0 = P1;
             }
         }
-        append_flagged(finished && uncertain, c, idx, count);
+        append_flagged(finished && uncertain, flag_entry(c, uk, qp.flag_link != 0), idx, count);
         if (finished) c = -1;
     }
 }
@@ -573,7 +577,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     FKf fk;
     Vec3<float> P0{0.0f, 0.0f, 0.0f};
     float d = 0.0f;
-    int i = 0, warm = -1;
+    int i = 0, warm = -1, uk = -1;   // uk: the first link with an uncertain decision
     bool uncertain = false;
     for (;;) {
         // refill the lanes without a configuration
@@ -597,6 +601,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                     P0 = skelf_point(fk, rb, qc, 0);
                     i = 0;
                     warm = -1;
+                    uk = -1;
                     d = p.maxd;
                     uncertain = false;
                 } else {
@@ -716,6 +721,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         // the link's value
         bool finished = false;
         if (busy) {
+            if (uncertain && uk < 0) uk = i;
             const bool hit = jh >= 0;
             const float lp = hit ? vh : prof;
             if (out.profile) out.profile[c * L + i] = lp;
@@ -738,7 +744,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                 if (out.verdict) out.verdict[c] = hit ? 0 : 1;
             }
         }
-        append_flagged(finished && uncertain, c, idx, count);
+        append_flagged(finished && uncertain, flag_entry(c, uk, qp.flag_link != 0), idx, count);
         if (finished) c = -1;
     }
 }
@@ -958,6 +964,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         int64_t c = 0;
         float d = p.maxd;
         int warm = first_warm;
+        int uk = -1;   // the first link with an uncertain decision
         bool uncertain = false;
         Vec3<float> P0{0.0f, 0.0f, 0.0f}, P1{0.0f, 0.0f, 0.0f};
         FKf fk;
@@ -978,8 +985,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                 fk.px = en.p[0];
                 fk.py = en.p[1];
                 fk.pz = en.p[2];
-                fk.seg = (int)(en.walk & 0xffffu);
-                fk.qi = (int)(en.walk >> 16);
+                fk.seg = (int)(en.walk & 0xffu);
+                fk.qi = (int)((en.walk >> 8) & 0xffu);
+                uk = (int)((en.walk >> 16) & 0xffu) - 1;
             } else {
                 c = perm ? (int64_t)perm[e] : e;
             }
@@ -1081,6 +1089,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         // the link's value
         bool finished = false;
         if (busy) {
+            if (uncertain && uk < 0) uk = i;
             const bool hit = jh >= 0;
             const float lp = hit ? vh : prof;
             if (out.profile) out.profile[c * L + i] = lp;
@@ -1097,7 +1106,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
                 if (out.verdict) out.verdict[c] = hit ? 0 : 1;
             }
         }
-        append_flagged(busy && finished && uncertain, c, idx, count);
+        append_flagged(busy && finished && uncertain, flag_entry(c, uk, qp.flag_link != 0), idx, count);
         // survivors, with the sort key of their next link's start point
         const bool keep = busy && !finished;
         sorted[threadIdx.x] = keep ? ((uint64_t)point_key(P1, kbox) << 32) | (uint64_t)threadIdx.x : ~0ull;
@@ -1121,7 +1130,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
             en.c = (int32_t)c;
             en.d = d;
             en.w = ((uint32_t)(best + 1) << 1) | (uncertain ? 1u : 0u);
-            en.walk = (uint32_t)fk.seg | ((uint32_t)fk.qi << 16);
+            en.walk = (uint32_t)fk.seg | ((uint32_t)fk.qi << 8) | ((uint32_t)(uk + 1) << 16);
             en.m[0] = fk.M.m0; en.m[1] = fk.M.m1; en.m[2] = fk.M.m2;
             en.m[3] = fk.M.m3; en.m[4] = fk.M.m4; en.m[5] = fk.M.m5;
             en.m[6] = fk.M.m6; en.m[7] = fk.M.m7; en.m[8] = fk.M.m8;
@@ -1557,11 +1566,13 @@ hipError_t launch_distance_fast(const RobotDev& rb, const EnvDev& env, const Que
                                 const int32_t* perm, uint32_t* links, int num_cus, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const bool warm = env.gorder && env.gcull;   // the grouped walk when the scene has group tables
+    QueryParams fq = qp;
+    fq.flag_link = flags_carry_link(n) ? 1 : 0;
     if (env.use_obb)
-        return warm ? launch_fast_shape<true, true>(rb, env, qp, q, n, out, idx, count, t, perm, links, num_cus, st)
-                    : launch_fast_shape<true, false>(rb, env, qp, q, n, out, idx, count, t, perm, links, num_cus, st);
-    return warm ? launch_fast_shape<false, true>(rb, env, qp, q, n, out, idx, count, t, perm, links, num_cus, st)
-                : launch_fast_shape<false, false>(rb, env, qp, q, n, out, idx, count, t, perm, links, num_cus, st);
+        return warm ? launch_fast_shape<true, true>(rb, env, fq, q, n, out, idx, count, t, perm, links, num_cus, st)
+                    : launch_fast_shape<true, false>(rb, env, fq, q, n, out, idx, count, t, perm, links, num_cus, st);
+    return warm ? launch_fast_shape<false, true>(rb, env, fq, q, n, out, idx, count, t, perm, links, num_cus, st)
+                : launch_fast_shape<false, false>(rb, env, fq, q, n, out, idx, count, t, perm, links, num_cus, st);
 }
 
 }  // namespace rvs

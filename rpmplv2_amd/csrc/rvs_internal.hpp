@@ -48,7 +48,23 @@ struct QueryParams {
     float maxd;     // MAX_DISTANCE
     float band;     // near-contact band of the FP32 path
     int32_t num_checks;
+    int32_t flag_link;   // set by the FP32 path's launches: flag_entry carries the start link
 };
+
+// Entries of the FP32 path's list of configurations for the exact pass.  In
+// a batch of at most 2^kFlagLinkShift configurations an entry is c | k <<
+// kFlagLinkShift, k the first link whose decision the bulk pass flagged: the
+// links before k were decided with the same margin as an unflagged
+// configuration's (no hit, values within the bulk pass's error), so the
+// exact pass keeps their values and goes on from link k.  Larger batches
+// list c alone (exact pass from link 0).  The bulk and refinement launches
+// both take the rule from n.
+constexpr int kFlagLinkShift = 25;   // 2^25 configurations, 6 bits of link (kMaxLinks = 64)
+static_assert(kMaxLinks <= 64, "a flag entry holds the link in 6 bits");
+inline bool flags_carry_link(int64_t n) { return n <= ((int64_t)1 << kFlagLinkShift); }
+__host__ __device__ inline int64_t flag_entry(int64_t c, int link, bool carry) {
+    return carry ? (c | ((int64_t)link << kFlagLinkShift)) : c;
+}
 
 // Outputs of the distance query.  Pointers may be null when not wanted.
 struct DistOut {
@@ -163,11 +179,12 @@ struct alignas(16) LinkEntry {
     int32_t c;       // configuration
     float d;         // running minimum over the links before
     uint32_t w;      // (warm pair + 1) << 1 | screening flag
-    uint32_t walk;   // the walk's next segment | next joint << 16
+    uint32_t walk;   // the walk's next segment | next joint << 8 | (first uncertain link + 1) << 16
     float m[9];      // the walk's frame: orientation (row major)
     float p[3];      // and origin
 };
 static_assert(sizeof(LinkEntry) == 64, "LinkEntry is four 16-byte pieces");
+static_assert(kMaxSeg < 255 && kMaxLinks < 255, "the walk word holds segment, joint and link in 8 bits each");
 constexpr int kLinksCounters = 2 * kMaxLinks + 2;
 struct LinksLayout {
     LinkEntry* list[2];

@@ -159,6 +159,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // ends after the first round with a hit.
 constexpr int kRefineWaves = 4;   // waves per workgroup of the refinement kernel (they share the staged obstacles)
 
+// Entry k of the refinement's work: configuration c and the first link to
+// recompute, k0 (entry k itself and k0 = 0 for a direct call, idx null; k0
+// = 0 too without a profile, which holds the values of the links before it).
+__device__ __attribute__((always_inline)) inline void flag_decode(const int32_t* idx, int64_t k, const QueryParams& qp,
+                                                                  const DistOut& out, int64_t& c, int& k0) {
+    if (!idx) {
+        c = k;
+        k0 = 0;
+        return;
+    }
+    const int64_t e = (int64_t)idx[k];
+    if (qp.flag_link) {
+        c = e & (((int64_t)1 << kFlagLinkShift) - 1);
+        k0 = out.profile ? (int)(e >> kFlagLinkShift) : 0;
+    } else {
+        c = e;
+        k0 = 0;
+    }
+}
+
 // With idx null the kernel serves a direct call: wave k takes configuration
 // k < n (latency-bound small batches: one wave per configuration instead of
 // one lane).  With nearest points every pair is evaluated (no broad phase)
@@ -177,16 +197,24 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
     const Params<float> p{qp.eq, qp.maxd};
     const int64_t total = idx ? (int64_t)*count : n_direct;
     for (int64_t k = (int64_t)blockIdx.x * kRefineWaves + wave; k < total; k += (int64_t)gridDim.x * kRefineWaves) {
-        const int64_t c = idx ? (int64_t)idx[k] : k;
+        int64_t c = 0;
+        int k0 = 0;   // the first link to recompute
+        flag_decode(idx, k, qp, out, c, k0);
         const float* qc = q + c * rb.num_dofs;
         FKStream fk;
         fk_init(fk);
         Vec3<float> P0 = skel_point<float>(fk, rb, qc, 0);
+        // the links before k0 keep the bulk pass's values (no hit)
         float d = p.maxd;
+        for (int i2 = 0; i2 < k0; ++i2) d = smin(d, out.profile[c * L + i2]);
         bool hit = false;
         int i = 0;
         for (; i < L; ++i) {
             const Vec3<float> P1 = skel_point<float>(fk, rb, qc, i + 1);
+            if (i < k0) {
+                P0 = P1;
+                continue;
+            }
             const float r = rb.radius[i];
             float lo = p.maxd;          // smallest filtered distance of this lane's obstacles
             int first = INT_MAX;        // first of them with filtered distance <= 0
@@ -331,12 +359,13 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine_pairs(Rob
     float* row = reinterpret_cast<float*>(smem + pairs_row_off(env)) + wave * kPairsMax;
     const Params<float> p{qp.eq, qp.maxd};
     const int64_t total = idx ? (int64_t)*count : n_direct;
-    for (int64_t k0 = (int64_t)blockIdx.x * kRefineWaves; k0 < total; k0 += (int64_t)gridDim.x * kRefineWaves) {
-        const int64_t k = k0 + wave;
+    for (int64_t kb = (int64_t)blockIdx.x * kRefineWaves; kb < total; kb += (int64_t)gridDim.x * kRefineWaves) {
+        const int64_t k = kb + wave;
         const bool live = k < total;   // uniform over the wave
         int64_t c = 0;
+        int k0 = 0;   // the first link to recompute; the pairs of the links before it are not evaluated
         if (live) {
-            c = idx ? (int64_t)idx[k] : k;
+            flag_decode(idx, k, qp, out, c, k0);
             const float* qc = q + c * rb.num_dofs;
             // lane m computes the sine and cosine of joint m (the same
             // function of the same value as a serial stream); the FK stream
@@ -354,7 +383,7 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine_pairs(Rob
                 const Vec3<float> P = skel_point_sc<float>(fk, rb, pt, sc);
                 if (t == pt) mine = P;
             }
-            for (int base = 0; base < npairs; base += 64) {   // uniform rounds: the shuffles see every lane
+            for (int base = k0 * M; base < npairs; base += 64) {   // uniform rounds: the shuffles see every lane
                 const int pp = base + t;
                 const int i = pp < npairs ? pp / M : 0;
                 const int j = pp - i * M;
@@ -380,7 +409,9 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine_pairs(Rob
         if (live) {
             float prof = p.maxd;
             bool hit = false;
-            if (t < L) {
+            if (t < k0) {
+                prof = out.profile[c * L + t];   // a link before k0: the bulk pass's value, no hit
+            } else if (t < L) {
                 for (int j = 0; j < M; ++j) {
                     prof = smin(prof, row[t * M + j]);
                     if (prof <= 0) {
@@ -393,7 +424,7 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine_pairs(Rob
             const int first_hit = hm ? __ffsll((unsigned long long)hm) - 1 : L;
             float d = (t < first_hit && t < L) ? prof : p.maxd;
             for (int o = 32; o > 0; o >>= 1) d = smin(d, __shfl_xor(d, o, 64));
-            if (t < L && out.profile) out.profile[c * L + t] = t <= first_hit ? prof : 0.0f;
+            if (t >= k0 && t < L && out.profile) out.profile[c * L + t] = t <= first_hit ? prof : 0.0f;
             if (t == 0) {
                 if (out.d) out.d[c] = hm ? 0.0f : d;
                 if (out.verdict) out.verdict[c] = hm ? 0 : 1;
@@ -806,6 +837,8 @@ hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const Q
                                   int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
                                   hipStream_t st) {
     if (n <= 0) return hipSuccess;
+    QueryParams rq = qp;   // the entries' form, as the bulk pass's launch chose it
+    rq.flag_link = flags_carry_link(n) ? 1 : 0;
     // enough waves to cover a typical flagged set in one round; the kernel
     // loops when more configurations are flagged
     const unsigned grid = (unsigned)std::min<int64_t>((n + kRefineWaves - 1) / kRefineWaves, 2048);
@@ -814,18 +847,18 @@ hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const Q
     if (refine_pairs_fit(rb, env)) {
         const size_t smp = pairs_row_off(env) + (size_t)kRefineWaves * kPairsMax * sizeof(float);
         if (env.use_obb)
-            hipLaunchKernelGGL((k_distance_refine_pairs<true>), dim3(grid), b, smp, st, rb, env, qp, q, out, idx, count,
+            hipLaunchKernelGGL((k_distance_refine_pairs<true>), dim3(grid), b, smp, st, rb, env, rq, q, out, idx, count,
                                (int64_t)0);
         else
-            hipLaunchKernelGGL((k_distance_refine_pairs<false>), dim3(grid), b, smp, st, rb, env, qp, q, out, idx,
+            hipLaunchKernelGGL((k_distance_refine_pairs<false>), dim3(grid), b, smp, st, rb, env, rq, q, out, idx,
                                count, (int64_t)0);
         return hipGetLastError();
     }
     if (env.use_obb)
-        hipLaunchKernelGGL((k_distance_refine<true, false>), dim3(grid), b, sm, st, rb, env, qp, q, out, idx, count,
+        hipLaunchKernelGGL((k_distance_refine<true, false>), dim3(grid), b, sm, st, rb, env, rq, q, out, idx, count,
                            (int64_t)0);
     else
-        hipLaunchKernelGGL((k_distance_refine<false, false>), dim3(grid), b, sm, st, rb, env, qp, q, out, idx, count,
+        hipLaunchKernelGGL((k_distance_refine<false, false>), dim3(grid), b, sm, st, rb, env, rq, q, out, idx, count,
                            (int64_t)0);
     return hipGetLastError();
 }

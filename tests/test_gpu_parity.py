@@ -481,6 +481,43 @@ def test_fp32_refine_first_hit_in_obstacle_order(oracle, engines_by_variant, sce
     assert (oprof <= 0).sum() > 10000 and len(np.unique(oprof[oprof <= 0])) > 100
 
 
+@pytest.mark.parametrize("variant", ["pool", "classic", "lanes", "links"])
+@pytest.mark.parametrize("far_boxes", [0, 199])
+def test_fp32_refine_from_the_first_uncertain_link(oracle, engines_by_variant, far_boxes, variant):
+    """The exact pass recomputes a flagged configuration from the first link
+    whose decision was uncertain; the links before it keep the bulk pass's
+    values.  Planar 2-DOF at q0 = 0: link 0 lies on the x axis, more than
+    0.5 m from every box; link 1 ends within 8e-5 m of contact with the box
+    above it (lo.y = 0.5), so every configuration is flagged at link 1 only.
+    Link 1's value must be the reference arithmetic's (exact pass), link 0's
+    within 1e-5 m of it, the verdicts and distances the reference's (d the
+    smaller link value, or 0 at a hit).  With 199 far boxes the 400 pairs
+    exceed the all-pairs refinement's limit, so the round-based form runs."""
+    engine = engines_by_variant[variant]
+    rb = R.planar_2dof(radius=0.02)
+    dims = [[1.3, 0.5, 1.0]] + [[0.2, 0.2, 0.2]] * far_boxes
+    pos = [[1.85, 0.75, 0.0]] + [[-3.0 + 0.3 * (k % 20), -3.0 + 0.3 * (k // 20), 4.0] for k in range(far_boxes)]
+    sc = W.Scene(dims, pos, [[1, 0, 0, 0]] * (1 + far_boxes), [0] * (1 + far_boxes), [W.MAX_DISTANCE] * (1 + far_boxes))
+    assert sc.aabb[0, 1] == np.float32(0.5)
+    rng = np.random.default_rng(91 + far_boxes)
+    delta = rng.uniform(-8e-5, 8e-5, 4096)
+    q = np.stack([np.zeros(4096), np.arcsin(0.48 + delta)], 1).astype(np.float32)
+    rid = engine.load_robot(rb)
+    eid = engine.create_env(sc)
+    d, prof, ver, _ = engine.distance(rid, eid, q, precision=rvs.PREC_FP32_REFINE)
+    refined = engine.last_refined()
+    od, oprof, over, _ = oracle.distance(rb, oracle.scene_of(sc), q, 32)
+    print(f"{far_boxes} far boxes, {variant}: refined {refined} of {len(q)}, free {ver.mean():.3f}")
+    assert refined == len(q)
+    assert 0 < over.mean() < 1
+    assert (ver == over).all()
+    assert ulps(prof[:, 1], oprof[:, 1]).max() <= 2
+    assert np.abs(prof[:, 0] - oprof[:, 0]).max() <= 1e-5 and (prof[:, 0] > 0.5).all()
+    free = ver == 1
+    assert np.array_equal(d[free], np.minimum(prof[free, 0], prof[free, 1])) and (d[~free] == 0).all()
+    assert np.abs(d - od).max() <= 1e-5
+
+
 @pytest.mark.parametrize("which", ["xarm6", "chain22", "planar2"])
 def test_enclosing_radii(oracle, engine, which):
     if which == "xarm6":
