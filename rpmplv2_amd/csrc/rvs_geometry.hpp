@@ -57,12 +57,16 @@ __host__ __device__ inline double vsqrt(double x) { return sqrt(x); }
 struct ExactMath {
     template <typename T> __host__ __device__ static T div(T a, T b) { return a / b; }
     template <typename T> __host__ __device__ static T root(T x) { return vsqrt(x); }
+    template <typename T> __host__ __device__ static T clamp(T x, T lo, T hi) { return fmin(fmax(x, lo), hi); }
 };
 
 // Relaxed FP32: v_rcp_f32 / v_sqrt_f32 (about 1 ulp), device only.
 struct FastMath {
     __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static float root(float x) { return __builtin_amdgcn_sqrtf(x); }
+    // x clamped to [lo, hi] (lo <= hi): the median of the three, one
+    // instruction (the value of min(max(x, lo), hi) for finite x)
+    __device__ static float clamp(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }
 };
 
 template <typename T, class MP = ExactMath>
@@ -432,52 +436,62 @@ __host__ __device__ inline float flat_capsule_box(const Vec3<float>& A, const Ve
     const float Dx = B.x - A.x, Dy = B.y - A.y, Dz = B.z - A.z;
     const float dd = Dx * Dx + (Dy * Dy + Dz * Dz);
     const float ix = MP::div(1.0f, Dx), iy = MP::div(1.0f, Dy), iz = MP::div(1.0f, Dz);
-    // crossing of the box shrunk by band (slab clipping; an axis the segment
-    // does not move along must lie inside the shrunk slab)
-    // (selects, not branches: an axis the segment does not move along gives
-    // infinite or NaN slab parameters, which the selects then ignore)
+    // crossing of the box shrunk by band (slab clipping).  An axis the
+    // segment does not move along has an infinite reciprocal: its slab
+    // parameters are -inf and +inf when the coordinate lies strictly inside
+    // the shrunk slab (the interval unchanged), both +inf or both -inf when
+    // outside (the interval empty), and NaN with an infinity when exactly on
+    // a shrunk bound (min / max drop the NaN: the interval empty again).
     float t0 = 0.0f, t1 = 1.0f;
-    bool in_slabs = true;
     {
-        auto slab = [&](float D, float inv, float a0, float lo, float hi) {
-            const float l = lo + band, h = hi - band;
-            const float a = (l - a0) * inv, c = (h - a0) * inv;
-            const bool moves = D != 0.0f;
-            t0 = moves ? fmaxf(t0, fminf(a, c)) : t0;
-            t1 = moves ? fminf(t1, fmaxf(a, c)) : t1;
-            in_slabs = in_slabs && (moves || (a0 > l && a0 < h));
+        auto slab = [&](float inv, float a0, float lo, float hi) {
+            const float a = (lo + band - a0) * inv, c = (hi - band - a0) * inv;
+            t0 = fmaxf(t0, fminf(a, c));
+            t1 = fminf(t1, fmaxf(a, c));
         };
-        slab(Dx, ix, A.x, b.lo[0], b.hi[0]);
-        slab(Dy, iy, A.y, b.lo[1], b.hi[1]);
-        slab(Dz, iz, A.z, b.lo[2], b.hi[2]);
+        slab(ix, A.x, b.lo[0], b.hi[0]);
+        slab(iy, A.y, b.lo[1], b.hi[1]);
+        slab(iz, A.z, b.lo[2], b.hi[2]);
     }
-    const bool deep = in_slabs && t0 <= t1 && b.hi[0] - b.lo[0] > 2.0f * band && b.hi[1] - b.lo[1] > 2.0f * band &&
+    const bool deep = t0 <= t1 && b.hi[0] - b.lo[0] > 2.0f * band && b.hi[1] - b.lo[1] > 2.0f * band &&
                       b.hi[2] - b.lo[2] > 2.0f * band;
     // derivative of half the squared distance at t
     auto g = [&](float t) {
         const float px = A.x + t * Dx, py = A.y + t * Dy, pz = A.z + t * Dz;
-        const float ex = px - fminf(fmaxf(px, b.lo[0]), b.hi[0]);
-        const float ey = py - fminf(fmaxf(py, b.lo[1]), b.hi[1]);
-        const float ez = pz - fminf(fmaxf(pz, b.lo[2]), b.hi[2]);
+        const float ex = px - MP::clamp(px, b.lo[0], b.hi[0]);
+        const float ey = py - MP::clamp(py, b.lo[1], b.hi[1]);
+        const float ez = pz - MP::clamp(pz, b.lo[2], b.hi[2]);
         return Dx * ex + (Dy * ey + Dz * ez);
     };
     float ta = 0.0f, tb = 1.0f;   // the bracket
-    const float g0 = g(0.0f), g1 = g(1.0f);
+    // at t = 0 the point is A itself
+    const float g0 = Dx * (A.x - MP::clamp(A.x, b.lo[0], b.hi[0])) +
+                     (Dy * (A.y - MP::clamp(A.y, b.lo[1], b.hi[1])) + Dz * (A.z - MP::clamp(A.z, b.lo[2], b.hi[2])));
+    const float g1 = g(1.0f);
     // every break point evaluated, its bracket update selected (a break
     // point outside (0, 1), or infinite or NaN for an axis the segment does
     // not move along, changes nothing)
-    auto consider = [&](float t) {
+    // g at a break point of axis K, without axis K's term (zero there up to
+    // rounding: the point lies on that slab bound)
+    auto gskip = [&](float t, int K) {
+        const float px = A.x + t * Dx, py = A.y + t * Dy, pz = A.z + t * Dz;
+        const float ex = K == 0 ? 0.0f : px - MP::clamp(px, b.lo[0], b.hi[0]);
+        const float ey = K == 1 ? 0.0f : py - MP::clamp(py, b.lo[1], b.hi[1]);
+        const float ez = K == 2 ? 0.0f : pz - MP::clamp(pz, b.lo[2], b.hi[2]);
+        return Dx * ex + (Dy * ey + Dz * ez);
+    };
+    auto consider = [&](float t, int K) {
         const bool in = t > 0.0f && t < 1.0f;
-        const float gt = g(in ? t : 0.0f);
+        const float gt = gskip(t, K);
         ta = (in && gt <= 0.0f) ? fmaxf(ta, t) : ta;
         tb = (in && !(gt <= 0.0f)) ? fminf(tb, t) : tb;
     };
-    consider((b.lo[0] - A.x) * ix);
-    consider((b.hi[0] - A.x) * ix);
-    consider((b.lo[1] - A.y) * iy);
-    consider((b.hi[1] - A.y) * iy);
-    consider((b.lo[2] - A.z) * iz);
-    consider((b.hi[2] - A.z) * iz);
+    consider((b.lo[0] - A.x) * ix, 0);
+    consider((b.hi[0] - A.x) * ix, 0);
+    consider((b.lo[1] - A.y) * iy, 1);
+    consider((b.hi[1] - A.y) * iy, 1);
+    consider((b.lo[2] - A.z) * iz, 2);
+    consider((b.hi[2] - A.z) * iz, 2);
     float ts;
     if (g0 >= 0.0f) {
         ts = 0.0f;
@@ -486,21 +500,23 @@ __host__ __device__ inline float flat_capsule_box(const Vec3<float>& A, const Ve
     } else {
         // the axes outside their slab on (ta, tb), at its midpoint
         const float tm = 0.5f * (ta + tb);
+        // (an axis is outside where its coordinate differs from its clamp,
+        // and then the clamp is the bound it is past)
         const float px = A.x + tm * Dx, py = A.y + tm * Dy, pz = A.z + tm * Dz;
-        const float cx = px < b.lo[0] ? A.x - b.lo[0] : (px > b.hi[0] ? A.x - b.hi[0] : 0.0f);
-        const float cy = py < b.lo[1] ? A.y - b.lo[1] : (py > b.hi[1] ? A.y - b.hi[1] : 0.0f);
-        const float cz = pz < b.lo[2] ? A.z - b.lo[2] : (pz > b.hi[2] ? A.z - b.hi[2] : 0.0f);
-        const float wx = px < b.lo[0] || px > b.hi[0] ? Dx : 0.0f;
-        const float wy = py < b.lo[1] || py > b.hi[1] ? Dy : 0.0f;
-        const float wz = pz < b.lo[2] || pz > b.hi[2] ? Dz : 0.0f;
+        const float qx = MP::clamp(px, b.lo[0], b.hi[0]);
+        const float qy = MP::clamp(py, b.lo[1], b.hi[1]);
+        const float qz = MP::clamp(pz, b.lo[2], b.hi[2]);
+        const float cx = px != qx ? A.x - qx : 0.0f, wx = px != qx ? Dx : 0.0f;
+        const float cy = py != qy ? A.y - qy : 0.0f, wy = py != qy ? Dy : 0.0f;
+        const float cz = pz != qz ? A.z - qz : 0.0f, wz = pz != qz ? Dz : 0.0f;
         const float num = cx * wx + (cy * wy + cz * wz);
         const float den = wx * wx + (wy * wy + wz * wz);
-        ts = den > 0.0f ? fminf(fmaxf(-MP::div(num, den), ta), tb) : ta;
+        ts = den > 0.0f ? MP::clamp(-MP::div(num, den), ta, tb) : ta;
     }
     const float px = A.x + ts * Dx, py = A.y + ts * Dy, pz = A.z + ts * Dz;
-    const float ex = px - fminf(fmaxf(px, b.lo[0]), b.hi[0]);
-    const float ey = py - fminf(fmaxf(py, b.lo[1]), b.hi[1]);
-    const float ez = pz - fminf(fmaxf(pz, b.lo[2]), b.hi[2]);
+    const float ex = px - MP::clamp(px, b.lo[0], b.hi[0]);
+    const float ey = py - MP::clamp(py, b.lo[1], b.hi[1]);
+    const float ez = pz - MP::clamp(pz, b.lo[2], b.hi[2]);
     const float raw = MP::root(ex * ex + (ey * ey + ez * ez));
     // screening and the value
     if (din > 0.0f) {

@@ -24,6 +24,7 @@ static bool same(double a, double b) { return (std::isnan(a) && std::isnan(b)) |
 struct AxisMath {
     template <typename U> static U div(U a, U b) { return a / b; }
     template <typename U> static U root(U x) { return vsqrt(x); }
+    template <typename U> static U clamp(U x, U lo, U hi) { return std::fmin(std::fmax(x, lo), hi); }
 };
 
 template <typename T> struct Rng {
