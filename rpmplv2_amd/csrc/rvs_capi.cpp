@@ -352,6 +352,30 @@ rvs_status rvs_robot_load(rvs_context* c, const rvs_robot_desc* d, int32_t* robo
         if (s.rev && !(nrm > 0)) return fail(c, RVS_EINVAL, "rvs_robot_load: zero joint axis");
         for (int i = 0; i < 3; ++i) s.axis[i] = s.rev ? a[i] / nrm : 0.0;
     }
+    // the FP32 bulk pass's float segments: R, K R and K^2 R in double, rounded
+    std::vector<SegF> segf(d->num_joints);
+    for (int k = 0; k < d->num_joints; ++k) {
+        const SegDev& s = seg[k];
+        SegF& f = segf[k];
+        std::memset(&f, 0, sizeof(f));
+        const double* a = s.axis;
+        const double K[9] = {0.0, -a[2], a[1], a[2], 0.0, -a[0], -a[1], a[0], 0.0};
+        double KR[9], K2R[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                KR[3 * i + j] = K[3 * i] * s.R[j] + K[3 * i + 1] * s.R[3 + j] + K[3 * i + 2] * s.R[6 + j];
+            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                K2R[3 * i + j] = K[3 * i] * KR[j] + K[3 * i + 1] * KR[3 + j] + K[3 * i + 2] * KR[6 + j];
+        for (int m = 0; m < 9; ++m) {
+            f.R[m] = (float)s.R[m];
+            f.KR[m] = (float)KR[m];
+            f.K2R[m] = (float)K2R[m];
+        }
+        for (int i = 0; i < 3; ++i) f.o[i] = (float)s.o[i];
+        f.rev = s.rev;
+    }
     const int L = d->num_links;
     std::vector<PointDev> pts(L + 1);
     for (auto& p : pts) std::memset(&p, 0, sizeof(p));
@@ -397,6 +421,7 @@ rvs_status rvs_robot_load(rvs_context* c, const rvs_robot_desc* d, int32_t* robo
     (void)hipSetDevice(c->opt.device);
     RobotRec r;
     hipError_t e = upload(r.allocs, seg.data(), seg.size(), &r.dev.seg);
+    if (e == hipSuccess) e = upload(r.allocs, segf.data(), segf.size(), &r.dev.segf);
     if (e == hipSuccess) e = upload(r.allocs, pts.data(), pts.size(), &r.dev.pts);
     if (e == hipSuccess) e = upload(r.allocs, rad.data(), rad.size(), &r.dev.radius);
     if (e == hipSuccess) e = upload(r.allocs, rn32.data(), rn32.size(), &r.dev.rnew32);

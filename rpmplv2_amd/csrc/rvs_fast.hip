@@ -46,25 +46,12 @@ struct Mat3f {
     float m0, m1, m2, m3, m4, m5, m6, m7, m8;
 };
 
-__device__ inline Mat3f loadf(const double* a) {
-    return Mat3f{(float)a[0], (float)a[1], (float)a[2], (float)a[3], (float)a[4],
-                 (float)a[5], (float)a[6], (float)a[7], (float)a[8]};
-}
-
 __device__ inline Mat3f mulf(const Mat3f& a, const Mat3f& b) {
     return Mat3f{a.m0 * b.m0 + a.m1 * b.m3 + a.m2 * b.m6, a.m0 * b.m1 + a.m1 * b.m4 + a.m2 * b.m7,
                  a.m0 * b.m2 + a.m1 * b.m5 + a.m2 * b.m8, a.m3 * b.m0 + a.m4 * b.m3 + a.m5 * b.m6,
                  a.m3 * b.m1 + a.m4 * b.m4 + a.m5 * b.m7, a.m3 * b.m2 + a.m4 * b.m5 + a.m5 * b.m8,
                  a.m6 * b.m0 + a.m7 * b.m3 + a.m8 * b.m6, a.m6 * b.m1 + a.m7 * b.m4 + a.m8 * b.m7,
                  a.m6 * b.m2 + a.m7 * b.m5 + a.m8 * b.m8};
-}
-
-__device__ inline Mat3f rot2f(const double* ad, float st, float ct) {
-    const float a0 = (float)ad[0], a1 = (float)ad[1], a2 = (float)ad[2];
-    const float vt = 1.0f - ct;
-    return Mat3f{ct + vt * a0 * a0,      -a2 * st + vt * a0 * a1, a1 * st + vt * a0 * a2,
-                 a2 * st + vt * a0 * a1, ct + vt * a1 * a1,       -a0 * st + vt * a1 * a2,
-                 -a1 * st + vt * a0 * a2, a0 * st + vt * a1 * a2, ct + vt * a2 * a2};
 }
 
 struct FKf {
@@ -74,20 +61,25 @@ struct FKf {
 };
 
 // Joint k of the configuration is q[k * qstride].
-__device__ __attribute__((always_inline)) inline void fkf_advance(FKf& s, const SegDev* seg, const float* q, int upto,
+__device__ __attribute__((always_inline)) inline void fkf_advance(FKf& s, const SegF* seg, const float* q, int upto,
                                                                  int qstride) {
     while (s.seg <= upto) {
-        const SegDev& g = seg[s.seg];
-        const Mat3f R = loadf(g.R);
+        const SegF& g = seg[s.seg];
         Mat3f SM;
         if (g.rev) {
             float st, ct;
             sincosf(q[(s.qi++) * qstride], &st, &ct);
-            SM = mulf(rot2f(g.axis, st, ct), R);
+            const float vt = 1.0f - ct;
+            // Rot(axis, q) R = R + sin q K R + (1 - cos q) K^2 R
+            SM = Mat3f{g.R[0] + st * g.KR[0] + vt * g.K2R[0], g.R[1] + st * g.KR[1] + vt * g.K2R[1],
+                       g.R[2] + st * g.KR[2] + vt * g.K2R[2], g.R[3] + st * g.KR[3] + vt * g.K2R[3],
+                       g.R[4] + st * g.KR[4] + vt * g.K2R[4], g.R[5] + st * g.KR[5] + vt * g.K2R[5],
+                       g.R[6] + st * g.KR[6] + vt * g.K2R[6], g.R[7] + st * g.KR[7] + vt * g.K2R[7],
+                       g.R[8] + st * g.KR[8] + vt * g.K2R[8]};
         } else {
-            SM = R;
+            SM = Mat3f{g.R[0], g.R[1], g.R[2], g.R[3], g.R[4], g.R[5], g.R[6], g.R[7], g.R[8]};
         }
-        const float o0 = (float)g.o[0], o1 = (float)g.o[1], o2 = (float)g.o[2];
+        const float o0 = g.o[0], o1 = g.o[1], o2 = g.o[2];
         const Mat3f& M = s.M;
         const float nx = M.m0 * o0 + M.m1 * o1 + M.m2 * o2 + s.px;
         const float ny = M.m3 * o0 + M.m4 * o1 + M.m5 * o2 + s.py;
@@ -104,7 +96,7 @@ __device__ __attribute__((always_inline)) inline Vec3<float> skelf_point(FKf& s,
                                                                        int k, int qstride = 1) {
     const PointDev pt = rb.pts[k];
     if (pt.frame < 0) return Vec3<float>{0.0f, 0.0f, 0.0f};
-    fkf_advance(s, rb.seg, q, pt.frame, qstride);
+    fkf_advance(s, rb.segf, q, pt.frame, qstride);
     const float w0 = pt.axis == 0 ? 1.0f : 0.0f, w1 = pt.axis == 1 ? 1.0f : 0.0f, w2 = pt.axis == 2 ? 1.0f : 0.0f;
     const Mat3f& M = s.M;
     const float ax = M.m0 * w0 + M.m1 * w1 + M.m2 * w2;

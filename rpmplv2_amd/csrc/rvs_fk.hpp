@@ -41,6 +41,20 @@ struct SegDev {
     int32_t pad;
 };
 
+// The FP32 bulk pass's copy of a segment, in float: the origin rotation R
+// and K R and K^2 R, K the cross-product matrix of the unit axis, so that the
+// segment's rotation at angle q, Rot(axis, q) R, is R + sin q K R +
+// (1 - cos q) K^2 R (Rodrigues' formula): 18 fused multiply-adds, with no
+// conversion from double on the device.
+struct SegF {
+    float R[9];
+    float KR[9];
+    float K2R[9];
+    float o[3];       // joint origin translation
+    int32_t rev;      // 1 revolute, 0 fixed
+    int32_t pad;
+};
+
 struct PointDev {
     int32_t frame;    // frame index, -1 for the base origin
     int32_t axis;     // 0 x, 1 y, 2 z column of that frame
@@ -59,6 +73,7 @@ struct RobotDev {
     int32_t num_pairs;
     int32_t kind;            // skeleton recipe (RVS_SKEL_CHAIN / RVS_SKEL_XARM6)
     const SegDev* seg;
+    const SegF* segf;        // the same segments for the FP32 bulk pass
     const PointDev* pts;     // num_links + 1
     const float* radius;     // num_links
     const float* rnew32;     // collision-tree inflated radius, float

@@ -455,19 +455,13 @@ __host__ __device__ inline float flat_capsule_box(const Vec3<float>& A, const Ve
     }
     const bool deep = t0 <= t1 && b.hi[0] - b.lo[0] > 2.0f * band && b.hi[1] - b.lo[1] > 2.0f * band &&
                       b.hi[2] - b.lo[2] > 2.0f * band;
-    // derivative of half the squared distance at t
-    auto g = [&](float t) {
-        const float px = A.x + t * Dx, py = A.y + t * Dy, pz = A.z + t * Dz;
-        const float ex = px - MP::clamp(px, b.lo[0], b.hi[0]);
-        const float ey = py - MP::clamp(py, b.lo[1], b.hi[1]);
-        const float ez = pz - MP::clamp(pz, b.lo[2], b.hi[2]);
-        return Dx * ex + (Dy * ey + Dz * ez);
-    };
     float ta = 0.0f, tb = 1.0f;   // the bracket
-    // at t = 0 the point is A itself
+    // g (the derivative of half the squared distance) at t = 0 and t = 1,
+    // where the points are A and B themselves
     const float g0 = Dx * (A.x - MP::clamp(A.x, b.lo[0], b.hi[0])) +
                      (Dy * (A.y - MP::clamp(A.y, b.lo[1], b.hi[1])) + Dz * (A.z - MP::clamp(A.z, b.lo[2], b.hi[2])));
-    const float g1 = g(1.0f);
+    const float g1 = Dx * (B.x - MP::clamp(B.x, b.lo[0], b.hi[0])) +
+                     (Dy * (B.y - MP::clamp(B.y, b.lo[1], b.hi[1])) + Dz * (B.z - MP::clamp(B.z, b.lo[2], b.hi[2])));
     // every break point evaluated, its bracket update selected (a break
     // point outside (0, 1), or infinite or NaN for an axis the segment does
     // not move along, changes nothing)
@@ -475,10 +469,10 @@ __host__ __device__ inline float flat_capsule_box(const Vec3<float>& A, const Ve
     // rounding: the point lies on that slab bound)
     auto gskip = [&](float t, int K) {
         const float px = A.x + t * Dx, py = A.y + t * Dy, pz = A.z + t * Dz;
-        const float ex = K == 0 ? 0.0f : px - MP::clamp(px, b.lo[0], b.hi[0]);
-        const float ey = K == 1 ? 0.0f : py - MP::clamp(py, b.lo[1], b.hi[1]);
-        const float ez = K == 2 ? 0.0f : pz - MP::clamp(pz, b.lo[2], b.hi[2]);
-        return Dx * ex + (Dy * ey + Dz * ez);
+        const float ex = Dx * (px - MP::clamp(px, b.lo[0], b.hi[0]));
+        const float ey = Dy * (py - MP::clamp(py, b.lo[1], b.hi[1]));
+        const float ez = Dz * (pz - MP::clamp(pz, b.lo[2], b.hi[2]));
+        return K == 0 ? ey + ez : (K == 1 ? ex + ez : ex + ey);
     };
     auto consider = [&](float t, int K) {
         const bool in = t > 0.0f && t < 1.0f;
