@@ -62,7 +62,7 @@ PEAK_HBM_GBPS = 8000.0
 # configurations of the profiled call.  roofline.achieved = this x the
 # configurations of a launch / the launch time measured here.
 COUNTER_FLOP_PER_CONFIG = {
-    ("xarm6_16aabb_ground", "k_distance_fast"): (2617.7, "profiles/k_distance_fast_xarm6_1M.md"),
+    ("xarm6_16aabb_ground", "k_distance_fast"): (2378.5, "profiles/k_distance_fast_xarm6_1M.md"),
     ("chain22_512rotated_boxes", "k_distance_fast_links"):
         (15429.2, "profiles/k_distance_fast_links_chain22_512obb.md"),
 }
@@ -72,10 +72,10 @@ COUNTER_FLOP_PER_CONFIG = {
 # configurations (profiles/k_distance_fast_links_chain22_512obb.md).
 LINKS_TRAFFIC_PER_CONFIG = (2.0 * 1735085.4 + 1899255.9) * 1024 / 2097152
 # HBM bytes per launch of the FP32 bulk kernel on the headline workload from
-# two `rocprofv3 --pmc` runs: FETCH_SIZE 130,780.8 KiB x 2.0 + WRITE_SIZE
-# 124,071.9 KiB x 1.0, the factors being what the same counters read for a
+# two `rocprofv3 --pmc` runs: FETCH_SIZE 161,397.5 KiB x 2.0 + WRITE_SIZE
+# 124,354.4 KiB x 1.0, the factors being what the same counters read for a
 # 256 MiB copy (profiles/k_distance_fast_xarm6_1M.md).
-TRAFFIC_BYTES = (2.0 * 130780.8 + 124071.9) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
+TRAFFIC_BYTES = (2.0 * 161397.5 + 124354.4) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
 BULK_KERNELS = {0: "k_distance_fast", 1: "k_distance_fast_pool", 2: "k_distance_fast_chunks", 3: "k_distance_fast_lanes",
                 4: "k_distance_fast_links"}
 

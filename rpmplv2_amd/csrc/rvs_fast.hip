@@ -366,9 +366,10 @@ __device__ __attribute__((always_inline)) inline void fast_one(const Staged<floa
 }
 
 // One lane per configuration: lane g takes configuration perm[g] of the walk
-// order (g itself without one).  Held to 96 VGPRs (5 waves per SIMD): the
-// walk-order form needs no scratch at that budget and ran 8% faster than at
-// 4 waves (6 waves spill and were slower).
+// order (g itself without one).  At least 5 waves per SIMD (96 VGPRs, no
+// scratch); the walk-order form for axis-aligned boxes takes 71 VGPRs and runs
+// 7 waves per SIMD, faster than with its workgroups' LDS padded down to 6 or 5
+// (workload 2: 0.182, 0.185 and 0.193 ms).
 template <bool OBB, bool ROWS, bool GROUPED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_distance_fast(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
