@@ -64,13 +64,13 @@ PEAK_HBM_GBPS = 8000.0
 COUNTER_FLOP_PER_CONFIG = {
     ("xarm6_16aabb_ground", "k_distance_fast"): (2378.5, "profiles/k_distance_fast_xarm6_1M.md"),
     ("chain22_512rotated_boxes", "k_distance_fast_links"):
-        (15429.2, "profiles/k_distance_fast_links_chain22_512obb.md"),
+        (14934.6, "profiles/k_distance_fast_links_chain22_512obb.md"),
 }
 # HBM bytes per configuration of the link-major bulk pass on the 22-DOF chain
-# with 512 rotated boxes (all 22 launches of a call): FETCH_SIZE 1,735,085.4
-# KiB x 2.0 + WRITE_SIZE 1,899,255.9 KiB x 1.0 per call on 2,097,152
+# with 512 rotated boxes (all 22 launches of a call): FETCH_SIZE 1,735,446.8
+# KiB x 2.0 + WRITE_SIZE 1,899,095.1 KiB x 1.0 per call on 2,097,152
 # configurations (profiles/k_distance_fast_links_chain22_512obb.md).
-LINKS_TRAFFIC_PER_CONFIG = (2.0 * 1735085.4 + 1899255.9) * 1024 / 2097152
+LINKS_TRAFFIC_PER_CONFIG = (2.0 * 1735446.8 + 1899095.1) * 1024 / 2097152
 # HBM bytes per launch of the FP32 bulk kernel on the headline workload from
 # two `rocprofv3 --pmc` runs: FETCH_SIZE 161,397.5 KiB x 2.0 + WRITE_SIZE
 # 124,354.4 KiB x 1.0, the factors being what the same counters read for a
