@@ -681,8 +681,15 @@ hipError_t fp32_graph_run(rvs_context* c, int32_t robot, int32_t env, const Robo
                           uint32_t* rcount, uint32_t* coh, uint32_t* links, const FastTuning& ft, uint32_t* refined) {
     hipError_t e = hipSuccess;
     if (!c->count_host) {
-        e = hipHostMalloc(reinterpret_cast<void**>(&c->count_host), 256, hipHostMallocDefault);
+        e = hipHostMalloc(reinterpret_cast<void**>(&c->count_host), 256, hipHostMallocMapped | hipHostMallocCoherent);
         if (e != hipSuccess) return e;
+        e = hipHostGetDevicePointer(reinterpret_cast<void**>(&c->count_dev), c->count_host, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(c->count_host);
+            c->count_host = nullptr;
+            c->count_dev = nullptr;
+            return e;
+        }
     }
     rvs_context::Graph* hit = nullptr;
     for (auto& g : c->graphs)
@@ -704,14 +711,16 @@ hipError_t fp32_graph_run(rvs_context* c, int32_t robot, int32_t env, const Robo
         hipGraph_t g = nullptr;
         e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return e;
-        hipError_t le = launch_zero_u32(rcount, 64, c->stream);
+        // the walk order's first kernel clears the counters (a separate
+        // clear without it), and the refinement stores the flagged count in
+        // the mapped host word: no copy node
         const int32_t* perm = nullptr;
-        if (le == hipSuccess) le = launch_coherence_order(rb, q, n, ft, coh, &perm, c->stream);
+        hipError_t le = launch_coherence_order(rb, q, n, ft, coh, &perm, c->stream, rcount, 64);
+        if (le == hipSuccess && !perm) le = launch_zero_u32(rcount, 64, c->stream);
         if (le == hipSuccess)
             le = launch_distance_fast(rb, ev, qp, q, n, out, ridx, rcount, ft, perm, links, c->num_cus, c->stream);
-        if (le == hipSuccess) le = launch_distance_refine(rb, ev, qp, q, n, out, ridx, rcount, c->stream);
         if (le == hipSuccess)
-            le = hipMemcpyAsync(c->count_host, rcount, 4, hipMemcpyDeviceToHost, c->stream);
+            le = launch_distance_refine(rb, ev, qp, q, n, out, ridx, rcount, c->stream, c->count_dev);
         e = hipStreamEndCapture(c->stream, &g);
         if (le != hipSuccess) e = le;
         hipGraphExec_t exec = nullptr;

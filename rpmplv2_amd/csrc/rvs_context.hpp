@@ -107,7 +107,8 @@ struct rvs_context {
     };
     std::vector<Graph> graphs;
     uint64_t graph_clock = 0;
-    uint32_t* count_host = nullptr;   // pinned: refinement count read back by the graphs
+    uint32_t* count_host = nullptr;   // pinned, coherent: refinement count the graphs' refinement stores
+    uint32_t* count_dev = nullptr;    // the same word's device address
     std::mutex tuning_mu;
     rvs::FastTuning tuning_for(int32_t robot, int32_t env, int32_t links, int32_t objects) {
         std::lock_guard<std::mutex> lk(tuning_mu);

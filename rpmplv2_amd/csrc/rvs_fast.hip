@@ -1202,10 +1202,13 @@ static_assert((1 << kCohLoBits) <= kCohSortThreads, "one thread per low key");
 // their buckets (key >> lo), stored bucket-major: hist[bucket * tiles + tile].
 __global__ __launch_bounds__(kCohTileThreads) void k_coh_hist(const float* __restrict__ q, int64_t n, int D, int nj,
                                                               int bits, int lo, int64_t tiles,
-                                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ hist) {
+                                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ hist,
+                                                              uint32_t* __restrict__ zero, int nzero) {
     __shared__ uint32_t cnt[kCohBuckets];
     const int t = threadIdx.x;
     if (t < kCohBuckets) cnt[t] = 0;
+    if (blockIdx.x == 0 && zero)   // the caller's counters, cleared on the way
+        for (int k = t; k < nzero; k += kCohTileThreads) zero[k] = 0u;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kCohTile;
     for (int k = 0; k < kCohPer; ++k) {
@@ -1407,7 +1410,7 @@ hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st) {
 }
 
 hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n, const FastTuning& t, uint32_t* coh,
-                                  const int32_t** perm, hipStream_t st) {
+                                  const int32_t** perm, hipStream_t st, uint32_t* zero, int nzero) {
     *perm = nullptr;
     const int bits = std::min<int>(t.coh_bits, kCohKeyBits);
     if (n <= 0 || !coh || bits <= 0 || rb.num_dofs <= 0) return hipSuccess;
@@ -1418,7 +1421,7 @@ hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n,
     const CohLayout L = coh_layout(coh, n);
     const unsigned tiles = (unsigned)L.tiles;
     hipLaunchKernelGGL(k_coh_hist, dim3(tiles), dim3(kCohTileThreads), 0, st, q, n, rb.num_dofs, nj, bits, lo, L.tiles,
-                       L.keys, L.hist);
+                       L.keys, L.hist, zero, nzero);
     hipLaunchKernelGGL(k_coh_scan, dim3(kCohBuckets), dim3(kCohThreads), 0, st, L.hist, L.tiles, L.sizes);
     hipLaunchKernelGGL(k_coh_bucket, dim3(tiles), dim3(kCohTileThreads), 0, st, L.keys, n, lo, L.hist, L.tiles, L.sizes,
                        L.key_lo, lo ? L.idx : L.perm);

@@ -163,9 +163,11 @@ inline CohLayout coh_layout(uint32_t* base, int64_t n) {
 // The walk order of the bulk pass: with t.coh_bits > 0 and coh scratch the
 // configurations are bucketed by their first joint angles (so that the lanes
 // of a wave hold similar poses) and *perm points at the order in coh;
-// otherwise *perm is null (the order given).
+// otherwise *perm is null (the order given).  When it runs, its first kernel
+// also clears zero[0, nzero) (zero may be null); with *perm null nothing
+// was cleared.
 hipError_t launch_coherence_order(const RobotDev& rb, const float* q, int64_t n, const FastTuning& t, uint32_t* coh,
-                                  const int32_t** perm, hipStream_t st);
+                                  const int32_t** perm, hipStream_t st, uint32_t* zero = nullptr, int nzero = 0);
 // Scratch of the link-major bulk pass (tuning variant 4), 256-byte aligned:
 // two lists of the configurations alive at a link, used in turn, whose
 // entries carry each configuration's state between links (read and written
@@ -227,9 +229,12 @@ int fast_links_blocks_per_cu(const EnvDev& env, int block);
 // Workgroup LDS of device dev, or of the current device when dev < 0
 // (hipDeviceAttributeMaxSharedMemoryPerBlock, read once per device).
 size_t device_lds_limit(int dev = -1);
+// The exact pass over the list the bulk pass wrote (idx, *count entries);
+// count_out (a device address, may be null) receives *count, for a caller
+// that reads it from mapped host memory instead of copying it.
 hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                   int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
-                                  hipStream_t st);
+                                  hipStream_t st, uint32_t* count_out = nullptr);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t st);
 
 

@@ -188,7 +188,7 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
                                                                        const float* __restrict__ q, DistOut out,
                                                                        const int32_t* __restrict__ idx,
                                                                        const uint32_t* __restrict__ count,
-                                                                       int64_t n_direct) {
+                                                                       int64_t n_direct, uint32_t* count_out) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<float> s = stage_env<float, OBB>(env, smem);
     const int M = env.num_obj, L = rb.num_links;
@@ -196,6 +196,7 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
     const uint32_t wave = threadIdx.x >> 6;
     const Params<float> p{qp.eq, qp.maxd};
     const int64_t total = idx ? (int64_t)*count : n_direct;
+    if (count_out && blockIdx.x == 0 && threadIdx.x == 0) *count_out = (uint32_t)total;
     for (int64_t k = (int64_t)blockIdx.x * kRefineWaves + wave; k < total; k += (int64_t)gridDim.x * kRefineWaves) {
         int64_t c = 0;
         int k0 = 0;   // the first link to recompute
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine_pairs(Rob
                                                                              const float* __restrict__ q, DistOut out,
                                                                              const int32_t* __restrict__ idx,
                                                                              const uint32_t* __restrict__ count,
-                                                                             int64_t n_direct) {
+                                                                             int64_t n_direct, uint32_t* count_out) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Staged<float> s = stage_env<float, OBB>(env, smem);
     const int M = env.num_obj, L = rb.num_links;
@@ -359,6 +360,7 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine_pairs(Rob
     float* row = reinterpret_cast<float*>(smem + pairs_row_off(env)) + wave * kPairsMax;
     const Params<float> p{qp.eq, qp.maxd};
     const int64_t total = idx ? (int64_t)*count : n_direct;
+    if (count_out && blockIdx.x == 0 && threadIdx.x == 0) *count_out = (uint32_t)total;
     for (int64_t kb = (int64_t)blockIdx.x * kRefineWaves; kb < total; kb += (int64_t)gridDim.x * kRefineWaves) {
         const int64_t k = kb + wave;
         const bool live = k < total;   // uniform over the wave
@@ -805,37 +807,38 @@ hipError_t launch_distance_ref32_waves(const RobotDev& rb, const EnvDev& env, co
     const dim3 b(64 * kRefineWaves);
     const int32_t* no_idx = nullptr;
     const uint32_t* no_count = nullptr;
+    uint32_t* no_out = nullptr;
     if (!out.nearest && refine_pairs_fit(rb, env)) {
         const size_t smp = pairs_row_off(env) + (size_t)kRefineWaves * kPairsMax * sizeof(float);
         if (env.use_obb)
             hipLaunchKernelGGL((k_distance_refine_pairs<true>), dim3(grid), b, smp, st, rb, env, qp, q, out, no_idx,
-                               no_count, n);
+                               no_count, n, no_out);
         else
             hipLaunchKernelGGL((k_distance_refine_pairs<false>), dim3(grid), b, smp, st, rb, env, qp, q, out, no_idx,
-                               no_count, n);
+                               no_count, n, no_out);
         return hipGetLastError();
     }
     if (env.use_obb) {
         if (out.nearest)
             hipLaunchKernelGGL((k_distance_refine<true, true>), dim3(grid), b, sm, st, rb, env, qp, q, out, no_idx,
-                               no_count, n);
+                               no_count, n, no_out);
         else
             hipLaunchKernelGGL((k_distance_refine<true, false>), dim3(grid), b, sm, st, rb, env, qp, q, out, no_idx,
-                               no_count, n);
+                               no_count, n, no_out);
     } else {
         if (out.nearest)
             hipLaunchKernelGGL((k_distance_refine<false, true>), dim3(grid), b, sm, st, rb, env, qp, q, out, no_idx,
-                               no_count, n);
+                               no_count, n, no_out);
         else
             hipLaunchKernelGGL((k_distance_refine<false, false>), dim3(grid), b, sm, st, rb, env, qp, q, out, no_idx,
-                               no_count, n);
+                               no_count, n, no_out);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q,
                                   int64_t n, const DistOut& out, const int32_t* idx, const uint32_t* count,
-                                  hipStream_t st) {
+                                  hipStream_t st, uint32_t* count_out) {
     if (n <= 0) return hipSuccess;
     QueryParams rq = qp;   // the entries' form, as the bulk pass's launch chose it
     rq.flag_link = flags_carry_link(n) ? 1 : 0;
@@ -848,18 +851,18 @@ hipError_t launch_distance_refine(const RobotDev& rb, const EnvDev& env, const Q
         const size_t smp = pairs_row_off(env) + (size_t)kRefineWaves * kPairsMax * sizeof(float);
         if (env.use_obb)
             hipLaunchKernelGGL((k_distance_refine_pairs<true>), dim3(grid), b, smp, st, rb, env, rq, q, out, idx, count,
-                               (int64_t)0);
+                               (int64_t)0, count_out);
         else
             hipLaunchKernelGGL((k_distance_refine_pairs<false>), dim3(grid), b, smp, st, rb, env, rq, q, out, idx,
-                               count, (int64_t)0);
+                               count, (int64_t)0, count_out);
         return hipGetLastError();
     }
     if (env.use_obb)
         hipLaunchKernelGGL((k_distance_refine<true, false>), dim3(grid), b, sm, st, rb, env, rq, q, out, idx, count,
-                           (int64_t)0);
+                           (int64_t)0, count_out);
     else
         hipLaunchKernelGGL((k_distance_refine<false, false>), dim3(grid), b, sm, st, rb, env, rq, q, out, idx, count,
-                           (int64_t)0);
+                           (int64_t)0, count_out);
     return hipGetLastError();
 }
 
