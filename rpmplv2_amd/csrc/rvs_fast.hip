@@ -54,6 +54,12 @@ __device__ inline Mat3f mulf(const Mat3f& a, const Mat3f& b) {
                  a.m6 * b.m2 + a.m7 * b.m5 + a.m8 * b.m8};
 }
 
+// Joint values (radians) the FP32 kinematics evaluates with the hardware
+// sine and cosine.  Their input, a / 2 pi, is rounded to float, so the
+// angle error grows as |a| 6e-8 rad: within 4 pi (twice the +-2 pi joint
+// limits of the reference's robots) it stays below 7.5e-7 rad.
+constexpr float kTrigRange = 12.566371f;
+
 struct FKf {
     Mat3f M;
     float px, py, pz;
@@ -67,8 +73,13 @@ __device__ __attribute__((always_inline)) inline void fkf_advance(FKf& s, const 
         const SegF& g = seg[s.seg];
         Mat3f SM;
         if (g.rev) {
-            float st, ct;
-            sincosf(q[(s.qi++) * qstride], &st, &ct);
+            // the hardware sine and cosine (v_sin_f32 / v_cos_f32 of a / 2 pi,
+            // errors of order 1e-6, far inside the screening band); a joint
+            // value beyond kTrigRange gives NaN, which flags the configuration
+            // for the exact pass
+            const float a = q[(s.qi++) * qstride];
+            const bool in_range = fabsf(a) <= kTrigRange;
+            const float st = in_range ? __sinf(a) : NAN, ct = in_range ? __cosf(a) : NAN;
             const float vt = 1.0f - ct;
             // Rot(axis, q) R = R + sin q K R + (1 - cos q) K^2 R
             SM = Mat3f{g.R[0] + st * g.KR[0] + vt * g.K2R[0], g.R[1] + st * g.KR[1] + vt * g.K2R[1],

@@ -481,6 +481,39 @@ def test_fp32_refine_first_hit_in_obstacle_order(oracle, engines_by_variant, sce
     assert (oprof <= 0).sum() > 10000 and len(np.unique(oprof[oprof <= 0])) > 100
 
 
+@pytest.mark.parametrize("variant", ["classic", "links"])
+def test_fp32_refine_joint_values_beyond_trig_range(oracle, engines_by_variant, variant):
+    """The FP32 kinematics takes sines and cosines from the hardware
+    functions, whose argument is rounded to turns: joint values beyond 4 pi
+    give a NaN skeleton there, so such configurations go to the exact pass;
+    values between 2 pi and 4 pi stay in the bulk pass.  Every configuration
+    with its first joint beyond 4 pi is refined (one with a later joint beyond
+    is refined when the walk reaches that joint: a hit before it ends the
+    configuration).  Verdicts equal the reference arithmetic's, distances and
+    profiles stay within 1e-5 m of FP64."""
+    engine = engines_by_variant[variant]
+    w = W.workload2(8192)
+    q = w.q.copy()
+    far = np.zeros(len(q), bool)
+    far[::7] = True
+    q[::7, 0] += np.float32(6 * np.pi)                   # 15.7 to 22.0 rad: beyond 4 pi
+    q[3::11, 2] -= np.float32(2 * np.pi)                 # 6.1 to 10.2 rad in magnitude: within 4 pi
+    q[5::13, 4] = np.float32(-4.2 * np.pi)
+    rid = engine.load_robot(w.robot)
+    eid = engine.create_env(w.scene)
+    d, prof, ver, _ = engine.distance(rid, eid, q, precision=rvs.PREC_FP32_REFINE)
+    refined = engine.last_refined()
+    sc = oracle.scene_of(w.scene)
+    od, oprof, over, _ = oracle.distance(w.robot, sc, q, 32)
+    d64, prof64, _, _ = oracle.distance(w.robot, sc, q, 64)
+    print(f"{variant}: refined {refined} of {len(q)} ({far.sum()} with the first joint beyond 4 pi)")
+    assert refined >= far.sum()
+    assert (ver == over).all()
+    assert np.abs(d - d64).max() <= 1e-5
+    ok = ver == 1
+    assert np.abs(prof[ok] - prof64[ok]).max() <= 1e-5
+
+
 @pytest.mark.parametrize("variant", ["pool", "classic", "lanes", "links"])
 @pytest.mark.parametrize("far_boxes", [0, 199])
 def test_fp32_refine_from_the_first_uncertain_link(oracle, engines_by_variant, far_boxes, variant):
