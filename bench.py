@@ -62,20 +62,20 @@ PEAK_HBM_GBPS = 8000.0
 # configurations of the profiled call.  roofline.achieved = this x the
 # configurations of a launch / the launch time measured here.
 COUNTER_FLOP_PER_CONFIG = {
-    ("xarm6_16aabb_ground", "k_distance_fast"): (2378.5, "profiles/k_distance_fast_xarm6_1M.md"),
+    ("xarm6_16aabb_ground", "k_distance_fast"): (2276.6, "profiles/k_distance_fast_xarm6_1M.md"),
     ("chain22_512rotated_boxes", "k_distance_fast_links"):
-        (14934.6, "profiles/k_distance_fast_links_chain22_512obb.md"),
+        (14804.1, "profiles/k_distance_fast_links_chain22_512obb.md"),
 }
 # HBM bytes per configuration of the link-major bulk pass on the 22-DOF chain
-# with 512 rotated boxes (all 22 launches of a call): FETCH_SIZE 1,735,446.8
-# KiB x 2.0 + WRITE_SIZE 1,899,095.1 KiB x 1.0 per call on 2,097,152
+# with 512 rotated boxes (all 22 launches of a call): FETCH_SIZE 1,730,503.6
+# KiB x 2.0 + WRITE_SIZE 1,899,228.7 KiB x 1.0 per call on 2,097,152
 # configurations (profiles/k_distance_fast_links_chain22_512obb.md).
-LINKS_TRAFFIC_PER_CONFIG = (2.0 * 1735446.8 + 1899095.1) * 1024 / 2097152
+LINKS_TRAFFIC_PER_CONFIG = (2.0 * 1730503.6 + 1899228.7) * 1024 / 2097152
 # HBM bytes per launch of the FP32 bulk kernel on the headline workload from
-# two `rocprofv3 --pmc` runs: FETCH_SIZE 161,397.5 KiB x 2.0 + WRITE_SIZE
-# 124,354.4 KiB x 1.0, the factors being what the same counters read for a
+# two `rocprofv3 --pmc` runs: FETCH_SIZE 161,910.5 KiB x 2.0 + WRITE_SIZE
+# 124,679.7 KiB x 1.0, the factors being what the same counters read for a
 # 256 MiB copy (profiles/k_distance_fast_xarm6_1M.md).
-TRAFFIC_BYTES = (2.0 * 161397.5 + 124354.4) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
+TRAFFIC_BYTES = (2.0 * 161910.5 + 124679.7) * 1024   # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch
 BULK_KERNELS = {0: "k_distance_fast", 1: "k_distance_fast_pool", 2: "k_distance_fast_chunks", 3: "k_distance_fast_lanes",
                 4: "k_distance_fast_links"}
 
