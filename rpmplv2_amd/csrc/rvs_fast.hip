@@ -1,13 +1,14 @@
 // FP32 bulk pass of RVS_PREC_FP32_REFINE (gfx950).
 //
 // The reference-exact kernel's early-exit loop over (link, obstacle) pairs in
-// relaxed single precision: kinematics in float, FMA contraction (this
-// translation unit is compiled with -ffp-contract=fast), hardware reciprocal
-// and square root, and the capsule-box distance by flat_capsule_box, a
-// branch-free form of the reference's case tree (rvs_geometry.hpp).  Its
-// results differ from the reference arithmetic by a few 1e-7 m, so every pair
-// it evaluates is screened and the configuration is sent to the exact
-// refinement pass when any pair is uncertain:
+// relaxed single precision: kinematics in float with the hardware sine and
+// cosine, FMA contraction (this translation unit is compiled with
+// -ffp-contract=fast), hardware reciprocal and square root, and the
+// capsule-box distance by flat_capsule_box, a branch-free form of the
+// reference's case tree (rvs_geometry.hpp).  Its results differ from the
+// reference arithmetic by up to about 1e-6 m, so every pair it evaluates is
+// screened and the configuration is sent to the exact refinement pass when
+// any pair is uncertain:
 //   end point within band of the box surface  the inside test could differ
 //   raw distance < EQ + band      the reference snaps distances below
 //                                 EQUALITY_THRESHOLD to 0 (then -r), touches
