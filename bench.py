@@ -66,6 +66,16 @@ COUNTER_FLOP_PER_CONFIG = {
     ("chain22_512rotated_boxes", "k_distance_fast_links"):
         (14804.1, "profiles/k_distance_fast_links_chain22_512obb.md"),
 }
+# VALU wave instructions per configuration of the same kernels (SQ_INSTS_VALU
+# / configurations, same profiles): the issue rate they reach against the
+# chip's 614.4 G wave instructions/s (1,024 SIMDs x 2.4 GHz / 4 cycles per
+# wave64 instruction), the limit these compare- and select-heavy kernels meet
+# long before the FP32 peak.
+COUNTER_VALU_PER_CONFIG = {
+    ("xarm6_16aabb_ground", "k_distance_fast"): 74.77,
+    ("chain22_512rotated_boxes", "k_distance_fast_links"): 763.49,
+}
+PEAK_VALU_ISSUE_G = 1024 * 2.4 / 4
 # HBM bytes per configuration of the link-major bulk pass on the 22-DOF chain
 # with 512 rotated boxes (all 22 launches of a call): FETCH_SIZE 1,730,503.6
 # KiB x 2.0 + WRITE_SIZE 1,899,228.7 KiB x 1.0 per call on 2,097,152
@@ -94,7 +104,15 @@ def valu_roofline(workload: str, kernel: str, n: int, kernel_ms: float, evaluate
         flop_cfg = evaluated_flop_cfg
         accounting = "evaluated-work model (no counter profile of this kernel on this workload)"
     achieved = flop_cfg * n / (kernel_ms * 1e-3) / 1e12
+    valu_cfg = COUNTER_VALU_PER_CONFIG.get((workload, kernel))
+    issue = None
+    if valu_cfg:
+        rate = valu_cfg * n / (kernel_ms * 1e-3) / 1e9
+        issue = {"valu_instr_per_config": valu_cfg, "achieved_G_per_s": rate, "peak_G_per_s": PEAK_VALU_ISSUE_G,
+                 "frac": rate / PEAK_VALU_ISSUE_G,
+                 "note": "VALU wave instructions issued per second (counters x configurations / kernel time)"}
     return {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "valu_issue": issue,
             "frac": achieved / PEAK_FP32_TFLOPS, "kernel": kernel, "kernel_ms": kernel_ms,
             "flop_per_config": flop_cfg, "flop_accounting": accounting,
             "peak_unpacked": PEAK_FP32_UNPACKED_TFLOPS, "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
