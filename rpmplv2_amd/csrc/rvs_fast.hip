@@ -857,26 +857,9 @@ __global__ __launch_bounds__(kBoxThreads) void k_links_setup(RobotDev rb, EnvDev
     if (threadIdx.x == 0) *warm0 = bj[0];
 }
 
-// A round's sort ranks (one word per thread) reuse the queue rings.
-static_assert(kLaneQueue * sizeof(int16_t) >= sizeof(uint32_t), "queue rings hold a word per thread");
-
-// In-place ascending bitonic sort of blockDim.x (a power of 2) 64-bit
-// entries in LDS, one a thread; the whole workgroup calls it.
-__device__ inline void block_sort(uint64_t* v) {
-    const int n = blockDim.x, t = threadIdx.x;
-    for (int k = 2; k <= n; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const int u = t ^ j;
-            if (u > t) {
-                const uint64_t a = v[t], b = v[u];
-                if ((a > b) == ((t & k) == 0)) {
-                    v[t] = b;
-                    v[u] = a;
-                }
-            }
-            __syncthreads();
-        }
-}
+// A round's cell counts and per-wave words (256 + 48 words) reuse the queue
+// rings, at least 256 threads' worth.
+static_assert(kLaneQueue * sizeof(int16_t) * 256 >= (256 + 48) * sizeof(uint32_t), "queue rings hold the counts");
 
 // The link-major walk (tuning variant 4): one launch per link index i over
 // the configurations still alive at link i.  A workgroup takes rounds of
@@ -916,20 +899,21 @@ __device__ inline void block_sort(uint64_t* v) {
 // inside its group's, and every pair is tested again with exactly the bound
 // the walk has when it reaches it).  A configuration that goes on keeps its
 // running minimum, warm pair, screening flag and walk state in its entry
-// of the next link's list; a round's survivors are sorted by the Morton code (over box) of
-// their next link's start point and appended to that list together, so that the
-// next launch's waves again hold nearby segments.  The lists' order between
-// rounds depends on timing; no configuration's results do.  (Rounds of one
-// 64-entry chunk per wave measured fastest: a window twice as long per
-// round, 5% slower on workload 5; four times, a smaller share of LDS left
-// for resident workgroups, 60% slower.)
+// of the next link's list; a round's survivors are ordered by the Morton code
+// (over box) of their next link's start point, at the resolution of 256 cells
+// of the round's own region of the curve, and appended to that list
+// together, so that the next launch's waves again hold nearby segments.  The
+// lists' order between rounds depends on timing; no configuration's results
+// do.  (Rounds of one 64-entry chunk per wave measured fastest: a window
+// twice as long per round, 5% slower on workload 5; four times, a smaller
+// share of LDS left for resident workgroups, 60% slower.  A full bitonic sort
+// of the round's keys was 4% slower than the counting sort.)
 template <bool OBB>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_distance_fast_links(
     RobotDev rb, EnvDev env, QueryParams qp, const float* __restrict__ q, int64_t n, DistOut out, int32_t* idx,
     uint32_t* count, int link, const int32_t* __restrict__ perm, const LinkEntry* __restrict__ in,
     const uint32_t* __restrict__ in_count, uint32_t* cursor, LinkEntry* __restrict__ next_list, uint32_t* next_count,
-    const float* __restrict__ box, const int32_t* __restrict__ warm0, size_t cull_off, size_t queue_off,
-    size_t sort_off) {
+    const float* __restrict__ box, const int32_t* __restrict__ warm0, size_t cull_off, size_t queue_off) {
     // configurations at this link: the first takes all n (in the walk order
     // perm, or the order given), the others the previous link's list
     const int64_t cnt = link == 0 ? n : (int64_t)*in_count;
@@ -941,7 +925,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     for (int k = threadIdx.x; k < 6 * env.num_obj; k += B) lcull[k] = env.cull[k];
     const Staged<float> s = stage_env<float, OBB>(env, smem);   // ends with a barrier
     int16_t* const ring = reinterpret_cast<int16_t*>(smem + queue_off) + threadIdx.x;   // slot k: ring[k * B]
-    uint64_t* const sorted = reinterpret_cast<uint64_t*>(smem + sort_off);   // [B] a round's survivors
     const const_ptr<float> gcull = as_const(env.gcull);
     const const_ptr<float> ccull = as_const(env.cull);
     const const_ptr<int32_t> gorder = as_const(env.gorder);
@@ -1114,21 +1097,59 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         append_flagged(busy && finished && uncertain, flag_entry(c, uk, qp.flag_link != 0), idx, count);
         // survivors, with the sort key of their next link's start point
         const bool keep = busy && !finished;
-        sorted[threadIdx.x] = keep ? ((uint64_t)point_key(P1, kbox) << 32) | (uint64_t)threadIdx.x : ~0ull;
-        // the round's survivors, sorted, appended together: each thread
-        // learns its place in the sorted order (every wave has left its walk,
-        // so the queue rings hold the ranks) and writes its own entry there
-        __syncthreads();
-        block_sort(sorted);
-        uint32_t* const rank = reinterpret_cast<uint32_t*>(smem + queue_off);
-        const uint64_t sk = sorted[threadIdx.x];
-        if (sk != ~0ull) rank[(uint32_t)sk] = threadIdx.x;
-        if (threadIdx.x == 0) {
-            int k = 0;   // survivors come first (the others hold the largest key)
-            for (int h = B; h > 0; h >>= 1)
-                if (k + h <= B && sorted[k + h - 1] != ~0ull) k += h;
-            out_base = k ? atomicAdd(next_count, (uint32_t)k) : 0u;
+        // the round's survivors, appended together in the order of their next
+        // link's start point: a counting sort on the 8 bits of its Morton code
+        // just below the round's common prefix (the round's keys lie between
+        // its smallest and largest, which share every bit above the highest
+        // one where they differ, so these bits cut the round's own region of
+        // the curve into 256 cells); ranks within a cell by LDS atomics
+        const uint32_t key = keep ? point_key(P1, kbox) : 0u;
+        uint32_t kmin = keep ? key : 0xffffffffu, kmax = keep ? key : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, o, 64));
+            kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, o, 64));
         }
+        uint32_t* const cell = reinterpret_cast<uint32_t*>(smem + queue_off);   // [256] cell counts, then starts
+        uint32_t* const wmm = cell + 256;                                       // [16][3] wave min, max, sum
+        const int wv = threadIdx.x >> 6, nw = B >> 6;
+        __syncthreads();   // every wave has left its walk: the queue rings are free
+        if (lane == 0) {
+            wmm[3 * wv] = kmin;
+            wmm[3 * wv + 1] = kmax;
+        }
+        if (threadIdx.x < 256) cell[threadIdx.x] = 0u;
+        __syncthreads();
+        kmin = 0xffffffffu;
+        kmax = 0u;
+        for (int k = 0; k < nw; ++k) {
+            kmin = min(kmin, wmm[3 * k]);
+            kmax = max(kmax, wmm[3 * k + 1]);
+        }
+        const int hb = 31 - __clz((int)((kmin ^ kmax) | 1u));
+        const int sh = hb > 7 ? hb - 7 : 0;
+        const uint32_t bucket = (key >> sh) & 255u;
+        uint32_t rk = 0;
+        if (keep) rk = atomicAdd(&cell[bucket], 1u);
+        __syncthreads();
+        // exclusive scan of the cell counts (one a thread, the first 256)
+        const uint32_t v = threadIdx.x < 256 ? cell[threadIdx.x] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wmm[3 * wv + 2] = x;
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+        for (int k = 0; k < nw; ++k) {
+            const uint32_t sk = wmm[3 * k + 2];
+            before += k < wv ? sk : 0u;
+            all += sk;
+        }
+        if (threadIdx.x < 256) cell[threadIdx.x] = before + x - v;
+        if (threadIdx.x == 0) out_base = all ? atomicAdd(next_count, all) : 0u;
         __syncthreads();
         if (keep) {
             LinkEntry en;
@@ -1140,9 +1161,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
             en.m[3] = fk.M.m3; en.m[4] = fk.M.m4; en.m[5] = fk.M.m5;
             en.m[6] = fk.M.m6; en.m[7] = fk.M.m7; en.m[8] = fk.M.m8;
             en.p[0] = fk.px; en.p[1] = fk.py; en.p[2] = fk.pz;
-            next_list[out_base + rank[threadIdx.x]] = en;
+            next_list[out_base + cell[bucket] + rk] = en;
         }
-        __syncthreads();   // round_id, the sort buffer and the rings are written again next round
+        __syncthreads();   // round_id and the rings are written again next round
     }
 }
 
@@ -1376,21 +1397,18 @@ size_t lanes_lds(const EnvDev& env, int block, size_t* cull_off, size_t* queue_o
     return total;
 }
 
-// LDS of the link-major kernel: that of the per-lane-queue kernel, then
-// (16-byte aligned) the sort buffer of a round's survivors (8 bytes per
-// thread); 0 when it cannot run on this obstacle set
-// (kStaticLds: room for the kernel's own shared variables).
+// LDS of the link-major kernel: that of the per-lane-queue kernel (the
+// rounds' survivor ranking reuses its queue rings); 0 when it cannot run on
+// this obstacle set or with fewer than 256 threads (kStaticLds: room for the
+// kernel's own shared variables).
 constexpr size_t kStaticLds = 1024;
-size_t links_lds(const EnvDev& env, int block, size_t* cull_off, size_t* queue_off, size_t* sort_off) {
+size_t links_lds(const EnvDev& env, int block, size_t* cull_off, size_t* queue_off) {
+    if (block < 256) return 0;
     size_t c = 0, qo = 0;
-    const size_t base = lanes_lds(env, block, &c, &qo);
-    if (!base) return 0;
-    const size_t so = (base + 15) & ~size_t(15);
-    const size_t total = so + (size_t)8 * block;
-    if (total + kStaticLds > device_lds_limit()) return 0;
+    const size_t total = lanes_lds(env, block, &c, &qo);
+    if (!total || total + kStaticLds > device_lds_limit()) return 0;
     if (cull_off) *cull_off = c;
     if (queue_off) *queue_off = qo;
-    if (sort_off) *sort_off = so;
     return total;
 }
 
@@ -1407,7 +1425,7 @@ int fast_lanes_blocks_per_cu(const EnvDev& env, int block) {
 }
 
 int fast_links_blocks_per_cu(const EnvDev& env, int block) {
-    const size_t sm = links_lds(env, block, nullptr, nullptr, nullptr);
+    const size_t sm = links_lds(env, block, nullptr, nullptr);
     if (!sm) return 0;
     int nb = 0;
     const hipError_t e =
@@ -1475,8 +1493,8 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
     if (t.variant == 4 && GROUPED && links && n <= INT32_MAX) {
         // one launch per link over the configurations alive at it
         const int BL = (t.block == 512 || t.block == 1024) ? t.block : 256;
-        size_t cull_off = 0, queue_off = 0, sort_off = 0;
-        const size_t lsm = links_lds(env, BL, &cull_off, &queue_off, &sort_off);
+        size_t cull_off = 0, queue_off = 0;
+        const size_t lsm = links_lds(env, BL, &cull_off, &queue_off);
         if (lsm) {
             const LinksLayout ll = links_layout(links, n);
             int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : fast_links_blocks_per_cu(env, BL);
@@ -1492,7 +1510,7 @@ hipError_t launch_fast_shape(const RobotDev& rb, const EnvDev& env, const QueryP
                 const LinkEntry* in = i == 0 ? nullptr : ll.list[(i - 1) & 1];
                 hipLaunchKernelGGL((k_distance_fast_links<OBB>), g, b, lsm, st, rb, env, qp, q, n, out, idx, count, i,
                                    perm, in, ll.count + i, ll.cursor + i, ll.list[i & 1], ll.count + i + 1, ll.box,
-                                   ll.warm0, cull_off, queue_off, sort_off);
+                                   ll.warm0, cull_off, queue_off);
                 e = hipGetLastError();
             }
             return e;
