@@ -441,6 +441,7 @@ rvs_status rvs_robot_load(rvs_context* c, const rvs_robot_desc* d, int32_t* robo
     r.num_dofs = ndof;
     r.num_links = L;
     r.num_pairs = r.dev.num_pairs;
+    r.radius = rad;
     c->robots.push_back(r);
     *robot_id = (int32_t)c->robots.size() - 1;
     return RVS_OK;
@@ -1820,6 +1821,8 @@ namespace {
 struct EdgePlan {
     EdgeGroups gr{};
     std::vector<int32_t> order, rounds;
+    std::vector<float> keys;          // axis-test tables: (radius, ground skipped) per table
+    std::vector<int32_t> link_key;    // table of each (group, link)
 };
 
 rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, int64_t num_edges, int32_t row_stride,
@@ -1865,8 +1868,54 @@ rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, i
         }
         if (acc > INT32_MAX / 8) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: too many states");
         gr.num_slots[g] = (int32_t)acc;
+        // axis-test table of each link: links with the same radius and the
+        // same ground rule share one
+        gr.key_off[g] = (int32_t)pl.link_key.size();
+        for (int i = 0; i < rb.num_links; ++i) {
+            const float r = rb.radius[i], skip = i < rb.dev.ground_included ? 1.0f : 0.0f;
+            int t = 0;
+            const int nk = (int)pl.keys.size() / 2;
+            while (t < nk && !(pl.keys[2 * t] == r && pl.keys[2 * t + 1] == skip)) ++t;
+            if (t == nk) {
+                pl.keys.push_back(r);
+                pl.keys.push_back(skip);
+            }
+            pl.link_key.push_back(t);
+        }
     }
     return RVS_OK;
+}
+
+// Arena bytes of a plan's device tables (before the per-call rows).
+std::vector<size_t> plan_bytes(const EdgePlan& pl, const EnvRec& ev) {
+    const size_t nk = ev.dev.use_obb ? 0 : pl.keys.size() / 2;
+    return {pl.order.size() * 4, pl.rounds.size() * 4, 256, pl.link_key.size() * 4, pl.keys.size() * 4,
+            nk * slab_entries(ev.dev.num_obj) * sizeof(SlabEntry)};
+}
+
+// Carves and uploads a plan's tables (in plan_bytes order); sets the table
+// pointers of pl.gr.
+struct PlanDev {
+    int32_t* order;
+    int32_t* rounds;
+    uint32_t* counter;
+    float* keys;
+    int nkeys;
+};
+hipError_t upload_plan(rvs_context* c, const EnvRec& ev, EdgePlan& pl, Carve& cv, PlanDev& d) {
+    d.order = cv.take<int32_t>(pl.order.size());
+    d.rounds = cv.take<int32_t>(pl.rounds.size());
+    d.counter = cv.take<uint32_t>(64);
+    int32_t* lk = cv.take<int32_t>(pl.link_key.size());
+    d.keys = cv.take<float>(pl.keys.size());
+    d.nkeys = ev.dev.use_obb ? 0 : (int)pl.keys.size() / 2;
+    pl.gr.slab = cv.take<SlabEntry>((size_t)d.nkeys * slab_entries(ev.dev.num_obj));
+    pl.gr.link_key = lk;
+    hipError_t e = h2d(d.order, pl.order.data(), pl.order.size() * 4, c->stream);
+    if (e == hipSuccess) e = h2d(d.rounds, pl.rounds.data(), pl.rounds.size() * 4, c->stream);
+    if (e == hipSuccess) e = h2d(lk, pl.link_key.data(), pl.link_key.size() * 4, c->stream);
+    if (e == hipSuccess) e = h2d(d.keys, pl.keys.data(), pl.keys.size() * 4, c->stream);
+    return e;
 }
 
 // Items of 64 * spl slots per group; returns the total.
@@ -1883,15 +1932,15 @@ int32_t set_items(EdgeGroups& gr, int spl) {
 
 // Runs a planned call on device arrays (q1, dir, first) with a launch shape.
 hipError_t run_edges(rvs_context* c, const EnvRec& ev, EdgePlan& pl, const float* q1, const float* dir,
-                     int32_t row_stride, float step, int32_t* first, int64_t num_edges, int32_t* od, int32_t* rd,
-                     uint32_t* counter, int waves_per_cu, int spl, bool timing) {
+                     int32_t row_stride, float step, int32_t* first, int64_t num_edges, const PlanDev& d,
+                     int waves_per_cu, int spl, bool timing) {
     const int32_t items = set_items(pl.gr, spl);
     const QueryParams qp = params_of(c);
     hipError_t e = hipSuccess;
     if (timing) e = hipEventRecord(c->ev[0], c->stream);
     if (e == hipSuccess)
-        e = launch_edge_first(pl.gr, ev.dev, qp, q1, dir, row_stride, step, od, rd, first, num_edges, counter, items,
-                              c->num_cus * waves_per_cu, spl, c->stream);
+        e = launch_edge_first(pl.gr, ev.dev, qp, q1, dir, row_stride, step, d.order, d.rounds, first, num_edges,
+                              d.counter, items, c->num_cus * waves_per_cu, spl, d.keys, d.nkeys, c->stream);
     if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
     return e;
 }
@@ -1919,19 +1968,17 @@ rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* 
     if (lds > device_lds_limit(c->opt.device)) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: robots too large for LDS staging");
     (void)hipSetDevice(c->opt.device);
     const size_t rows = (size_t)num_edges * row_stride;
-    hipError_t e = c->arena.ensure(Carve::need({pl.order.size() * 4, pl.rounds.size() * 4, 256,
-                                                (flags & RVS_F_DEVICE) ? 0 : rows * 4,
-                                                (flags & RVS_F_DEVICE) ? 0 : rows * 4,
-                                                (flags & RVS_F_DEVICE) ? 0 : (size_t)num_edges * 4}));
+    std::vector<size_t> need = plan_bytes(pl, ev);
+    if (!(flags & RVS_F_DEVICE)) need.insert(need.end(), {rows * 4, rows * 4, (size_t)num_edges * 4});
+    size_t total = 256;
+    for (size_t b : need) total += (b + 255) & ~size_t(255);
+    hipError_t e = c->arena.ensure(total);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_first_collision: staging");
     Carve cv(c->arena.p);
-    int32_t* od = cv.take<int32_t>(pl.order.size());
-    int32_t* rd = cv.take<int32_t>(pl.rounds.size());
-    uint32_t* counter = cv.take<uint32_t>(64);
+    PlanDev pd{};
     const float *qa = q1, *qd = dir;
     int32_t* fd = first;
-    e = h2d(od, pl.order.data(), pl.order.size() * 4, c->stream);
-    if (e == hipSuccess) e = h2d(rd, pl.rounds.data(), pl.rounds.size() * 4, c->stream);
+    e = upload_plan(c, ev, pl, cv, pd);
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
         float* as = cv.take<float>(rows);
         float* ds = cv.take<float>(rows);
@@ -1943,8 +1990,8 @@ rvs_status rvs_edge_first_collision(rvs_context* c, int32_t env, const int32_t* 
     }
     const bool timing = (flags & RVS_F_TIMING) != 0;
     if (e == hipSuccess)
-        e = run_edges(c, ev, pl, qa, qd, row_stride, step, fd, num_edges, od, rd, counter, c->edge_waves_per_cu,
-                      c->edge_spl, timing);
+        e = run_edges(c, ev, pl, qa, qd, row_stride, step, fd, num_edges, pd, c->edge_waves_per_cu, c->edge_spl,
+                      timing);
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(first, fd, (size_t)num_edges * 4, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_first_collision");
@@ -1982,19 +2029,19 @@ rvs_status rvs_autotune_edges(rvs_context* c, int32_t env, const int32_t* robots
         return fail(c, RVS_ERANGE, "rvs_autotune_edges: robots too large for LDS staging");
     (void)hipSetDevice(c->opt.device);
     const size_t rows = (size_t)num_edges * row_stride;
-    hipError_t e = c->arena.ensure(Carve::need({pl.order.size() * 4, pl.rounds.size() * 4, 256, rows * 4, rows * 4,
-                                                (size_t)num_edges * 4}));
+    std::vector<size_t> need = plan_bytes(pl, ev);
+    need.insert(need.end(), {rows * 4, rows * 4, (size_t)num_edges * 4});
+    size_t total = 256;
+    for (size_t b : need) total += (b + 255) & ~size_t(255);
+    hipError_t e = c->arena.ensure(total);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_autotune_edges: staging");
     Carve cv(c->arena.p);
-    int32_t* od = cv.take<int32_t>(pl.order.size());
-    int32_t* rd = cv.take<int32_t>(pl.rounds.size());
-    uint32_t* counter = cv.take<uint32_t>(64);
+    PlanDev pd{};
+    e = upload_plan(c, ev, pl, cv, pd);
     float* as = cv.take<float>(rows);
     float* ds = cv.take<float>(rows);
     int32_t* fd = cv.take<int32_t>(num_edges);
     const hipMemcpyKind kind = (flags & RVS_F_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    e = h2d(od, pl.order.data(), pl.order.size() * 4, c->stream);
-    if (e == hipSuccess) e = h2d(rd, pl.rounds.data(), pl.rounds.size() * 4, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(as, q1, rows * 4, kind, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(ds, dir, rows * 4, kind, c->stream);
     int win_w = 16, win_s = 1;
@@ -2003,7 +2050,7 @@ rvs_status rvs_autotune_edges(rvs_context* c, int32_t env, const int32_t* robots
         for (int spl : {1, 2, 4}) {
             float best_ms = 1e30f;
             for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {   // first run warms up
-                e = run_edges(c, ev, pl, as, ds, row_stride, step, fd, num_edges, od, rd, counter, w, spl, true);
+                e = run_edges(c, ev, pl, as, ds, row_stride, step, fd, num_edges, pd, w, spl, true);
                 if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
                 float ms = 0;
                 if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);

@@ -41,6 +41,7 @@ struct RobotRec {
     RobotDev dev{};
     std::vector<void*> allocs;
     int num_dofs = 0, num_links = 0, num_pairs = 0;
+    std::vector<float> radius;   // capsule radius per link (host copy)
 };
 
 struct EnvRec {

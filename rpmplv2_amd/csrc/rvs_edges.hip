@@ -15,6 +15,19 @@
 // that lie beyond an already found collision of their edge, and records a
 // collision with atomicMin on the edge's first index.  Robot ranges are padded
 // to whole items, so the robot is uniform across a wave.
+//
+// On axis-aligned scenes (k_edge_first_aabb) each link meets the obstacles
+// through an axis-test table first (SlabEntry): collisionCapsuleToBox
+// (CollisionAndDistance.cpp:4-42) rejects a capsule whose two end points lie
+// beyond one face of the box grown by the radius r on the same side (its
+// second test), and its first test, an end point inside the box grown by
+// r_new = r sqrt(3) / 3 <= r, can only hold when that one cannot: a point
+// below fl(lo - r) is below fl(lo - r_new) as rounding is monotone.  So a
+// pair that the axis test separates is a pair the full test rejects, and
+// testing it first changes no verdict.  The axis test of a group of obstacles
+// uses the least lo and the greatest hi of its members, which separates every
+// member it separates.  The tables hold fl(lo - r) and fl(hi + r) as the
+// full test rounds them, filled on the device from the scene at launch.
 // No FMA contraction anywhere in this translation unit, headers included:
 // the float kernels must keep the reference's rounding.
 #pragma clang fp contract(off)
@@ -204,6 +217,161 @@ __global__ __launch_bounds__(64) void k_edge_first(EdgeGroups gr, EnvDev env, Qu
     }
 }
 
+// Axis-test tables from the scene's current boxes: block (x, y) fills
+// groups of table y (keys[2y] = radius, keys[2y + 1] = 1 when the links of
+// this key skip ground boxes).
+__global__ void k_slab_build(EnvDev env, const float* __restrict__ keys, int nkeys, SlabEntry* slab) {
+    const int M = env.num_obj, G = (M + kSlabGroup - 1) / kSlabGroup;
+    const int t = blockIdx.y, g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nkeys || g >= G) return;
+    const float r = keys[2 * t];
+    const bool skip = keys[2 * t + 1] != 0.0f;
+    SlabEntry* tab = slab + (size_t)t * (G + M);
+    SlabEntry u{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}, 0, 0};
+    for (int m = 0; m < kSlabGroup; ++m) {
+        const int b = g * kSlabGroup + m;
+        if (b >= M) break;
+        const int j = env.gorder ? env.gorder[b] : b;
+        SlabEntry x{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}, -1, 0};
+        if (!(skip && (env.flags[j] & kObjGround))) {
+            x.idx = j;
+            for (int k = 0; k < 3; ++k) {
+                x.lo[k] = env.aabb[6 * j + k] - r;
+                x.hi[k] = env.aabb[6 * j + 3 + k] + r;
+                u.lo[k] = fminf(u.lo[k], x.lo[k]);
+                u.hi[k] = fmaxf(u.hi[k], x.hi[k]);
+            }
+        }
+        tab[G + b] = x;
+    }
+    tab[g] = u;
+}
+
+// One link (P0, P1) of radius r against the obstacles of table tab, for the
+// lanes with act set: true where some obstacle collides
+// (collisionCapsuleToBox).  The table pointer and the loop counts are
+// uniform, so the entries arrive through scalar loads.
+__device__ inline bool link_hits(const Vec3<float>& P0, const Vec3<float>& P1, float r, float rn,
+                                 const SlabEntry* __restrict__ tab, int G, int M, const Staged<float>& s,
+                                 const Params<float>& p, bool act) {
+    // per axis, the larger and the smaller end point coordinate: "both below
+    // L" is "the larger below L" unless a coordinate is NaN, and then every
+    // obstacle goes to the full test
+    const float mxx = fmaxf(P0.x, P1.x), mxy = fmaxf(P0.y, P1.y), mxz = fmaxf(P0.z, P1.z);
+    const float mnx = fminf(P0.x, P1.x), mny = fminf(P0.y, P1.y), mnz = fminf(P0.z, P1.z);
+    const float sum = ((P0.x + P0.y) + (P0.z + P1.x)) + (P1.y + P1.z);
+    const bool exact = !(sum == sum);
+    auto apart = [&](const SlabEntry& x) {
+        return (mxx < x.lo[0]) | (mxy < x.lo[1]) | (mxz < x.lo[2]) | (mnx > x.hi[0]) | (mny > x.hi[1]) |
+               (mnz > x.hi[2]);
+    };
+    bool hit = false;
+    for (int g = 0; g < G; ++g) {
+        const SlabEntry gx = tab[g];
+        if (act && !hit && (exact || !apart(gx))) {
+            const int b1 = min(M, (g + 1) * kSlabGroup);
+            for (int b = g * kSlabGroup; b < b1; ++b) {
+                const SlabEntry x = tab[G + b];
+                if (!hit && x.idx >= 0 && (exact || !apart(x)) && collision_box(P0, P1, r, rn, s.box[x.idx], p))
+                    hit = true;
+            }
+        }
+    }
+    return hit;
+}
+
+// The first-collision search on axis-aligned scenes: k_edge_first's items
+// and slots, with the state test of each lane written as uniform loops
+// (a lane that is done, or has no state, rides along masked off) so that the
+// tables arrive through scalar loads, and the joint values read from the
+// edge's rows (q = q1 + (k step) dir per joint, rounded as the reference's
+// interpolation is) instead of an LDS copy; LDS holds the obstacles and the
+// skeletons for the self pairs.
+__global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev env, QueryParams qp,
+                                                        const float* __restrict__ q1, const float* __restrict__ dir,
+                                                        int32_t row_stride, float step,
+                                                        const int32_t* __restrict__ order,
+                                                        const int32_t* __restrict__ rounds,
+                                                        int32_t* __restrict__ first, uint32_t* __restrict__ counter,
+                                                        int32_t total_items, int spl) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Staged<float> s = stage_env<float, false>(env, smem);
+    float* skel = reinterpret_cast<float*>(smem + ((staged_bytes<float>(env) + 15) & ~size_t(15)));
+    const int lane = threadIdx.x;
+    const int M = env.num_obj, G = (M + kSlabGroup - 1) / kSlabGroup;
+    const Params<float> p{qp.eq, qp.maxd};
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = (int)atomicAdd(counter, 1u);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));   // uniform: scalar registers
+        if (item >= total_items) break;
+        int g = 0;
+        while (g + 1 < gr.num && item >= gr.item_begin[g + 1]) ++g;
+        const RobotDev& rb = gr.rb[g];
+        const int32_t* keys = gr.link_key + gr.key_off[g];
+        for (int sub = 0; sub < spl; ++sub) {
+            const int32_t slot = ((item - gr.item_begin[g]) * spl + sub) * 64 + lane;
+            bool live = slot < gr.num_slots[g];
+            int k = 0;
+            int32_t e = 0;
+            if (live) {
+                k = round_of(rounds + gr.round_off[g], gr.num_rounds[g] + 1, slot);
+                e = order[gr.order_off[g] + (slot - rounds[gr.round_off[g] + k])];
+                live = __atomic_load_n(&first[e], __ATOMIC_RELAXED) > k;
+            }
+            if (!__any(live)) continue;
+            const float* qa = q1 + (int64_t)e * row_stride;
+            const float* qd = dir + (int64_t)e * row_stride;
+            const float sk = step * (float)k;
+            auto sc = [&](int m, double* st, double* ct) {
+                const float qv = qa[m] + sk * qd[m];
+                sincos((double)qv, st, ct);
+            };
+            const bool pairs = rb.num_pairs > 0;
+            FKStream fk;
+            fk_init(fk);
+            Vec3<float> P0 = skel_point_sc<float>(fk, rb, 0, sc);
+            if (pairs) {
+                skel[lane] = P0.x;
+                skel[64 + lane] = P0.y;
+                skel[128 + lane] = P0.z;
+            }
+            bool done = !live, bad = false;
+            for (int i = 0; i < rb.num_links; ++i) {
+                if (!__any(!done)) break;
+                const Vec3<float> P1 = skel_point_sc<float>(fk, rb, i + 1, sc);
+                if (pairs) {
+                    skel[(3 * i + 3) * 64 + lane] = P1.x;
+                    skel[(3 * i + 4) * 64 + lane] = P1.y;
+                    skel[(3 * i + 5) * 64 + lane] = P1.z;
+                }
+                const SlabEntry* tab = gr.slab + (size_t)keys[i] * (G + M);
+                if (link_hits(P0, P1, rb.radius[i], rb.rnew32[i], tab, G, M, s, p, !done)) {
+                    bad = true;
+                    done = true;
+                }
+                P0 = P1;
+            }
+            for (int m = 0; m < rb.num_pairs; ++m) {
+                if (!__any(!done)) break;
+                if (!done) {
+                    const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
+                    auto pt = [&](int kk) {
+                        return Vec3<float>{skel[(3 * kk) * 64 + lane], skel[(3 * kk + 1) * 64 + lane],
+                                           skel[(3 * kk + 2) * 64 + lane]};
+                    };
+                    const float sd = seg_seg<float, false>(pt(a), pt(a + 1), pt(b), pt(b + 1), p, nullptr);
+                    if (sd - rb.radius[a] - rb.radius[b] < 0) {
+                        bad = true;
+                        done = true;
+                    }
+                }
+            }
+            if (live && bad) atomicMin(&first[e], k);
+        }
+    }
+}
+
 inline dim3 blocks_for(int64_t n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
 
 }  // namespace
@@ -225,24 +393,30 @@ hipError_t launch_edge_valid(const RobotDev& rb, const EnvDev& env, const QueryP
 
 size_t edge_first_lds_bytes(const EdgeGroups& gr, const EnvDev& env) {
     return ((staged_bytes<float>(env) + 15) & ~size_t(15)) +
-           (size_t)64 * 4 * (gr.max_dofs + 3 * (gr.max_links + 1));
+           (size_t)64 * 4 * ((env.use_obb ? gr.max_dofs : 0) + 3 * (gr.max_links + 1));
 }
 
 hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const QueryParams& qp, const float* q1,
                              const float* dir, int32_t row_stride, float step, const int32_t* order,
                              const int32_t* rounds, int32_t* first, int64_t num_edges, uint32_t* counter,
-                             int32_t total_items, int waves, int spl, hipStream_t st) {
+                             int32_t total_items, int waves, int spl, const float* keys, int nkeys,
+                             hipStream_t st) {
     if (num_edges <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_first_init, blocks_for(num_edges, 256), dim3(256), 0, st, first, num_edges, counter);
     if (total_items > 0) {
         const int grid = std::max(1, std::min(total_items, waves));
         const size_t sm = edge_first_lds_bytes(gr, env);
-        if (env.use_obb)
+        if (!env.use_obb) {
+            const int G = (env.num_obj + kSlabGroup - 1) / kSlabGroup;
+            if (nkeys > 0 && G > 0)
+                hipLaunchKernelGGL(k_slab_build, dim3((unsigned)((G + 63) / 64), (unsigned)nkeys), dim3(64), 0, st,
+                                   env, keys, nkeys, const_cast<SlabEntry*>(gr.slab));
+            hipLaunchKernelGGL(k_edge_first_aabb, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride,
+                               step, order, rounds, first, counter, total_items, spl);
+        } else {
             hipLaunchKernelGGL(k_edge_first<true>, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride, step,
                                order, rounds, first, counter, total_items, spl);
-        else
-            hipLaunchKernelGGL(k_edge_first<false>, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride, step,
-                               order, rounds, first, counter, total_items, spl);
+        }
     }
     hipLaunchKernelGGL(k_first_final, blocks_for(num_edges, 256), dim3(256), 0, st, first, num_edges);
     return hipGetLastError();

@@ -85,6 +85,22 @@ constexpr int kMaxLdsObstacles = 1024;
 // group is state k of edge order[order_off[g] + s - rounds[round_off[g] + k]]
 // where k is the largest round with rounds[round_off[g] + k] <= s.
 constexpr int kMaxEdgeGroups = 8;
+
+// Axis-test table entry of the first-collision kernel on axis-aligned
+// scenes: an obstacle's bounds minus (lo) and plus (hi) one link radius, in
+// float as collisionCapsuleToBox's second test rounds them, and the
+// obstacle's index (-1: the link skips the obstacle, a ground box below
+// ground_included).  One table per distinct (radius, ground skipped) key:
+// ceil(M / kSlabGroup) group entries (the element-wise min of lo and max of
+// hi over kSlabGroup consecutive obstacles of the walk order) followed by M
+// obstacle entries in walk order.
+constexpr int kSlabGroup = 8;
+struct SlabEntry {
+    float lo[3], hi[3];
+    int32_t idx, pad;
+};
+static_assert(sizeof(SlabEntry) == 32, "one 32-byte scalar load per entry");
+
 struct EdgeGroups {
     RobotDev rb[kMaxEdgeGroups];
     int32_t num;
@@ -96,15 +112,24 @@ struct EdgeGroups {
     int32_t num_rounds[kMaxEdgeGroups];
     int32_t round_off[kMaxEdgeGroups];
     int32_t order_off[kMaxEdgeGroups];
+    // axis-test tables (axis-aligned scenes): link i of group g uses table
+    // link_key[key_off[g] + i] of slab
+    int32_t key_off[kMaxEdgeGroups];
+    const int32_t* link_key;
+    const SlabEntry* slab;
 };
 
 hipError_t launch_edge_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q1,
                              const float* q2, int64_t n, uint8_t* valid, hipStream_t st);
 size_t edge_first_lds_bytes(const EdgeGroups& gr, const EnvDev& env);
+// keys: (radius, ground skipped as 0 / 1) of each axis-test table, nkeys of
+// them; the launch fills gr.slab from the scene before the search.
 hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const QueryParams& qp, const float* q1,
                              const float* dir, int32_t row_stride, float step, const int32_t* order,
                              const int32_t* rounds, int32_t* first, int64_t num_edges, uint32_t* counter,
-                             int32_t total_items, int waves, int spl, hipStream_t st);
+                             int32_t total_items, int waves, int spl, const float* keys, int nkeys,
+                             hipStream_t st);
+inline size_t slab_entries(int num_obj) { return (size_t)(num_obj + kSlabGroup - 1) / kSlabGroup + num_obj; }
 
 // FP32 bulk pass: relaxed arithmetic, writes its results and appends every
 // configuration with an uncertain pair to idx (count in *count, which the

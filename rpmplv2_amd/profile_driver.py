@@ -1,6 +1,6 @@
 """Small driver for rocprofv3 runs of the distance kernels.
 
-    python -m rpmplv2_amd.profile_driver [--workload 2|5|bur|traj|edgeself] [--n N] [--calls K] [--calibrate]
+    python -m rpmplv2_amd.profile_driver [--workload 2|5|bur|traj|edgeself|edges] [--n N] [--calls K] [--calibrate]
                                          [--tuning VARIANT,BLOCK,BLOCKS_PER_CU,COHERENCE_BITS]
 
 Builds the seeded workload, optionally runs a 256 MiB elementwise copy first
@@ -8,8 +8,11 @@ Builds the seeded workload, optionally runs a 256 MiB elementwise copy first
 times on device-resident inputs: rvs_distance per precision (workloads 2 and
 5), generalized spines from one root with underestimation and real-distance
 layers (bur: 65,536 targets), trajectory safety (traj: 8,192 straight
-60-point trajectories) or xArm6 edge self-collision (edgeself: 16,384
-edges).  --tuning records a launch shape for the FP32 bulk pass (as
+60-point trajectories), xArm6 edge self-collision (edgeself: 16,384
+edges) or the first collision along the planner-start edges of the four
+chains (edges: 4 x 4,096 edges in one persistent launch, starts judged by
+the engine's own reference-arithmetic edge search, shape 32 waves per CU x
+2 slots, the autotuner's choice).  --tuning records a launch shape for the FP32 bulk pass (as
 rvs_set_tuning does), so that a profile can use the shape an autotuned run
 chose.  Used as the profiled
 program in `rocprofv3 --kernel-trace --stats -- python3 -m ...` and
@@ -44,6 +47,9 @@ def main():
 
     if args.workload in ("bur", "traj", "edgeself"):
         _bubble(args, torch, rvs, W, dev)
+        return
+    if args.workload == "edges":
+        _edges(args, torch, rvs, W, dev)
         return
     w = W.workload2(args.n) if args.workload == "2" else W.workload5(args.n)
     eng = rvs.Engine()
@@ -108,6 +114,33 @@ def _bubble(args, torch, rvs, W, dev):
         rid = eng.load_robot(rb)
         for _ in range(args.calls):
             eng.edge_self_collision(rid, q1, q2, 0.1)
+    torch.cuda.synchronize()
+    print("profile driver done", flush=True)
+    eng.close()
+
+
+def _edges(args, torch, rvs, W, dev):
+    import numpy as np
+    eng = rvs.Engine()
+    loaded = {}
+
+    def judge(rb, scene, q):
+        if id(scene) not in loaded:
+            loaded[id(scene)] = eng.create_env(scene)
+        if id(rb) not in loaded:
+            loaded[id(rb)] = eng.load_robot(rb)
+        first = eng.edge_first_collision(loaded[id(scene)], np.full(len(q), loaded[id(rb)], np.int32), q,
+                                         np.zeros_like(q), np.ones(len(q), np.int32))
+        return first < 0
+
+    w = W.workload3_planner(judge=judge)
+    eid = eng.create_env(w.scene)
+    ids, q1, u, K = eng.stack_edges(w.batches)
+    q1d, ud = torch.from_numpy(q1).to(dev), torch.from_numpy(u).to(dev)
+    first = torch.empty(len(ids), dtype=torch.int32, device=dev)
+    eng.set_edge_tuning(32, 2)
+    for _ in range(args.calls):
+        eng.edge_first_collision_device(eid, ids, q1d, ud, K, first)
     torch.cuda.synchronize()
     print("profile driver done", flush=True)
     eng.close()

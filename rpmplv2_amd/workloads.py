@@ -305,7 +305,7 @@ def _capsule_overlaps_box(P: np.ndarray, radius, aabb: np.ndarray, margin: float
 
 
 def workload3_planner(edges_per_robot: int = 4096, dofs=(10, 14, 18, 22), spread: float = 0.35,
-                      oracle=None) -> EdgeWorkload:
+                      oracle=None, judge=None) -> EdgeWorkload:
     """Edges as a tree-based planner checks them: RRTConnect::extend checks
     only edges that leave a tree node, and tree nodes are collision-free
     (RRTConnect.cpp:92-105 -> RealVectorSpace.cpp:261-275; the scene and
@@ -329,9 +329,12 @@ def workload3_planner(edges_per_robot: int = 4096, dofs=(10, 14, 18, 22), spread
     self-collision alone).
 
     `oracle` is the module that judges the starts (pyoracle): the generator
-    is test and benchmark infrastructure."""
-    if oracle is None:
-        raise ValueError("workload3_planner needs the oracle module to judge start states")
+    is test and benchmark infrastructure.  Or `judge(robot, scene, q)`, the
+    collision-free mask of the one-state edges q by another predicate with
+    the same verdicts (the engine's reference-arithmetic edge search, for
+    programs that do not load the oracle): the same starts."""
+    if oracle is None and judge is None:
+        raise ValueError("workload3_planner needs the oracle module or a judge of start states")
     robots = [b.robot for b in workload3(16, dofs).batches]
     homes = [R.skeleton_np(rb, np.zeros(rb.num_dofs, np.float32))[0] for rb in robots]
     rng = np.random.default_rng(1013)
@@ -344,7 +347,11 @@ def workload3_planner(edges_per_robot: int = 4096, dofs=(10, 14, 18, 22), spread
     sc = kept[0]
     for b in kept[1:]:
         sc = sc.concat(b)
-    osc = oracle.scene_of(sc)
+    if judge is None:
+        osc = oracle.scene_of(sc)
+
+        def judge(rb, scene, q):
+            return oracle.edge_first_collision(rb, osc, q, np.zeros_like(q), np.ones(len(q), np.int32), 0.05) < 0
     batches = []
     for rb in robots:
         n = rb.num_dofs
@@ -354,8 +361,7 @@ def workload3_planner(edges_per_robot: int = 4096, dofs=(10, 14, 18, 22), spread
             q = np.empty((m, n), np.float32)
             q[:, 0] = rng.uniform(-np.pi, np.pi, m)
             q[:, 1:] = rng.uniform(-spread, spread, (m, n - 1))
-            first = oracle.edge_first_collision(rb, osc, q, np.zeros_like(q), np.ones(m, np.int32), 0.05)
-            starts = np.concatenate([starts, q[first < 0]])
+            starts = np.concatenate([starts, q[judge(rb, sc, q)]])
         q1 = np.ascontiguousarray(starts[:edges_per_robot])
         u = rng.normal(size=(edges_per_robot, n))
         u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)

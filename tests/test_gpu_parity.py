@@ -336,11 +336,54 @@ def test_edge_first_collision_planner_workload(oracle, engine):
     allf = np.concatenate(expect)
     print(f"free {(allf < 0).mean():.3f}, median first collision {np.median(allf[allf >= 0])}")
     assert (allf != 0).all() and 0.1 < (allf < 0).mean() < 0.6 and np.median(allf[allf >= 0]) >= 5
+    # judged by the engine's own one-state edges instead of the oracle, the
+    # generator draws the same starts (the profile driver builds it that way)
+    loaded = {}
+
+    def judge(rb, scene, q):
+        if id(rb) not in loaded:
+            loaded[id(rb)] = engine.load_robot(rb)
+        one = np.ones(len(q), np.int32)
+        return engine.edge_first_collision(eid, np.full(len(q), loaded[id(rb)], np.int32), q, np.zeros_like(q),
+                                           one) < 0
+
+    wj = W.workload3_planner(1024, judge=judge)
+    for a, b in zip(w.batches, wj.batches):
+        assert np.array_equal(a.q1, b.q1) and np.array_equal(a.dir, b.dir), a.robot.name
     for wpc, spl in ((4, 2), (16, 1), (32, 4)):
         engine.set_edge_tuning(wpc, spl)
         got = engine.edge_batches(eid, w.batches)
         for b, g, e in zip(w.batches, got, expect):
             assert np.array_equal(g, e), (b.robot.name, wpc, spl, int((g != e).sum()))
+
+
+def test_edge_first_collision_radii_ground_and_nan(oracle, engine):
+    """xArm6 with a different radius on every link and the ground box
+    skipped by its first two links, among the ground box and 16 boxes, from
+    uniform starts with some NaN and infinite joint values: the first
+    colliding state of every edge equals the oracle's (the axis-test tables
+    hold one entry set per (radius, ground rule) and send NaN end points to
+    the full test)."""
+    w = W.workload2(4096)
+    rb = R.xarm6(radius=[0.05, 0.045, 0.04, 0.035, 0.03, 0.02], ground_included=2)
+    eid = engine.create_env(w.scene)
+    rng = np.random.default_rng(23)
+    n = 3000
+    q1 = w.q[:n].copy()
+    q1[::97, 2] = np.nan
+    q1[5::101, 0] = np.inf
+    q1[7::89, 4] = -np.inf
+    u = rng.normal(size=(n, 6))
+    u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+    K = rng.integers(1, 80, n).astype(np.int32)
+    b = W.EdgeBatch(rb, q1, u, K)
+    of = oracle.edge_first_collision(rb, oracle.scene_of(w.scene), q1, u, K, 0.05)
+    assert (of == 0).mean() < 0.9 and (of > 0).sum() > 100 and (of < 0).sum() > 20
+    for wpc, spl in ((16, 2), (4, 1)):
+        engine.set_edge_tuning(wpc, spl)
+        f = engine.edge_batches(eid, [b])[0]
+        bad = np.nonzero(f != of)[0]
+        assert len(bad) == 0, (wpc, spl, bad[:10], f[bad[:10]], of[bad[:10]])
 
 
 def test_edge_first_collision_sparse_scene(oracle, engine):
