@@ -1823,6 +1823,8 @@ struct EdgePlan {
     std::vector<int32_t> order, rounds;
     std::vector<float> keys;          // axis-test tables: (radius, ground skipped) per table
     std::vector<int32_t> link_key;    // table of each (group, link)
+    std::vector<int2> items;          // item table of the current run (kept until the call syncs)
+    int2* items_dev = nullptr;        // its device copy (room for max_items entries)
 };
 
 rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, int64_t num_edges, int32_t row_stride,
@@ -1886,11 +1888,40 @@ rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, i
     return RVS_OK;
 }
 
+// Items of 64 * spl slots, ordered by the state index of their first slot and
+// then by group, so that the early states of every robot's edges come before
+// the late states of any (the search skips the states past a found collision
+// only if that collision was found before it takes them).  Table entries are
+// (group, first slot).
+void item_table(const EdgePlan& pl, int spl, std::vector<int2>& t) {
+    struct Item { int32_t k, g, s0; };
+    std::vector<Item> it;
+    const int32_t per = 64 * spl;
+    for (int g = 0; g < pl.gr.num; ++g) {
+        const int32_t* r = pl.rounds.data() + pl.gr.round_off[g];
+        int32_t k = 0;
+        for (int32_t s0 = 0; s0 < pl.gr.num_slots[g]; s0 += per) {
+            while (k + 1 <= pl.gr.num_rounds[g] && r[k + 1] <= s0) ++k;
+            it.push_back({k, g, s0});
+        }
+    }
+    std::stable_sort(it.begin(), it.end(), [](const Item& x, const Item& y) { return x.k < y.k; });
+    t.resize(it.size());
+    for (size_t i = 0; i < it.size(); ++i) t[i] = make_int2(it[i].g, it[i].s0);
+}
+
+// Slots of a plan in 64-slot items (the largest item table of any spl).
+size_t max_items(const EdgePlan& pl) {
+    size_t n = 0;
+    for (int g = 0; g < pl.gr.num; ++g) n += ((size_t)pl.gr.num_slots[g] + 63) / 64;
+    return n;
+}
+
 // Arena bytes of a plan's device tables (before the per-call rows).
 std::vector<size_t> plan_bytes(const EdgePlan& pl, const EnvRec& ev) {
     const size_t nk = ev.dev.use_obb ? 0 : pl.keys.size() / 2;
     return {pl.order.size() * 4, pl.rounds.size() * 4, 256, pl.link_key.size() * 4, pl.keys.size() * 4,
-            nk * slab_entries(ev.dev.num_obj) * sizeof(SlabEntry)};
+            nk * slab_entries(ev.dev.num_obj) * sizeof(SlabEntry), max_items(pl) * sizeof(int2)};
 }
 
 // Carves and uploads a plan's tables (in plan_bytes order); sets the table
@@ -1910,6 +1941,7 @@ hipError_t upload_plan(rvs_context* c, const EnvRec& ev, EdgePlan& pl, Carve& cv
     d.keys = cv.take<float>(pl.keys.size());
     d.nkeys = ev.dev.use_obb ? 0 : (int)pl.keys.size() / 2;
     pl.gr.slab = cv.take<SlabEntry>((size_t)d.nkeys * slab_entries(ev.dev.num_obj));
+    pl.items_dev = cv.take<int2>(max_items(pl));
     pl.gr.link_key = lk;
     hipError_t e = h2d(d.order, pl.order.data(), pl.order.size() * 4, c->stream);
     if (e == hipSuccess) e = h2d(d.rounds, pl.rounds.data(), pl.rounds.size() * 4, c->stream);
@@ -1918,26 +1950,16 @@ hipError_t upload_plan(rvs_context* c, const EnvRec& ev, EdgePlan& pl, Carve& cv
     return e;
 }
 
-// Items of 64 * spl slots per group; returns the total.
-int32_t set_items(EdgeGroups& gr, int spl) {
-    int64_t items = 0;
-    const int64_t per = 64 * (int64_t)spl;
-    for (int g = 0; g < gr.num; ++g) {
-        gr.item_begin[g] = (int32_t)items;
-        items += (gr.num_slots[g] + per - 1) / per;
-    }
-    gr.item_begin[gr.num] = (int32_t)items;
-    return (int32_t)items;
-}
-
 // Runs a planned call on device arrays (q1, dir, first) with a launch shape.
 hipError_t run_edges(rvs_context* c, const EnvRec& ev, EdgePlan& pl, const float* q1, const float* dir,
                      int32_t row_stride, float step, int32_t* first, int64_t num_edges, const PlanDev& d,
                      int waves_per_cu, int spl, bool timing) {
-    const int32_t items = set_items(pl.gr, spl);
+    item_table(pl, spl, pl.items);
+    const int32_t items = (int32_t)pl.items.size();
+    pl.gr.items = pl.items_dev;
     const QueryParams qp = params_of(c);
-    hipError_t e = hipSuccess;
-    if (timing) e = hipEventRecord(c->ev[0], c->stream);
+    hipError_t e = h2d(pl.items_dev, pl.items.data(), pl.items.size() * sizeof(int2), c->stream);
+    if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
     if (e == hipSuccess)
         e = launch_edge_first(pl.gr, ev.dev, qp, q1, dir, row_stride, step, d.order, d.rounds, first, num_edges,
                               d.counter, items, c->num_cus * waves_per_cu, spl, d.keys, d.nkeys, c->stream);

@@ -187,13 +187,13 @@ __global__ __launch_bounds__(64) void k_edge_first(EdgeGroups gr, EnvDev env, Qu
         if (lane == 0) item = (int)atomicAdd(counter, 1u);
         item = __shfl(item, 0, 64);
         if (item >= total_items) break;
-        // robot group of this item (uniform across the wave)
-        int g = 0;
-        while (g + 1 < gr.num && item >= gr.item_begin[g + 1]) ++g;
+        // robot group and first slot of this item (uniform across the wave)
+        const int2 it = gr.items[item];
+        const int g = it.x;
         const RobotDev& rb = gr.rb[g];
         // an item holds 64 * spl slots: lane takes slots lane, lane + 64, ...
         for (int sub = 0; sub < spl; ++sub) {
-            const int32_t slot = ((item - gr.item_begin[g]) * spl + sub) * 64 + lane;
+            const int32_t slot = it.y + sub * 64 + lane;
             if (slot >= gr.num_slots[g]) break;
             const int k = round_of(rounds + gr.round_off[g], gr.num_rounds[g] + 1, slot);
             const int32_t e = order[gr.order_off[g] + (slot - rounds[gr.round_off[g] + k])];
@@ -247,12 +247,33 @@ __global__ void k_slab_build(EnvDev env, const float* __restrict__ keys, int nke
     tab[g] = u;
 }
 
+// Read-only tables seen through the constant address space: their loads at
+// uniform addresses become scalar loads even after the LDS barriers of the
+// kernel (which the compiler would otherwise take for possible writes).
+template <typename T>
+using ConstPtr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ inline ConstPtr<T> as_const(const T* p) {
+    return (ConstPtr<T>)p;
+}
+__device__ inline SlabEntry load_entry(ConstPtr<SlabEntry> p) {
+    SlabEntry x;
+    for (int k = 0; k < 3; ++k) {
+        x.lo[k] = p->lo[k];
+        x.hi[k] = p->hi[k];
+    }
+    x.idx = p->idx;
+    x.pad = 0;
+    return x;
+}
+
 // One link (P0, P1) of radius r against the obstacles of table tab, for the
 // lanes with act set: true where some obstacle collides
 // (collisionCapsuleToBox).  The table pointer and the loop counts are
-// uniform, so the entries arrive through scalar loads.
+// uniform, so the entries, and the bounds of an obstacle that goes to the
+// full test, arrive through scalar loads.
 __device__ inline bool link_hits(const Vec3<float>& P0, const Vec3<float>& P1, float r, float rn,
-                                 const SlabEntry* __restrict__ tab, int G, int M, const Staged<float>& s,
+                                 ConstPtr<SlabEntry> tab, int G, int M, ConstPtr<float> aabb,
                                  const Params<float>& p, bool act) {
     // per axis, the larger and the smaller end point coordinate: "both below
     // L" is "the larger below L" unless a coordinate is NaN, and then every
@@ -267,13 +288,16 @@ __device__ inline bool link_hits(const Vec3<float>& P0, const Vec3<float>& P1, f
     };
     bool hit = false;
     for (int g = 0; g < G; ++g) {
-        const SlabEntry gx = tab[g];
+        const SlabEntry gx = load_entry(tab + g);
         if (act && !hit && (exact || !apart(gx))) {
             const int b1 = min(M, (g + 1) * kSlabGroup);
             for (int b = g * kSlabGroup; b < b1; ++b) {
-                const SlabEntry x = tab[G + b];
-                if (!hit && x.idx >= 0 && (exact || !apart(x)) && collision_box(P0, P1, r, rn, s.box[x.idx], p))
-                    hit = true;
+                const SlabEntry x = load_entry(tab + G + b);
+                if (!hit && x.idx >= 0 && (exact || !apart(x))) {
+                    ConstPtr<float> o = aabb + 6 * x.idx;
+                    const Box<float> bx{{o[0], o[1], o[2]}, {o[3], o[4], o[5]}};
+                    if (collision_box(P0, P1, r, rn, bx, p)) hit = true;
+                }
             }
         }
     }
@@ -281,12 +305,15 @@ __device__ inline bool link_hits(const Vec3<float>& P0, const Vec3<float>& P1, f
 }
 
 // The first-collision search on axis-aligned scenes: k_edge_first's items
-// and slots, with the state test of each lane written as uniform loops
-// (a lane that is done, or has no state, rides along masked off) so that the
-// tables arrive through scalar loads, and the joint values read from the
-// edge's rows (q = q1 + (k step) dir per joint, rounded as the reference's
-// interpolation is) instead of an LDS copy; LDS holds the obstacles and the
-// skeletons for the self pairs.
+// and slots, with two changes.  The states that are still needed (no
+// collision found yet at an earlier state of their edge) are gathered into a
+// queue in LDS, 64 per evaluation, so that slots of finished edges take no
+// lanes; and the state test of each lane is written as uniform loops (a lane
+// that is done, or has no state, rides along masked off) so that the tables
+// arrive through scalar loads.  The joint values come from the edge's rows
+// (q = q1 + (k step) dir per joint, rounded as the reference's interpolation
+// is); LDS holds the queue and the skeletons for the self pairs.
+constexpr int kQueue = 128;
 __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev env, QueryParams qp,
                                                         const float* __restrict__ q1, const float* __restrict__ dir,
                                                         int32_t row_stride, float step,
@@ -295,22 +322,33 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
                                                         int32_t* __restrict__ first, uint32_t* __restrict__ counter,
                                                         int32_t total_items, int spl) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<float> s = stage_env<float, false>(env, smem);
-    float* skel = reinterpret_cast<float*>(smem + ((staged_bytes<float>(env) + 15) & ~size_t(15)));
+    int2* queue = reinterpret_cast<int2*>(smem);
+    float* skel = reinterpret_cast<float*>(smem + kQueue * sizeof(int2));
     const int lane = threadIdx.x;
     const int M = env.num_obj, G = (M + kSlabGroup - 1) / kSlabGroup;
     const Params<float> p{qp.eq, qp.maxd};
+    // uniform state: queued entries and their group, the current item and
+    // its next 64-slot slice
+    int qn = 0, qg = 0, item = 0, sub = spl;
+    bool drained = false;
     for (;;) {
-        int item = 0;
-        if (lane == 0) item = (int)atomicAdd(counter, 1u);
-        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));   // uniform: scalar registers
-        if (item >= total_items) break;
-        int g = 0;
-        while (g + 1 < gr.num && item >= gr.item_begin[g + 1]) ++g;
-        const RobotDev& rb = gr.rb[g];
-        const int32_t* keys = gr.link_key + gr.key_off[g];
-        for (int sub = 0; sub < spl; ++sub) {
-            const int32_t slot = ((item - gr.item_begin[g]) * spl + sub) * 64 + lane;
+        // gather: slices until 64 states wait, the group changes or the items run out
+        while (qn < 64 && !drained) {
+            if (sub == spl) {
+                int it = 0;
+                if (lane == 0) it = (int)atomicAdd(counter, 1u);
+                item = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64));
+                sub = 0;
+                if (item >= total_items) {
+                    drained = true;
+                    break;
+                }
+            }
+            const int g = as_const(gr.items)[item].x;
+            if (qn > 0 && g != qg) break;   // the queued group is evaluated first
+            qg = g;
+            const int32_t slot = as_const(gr.items)[item].y + sub * 64 + lane;
+            ++sub;
             bool live = slot < gr.num_slots[g];
             int k = 0;
             int32_t e = 0;
@@ -319,56 +357,82 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
                 e = order[gr.order_off[g] + (slot - rounds[gr.round_off[g] + k])];
                 live = __atomic_load_n(&first[e], __ATOMIC_RELAXED) > k;
             }
-            if (!__any(live)) continue;
-            const float* qa = q1 + (int64_t)e * row_stride;
-            const float* qd = dir + (int64_t)e * row_stride;
-            const float sk = step * (float)k;
-            auto sc = [&](int m, double* st, double* ct) {
-                const float qv = qa[m] + sk * qd[m];
-                sincos((double)qv, st, ct);
-            };
-            const bool pairs = rb.num_pairs > 0;
-            FKStream fk;
-            fk_init(fk);
-            Vec3<float> P0 = skel_point_sc<float>(fk, rb, 0, sc);
-            if (pairs) {
-                skel[lane] = P0.x;
-                skel[64 + lane] = P0.y;
-                skel[128 + lane] = P0.z;
+            const uint64_t m = __ballot(live);
+            if (live) {
+                const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                queue[qn + rank] = make_int2(e, k);
             }
-            bool done = !live, bad = false;
-            for (int i = 0; i < rb.num_links; ++i) {
-                if (!__any(!done)) break;
-                const Vec3<float> P1 = skel_point_sc<float>(fk, rb, i + 1, sc);
-                if (pairs) {
-                    skel[(3 * i + 3) * 64 + lane] = P1.x;
-                    skel[(3 * i + 4) * 64 + lane] = P1.y;
-                    skel[(3 * i + 5) * 64 + lane] = P1.z;
-                }
-                const SlabEntry* tab = gr.slab + (size_t)keys[i] * (G + M);
-                if (link_hits(P0, P1, rb.radius[i], rb.rnew32[i], tab, G, M, s, p, !done)) {
+            qn += __builtin_popcountll(m);
+            __syncthreads();
+        }
+        if (qn == 0) {
+            if (drained) break;
+            continue;
+        }
+        // evaluate the last min(qn, 64) queued states
+        const int take = min(qn, 64);
+        qn -= take;
+        bool live = lane < take;
+        int32_t e = 0;
+        int k = 0;
+        if (live) {
+            const int2 x = queue[qn + lane];
+            e = x.x;
+            k = x.y;
+            live = __atomic_load_n(&first[e], __ATOMIC_RELAXED) > k;
+        }
+        __syncthreads();
+        const RobotDev& rb = gr.rb[qg];
+        ConstPtr<int32_t> keys = as_const(gr.link_key) + gr.key_off[qg];
+        const float* qa = q1 + (int64_t)e * row_stride;
+        const float* qd = dir + (int64_t)e * row_stride;
+        const float sk = step * (float)k;
+        auto sc = [&](int j, double* st, double* ct) {
+            const float qv = qa[j] + sk * qd[j];
+            sincos((double)qv, st, ct);
+        };
+        const bool pairs = rb.num_pairs > 0;
+        FKStream fk;
+        fk_init(fk);
+        Vec3<float> P0 = skel_point_sc<float>(fk, rb, 0, sc);
+        if (pairs) {
+            skel[lane] = P0.x;
+            skel[64 + lane] = P0.y;
+            skel[128 + lane] = P0.z;
+        }
+        bool done = !live, bad = false;
+        for (int i = 0; i < rb.num_links; ++i) {
+            if (!__any(!done)) break;
+            const Vec3<float> P1 = skel_point_sc<float>(fk, rb, i + 1, sc);
+            if (pairs) {
+                skel[(3 * i + 3) * 64 + lane] = P1.x;
+                skel[(3 * i + 4) * 64 + lane] = P1.y;
+                skel[(3 * i + 5) * 64 + lane] = P1.z;
+            }
+            ConstPtr<SlabEntry> tab = as_const(gr.slab) + (size_t)keys[i] * (G + M);
+            if (link_hits(P0, P1, rb.radius[i], rb.rnew32[i], tab, G, M, as_const(env.aabb), p, !done)) {
+                bad = true;
+                done = true;
+            }
+            P0 = P1;
+        }
+        for (int m = 0; m < rb.num_pairs; ++m) {
+            if (!__any(!done)) break;
+            if (!done) {
+                const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
+                auto pt = [&](int kk) {
+                    return Vec3<float>{skel[(3 * kk) * 64 + lane], skel[(3 * kk + 1) * 64 + lane],
+                                       skel[(3 * kk + 2) * 64 + lane]};
+                };
+                const float sd = seg_seg<float, false>(pt(a), pt(a + 1), pt(b), pt(b + 1), p, nullptr);
+                if (sd - rb.radius[a] - rb.radius[b] < 0) {
                     bad = true;
                     done = true;
                 }
-                P0 = P1;
             }
-            for (int m = 0; m < rb.num_pairs; ++m) {
-                if (!__any(!done)) break;
-                if (!done) {
-                    const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
-                    auto pt = [&](int kk) {
-                        return Vec3<float>{skel[(3 * kk) * 64 + lane], skel[(3 * kk + 1) * 64 + lane],
-                                           skel[(3 * kk + 2) * 64 + lane]};
-                    };
-                    const float sd = seg_seg<float, false>(pt(a), pt(a + 1), pt(b), pt(b + 1), p, nullptr);
-                    if (sd - rb.radius[a] - rb.radius[b] < 0) {
-                        bad = true;
-                        done = true;
-                    }
-                }
-            }
-            if (live && bad) atomicMin(&first[e], k);
         }
+        if (live && bad) atomicMin(&first[e], k);
     }
 }
 
@@ -392,8 +456,9 @@ hipError_t launch_edge_valid(const RobotDev& rb, const EnvDev& env, const QueryP
 }
 
 size_t edge_first_lds_bytes(const EdgeGroups& gr, const EnvDev& env) {
-    return ((staged_bytes<float>(env) + 15) & ~size_t(15)) +
-           (size_t)64 * 4 * ((env.use_obb ? gr.max_dofs : 0) + 3 * (gr.max_links + 1));
+    const size_t skel = (size_t)64 * 4 * 3 * (gr.max_links + 1);
+    if (!env.use_obb) return kQueue * sizeof(int2) + skel;
+    return ((staged_bytes<float>(env) + 15) & ~size_t(15)) + (size_t)64 * 4 * gr.max_dofs + skel;
 }
 
 hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const QueryParams& qp, const float* q1,

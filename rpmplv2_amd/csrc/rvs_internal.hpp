@@ -79,11 +79,12 @@ struct DistOut {
 // Largest obstacle count staged whole in LDS by the distance kernels.
 constexpr int kMaxLdsObstacles = 1024;
 
-// Edges of several robots for the persistent first-collision kernel.  Group g
-// owns items [item_begin[g], item_begin[g+1]) of 64 * spl slots each (spl =
-// slots per lane); slot s of the
-// group is state k of edge order[order_off[g] + s - rounds[round_off[g] + k]]
-// where k is the largest round with rounds[round_off[g] + k] <= s.
+// Edges of several robots for the persistent first-collision kernel.  Item t
+// is 64 * spl consecutive slots (spl = slots per lane) of group items[t].x
+// from slot items[t].y on; slot s of group g is state k of edge
+// order[order_off[g] + s - rounds[round_off[g] + k]] where k is the largest
+// round with rounds[round_off[g] + k] <= s.  Items are ordered by the state
+// index of their first slot, then by group.
 constexpr int kMaxEdgeGroups = 8;
 
 // Axis-test table entry of the first-collision kernel on axis-aligned
@@ -107,7 +108,7 @@ struct EdgeGroups {
     int32_t max_dofs;
     int32_t max_links;
     int32_t pad;
-    int32_t item_begin[kMaxEdgeGroups + 1];
+    const int2* items;
     int32_t num_slots[kMaxEdgeGroups];
     int32_t num_rounds[kMaxEdgeGroups];
     int32_t round_off[kMaxEdgeGroups];
