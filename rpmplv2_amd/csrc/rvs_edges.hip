@@ -28,6 +28,19 @@
 // uses the least lo and the greatest hi of its members, which separates every
 // member it separates.  The tables hold fl(lo - r) and fl(hi + r) as the
 // full test rounds them, filled on the device from the scene at launch.
+//
+// The self pairs get a bound first too.  distanceLineSegToLineSeg
+// (CollisionAndDistance.cpp:217-320) returns 0 (below the equality threshold
+// eq), max_distance (degenerate input), or the float norm of p1 - p0 for
+// points that are end points or a + o (b - a) with o in [0, 1], each within
+// 5 eps C of its segment (eps = 2^-24, C the largest coordinate magnitude), so
+// the returned distance is at least (D - 21 eps C)(1 - 2 eps) with D the
+// distance of the two segments' bounding boxes.  A pair whose D, computed in
+// float, exceeds R = max(r_a + r_b, eq) by 4e-6 (C + R) (about 67 eps (C + R),
+// which also covers the rounding of D itself) therefore gets a distance of
+// at least r_a + r_b and is no collision, provided max_distance >= R; the
+// full test runs for every other pair, and for every pair of a state with a
+// NaN coordinate.
 // No FMA contraction anywhere in this translation unit, headers included:
 // the float kernels must keep the reference's rounding.
 #pragma clang fp contract(off)
@@ -396,6 +409,10 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
         FKStream fk;
         fk_init(fk);
         Vec3<float> P0 = skel_point_sc<float>(fk, rb, 0, sc);
+        // largest coordinate magnitude and NaN check of the skeleton, for the
+        // self-pair bound
+        float cmax = fmaxf(fmaxf(fabsf(P0.x), fabsf(P0.y)), fabsf(P0.z));
+        float csum = (P0.x + P0.y) + P0.z;
         if (pairs) {
             skel[lane] = P0.x;
             skel[64 + lane] = P0.y;
@@ -409,6 +426,8 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
                 skel[(3 * i + 3) * 64 + lane] = P1.x;
                 skel[(3 * i + 4) * 64 + lane] = P1.y;
                 skel[(3 * i + 5) * 64 + lane] = P1.z;
+                cmax = fmaxf(cmax, fmaxf(fmaxf(fabsf(P1.x), fabsf(P1.y)), fabsf(P1.z)));
+                csum = csum + ((P1.x + P1.y) + P1.z);
             }
             ConstPtr<SlabEntry> tab = as_const(gr.slab) + (size_t)keys[i] * (G + M);
             if (link_hits(P0, P1, rb.radius[i], rb.rnew32[i], tab, G, M, as_const(env.aabb), p, !done)) {
@@ -417,16 +436,26 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
             }
             P0 = P1;
         }
+        const bool bound_ok = csum == csum;
         for (int m = 0; m < rb.num_pairs; ++m) {
             if (!__any(!done)) break;
-            if (!done) {
-                const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
-                auto pt = [&](int kk) {
-                    return Vec3<float>{skel[(3 * kk) * 64 + lane], skel[(3 * kk + 1) * 64 + lane],
-                                       skel[(3 * kk + 2) * 64 + lane]};
-                };
-                const float sd = seg_seg<float, false>(pt(a), pt(a + 1), pt(b), pt(b + 1), p, nullptr);
-                if (sd - rb.radius[a] - rb.radius[b] < 0) {
+            const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
+            const float ra = rb.radius[a], rbb = rb.radius[b];
+            auto pt = [&](int kk) {
+                return Vec3<float>{skel[(3 * kk) * 64 + lane], skel[(3 * kk + 1) * 64 + lane],
+                                   skel[(3 * kk + 2) * 64 + lane]};
+            };
+            const Vec3<float> A = pt(a), B = pt(a + 1), C = pt(b), D = pt(b + 1);
+            // distance of the two bounding boxes, squared
+            const float gx = fmaxf(fmaxf(fminf(C.x, D.x) - fmaxf(A.x, B.x), fminf(A.x, B.x) - fmaxf(C.x, D.x)), 0.0f);
+            const float gy = fmaxf(fmaxf(fminf(C.y, D.y) - fmaxf(A.y, B.y), fminf(A.y, B.y) - fmaxf(C.y, D.y)), 0.0f);
+            const float gz = fmaxf(fmaxf(fminf(C.z, D.z) - fmaxf(A.z, B.z), fminf(A.z, B.z) - fmaxf(C.z, D.z)), 0.0f);
+            const float R = fmaxf(ra + rbb, p.eq);
+            const float thr = R + 4e-6f * (cmax + R);
+            const bool apart = bound_ok && thr <= p.maxd && (gx * gx + gy * gy) + gz * gz > thr * thr;
+            if (!done && !apart) {
+                const float sd = seg_seg<float, false>(A, B, C, D, p, nullptr);
+                if (sd - ra - rbb < 0) {
                     bad = true;
                     done = true;
                 }

@@ -386,6 +386,34 @@ def test_edge_first_collision_radii_ground_and_nan(oracle, engine):
         assert len(bad) == 0, (wpc, spl, bad[:10], f[bad[:10]], of[bad[:10]])
 
 
+def test_edge_first_collision_folded_chains(oracle, engine):
+    """Chains whose joints swing far (|q| up to 2 rad) fold onto
+    themselves: many states have self pairs at or near contact, where the
+    bounding-box bound of the self pairs must leave the verdict to the full
+    segment-segment test.  Two box-free scenes far from the chains, so that
+    self-collision alone ends the edges; first collisions equal the
+    oracle's."""
+    rng = np.random.default_rng(41)
+    sc = W._boxes(rng, 4, 0.05, 0.1, 0.5, center=(10.0, 10.0, 10.0))
+    eid = engine.create_env(sc)
+    batches = []
+    for n, r in ((8, 0.03), (12, 0.02)):
+        rb = R.chain(rng.uniform(0.06, 0.12, n), radius=r, self_collision=True)
+        m = 1500
+        q1 = rng.uniform(-2.0, 2.0, (m, n)).astype(np.float32)
+        u = rng.normal(size=(m, n))
+        u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+        K = rng.integers(1, 40, m).astype(np.int32)
+        batches.append(W.EdgeBatch(rb, q1, u, K))
+    osc = oracle.scene_of(sc)
+    expect = [oracle.edge_first_collision(b.robot, osc, b.q1, b.dir, b.K, b.step) for b in batches]
+    allf = np.concatenate(expect)
+    assert (allf == 0).mean() > 0.2 and (allf > 0).sum() > 400 and (allf < 0).sum() > 1000
+    got = engine.edge_batches(eid, batches)
+    for b, g, e in zip(batches, got, expect):
+        assert np.array_equal(g, e), (b.robot.num_dofs, int((g != e).sum()))
+
+
 def test_edge_first_collision_sparse_scene(oracle, engine):
     # few, small obstacles so many edges run to their last state
     rng = np.random.default_rng(11)
