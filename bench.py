@@ -64,7 +64,7 @@ PEAK_HBM_GBPS = 8000.0
 COUNTER_FLOP_PER_CONFIG = {
     ("xarm6_16aabb_ground", "k_distance_fast"): (2276.6, "profiles/k_distance_fast_xarm6_1M.md"),
     ("chain22_512rotated_boxes", "k_distance_fast_links"):
-        (14804.1, "profiles/k_distance_fast_links_chain22_512obb.md"),
+        (15056.3, "profiles/k_distance_fast_links_chain22_512obb.md"),
 }
 # VALU wave instructions per configuration of the same kernels (SQ_INSTS_VALU
 # / configurations, same profiles): the issue rate they reach against the
@@ -73,14 +73,24 @@ COUNTER_FLOP_PER_CONFIG = {
 # long before the FP32 peak.
 COUNTER_VALU_PER_CONFIG = {
     ("xarm6_16aabb_ground", "k_distance_fast"): 74.77,
-    ("chain22_512rotated_boxes", "k_distance_fast_links"): 763.49,
+    ("chain22_512rotated_boxes", "k_distance_fast_links"): 775.31,
 }
 PEAK_VALU_ISSUE_G = 1024 * 2.4 / 4
 # HBM bytes per configuration of the link-major bulk pass on the 22-DOF chain
 # with 512 rotated boxes (all 22 launches of a call): FETCH_SIZE 1,730,503.6
-# KiB x 2.0 + WRITE_SIZE 1,899,228.7 KiB x 1.0 per call on 2,097,152
+# KiB x 2.0 + WRITE_SIZE 1,900,112.3 KiB x 1.0 per call on 2,097,152
 # configurations (profiles/k_distance_fast_links_chain22_512obb.md).
-LINKS_TRAFFIC_PER_CONFIG = (2.0 * 1730503.6 + 1899228.7) * 1024 / 2097152
+LINKS_TRAFFIC_PER_CONFIG = (2.0 * 1733610.1 + 1900112.3) * 1024 / 2097152
+# The first-collision search on the planner-start edges (four chains, 4,096
+# edges each, shape 8 waves per CU x 4 slots), per launch, from rocprofv3
+# counters (profiles/k_edge_first_chains_planner_starts.md): VALU wave
+# instructions, FP32 and FP64 operations executed (SQ_INSTS_VALU_FLOPS_FP32 /
+# _FP64 x 23.64 lanes active per VALU instruction) and HBM bytes (FETCH_SIZE
+# 35,351 KiB x 2.0 + WRITE_SIZE 4,775.5 KiB).
+EDGE_COUNTERS = {"valu_instr": 3.4269e8, "fp32_flop": 1.3266e8 * 23.64, "fp64_flop": 3.3709e7 * 23.64,
+                 "traffic_bytes": (2.0 * 35351.0 + 4775.5) * 1024,
+                 "source": "profiles/k_edge_first_chains_planner_starts.md"}
+PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per launch of the FP32 bulk kernel on the headline workload from
 # two `rocprofv3 --pmc` runs: FETCH_SIZE 161,910.5 KiB x 2.0 + WRITE_SIZE
 # 124,679.7 KiB x 1.0, the factors being what the same counters read for a
@@ -367,9 +377,11 @@ def workload1_section(eng, cores: int) -> dict:
             "cpu_oracle_fp64": {"configs_per_s": n / cpu_s, "cores": cores}}
 
 
-def _edge_run(eng, O, w, cores: int) -> dict:
+def _edge_run(eng, O, w, cores: int, profiled: bool = False) -> dict:
     """One persistent first-collision launch over every edge of the four
-    chains (after rvs_autotune_edges), checked against the oracle."""
+    chains (after rvs_autotune_edges), checked against the oracle.  profiled:
+    the workload of the counter profile (EDGE_COUNTERS), which then gives
+    the achieved rates over this run's kernel time."""
     import numpy as np
     import torch
     eid = eng.create_env(w.scene)
@@ -404,19 +416,31 @@ def _edge_run(eng, O, w, cores: int) -> dict:
         off += m
     nominal = int(K.sum())
     dense = flop_eval / (kms * 1e-3) / 1e12
+    roof = {"bound": "valu", "kernel": "k_edge_first_aabb", "achieved": None, "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "dense_equivalent_tflops": dense, "dense_frac": dense / PEAK_FP32_TFLOPS,
+            "dense_note": "nominal work of every state a search reaches (edge start to its first collision or "
+                          "its end): 100 per FK joint + 160 per capsule-box pair + 60 per self pair; the axis "
+                          "tests and early exits evaluate far less"}
+    if profiled:
+        C = EDGE_COUNTERS
+        sec = kms * 1e-3
+        fp32 = C["fp32_flop"] / sec / 1e12
+        roof.update({"achieved": fp32, "frac": fp32 / PEAK_FP32_TFLOPS,
+                     "flop_accounting": "FP32 operations executed per launch from rocprofv3 counters (" +
+                                        C["source"] + ") / this run's kernel time",
+                     "fp64": {"achieved": C["fp64_flop"] / sec / 1e12, "peak": PEAK_FP64_TFLOPS,
+                              "note": "the forward kinematics in double, as the reference's KDL chain"},
+                     "valu_issue": {"valu_instr_per_launch": C["valu_instr"],
+                                    "achieved_G_per_s": C["valu_instr"] / sec / 1e9, "peak_G_per_s": PEAK_VALU_ISSUE_G,
+                                    "frac": C["valu_instr"] / sec / 1e9 / PEAK_VALU_ISSUE_G},
+                     "traffic": C["traffic_bytes"]})
     return {"edges": len(ids), "states_nominal": nominal, "states_evaluated": states_eval,
             "ms": ms, "kernel_ms": kms, "edges_per_s": len(ids) / (ms * 1e-3),
             "states_evaluated_per_s": states_eval / (ms * 1e-3),
             "states_nominal_per_s": nominal / (ms * 1e-3), "tuning": tune,
             "first_collision_mismatch_vs_oracle": mism,
             "edges_colliding_at_state0": int((got == 0).sum()), "edges_free": int((got == -1).sum()),
-            "roofline": {"bound": "valu", "kernel": "k_edge_first", "achieved": None, "peak": PEAK_FP32_TFLOPS,
-                         "unit": "TFLOP/s", "dense_equivalent_tflops": dense,
-                         "dense_frac": dense / PEAK_FP32_TFLOPS,
-                         "dense_note": "nominal work of every state a search reaches (edge start to its first "
-                                       "collision or its end): 100 per FK joint + 160 per capsule-box pair + 60 "
-                                       "per self pair; the kernel's broad phase and early exit evaluate less, and "
-                                       "no counter profile of this kernel is recorded, so no achieved rate"}}
+            "roofline": roof}
 
 
 def edges_section(eng, cores: int) -> dict:
@@ -428,7 +452,7 @@ def edges_section(eng, cores: int) -> dict:
     O = oracle_module()
     out = {"workload": "chains_10_14_18_22dof_64boxes_4096edges_each"}
     out["as_specified"] = _edge_run(eng, O, W.workload3(), cores)
-    out["planner_starts"] = _edge_run(eng, O, W.workload3_planner(oracle=O), cores)
+    out["planner_starts"] = _edge_run(eng, O, W.workload3_planner(oracle=O), cores, profiled=True)
     return out
 
 

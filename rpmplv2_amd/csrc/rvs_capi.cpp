@@ -305,6 +305,7 @@ void rvs_destroy(rvs_context* c) {
     for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     if (c->count_host) (void)hipHostFree(c->count_host);
     c->arena.release();
+    c->staging.release();
     for (int k = 0; k < 4; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->async_ev) (void)hipEventDestroy(c->async_ev);
@@ -1823,8 +1824,8 @@ struct EdgePlan {
     std::vector<int32_t> order, rounds;
     std::vector<float> keys;          // axis-test tables: (radius, ground skipped) per table
     std::vector<int32_t> link_key;    // table of each (group, link)
-    std::vector<int2> items;          // item table of the current run (kept until the call syncs)
-    int2* items_dev = nullptr;        // its device copy (room for max_items entries)
+    int2* items_host = nullptr;       // item table of the current run, in pinned staging
+    int2* items_dev = nullptr;        // its device copy (room for max_items entries each)
 };
 
 rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, int64_t num_edges, int32_t row_stride,
@@ -1854,7 +1855,18 @@ rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, i
         gr.max_dofs = std::max(gr.max_dofs, rb.num_dofs);
         gr.max_links = std::max(gr.max_links, rb.num_links);
         auto& m = members[g];
-        std::stable_sort(m.begin(), m.end(), [&](int32_t x, int32_t y) { return K[x] > K[y]; });
+        // longest first, stable: a counting sort when the lengths are few
+        int32_t kmx = 0;
+        for (int32_t e : m) kmx = std::max(kmx, K[e]);
+        if ((size_t)kmx <= 4 * m.size() + 1024) {
+            std::vector<int32_t> cnt((size_t)kmx + 2, 0), out(m.size());
+            for (int32_t e : m) ++cnt[(size_t)(kmx - K[e]) + 1];
+            for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
+            for (int32_t e : m) out[(size_t)cnt[(size_t)(kmx - K[e])]++] = e;
+            m.swap(out);
+        } else {
+            std::stable_sort(m.begin(), m.end(), [&](int32_t x, int32_t y) { return K[x] > K[y]; });
+        }
         gr.order_off[g] = (int32_t)pl.order.size();
         pl.order.insert(pl.order.end(), m.begin(), m.end());
         const int32_t kmax = m.empty() ? 0 : K[m[0]];
@@ -1893,21 +1905,25 @@ rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, i
 // the late states of any (the search skips the states past a found collision
 // only if that collision was found before it takes them).  Table entries are
 // (group, first slot).
-void item_table(const EdgePlan& pl, int spl, std::vector<int2>& t) {
-    struct Item { int32_t k, g, s0; };
-    std::vector<Item> it;
+size_t item_table(const EdgePlan& pl, int spl, int2* t) {
     const int32_t per = 64 * spl;
-    for (int g = 0; g < pl.gr.num; ++g) {
-        const int32_t* r = pl.rounds.data() + pl.gr.round_off[g];
-        int32_t k = 0;
-        for (int32_t s0 = 0; s0 < pl.gr.num_slots[g]; s0 += per) {
-            while (k + 1 <= pl.gr.num_rounds[g] && r[k + 1] <= s0) ++k;
-            it.push_back({k, g, s0});
+    int32_t cur[kMaxEdgeGroups] = {}, rk[kMaxEdgeGroups] = {}, kmax = 0;
+    for (int g = 0; g < pl.gr.num; ++g) kmax = std::max(kmax, pl.gr.num_rounds[g]);
+    size_t n = 0;
+    // round by round, groups in order, each group's items in slot order: the
+    // order of a stable sort of the group-by-group list by round
+    for (int32_t k = 0; k <= kmax; ++k) {
+        for (int g = 0; g < pl.gr.num; ++g) {
+            const int32_t* r = pl.rounds.data() + pl.gr.round_off[g];
+            while (cur[g] < pl.gr.num_slots[g]) {
+                while (rk[g] + 1 <= pl.gr.num_rounds[g] && r[rk[g] + 1] <= cur[g]) ++rk[g];
+                if (rk[g] != k) break;
+                t[n++] = make_int2(g, cur[g]);
+                cur[g] += per;
+            }
         }
     }
-    std::stable_sort(it.begin(), it.end(), [](const Item& x, const Item& y) { return x.k < y.k; });
-    t.resize(it.size());
-    for (size_t i = 0; i < it.size(); ++i) t[i] = make_int2(it[i].g, it[i].s0);
+    return n;
 }
 
 // Slots of a plan in 64-slot items (the largest item table of any spl).
@@ -1934,31 +1950,43 @@ struct PlanDev {
     int nkeys;
 };
 hipError_t upload_plan(rvs_context* c, const EnvRec& ev, EdgePlan& pl, Carve& cv, PlanDev& d) {
+    const size_t base = cv.off;
     d.order = cv.take<int32_t>(pl.order.size());
     d.rounds = cv.take<int32_t>(pl.rounds.size());
     d.counter = cv.take<uint32_t>(64);
     int32_t* lk = cv.take<int32_t>(pl.link_key.size());
     d.keys = cv.take<float>(pl.keys.size());
+    const size_t end = cv.off;
     d.nkeys = ev.dev.use_obb ? 0 : (int)pl.keys.size() / 2;
     pl.gr.slab = cv.take<SlabEntry>((size_t)d.nkeys * slab_entries(ev.dev.num_obj));
     pl.items_dev = cv.take<int2>(max_items(pl));
     pl.gr.link_key = lk;
-    hipError_t e = h2d(d.order, pl.order.data(), pl.order.size() * 4, c->stream);
-    if (e == hipSuccess) e = h2d(d.rounds, pl.rounds.data(), pl.rounds.size() * 4, c->stream);
-    if (e == hipSuccess) e = h2d(lk, pl.link_key.data(), pl.link_key.size() * 4, c->stream);
-    if (e == hipSuccess) e = h2d(d.keys, pl.keys.data(), pl.keys.size() * 4, c->stream);
-    return e;
+    // the tables through a pinned mirror of [base, end) in one copy; the item
+    // table of each run follows them in the staging buffer
+    const size_t tables = (end - base + 255) & ~size_t(255);
+    hipError_t e = c->staging.ensure(tables + max_items(pl) * sizeof(int2));
+    if (e != hipSuccess) return e;
+    char* h = static_cast<char*>(c->staging.p);
+    char* dbase = cv.base + base;
+    auto put = [&](const void* dst, const void* src, size_t bytes) {
+        if (bytes) std::memcpy(h + (static_cast<const char*>(dst) - dbase), src, bytes);
+    };
+    put(d.order, pl.order.data(), pl.order.size() * 4);
+    put(d.rounds, pl.rounds.data(), pl.rounds.size() * 4);
+    put(lk, pl.link_key.data(), pl.link_key.size() * 4);
+    put(d.keys, pl.keys.data(), pl.keys.size() * 4);
+    pl.items_host = reinterpret_cast<int2*>(h + tables);
+    return hipMemcpyAsync(dbase, h, end - base, hipMemcpyHostToDevice, c->stream);
 }
 
 // Runs a planned call on device arrays (q1, dir, first) with a launch shape.
 hipError_t run_edges(rvs_context* c, const EnvRec& ev, EdgePlan& pl, const float* q1, const float* dir,
                      int32_t row_stride, float step, int32_t* first, int64_t num_edges, const PlanDev& d,
                      int waves_per_cu, int spl, bool timing) {
-    item_table(pl, spl, pl.items);
-    const int32_t items = (int32_t)pl.items.size();
+    const int32_t items = (int32_t)item_table(pl, spl, pl.items_host);
     pl.gr.items = pl.items_dev;
     const QueryParams qp = params_of(c);
-    hipError_t e = h2d(pl.items_dev, pl.items.data(), pl.items.size() * sizeof(int2), c->stream);
+    hipError_t e = h2d(pl.items_dev, pl.items_host, (size_t)items * sizeof(int2), c->stream);
     if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
     if (e == hipSuccess)
         e = launch_edge_first(pl.gr, ev.dev, qp, q1, dir, row_stride, step, d.order, d.rounds, first, num_edges,

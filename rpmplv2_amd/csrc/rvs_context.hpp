@@ -37,6 +37,28 @@ struct DevBuf {
     }
 };
 
+// Pinned host staging for the per-call tables of the edge search (copied to
+// the device asynchronously; the call synchronizes before it returns).
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 1 << 20);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 struct RobotRec {
     RobotDev dev{};
     std::vector<void*> allocs;
@@ -70,6 +92,7 @@ struct rvs_context {
     std::vector<rvs::RobotRec> robots;
     std::vector<rvs::EnvRec> envs;
     rvs::DevBuf arena;
+    rvs::PinnedBuf staging;
     int64_t last_refined = 0;
     // Asynchronous calls (rvs_*_async) enqueue on a caller's stream and
     // return; async_ev is recorded after each of them (each first waits for
