@@ -456,6 +456,48 @@ def edges_section(eng, cores: int) -> dict:
     return out
 
 
+def edge_validity_section(eng, cores: int) -> dict:
+    """RealVectorSpace::isValid(q1, q2) for 65,536 short xArm6 edges in the
+    headline scene (10 interpolated states each, k = 10..1), device arrays,
+    checked edge by edge against the oracle, which is also the CPU baseline."""
+    import numpy as np
+    import torch
+    from rpmplv2_amd import workloads as W
+    O = oracle_module()
+    w = W.workload2(1 << 17)
+    n = 1 << 16
+    q1 = np.ascontiguousarray(w.q[:n])
+    q2 = (q1 + 0.05 * (w.q[n:2 * n] - q1)).astype(np.float32)
+    rid = eng.load_robot(w.robot)
+    eid = eng.create_env(w.scene)
+    dev = torch.device("cuda:0")
+    q1d, q2d = torch.from_numpy(q1).to(dev), torch.from_numpy(q2).to(dev)
+    out = torch.empty(n, dtype=torch.uint8, device=dev)
+    eng.edge_is_valid(rid, eid, q1d, q2d, out)
+    torch.cuda.synchronize()
+    ks = []
+    for _ in range(5):
+        eng.edge_is_valid(rid, eid, q1d, q2d, out, timing=True)
+        ks.append(eng.last_timing()[0])
+    t0 = time.perf_counter()
+    for _ in range(5):
+        eng.edge_is_valid(rid, eid, q1d, q2d, out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / 5
+    got = out.cpu().numpy()
+    sc = O.scene_of(w.scene)
+    t0 = time.perf_counter()
+    ov = O.edge_valid(w.robot, sc, q1, q2, nthreads=cores)
+    cpu_s = time.perf_counter() - t0
+    kms = float(np.min(ks))
+    return {"workload": "xarm6_16aabb_ground_65536_short_edges_10_checks", "edges": n, "states": n * 10,
+            "ms": ms, "kernel_ms": kms, "edges_per_s": n / (ms * 1e-3), "states_per_s": n * 10 / (ms * 1e-3),
+            "valid_fraction": float(got.mean()), "mismatch_vs_oracle": int((got != ov).sum()),
+            "kernel": "k_edge_valid_aabb",
+            "cpu_baseline": {"value": n / cpu_s, "unit": "edges/s", "cores": cores, "kind": "port",
+                             "sample": "the same 65,536 edges"}}
+
+
 def spines_section(eng, cores: int) -> dict:
     """Generalized bur: extendGenSpine from one root towards 65,536 targets."""
     import numpy as np
@@ -684,6 +726,8 @@ def main():
         out["workload5"] = workload5_section(eng, cores, args.w5_n)
         print("bench: edges", file=sys.stderr, flush=True)
         out["edges"] = edges_section(eng, cores)
+        print("bench: edge validity", file=sys.stderr, flush=True)
+        out["edge_validity"] = edge_validity_section(eng, cores)
         print("bench: spines", file=sys.stderr, flush=True)
         out["spines"] = spines_section(eng, cores)
         print("bench: trajectories, edge self-collision", file=sys.stderr, flush=True)

@@ -1776,6 +1776,10 @@ rvs_status rvs_edge_self_collision(rvs_context* c, int32_t robot, const float* q
     return RVS_OK;
 }
 
+namespace {
+void link_keys(const RobotRec& rb, std::vector<float>& keys, std::vector<int32_t>& link_key);
+}
+
 rvs_status rvs_edge_is_valid(rvs_context* c, int32_t robot, int32_t env, const float* q1, const float* q2,
                              int64_t n, uint8_t* valid, uint32_t flags) {
     rvs_status s = check_query(c, robot, env, q1, n, true);
@@ -1788,29 +1792,50 @@ rvs_status rvs_edge_is_valid(rvs_context* c, int32_t robot, int32_t env, const f
     if (n == 0) return RVS_OK;
     if (c->opt.num_checks <= 0) return fail(c, RVS_EINVAL, "rvs_edge_is_valid: num_checks must be positive");
     const int D = rb.num_dofs;
-    if (((staged_bytes<float>(ev.dev) + 15) & ~size_t(15)) + (size_t)D * 256 * 4 > device_lds_limit(c->opt.device))
+    if (ev.dev.use_obb &&
+        ((staged_bytes<float>(ev.dev) + 15) & ~size_t(15)) + (size_t)D * 256 * 4 > device_lds_limit(c->opt.device))
         return fail(c, RVS_ERANGE, "rvs_edge_is_valid: obstacle set too large for LDS staging");
     (void)hipSetDevice(c->opt.device);
     const QueryParams qp = params_of(c);
-    hipError_t e = hipSuccess;
+    // axis-test tables of the robot's links (axis-aligned scenes)
+    std::vector<float> keys;
+    std::vector<int32_t> lkey;
+    if (!ev.dev.use_obb) link_keys(rb, keys, lkey);
+    const int nkeys = (int)keys.size() / 2;
+    const size_t rows = (flags & RVS_F_DEVICE) ? 0 : (size_t)n * D;
+    hipError_t e = c->arena.ensure(Carve::need({lkey.size() * 4, keys.size() * 4,
+                                                (size_t)nkeys * slab_entries(ev.dev.num_obj) * sizeof(SlabEntry),
+                                                rows * 4, rows * 4, (flags & RVS_F_DEVICE) ? 0 : (size_t)n}));
+    if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_is_valid: staging");
+    Carve cv(c->arena.p);
+    int32_t* lkd = cv.take<int32_t>(lkey.size());
+    float* kd = cv.take<float>(keys.size());
+    SlabEntry* slab = cv.take<SlabEntry>((size_t)nkeys * slab_entries(ev.dev.num_obj));
+    e = h2d(lkd, lkey.data(), lkey.size() * 4, c->stream);
+    if (e == hipSuccess) e = h2d(kd, keys.data(), keys.size() * 4, c->stream);
     const float *a = q1, *b = q2;
     uint8_t* vd = valid;
-    if (!(flags & RVS_F_DEVICE)) {
-        e = c->arena.ensure(Carve::need({(size_t)n * D * 4, (size_t)n * D * 4, (size_t)n}));
-        if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_is_valid: staging");
-        Carve cv(c->arena.p);
-        float* as = cv.take<float>((size_t)n * D);
-        float* bs = cv.take<float>((size_t)n * D);
+    if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
+        float* as = cv.take<float>(rows);
+        float* bs = cv.take<float>(rows);
         vd = cv.take<uint8_t>(n);
-        e = h2d(as, q1, (size_t)n * D * 4, c->stream);
-        if (e == hipSuccess) e = h2d(bs, q2, (size_t)n * D * 4, c->stream);
+        e = h2d(as, q1, rows * 4, c->stream);
+        if (e == hipSuccess) e = h2d(bs, q2, rows * 4, c->stream);
         a = as;
         b = bs;
     }
-    if (e == hipSuccess) e = launch_edge_valid(rb.dev, ev.dev, qp, a, b, n, vd, c->stream);
+    const bool timing = (flags & RVS_F_TIMING) != 0;
+    if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
+    if (e == hipSuccess) e = launch_edge_valid(rb.dev, ev.dev, qp, a, b, n, vd, lkd, kd, nkeys, slab, c->stream);
+    if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(valid, vd, (size_t)n, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_edge_is_valid");
+    if (timing) {
+        c->nstages = 1;
+        c->stage_ms[0] = 0;
+        (void)hipEventElapsedTime(&c->stage_ms[0], c->ev[0], c->ev[1]);
+    }
     return RVS_OK;
 }
 
@@ -1827,6 +1852,23 @@ struct EdgePlan {
     int2* items_host = nullptr;       // item table of the current run, in pinned staging
     int2* items_dev = nullptr;        // its device copy (room for max_items entries each)
 };
+
+// Axis-test table of each link of a robot (appended to link_key): links with
+// the same radius and the same ground rule share one; new (radius, ground
+// skipped) keys are appended to keys.
+void link_keys(const RobotRec& rb, std::vector<float>& keys, std::vector<int32_t>& link_key) {
+    for (int i = 0; i < rb.num_links; ++i) {
+        const float r = rb.radius[i], skip = i < rb.dev.ground_included ? 1.0f : 0.0f;
+        int t = 0;
+        const int nk = (int)keys.size() / 2;
+        while (t < nk && !(keys[2 * t] == r && keys[2 * t + 1] == skip)) ++t;
+        if (t == nk) {
+            keys.push_back(r);
+            keys.push_back(skip);
+        }
+        link_key.push_back(t);
+    }
+}
 
 rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, int64_t num_edges, int32_t row_stride,
                       EdgePlan& pl) {
@@ -1882,20 +1924,8 @@ rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, i
         }
         if (acc > INT32_MAX / 8) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: too many states");
         gr.num_slots[g] = (int32_t)acc;
-        // axis-test table of each link: links with the same radius and the
-        // same ground rule share one
         gr.key_off[g] = (int32_t)pl.link_key.size();
-        for (int i = 0; i < rb.num_links; ++i) {
-            const float r = rb.radius[i], skip = i < rb.dev.ground_included ? 1.0f : 0.0f;
-            int t = 0;
-            const int nk = (int)pl.keys.size() / 2;
-            while (t < nk && !(pl.keys[2 * t] == r && pl.keys[2 * t + 1] == skip)) ++t;
-            if (t == nk) {
-                pl.keys.push_back(r);
-                pl.keys.push_back(skip);
-            }
-            pl.link_key.push_back(t);
-        }
+        link_keys(rb, pl.keys, pl.link_key);
     }
     return RVS_OK;
 }

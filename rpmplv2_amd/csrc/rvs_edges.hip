@@ -1,7 +1,8 @@
 // Edge checking kernels (gfx950).
 //
-// k_edge_valid: RealVectorSpace::isValid(q1, q2) (RealVectorSpace.cpp:
-// 261-275) for a batch of edges.  The reference walks k = K..1 and stops at
+// k_edge_valid (oriented boxes) and k_edge_valid_aabb:
+// RealVectorSpace::isValid(q1, q2) (RealVectorSpace.cpp:261-275) for a batch
+// of edges.  The reference walks k = K..1 and stops at
 // the first invalid state; the edge is valid exactly when all K states are,
 // so here each lane takes one (edge, k) state and a collision clears the
 // edge's flag (plain stores of 0; the flags start at 1).
@@ -465,22 +466,72 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
     }
 }
 
+// isValid(q1, q2) on axis-aligned scenes: k_edge_valid's lanes and
+// interpolation, the collision tree through the axis-test tables as in
+// k_edge_first_aabb (uniform loops per wave, joint values computed where the
+// kinematics reads them), no LDS.
+__global__ __launch_bounds__(256) void k_edge_valid_aabb(RobotDev rb, EnvDev env, QueryParams qp,
+                                                         const float* __restrict__ q1, const float* __restrict__ q2,
+                                                         int64_t n, uint8_t* __restrict__ valid,
+                                                         const int32_t* __restrict__ link_key,
+                                                         const SlabEntry* __restrict__ slab) {
+    const int K = qp.num_checks;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = g < n * K;
+    const int64_t e = live ? g / K : 0;
+    const int kk = live ? K - (int)(g % K) : K;   // k = K .. 1
+    const int D = rb.num_dofs;
+    const float* a = q1 + e * D;
+    const float* b = q2 + e * D;
+    const float dist = vecxf_norm_diff(a, b, D);
+    // RealVectorSpace::interpolateEdge (RealVectorSpace.cpp:75-88) with
+    // step = k / num_checks * dist
+    const float step = (float)kk / (float)K * dist;
+    const float f = dist > 0 ? step / dist : 0.0f;
+    auto sc = [&](int j, double* st, double* ct) {
+        const float qv = dist > 0 ? a[j] + (b[j] - a[j]) * f : b[j];
+        sincos((double)qv, st, ct);
+    };
+    const int M = env.num_obj, G = (M + kSlabGroup - 1) / kSlabGroup;
+    const Params<float> p{qp.eq, qp.maxd};
+    FKStream fk;
+    fk_init(fk);
+    Vec3<float> P0 = skel_point_sc<float>(fk, rb, 0, sc);
+    bool done = !live;
+    for (int i = 0; i < rb.num_links; ++i) {
+        if (!__any(!done)) break;
+        const Vec3<float> P1 = skel_point_sc<float>(fk, rb, i + 1, sc);
+        ConstPtr<SlabEntry> tab = as_const(slab) + (size_t)as_const(link_key)[i] * (G + M);
+        if (link_hits(P0, P1, rb.radius[i], rb.rnew32[i], tab, G, M, as_const(env.aabb), p, !done)) {
+            valid[e] = 0;
+            done = true;
+        }
+        P0 = P1;
+    }
+}
+
 inline dim3 blocks_for(int64_t n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
 
 }  // namespace
 
 hipError_t launch_edge_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q1,
-                             const float* q2, int64_t n, uint8_t* valid, hipStream_t st) {
+                             const float* q2, int64_t n, uint8_t* valid, const int32_t* link_key,
+                             const float* keys, int nkeys, SlabEntry* slab, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u8, blocks_for(n, 256), dim3(256), 0, st, valid, n, (uint8_t)1);
+    if (!env.use_obb) {
+        const int G = (env.num_obj + kSlabGroup - 1) / kSlabGroup;
+        if (nkeys > 0 && G > 0)
+            hipLaunchKernelGGL(k_slab_build, dim3((unsigned)((G + 63) / 64), (unsigned)nkeys), dim3(64), 0, st, env,
+                               keys, nkeys, slab);
+        hipLaunchKernelGGL(k_edge_valid_aabb, blocks_for(n * qp.num_checks, 256), dim3(256), 0, st, rb, env, qp, q1,
+                           q2, n, valid, link_key, slab);
+        return hipGetLastError();
+    }
     const int B = 256;
     const size_t sm = ((staged_bytes<float>(env) + 15) & ~size_t(15)) + (size_t)rb.num_dofs * B * 4;
-    if (env.use_obb)
-        hipLaunchKernelGGL(k_edge_valid<true>, blocks_for(n * qp.num_checks, B), dim3(B), sm, st, rb, env, qp, q1, q2,
-                           n, valid);
-    else
-        hipLaunchKernelGGL(k_edge_valid<false>, blocks_for(n * qp.num_checks, B), dim3(B), sm, st, rb, env, qp, q1, q2,
-                           n, valid);
+    hipLaunchKernelGGL(k_edge_valid<true>, blocks_for(n * qp.num_checks, B), dim3(B), sm, st, rb, env, qp, q1, q2, n,
+                       valid);
     return hipGetLastError();
 }
 

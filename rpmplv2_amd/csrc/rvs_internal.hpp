@@ -120,8 +120,11 @@ struct EdgeGroups {
     const SlabEntry* slab;
 };
 
+// Axis-aligned scenes use the axis-test tables: link i of the robot uses
+// table link_key[i] of slab (nkeys tables, filled by the launch from keys).
 hipError_t launch_edge_valid(const RobotDev& rb, const EnvDev& env, const QueryParams& qp, const float* q1,
-                             const float* q2, int64_t n, uint8_t* valid, hipStream_t st);
+                             const float* q2, int64_t n, uint8_t* valid, const int32_t* link_key,
+                             const float* keys, int nkeys, SlabEntry* slab, hipStream_t st);
 size_t edge_first_lds_bytes(const EdgeGroups& gr, const EnvDev& env);
 // keys: (radius, ground skipped as 0 / 1) of each axis-test table, nkeys of
 // them; the launch fills gr.slab from the scene before the search.

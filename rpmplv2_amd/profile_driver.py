@@ -11,8 +11,8 @@ layers (bur: 65,536 targets), trajectory safety (traj: 8,192 straight
 60-point trajectories), xArm6 edge self-collision (edgeself: 16,384
 edges) or the first collision along the planner-start edges of the four
 chains (edges: 4 x 4,096 edges in one persistent launch, starts judged by
-the engine's own reference-arithmetic edge search, shape 32 waves per CU x
-2 slots, the autotuner's choice).  --tuning records a launch shape for the FP32 bulk pass (as
+the engine's own reference-arithmetic edge search, shape 8 waves per CU x
+4 slots, the autotuner's choice).  --tuning records a launch shape for the FP32 bulk pass (as
 rvs_set_tuning does), so that a profile can use the shape an autotuned run
 chose.  Used as the profiled
 program in `rocprofv3 --kernel-trace --stats -- python3 -m ...` and
@@ -138,7 +138,7 @@ def _edges(args, torch, rvs, W, dev):
     ids, q1, u, K = eng.stack_edges(w.batches)
     q1d, ud = torch.from_numpy(q1).to(dev), torch.from_numpy(u).to(dev)
     first = torch.empty(len(ids), dtype=torch.int32, device=dev)
-    eng.set_edge_tuning(32, 2)
+    eng.set_edge_tuning(8, 4)
     for _ in range(args.calls):
         eng.edge_first_collision_device(eid, ids, q1d, ud, K, first)
     torch.cuda.synchronize()

@@ -599,12 +599,19 @@ class Engine:
                 "coherence_bits": t.coherence_bits}
 
     # ---- edges
-    def edge_is_valid(self, robot: int, env: int, q1, q2):
+    def edge_is_valid(self, robot: int, env: int, q1, q2, out=None, timing: bool = False):
+        """numpy rows (returns the flags), or torch device tensors with a
+        uint8 device tensor out (RVS_F_DEVICE)."""
         rb = self.robots[robot]
+        fl = F_TIMING if timing else 0
+        if _is_torch(q1):
+            self._check(lib().rvs_edge_is_valid(self._ctx, robot, env, _ptr(q1), _ptr(q2), q1.shape[0], _ptr(out),
+                                                fl | F_DEVICE), "rvs_edge_is_valid")
+            return out
         q1 = np.ascontiguousarray(q1, np.float32).reshape(-1, rb.num_dofs)
         q2 = np.ascontiguousarray(q2, np.float32).reshape(-1, rb.num_dofs)
         out = np.zeros(len(q1), np.uint8)
-        self._check(lib().rvs_edge_is_valid(self._ctx, robot, env, _ptr(q1), _ptr(q2), len(q1), _ptr(out), 0),
+        self._check(lib().rvs_edge_is_valid(self._ctx, robot, env, _ptr(q1), _ptr(q2), len(q1), _ptr(out), fl),
                     "rvs_edge_is_valid")
         return out
 
