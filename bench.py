@@ -456,6 +456,48 @@ def edges_section(eng, cores: int) -> dict:
     return out
 
 
+def is_valid_section(eng, cores: int) -> dict:
+    """RealVectorSpace::isValid(q) (the collision tree) on the headline
+    batch, device arrays, reference arithmetic and FP64, checked against the
+    oracle, which is also the CPU baseline."""
+    import numpy as np
+    import torch
+    from rpmplv2_amd import workloads as W
+    O = oracle_module()
+    w = W.workload2()
+    n = len(w.q)
+    rid = eng.load_robot(w.robot)
+    eid = eng.create_env(w.scene)
+    qd = torch.from_numpy(w.q).to(torch.device("cuda:0"))
+    out = torch.empty(n, dtype=torch.uint8, device=qd.device)
+    sc = O.scene_of(w.scene)
+    res = {"workload": "xarm6_16aabb_ground", "configs": n}
+    from rpmplv2_amd import rvs
+    for name, prec, bits in (("ref32", rvs.PREC_REF32, 32), ("fp64", rvs.PREC_FP64, 64)):
+        eng.is_valid(rid, eid, qd, prec, out)
+        torch.cuda.synchronize()
+        ks = []
+        for _ in range(5):
+            eng.is_valid(rid, eid, qd, prec, out, timing=True)
+            ks.append(eng.last_timing())
+        t0 = time.perf_counter()
+        for _ in range(5):
+            eng.is_valid(rid, eid, qd, prec, out)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / 5
+        got = out.cpu().numpy()
+        O.is_valid(w.robot, sc, w.q[:4096], bits, nthreads=cores)   # warm-up
+        t0 = time.perf_counter()
+        ov = O.is_valid(w.robot, sc, w.q, bits, nthreads=cores)
+        cpu_s = time.perf_counter() - t0
+        res[name] = {"configs_per_s": n / (ms * 1e-3), "ms": ms, "kernel_ms": float(np.min([k[0] for k in ks])),
+                     "walk_order_ms": float(np.min([k[1] for k in ks])), "kernel": "k_valid",
+                     "valid_fraction": float(got.mean()), "mismatch_vs_oracle": int((got != ov).sum()),
+                     "cpu_baseline": {"value": n / cpu_s, "unit": "configs/s", "cores": cores, "kind": "port",
+                                      "sample": "the same 1,048,576 configurations"}}
+    return res
+
+
 def edge_validity_section(eng, cores: int) -> dict:
     """RealVectorSpace::isValid(q1, q2) for 65,536 short xArm6 edges in the
     headline scene (10 interpolated states each, k = 10..1), device arrays,
@@ -486,6 +528,7 @@ def edge_validity_section(eng, cores: int) -> dict:
     ms = (time.perf_counter() - t0) * 1e3 / 5
     got = out.cpu().numpy()
     sc = O.scene_of(w.scene)
+    O.edge_valid(w.robot, sc, q1[:4096], q2[:4096], nthreads=cores)   # warm-up
     t0 = time.perf_counter()
     ov = O.edge_valid(w.robot, sc, q1, q2, nthreads=cores)
     cpu_s = time.perf_counter() - t0
@@ -728,6 +771,8 @@ def main():
         out["edges"] = edges_section(eng, cores)
         print("bench: edge validity", file=sys.stderr, flush=True)
         out["edge_validity"] = edge_validity_section(eng, cores)
+        print("bench: isValid", file=sys.stderr, flush=True)
+        out["is_valid"] = is_valid_section(eng, cores)
         print("bench: spines", file=sys.stderr, flush=True)
         out["spines"] = spines_section(eng, cores)
         print("bench: trajectories, edge self-collision", file=sys.stderr, flush=True)

@@ -1296,13 +1296,24 @@ rvs_status rvs_is_valid(rvs_context* c, int32_t robot, int32_t env, const float*
         coh = cv.take<uint32_t>(coh_need / 4);
     }
     const int32_t* perm = nullptr;
+    const bool timing = (flags & RVS_F_TIMING) != 0;
+    if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
     if (e == hipSuccess && coh)
         e = launch_coherence_order(rb.dev, qd, n, c->tuning_for(robot, env, rb.num_links, ev.dev.num_obj), coh, &perm,
                                    c->stream);
+    if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
     if (e == hipSuccess) e = launch_is_valid(rb.dev, ev.dev, qp, qd, n, vd, fp64, c->stream, perm);
+    if (e == hipSuccess && timing) e = hipEventRecord(c->ev[2], c->stream);
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) e = d2h(valid, vd, (size_t)n, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_is_valid");
+    if (timing) {
+        // stage 0: the collision tree; stage 1: the walk-order sort
+        c->nstages = 2;
+        c->stage_ms[0] = c->stage_ms[1] = 0;
+        (void)hipEventElapsedTime(&c->stage_ms[0], c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&c->stage_ms[1], c->ev[0], c->ev[1]);
+    }
     return RVS_OK;
 }
 

@@ -597,19 +597,26 @@ __host__ __device__ inline bool rect_collision(const Vec3<T>& A, const Vec3<T>& 
 
 // collisionCapsuleToBox, CollisionAndDistance.cpp:4-42.  r_new is the
 // inflated radius radius*sqrt(3)/3 evaluated in double and stored in T
-// (line 7); it depends only on the link, so callers pass it in.
+// (line 7); it depends only on the link, so callers pass it in.  The
+// reference tests "an end point inside the box grown by r_new" (true) before
+// "both end points beyond one face of the box grown by radius" (false); here
+// the second runs first, since it settles most pairs.  The order changes no
+// result for 0 <= r_new <= radius: a point below fl(lo - radius) is below
+// fl(lo - r_new) (rounding is monotone), so it is not inside, and likewise
+// above fl(hi + radius); the coordinates that decide the first test are
+// numbers, whatever the others hold.
 template <typename T>
 __host__ __device__ inline bool collision_box(const Vec3<T>& A, const Vec3<T>& B, T radius, T r_new, const Box<T>& b,
                                               const Params<T>& p) {
+    if ((A.x < b.lo[0] - radius && B.x < b.lo[0] - radius) || (A.x > b.hi[0] + radius && B.x > b.hi[0] + radius) ||
+        (A.y < b.lo[1] - radius && B.y < b.lo[1] - radius) || (A.y > b.hi[1] + radius && B.y > b.hi[1] + radius) ||
+        (A.z < b.lo[2] - radius && B.z < b.lo[2] - radius) || (A.z > b.hi[2] + radius && B.z > b.hi[2] + radius))
+        return false;
     if ((A.x > b.lo[0] - r_new && A.y > b.lo[1] - r_new && A.z > b.lo[2] - r_new &&
          A.x < b.hi[0] + r_new && A.y < b.hi[1] + r_new && A.z < b.hi[2] + r_new) ||
         (B.x > b.lo[0] - r_new && B.y > b.lo[1] - r_new && B.z > b.lo[2] - r_new &&
          B.x < b.hi[0] + r_new && B.y < b.hi[1] + r_new && B.z < b.hi[2] + r_new))
         return true;
-    if ((A.x < b.lo[0] - radius && B.x < b.lo[0] - radius) || (A.x > b.hi[0] + radius && B.x > b.hi[0] + radius) ||
-        (A.y < b.lo[1] - radius && B.y < b.lo[1] - radius) || (A.y > b.hi[1] + radius && B.y > b.hi[1] + radius) ||
-        (A.z < b.lo[2] - radius && B.z < b.lo[2] - radius) || (A.z > b.hi[2] + radius && B.z > b.hi[2] + radius))
-        return false;
     // one face per axis, chosen by A's position, tested in x, y, z order
 #pragma unroll
     for (int k = 0; k < 3; ++k) {

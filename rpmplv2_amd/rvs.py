@@ -360,11 +360,17 @@ class Engine:
                     "rvs_distance_f64")
         return d, prof, ver
 
-    def is_valid(self, robot: int, env: int, q, precision: Optional[int] = None):
+    def is_valid(self, robot: int, env: int, q, precision: Optional[int] = None, out=None, timing: bool = False):
+        """numpy rows (returns the flags), or a torch device tensor with a
+        uint8 device tensor out (RVS_F_DEVICE)."""
         rb = self.robots[robot]
+        fl = (F_PREC(precision) if precision is not None else 0) | (F_TIMING if timing else 0)
+        if _is_torch(q):
+            self._check(lib().rvs_is_valid(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(out), fl | F_DEVICE),
+                        "rvs_is_valid")
+            return out
         q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
         out = np.zeros(len(q), np.uint8)
-        fl = F_PREC(precision) if precision is not None else 0
         self._check(lib().rvs_is_valid(self._ctx, robot, env, _ptr(q), len(q), _ptr(out), fl), "rvs_is_valid")
         return out
 
