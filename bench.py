@@ -498,6 +498,53 @@ def is_valid_section(eng, cores: int) -> dict:
     return res
 
 
+def underestimation_section(eng, cores: int) -> dict:
+    """RealVectorSpace::computeDistanceUnderestimation on the headline batch
+    against one nearest-point set (the oracle's, of the first collision-free
+    configuration), device arrays, checked against the oracle, which is also
+    the CPU baseline."""
+    import numpy as np
+    import torch
+    from rpmplv2_amd import workloads as W
+    O = oracle_module()
+    w = W.workload2()
+    n, L = len(w.q), w.robot.num_links
+    sc = O.scene_of(w.scene)
+    _, _, over, onp = O.distance(w.robot, sc, w.q[:64], 32, want_np=True, nthreads=cores)
+    ref = np.ascontiguousarray(onp[int(np.argmax(over == 1))])
+    rid = eng.load_robot(w.robot)
+    eid = eng.create_env(w.scene)
+    dev = torch.device("cuda:0")
+    qd, refd = torch.from_numpy(w.q).to(dev), torch.from_numpy(ref).to(dev)
+    d = torch.empty(n, device=dev)
+    prof = torch.empty((n, L), device=dev)
+    st = torch.empty(n, dtype=torch.uint8, device=dev)
+    eng.underestimation_device(rid, eid, qd, refd, d, prof, st)
+    torch.cuda.synchronize()
+    ks = []
+    for _ in range(5):
+        eng.underestimation_device(rid, eid, qd, refd, d, prof, st, timing=True)
+        ks.append(eng.last_timing()[0])
+    t0 = time.perf_counter()
+    for _ in range(5):
+        eng.underestimation_device(rid, eid, qd, refd, d, prof, st)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / 5
+    gd, gst = d.cpu().numpy(), st.cpu().numpy()
+    O.underestimation(w.robot, sc, w.q[:4096], ref, nthreads=cores)   # warm-up
+    t0 = time.perf_counter()
+    ud, _, ust = O.underestimation(w.robot, sc, w.q, ref, nthreads=cores)
+    cpu_s = time.perf_counter() - t0
+    same = gst == ust
+    ia, ib = gd.view(np.int32).astype(np.int64), ud.view(np.int32).astype(np.int64)
+    ulp = int(np.abs(np.where(ia < 0, -(ia & 0x7FFFFFFF), ia) - np.where(ib < 0, -(ib & 0x7FFFFFFF), ib)).max())
+    return {"workload": "xarm6_16aabb_ground", "configs": n, "ms": ms, "kernel_ms": float(np.min(ks)),
+            "configs_per_s": n / (ms * 1e-3), "kernel": "k_under", "stored_fraction": float(gst.mean()),
+            "stored_mismatch_vs_oracle": int((~same).sum()), "max_ulp_d_vs_oracle": ulp,
+            "cpu_baseline": {"value": n / cpu_s, "unit": "configs/s", "cores": cores, "kind": "port",
+                             "sample": "the same 1,048,576 configurations"}}
+
+
 def edge_validity_section(eng, cores: int) -> dict:
     """RealVectorSpace::isValid(q1, q2) for 65,536 short xArm6 edges in the
     headline scene (10 interpolated states each, k = 10..1), device arrays,
@@ -773,6 +820,8 @@ def main():
         out["edge_validity"] = edge_validity_section(eng, cores)
         print("bench: isValid", file=sys.stderr, flush=True)
         out["is_valid"] = is_valid_section(eng, cores)
+        print("bench: underestimation", file=sys.stderr, flush=True)
+        out["underestimation"] = underestimation_section(eng, cores)
         print("bench: spines", file=sys.stderr, flush=True)
         out["spines"] = spines_section(eng, cores)
         print("bench: trajectories, edge self-collision", file=sys.stderr, flush=True)

@@ -191,7 +191,9 @@ rvs_status rvs_distance_f64(rvs_context* ctx, int32_t robot, int32_t env, const 
                             double* profile, uint8_t* verdict, uint32_t flags);
 
 /* RealVectorSpace::isValid(q) (RealVectorSpace.cpp:277-309): collision tree,
- * environment only.  valid[k] = 1 when configuration k is collision-free. */
+ * environment only.  valid[k] = 1 when configuration k is collision-free.
+ * RVS_F_TIMING records the collision tree's kernel time as stage 0 and the
+ * walk-order sort's as stage 1 of rvs_last_timing. */
 rvs_status rvs_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n, uint8_t* valid,
                         uint32_t flags);
 
@@ -238,7 +240,8 @@ rvs_status rvs_self_collision_async(rvs_context* ctx, int32_t robot, const float
  * (RealVectorSpace.cpp:402-446) on fresh states.  nearest_ref is [M][L][6]
  * shared by all configurations (ref_stride = 0) or one set per configuration
  * (ref_stride = M*L*6).  stored[k] = 0 where the reference returns 0 without
- * storing a profile (a negative plane distance); profile rows are then 0. */
+ * storing a profile (a negative plane distance); profile rows are then 0.
+ * RVS_F_TIMING records the kernel time as stage 0 of rvs_last_timing. */
 rvs_status rvs_distance_underestimation(rvs_context* ctx, int32_t robot, int32_t env, const float* q, int64_t n,
                                         const float* nearest_ref, int64_t ref_stride, float* d, float* profile,
                                         uint8_t* stored, uint32_t flags);
@@ -350,7 +353,8 @@ rvs_status rvs_extend_gen_spines(rvs_context* ctx, int32_t robot, int32_t env, c
                                  uint32_t flags);
 
 /* RealVectorSpace::isValid(q1, q2) (RealVectorSpace.cpp:261-275): num_checks
- * interpolated states from the far end, q1 not checked. */
+ * interpolated states from the far end, q1 not checked.  RVS_F_TIMING
+ * records the kernel time as stage 0 of rvs_last_timing. */
 rvs_status rvs_edge_is_valid(rvs_context* ctx, int32_t robot, int32_t env, const float* q1, const float* q2,
                              int64_t n, uint8_t* valid, uint32_t flags);
 

@@ -1353,7 +1353,10 @@ rvs_status rvs_distance_underestimation(rvs_context* c, int32_t robot, int32_t e
         qd = qs;
         npd = ns;
     }
+    const bool timing = (flags & RVS_F_TIMING) != 0;
+    if (e == hipSuccess && timing) e = hipEventRecord(c->ev[0], c->stream);
     if (e == hipSuccess) e = launch_underestimation(rb.dev, ev.dev, qp, qd, n, npd, ref_stride, dd, pd, sd, c->stream);
+    if (e == hipSuccess && timing) e = hipEventRecord(c->ev[1], c->stream);
     if (e == hipSuccess && !(flags & RVS_F_DEVICE)) {
         e = d2h(d, dd, (size_t)n * 4, c->stream);
         if (e == hipSuccess) e = d2h(profile, pd, (size_t)n * L * 4, c->stream);
@@ -1361,6 +1364,11 @@ rvs_status rvs_distance_underestimation(rvs_context* c, int32_t robot, int32_t e
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "rvs_distance_underestimation");
+    if (timing) {
+        c->nstages = 1;
+        c->stage_ms[0] = 0;
+        (void)hipEventElapsedTime(&c->stage_ms[0], c->ev[0], c->ev[1]);
+    }
     return RVS_OK;
 }
 

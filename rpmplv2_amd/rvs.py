@@ -374,6 +374,16 @@ class Engine:
         self._check(lib().rvs_is_valid(self._ctx, robot, env, _ptr(q), len(q), _ptr(out), fl), "rvs_is_valid")
         return out
 
+    def underestimation_device(self, robot: int, env: int, q, nearest_ref, d, profile, stored, timing: bool = False):
+        """Device tensors throughout (RVS_F_DEVICE); nearest_ref [M][L][6]
+        shared by all configurations or [n][M][L][6]."""
+        rb = self.robots[robot]
+        stride = 0 if nearest_ref.dim() == 3 else self.envs[env].num_obj * rb.num_links * 6
+        fl = F_DEVICE | (F_TIMING if timing else 0)
+        self._check(lib().rvs_distance_underestimation(self._ctx, robot, env, _ptr(q), q.shape[0], _ptr(nearest_ref),
+                                                       stride, _ptr(d), _ptr(profile), _ptr(stored), fl),
+                    "rvs_distance_underestimation")
+
     def underestimation(self, robot: int, env: int, q, nearest_ref):
         rb = self.robots[robot]
         q = np.ascontiguousarray(q, np.float32).reshape(-1, rb.num_dofs)
