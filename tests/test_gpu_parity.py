@@ -414,6 +414,34 @@ def test_edge_first_collision_folded_chains(oracle, engine):
         assert np.array_equal(g, e), (b.robot.num_dofs, int((g != e).sum()))
 
 
+def test_edges_many_obstacles(oracle, engine):
+    """1,000 axis-aligned boxes (125 groups of 8 in the axis-test tables, the
+    boxes of the full test read from global memory): first collisions of
+    chain edges and isValid(q1, q2) of xArm6 edges equal the oracle's."""
+    rng = np.random.default_rng(77)
+    sc = W._boxes(rng, 1000, 0.015, 0.03, 3.0, (0.0, 0.0, 0.5))
+    eid = engine.create_env(sc)
+    osc = oracle.scene_of(sc)
+    rb = R.chain(rng.uniform(0.08, 0.15, 10), radius=0.02, self_collision=True)
+    m = 2000
+    q1 = rng.uniform(-1.0, 1.0, (m, 10)).astype(np.float32)
+    u = rng.normal(size=(m, 10))
+    u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+    K = rng.integers(1, 60, m).astype(np.int32)
+    b = W.EdgeBatch(rb, q1, u, K)
+    of = oracle.edge_first_collision(rb, osc, q1, u, K, 0.05)
+    assert (of == 0).mean() < 0.5 and (of > 0).sum() > 200 and (of < 0).sum() > 200
+    f = engine.edge_batches(eid, [b])[0]
+    assert np.array_equal(f, of), int((f != of).sum())
+    xa = R.xarm6()
+    rid = engine.load_robot(xa)
+    w = W.workload2(8192)
+    qa, qb = w.q[:4096], (w.q[:4096] + 0.05 * (w.q[4096:] - w.q[:4096])).astype(np.float32)
+    v = engine.edge_is_valid(rid, eid, qa, qb)
+    ov = oracle.edge_valid(xa, osc, qa, qb)
+    assert 0 < v.mean() < 1 and np.array_equal(v, ov), int((v != ov).sum())
+
+
 def test_edge_first_collision_sparse_scene(oracle, engine):
     # few, small obstacles so many edges run to their last state
     rng = np.random.default_rng(11)
