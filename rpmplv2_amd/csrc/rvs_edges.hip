@@ -10,12 +10,13 @@
 // k_edge_first: discretised edges of several robots in ONE persistent launch.
 // The states of each robot's edges are laid out round-major (state 0 of every
 // edge, then state 1 of every edge that has one, ...) after sorting the edges
-// by length, so consecutive slots share a robot and a state index and early
-// states are visited first.  Each wave repeatedly takes the next 64-slot item
-// from a global counter until the counter passes the last item, skips states
-// that lie beyond an already found collision of their edge, and records a
-// collision with atomicMin on the edge's first index.  Robot ranges are padded
-// to whole items, so the robot is uniform across a wave.
+// by length, so consecutive slots share a robot and a state index.  Items of
+// 64 x spl slots of one robot are handed out from a global counter in the
+// order of their first slot's state index across the robots, so that early
+// states of every robot come first; each wave takes items until the counter
+// passes the last one, skips states that lie beyond an already found
+// collision of their edge, and records a collision with atomicMin on the
+// edge's first index.  The robot is uniform across a wave.
 //
 // On axis-aligned scenes (k_edge_first_aabb) each link meets the obstacles
 // through an axis-test table first (SlabEntry): collisionCapsuleToBox
