@@ -435,7 +435,13 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
             if (link_hits(P0, P1, rb.radius[i], rb.rnew32[i], tab, G, M, as_const(env.aabb), p, !done)) {
                 bad = true;
                 done = true;
+                atomicMin(&first[e], k);   // at once: the edge's later states in flight stop at their next link
             }
+            // a collision found meanwhile at an earlier state of this edge (by
+            // another wave) ends this state's test: its verdict no longer
+            // matters.  The first colliding state is never cut short, since
+            // only a colliding state before it could cut it.
+            if (!done && __atomic_load_n(&first[e], __ATOMIC_RELAXED) <= k) done = true;
             P0 = P1;
         }
         const bool bound_ok = csum == csum;
