@@ -85,10 +85,10 @@ LINKS_TRAFFIC_PER_CONFIG = (2.0 * 1733610.1 + 1900112.3) * 1024 / 2097152
 # edges each, shape 8 waves per CU x 4 slots), per launch, from rocprofv3
 # counters (profiles/k_edge_first_chains_planner_starts.md): VALU wave
 # instructions, FP32 and FP64 operations executed (SQ_INSTS_VALU_FLOPS_FP32 /
-# _FP64 x 23.59 lanes active per VALU instruction) and HBM bytes (FETCH_SIZE
-# 35,387 KiB x 2.0 + WRITE_SIZE 4,743.3 KiB).
-EDGE_COUNTERS = {"valu_instr": 3.4298e8, "fp32_flop": 1.3266e8 * 23.59, "fp64_flop": 3.3691e7 * 23.59,
-                 "traffic_bytes": (2.0 * 35387.0 + 4743.3) * 1024,
+# _FP64 x 26.14 lanes active per VALU instruction) and HBM bytes (FETCH_SIZE
+# 38,299 KiB x 2.0 + WRITE_SIZE 2,536.2 KiB).
+EDGE_COUNTERS = {"valu_instr": 2.7828e8, "fp32_flop": 1.0559e8 * 26.14, "fp64_flop": 3.0127e7 * 26.14,
+                 "traffic_bytes": (2.0 * 38299.0 + 2536.2) * 1024,
                  "source": "profiles/k_edge_first_chains_planner_starts.md"}
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per launch of the FP32 bulk kernel on the headline workload from

@@ -466,10 +466,11 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
                 if (sd - ra - rbb < 0) {
                     bad = true;
                     done = true;
+                    atomicMin(&first[e], k);
                 }
             }
+            if ((m & 15) == 15 && !done && __atomic_load_n(&first[e], __ATOMIC_RELAXED) <= k) done = true;
         }
-        if (live && bad) atomicMin(&first[e], k);
     }
 }
 
