@@ -202,16 +202,31 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_distance_refine(RobotDev 
         int k0 = 0;   // the first link to recompute
         flag_decode(idx, k, qp, out, c, k0);
         const float* qc = q + c * rb.num_dofs;
+        // joint sines and cosines: with at most 64 joints lane m computes
+        // joint m's (the same function of the same value as the serial
+        // stream) and the kinematics take them by shuffles, so the chain of
+        // frame products no longer waits on each joint's sincos in turn
+        const bool par = rb.num_dofs <= 64;
+        double my_s = 0.0, my_c = 1.0;
+        if (par && t < rb.num_dofs) sincos((double)qc[t], &my_s, &my_c);
+        const auto sc = [&](int m, double* st, double* ct) {
+            if (par) {
+                *st = __shfl(my_s, m, 64);
+                *ct = __shfl(my_c, m, 64);
+            } else {
+                sincos((double)qc[m], st, ct);
+            }
+        };
         FKStream fk;
         fk_init(fk);
-        Vec3<float> P0 = skel_point<float>(fk, rb, qc, 0);
+        Vec3<float> P0 = skel_point_sc<float>(fk, rb, 0, sc);
         // the links before k0 keep the bulk pass's values (no hit)
         float d = p.maxd;
         for (int i2 = 0; i2 < k0; ++i2) d = smin(d, out.profile[c * L + i2]);
         bool hit = false;
         int i = 0;
         for (; i < L; ++i) {
-            const Vec3<float> P1 = skel_point<float>(fk, rb, qc, i + 1);
+            const Vec3<float> P1 = skel_point_sc<float>(fk, rb, i + 1, sc);
             if (i < k0) {
                 P0 = P1;
                 continue;
