@@ -1892,21 +1892,26 @@ void link_keys(const RobotRec& rb, std::vector<float>& keys, std::vector<int32_t
 rvs_status plan_edges(rvs_context* c, const int32_t* robots, const int32_t* K, int64_t num_edges, int32_t row_stride,
                       EdgePlan& pl) {
     std::vector<int32_t> gid;     // robot ids in order of first appearance
-    std::vector<std::vector<int32_t>> members;
+    // group of each loaded robot (-1: none yet), and the group sizes: one
+    // pass to validate and count, one to fill the member lists in place
+    std::vector<int32_t> group_of(c->robots.size(), -1);
+    std::vector<int64_t> size;
     for (int64_t e = 0; e < num_edges; ++e) {
         const int32_t r = robots[e];
         if (!robot_ok(c, r)) return fail(c, RVS_EINVAL, "rvs_edge_first_collision: bad robot id");
         if (K[e] < 0 || K[e] > (1 << 20)) return fail(c, RVS_ERANGE, "rvs_edge_first_collision: state count");
-        auto it = std::find(gid.begin(), gid.end(), r);
-        if (it == gid.end()) {
+        if (group_of[r] < 0) {
             if ((int)gid.size() == kMaxEdgeGroups)
                 return fail(c, RVS_ERANGE, "rvs_edge_first_collision: at most 8 robots per call");
+            group_of[r] = (int32_t)gid.size();
             gid.push_back(r);
-            members.emplace_back();
-            it = gid.end() - 1;
+            size.push_back(0);
         }
-        members[it - gid.begin()].push_back((int32_t)e);
+        ++size[group_of[r]];
     }
+    std::vector<std::vector<int32_t>> members(gid.size());
+    for (size_t g = 0; g < gid.size(); ++g) members[g].reserve((size_t)size[g]);
+    for (int64_t e = 0; e < num_edges; ++e) members[group_of[robots[e]]].push_back((int32_t)e);
     EdgeGroups& gr = pl.gr;
     gr.num = (int32_t)gid.size();
     for (int g = 0; g < gr.num; ++g) {
