@@ -184,6 +184,23 @@ __device__ inline int round_of(const int32_t* off, int nrounds, int32_t s) {
     return lo;
 }
 
+// Self pair (A, B) x (C, D) of radii ra, rbb: true when the capsules
+// collide (distanceLineSegToLineSeg - r_a - r_b < 0), the bounding-box bound
+// first (the file comment gives the argument); cmax is the largest coordinate
+// magnitude of the skeleton and bound_ok false when one of its coordinates is
+// NaN.
+__device__ inline bool self_pair_hits(const Vec3<float>& A, const Vec3<float>& B, const Vec3<float>& C,
+                                      const Vec3<float>& D, float ra, float rbb, float cmax, bool bound_ok,
+                                      const Params<float>& p) {
+    const float gx = fmaxf(fmaxf(fminf(C.x, D.x) - fmaxf(A.x, B.x), fminf(A.x, B.x) - fmaxf(C.x, D.x)), 0.0f);
+    const float gy = fmaxf(fmaxf(fminf(C.y, D.y) - fmaxf(A.y, B.y), fminf(A.y, B.y) - fmaxf(C.y, D.y)), 0.0f);
+    const float gz = fmaxf(fmaxf(fminf(C.z, D.z) - fmaxf(A.z, B.z), fminf(A.z, B.z) - fmaxf(C.z, D.z)), 0.0f);
+    const float R = fmaxf(ra + rbb, p.eq);
+    const float thr = R + 4e-6f * (cmax + R);
+    if (bound_ok && thr <= p.maxd && (gx * gx + gy * gy) + gz * gz > thr * thr) return false;
+    return seg_seg<float, false>(A, B, C, D, p, nullptr) - ra - rbb < 0;
+}
+
 template <bool OBB>
 __global__ __launch_bounds__(64) void k_edge_first(EdgeGroups gr, EnvDev env, QueryParams qp,
                                                    const float* __restrict__ q1, const float* __restrict__ dir,
@@ -218,14 +235,22 @@ __global__ __launch_bounds__(64) void k_edge_first(EdgeGroups gr, EnvDev env, Qu
             for (int i = 0; i < D; ++i)
                 qbuf[i * 64 + lane] = q1[(int64_t)e * row_stride + i] + sk * dir[(int64_t)e * row_stride + i];
             bool bad = !env_free<OBB>(rb, s, env.num_obj, p, qbuf + lane, 64, rb.num_pairs ? skel + lane : nullptr);
-            for (int m = 0; !bad && m < rb.num_pairs; ++m) {
-                const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
+            if (!bad && rb.num_pairs > 0) {
                 auto pt = [&](int kk) {
                     return Vec3<float>{skel[(3 * kk) * 64 + lane], skel[(3 * kk + 1) * 64 + lane],
                                        skel[(3 * kk + 2) * 64 + lane]};
                 };
-                const float sd = seg_seg<float, false>(pt(a), pt(a + 1), pt(b), pt(b + 1), p, nullptr);
-                if (sd - rb.radius[a] - rb.radius[b] < 0) bad = true;
+                float cmax = 0.0f, csum = 0.0f;
+                for (int kk = 0; kk <= rb.num_links; ++kk) {
+                    const Vec3<float> P = pt(kk);
+                    cmax = fmaxf(cmax, fmaxf(fmaxf(fabsf(P.x), fabsf(P.y)), fabsf(P.z)));
+                    csum = csum + ((P.x + P.y) + P.z);
+                }
+                for (int m = 0; !bad && m < rb.num_pairs; ++m) {
+                    const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
+                    bad = self_pair_hits(pt(a), pt(a + 1), pt(b), pt(b + 1), rb.radius[a], rb.radius[b], cmax,
+                                         csum == csum, p);
+                }
             }
             if (bad) atomicMin(&first[e], k);
         }
@@ -454,7 +479,9 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
                                    skel[(3 * kk + 2) * 64 + lane]};
             };
             const Vec3<float> A = pt(a), B = pt(a + 1), C = pt(b), D = pt(b + 1);
-            // distance of the two bounding boxes, squared
+            // distance of the two bounding boxes, squared (self_pair_hits's
+            // bound, written out here: the whole wave tests it, and only the
+            // lanes that it does not settle take the full test)
             const float gx = fmaxf(fmaxf(fminf(C.x, D.x) - fmaxf(A.x, B.x), fminf(A.x, B.x) - fmaxf(C.x, D.x)), 0.0f);
             const float gy = fmaxf(fmaxf(fminf(C.y, D.y) - fmaxf(A.y, B.y), fminf(A.y, B.y) - fmaxf(C.y, D.y)), 0.0f);
             const float gz = fmaxf(fmaxf(fminf(C.z, D.z) - fmaxf(A.z, B.z), fminf(A.z, B.z) - fmaxf(C.z, D.z)), 0.0f);
