@@ -7,7 +7,7 @@
 // so here each lane takes one (edge, k) state and a collision clears the
 // edge's flag (plain stores of 0; the flags start at 1).
 //
-// k_edge_first: discretised edges of several robots in ONE persistent launch.
+// k_edge_first_q: discretised edges of several robots in ONE persistent launch.
 // The states of each robot's edges are laid out round-major (state 0 of every
 // edge, then state 1 of every edge that has one, ...) after sorting the edges
 // by length, so consecutive slots share a robot and a state index.  Items of
@@ -18,7 +18,7 @@
 // collision of their edge, and records a collision with atomicMin on the
 // edge's first index.  The robot is uniform across a wave.
 //
-// On axis-aligned scenes (k_edge_first_aabb) each link meets the obstacles
+// On axis-aligned scenes (k_edge_first_q<false>) each link meets the obstacles
 // through an axis-test table first (SlabEntry): collisionCapsuleToBox
 // (CollisionAndDistance.cpp:4-42) rejects a capsule whose two end points lie
 // beyond one face of the box grown by the radius r on the same side (its
@@ -184,79 +184,6 @@ __device__ inline int round_of(const int32_t* off, int nrounds, int32_t s) {
     return lo;
 }
 
-// Self pair (A, B) x (C, D) of radii ra, rbb: true when the capsules
-// collide (distanceLineSegToLineSeg - r_a - r_b < 0), the bounding-box bound
-// first (the file comment gives the argument); cmax is the largest coordinate
-// magnitude of the skeleton and bound_ok false when one of its coordinates is
-// NaN.
-__device__ inline bool self_pair_hits(const Vec3<float>& A, const Vec3<float>& B, const Vec3<float>& C,
-                                      const Vec3<float>& D, float ra, float rbb, float cmax, bool bound_ok,
-                                      const Params<float>& p) {
-    const float gx = fmaxf(fmaxf(fminf(C.x, D.x) - fmaxf(A.x, B.x), fminf(A.x, B.x) - fmaxf(C.x, D.x)), 0.0f);
-    const float gy = fmaxf(fmaxf(fminf(C.y, D.y) - fmaxf(A.y, B.y), fminf(A.y, B.y) - fmaxf(C.y, D.y)), 0.0f);
-    const float gz = fmaxf(fmaxf(fminf(C.z, D.z) - fmaxf(A.z, B.z), fminf(A.z, B.z) - fmaxf(C.z, D.z)), 0.0f);
-    const float R = fmaxf(ra + rbb, p.eq);
-    const float thr = R + 4e-6f * (cmax + R);
-    if (bound_ok && thr <= p.maxd && (gx * gx + gy * gy) + gz * gz > thr * thr) return false;
-    return seg_seg<float, false>(A, B, C, D, p, nullptr) - ra - rbb < 0;
-}
-
-template <bool OBB>
-__global__ __launch_bounds__(64) void k_edge_first(EdgeGroups gr, EnvDev env, QueryParams qp,
-                                                   const float* __restrict__ q1, const float* __restrict__ dir,
-                                                   int32_t row_stride, float step, const int32_t* __restrict__ order,
-                                                   const int32_t* __restrict__ rounds, int32_t* first,
-                                                   uint32_t* counter, int32_t total_items, int spl) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<float> s = stage_env<float, OBB>(env, smem);
-    const size_t base = (staged_bytes<float>(env) + 15) & ~size_t(15);
-    float* qbuf = reinterpret_cast<float*>(smem + base);
-    float* skel = qbuf + gr.max_dofs * 64;
-    const int lane = threadIdx.x;
-    const Params<float> p{qp.eq, qp.maxd};
-    for (;;) {
-        int item = 0;
-        if (lane == 0) item = (int)atomicAdd(counter, 1u);
-        item = __shfl(item, 0, 64);
-        if (item >= total_items) break;
-        // robot group and first slot of this item (uniform across the wave)
-        const int2 it = gr.items[item];
-        const int g = it.x;
-        const RobotDev& rb = gr.rb[g];
-        // an item holds 64 * spl slots: lane takes slots lane, lane + 64, ...
-        for (int sub = 0; sub < spl; ++sub) {
-            const int32_t slot = it.y + sub * 64 + lane;
-            if (slot >= gr.num_slots[g]) break;
-            const int k = round_of(rounds + gr.round_off[g], gr.num_rounds[g] + 1, slot);
-            const int32_t e = order[gr.order_off[g] + (slot - rounds[gr.round_off[g] + k])];
-            if (__atomic_load_n(&first[e], __ATOMIC_RELAXED) <= k) continue;
-            const int D = rb.num_dofs;
-            const float sk = step * (float)k;
-            for (int i = 0; i < D; ++i)
-                qbuf[i * 64 + lane] = q1[(int64_t)e * row_stride + i] + sk * dir[(int64_t)e * row_stride + i];
-            bool bad = !env_free<OBB>(rb, s, env.num_obj, p, qbuf + lane, 64, rb.num_pairs ? skel + lane : nullptr);
-            if (!bad && rb.num_pairs > 0) {
-                auto pt = [&](int kk) {
-                    return Vec3<float>{skel[(3 * kk) * 64 + lane], skel[(3 * kk + 1) * 64 + lane],
-                                       skel[(3 * kk + 2) * 64 + lane]};
-                };
-                float cmax = 0.0f, csum = 0.0f;
-                for (int kk = 0; kk <= rb.num_links; ++kk) {
-                    const Vec3<float> P = pt(kk);
-                    cmax = fmaxf(cmax, fmaxf(fmaxf(fabsf(P.x), fabsf(P.y)), fabsf(P.z)));
-                    csum = csum + ((P.x + P.y) + P.z);
-                }
-                for (int m = 0; !bad && m < rb.num_pairs; ++m) {
-                    const int a = rb.pairs[2 * m], b = rb.pairs[2 * m + 1];
-                    bad = self_pair_hits(pt(a), pt(a + 1), pt(b), pt(b + 1), rb.radius[a], rb.radius[b], cmax,
-                                         csum == csum, p);
-                }
-            }
-            if (bad) atomicMin(&first[e], k);
-        }
-    }
-}
-
 // Axis-test tables from the scene's current boxes: block (x, y) fills
 // groups of table y (keys[2y] = radius, keys[2y + 1] = 1 when the links of
 // this key skip ground boxes).
@@ -344,17 +271,40 @@ __device__ inline bool link_hits(const Vec3<float>& P0, const Vec3<float>& P1, f
     return hit;
 }
 
-// The first-collision search on axis-aligned scenes: k_edge_first's items
-// and slots, with two changes.  The states that are still needed (no
-// collision found yet at an earlier state of their edge) are gathered into a
-// queue in LDS, 64 per evaluation, so that slots of finished edges take no
-// lanes; and the state test of each lane is written as uniform loops (a lane
-// that is done, or has no state, rides along masked off) so that the tables
-// arrive through scalar loads.  The joint values come from the edge's rows
-// (q = q1 + (k step) dir per joint, rounded as the reference's interpolation
-// is); LDS holds the queue and the skeletons for the self pairs.
+// One link (P0, P1) of radius r against every oriented box, for the lanes
+// with act set: true where some box collides (collisionCapsuleToBox in the
+// box's frame, as k_edge_first's OBB form evaluates it).  The boxes are read
+// with wave-uniform scalar loads (centre and rotation rows, then the half
+// extents); links below ground_included skip ground boxes.
+__device__ inline bool link_hits_obb(const Vec3<float>& P0, const Vec3<float>& P1, float r, float rn,
+                                     ConstPtr<float> obb, ConstPtr<uint8_t> flags, int M, bool skip_ground,
+                                     const Params<float>& p, bool act) {
+    bool hit = false;
+    for (int j = 0; j < M; ++j) {
+        if ((j & 7) == 0 && !__any(act && !hit)) break;
+        ConstPtr<float> o = obb + 15 * j;
+        float x[12];
+        for (int k = 0; k < 12; ++k) x[k] = o[k];
+        const Box<float> bx{{-o[12], -o[13], -o[14]}, {o[12], o[13], o[14]}};
+        const bool skip = skip_ground && (flags[j] & kObjGround);
+        if (act && !hit && !skip && collision_box(to_box(x, P0), to_box(x, P1), r, rn, bx, p)) hit = true;
+    }
+    return hit;
+}
+
+// The first-collision search over the items and slots above.  The states
+// that are still needed (no collision found yet at an earlier state of their
+// edge) are gathered into a queue in LDS, 64 per evaluation, so that slots of
+// finished edges take no lanes; the state test of each lane is written as
+// uniform loops (a lane that is done, or has no state, rides along masked
+// off) so that the tables, or the oriented boxes (OBB: every box through the
+// full test in its frame), arrive through scalar loads.  The joint values
+// come from the edge's rows (q = q1 + (k step) dir per joint, rounded as the
+// reference's interpolation is); LDS holds the queue and the skeletons for
+// the self pairs.
 constexpr int kQueue = 128;
-__global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev env, QueryParams qp,
+template <bool OBB>
+__global__ __launch_bounds__(64) void k_edge_first_q(EdgeGroups gr, EnvDev env, QueryParams qp,
                                                         const float* __restrict__ q1, const float* __restrict__ dir,
                                                         int32_t row_stride, float step,
                                                         const int32_t* __restrict__ order,
@@ -456,8 +406,15 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
                 cmax = fmaxf(cmax, fmaxf(fmaxf(fabsf(P1.x), fabsf(P1.y)), fabsf(P1.z)));
                 csum = csum + ((P1.x + P1.y) + P1.z);
             }
-            ConstPtr<SlabEntry> tab = as_const(gr.slab) + (size_t)keys[i] * (G + M);
-            if (link_hits(P0, P1, rb.radius[i], rb.rnew32[i], tab, G, M, as_const(env.aabb), p, !done)) {
+            bool h;
+            if (OBB) {
+                h = link_hits_obb(P0, P1, rb.radius[i], rb.rnew32[i], as_const(env.obb), as_const(env.flags), M,
+                                  i < rb.ground_included, p, !done);
+            } else {
+                ConstPtr<SlabEntry> tab = as_const(gr.slab) + (size_t)keys[i] * (G + M);
+                h = link_hits(P0, P1, rb.radius[i], rb.rnew32[i], tab, G, M, as_const(env.aabb), p, !done);
+            }
+            if (h) {
                 bad = true;
                 done = true;
                 atomicMin(&first[e], k);   // at once: the edge's later states in flight stop at their next link
@@ -503,7 +460,7 @@ __global__ __launch_bounds__(64) void k_edge_first_aabb(EdgeGroups gr, EnvDev en
 
 // isValid(q1, q2) on axis-aligned scenes: k_edge_valid's lanes and
 // interpolation, the collision tree through the axis-test tables as in
-// k_edge_first_aabb (uniform loops per wave, joint values computed where the
+// k_edge_first_q (uniform loops per wave, joint values computed where the
 // kinematics reads them), no LDS.
 __global__ __launch_bounds__(256) void k_edge_valid_aabb(RobotDev rb, EnvDev env, QueryParams qp,
                                                          const float* __restrict__ q1, const float* __restrict__ q2,
@@ -571,9 +528,8 @@ hipError_t launch_edge_valid(const RobotDev& rb, const EnvDev& env, const QueryP
 }
 
 size_t edge_first_lds_bytes(const EdgeGroups& gr, const EnvDev& env) {
-    const size_t skel = (size_t)64 * 4 * 3 * (gr.max_links + 1);
-    if (!env.use_obb) return kQueue * sizeof(int2) + skel;
-    return ((staged_bytes<float>(env) + 15) & ~size_t(15)) + (size_t)64 * 4 * gr.max_dofs + skel;
+    (void)env;
+    return kQueue * sizeof(int2) + (size_t)64 * 4 * 3 * (gr.max_links + 1);
 }
 
 hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const QueryParams& qp, const float* q1,
@@ -591,11 +547,11 @@ hipError_t launch_edge_first(const EdgeGroups& gr, const EnvDev& env, const Quer
             if (nkeys > 0 && G > 0)
                 hipLaunchKernelGGL(k_slab_build, dim3((unsigned)((G + 63) / 64), (unsigned)nkeys), dim3(64), 0, st,
                                    env, keys, nkeys, const_cast<SlabEntry*>(gr.slab));
-            hipLaunchKernelGGL(k_edge_first_aabb, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride,
+            hipLaunchKernelGGL(k_edge_first_q<false>, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride,
                                step, order, rounds, first, counter, total_items, spl);
         } else {
-            hipLaunchKernelGGL(k_edge_first<true>, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride, step,
-                               order, rounds, first, counter, total_items, spl);
+            hipLaunchKernelGGL(k_edge_first_q<true>, dim3(grid), dim3(64), sm, st, gr, env, qp, q1, dir, row_stride,
+                               step, order, rounds, first, counter, total_items, spl);
         }
     }
     hipLaunchKernelGGL(k_first_final, blocks_for(num_edges, 256), dim3(256), 0, st, first, num_edges);

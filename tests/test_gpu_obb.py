@@ -173,3 +173,25 @@ def test_walk_order_exact_paths(engine, kind):
     v = engine.is_valid(rid, eid, w.q)
     assert np.array_equal(v, np.concatenate([engine.is_valid(rid, eid, w.q[s]) for s in parts]))
     assert 0 < big[2].sum() or kind == rvs.ENV_AABB
+
+
+def test_obb_edge_search_planner_starts(oracle, engine):
+    """The four chains' planner-start edges (workload 3) among the same boxes
+    turned by random rotations, so that edges walk far in an oriented-box
+    scene (every box through the full test in its frame, self pairs with
+    their bound): the first colliding state of every edge equals the
+    oracle's, for two launch shapes."""
+    w = W.workload3_planner(512, oracle=oracle)
+    sc = w.scene
+    quat = W._shoemake(np.random.default_rng(5), len(sc.dim)).astype(np.float32)
+    rot = W.Scene(sc.dim, sc.pos, quat, sc.is_ground, sc.min_dist_tol)
+    eid = engine.create_env(rot, rvs.ENV_OBB)
+    osc = oracle.scene_of(rot, oriented=True)
+    expect = [oracle.edge_first_collision(b.robot, osc, b.q1, b.dir, b.K, b.step) for b in w.batches]
+    allf = np.concatenate(expect)
+    assert (allf > 0).sum() > 800 and (allf < 0).sum() > 200
+    for wpc, spl in ((8, 4), (16, 1)):
+        engine.set_edge_tuning(wpc, spl)
+        got = engine.edge_batches(eid, w.batches)
+        for b, g, e in zip(w.batches, got, expect):
+            assert np.array_equal(g, e), (b.robot.name, wpc, spl, int((g != e).sum()))
