@@ -85,10 +85,10 @@ LINKS_TRAFFIC_PER_CONFIG = (2.0 * 1733610.1 + 1900112.3) * 1024 / 2097152
 # edges each, shape 8 waves per CU x 4 slots), per launch, from rocprofv3
 # counters (profiles/k_edge_first_chains_planner_starts.md): VALU wave
 # instructions, FP32 and FP64 operations executed (SQ_INSTS_VALU_FLOPS_FP32 /
-# _FP64 x 26.14 lanes active per VALU instruction) and HBM bytes (FETCH_SIZE
-# 38,299 KiB x 2.0 + WRITE_SIZE 2,536.2 KiB).
-EDGE_COUNTERS = {"valu_instr": 2.7828e8, "fp32_flop": 1.0559e8 * 26.14, "fp64_flop": 3.0127e7 * 26.14,
-                 "traffic_bytes": (2.0 * 38299.0 + 2536.2) * 1024,
+# _FP64 x 26.11 lanes active per VALU instruction) and HBM bytes (FETCH_SIZE
+# 38,058 KiB x 2.0 + WRITE_SIZE 2,540.1 KiB).
+EDGE_COUNTERS = {"valu_instr": 2.7824e8, "fp32_flop": 1.0554e8 * 26.11, "fp64_flop": 3.0155e7 * 26.11,
+                 "traffic_bytes": (2.0 * 38058.0 + 2540.1) * 1024,
                  "source": "profiles/k_edge_first_chains_planner_starts.md"}
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per launch of the FP32 bulk kernel on the headline workload from
@@ -416,7 +416,7 @@ def _edge_run(eng, O, w, cores: int, profiled: bool = False) -> dict:
         off += m
     nominal = int(K.sum())
     dense = flop_eval / (kms * 1e-3) / 1e12
-    roof = {"bound": "valu", "kernel": "k_edge_first_aabb", "achieved": None, "peak": PEAK_FP32_TFLOPS,
+    roof = {"bound": "valu", "kernel": "k_edge_first_q", "achieved": None, "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "dense_equivalent_tflops": dense, "dense_frac": dense / PEAK_FP32_TFLOPS,
             "dense_note": "nominal work of every state a search reaches (edge start to its first collision or "
                           "its end): 100 per FK joint + 160 per capsule-box pair + 60 per self pair; the axis "
