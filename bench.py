@@ -583,7 +583,7 @@ def edge_validity_section(eng, cores: int) -> dict:
     return {"workload": "xarm6_16aabb_ground_65536_short_edges_10_checks", "edges": n, "states": n * 10,
             "ms": ms, "kernel_ms": kms, "edges_per_s": n / (ms * 1e-3), "states_per_s": n * 10 / (ms * 1e-3),
             "valid_fraction": float(got.mean()), "mismatch_vs_oracle": int((got != ov).sum()),
-            "kernel": "k_edge_valid_aabb",
+            "kernel": "k_edge_valid_q",
             "cpu_baseline": {"value": n / cpu_s, "unit": "edges/s", "cores": cores, "kind": "port",
                              "sample": "the same 65,536 edges"}}
 

@@ -1811,9 +1811,6 @@ rvs_status rvs_edge_is_valid(rvs_context* c, int32_t robot, int32_t env, const f
     if (n == 0) return RVS_OK;
     if (c->opt.num_checks <= 0) return fail(c, RVS_EINVAL, "rvs_edge_is_valid: num_checks must be positive");
     const int D = rb.num_dofs;
-    if (ev.dev.use_obb &&
-        ((staged_bytes<float>(ev.dev) + 15) & ~size_t(15)) + (size_t)D * 256 * 4 > device_lds_limit(c->opt.device))
-        return fail(c, RVS_ERANGE, "rvs_edge_is_valid: obstacle set too large for LDS staging");
     (void)hipSetDevice(c->opt.device);
     const QueryParams qp = params_of(c);
     // axis-test tables of the robot's links (axis-aligned scenes)

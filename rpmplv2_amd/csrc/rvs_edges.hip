@@ -1,8 +1,7 @@
 // Edge checking kernels (gfx950).
 //
-// k_edge_valid (oriented boxes) and k_edge_valid_aabb:
-// RealVectorSpace::isValid(q1, q2) (RealVectorSpace.cpp:261-275) for a batch
-// of edges.  The reference walks k = K..1 and stops at
+// k_edge_valid_q: RealVectorSpace::isValid(q1, q2) (RealVectorSpace.cpp:
+// 261-275) for a batch of edges.  The reference walks k = K..1 and stops at
 // the first invalid state; the edge is valid exactly when all K states are,
 // so here each lane takes one (edge, k) state and a collision clears the
 // edge's flag (plain stores of 0; the flags start at 1).
@@ -90,71 +89,6 @@ __device__ inline float vecxf_norm_diff(const float* a, const float* b, int n) {
     float r = (p0 + p2) + (p1 + p3);
     for (int i = aligned; i < n; ++i) r = r + sq(i);
     return sqrtf(r);
-}
-
-// Collision tree against every obstacle (RealVectorSpace::isValid(q)) on a
-// configuration held in LDS column 'lane' (joint k at qcol[k * stride]).
-// Skeleton points are written to skel (point k, coordinate a at
-// skel[(3k + a) * stride]) when skel is not null.
-template <bool OBB>
-__device__ inline bool env_free(const RobotDev& rb, const Staged<float>& s, int M, const Params<float>& p,
-                                const float* qcol, int stride, float* skel) {
-    FKStream fk;
-    fk_init(fk);
-    Vec3<float> P0 = skel_point<float>(fk, rb, qcol, 0, stride);
-    if (skel) { skel[0] = P0.x; skel[stride] = P0.y; skel[2 * stride] = P0.z; }
-    for (int i = 0; i < rb.num_links; ++i) {
-        const Vec3<float> P1 = skel_point<float>(fk, rb, qcol, i + 1, stride);
-        if (skel) {
-            skel[(3 * i + 3) * stride] = P1.x;
-            skel[(3 * i + 4) * stride] = P1.y;
-            skel[(3 * i + 5) * stride] = P1.z;
-        }
-        const float r = rb.radius[i];
-        const float rn = rb.rnew32[i];
-        for (int j = 0; j < M; ++j) {
-            if (s.gnd[j] && i < rb.ground_included) continue;
-            const Box<float> bx = s.box[j];
-            if (OBB) {
-                const float* x = s.xf + 12 * j;
-                if (collision_box(to_box(x, P0), to_box(x, P1), r, rn, bx, p)) return false;
-            } else if (collision_box(P0, P1, r, rn, bx, p)) {
-                return false;
-            }
-        }
-        P0 = P1;
-    }
-    return true;
-}
-
-template <bool OBB>
-__global__ __launch_bounds__(256) void k_edge_valid(RobotDev rb, EnvDev env, QueryParams qp,
-                                                    const float* __restrict__ q1, const float* __restrict__ q2,
-                                                    int64_t n, uint8_t* valid) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const Staged<float> s = stage_env<float, OBB>(env, smem);
-    float* qbuf = reinterpret_cast<float*>(smem + ((staged_bytes<float>(env) + 15) & ~size_t(15)));
-    const int K = qp.num_checks;
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n * K) return;
-    const int64_t e = g / K;
-    const int kk = K - (int)(g % K);          // k = K .. 1
-    const int D = rb.num_dofs;
-    const float* a = q1 + e * D;
-    const float* b = q2 + e * D;
-    const float dist = vecxf_norm_diff(a, b, D);
-    // RealVectorSpace::interpolateEdge (RealVectorSpace.cpp:75-88) with
-    // step = k / num_checks * dist
-    const float step = (float)kk / (float)K * dist;
-    const int B = blockDim.x, t = threadIdx.x;
-    if (dist > 0) {
-        const float f = step / dist;
-        for (int i = 0; i < D; ++i) qbuf[i * B + t] = a[i] + (b[i] - a[i]) * f;
-    } else {
-        for (int i = 0; i < D; ++i) qbuf[i * B + t] = b[i];
-    }
-    const Params<float> p{qp.eq, qp.maxd};
-    if (!env_free<OBB>(rb, s, env.num_obj, p, qbuf + t, B, nullptr)) valid[e] = 0;
 }
 
 __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
@@ -458,11 +392,12 @@ __global__ __launch_bounds__(64) void k_edge_first_q(EdgeGroups gr, EnvDev env, 
     }
 }
 
-// isValid(q1, q2) on axis-aligned scenes: k_edge_valid's lanes and
-// interpolation, the collision tree through the axis-test tables as in
-// k_edge_first_q (uniform loops per wave, joint values computed where the
-// kinematics reads them), no LDS.
-__global__ __launch_bounds__(256) void k_edge_valid_aabb(RobotDev rb, EnvDev env, QueryParams qp,
+// isValid(q1, q2): one lane per (edge, state), the collision tree through
+// the axis-test tables (axis-aligned scenes) or every oriented box in its
+// frame (OBB), as in k_edge_first_q (uniform loops per wave, joint values
+// computed where the kinematics reads them), no LDS.
+template <bool OBB>
+__global__ __launch_bounds__(256) void k_edge_valid_q(RobotDev rb, EnvDev env, QueryParams qp,
                                                          const float* __restrict__ q1, const float* __restrict__ q2,
                                                          int64_t n, uint8_t* __restrict__ valid,
                                                          const int32_t* __restrict__ link_key,
@@ -493,8 +428,15 @@ __global__ __launch_bounds__(256) void k_edge_valid_aabb(RobotDev rb, EnvDev env
     for (int i = 0; i < rb.num_links; ++i) {
         if (!__any(!done)) break;
         const Vec3<float> P1 = skel_point_sc<float>(fk, rb, i + 1, sc);
-        ConstPtr<SlabEntry> tab = as_const(slab) + (size_t)as_const(link_key)[i] * (G + M);
-        if (link_hits(P0, P1, rb.radius[i], rb.rnew32[i], tab, G, M, as_const(env.aabb), p, !done)) {
+        bool h;
+        if (OBB) {
+            h = link_hits_obb(P0, P1, rb.radius[i], rb.rnew32[i], as_const(env.obb), as_const(env.flags), M,
+                              i < rb.ground_included, p, !done);
+        } else {
+            ConstPtr<SlabEntry> tab = as_const(slab) + (size_t)as_const(link_key)[i] * (G + M);
+            h = link_hits(P0, P1, rb.radius[i], rb.rnew32[i], tab, G, M, as_const(env.aabb), p, !done);
+        }
+        if (h) {
             valid[e] = 0;
             done = true;
         }
@@ -516,14 +458,12 @@ hipError_t launch_edge_valid(const RobotDev& rb, const EnvDev& env, const QueryP
         if (nkeys > 0 && G > 0)
             hipLaunchKernelGGL(k_slab_build, dim3((unsigned)((G + 63) / 64), (unsigned)nkeys), dim3(64), 0, st, env,
                                keys, nkeys, slab);
-        hipLaunchKernelGGL(k_edge_valid_aabb, blocks_for(n * qp.num_checks, 256), dim3(256), 0, st, rb, env, qp, q1,
-                           q2, n, valid, link_key, slab);
-        return hipGetLastError();
+        hipLaunchKernelGGL(k_edge_valid_q<false>, blocks_for(n * qp.num_checks, 256), dim3(256), 0, st, rb, env, qp,
+                           q1, q2, n, valid, link_key, slab);
+    } else {
+        hipLaunchKernelGGL(k_edge_valid_q<true>, blocks_for(n * qp.num_checks, 256), dim3(256), 0, st, rb, env, qp,
+                           q1, q2, n, valid, link_key, slab);
     }
-    const int B = 256;
-    const size_t sm = ((staged_bytes<float>(env) + 15) & ~size_t(15)) + (size_t)rb.num_dofs * B * 4;
-    hipLaunchKernelGGL(k_edge_valid<true>, blocks_for(n * qp.num_checks, B), dim3(B), sm, st, rb, env, qp, q1, q2, n,
-                       valid);
     return hipGetLastError();
 }
 
