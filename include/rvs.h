@@ -134,11 +134,12 @@ enum {
     RVS_ENV_AABB = 0,
     /* Build extension: rotated boxes are queried as oriented boxes (the
      * capsule is moved into the box frame; nearest points come back in the
-     * world frame).  Accepted by every environment query.  The obstacle set is
-     * staged in the workgroup LDS the device reports
-     * (hipDeviceAttributeMaxSharedMemoryPerBlock): with 64 KB, up to 851
-     * oriented boxes for float queries and 624 for RVS_PREC_FP64 (RVS_ERANGE
-     * beyond); environments hold at most 1024 boxes. */
+     * world frame).  Accepted by every environment query.  The distance and
+     * collision-tree queries stage the obstacle set in the workgroup LDS the
+     * device reports (hipDeviceAttributeMaxSharedMemoryPerBlock): with 64 KB,
+     * up to 851 oriented boxes for float queries and 624 for RVS_PREC_FP64
+     * (RVS_ERANGE beyond); the edge queries read the boxes from device memory
+     * and take any environment; environments hold at most 1024 boxes. */
     RVS_ENV_OBB = 1
 };
 
