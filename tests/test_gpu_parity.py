@@ -442,6 +442,35 @@ def test_edges_many_obstacles(oracle, engine):
     assert 0 < v.mean() < 1 and np.array_equal(v, ov), int((v != ov).sum())
 
 
+@pytest.mark.parametrize("boxes", [0, 3])
+def test_edge_first_collision_long_chain(oracle, engine, boxes):
+    """A 40-link chain with self-collision (skeletons of 41 points in LDS)
+    beside a 10-link one in the same launch, in an empty scene (no axis-test
+    groups at all) and in a 3-box scene (one partial group): first
+    collisions equal the oracle's."""
+    rng = np.random.default_rng(90 + boxes)
+    if boxes:
+        sc = W._boxes(rng, boxes, 0.1, 0.3, 1.0, (0.0, 0.0, 0.5))
+    else:
+        sc = W.Scene(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros((0, 4)), np.zeros(0, np.uint8),
+                     np.zeros(0, np.float32))
+    eid = engine.create_env(sc)
+    osc = oracle.scene_of(sc)
+    batches = []
+    for n, m in ((40, 600), (10, 600)):
+        rb = R.chain(rng.uniform(0.03, 0.06, n), radius=0.01, self_collision=True)
+        q1 = rng.uniform(-1.2, 1.2, (m, n)).astype(np.float32)
+        u = rng.normal(size=(m, n))
+        u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+        batches.append(W.EdgeBatch(rb, q1, u, rng.integers(1, 40, m).astype(np.int32)))
+    expect = [oracle.edge_first_collision(b.robot, osc, b.q1, b.dir, b.K, b.step) for b in batches]
+    allf = np.concatenate(expect)
+    assert (allf > 0).sum() > 20 and (allf < 0).sum() > 20
+    got = engine.edge_batches(eid, batches)
+    for b, g, e in zip(batches, got, expect):
+        assert np.array_equal(g, e), (b.robot.num_dofs, int((g != e).sum()))
+
+
 def test_edge_first_collision_sparse_scene(oracle, engine):
     # few, small obstacles so many edges run to their last state
     rng = np.random.default_rng(11)
